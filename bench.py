@@ -1,0 +1,224 @@
+"""Training-step throughput of the plain 3DGS method on one MI355X.
+
+Workload: one Trainer.step() (render, L1 + D-SSIM loss, backward, densification
+statistics, Adam with activation) at 1.1M Gaussians, 980x545, SH degree 3, FP32,
+on the synthetic ring scene: 263 training poses (radius 6 +- 0.5, height N(0, 0.3),
+fov 50 deg) whose targets are renders of a seed-7 teacher set of 1.1M Gaussians
+with N(0, 0.01) pixel noise. The student is a seed-0 set of the same size and
+distribution in the 10-unit box, trained from iteration 15,000 (past the
+densification window, SH degree 3), so every timed step is a full optimizer step.
+
+Prints one JSON line. `roofline` is the dominant kernel's algorithmic bytes over
+its mean launch time (HIP events on the library's stream); `cpu_baseline` times
+the float64 CPU restatement of the same step (oracle/, OpenMP) on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_GAUSSIANS = 1_100_000
+WIDTH, HEIGHT = 980, 545
+TRAIN_VIEWS, HELD_OUT = 263, 33
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per unit (useful bytes only): per Gaussian, per (splat, tile)
+# pair, per pixel. Sort is charged at the reference's 6 x 8-bit pass model over the
+# 48 significant key bits.
+STAGE_BYTES = {
+    "preprocess": (284, 0, 0),
+    "scan": (8, 0, 0),
+    "sort": (0, 144, 0),
+    "rasterize": (0, 40, 24),
+    "loss": (0, 0, 32),
+    "rasterize_backward": (36, 40, 32),
+    "preprocess_backward": (512, 0, 0),
+    "adam": (1708, 0, 0),
+}
+STEP_BYTES = (2612, 240, 88)
+
+
+def log(*parts):
+    print("[bench]", *parts, file=sys.stderr, flush=True)
+
+
+def ring_poses(count):
+    from diorama_amd import scene
+
+    return scene.ring_cameras(count, WIDTH, HEIGHT, 50.0, 6.0, seed=3)
+
+
+def split_views(cams):
+    """263 training poses and 33 held-out ones taken every 8th."""
+    held = [k for k in range(len(cams)) if k % 8 == 7][:HELD_OUT]
+    train = [c for k, c in enumerate(cams) if k not in set(held)]
+    return train[:TRAIN_VIEWS], [cams[k] for k in held]
+
+
+def teacher_targets(ctx, cams, n):
+    """rgba8 targets: the seed-7 teacher rendered on the GPU plus N(0, 0.01) noise."""
+    import diorama_amd
+    from diorama_amd import scene
+
+    params, sh = scene.random_raw_gaussians(n, 7, box=5.0)
+    r = diorama_amd.GaussianRenderer(ctx)
+    r.set_gaussians(scene.activate(params), sh, 3)
+    rng = np.random.default_rng(7)
+    images = []
+    for cam in cams:
+        r.encode(None, cam, [0, 0, 0])
+        color = r.pixel_buffer.download(np.float32, WIDTH * HEIGHT * 4).reshape(-1, 4)[:, :3]
+        noisy = color + rng.standard_normal(color.shape, dtype=np.float32) * np.float32(0.01)
+        q = np.rint(np.clip(noisy, 0, 1) * 255).astype(np.uint32)
+        images.append(q[:, 0] | (q[:, 1] << 8) | (q[:, 2] << 16) | np.uint32(255 << 24))
+    r.after_submit()
+    r.destroy()
+    return images
+
+
+def cpu_baseline(params, sh, cam, image, sh_degree, budget_s=30.0):
+    """One step of the float64 CPU restatement on the host cores: the full step when
+    it fits the budget, else the same step on a random subset of the Gaussians with
+    the time scaled to the full count."""
+    from oracle import oracle
+
+    threads = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    n = len(params)
+    rng = np.random.default_rng(0)
+    target = np.zeros((WIDTH * HEIGHT, 4), dtype=np.uint8)
+    for c in range(3):
+        target[:, c] = (image >> (8 * c)) & 255
+    lr = [1 - 0.9 ** 15000, 1 - 0.999 ** 15000, 1.6e-4 * 0.01 ** 0.5, 0.025, 0.005, 0.001, 0.0025, 0.0025 / 20]
+    def run(sample):
+        idx = np.sort(rng.choice(n, size=sample, replace=False)) if sample < n else np.arange(n)
+        p = params[idx].astype(np.float64)
+        s = sh[idx].astype(np.float64)
+        m = np.zeros((sample, 122))
+        st = np.zeros((sample, 4))
+        g = oracle.activate(p)
+        t0 = time.perf_counter()
+        oracle.train_step(p, s, m, st, g, sh_degree, cam, [0, 0, 0], target, lr)
+        return time.perf_counter() - t0
+
+    sample = min(n, 110_000)
+    dt = run(sample)
+    if sample < n and dt * n / sample <= budget_s:
+        sample, dt = n, run(n)
+    scaled = dt * n / sample
+    return {"value": 1.0 / scaled, "unit": "steps/s", "cores": oracle.threads(), "kind": "port",
+            "sample": f"one optimize() step (render, loss, backward, stats, Adam) in float64 on {sample:,} of "
+                      f"the {n:,} Gaussians at {WIDTH}x{HEIGHT}: {dt:.2f} s"
+                      + ("" if sample == n else f", time scaled by {n / sample:.2f} to the full count")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--repeats", type=int, default=20, help="launches per stage timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(multi-GPU sharding is a design, not an implementation)", file=sys.stderr)
+        return 2
+
+    import diorama_amd
+    from diorama_amd import scene
+
+    ctx = diorama_amd.Context(0)
+    cams = ring_poses(TRAIN_VIEWS + HELD_OUT)
+    train, _ = split_views(cams)
+    t0 = time.perf_counter()
+    images = teacher_targets(ctx, train, args.gaussians)
+    log(f"teacher targets: {len(images)} views in {time.perf_counter() - t0:.1f} s")
+    params, sh = scene.random_raw_gaussians(args.gaussians, 0, box=5.0)
+    extent = 1.1 * max(math.sqrt(sum(v * v for v in _eye(c))) for c in train)
+    settings = diorama_amd.default_settings(extent)
+    trainer = diorama_amd.Trainer(ctx, params, sh, train, images, settings)
+    trainer.iteration = 15_000
+    trainer.sh_degree = 3
+    for _ in range(args.warmup):
+        trainer.step()
+    ctx.synchronize()
+    log(f"warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        trainer.step()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    steps_per_s = args.steps / elapsed
+    log(f"timed {args.steps} steps: {elapsed * 1000 / args.steps:.3f} ms/step")
+
+    renderer = trainer.renderer
+    tiles = renderer.tiles
+    tiles.after_submit()
+    pairs = tiles.last_pair_count
+    n = trainer.count
+    x = WIDTH * HEIGHT
+    stages = {}
+    for name, (bg, bp, bx) in STAGE_BYTES.items():
+        ms = trainer.time_stage(name, args.repeats)
+        nbytes = bg * n + bp * pairs + bx * x
+        stages[name] = {"ms": ms, "bytes": nbytes, "GB/s": nbytes / (ms * 1e-3) / 1e9}
+        log(f"{name:20s} {ms * 1000:9.1f} us  {stages[name]['GB/s']:8.1f} GB/s algorithmic")
+    dominant = max(stages, key=lambda k: stages[k]["ms"])
+    d = stages[dominant]
+    roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(d["GB/s"], 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(d["GB/s"] / HBM_PEAK_GBS, 4), "traffic": None}
+    step_bytes = STEP_BYTES[0] * n + STEP_BYTES[1] * pairs + STEP_BYTES[2] * x
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(params, sh, train[0], images[0], 3)
+        log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads ({time.perf_counter() - t0:.1f} s)")
+    trainer.destroy()
+    ctx.close()
+    line = {
+        "metric": "train steps/s @1.1M Gaussians 980x545 FP32",
+        "value": round(steps_per_s, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1000 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: seed-7 teacher renders + N(0,0.01) noise on 263 ring poses; seed-0 student",
+        "config": {"workload": "plain 3DGS training step (Trainer.step: render, L1+D-SSIM, backward, stats, Adam)",
+                   "gaussians": n, "width": WIDTH, "height": HEIGHT, "sh_degree": 3, "train_views": len(train),
+                   "pairs": pairs, "views_per_s": round(steps_per_s, 2),
+                   "step_bytes_model": step_bytes,
+                   "step_GBps_model": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
+                   "parallelism": "single GPU"},
+        "stages_us": {k: round(v["ms"] * 1000, 1) for k, v in stages.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["config"]["speedup_vs_cpu"] = round(steps_per_s / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _eye(cam):
+    """Camera centre of a view matrix [R | t]: -Rᵀ t."""
+    v = cam["view"]
+    return [-(v[c * 4] * v[12] + v[c * 4 + 1] * v[13] + v[c * 4 + 2] * v[14]) for c in range(3)]
+
+
+if __name__ == "__main__":
+    sys.exit(main())

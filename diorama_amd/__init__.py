@@ -1,0 +1,599 @@
+"""Python host mirror of the reference's operator classes over the C ABI
+(include/diorama.h, built as libdiorama_hip.so next to this file).
+
+Classes keep the names and methods of src/prefixSum.ts, src/radixSort.ts,
+src/tileRasterizer.ts, src/gaussianRenderer.ts, src/trainingLoss.ts and
+src/trainer.ts; buffers are `Buffer` objects wrapping device allocations. The HIP
+library is required: importing works without it, but constructing a context
+raises if the library or a GPU is missing (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdiorama_hip.so")
+
+_lib = None
+
+
+class DioramaError(RuntimeError):
+    pass
+
+
+class _Camera(ctypes.Structure):
+    _fields_ = [("view", ctypes.c_float * 16), ("proj", ctypes.c_float * 16), ("viewport", ctypes.c_float * 2)]
+
+
+class _Settings(ctypes.Structure):
+    _fields_ = [
+        ("extent", ctypes.c_float),
+        ("position_lr_steps", ctypes.c_uint32),
+        ("position_lr", ctypes.c_float * 2),
+        ("opacity_lr", ctypes.c_float),
+        ("scale_lr", ctypes.c_float),
+        ("rotation_lr", ctypes.c_float),
+        ("color_lr", ctypes.c_float),
+        ("sh_lr", ctypes.c_float),
+        ("sh_degree_interval", ctypes.c_uint32),
+        ("densify_from", ctypes.c_uint32),
+        ("densify_until", ctypes.c_uint32),
+        ("densify_interval", ctypes.c_uint32),
+        ("densify_grad_threshold", ctypes.c_float),
+        ("percent_dense", ctypes.c_float),
+        ("min_opacity", ctypes.c_float),
+        ("opacity_reset_interval", ctypes.c_uint32),
+        ("background", ctypes.c_float * 3),
+    ]
+
+
+class _Densification(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint32) for k in ("step", "before", "after", "kept", "cloned", "split", "removed")]
+
+
+# Every exported symbol of include/diorama.h with (restype, argtypes).
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_I = ctypes.c_int
+_F3 = ctypes.POINTER(ctypes.c_float)
+SIGNATURES = {
+    "dio_last_error": (ctypes.c_char_p, []),
+    "dio_version": (ctypes.c_char_p, []),
+    "dio_ctx_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "dio_ctx_synchronize": (_I, [_P]),
+    "dio_ctx_destroy": (None, [_P]),
+    "dio_buf_create": (_I, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "dio_buf_upload": (_I, [_P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+    "dio_buf_download": (_I, [_P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+    "dio_buf_clear": (_I, [_P]),
+    "dio_buf_size": (ctypes.c_size_t, [_P]),
+    "dio_buf_destroy": (None, [_P]),
+    "dio_prefix_sum_create": (_I, [_P, _P, _U32, _P, ctypes.POINTER(_P)]),
+    "dio_prefix_sum_total": (_P, [_P]),
+    "dio_prefix_sum_encode": (_I, [_P]),
+    "dio_prefix_sum_destroy": (None, [_P]),
+    "dio_radix_sort_create": (_I, [_P, _U32, _U32, _P, ctypes.POINTER(_P)]),
+    "dio_radix_sort_keys": (_P, [_P]),
+    "dio_radix_sort_values": (_P, [_P]),
+    "dio_radix_sort_capacity": (_U32, [_P]),
+    "dio_radix_sort_encode": (_I, [_P]),
+    "dio_radix_sort_destroy": (None, [_P]),
+    "dio_tile_rasterizer_create": (_I, [_P, _P, _U32, _U32, ctypes.POINTER(_P)]),
+    "dio_tile_rasterizer_encode": (_I, [_P, _P, _U32, _U32, _F3]),
+    "dio_tile_rasterizer_encode_backward": (_I, [_P, _U32, _U32, _P, _P]),
+    "dio_tile_rasterizer_after_submit": (_I, [_P]),
+    "dio_tile_rasterizer_last_pair_count": (_U32, [_P]),
+    "dio_tile_rasterizer_pair_capacity": (_U32, [_P]),
+    "dio_tile_rasterizer_pair_splats": (_P, [_P]),
+    "dio_tile_rasterizer_ranges": (_P, [_P]),
+    "dio_tile_rasterizer_pixel_color": (_P, [_P]),
+    "dio_tile_rasterizer_pixel_last": (_P, [_P]),
+    "dio_tile_rasterizer_destroy": (None, [_P]),
+    "dio_renderer_create": (_I, [_P, _U32, ctypes.POINTER(_P)]),
+    "dio_renderer_set_gaussians": (_I, [_P, _P, _U32, _P, _U32]),
+    "dio_renderer_adopt_gaussians": (_I, [_P, _U32, _P, _P, _U32]),
+    "dio_renderer_set_sh_degree": (_I, [_P, _U32]),
+    "dio_renderer_encode": (_I, [_P, _P, ctypes.POINTER(_Camera), _F3]),
+    "dio_renderer_encode_preprocess": (_I, [_P, ctypes.POINTER(_Camera)]),
+    "dio_renderer_encode_tiles": (_I, [_P, _P, _F3]),
+    "dio_renderer_encode_draw": (_I, [_P, _P, _P, _U32, _U32, _U32, _F3]),
+    "dio_renderer_encode_backward": (_I, [_P, _P]),
+    "dio_renderer_after_submit": (_I, [_P]),
+    "dio_renderer_count": (_U32, [_P]),
+    "dio_renderer_splat_buffer": (_P, [_P]),
+    "dio_renderer_gaussian_buffer": (_P, [_P]),
+    "dio_renderer_sh_buffer": (_P, [_P]),
+    "dio_renderer_tiles": (_P, [_P]),
+    "dio_renderer_pixel_buffer": (_P, [_P]),
+    "dio_renderer_pixel_last_buffer": (_P, [_P]),
+    "dio_renderer_gradient_gaussians": (_P, [_P]),
+    "dio_renderer_gradient_sh": (_P, [_P]),
+    "dio_renderer_gradient_splats": (_P, [_P]),
+    "dio_renderer_destroy": (None, [_P]),
+    "dio_training_loss_create": (_I, [_P, _U32, _U32, ctypes.POINTER(_P)]),
+    "dio_training_loss_encode": (_I, [_P, _P, _P]),
+    "dio_training_loss_read": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "dio_training_loss_pixel_grads": (_P, [_P]),
+    "dio_training_loss_pixel_loss": (_P, [_P]),
+    "dio_training_loss_destroy": (None, [_P]),
+    "dio_default_settings": (None, [ctypes.c_float, ctypes.POINTER(_Settings)]),
+    "dio_trainer_create": (_I, [_P, _P, _P, _U32, ctypes.POINTER(_Camera), ctypes.POINTER(_P), _U32,
+                                ctypes.POINTER(_Settings), ctypes.POINTER(_P)]),
+    "dio_trainer_step": (_I, [_P]),
+    "dio_trainer_optimize": (_I, [_P]),
+    "dio_trainer_maintain": (_I, [_P]),
+    "dio_trainer_densify": (_I, [_P, _I]),
+    "dio_trainer_reset_opacity": (_I, [_P]),
+    "dio_trainer_export_raw": (_I, [_P, _P, _P]),
+    "dio_trainer_last_loss": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "dio_trainer_evaluate": (_I, [_P, ctypes.POINTER(_Camera), ctypes.POINTER(_P), _U32, _I,
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "dio_trainer_count": (_U32, [_P]),
+    "dio_trainer_iteration": (_U32, [_P]),
+    "dio_trainer_set_iteration": (None, [_P, _U32]),
+    "dio_trainer_sh_degree": (_U32, [_P]),
+    "dio_trainer_set_sh_degree": (_I, [_P, _U32]),
+    "dio_trainer_densification_count": (_U32, [_P]),
+    "dio_trainer_densification": (_I, [_P, _U32, ctypes.POINTER(_Densification)]),
+    "dio_trainer_params": (_P, [_P]),
+    "dio_trainer_moments": (_P, [_P]),
+    "dio_trainer_stats": (_P, [_P]),
+    "dio_trainer_renderer": (_P, [_P]),
+    "dio_trainer_destroy": (None, [_P]),
+    "dio_trainer_time_stage": (_I, [_P, ctypes.c_char_p, _U32, ctypes.POINTER(ctypes.c_double)]),
+}
+
+
+def lib():
+    """The loaded C library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DioramaError(f"{LIB_PATH} is missing: build it with `python -m diorama_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise DioramaError(lib().dio_last_error().decode())
+
+
+def _floats(values, n):
+    arr = (ctypes.c_float * n)(*[float(v) for v in values])
+    return arr
+
+
+def _camera(cam):
+    c = _Camera()
+    c.view[:] = [float(v) for v in cam["view"]]
+    c.proj[:] = [float(v) for v in cam["proj"]]
+    c.viewport[:] = [float(v) for v in cam["viewport"]]
+    return c
+
+
+class Context:
+    """One device and its in-order HIP stream (replaces initGpu, src/gpu.ts:8-47)."""
+
+    def __init__(self, device=0):
+        h = _P()
+        _check(lib().dio_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+
+    def synchronize(self):
+        _check(lib().dio_ctx_synchronize(self.handle))
+
+    def buffer(self, nbytes):
+        return Buffer.create(self, nbytes)
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        b = Buffer.create(self, a.nbytes)
+        b.upload(a)
+        return b
+
+    def close(self):
+        if self.handle:
+            lib().dio_ctx_destroy(self.handle)
+            self.handle = None
+
+
+class Buffer:
+    """A device allocation. `owned` buffers are freed with the object."""
+
+    def __init__(self, handle, owned, keep=None):
+        self.handle = handle
+        self.owned = owned
+        self._keep = keep
+
+    @classmethod
+    def create(cls, ctx, nbytes):
+        h = _P()
+        _check(lib().dio_buf_create(ctx.handle, int(nbytes), ctypes.byref(h)))
+        return cls(h, True)
+
+    @property
+    def nbytes(self):
+        return lib().dio_buf_size(self.handle)
+
+    def upload(self, array, offset=0):
+        a = np.ascontiguousarray(array)
+        _check(lib().dio_buf_upload(self.handle, offset, a.ctypes.data_as(_P), a.nbytes))
+
+    def download(self, dtype=np.float32, count=None, offset=0):
+        itemsize = np.dtype(dtype).itemsize
+        if count is None:
+            count = (self.nbytes - offset) // itemsize
+        out = np.empty(count, dtype=dtype)
+        _check(lib().dio_buf_download(self.handle, offset, out.ctypes.data_as(_P), out.nbytes))
+        return out
+
+    def clear(self):
+        _check(lib().dio_buf_clear(self.handle))
+
+    def destroy(self):
+        if self.owned and self.handle:
+            lib().dio_buf_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _view(handle, keep):
+    return None if not handle else Buffer(ctypes.c_void_p(handle), False, keep)
+
+
+class PrefixSum:
+    """src/prefixSum.ts:10-103: exclusive scan of `data` in place."""
+
+    def __init__(self, ctx, data, capacity, count=None):
+        h = _P()
+        _check(lib().dio_prefix_sum_create(ctx.handle, data.handle, capacity, count.handle if count else None,
+                                           ctypes.byref(h)))
+        self.handle, self._keep = h, (data, count)
+
+    @property
+    def total(self):
+        return _view(lib().dio_prefix_sum_total(self.handle), self)
+
+    def encode(self):
+        _check(lib().dio_prefix_sum_encode(self.handle))
+
+    def destroy(self):
+        if self.handle:
+            lib().dio_prefix_sum_destroy(self.handle)
+            self.handle = None
+
+
+class RadixSort:
+    """src/radixSort.ts:27-142: stable sort of u32 or (low, high) u64 keys with u32 values."""
+
+    def __init__(self, ctx, capacity, high_bits=0, count=None):
+        h = _P()
+        _check(lib().dio_radix_sort_create(ctx.handle, capacity, high_bits, count.handle if count else None,
+                                           ctypes.byref(h)))
+        self.handle, self._keep = h, count
+        self.capacity = capacity
+
+    @property
+    def keys(self):
+        return _view(lib().dio_radix_sort_keys(self.handle), self)
+
+    @property
+    def values(self):
+        return _view(lib().dio_radix_sort_values(self.handle), self)
+
+    def encode(self):
+        _check(lib().dio_radix_sort_encode(self.handle))
+
+    def destroy(self):
+        if self.handle:
+            lib().dio_radix_sort_destroy(self.handle)
+            self.handle = None
+
+
+class TileRasterizer:
+    """src/tileRasterizer.ts:29-317 (for splats already on the device)."""
+
+    def __init__(self, ctx=None, splats=None, count=0, initial_pair_capacity=0, handle=None, keep=None):
+        if handle is not None:
+            self.handle, self._keep, self.owned = handle, keep, False
+            return
+        h = _P()
+        _check(lib().dio_tile_rasterizer_create(ctx.handle, splats.handle, count, initial_pair_capacity,
+                                                ctypes.byref(h)))
+        self.handle, self._keep, self.owned = h, splats, True
+
+    def encode(self, target, viewport, background):
+        _check(lib().dio_tile_rasterizer_encode(self.handle, target.handle if target else None, int(viewport[0]),
+                                                int(viewport[1]), _floats(background, 3)))
+
+    def encode_backward(self, viewport, pixel_grads, splat_grads):
+        _check(lib().dio_tile_rasterizer_encode_backward(self.handle, int(viewport[0]), int(viewport[1]),
+                                                         pixel_grads.handle, splat_grads.handle))
+
+    def after_submit(self):
+        _check(lib().dio_tile_rasterizer_after_submit(self.handle))
+
+    @property
+    def last_pair_count(self):
+        return lib().dio_tile_rasterizer_last_pair_count(self.handle)
+
+    @property
+    def pair_capacity(self):
+        return lib().dio_tile_rasterizer_pair_capacity(self.handle)
+
+    @property
+    def debug_buffers(self):
+        return {"pairSplats": _view(lib().dio_tile_rasterizer_pair_splats(self.handle), self),
+                "ranges": _view(lib().dio_tile_rasterizer_ranges(self.handle), self)}
+
+    @property
+    def pixel_color(self):
+        return _view(lib().dio_tile_rasterizer_pixel_color(self.handle), self)
+
+    @property
+    def pixel_last(self):
+        return _view(lib().dio_tile_rasterizer_pixel_last(self.handle), self)
+
+    def destroy(self):
+        if self.owned and self.handle:
+            lib().dio_tile_rasterizer_destroy(self.handle)
+        self.handle = None
+
+
+class GaussianRenderer:
+    """src/gaussianRenderer.ts:55-311."""
+
+    def __init__(self, ctx, initial_pair_capacity=0, handle=None, keep=None):
+        self.ctx = ctx
+        if handle is not None:
+            self.handle, self.owned, self._keep = handle, False, keep
+            return
+        h = _P()
+        _check(lib().dio_renderer_create(ctx.handle, initial_pair_capacity, ctypes.byref(h)))
+        self.handle, self.owned = h, True
+
+    def set_gaussians(self, data, sh=None, degree=0):
+        g = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+        n = g.size // 16
+        s = None if sh is None else np.ascontiguousarray(sh, dtype=np.float32).reshape(-1)
+        if s is not None and s.size != n * 45:
+            raise DioramaError(f"Expected {n * 45} spherical-harmonic floats, got {s.size}.")
+        _check(lib().dio_renderer_set_gaussians(self.handle, g.ctypes.data_as(_P), n,
+                                                None if s is None else s.ctypes.data_as(_P), degree))
+
+    def set_sh_degree(self, degree):
+        _check(lib().dio_renderer_set_sh_degree(self.handle, degree))
+
+    def encode(self, target, camera, background):
+        c = _camera(camera)
+        _check(lib().dio_renderer_encode(self.handle, target.handle if target else None, ctypes.byref(c),
+                                         _floats(background, 3)))
+
+    def encode_preprocess(self, camera):
+        c = _camera(camera)
+        _check(lib().dio_renderer_encode_preprocess(self.handle, ctypes.byref(c)))
+
+    def encode_tiles(self, target, background):
+        _check(lib().dio_renderer_encode_tiles(self.handle, target.handle if target else None, _floats(background, 3)))
+
+    def encode_draw(self, target, splats, count, viewport, background):
+        _check(lib().dio_renderer_encode_draw(self.handle, target.handle if target else None, splats.handle, count,
+                                              int(viewport[0]), int(viewport[1]), _floats(background, 3)))
+
+    def encode_backward(self, pixel_grads):
+        _check(lib().dio_renderer_encode_backward(self.handle, pixel_grads.handle))
+
+    def after_submit(self):
+        _check(lib().dio_renderer_after_submit(self.handle))
+
+    @property
+    def count(self):
+        return lib().dio_renderer_count(self.handle)
+
+    @property
+    def splat_buffer(self):
+        return _view(lib().dio_renderer_splat_buffer(self.handle), self)
+
+    @property
+    def gaussian_buffer(self):
+        return _view(lib().dio_renderer_gaussian_buffer(self.handle), self)
+
+    @property
+    def sh_buffer(self):
+        return _view(lib().dio_renderer_sh_buffer(self.handle), self)
+
+    @property
+    def tiles(self):
+        h = lib().dio_renderer_tiles(self.handle)
+        return None if not h else TileRasterizer(handle=ctypes.c_void_p(h), keep=self)
+
+    @property
+    def pixel_buffer(self):
+        return _view(lib().dio_renderer_pixel_buffer(self.handle), self)
+
+    @property
+    def pixel_last_buffer(self):
+        return _view(lib().dio_renderer_pixel_last_buffer(self.handle), self)
+
+    @property
+    def gradients(self):
+        g = lib().dio_renderer_gradient_gaussians(self.handle)
+        if not g:
+            return None
+        return {"gaussians": _view(g, self), "sh": _view(lib().dio_renderer_gradient_sh(self.handle), self),
+                "splats": _view(lib().dio_renderer_gradient_splats(self.handle), self)}
+
+    def destroy(self):
+        if self.owned and self.handle:
+            lib().dio_renderer_destroy(self.handle)
+        self.handle = None
+
+
+class TrainingLoss:
+    """src/trainingLoss.ts:16-130."""
+
+    def __init__(self, ctx, width, height):
+        h = _P()
+        _check(lib().dio_training_loss_create(ctx.handle, width, height, ctypes.byref(h)))
+        self.handle, self.width, self.height = h, width, height
+
+    def encode(self, rendered, target):
+        _check(lib().dio_training_loss_encode(self.handle, rendered.handle, target.handle))
+
+    def read(self):
+        v = ctypes.c_double()
+        _check(lib().dio_training_loss_read(self.handle, ctypes.byref(v)))
+        return v.value
+
+    @property
+    def pixel_grads(self):
+        return _view(lib().dio_training_loss_pixel_grads(self.handle), self)
+
+    @property
+    def pixel_loss(self):
+        return _view(lib().dio_training_loss_pixel_loss(self.handle), self)
+
+    def destroy(self):
+        if self.handle:
+            lib().dio_training_loss_destroy(self.handle)
+            self.handle = None
+
+
+def default_settings(extent):
+    """DEFAULT_SETTINGS (src/trainer.ts:65-82) with the scene extent, as a dict."""
+    s = _Settings()
+    lib().dio_default_settings(float(extent), ctypes.byref(s))
+    out = {}
+    for name, _ in _Settings._fields_:
+        v = getattr(s, name)
+        out[name] = list(v) if hasattr(v, "__len__") else v
+    return out
+
+
+def _settings(d):
+    s = _Settings()
+    for name, _ in _Settings._fields_:
+        v = d[name]
+        if hasattr(v, "__len__"):
+            getattr(s, name)[:] = [float(x) for x in v]
+        else:
+            setattr(s, name, v)
+    return s
+
+
+def _images(images):
+    arrs = [np.ascontiguousarray(im, dtype=np.uint32).reshape(-1) for im in images]
+    ptrs = (_P * len(arrs))(*[a.ctypes.data_as(_P) for a in arrs])
+    return arrs, ptrs
+
+
+class Trainer:
+    """src/trainer.ts:103-446: one training view per step, plain 3DGS schedule."""
+
+    def __init__(self, ctx, params, sh, cameras, images, settings):
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        s = np.ascontiguousarray(sh, dtype=np.float32).reshape(-1)
+        n = p.size // 16
+        cams = (_Camera * len(cameras))(*[_camera(c) for c in cameras])
+        arrs, ptrs = _images(images)
+        st = _settings(settings)
+        h = _P()
+        _check(lib().dio_trainer_create(ctx.handle, p.ctypes.data_as(_P), s.ctypes.data_as(_P), n, cams, ptrs,
+                                        len(cameras), ctypes.byref(st), ctypes.byref(h)))
+        self.handle, self.ctx = h, ctx
+
+    def step(self):
+        _check(lib().dio_trainer_step(self.handle))
+
+    def optimize(self):
+        _check(lib().dio_trainer_optimize(self.handle))
+
+    def maintain(self):
+        _check(lib().dio_trainer_maintain(self.handle))
+
+    def densify(self, prune_large):
+        _check(lib().dio_trainer_densify(self.handle, 1 if prune_large else 0))
+
+    def reset_opacity(self):
+        _check(lib().dio_trainer_reset_opacity(self.handle))
+
+    def export_raw(self):
+        n = self.count
+        p = np.empty(n * 16, dtype=np.float32)
+        s = np.empty(n * 45, dtype=np.float32)
+        _check(lib().dio_trainer_export_raw(self.handle, p.ctypes.data_as(_P), s.ctypes.data_as(_P)))
+        return {"params": p.reshape(n, 16), "sh": s.reshape(n, 45)}
+
+    def last_loss(self):
+        v = ctypes.c_double()
+        _check(lib().dio_trainer_last_loss(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def evaluate(self, cameras, images, with_ssim=True):
+        cams = (_Camera * len(cameras))(*[_camera(c) for c in cameras])
+        arrs, ptrs = _images(images)
+        p, s = ctypes.c_double(), ctypes.c_double()
+        _check(lib().dio_trainer_evaluate(self.handle, cams, ptrs, len(cameras), 1 if with_ssim else 0,
+                                          ctypes.byref(p), ctypes.byref(s)))
+        return {"psnr": p.value, "ssim": s.value}
+
+    def time_stage(self, stage, repeats=10):
+        v = ctypes.c_double()
+        _check(lib().dio_trainer_time_stage(self.handle, stage.encode(), repeats, ctypes.byref(v)))
+        return v.value
+
+    @property
+    def count(self):
+        return lib().dio_trainer_count(self.handle)
+
+    @property
+    def iteration(self):
+        return lib().dio_trainer_iteration(self.handle)
+
+    @iteration.setter
+    def iteration(self, value):
+        lib().dio_trainer_set_iteration(self.handle, int(value))
+
+    @property
+    def sh_degree(self):
+        return lib().dio_trainer_sh_degree(self.handle)
+
+    @sh_degree.setter
+    def sh_degree(self, value):
+        _check(lib().dio_trainer_set_sh_degree(self.handle, int(value)))
+
+    @property
+    def densifications(self):
+        out = []
+        for k in range(lib().dio_trainer_densification_count(self.handle)):
+            d = _Densification()
+            _check(lib().dio_trainer_densification(self.handle, k, ctypes.byref(d)))
+            out.append({name: getattr(d, name) for name, _ in _Densification._fields_})
+        return out
+
+    @property
+    def buffers(self):
+        return {"params": _view(lib().dio_trainer_params(self.handle), self),
+                "moments": _view(lib().dio_trainer_moments(self.handle), self),
+                "stats": _view(lib().dio_trainer_stats(self.handle), self)}
+
+    @property
+    def renderer(self):
+        return GaussianRenderer(self.ctx, handle=ctypes.c_void_p(lib().dio_trainer_renderer(self.handle)), keep=self)
+
+    def destroy(self):
+        if self.handle:
+            lib().dio_trainer_destroy(self.handle)
+            self.handle = None

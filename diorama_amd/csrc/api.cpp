@@ -1,0 +1,1367 @@
+// The C ABI (include/diorama.h): the reference's operator classes as opaque
+// handles over one HIP stream. The host logic follows src/prefixSum.ts,
+// src/radixSort.ts, src/tileRasterizer.ts, src/gaussianRenderer.ts,
+// src/trainingLoss.ts and src/trainer.ts; the arithmetic is in the HIP kernels.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/diorama.h"
+#include "dio_internal.hpp"
+
+namespace dio {
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(DIO_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace dio
+
+using dio::Error;
+
+namespace {
+thread_local std::string g_error;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return DIO_OK;
+  } catch (const Error& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_error = "host allocation failed";
+    return DIO_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return DIO_ERR_INTERNAL;
+  }
+}
+
+void invalid(const std::string& message) { throw Error(DIO_ERR_INVALID, message); }
+}  // namespace
+
+struct dio_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct dio_buf {
+  dio_ctx* ctx = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+namespace {
+
+dio_buf* alloc(dio_ctx* ctx, size_t bytes, bool zero = false) {
+  auto* b = new dio_buf;
+  b->ctx = ctx;
+  b->bytes = bytes;
+  hipError_t e = hipMalloc(&b->ptr, std::max<size_t>(bytes, 256));
+  if (e != hipSuccess) {
+    delete b;
+    (void)hipGetLastError();
+    throw Error(DIO_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  if (zero) dio::fill_u32(ctx->stream, static_cast<uint32_t*>(b->ptr), 0u, (std::max<size_t>(bytes, 256) + 3) / 4);
+  return b;
+}
+
+void release(dio_buf*& b) {
+  if (b) {
+    (void)hipFree(b->ptr);
+    delete b;
+    b = nullptr;
+  }
+}
+
+template <class T>
+T* dev(dio_buf* b) {
+  return b ? static_cast<T*>(b->ptr) : nullptr;
+}
+
+void need(const void* p, const char* what) {
+  if (!p) invalid(std::string(what) + " is null");
+}
+
+void sync(dio_ctx* ctx) { DIO_HIP(hipStreamSynchronize(ctx->stream)); }
+
+void to_camera(const dio_camera& c, dio::CameraArgs& out) {
+  std::memcpy(out.view, c.view, sizeof(out.view));
+  std::memcpy(out.proj, c.proj, sizeof(out.proj));
+  out.viewport[0] = c.viewport[0];
+  out.viewport[1] = c.viewport[1];
+}
+
+uint32_t next_pow2_capacity(uint32_t count) {
+  double want = std::ceil(std::log2(count * 1.25));
+  return static_cast<uint32_t>(std::min(std::pow(2.0, want), 4294967295.0));
+}
+
+}  // namespace
+
+// =================================================================== runtime
+
+extern "C" const char* dio_last_error(void) { return g_error.c_str(); }
+extern "C" const char* dio_version(void) { return "diorama-mi355x 0.1 gfx950"; }
+
+extern "C" int dio_ctx_create(int device, dio_ctx** out) {
+  return guard([&] {
+    need(out, "out");
+    DIO_HIP(hipSetDevice(device));
+    auto* c = new dio_ctx;
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      dio::check_hip(e, "hipStreamCreate");
+    }
+    *out = c;
+  });
+}
+
+extern "C" int dio_ctx_synchronize(dio_ctx* ctx) {
+  return guard([&] {
+    need(ctx, "ctx");
+    sync(ctx);
+  });
+}
+
+extern "C" void dio_ctx_destroy(dio_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" int dio_buf_create(dio_ctx* ctx, size_t bytes, dio_buf** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    *out = alloc(ctx, bytes, true);
+  });
+}
+
+extern "C" int dio_buf_upload(dio_buf* buf, size_t offset, const void* src, size_t bytes) {
+  return guard([&] {
+    need(buf, "buffer");
+    if (offset + bytes > buf->bytes) invalid("upload past the end of the buffer");
+    if (bytes == 0) return;
+    need(src, "source");
+    DIO_HIP(hipMemcpyAsync(static_cast<char*>(buf->ptr) + offset, src, bytes, hipMemcpyHostToDevice, buf->ctx->stream));
+    sync(buf->ctx);  // the caller may reuse its host memory at once, as with queue.writeBuffer
+  });
+}
+
+extern "C" int dio_buf_download(const dio_buf* buf, size_t offset, void* dst, size_t bytes) {
+  return guard([&] {
+    need(buf, "buffer");
+    if (offset + bytes > buf->bytes) invalid("download past the end of the buffer");
+    if (bytes == 0) return;
+    need(dst, "destination");
+    DIO_HIP(hipMemcpyAsync(dst, static_cast<const char*>(buf->ptr) + offset, bytes, hipMemcpyDeviceToHost,
+                           buf->ctx->stream));
+    sync(buf->ctx);
+  });
+}
+
+extern "C" int dio_buf_clear(dio_buf* buf) {
+  return guard([&] {
+    need(buf, "buffer");
+    dio::fill_u32(buf->ctx->stream, static_cast<uint32_t*>(buf->ptr), 0u, (buf->bytes + 3) / 4);
+  });
+}
+
+extern "C" size_t dio_buf_size(const dio_buf* buf) { return buf ? buf->bytes : 0; }
+
+extern "C" void dio_buf_destroy(dio_buf* buf) {
+  if (!buf) return;
+  (void)hipStreamSynchronize(buf->ctx->stream);
+  release(buf);
+}
+
+// ================================================================= PrefixSum
+
+struct dio_prefix_sum {
+  dio_ctx* ctx = nullptr;
+  dio_buf* data = nullptr;
+  dio_buf* count = nullptr;
+  uint32_t capacity = 0;
+  dio_buf* total = nullptr;
+  dio_buf* ws = nullptr;
+};
+
+extern "C" int dio_prefix_sum_create(dio_ctx* ctx, dio_buf* data, uint32_t capacity, dio_buf* count,
+                                     dio_prefix_sum** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(data, "data");
+    need(out, "out");
+    if (static_cast<size_t>(capacity) * 4 > data->bytes) invalid("the data buffer is smaller than the capacity");
+    auto* s = new dio_prefix_sum;
+    s->ctx = ctx;
+    s->data = data;
+    s->count = count;
+    s->capacity = capacity;
+    try {
+      s->total = alloc(ctx, 16, true);
+      s->ws = alloc(ctx, dio::scan_workspace_bytes(capacity), true);
+    } catch (...) {
+      release(s->total);
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+extern "C" dio_buf* dio_prefix_sum_total(dio_prefix_sum* scan) { return scan ? scan->total : nullptr; }
+
+extern "C" int dio_prefix_sum_encode(dio_prefix_sum* s) {
+  return guard([&] {
+    need(s, "prefix sum");
+    dio::scan_exclusive(s->ctx->stream, dev<uint32_t>(s->data), dev<uint32_t>(s->data), dev<uint32_t>(s->count),
+                        s->capacity, dev<uint32_t>(s->total), dio::scan_workspace(s->ws->ptr, s->capacity));
+  });
+}
+
+extern "C" void dio_prefix_sum_destroy(dio_prefix_sum* s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s->ctx->stream);
+  release(s->total);
+  release(s->ws);
+  delete s;
+}
+
+// ================================================================= RadixSort
+
+struct dio_radix_sort {
+  dio_ctx* ctx = nullptr;
+  uint32_t capacity = 0, high_bits = 0;
+  bool wide = false;
+  dio_buf* count = nullptr;
+  dio_buf *keys = nullptr, *values = nullptr, *alt_keys = nullptr, *alt_values = nullptr, *ws = nullptr;
+};
+
+extern "C" int dio_radix_sort_create(dio_ctx* ctx, uint32_t capacity, uint32_t high_bits, dio_buf* count,
+                                     dio_radix_sort** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    if (high_bits > 32) invalid("highBits must be at most 32");
+    std::unique_ptr<dio_radix_sort> s(new dio_radix_sort);
+    s->ctx = ctx;
+    s->capacity = capacity;
+    s->high_bits = high_bits;
+    s->wide = high_bits > 0;
+    s->count = count;
+    const size_t kb = s->wide ? 8 : 4;
+    try {
+      s->keys = alloc(ctx, static_cast<size_t>(capacity) * kb, true);
+      s->values = alloc(ctx, static_cast<size_t>(capacity) * 4, true);
+      s->alt_keys = alloc(ctx, static_cast<size_t>(capacity) * kb);
+      s->alt_values = alloc(ctx, static_cast<size_t>(capacity) * 4);
+      s->ws = alloc(ctx, dio::sort_workspace_bytes(capacity), true);
+    } catch (...) {
+      release(s->keys); release(s->values); release(s->alt_keys); release(s->alt_values); release(s->ws);
+      throw;
+    }
+    *out = s.release();
+  });
+}
+
+extern "C" dio_buf* dio_radix_sort_keys(dio_radix_sort* s) { return s ? s->keys : nullptr; }
+extern "C" dio_buf* dio_radix_sort_values(dio_radix_sort* s) { return s ? s->values : nullptr; }
+extern "C" uint32_t dio_radix_sort_capacity(const dio_radix_sort* s) { return s ? s->capacity : 0; }
+
+extern "C" int dio_radix_sort_encode(dio_radix_sort* s) {
+  return guard([&] {
+    need(s, "radix sort");
+    // Four passes over the low word, then two per 16 bits of the high word
+    // (src/radixSort.ts:103-104): an even count, so the result lands back in place.
+    const uint32_t end = s->wide ? 32u + 16u * ((s->high_bits + 15) / 16) : 32u;
+    const bool flipped = dio::radix_sort(s->ctx->stream, s->wide, s->keys->ptr, dev<uint32_t>(s->values),
+                                         s->alt_keys->ptr, dev<uint32_t>(s->alt_values), dev<uint32_t>(s->count),
+                                         s->capacity, 0, end, dio::sort_workspace(s->ws->ptr, s->capacity));
+    if (flipped) throw Error(DIO_ERR_INTERNAL, "radix sort ended in its scratch buffers");
+  });
+}
+
+extern "C" void dio_radix_sort_destroy(dio_radix_sort* s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s->ctx->stream);
+  release(s->keys); release(s->values); release(s->alt_keys); release(s->alt_values); release(s->ws);
+  delete s;
+}
+
+// ============================================================ TileRasterizer
+
+struct dio_tile_rasterizer {
+  dio_ctx* ctx = nullptr;
+  dio_buf* splats = nullptr;  // not owned
+  uint32_t count = 0;
+  uint32_t capacity = 0;
+  // Per splat: tile counts, depth sort keys/values (+ scratch), pair offsets.
+  dio_buf *tile_counts = nullptr, *depth_keys = nullptr, *depth_values = nullptr, *depth_alt_keys = nullptr,
+          *depth_alt_values = nullptr, *offsets = nullptr, *splat_scan_ws = nullptr, *splat_sort_ws = nullptr;
+  dio_buf* pair_total = nullptr;  // element 0: pairs this frame needs
+  // Pairs: tile index and splat, ping-pong buffers for the tile sort.
+  dio_buf *pair_tiles = nullptr, *pair_splats = nullptr, *alt_tiles = nullptr, *alt_splats = nullptr,
+          *pair_sort_ws = nullptr;
+  bool sorted_in_alt = false;
+  dio_buf* ranges = nullptr;
+  uint32_t ranges_tiles = 0;
+  dio_buf *pixel_color = nullptr, *pixel_last = nullptr;
+  uint32_t pixel_count = 0;
+  uint32_t last_pair_count = 0;
+  bool readback_queued = false;
+  uint32_t* readback_host = nullptr;  // pinned
+  hipEvent_t readback_done = nullptr;
+  bool framed = false;
+  uint32_t frame_w = 0, frame_h = 0;
+  float frame_bg[3] = {0.f, 0.f, 0.f};
+};
+
+namespace {
+
+void free_pairs(dio_tile_rasterizer* t) {
+  release(t->pair_tiles); release(t->pair_splats); release(t->alt_tiles); release(t->alt_splats);
+  release(t->pair_sort_ws);
+}
+
+void create_pairs(dio_tile_rasterizer* t, uint32_t capacity) {
+  t->capacity = capacity;
+  t->pair_tiles = alloc(t->ctx, static_cast<size_t>(capacity) * 4);
+  t->pair_splats = alloc(t->ctx, static_cast<size_t>(capacity) * 4, true);
+  t->alt_tiles = alloc(t->ctx, static_cast<size_t>(capacity) * 4);
+  t->alt_splats = alloc(t->ctx, static_cast<size_t>(capacity) * 4, true);
+  t->pair_sort_ws = alloc(t->ctx, dio::sort_workspace_bytes(capacity), true);
+}
+
+void destroy_tiles(dio_tile_rasterizer* t) {
+  if (!t) return;
+  (void)hipStreamSynchronize(t->ctx->stream);
+  release(t->tile_counts); release(t->depth_keys); release(t->depth_values); release(t->depth_alt_keys);
+  release(t->depth_alt_values); release(t->offsets); release(t->splat_scan_ws); release(t->splat_sort_ws);
+  release(t->pair_total);
+  free_pairs(t);
+  release(t->ranges);
+  release(t->pixel_color);
+  release(t->pixel_last);
+  if (t->readback_host) (void)hipHostFree(t->readback_host);
+  if (t->readback_done) (void)hipEventDestroy(t->readback_done);
+  delete t;
+}
+
+dio_tile_rasterizer* create_tiles(dio_ctx* ctx, dio_buf* splats, uint32_t count, uint32_t initial) {
+  auto* t = new dio_tile_rasterizer;
+  t->ctx = ctx;
+  t->splats = splats;
+  t->count = count;
+  try {
+    const size_t n = std::max<uint32_t>(count, 1);
+    t->tile_counts = alloc(ctx, n * 4, true);
+    t->depth_keys = alloc(ctx, n * 4);
+    t->depth_values = alloc(ctx, n * 4);
+    t->depth_alt_keys = alloc(ctx, n * 4);
+    t->depth_alt_values = alloc(ctx, n * 4);
+    t->offsets = alloc(ctx, n * 4);
+    t->splat_scan_ws = alloc(ctx, dio::scan_workspace_bytes(count), true);
+    t->splat_sort_ws = alloc(ctx, dio::sort_workspace_bytes(count), true);
+    t->pair_total = alloc(ctx, 16, true);
+    create_pairs(t, initial ? initial : std::max<uint32_t>(1u << 16, count * 8u));
+    DIO_HIP(hipHostMalloc(reinterpret_cast<void**>(&t->readback_host), 16, hipHostMallocDefault));
+    DIO_HIP(hipEventCreateWithFlags(&t->readback_done, hipEventDisableTiming));
+  } catch (...) {
+    destroy_tiles(t);
+    throw;
+  }
+  return t;
+}
+
+void ensure_frame_buffers(dio_tile_rasterizer* t, uint32_t tiles, uint32_t pixels) {
+  if (!t->ranges || t->ranges_tiles < tiles) {
+    release(t->ranges);
+    t->ranges = alloc(t->ctx, static_cast<size_t>(tiles) * 8, true);
+    t->ranges_tiles = tiles;
+  }
+  if (!t->pixel_color || t->pixel_count < pixels) {
+    release(t->pixel_color);
+    release(t->pixel_last);
+    t->pixel_color = alloc(t->ctx, static_cast<size_t>(pixels) * 16, true);
+    t->pixel_last = alloc(t->ctx, static_cast<size_t>(pixels) * 4, true);
+    t->pixel_count = pixels;
+  }
+}
+
+// Bits of the tile index the tile sort must look at, in whole 8-bit digits.
+uint32_t tile_sort_bits(uint32_t tiles) {
+  uint32_t bits = 0;
+  while ((1u << bits) < tiles && bits < 32) bits++;
+  return std::max<uint32_t>(8u, (bits + 7) / 8 * 8);
+}
+
+// The tile stage of TileRasterizer.encode (src/tileRasterizer.ts:127-192) after
+// the tile counts and depth keys exist.
+void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t h, const float bg[3],
+                  bool counts_ready) {
+  const uint32_t tiles_x = dio::ceil_div(w, dio::kTile), tiles_y = dio::ceil_div(h, dio::kTile);
+  const uint32_t tiles = tiles_x * tiles_y;
+  if (tiles > 65536u) {
+    invalid("A " + std::to_string(w) + "×" + std::to_string(h) + " viewport has more tiles than the sort can tell apart.");
+  }
+  if (target && target->bytes < static_cast<size_t>(w) * h * 4) invalid("the target is smaller than the viewport");
+  ensure_frame_buffers(t, tiles, w * h);
+  hipStream_t s = t->ctx->stream;
+  const float* splats = dev<float>(t->splats);
+  if (!counts_ready)
+    dio::count_tiles(s, splats, t->count, w, h, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
+                     dev<uint32_t>(t->depth_values));
+  // Depth order of the splats (culled ones last), then each splat's first pair slot.
+  if (dio::radix_sort(s, false, t->depth_keys->ptr, dev<uint32_t>(t->depth_values), t->depth_alt_keys->ptr,
+                      dev<uint32_t>(t->depth_alt_values), nullptr, t->count, 0, 32,
+                      dio::sort_workspace(t->splat_sort_ws->ptr, t->count)))
+    throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
+  dio::scan_exclusive_gather(s, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_values),
+                             dev<uint32_t>(t->offsets), nullptr, t->count, dev<uint32_t>(t->pair_total),
+                             dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
+  dio::duplicate(s, splats, dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets), dev<uint32_t>(t->tile_counts),
+                 t->count, w, h, t->capacity, dev<uint32_t>(t->pair_tiles), dev<uint32_t>(t->pair_splats));
+  t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats), t->alt_tiles->ptr,
+                                     dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total), t->capacity, 0,
+                                     tile_sort_bits(tiles), dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity));
+  dio_buf* sorted_tiles = t->sorted_in_alt ? t->alt_tiles : t->pair_tiles;
+  dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
+  dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(t->pair_total), t->capacity, tiles,
+                   dev<uint32_t>(t->ranges));
+  dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), w, h, bg, dev<uint32_t>(target),
+                 dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last));
+  t->framed = true;
+  t->frame_w = w;
+  t->frame_h = h;
+  for (int k = 0; k < 3; k++) t->frame_bg[k] = bg[k];
+  if (!t->readback_queued) {
+    DIO_HIP(hipMemcpyAsync(t->readback_host, t->pair_total->ptr, 4, hipMemcpyDeviceToHost, s));
+    DIO_HIP(hipEventRecord(t->readback_done, s));
+    t->readback_queued = true;
+  }
+}
+
+// afterSubmit: `wait` blocks for the readback; otherwise only a finished one is taken.
+void tiles_after_submit(dio_tile_rasterizer* t, bool wait) {
+  if (!t->readback_queued) return;
+  if (wait) {
+    DIO_HIP(hipEventSynchronize(t->readback_done));
+  } else {
+    hipError_t e = hipEventQuery(t->readback_done);
+    if (e == hipErrorNotReady) return;
+    dio::check_hip(e, "hipEventQuery");
+  }
+  t->readback_queued = false;
+  t->last_pair_count = t->readback_host[0];
+  if (t->last_pair_count > t->capacity) {
+    sync(t->ctx);
+    free_pairs(t);
+    create_pairs(t, next_pow2_capacity(t->last_pair_count));
+  }
+}
+
+// encodeBackward (src/tileRasterizer.ts:200-222) of the frame encoded last.
+void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pixel_grads, dio_buf* splat_grads) {
+  hipStream_t s = t->ctx->stream;
+  dio::fill_u32(s, dev<uint32_t>(splat_grads), 0u, static_cast<size_t>(t->count) * 9);  // the pass adds into it
+  dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
+  dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), w, h,
+                          t->frame_bg, dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
+                          dev<float>(pixel_grads), dev<float>(splat_grads));
+}
+
+}  // namespace
+
+extern "C" int dio_tile_rasterizer_create(dio_ctx* ctx, dio_buf* splats, uint32_t count,
+                                          uint32_t initial_pair_capacity, dio_tile_rasterizer** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(splats, "splats");
+    need(out, "out");
+    if (splats->bytes < static_cast<size_t>(count) * dio::kFloatsPerSplat * 4)
+      invalid("the splat buffer is smaller than count splats");
+    *out = create_tiles(ctx, splats, count, initial_pair_capacity);
+  });
+}
+
+extern "C" int dio_tile_rasterizer_encode(dio_tile_rasterizer* t, dio_buf* target, uint32_t width, uint32_t height,
+                                          const float background[3]) {
+  return guard([&] {
+    need(t, "tile rasterizer");
+    need(background, "background");
+    encode_tiles(t, target, width, height, background, false);
+  });
+}
+
+extern "C" int dio_tile_rasterizer_encode_backward(dio_tile_rasterizer* t, uint32_t width, uint32_t height,
+                                                   dio_buf* pixel_grads, dio_buf* splat_grads) {
+  return guard([&] {
+    need(t, "tile rasterizer");
+    if (!t->framed) invalid("Encode a frame before its backward pass.");
+    need(pixel_grads, "pixel gradients");
+    need(splat_grads, "splat gradients");
+    if (splat_grads->bytes < static_cast<size_t>(t->count) * 36) invalid("the splat gradient buffer is too small");
+    if (pixel_grads->bytes < static_cast<size_t>(width) * height * 16) invalid("the pixel gradient buffer is too small");
+    backward_tiles(t, width, height, pixel_grads, splat_grads);
+  });
+}
+
+extern "C" int dio_tile_rasterizer_after_submit(dio_tile_rasterizer* t) {
+  return guard([&] {
+    need(t, "tile rasterizer");
+    tiles_after_submit(t, true);
+  });
+}
+
+extern "C" uint32_t dio_tile_rasterizer_last_pair_count(const dio_tile_rasterizer* t) {
+  return t ? t->last_pair_count : 0;
+}
+extern "C" uint32_t dio_tile_rasterizer_pair_capacity(const dio_tile_rasterizer* t) { return t ? t->capacity : 0; }
+extern "C" dio_buf* dio_tile_rasterizer_pair_splats(dio_tile_rasterizer* t) {
+  return t ? (t->sorted_in_alt ? t->alt_splats : t->pair_splats) : nullptr;
+}
+extern "C" dio_buf* dio_tile_rasterizer_ranges(dio_tile_rasterizer* t) { return t ? t->ranges : nullptr; }
+extern "C" dio_buf* dio_tile_rasterizer_pixel_color(dio_tile_rasterizer* t) { return t ? t->pixel_color : nullptr; }
+extern "C" dio_buf* dio_tile_rasterizer_pixel_last(dio_tile_rasterizer* t) { return t ? t->pixel_last : nullptr; }
+extern "C" void dio_tile_rasterizer_destroy(dio_tile_rasterizer* t) { destroy_tiles(t); }
+
+// ========================================================== GaussianRenderer
+
+struct dio_renderer {
+  dio_ctx* ctx = nullptr;
+  uint32_t initial_pair_capacity = 0;
+  struct Scene {
+    uint32_t count = 0;
+    dio_buf *gaussians = nullptr, *splats = nullptr, *sh = nullptr;
+    uint32_t degree = 0;
+    dio_tile_rasterizer* rasterizer = nullptr;
+    dio_buf *grad_splats = nullptr, *grad_gaussians = nullptr, *grad_sh = nullptr;
+  };
+  Scene* scene = nullptr;
+  dio_tile_rasterizer* draw_rasterizer = nullptr;
+  dio::CameraArgs camera{};
+  bool preprocessed = false;
+};
+
+namespace {
+
+void destroy_scene(dio_renderer* r) {
+  auto* sc = r->scene;
+  if (!sc) return;
+  (void)hipStreamSynchronize(r->ctx->stream);
+  release(sc->gaussians); release(sc->splats); release(sc->sh);
+  release(sc->grad_splats); release(sc->grad_gaussians); release(sc->grad_sh);
+  destroy_tiles(sc->rasterizer);
+  delete sc;
+  r->scene = nullptr;
+}
+
+// Takes ownership of `gaussians` and `sh` (createScene, src/gaussianRenderer.ts:246-270).
+void create_scene(dio_renderer* r, uint32_t count, dio_buf* gaussians, dio_buf* sh, uint32_t degree,
+                  uint32_t pair_capacity) {
+  auto* sc = new dio_renderer::Scene;
+  sc->count = count;
+  sc->gaussians = gaussians;
+  sc->sh = sh;
+  sc->degree = degree;
+  r->scene = sc;
+  sc->splats = alloc(r->ctx, static_cast<size_t>(count) * dio::kFloatsPerSplat * 4, true);
+  sc->rasterizer = create_tiles(r->ctx, sc->splats, count, pair_capacity);
+}
+
+void fill_background(dio_ctx* ctx, dio_buf* target, uint32_t w, uint32_t h, const float bg[3]) {
+  if (!target) return;
+  auto u8 = [](float v) { return static_cast<uint32_t>(std::min(std::max(v, 0.f), 1.f) * 255.f + 0.5f); };
+  dio::fill_u32(ctx->stream, dev<uint32_t>(target), u8(bg[0]) | (u8(bg[1]) << 8) | (u8(bg[2]) << 16) | (255u << 24),
+                std::min<size_t>(static_cast<size_t>(w) * h, target->bytes / 4));
+}
+
+void renderer_preprocess(dio_renderer* r, const dio_camera& camera) {
+  to_camera(camera, r->camera);
+  auto* sc = r->scene;
+  if (!sc) return;
+  auto* t = sc->rasterizer;
+  dio::preprocess(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
+                  dev<float>(sc->splats), dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
+                  dev<uint32_t>(t->depth_values));
+  r->preprocessed = true;
+}
+
+void renderer_tiles(dio_renderer* r, dio_buf* target, const float bg[3]) {
+  const uint32_t w = static_cast<uint32_t>(r->camera.viewport[0]), h = static_cast<uint32_t>(r->camera.viewport[1]);
+  if (!r->scene) {
+    fill_background(r->ctx, target, w, h, bg);
+    return;
+  }
+  if (!r->preprocessed) invalid("encode_preprocess must come before encode_tiles");
+  r->preprocessed = false;
+  encode_tiles(r->scene->rasterizer, target, w, h, bg, true);
+}
+
+void renderer_backward(dio_renderer* r, dio_buf* pixel_grads) {
+  auto* sc = r->scene;
+  if (!sc) return;
+  if (!sc->grad_splats) {
+    sc->grad_splats = alloc(r->ctx, static_cast<size_t>(sc->count) * 9 * 4, true);
+    sc->grad_gaussians = alloc(r->ctx, static_cast<size_t>(sc->count) * dio::kFloatsPerGaussian * 4, true);
+    sc->grad_sh = alloc(r->ctx, static_cast<size_t>(sc->count) * dio::kShRest * 4, true);
+  }
+  auto* t = sc->rasterizer;
+  if (!t->framed) invalid("Encode a frame before its backward pass.");
+  backward_tiles(t, t->frame_w, t->frame_h, pixel_grads, sc->grad_splats);
+  dio::preprocess_backward(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count,
+                           r->camera, dev<float>(sc->splats), dev<float>(sc->grad_splats),
+                           dev<float>(sc->grad_gaussians), dev<float>(sc->grad_sh));
+}
+
+}  // namespace
+
+extern "C" int dio_renderer_create(dio_ctx* ctx, uint32_t initial_pair_capacity, dio_renderer** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    auto* r = new dio_renderer;
+    r->ctx = ctx;
+    r->initial_pair_capacity = initial_pair_capacity;
+    *out = r;
+  });
+}
+
+extern "C" int dio_renderer_set_gaussians(dio_renderer* r, const float* gaussians, uint32_t count, const float* sh_rest,
+                                          uint32_t sh_degree) {
+  return guard([&] {
+    need(r, "renderer");
+    if (count == 0) {
+      destroy_scene(r);
+      return;
+    }
+    need(gaussians, "gaussians");
+    dio_buf* g = alloc(r->ctx, static_cast<size_t>(count) * dio::kFloatsPerGaussian * 4);
+    dio_buf* s = nullptr;
+    try {
+      s = alloc(r->ctx, sh_rest ? static_cast<size_t>(count) * dio::kShRest * 4 : 16, true);
+      DIO_HIP(hipMemcpyAsync(g->ptr, gaussians, g->bytes, hipMemcpyHostToDevice, r->ctx->stream));
+      if (sh_rest) DIO_HIP(hipMemcpyAsync(s->ptr, sh_rest, s->bytes, hipMemcpyHostToDevice, r->ctx->stream));
+      sync(r->ctx);
+    } catch (...) {
+      release(g);
+      release(s);
+      throw;
+    }
+    destroy_scene(r);
+    // Without coefficients the shader never reads them; the degree is then 0.
+    create_scene(r, count, g, s, sh_rest ? sh_degree : 0, r->initial_pair_capacity);
+    if (!sh_rest) r->scene->degree = 0;
+  });
+}
+
+extern "C" int dio_renderer_adopt_gaussians(dio_renderer* r, uint32_t count, dio_buf* gaussians, dio_buf* sh,
+                                            uint32_t sh_degree) {
+  return guard([&] {
+    need(r, "renderer");
+    need(gaussians, "gaussians");
+    need(sh, "sh");
+    const uint32_t capacity = r->scene ? r->scene->rasterizer->capacity : r->initial_pair_capacity;
+    destroy_scene(r);
+    create_scene(r, count, gaussians, sh, sh_degree, capacity);
+  });
+}
+
+extern "C" int dio_renderer_set_sh_degree(dio_renderer* r, uint32_t degree) {
+  return guard([&] {
+    need(r, "renderer");
+    if (degree > 3) invalid("the SH degree is at most 3");
+    if (r->scene) r->scene->degree = degree;
+  });
+}
+
+extern "C" int dio_renderer_encode(dio_renderer* r, dio_buf* target, const dio_camera* camera,
+                                   const float background[3]) {
+  return guard([&] {
+    need(r, "renderer");
+    need(camera, "camera");
+    need(background, "background");
+    renderer_preprocess(r, *camera);
+    renderer_tiles(r, target, background);
+  });
+}
+
+extern "C" int dio_renderer_encode_preprocess(dio_renderer* r, const dio_camera* camera) {
+  return guard([&] {
+    need(r, "renderer");
+    need(camera, "camera");
+    renderer_preprocess(r, *camera);
+  });
+}
+
+extern "C" int dio_renderer_encode_tiles(dio_renderer* r, dio_buf* target, const float background[3]) {
+  return guard([&] {
+    need(r, "renderer");
+    need(background, "background");
+    renderer_tiles(r, target, background);
+  });
+}
+
+extern "C" int dio_renderer_encode_draw(dio_renderer* r, dio_buf* target, dio_buf* splats, uint32_t count,
+                                        uint32_t width, uint32_t height, const float background[3]) {
+  return guard([&] {
+    need(r, "renderer");
+    need(splats, "splats");
+    need(background, "background");
+    if (splats->bytes < static_cast<size_t>(count) * dio::kFloatsPerSplat * 4)
+      invalid("the splat buffer is smaller than count splats");
+    destroy_tiles(r->draw_rasterizer);
+    r->draw_rasterizer = nullptr;
+    r->draw_rasterizer = create_tiles(r->ctx, splats, count, r->initial_pair_capacity);
+    encode_tiles(r->draw_rasterizer, target, width, height, background, false);
+  });
+}
+
+extern "C" int dio_renderer_encode_backward(dio_renderer* r, dio_buf* pixel_grads) {
+  return guard([&] {
+    need(r, "renderer");
+    need(pixel_grads, "pixel gradients");
+    renderer_backward(r, pixel_grads);
+  });
+}
+
+extern "C" int dio_renderer_after_submit(dio_renderer* r) {
+  return guard([&] {
+    need(r, "renderer");
+    if (r->scene) tiles_after_submit(r->scene->rasterizer, true);
+  });
+}
+
+extern "C" uint32_t dio_renderer_count(const dio_renderer* r) { return r && r->scene ? r->scene->count : 0; }
+extern "C" dio_buf* dio_renderer_splat_buffer(dio_renderer* r) { return r && r->scene ? r->scene->splats : nullptr; }
+extern "C" dio_buf* dio_renderer_gaussian_buffer(dio_renderer* r) {
+  return r && r->scene ? r->scene->gaussians : nullptr;
+}
+extern "C" dio_buf* dio_renderer_sh_buffer(dio_renderer* r) { return r && r->scene ? r->scene->sh : nullptr; }
+extern "C" dio_tile_rasterizer* dio_renderer_tiles(dio_renderer* r) {
+  if (!r) return nullptr;
+  if (r->draw_rasterizer) return r->draw_rasterizer;
+  return r->scene ? r->scene->rasterizer : nullptr;
+}
+extern "C" dio_buf* dio_renderer_pixel_buffer(dio_renderer* r) {
+  return r && r->scene ? r->scene->rasterizer->pixel_color : nullptr;
+}
+extern "C" dio_buf* dio_renderer_pixel_last_buffer(dio_renderer* r) {
+  return r && r->scene ? r->scene->rasterizer->pixel_last : nullptr;
+}
+extern "C" dio_buf* dio_renderer_gradient_gaussians(dio_renderer* r) {
+  return r && r->scene ? r->scene->grad_gaussians : nullptr;
+}
+extern "C" dio_buf* dio_renderer_gradient_sh(dio_renderer* r) { return r && r->scene ? r->scene->grad_sh : nullptr; }
+extern "C" dio_buf* dio_renderer_gradient_splats(dio_renderer* r) {
+  return r && r->scene ? r->scene->grad_splats : nullptr;
+}
+
+extern "C" void dio_renderer_destroy(dio_renderer* r) {
+  if (!r) return;
+  destroy_scene(r);
+  destroy_tiles(r->draw_rasterizer);
+  delete r;
+}
+
+// ============================================================== TrainingLoss
+
+struct dio_training_loss {
+  dio_ctx* ctx = nullptr;
+  uint32_t width = 0, height = 0;
+  dio_buf *partials = nullptr, *pixel_grads = nullptr, *pixel_loss = nullptr;
+};
+
+namespace {
+void destroy_loss(dio_training_loss* l) {
+  if (!l) return;
+  (void)hipStreamSynchronize(l->ctx->stream);
+  release(l->partials); release(l->pixel_grads); release(l->pixel_loss);
+  delete l;
+}
+
+double loss_read(dio_training_loss* l) {
+  std::vector<float> shares(static_cast<size_t>(l->width) * l->height);
+  DIO_HIP(hipMemcpyAsync(shares.data(), l->pixel_loss->ptr, shares.size() * 4, hipMemcpyDeviceToHost, l->ctx->stream));
+  sync(l->ctx);
+  double sum = 0;
+  for (float v : shares) sum += v;
+  return sum + 0.2;
+}
+}  // namespace
+
+extern "C" int dio_training_loss_create(dio_ctx* ctx, uint32_t width, uint32_t height, dio_training_loss** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    auto* l = new dio_training_loss;
+    l->ctx = ctx;
+    l->width = width;
+    l->height = height;
+    const size_t px = std::max<size_t>(static_cast<size_t>(width) * height, 1);
+    try {
+      l->partials = alloc(ctx, px * 48);
+      l->pixel_grads = alloc(ctx, px * 16, true);
+      l->pixel_loss = alloc(ctx, px * 4, true);
+    } catch (...) {
+      destroy_loss(l);
+      throw;
+    }
+    *out = l;
+  });
+}
+
+extern "C" int dio_training_loss_encode(dio_training_loss* l, dio_buf* rendered, dio_buf* target) {
+  return guard([&] {
+    need(l, "loss");
+    need(rendered, "rendered");
+    need(target, "target");
+    const size_t px = static_cast<size_t>(l->width) * l->height;
+    if (rendered->bytes < px * 16 || target->bytes < px * 4) invalid("the loss inputs are smaller than the image");
+    dio::training_loss(l->ctx->stream, l->width, l->height, dev<float>(rendered), dev<uint32_t>(target),
+                       dev<float>(l->partials), dev<float>(l->pixel_grads), dev<float>(l->pixel_loss));
+  });
+}
+
+extern "C" int dio_training_loss_read(dio_training_loss* l, double* out) {
+  return guard([&] {
+    need(l, "loss");
+    need(out, "out");
+    *out = loss_read(l);
+  });
+}
+
+extern "C" dio_buf* dio_training_loss_pixel_grads(dio_training_loss* l) { return l ? l->pixel_grads : nullptr; }
+extern "C" dio_buf* dio_training_loss_pixel_loss(dio_training_loss* l) { return l ? l->pixel_loss : nullptr; }
+extern "C" void dio_training_loss_destroy(dio_training_loss* l) { destroy_loss(l); }
+
+// =================================================================== Trainer
+
+namespace {
+
+// mulberry32 (src/random.ts:3-12).
+struct Mulberry32 {
+  uint32_t state;
+  explicit Mulberry32(uint32_t seed) : state(seed) {}
+  double next() {
+    state += 0x6d2b79f5u;
+    uint32_t t = state;
+    t = (t ^ (t >> 15)) * (t | 1u);
+    t ^= t + (t ^ (t >> 7)) * (t | 61u);
+    return static_cast<double>(t ^ (t >> 14)) / 4294967296.0;
+  }
+};
+
+// CPU image metrics on 4-value pixels (src/metrics.ts:5-72).
+double psnr4(const std::vector<double>& a, const std::vector<double>& b) {
+  double se = 0;
+  for (size_t i = 0; i < a.size(); i += 4)
+    for (int c = 0; c < 3; c++) {
+      double d = a[i + c] - b[i + c];
+      se += d * d;
+    }
+  return 10.0 * std::log10(1.0 / (se / (3.0 * a.size() / 4)));
+}
+
+double ssim4(const std::vector<double>& a, const std::vector<double>& b, int w, int h) {
+  double window[11], sum = 0;
+  for (int k = 0; k < 11; k++) sum += window[k] = std::exp(-((k - 5) * (k - 5)) / (2 * 1.5 * 1.5));
+  for (double& v : window) v /= sum;
+  const size_t n = static_cast<size_t>(w) * h;
+  std::vector<double> maps[5], scratch(n);
+  for (auto& m : maps) m.resize(n);
+  auto blur = [&](const std::vector<double>& src, std::vector<double>& dst, int dx, int dy) {
+    for (int y = 0; y < h; y++)
+      for (int x = 0; x < w; x++) {
+        double s = 0;
+        for (int k = -5; k <= 5; k++) {
+          int sx = x + k * dx, sy = y + k * dy;
+          if (sx >= 0 && sx < w && sy >= 0 && sy < h) s += window[k + 5] * src[static_cast<size_t>(sy) * w + sx];
+        }
+        dst[static_cast<size_t>(y) * w + x] = s;
+      }
+  };
+  double total = 0;
+  for (int c = 0; c < 3; c++) {
+    for (size_t p = 0; p < n; p++) {
+      double x = a[p * 4 + c], y = b[p * 4 + c];
+      maps[0][p] = x; maps[1][p] = y; maps[2][p] = x * x; maps[3][p] = y * y; maps[4][p] = x * y;
+    }
+    for (auto& m : maps) {
+      blur(m, scratch, 1, 0);
+      blur(scratch, m, 0, 1);
+    }
+    for (size_t p = 0; p < n; p++) {
+      double mx = maps[0][p], my = maps[1][p];
+      double vx = maps[2][p] - mx * mx, vy = maps[3][p] - my * my, cxy = maps[4][p] - mx * my;
+      total += ((2 * mx * my + 1e-4) * (2 * cxy + 9e-4)) / ((mx * mx + my * my + 1e-4) * (vx + vy + 9e-4));
+    }
+  }
+  return total / (3.0 * n);
+}
+
+}  // namespace
+
+struct dio_trainer {
+  dio_ctx* ctx = nullptr;
+  dio_training_settings settings{};
+  uint32_t iteration = 0, sh_degree = 0, adam_steps = 0, count = 0;
+  uint32_t width = 0, height = 0;
+  dio_renderer* renderer = nullptr;
+  dio_training_loss* loss = nullptr;
+  dio_buf *params = nullptr, *moments = nullptr, *stats = nullptr;
+  struct View {
+    dio_camera camera;
+    dio_buf* image;
+  };
+  std::vector<View> views;
+  std::vector<uint32_t> queue;
+  Mulberry32 random{1};
+  std::vector<dio_densification> densifications;
+  bool due_densify = false, due_reset = false;
+  dio::AdamArgs last_adam{};
+  uint32_t last_view = 0;
+};
+
+namespace {
+
+void destroy_trainer(dio_trainer* t) {
+  if (!t) return;
+  (void)hipStreamSynchronize(t->ctx->stream);
+  dio_renderer_destroy(t->renderer);
+  destroy_loss(t->loss);
+  release(t->params); release(t->moments); release(t->stats);
+  for (auto& v : t->views) release(v.image);
+  delete t;
+}
+
+const dio_trainer::View& next_view(dio_trainer* t) {
+  if (t->queue.empty()) {
+    for (uint32_t i = 0; i < t->views.size(); i++) t->queue.push_back(i);
+    for (size_t i = t->queue.size() - 1; i > 0; i--) {
+      size_t j = static_cast<size_t>(std::floor(t->random.next() * static_cast<double>(i + 1)));
+      std::swap(t->queue[i], t->queue[j]);
+    }
+  }
+  t->last_view = t->queue.back();
+  t->queue.pop_back();
+  return t->views[t->last_view];
+}
+
+void trainer_optimize(dio_trainer* t) {
+  const uint32_t step = ++t->iteration;
+  const auto& st = t->settings;
+  if (st.sh_degree_interval && step % st.sh_degree_interval == 0 && t->sh_degree < 3) {
+    t->sh_degree++;
+    if (t->renderer->scene) t->renderer->scene->degree = t->sh_degree;
+  }
+  const bool densifying = step < st.densify_until;
+  const bool densify_now = densifying && step > st.densify_from && st.densify_interval &&
+                           step % st.densify_interval == 0;
+  const auto& view = next_view(t);
+  renderer_preprocess(t->renderer, view.camera);
+  renderer_tiles(t->renderer, nullptr, st.background);
+  dio::training_loss(t->ctx->stream, t->width, t->height, dev<float>(t->renderer->scene->rasterizer->pixel_color),
+                     dev<uint32_t>(view.image), dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
+                     dev<float>(t->loss->pixel_loss));
+  renderer_backward(t->renderer, t->loss->pixel_grads);
+  if (!densify_now) t->adam_steps++;
+  const double progress = std::min(1.0, static_cast<double>(step) / st.position_lr_steps);
+  const double position_lr =
+      std::exp(std::log(static_cast<double>(st.position_lr[0])) * (1 - progress) +
+               std::log(static_cast<double>(st.position_lr[1])) * progress) * st.extent;
+  dio::AdamArgs a;
+  a.bias_correction1 = static_cast<float>(1 - std::pow(0.9, t->adam_steps));
+  a.bias_correction2 = static_cast<float>(1 - std::pow(0.999, t->adam_steps));
+  a.position_lr = static_cast<float>(position_lr);
+  a.opacity_lr = st.opacity_lr;
+  a.scale_lr = st.scale_lr;
+  a.rotation_lr = st.rotation_lr;
+  a.color_lr = st.color_lr;
+  a.sh_lr = st.sh_lr;
+  a.viewport[0] = view.camera.viewport[0];
+  a.viewport[1] = view.camera.viewport[1];
+  t->last_adam = a;
+  auto* sc = t->renderer->scene;
+  dio::accumulate_stats(t->ctx->stream, t->count, dev<float>(sc->splats), dev<float>(sc->grad_splats), a,
+                        dev<float>(t->stats));
+  if (!densify_now)
+    dio::adam_step(t->ctx->stream, t->count, dev<float>(t->params), dev<float>(sc->sh), dev<float>(t->moments),
+                   dev<float>(sc->grad_gaussians), dev<float>(sc->grad_sh), a, dev<float>(sc->gaussians));
+  tiles_after_submit(sc->rasterizer, false);
+  t->due_densify = densify_now;
+  t->due_reset = densifying && st.opacity_reset_interval && step % st.opacity_reset_interval == 0;
+}
+
+void trainer_reset_opacity(dio_trainer* t) {
+  dio::reset_opacity(t->ctx->stream, t->count, dev<float>(t->params), dev<float>(t->moments),
+                     dev<float>(t->renderer->scene->gaussians));
+}
+
+void trainer_densify(dio_trainer* t, bool prune_large) {
+  const uint32_t n = t->count;
+  const auto& st = t->settings;
+  dio::DensifyArgs a;
+  a.grad_threshold = st.densify_grad_threshold;
+  a.clone_max_size = st.percent_dense * st.extent;
+  a.min_opacity = st.min_opacity;
+  a.max_size = prune_large ? 0.1f * st.extent : 0.f;
+  a.seed = t->iteration;
+  hipStream_t s = t->ctx->stream;
+  dio_buf* outputs = alloc(t->ctx, static_cast<size_t>(n) * 4);
+  dio_buf* scratch = nullptr;
+  dio_buf *params = nullptr, *sh = nullptr, *moments = nullptr, *stats = nullptr, *gaussians = nullptr;
+  try {
+    scratch = alloc(t->ctx, 64 + dio::scan_workspace_bytes(n), true);
+    uint32_t* counters = dev<uint32_t>(scratch);  // [0..3] tally, [4] total
+    dio::densify_count(s, n, dev<float>(t->params), dev<float>(t->stats), a, dev<uint32_t>(outputs), counters);
+    dio::scan_exclusive(s, dev<uint32_t>(outputs), dev<uint32_t>(outputs), nullptr, n, counters + 4,
+                        dio::scan_workspace(static_cast<char*>(scratch->ptr) + 64, n));
+    uint32_t host[5];
+    DIO_HIP(hipMemcpyAsync(host, counters, sizeof(host), hipMemcpyDeviceToHost, s));
+    sync(t->ctx);
+    const uint32_t new_count = host[4];
+    if (new_count == 0) invalid("Densification removed every Gaussian.");
+    t->densifications.push_back({t->iteration, n, new_count, host[1], host[2], host[3], host[0]});
+    params = alloc(t->ctx, static_cast<size_t>(new_count) * dio::kFloatsPerGaussian * 4, true);
+    sh = alloc(t->ctx, static_cast<size_t>(new_count) * dio::kShRest * 4, true);
+    moments = alloc(t->ctx, static_cast<size_t>(new_count) * dio::kMomentSlots * 8, true);
+    stats = alloc(t->ctx, static_cast<size_t>(new_count) * 16, true);
+    gaussians = alloc(t->ctx, static_cast<size_t>(new_count) * dio::kFloatsPerGaussian * 4, true);
+    dio::densify_scatter(s, n, dev<float>(t->params), dev<float>(t->stats), dev<uint32_t>(outputs),
+                         dev<float>(t->renderer->scene->sh), dev<float>(t->moments), a, dev<float>(params),
+                         dev<float>(sh), dev<float>(moments));
+    dio::activate(s, new_count, dev<float>(params), dev<float>(gaussians));
+  } catch (...) {
+    release(outputs); release(scratch); release(params); release(sh); release(moments); release(stats);
+    release(gaussians);
+    throw;
+  }
+  sync(t->ctx);
+  release(outputs);
+  release(scratch);
+  const uint32_t new_count = t->densifications.back().after;
+  dio_renderer* r = t->renderer;
+  const uint32_t capacity = r->scene->rasterizer->capacity;
+  destroy_scene(r);
+  create_scene(r, new_count, gaussians, sh, t->sh_degree, capacity);
+  release(t->params); release(t->moments); release(t->stats);
+  t->params = params;
+  t->moments = moments;
+  t->stats = stats;
+  t->count = new_count;
+}
+
+void trainer_maintain(dio_trainer* t) {
+  const bool densify = t->due_densify, reset = t->due_reset;
+  t->due_densify = t->due_reset = false;
+  if (densify) trainer_densify(t, t->iteration > t->settings.opacity_reset_interval);
+  if (reset) trainer_reset_opacity(t);
+}
+
+}  // namespace
+
+extern "C" void dio_default_settings(float extent, dio_training_settings* out) {
+  if (!out) return;
+  dio_training_settings s{};
+  s.extent = extent;
+  s.position_lr_steps = 30000;
+  s.position_lr[0] = 0.00016f;
+  s.position_lr[1] = 0.0000016f;
+  s.opacity_lr = 0.025f;
+  s.scale_lr = 0.005f;
+  s.rotation_lr = 0.001f;
+  s.color_lr = 0.0025f;
+  s.sh_lr = 0.0025f / 20.f;
+  s.sh_degree_interval = 1000;
+  s.densify_from = 500;
+  s.densify_until = 15000;
+  s.densify_interval = 100;
+  s.densify_grad_threshold = 0.0002f;
+  s.percent_dense = 0.01f;
+  s.min_opacity = 0.005f;
+  s.opacity_reset_interval = 3000;
+  s.background[0] = s.background[1] = s.background[2] = 0.f;
+  *out = s;
+}
+
+extern "C" int dio_trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t count,
+                                  const dio_camera* cameras, const uint32_t* const* images, uint32_t view_count,
+                                  const dio_training_settings* settings, dio_trainer** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(params, "params");
+    need(sh, "sh");
+    need(cameras, "cameras");
+    need(images, "images");
+    need(settings, "settings");
+    need(out, "out");
+    if (view_count == 0) invalid("Training needs at least one view.");
+    if (count == 0) invalid("Training needs at least one Gaussian.");
+    const float w = cameras[0].viewport[0], h = cameras[0].viewport[1];
+    for (uint32_t v = 0; v < view_count; v++)
+      if (cameras[v].viewport[0] != w || cameras[v].viewport[1] != h)
+        invalid("All training views must be the same size.");
+    auto* t = new dio_trainer;
+    try {
+      t->ctx = ctx;
+      t->settings = *settings;
+      t->count = count;
+      t->width = static_cast<uint32_t>(w);
+      t->height = static_cast<uint32_t>(h);
+      const size_t px = static_cast<size_t>(t->width) * t->height;
+      for (uint32_t v = 0; v < view_count; v++) {
+        need(images[v], "image");
+        dio_buf* img = alloc(ctx, px * 4);
+        t->views.push_back({cameras[v], img});
+        DIO_HIP(hipMemcpyAsync(img->ptr, images[v], px * 4, hipMemcpyHostToDevice, ctx->stream));
+      }
+      t->params = alloc(ctx, static_cast<size_t>(count) * dio::kFloatsPerGaussian * 4);
+      DIO_HIP(hipMemcpyAsync(t->params->ptr, params, t->params->bytes, hipMemcpyHostToDevice, ctx->stream));
+      t->moments = alloc(ctx, static_cast<size_t>(count) * dio::kMomentSlots * 8, true);
+      t->stats = alloc(ctx, static_cast<size_t>(count) * 16, true);
+      t->renderer = new dio_renderer;
+      t->renderer->ctx = ctx;
+      // The renderer gets the activated copy; from here on Adam keeps it in step.
+      dio_buf* g = alloc(ctx, static_cast<size_t>(count) * dio::kFloatsPerGaussian * 4, true);
+      dio_buf* s = nullptr;
+      try {
+        s = alloc(ctx, static_cast<size_t>(count) * dio::kShRest * 4);
+        DIO_HIP(hipMemcpyAsync(s->ptr, sh, s->bytes, hipMemcpyHostToDevice, ctx->stream));
+        dio::activate(ctx->stream, count, dev<float>(t->params), dev<float>(g));
+      } catch (...) {
+        release(g);
+        release(s);
+        throw;
+      }
+      create_scene(t->renderer, count, g, s, 0, 0);
+      dio_training_loss* l = nullptr;
+      if (dio_training_loss_create(ctx, t->width, t->height, &l) != DIO_OK) throw Error(DIO_ERR_MEMORY, g_error);
+      t->loss = l;
+      sync(ctx);
+    } catch (...) {
+      destroy_trainer(t);
+      throw;
+    }
+    *out = t;
+  });
+}
+
+extern "C" int dio_trainer_step(dio_trainer* t) {
+  return guard([&] {
+    need(t, "trainer");
+    trainer_optimize(t);
+    trainer_maintain(t);
+  });
+}
+
+extern "C" int dio_trainer_optimize(dio_trainer* t) {
+  return guard([&] {
+    need(t, "trainer");
+    trainer_optimize(t);
+  });
+}
+
+extern "C" int dio_trainer_maintain(dio_trainer* t) {
+  return guard([&] {
+    need(t, "trainer");
+    trainer_maintain(t);
+  });
+}
+
+extern "C" int dio_trainer_densify(dio_trainer* t, int prune_large) {
+  return guard([&] {
+    need(t, "trainer");
+    trainer_densify(t, prune_large != 0);
+  });
+}
+
+extern "C" int dio_trainer_reset_opacity(dio_trainer* t) {
+  return guard([&] {
+    need(t, "trainer");
+    trainer_reset_opacity(t);
+  });
+}
+
+extern "C" int dio_trainer_export_raw(dio_trainer* t, float* params, float* sh) {
+  return guard([&] {
+    need(t, "trainer");
+    need(params, "params");
+    need(sh, "sh");
+    DIO_HIP(hipMemcpyAsync(params, t->params->ptr, t->params->bytes, hipMemcpyDeviceToHost, t->ctx->stream));
+    DIO_HIP(hipMemcpyAsync(sh, t->renderer->scene->sh->ptr, static_cast<size_t>(t->count) * dio::kShRest * 4,
+                           hipMemcpyDeviceToHost, t->ctx->stream));
+    sync(t->ctx);
+  });
+}
+
+extern "C" int dio_trainer_last_loss(dio_trainer* t, double* out) {
+  return guard([&] {
+    need(t, "trainer");
+    need(out, "out");
+    *out = loss_read(t->loss);
+  });
+}
+
+extern "C" int dio_trainer_evaluate(dio_trainer* t, const dio_camera* cameras, const uint32_t* const* images,
+                                    uint32_t view_count, int with_ssim, double* psnr, double* ssim) {
+  return guard([&] {
+    need(t, "trainer");
+    need(cameras, "cameras");
+    need(images, "images");
+    double psnr_total = 0, ssim_total = 0;
+    for (uint32_t v = 0; v < view_count; v++) {
+      const uint32_t w = static_cast<uint32_t>(cameras[v].viewport[0]), h = static_cast<uint32_t>(cameras[v].viewport[1]);
+      renderer_preprocess(t->renderer, cameras[v]);
+      renderer_tiles(t->renderer, nullptr, t->settings.background);
+      const size_t n = static_cast<size_t>(w) * h;
+      std::vector<float> pixels(n * 4);
+      DIO_HIP(hipMemcpyAsync(pixels.data(), t->renderer->scene->rasterizer->pixel_color->ptr, n * 16,
+                             hipMemcpyDeviceToHost, t->ctx->stream));
+      sync(t->ctx);
+      std::vector<double> rendered(n * 4), photo(n * 4);
+      for (size_t i = 0; i < n * 4; i++) {
+        double x = std::min(1.0, std::max(0.0, static_cast<double>(pixels[i])));
+        rendered[i] = std::round(x * 255) / 255;
+        photo[i] = ((images[v][i / 4] >> (8 * (i % 4))) & 0xffu) / 255.0;
+      }
+      psnr_total += psnr4(rendered, photo);
+      if (with_ssim) ssim_total += ssim4(rendered, photo, static_cast<int>(w), static_cast<int>(h));
+    }
+    if (psnr) *psnr = psnr_total / view_count;
+    if (ssim) *ssim = with_ssim ? ssim_total / view_count : NAN;
+  });
+}
+
+extern "C" uint32_t dio_trainer_count(const dio_trainer* t) { return t ? t->count : 0; }
+extern "C" uint32_t dio_trainer_iteration(const dio_trainer* t) { return t ? t->iteration : 0; }
+extern "C" void dio_trainer_set_iteration(dio_trainer* t, uint32_t iteration) {
+  if (t) t->iteration = iteration;
+}
+extern "C" uint32_t dio_trainer_sh_degree(const dio_trainer* t) { return t ? t->sh_degree : 0; }
+extern "C" int dio_trainer_set_sh_degree(dio_trainer* t, uint32_t degree) {
+  return guard([&] {
+    need(t, "trainer");
+    if (degree > 3) invalid("the SH degree is at most 3");
+    t->sh_degree = degree;
+    if (t->renderer->scene) t->renderer->scene->degree = degree;
+  });
+}
+extern "C" uint32_t dio_trainer_densification_count(const dio_trainer* t) {
+  return t ? static_cast<uint32_t>(t->densifications.size()) : 0;
+}
+extern "C" int dio_trainer_densification(const dio_trainer* t, uint32_t index, dio_densification* out) {
+  return guard([&] {
+    need(t, "trainer");
+    need(out, "out");
+    if (index >= t->densifications.size()) invalid("no such densification");
+    *out = t->densifications[index];
+  });
+}
+extern "C" dio_buf* dio_trainer_params(dio_trainer* t) { return t ? t->params : nullptr; }
+extern "C" dio_buf* dio_trainer_moments(dio_trainer* t) { return t ? t->moments : nullptr; }
+extern "C" dio_buf* dio_trainer_stats(dio_trainer* t) { return t ? t->stats : nullptr; }
+extern "C" dio_renderer* dio_trainer_renderer(dio_trainer* t) { return t ? t->renderer : nullptr; }
+extern "C" void dio_trainer_destroy(dio_trainer* t) { destroy_trainer(t); }
+
+// =============================================================== measurement
+
+extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_t repeats, double* ms) {
+  return guard([&] {
+    need(t, "trainer");
+    need(stage, "stage");
+    need(ms, "ms");
+    if (repeats == 0) invalid("repeats must be positive");
+    auto* r = t->renderer;
+    auto* sc = r->scene;
+    auto* tr = sc->rasterizer;
+    if (!tr->framed || !sc->grad_splats) invalid("run an optimize() step before timing a stage");
+    const std::string name(stage);
+    hipStream_t s = t->ctx->stream;
+    const auto& view = t->views[t->last_view];
+    dio_buf* sorted_splats = tr->sorted_in_alt ? tr->alt_splats : tr->pair_splats;
+    std::function<void()> run;
+    if (name == "preprocess") {
+      run = [&] {
+        dio::preprocess(s, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
+                        dev<float>(sc->splats), dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_keys),
+                        dev<uint32_t>(tr->depth_values));
+      };
+    } else if (name == "scan") {
+      run = [&] {
+        dio::scan_exclusive_gather(s, dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_values),
+                                   dev<uint32_t>(tr->offsets), nullptr, tr->count, dev<uint32_t>(tr->pair_total),
+                                   dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
+      };
+    } else if (name == "sort") {
+      // The whole binning sort: Gaussians by depth, then pairs by tile.
+      run = [&] {
+        const uint32_t tiles = dio::ceil_div(tr->frame_w, dio::kTile) * dio::ceil_div(tr->frame_h, dio::kTile);
+        dio::radix_sort(s, false, tr->depth_keys->ptr, dev<uint32_t>(tr->depth_values), tr->depth_alt_keys->ptr,
+                        dev<uint32_t>(tr->depth_alt_values), nullptr, tr->count, 0, 32,
+                        dio::sort_workspace(tr->splat_sort_ws->ptr, tr->count));
+        dio::radix_sort(s, false, sorted_splats == tr->alt_splats ? tr->alt_tiles->ptr : tr->pair_tiles->ptr,
+                        dev<uint32_t>(sorted_splats),
+                        sorted_splats == tr->alt_splats ? tr->pair_tiles->ptr : tr->alt_tiles->ptr,
+                        dev<uint32_t>(sorted_splats == tr->alt_splats ? tr->pair_splats : tr->alt_splats),
+                        dev<uint32_t>(tr->pair_total), tr->capacity, 0, tile_sort_bits(tiles),
+                        dio::sort_workspace(tr->pair_sort_ws->ptr, tr->capacity));
+      };
+    } else if (name == "rasterize") {
+      run = [&] {
+        dio::rasterize(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges), tr->frame_w,
+                       tr->frame_h, tr->frame_bg, nullptr, dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last));
+      };
+    } else if (name == "rasterize_backward") {
+      run = [&] {
+        dio::rasterize_backward(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges),
+                                tr->frame_w, tr->frame_h, tr->frame_bg, dev<float>(tr->pixel_color),
+                                dev<uint32_t>(tr->pixel_last), dev<float>(t->loss->pixel_grads),
+                                dev<float>(sc->grad_splats));
+      };
+    } else if (name == "preprocess_backward") {
+      run = [&] {
+        dio::preprocess_backward(s, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
+                                 dev<float>(sc->splats), dev<float>(sc->grad_splats), dev<float>(sc->grad_gaussians),
+                                 dev<float>(sc->grad_sh));
+      };
+    } else if (name == "loss") {
+      run = [&] {
+        dio::training_loss(s, t->width, t->height, dev<float>(tr->pixel_color), dev<uint32_t>(view.image),
+                           dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
+                           dev<float>(t->loss->pixel_loss));
+      };
+    } else if (name == "adam") {
+      run = [&] {
+        dio::adam_step(s, t->count, dev<float>(t->params), dev<float>(sc->sh), dev<float>(t->moments),
+                       dev<float>(sc->grad_gaussians), dev<float>(sc->grad_sh), t->last_adam, dev<float>(sc->gaussians));
+      };
+    } else {
+      invalid("unknown stage " + name);
+    }
+    hipEvent_t start, stop;
+    DIO_HIP(hipEventCreate(&start));
+    DIO_HIP(hipEventCreate(&stop));
+    run();  // warm
+    DIO_HIP(hipEventRecord(start, s));
+    for (uint32_t k = 0; k < repeats; k++) run();
+    DIO_HIP(hipEventRecord(stop, s));
+    DIO_HIP(hipEventSynchronize(stop));
+    float elapsed = 0;
+    DIO_HIP(hipEventElapsedTime(&elapsed, start, stop));
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    *ms = elapsed / repeats;
+  });
+}

@@ -1,0 +1,133 @@
+// Internal declarations shared by the HIP kernels and the C-ABI runtime.
+// Everything here is gfx950 (CDNA4, wave64) code; nothing is portable by design.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace dio {
+
+constexpr int kTile = 16;              // pixels per tile side (common.wgsl:29)
+constexpr int kWave = 64;              // CDNA wavefront width
+constexpr uint32_t kFloatsPerGaussian = 16;
+constexpr uint32_t kFloatsPerSplat = 12;
+constexpr uint32_t kShRest = 45;
+constexpr uint32_t kMomentSlots = kFloatsPerGaussian + kShRest;  // 61 Adam slots
+
+// Thrown inside the runtime and turned into a status code at the C boundary.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check_hip(hipError_t e, const char* what);
+#define DIO_HIP(call) ::dio::check_hip((call), #call)
+// Checks the launch itself (bad configuration) without waiting for the kernel.
+#define DIO_LAUNCHED(name) ::dio::check_hip(hipGetLastError(), name)
+
+__host__ __device__ inline uint32_t ceil_div(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ scan --
+// Exclusive prefix sum of u32 values (decoupled look-back, one pass over HBM).
+// `count` (device, element 0) caps how many of `capacity` values are scanned;
+// null scans all of them. `total` (device) receives the grand total.
+struct ScanWorkspace {
+  uint64_t* tile_state = nullptr;  // one (flag, value) word per tile
+  uint32_t* ticket = nullptr;      // dynamic tile counter
+  uint32_t tiles = 0;
+};
+constexpr uint32_t kScanTile = 4096;  // 256 threads x 16 values
+size_t scan_workspace_bytes(uint32_t capacity);
+ScanWorkspace scan_workspace(void* base, uint32_t capacity);
+void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
+                    uint32_t* total, const ScanWorkspace& ws);
+
+// Exclusive scan of in[idx[j]] for j < count: the counts gathered in another order.
+void scan_exclusive_gather(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t* out,
+                           const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws);
+
+// ------------------------------------------------------------------ sort --
+// Stable LSD radix sort, 8-bit digits, of u32 keys or u64 keys stored as
+// (low word, high word), each carrying a u32 value. Sorts key bits
+// [begin_bit, end_bit); the passes ping-pong between (keys, values) and the
+// alternate buffers. Returns true when the result ended in the alternate buffers.
+constexpr uint32_t kSortTile = 4096;  // 256 threads x 16 keys
+struct SortWorkspace {
+  uint32_t* counts = nullptr;      // 256 x tiles digit counts, digit-major
+  uint32_t* counts_len = nullptr;  // device: 256 x active tiles
+  ScanWorkspace scan;
+  uint32_t tiles = 0;
+};
+size_t sort_workspace_bytes(uint32_t capacity);
+SortWorkspace sort_workspace(void* base, uint32_t capacity);
+bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
+                const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
+                const SortWorkspace& ws);
+
+// ------------------------------------------------------------ renderer --
+struct CameraArgs {
+  float view[16];
+  float proj[16];
+  float viewport[2];
+};
+
+void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
+                const CameraArgs& cam, float* splats, uint32_t* tile_counts, uint32_t* depth_keys,
+                uint32_t* depth_values);
+
+// The tile stage from existing splats: counts (+ depth keys) only.
+void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width, uint32_t height,
+                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values);
+
+void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
+               const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
+               uint32_t* pair_tiles, uint32_t* pair_splats);
+
+void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
+                 uint32_t tiles, uint32_t* ranges);
+
+void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
+               uint32_t width, uint32_t height, const float* background, uint32_t* target, float* pixel_color,
+               uint32_t* pixel_last);
+
+void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
+                        uint32_t width, uint32_t height, const float* background, const float* pixel_color,
+                        const uint32_t* pixel_last, const float* pixel_grads, float* splat_grads);
+
+void preprocess_backward(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
+                         const CameraArgs& cam, const float* splats, const float* splat_grads,
+                         float* gaussian_grads, float* sh_grads);
+
+// ---------------------------------------------------------------- loss --
+void training_loss(hipStream_t s, uint32_t width, uint32_t height, const float* rendered, const uint32_t* target,
+                   float* partials, float* pixel_grads, float* pixel_loss);
+
+// ----------------------------------------------------------- optimizer --
+struct AdamArgs {
+  float bias_correction1, bias_correction2;
+  float position_lr, opacity_lr, scale_lr, rotation_lr, color_lr, sh_lr;
+  float viewport[2];
+};
+void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const float* splat_grads,
+                      const AdamArgs& a, float* stats);
+void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* moments, const float* gaussian_grads,
+               const float* sh_grads, const AdamArgs& a, float* gaussians);
+void activate(hipStream_t s, uint32_t n, const float* params, float* gaussians);
+void reset_opacity(hipStream_t s, uint32_t n, float* params, float* moments, float* gaussians);
+
+struct DensifyArgs {
+  float grad_threshold, clone_max_size, min_opacity, max_size;
+  uint32_t seed;
+};
+void densify_count(hipStream_t s, uint32_t n, const float* params, const float* stats, const DensifyArgs& a,
+                   uint32_t* outputs, uint32_t* counters);
+void densify_scatter(hipStream_t s, uint32_t n, const float* params, const float* stats, const uint32_t* outputs,
+                     const float* sh, const float* moments, const DensifyArgs& a, float* new_params, float* new_sh,
+                     float* new_moments);
+
+void fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count);
+
+}  // namespace dio

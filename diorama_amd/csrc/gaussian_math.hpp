@@ -1,0 +1,147 @@
+// Device math shared by the projection and its backward pass, so the two cannot
+// drift apart: quaternion rotation, the SH basis of degrees 1-3 and its gradient
+// (src/shaders/gaussianMath.wgsl:5-76), and the EWA projection of one Gaussian
+// (src/shaders/preprocess.wgsl:28-68). All FP32.
+#pragma once
+#include "dio_internal.hpp"
+
+namespace dio {
+
+constexpr float kNearCull = 0.2f;
+constexpr float kBlur = 0.3f;
+constexpr float kShC0 = 0.28209479177387814f;
+constexpr float kShC1 = 0.4886025119029199f;
+constexpr float kShC2_0 = 1.0925484305920792f, kShC2_1 = -1.0925484305920792f, kShC2_2 = 0.31539156525252005f,
+                kShC2_3 = -1.0925484305920792f, kShC2_4 = 0.5462742152789498f;
+constexpr float kShC3_0 = -0.5900435899266435f, kShC3_1 = 2.890611442640554f, kShC3_2 = -0.4570457994644658f,
+                kShC3_3 = 0.3731763325901154f, kShC3_4 = -0.4570457994644658f, kShC3_5 = 1.445305721320277f,
+                kShC3_6 = -0.5900435899266435f;
+
+__device__ inline int sh_count(uint32_t degree) { return static_cast<int>((degree + 1) * (degree + 1) - 1); }
+
+// r[row][col] of the normalised (w, x, y, z) quaternion.
+__device__ inline void rotation_matrix(float qw, float qx, float qy, float qz, float r[3][3], float q[4],
+                                       float* length) {
+  float l = sqrtf(qw * qw + qx * qx + qy * qy + qz * qz);
+  float w = qw / l, x = qx / l, y = qy / l, z = qz / l;
+  r[0][0] = 1.f - 2.f * (y * y + z * z); r[0][1] = 2.f * (x * y - w * z); r[0][2] = 2.f * (x * z + w * y);
+  r[1][0] = 2.f * (x * y + w * z); r[1][1] = 1.f - 2.f * (x * x + z * z); r[1][2] = 2.f * (y * z - w * x);
+  r[2][0] = 2.f * (x * z - w * y); r[2][1] = 2.f * (y * z + w * x); r[2][2] = 1.f - 2.f * (x * x + y * y);
+  q[0] = w; q[1] = x; q[2] = y; q[3] = z;
+  *length = l;
+}
+
+__device__ inline void sh_basis(float x, float y, float z, float b[15]) {
+  float xx = x * x, yy = y * y, zz = z * z;
+  b[0] = -kShC1 * y; b[1] = kShC1 * z; b[2] = -kShC1 * x;
+  b[3] = kShC2_0 * x * y; b[4] = kShC2_1 * y * z; b[5] = kShC2_2 * (2.f * zz - xx - yy);
+  b[6] = kShC2_3 * x * z; b[7] = kShC2_4 * (xx - yy);
+  b[8] = kShC3_0 * y * (3.f * xx - yy); b[9] = kShC3_1 * x * y * z;
+  b[10] = kShC3_2 * y * (4.f * zz - xx - yy); b[11] = kShC3_3 * z * (2.f * zz - 3.f * xx - 3.f * yy);
+  b[12] = kShC3_4 * x * (4.f * zz - xx - yy); b[13] = kShC3_5 * z * (xx - yy);
+  b[14] = kShC3_6 * x * (xx - 3.f * yy);
+}
+
+// d(basis k)/d(x, y, z) as three arrays.
+__device__ inline void sh_basis_gradient(float x, float y, float z, float gx[15], float gy[15], float gz[15]) {
+  float xx = x * x, yy = y * y, zz = z * z;
+  gx[0] = 0; gy[0] = -kShC1; gz[0] = 0;
+  gx[1] = 0; gy[1] = 0; gz[1] = kShC1;
+  gx[2] = -kShC1; gy[2] = 0; gz[2] = 0;
+  gx[3] = kShC2_0 * y; gy[3] = kShC2_0 * x; gz[3] = 0;
+  gx[4] = 0; gy[4] = kShC2_1 * z; gz[4] = kShC2_1 * y;
+  gx[5] = kShC2_2 * -2.f * x; gy[5] = kShC2_2 * -2.f * y; gz[5] = kShC2_2 * 4.f * z;
+  gx[6] = kShC2_3 * z; gy[6] = 0; gz[6] = kShC2_3 * x;
+  gx[7] = kShC2_4 * 2.f * x; gy[7] = kShC2_4 * -2.f * y; gz[7] = 0;
+  gx[8] = kShC3_0 * 6.f * x * y; gy[8] = kShC3_0 * (3.f * xx - 3.f * yy); gz[8] = 0;
+  gx[9] = kShC3_1 * y * z; gy[9] = kShC3_1 * x * z; gz[9] = kShC3_1 * x * y;
+  gx[10] = kShC3_2 * -2.f * x * y; gy[10] = kShC3_2 * (4.f * zz - xx - 3.f * yy); gz[10] = kShC3_2 * 8.f * y * z;
+  gx[11] = kShC3_3 * -6.f * x * z; gy[11] = kShC3_3 * -6.f * y * z; gz[11] = kShC3_3 * (6.f * zz - 3.f * xx - 3.f * yy);
+  gx[12] = kShC3_4 * (4.f * zz - 3.f * xx - yy); gy[12] = kShC3_4 * -2.f * x * y; gz[12] = kShC3_4 * 8.f * x * z;
+  gx[13] = kShC3_5 * 2.f * x * z; gy[13] = kShC3_5 * -2.f * y * z; gz[13] = kShC3_5 * (xx - yy);
+  gx[14] = kShC3_6 * (3.f * xx - 3.f * yy); gy[14] = kShC3_6 * -6.f * x * y; gz[14] = 0;
+}
+
+// The projection of one Gaussian and every intermediate its backward pass needs.
+struct Projection {
+  float t[3], depth;
+  float r[3][3], q[4], q_length;
+  float m[3][3], cov3d[3][3];
+  float focal[2], limit[2], ratio[2], tx, ty;
+  float jw[2][3];
+  float a, b, c, det;
+  float clip[4];
+  float mean[2];
+  float radius;
+};
+
+// preprocess.wgsl:28-87. Returns false when culled (near plane, degenerate
+// covariance or entirely off screen). Every field is filled in either way, so the
+// backward pass can recompute a visible splat's intermediates unconditionally.
+__device__ inline bool project(const float* g, const CameraArgs& cam, Projection& p) {
+  const float* v = cam.view;
+  for (int r = 0; r < 3; r++) p.t[r] = v[r] * g[0] + v[4 + r] * g[1] + v[8 + r] * g[2] + v[12 + r];
+  p.depth = -p.t[2];
+  rotation_matrix(g[8], g[9], g[10], g[11], p.r, p.q, &p.q_length);
+  for (int r = 0; r < 3; r++)
+    for (int k = 0; k < 3; k++) p.m[r][k] = p.r[r][k] * g[4 + k];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) p.cov3d[r][c] = p.m[r][0] * p.m[c][0] + p.m[r][1] * p.m[c][1] + p.m[r][2] * p.m[c][2];
+  p.focal[0] = cam.proj[0] * cam.viewport[0] * 0.5f;
+  p.focal[1] = cam.proj[5] * cam.viewport[1] * 0.5f;
+  const float d = p.depth;
+  for (int k = 0; k < 2; k++) {
+    p.limit[k] = 1.3f * cam.viewport[k] * 0.5f / p.focal[k];
+    p.ratio[k] = p.t[k] / d;
+  }
+  p.tx = fminf(fmaxf(p.ratio[0], -p.limit[0]), p.limit[0]) * d;
+  p.ty = fminf(fmaxf(p.ratio[1], -p.limit[1]), p.limit[1]) * d;
+  const float j00 = p.focal[0] / d, j02 = p.focal[0] * p.tx / (d * d);
+  const float j11 = -p.focal[1] / d, j12 = -p.focal[1] * p.ty / (d * d);
+  // W[r][c] = view[c * 4 + r]; JW = J W.
+  for (int c = 0; c < 3; c++) {
+    p.jw[0][c] = j00 * v[c * 4 + 0] + j02 * v[c * 4 + 2];
+    p.jw[1][c] = j11 * v[c * 4 + 1] + j12 * v[c * 4 + 2];
+  }
+  float tmp[2][3];
+  for (int r = 0; r < 2; r++)
+    for (int c = 0; c < 3; c++)
+      tmp[r][c] = p.jw[r][0] * p.cov3d[0][c] + p.jw[r][1] * p.cov3d[1][c] + p.jw[r][2] * p.cov3d[2][c];
+  const float c00 = tmp[0][0] * p.jw[0][0] + tmp[0][1] * p.jw[0][1] + tmp[0][2] * p.jw[0][2];
+  const float c01 = tmp[0][0] * p.jw[1][0] + tmp[0][1] * p.jw[1][1] + tmp[0][2] * p.jw[1][2];
+  const float c11 = tmp[1][0] * p.jw[1][0] + tmp[1][1] * p.jw[1][1] + tmp[1][2] * p.jw[1][2];
+  p.a = c00 + kBlur;
+  p.b = c01;
+  p.c = c11 + kBlur;
+  p.det = p.a * p.c - p.b * p.b;
+  const float mid = 0.5f * (p.a + p.c);
+  const float lambda = mid + sqrtf(fmaxf(0.1f, mid * mid - p.det));
+  p.radius = ceilf(3.f * sqrtf(lambda));
+  const float* P = cam.proj;
+  for (int r = 0; r < 4; r++) p.clip[r] = P[r] * p.t[0] + P[4 + r] * p.t[1] + P[8 + r] * p.t[2] + P[12 + r];
+  p.mean[0] = (p.clip[0] / p.clip[3] * 0.5f + 0.5f) * cam.viewport[0];
+  p.mean[1] = (p.clip[1] / p.clip[3] * -0.5f + 0.5f) * cam.viewport[1];
+  if (p.depth <= kNearCull || p.det <= 0.f) return false;
+  if (p.mean[0] + p.radius < 0.f || p.mean[1] + p.radius < 0.f || p.mean[0] - p.radius > cam.viewport[0] ||
+      p.mean[1] - p.radius > cam.viewport[1])
+    return false;
+  return true;
+}
+
+__device__ inline void camera_position(const CameraArgs& cam, float cp[3]) {
+  const float* v = cam.view;
+  for (int c = 0; c < 3; c++) cp[c] = -(v[c * 4] * v[12] + v[c * 4 + 1] * v[13] + v[c * 4 + 2] * v[14]);
+}
+
+// The tiles a splat's square touches: [x0, x1) x [y0, y1) (common.wgsl:37-42).
+__device__ inline void tile_rect(float mx, float my, float radius, uint32_t tiles_x, uint32_t tiles_y,
+                                 uint32_t rect[4]) {
+  const float size = static_cast<float>(kTile);
+  const float tx = static_cast<float>(tiles_x), ty = static_cast<float>(tiles_y);
+  rect[0] = static_cast<uint32_t>(fminf(fmaxf(floorf((mx - radius) / size), 0.f), tx));
+  rect[1] = static_cast<uint32_t>(fminf(fmaxf(floorf((my - radius) / size), 0.f), ty));
+  rect[2] = static_cast<uint32_t>(fminf(fmaxf(floorf((mx + radius) / size) + 1.f, 0.f), tx));
+  rect[3] = static_cast<uint32_t>(fminf(fmaxf(floorf((my + radius) / size) + 1.f, 0.f), ty));
+}
+
+}  // namespace dio

@@ -1,0 +1,258 @@
+// The optimizer stage: densification statistics, Adam with activation, opacity
+// reset, and clone/split/prune by stream compaction (src/shaders/adam.wgsl:51-134,
+// src/shaders/densify.wgsl:64-180).
+//
+// Adam runs one thread per (Gaussian, moment slot) rather than one thread per
+// Gaussian: the 61 float2 moments of a Gaussian are contiguous, so a flat index
+// over slots reads and writes them fully coalesced, which is what bounds this
+// kernel (1,708 bytes per Gaussian per step).
+#include "gaussian_math.hpp"
+
+namespace dio {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr float kBeta1 = 0.9f, kBeta2 = 0.999f, kEpsilon = 1e-15f;
+constexpr float kResetLogit = -4.59511985013459f;  // logit(0.01)
+
+__device__ inline float sigmoid(float x) { return 1.f / (1.f + __expf(-x)); }
+
+__device__ inline float activated(uint32_t slot, float raw) {
+  if (slot == 3) return sigmoid(raw);
+  if (slot >= 4 && slot < 7) return __expf(raw);
+  if (slot >= 12 && slot < 15) return 0.5f + kShC0 * raw;
+  return raw;
+}
+
+__global__ __launch_bounds__(kThreads) void adam_kernel(uint32_t n, float* __restrict__ params, float* __restrict__ sh,
+                                                         float2* __restrict__ moments,
+                                                         const float* __restrict__ gaussian_grads,
+                                                         const float* __restrict__ sh_grads, AdamArgs a,
+                                                         float* __restrict__ gaussians) {
+  const size_t f = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (f >= static_cast<size_t>(n) * kMomentSlots) return;
+  const uint32_t i = static_cast<uint32_t>(f / kMomentSlots);
+  const uint32_t slot = static_cast<uint32_t>(f - static_cast<size_t>(i) * kMomentSlots);
+  if (slot == 7 || slot == 15) return;  // padding: kind and birth stamps, never trained
+  float* value;
+  float grad, lr;
+  if (slot < kFloatsPerGaussian) {
+    value = params + static_cast<size_t>(i) * kFloatsPerGaussian + slot;
+    grad = gaussian_grads[static_cast<size_t>(i) * kFloatsPerGaussian + slot];
+    if (slot < 3) {
+      lr = a.position_lr;
+    } else if (slot == 3) {
+      const float o = sigmoid(*value);
+      grad *= o * (1.f - o);
+      lr = a.opacity_lr;
+    } else if (slot < 7) {
+      grad *= __expf(*value);
+      lr = a.scale_lr;
+    } else if (slot < 12) {
+      lr = a.rotation_lr;
+    } else {
+      grad *= kShC0;
+      lr = a.color_lr;
+    }
+  } else {
+    const size_t k = static_cast<size_t>(i) * kShRest + (slot - kFloatsPerGaussian);
+    value = sh + k;
+    grad = sh_grads[k];
+    lr = a.sh_lr;
+  }
+  float2 m = moments[f];
+  m.x = kBeta1 * m.x + (1.f - kBeta1) * grad;
+  m.y = kBeta2 * m.y + (1.f - kBeta2) * grad * grad;
+  moments[f] = m;
+  const float updated = *value - lr * (m.x / a.bias_correction1) / (sqrtf(m.y / a.bias_correction2) + kEpsilon);
+  *value = updated;
+  if (gaussians && slot < kFloatsPerGaussian)
+    gaussians[static_cast<size_t>(i) * kFloatsPerGaussian + slot] = activated(slot, updated);
+}
+
+__global__ __launch_bounds__(kThreads) void activate_kernel(uint32_t n, const float* __restrict__ params,
+                                                             float* __restrict__ gaussians) {
+  const size_t f = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (f >= static_cast<size_t>(n) * kFloatsPerGaussian) return;
+  const uint32_t slot = static_cast<uint32_t>(f % kFloatsPerGaussian);
+  if (slot == 7 || slot == 15) return;
+  gaussians[f] = activated(slot, params[f]);
+}
+
+__global__ __launch_bounds__(kThreads) void stats_kernel(uint32_t n, const float4* __restrict__ splats,
+                                                          const float* __restrict__ splat_grads, float vx, float vy,
+                                                          float4* __restrict__ stats) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const float radius = splats[i * 3].w;
+  if (radius == 0.f) return;
+  const float gx = splat_grads[static_cast<size_t>(i) * 9] * vx * 0.5f;
+  const float gy = splat_grads[static_cast<size_t>(i) * 9 + 1] * vy * 0.5f;
+  const float4 s = stats[i];
+  stats[i] = make_float4(s.x + sqrtf(gx * gx + gy * gy), s.y + 1.f, fmaxf(s.z, radius), 0.f);
+}
+
+__global__ __launch_bounds__(kThreads) void reset_opacity_kernel(uint32_t n, float* __restrict__ params,
+                                                                  float2* __restrict__ moments,
+                                                                  float* __restrict__ gaussians) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  float* p = params + static_cast<size_t>(i) * kFloatsPerGaussian;
+  const float logit = fminf(p[3], kResetLogit);
+  p[3] = logit;
+  moments[static_cast<size_t>(i) * kMomentSlots + 3] = make_float2(0.f, 0.f);
+  if (gaussians) gaussians[static_cast<size_t>(i) * kFloatsPerGaussian + 3] = sigmoid(logit);
+}
+
+enum : uint32_t { kDrop = 0, kKeep = 1, kClone = 2, kSplit = 3 };
+constexpr float kSplitShrink = 1.6f;
+
+__device__ inline bool removed(const DensifyArgs& a, float opacity, float size) {
+  return opacity < a.min_opacity || (a.max_size > 0.f && size > a.max_size);
+}
+
+// densify.wgsl:72-85: the reference clones, then splits, then prunes the result.
+__device__ inline uint32_t action(const float* params, const float4* stats, const DensifyArgs& a, uint32_t i) {
+  const float* p = params + static_cast<size_t>(i) * kFloatsPerGaussian;
+  const float4 s = stats[i];
+  const float grad = s.y > 0.f ? s.x / s.y : 0.f;
+  const float size = __expf(fmaxf(p[4], fmaxf(p[5], p[6])));
+  const float opacity = sigmoid(p[3]);
+  if (grad >= a.grad_threshold) {
+    if (size <= a.clone_max_size) return removed(a, opacity, size) ? kDrop : kClone;
+    return removed(a, opacity, size / kSplitShrink) ? kDrop : kSplit;
+  }
+  return removed(a, opacity, size) ? kDrop : kKeep;
+}
+
+__global__ __launch_bounds__(kThreads) void densify_count_kernel(uint32_t n, const float* __restrict__ params,
+                                                                  const float4* __restrict__ stats, DensifyArgs a,
+                                                                  uint32_t* __restrict__ outputs,
+                                                                  uint32_t* __restrict__ counters) {
+  __shared__ uint32_t tally[4];
+  if (threadIdx.x < 4) tally[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) {
+    const uint32_t act = action(params, stats, a, i);
+    outputs[i] = act == kDrop ? 0u : (act == kKeep ? 1u : 2u);
+    atomicAdd(&tally[act], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 && tally[threadIdx.x]) atomicAdd(&counters[threadIdx.x], tally[threadIdx.x]);
+}
+
+__device__ inline uint32_t pcg(uint32_t v) {
+  const uint32_t state = v * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+__device__ inline float unit(uint32_t h) { return (static_cast<float>(h >> 8) + 0.5f) / 16777216.f; }
+
+__device__ inline void copy_gaussian(uint32_t i, uint32_t o, bool keep_moments, const float* params, const float* sh,
+                                     const float2* moments, float* new_params, float* new_sh, float2* new_moments) {
+  for (uint32_t k = 0; k < kFloatsPerGaussian; k++)
+    new_params[static_cast<size_t>(o) * kFloatsPerGaussian + k] = params[static_cast<size_t>(i) * kFloatsPerGaussian + k];
+  for (uint32_t k = 0; k < kShRest; k++)
+    new_sh[static_cast<size_t>(o) * kShRest + k] = sh[static_cast<size_t>(i) * kShRest + k];
+  for (uint32_t k = 0; k < kMomentSlots; k++)
+    new_moments[static_cast<size_t>(o) * kMomentSlots + k] =
+        keep_moments ? moments[static_cast<size_t>(i) * kMomentSlots + k] : make_float2(0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kThreads) void densify_scatter_kernel(
+    uint32_t n, const float* __restrict__ params, const float4* __restrict__ stats,
+    const uint32_t* __restrict__ outputs, const float* __restrict__ sh, const float2* __restrict__ moments,
+    DensifyArgs a, float* __restrict__ new_params, float* __restrict__ new_sh, float2* __restrict__ new_moments) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t act = action(params, stats, a, i);
+  const uint32_t o = outputs[i];
+  if (act == kKeep || act == kClone) copy_gaussian(i, o, true, params, sh, moments, new_params, new_sh, new_moments);
+  if (act == kClone) {
+    copy_gaussian(i, o + 1, false, params, sh, moments, new_params, new_sh, new_moments);
+    new_params[static_cast<size_t>(o + 1) * kFloatsPerGaussian + 7] = 1.f;
+    new_params[static_cast<size_t>(o + 1) * kFloatsPerGaussian + 15] = static_cast<float>(a.seed);
+  }
+  if (act == kSplit) {
+    const float* p = params + static_cast<size_t>(i) * kFloatsPerGaussian;
+    float r[3][3], q[4], l;
+    rotation_matrix(p[8], p[9], p[10], p[11], r, q, &l);
+    const float sx = __expf(p[4]), sy = __expf(p[5]), sz = __expf(p[6]);
+    const float shrink = logf(kSplitShrink);
+    for (uint32_t c = 0; c < 2; c++) {
+      const uint32_t child = o + c;
+      copy_gaussian(i, child, false, params, sh, moments, new_params, new_sh, new_moments);
+      // normal3(pcg(seed ^ pcg(2i + c))) of densify.wgsl:270-279,305: the seed is hashed once more.
+      const uint32_t h1 = pcg(pcg(a.seed ^ pcg(2u * i + c))), h2 = pcg(h1), h3 = pcg(h2), h4 = pcg(h3);
+      // Box-Muller: three standard normal numbers from four hashes.
+      const float tau = 6.283185307179586f;
+      const float r1 = sqrtf(-2.f * logf(unit(h1))), r2 = sqrtf(-2.f * logf(unit(h3)));
+      const float nx = r1 * cosf(tau * unit(h2)), ny = r1 * sinf(tau * unit(h2)), nz = r2 * cosf(tau * unit(h4));
+      const float vx = sx * nx, vy = sy * ny, vz = sz * nz;
+      float* out = new_params + static_cast<size_t>(child) * kFloatsPerGaussian;
+      for (int k = 0; k < 3; k++) {
+        out[k] = p[k] + r[k][0] * vx + r[k][1] * vy + r[k][2] * vz;
+        out[4 + k] = p[4 + k] - shrink;
+      }
+      out[7] = 2.f;
+      out[15] = static_cast<float>(a.seed);
+    }
+  }
+}
+
+}  // namespace
+
+void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const float* splat_grads, const AdamArgs& a,
+                      float* stats) {
+  if (n == 0) return;
+  stats_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, reinterpret_cast<const float4*>(splats), splat_grads,
+                                                          a.viewport[0], a.viewport[1],
+                                                          reinterpret_cast<float4*>(stats));
+  DIO_LAUNCHED("accumulate_stats");
+}
+
+void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* moments, const float* gaussian_grads,
+               const float* sh_grads, const AdamArgs& a, float* gaussians) {
+  if (n == 0) return;
+  const size_t slots = static_cast<size_t>(n) * kMomentSlots;
+  adam_kernel<<<ceil_div(slots, kThreads), kThreads, 0, s>>>(n, params, sh, reinterpret_cast<float2*>(moments),
+                                                             gaussian_grads, sh_grads, a, gaussians);
+  DIO_LAUNCHED("adam");
+}
+
+void activate(hipStream_t s, uint32_t n, const float* params, float* gaussians) {
+  if (n == 0) return;
+  activate_kernel<<<ceil_div(static_cast<size_t>(n) * kFloatsPerGaussian, kThreads), kThreads, 0, s>>>(n, params,
+                                                                                                        gaussians);
+  DIO_LAUNCHED("activate");
+}
+
+void reset_opacity(hipStream_t s, uint32_t n, float* params, float* moments, float* gaussians) {
+  if (n == 0) return;
+  reset_opacity_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, params, reinterpret_cast<float2*>(moments),
+                                                                   gaussians);
+  DIO_LAUNCHED("reset_opacity");
+}
+
+void densify_count(hipStream_t s, uint32_t n, const float* params, const float* stats, const DensifyArgs& a,
+                   uint32_t* outputs, uint32_t* counters) {
+  fill_u32(s, counters, 0u, 4);
+  if (n == 0) return;
+  densify_count_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, params, reinterpret_cast<const float4*>(stats), a,
+                                                                  outputs, counters);
+  DIO_LAUNCHED("densify_count");
+}
+
+void densify_scatter(hipStream_t s, uint32_t n, const float* params, const float* stats, const uint32_t* outputs,
+                     const float* sh, const float* moments, const DensifyArgs& a, float* new_params, float* new_sh,
+                     float* new_moments) {
+  if (n == 0) return;
+  densify_scatter_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(
+      n, params, reinterpret_cast<const float4*>(stats), outputs, sh, reinterpret_cast<const float2*>(moments), a,
+      new_params, new_sh, reinterpret_cast<float2*>(new_moments));
+  DIO_LAUNCHED("densify_scatter");
+}
+
+}  // namespace dio

@@ -1,0 +1,196 @@
+// Exclusive prefix sum of u32 values in one pass over HBM: each workgroup scans a
+// 4096-value tile in registers (16 values per thread, wave64 shuffles across the
+// threads) and learns the total of every tile before it by decoupled look-back:
+// a tile publishes its own sum ("aggregate") at once and its inclusive prefix
+// once known, and one wavefront reads up to 64 predecessors per step.
+// Tiles are handed out by an atomic ticket, so a tile only ever waits for tiles
+// that started before it. Replaces the Blelloch block scan + recursive levels of
+// src/shaders/prefixSum.wgsl:30-91 / src/prefixSum.ts:59-98 (same results: an
+// exclusive scan of min(count, capacity) values, total in element 0 of `total`).
+#include "dio_internal.hpp"
+
+namespace dio {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = kScanTile / kThreads;  // 16
+constexpr uint64_t kFlagAggregate = 1ull << 32;
+constexpr uint64_t kFlagPrefix = 2ull << 32;
+
+__device__ inline uint32_t wave_inclusive_scan(uint32_t x, int lane) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+__device__ inline uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
+__device__ inline void publish(uint64_t* state, uint32_t tile, uint64_t word) {
+  __hip_atomic_store(state + tile, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint64_t peek(const uint64_t* state, int32_t tile) {
+  return __hip_atomic_load(const_cast<uint64_t*>(state) + tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The exclusive prefix of `tile`: sums of the aggregates back to the first
+// published inclusive prefix, 64 predecessors per step. Wave 0 only.
+__device__ uint32_t look_back(const uint64_t* state, uint32_t tile, int lane) {
+  uint32_t exclusive = 0;
+  int32_t base = static_cast<int32_t>(tile) - 1;
+  for (;;) {
+    int32_t j = base - lane;
+    uint64_t word = j >= 0 ? peek(state, j) : kFlagPrefix;
+    for (;;) {
+      bool waiting = (word >> 32) == 0;
+      if (!__any(waiting)) break;
+      if (waiting) word = peek(state, j);
+    }
+    uint64_t prefixes = __ballot((word >> 32) == 2);
+    int first = prefixes ? __ffsll(static_cast<long long>(prefixes)) - 1 : kWave - 1;
+    uint32_t v = lane <= first ? static_cast<uint32_t>(word) : 0u;
+    exclusive += wave_sum(v);
+    if (prefixes) return exclusive;
+    base -= kWave;
+  }
+}
+
+template <bool kGather>
+__global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in,
+                                                         const uint32_t* __restrict__ idx, uint32_t* out,
+                                                         const uint32_t* count, uint32_t capacity, uint32_t* total,
+                                                         uint64_t* state, uint32_t* ticket) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wave[kThreads / kWave];
+  __shared__ uint32_t s_prefix;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t n = count ? min(*count, capacity) : capacity;
+  const uint32_t tiles = n == 0 ? 1 : ceil_div(n, kScanTile);
+  if (tile >= tiles) return;
+
+  const uint32_t base = tile * kScanTile + tid * kItems;
+  uint32_t v[kItems];
+  if (!kGather && base + kItems <= n) {
+    const uint4* src = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+    for (int q = 0; q < kItems / 4; q++) {
+      uint4 w = src[q];
+      v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+      uint32_t i = base + k;
+      v[k] = i < n ? (kGather ? in[idx[i]] : in[i]) : 0u;
+    }
+  }
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; k++) {
+    uint32_t x = v[k];
+    v[k] = sum;  // exclusive within the thread
+    sum += x;
+  }
+  const uint32_t inclusive = wave_inclusive_scan(sum, lane);
+  if (lane == kWave - 1) s_wave[wave] = inclusive;
+  __syncthreads();
+  uint32_t before = 0, aggregate = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / kWave; w++) {
+    uint32_t t = s_wave[w];
+    before += w < wave ? t : 0u;
+    aggregate += t;
+  }
+  const uint32_t thread_prefix = before + inclusive - sum;
+
+  if (wave == 0) {
+    uint32_t exclusive = 0;
+    if (tile == 0) {
+      if (lane == 0) publish(state, 0, kFlagPrefix | aggregate);
+    } else {
+      if (lane == 0) publish(state, tile, kFlagAggregate | aggregate);
+      exclusive = look_back(state, tile, lane);
+      if (lane == 0) publish(state, tile, kFlagPrefix | (exclusive + aggregate));
+    }
+    if (lane == 0) {
+      s_prefix = exclusive;
+      if (tile == tiles - 1) *total = exclusive + aggregate;
+    }
+  }
+  __syncthreads();
+  const uint32_t add = s_prefix + thread_prefix;
+  if (base + kItems <= n) {
+    uint4* dst = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+    for (int q = 0; q < kItems / 4; q++)
+      dst[q] = make_uint4(v[4 * q] + add, v[4 * q + 1] + add, v[4 * q + 2] + add, v[4 * q + 3] + add);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kItems; k++)
+      if (base + k < n) out[base + k] = v[k] + add;
+  }
+}
+
+__global__ void fill_kernel(uint32_t* p, uint32_t value, size_t count) {
+  size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (; i < count; i += stride) p[i] = value;
+}
+
+}  // namespace
+
+void fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count) {
+  if (count == 0) return;
+  uint32_t blocks = ceil_div(count, 256);
+  if (blocks > 4096) blocks = 4096;
+  fill_kernel<<<blocks, 256, 0, s>>>(p, value, count);
+  DIO_LAUNCHED("fill");
+}
+
+size_t scan_workspace_bytes(uint32_t capacity) {
+  uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kScanTile);
+  return static_cast<size_t>(tiles) * 8 + 256;
+}
+
+ScanWorkspace scan_workspace(void* base, uint32_t capacity) {
+  ScanWorkspace ws;
+  ws.tiles = capacity == 0 ? 1 : ceil_div(capacity, kScanTile);
+  ws.tile_state = static_cast<uint64_t*>(base);
+  ws.ticket = reinterpret_cast<uint32_t*>(static_cast<char*>(base) + static_cast<size_t>(ws.tiles) * 8);
+  return ws;
+}
+
+static void scan_launch(hipStream_t s, bool gather, const uint32_t* in, const uint32_t* idx, uint32_t* out,
+                        const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws) {
+  uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kScanTile);
+  if (tiles > ws.tiles) throw Error(-2, "scan workspace too small");
+  // Tile states and the ticket are one contiguous run of words.
+  fill_u32(s, reinterpret_cast<uint32_t*>(ws.tile_state), 0u, static_cast<size_t>(ws.tiles) * 2 + 1);
+  if (gather)
+    scan_kernel<true><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket);
+  else
+    scan_kernel<false><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket);
+  DIO_LAUNCHED("scan");
+}
+
+void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
+                    uint32_t* total, const ScanWorkspace& ws) {
+  scan_launch(s, false, in, nullptr, out, count, capacity, total, ws);
+}
+
+void scan_exclusive_gather(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t* out,
+                           const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws) {
+  scan_launch(s, true, in, idx, out, count, capacity, total, ws);
+}
+
+}  // namespace dio

@@ -1,0 +1,228 @@
+/*
+ * diorama-mi355x: the C ABI of the MI355X (gfx950) Gaussian-splatting renderer
+ * and trainer. Plain pointers and sizes only; every object is an opaque handle.
+ *
+ * Each group replaces one operator class of the reference (paths relative to the
+ * reference repository) and keeps its method names, argument meaning, buffer
+ * layouts and error behaviour. Work is enqueued on the context's HIP stream in
+ * call order, the analogue of recording into a GPUCommandEncoder and submitting:
+ * `encode` calls return without waiting for the GPU; downloads, reads and
+ * `after_submit` wait.
+ *
+ * Errors: functions return DIO_OK (0) or a negative code, and dio_last_error()
+ * gives the message (thread-local). Contract violations the reference throws for
+ * (src/gaussianRenderer.ts:99-101, src/tileRasterizer.ts:136-138,202,
+ * src/trainer.ts:133-135,295) return DIO_ERR_INVALID with the same message.
+ * Handles are not thread-safe; one host thread drives a context.
+ *
+ * Buffer layouts (src/gaussians.ts:32-70):
+ *   Gaussian  16 floats: position xyz, opacity, scale xyz, pad, quaternion wxyz, rgb, pad
+ *   Splat2D   12 floats: mean xy (pixels, y down), depth, radius (0 = culled),
+ *             conic abc, pad, rgb, opacity
+ *   SH rest   45 floats: coefficients 1..15, rgb each, coefficient-major
+ *   Camera    view[16], proj[16] column-major (src/mat4.ts:1-3), viewport[2] pixels
+ *   images    rgba8 as one u32 per pixel (r in the low byte), row-major
+ *   pixels    colour buffers: 4 floats per pixel (rgb, final transmittance)
+ */
+#ifndef DIORAMA_H
+#define DIORAMA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DIO_OK 0
+#define DIO_ERR_INVALID (-1)   /* contract violation (the reference throws an Error) */
+#define DIO_ERR_INTERNAL (-2)  /* workspace or internal invariant */
+#define DIO_ERR_HIP (-3)       /* a HIP runtime call failed */
+#define DIO_ERR_MEMORY (-4)    /* device allocation failed */
+
+typedef struct dio_ctx dio_ctx;
+typedef struct dio_buf dio_buf;
+typedef struct dio_prefix_sum dio_prefix_sum;
+typedef struct dio_radix_sort dio_radix_sort;
+typedef struct dio_tile_rasterizer dio_tile_rasterizer;
+typedef struct dio_renderer dio_renderer;
+typedef struct dio_training_loss dio_training_loss;
+typedef struct dio_trainer dio_trainer;
+
+/* CameraData (src/gaussianRenderer.ts:11-15). */
+typedef struct dio_camera {
+  float view[16];
+  float proj[16];
+  float viewport[2];
+} dio_camera;
+
+/* ------------------------------------------------------------ runtime -- */
+/* Replaces initGpu (src/gpu.ts:8-47): one device, one in-order stream. */
+const char* dio_last_error(void);
+const char* dio_version(void);
+int dio_ctx_create(int device, dio_ctx** out);
+int dio_ctx_synchronize(dio_ctx* ctx);
+void dio_ctx_destroy(dio_ctx* ctx);
+
+/* Device buffers (GPUBuffer + queue.writeBuffer / readBuffer, src/readback.ts). */
+int dio_buf_create(dio_ctx* ctx, size_t bytes, dio_buf** out);
+int dio_buf_upload(dio_buf* buf, size_t offset, const void* src, size_t bytes);
+int dio_buf_download(const dio_buf* buf, size_t offset, void* dst, size_t bytes);
+int dio_buf_clear(dio_buf* buf); /* encoder.clearBuffer */
+size_t dio_buf_size(const dio_buf* buf);
+void dio_buf_destroy(dio_buf* buf);
+
+/* ------------------------------------------------------------ PrefixSum -- */
+/* src/prefixSum.ts:10-103: exclusive u32 scan of `data` in place. `count`
+ * (nullable) holds in element 0 how many values to scan (at most `capacity`).
+ * After encode, element 0 of total() is the sum of the scanned values. */
+int dio_prefix_sum_create(dio_ctx* ctx, dio_buf* data, uint32_t capacity, dio_buf* count, dio_prefix_sum** out);
+dio_buf* dio_prefix_sum_total(dio_prefix_sum* scan);
+int dio_prefix_sum_encode(dio_prefix_sum* scan);
+void dio_prefix_sum_destroy(dio_prefix_sum* scan);
+
+/* ------------------------------------------------------------ RadixSort -- */
+/* src/radixSort.ts:27-142: stable sort of u32 keys (high_bits = 0) or of 64-bit
+ * keys stored as (low, high) u32 pairs, looking at the low `high_bits` bits of the
+ * high word (at most 32). `count` (nullable) holds how many keys to sort. The
+ * result lands back in keys() and values(). */
+int dio_radix_sort_create(dio_ctx* ctx, uint32_t capacity, uint32_t high_bits, dio_buf* count, dio_radix_sort** out);
+dio_buf* dio_radix_sort_keys(dio_radix_sort* sort);
+dio_buf* dio_radix_sort_values(dio_radix_sort* sort);
+uint32_t dio_radix_sort_capacity(const dio_radix_sort* sort);
+int dio_radix_sort_encode(dio_radix_sort* sort);
+void dio_radix_sort_destroy(dio_radix_sort* sort);
+
+/* ------------------------------------------------------- TileRasterizer -- */
+/* src/tileRasterizer.ts:29-317: bins `count` splats of `splats` into 16x16 tiles
+ * (sorted front to back), blends every pixel, and keeps what the backward pass
+ * needs. initial_pair_capacity 0 means max(65536, 8 * count) (:97). The camera
+ * buffer of the reference is not needed: the viewport is an argument. */
+int dio_tile_rasterizer_create(dio_ctx* ctx, dio_buf* splats, uint32_t count, uint32_t initial_pair_capacity,
+                               dio_tile_rasterizer** out);
+/* encode (:127-192). `target` (nullable) receives rgba8 pixels. */
+int dio_tile_rasterizer_encode(dio_tile_rasterizer* tr, dio_buf* target, uint32_t width, uint32_t height,
+                               const float background[3]);
+/* encodeBackward (:200-222): pixel_grads vec4 per pixel -> splat_grads 9 floats per splat. */
+int dio_tile_rasterizer_encode_backward(dio_tile_rasterizer* tr, uint32_t width, uint32_t height, dio_buf* pixel_grads,
+                                        dio_buf* splat_grads);
+/* afterSubmit (:229-244): reads back the frame's pair count and grows the pair
+ * buffers to 2^ceil(log2(1.25 * count)) when they were too small. Waits. */
+int dio_tile_rasterizer_after_submit(dio_tile_rasterizer* tr);
+uint32_t dio_tile_rasterizer_last_pair_count(const dio_tile_rasterizer* tr);
+uint32_t dio_tile_rasterizer_pair_capacity(const dio_tile_rasterizer* tr);
+/* debugBuffers: each sorted pair's splat, and each tile's [start, end) run. */
+dio_buf* dio_tile_rasterizer_pair_splats(dio_tile_rasterizer* tr);
+dio_buf* dio_tile_rasterizer_ranges(dio_tile_rasterizer* tr);
+dio_buf* dio_tile_rasterizer_pixel_color(dio_tile_rasterizer* tr);
+dio_buf* dio_tile_rasterizer_pixel_last(dio_tile_rasterizer* tr);
+void dio_tile_rasterizer_destroy(dio_tile_rasterizer* tr);
+
+/* ----------------------------------------------------- GaussianRenderer -- */
+/* src/gaussianRenderer.ts:55-311. initial_pair_capacity 0 = the rasterizer's default. */
+int dio_renderer_create(dio_ctx* ctx, uint32_t initial_pair_capacity, dio_renderer** out);
+/* setGaussians (:93-110): copies count x 16 floats; sh_rest (nullable) count x 45. */
+int dio_renderer_set_gaussians(dio_renderer* r, const float* gaussians, uint32_t count, const float* sh_rest,
+                               uint32_t sh_degree);
+/* adoptGaussians (:118-122): takes ownership of both buffers. */
+int dio_renderer_adopt_gaussians(dio_renderer* r, uint32_t count, dio_buf* gaussians, dio_buf* sh, uint32_t sh_degree);
+int dio_renderer_set_sh_degree(dio_renderer* r, uint32_t degree);
+/* encode (:169-192) = encode_preprocess + encode_tiles; the split is the
+ * afterPreprocess hook point (src/insideView.ts:221). */
+int dio_renderer_encode(dio_renderer* r, dio_buf* target, const dio_camera* camera, const float background[3]);
+int dio_renderer_encode_preprocess(dio_renderer* r, const dio_camera* camera);
+int dio_renderer_encode_tiles(dio_renderer* r, dio_buf* target, const float background[3]);
+/* encodeDraw (:198-211): ready-made splats, skipping the preprocess. */
+int dio_renderer_encode_draw(dio_renderer* r, dio_buf* target, dio_buf* splats, uint32_t count, uint32_t width,
+                             uint32_t height, const float background[3]);
+/* encodeBackward (:217-227): results in the gradient buffers below. */
+int dio_renderer_encode_backward(dio_renderer* r, dio_buf* pixel_grads);
+int dio_renderer_after_submit(dio_renderer* r);
+uint32_t dio_renderer_count(const dio_renderer* r);
+dio_buf* dio_renderer_splat_buffer(dio_renderer* r);
+dio_buf* dio_renderer_gaussian_buffer(dio_renderer* r);
+dio_buf* dio_renderer_sh_buffer(dio_renderer* r);
+dio_tile_rasterizer* dio_renderer_tiles(dio_renderer* r);
+dio_buf* dio_renderer_pixel_buffer(dio_renderer* r);
+dio_buf* dio_renderer_pixel_last_buffer(dio_renderer* r);
+/* gradients (:29-36): 16 floats per Gaussian, 45 SH floats, 9 splat floats. */
+dio_buf* dio_renderer_gradient_gaussians(dio_renderer* r);
+dio_buf* dio_renderer_gradient_sh(dio_renderer* r);
+dio_buf* dio_renderer_gradient_splats(dio_renderer* r);
+void dio_renderer_destroy(dio_renderer* r);
+
+/* --------------------------------------------------------- TrainingLoss -- */
+/* src/trainingLoss.ts:16-130: 0.8 * L1 + 0.2 * (1 - SSIM) and dL/dpixel. */
+int dio_training_loss_create(dio_ctx* ctx, uint32_t width, uint32_t height, dio_training_loss** out);
+/* rendered: vec4 per pixel (rgb used); target: rgba8 per pixel. */
+int dio_training_loss_encode(dio_training_loss* loss, dio_buf* rendered, dio_buf* target);
+/* read (:110-125): sum of the per-pixel shares + 0.2. Waits. */
+int dio_training_loss_read(dio_training_loss* loss, double* out);
+dio_buf* dio_training_loss_pixel_grads(dio_training_loss* loss);
+dio_buf* dio_training_loss_pixel_loss(dio_training_loss* loss);
+void dio_training_loss_destroy(dio_training_loss* loss);
+
+/* ---------------------------------------------------------------- Trainer -- */
+/* TrainingSettings / DEFAULT_SETTINGS (src/trainer.ts:37-82). */
+typedef struct dio_training_settings {
+  float extent;
+  uint32_t position_lr_steps;
+  float position_lr[2];
+  float opacity_lr, scale_lr, rotation_lr, color_lr, sh_lr;
+  uint32_t sh_degree_interval;
+  uint32_t densify_from, densify_until, densify_interval;
+  float densify_grad_threshold;
+  float percent_dense;
+  float min_opacity;
+  uint32_t opacity_reset_interval;
+  float background[3];
+} dio_training_settings;
+
+/* Densification record (src/trainer.ts:86-95). */
+typedef struct dio_densification {
+  uint32_t step, before, after, kept, cloned, split, removed;
+} dio_densification;
+
+void dio_default_settings(float extent, dio_training_settings* out);
+/* Trainer constructor (:127-168): raw params (count x 16: position, opacity logit,
+ * log scale, pad, rotation, degree-0 coefficient, pad) and SH (count x 45);
+ * one rgba8 image per view, all views the same size. */
+int dio_trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t count, const dio_camera* cameras,
+                       const uint32_t* const* images, uint32_t view_count, const dio_training_settings* settings,
+                       dio_trainer** out);
+int dio_trainer_step(dio_trainer* t);      /* optimize() then maintain() */
+int dio_trainer_optimize(dio_trainer* t);  /* render, loss, backward, stats, Adam; no wait */
+int dio_trainer_maintain(dio_trainer* t);  /* densify / reset when due; waits when densifying */
+int dio_trainer_densify(dio_trainer* t, int prune_large);
+int dio_trainer_reset_opacity(dio_trainer* t);
+int dio_trainer_export_raw(dio_trainer* t, float* params, float* sh); /* count x 16, count x 45 */
+int dio_trainer_last_loss(dio_trainer* t, double* out);
+/* evaluate (:371-386): mean PSNR/SSIM of renders rounded to 8 bits. */
+int dio_trainer_evaluate(dio_trainer* t, const dio_camera* cameras, const uint32_t* const* images, uint32_t view_count,
+                         int with_ssim, double* psnr, double* ssim);
+uint32_t dio_trainer_count(const dio_trainer* t);
+uint32_t dio_trainer_iteration(const dio_trainer* t);
+void dio_trainer_set_iteration(dio_trainer* t, uint32_t iteration);
+uint32_t dio_trainer_sh_degree(const dio_trainer* t);
+/* shDegree is a public field of the reference Trainer (src/trainer.ts:105). */
+int dio_trainer_set_sh_degree(dio_trainer* t, uint32_t degree);
+uint32_t dio_trainer_densification_count(const dio_trainer* t);
+int dio_trainer_densification(const dio_trainer* t, uint32_t index, dio_densification* out);
+dio_buf* dio_trainer_params(dio_trainer* t);
+dio_buf* dio_trainer_moments(dio_trainer* t);
+dio_buf* dio_trainer_stats(dio_trainer* t);
+dio_renderer* dio_trainer_renderer(dio_trainer* t);
+void dio_trainer_destroy(dio_trainer* t);
+
+/* --------------------------------------------------------- measurement -- */
+/* Times `repeats` launches of one named stage between HIP events recorded on the
+ * context's stream, for the benchmark's roofline figures. Stages: "preprocess",
+ * "scan", "sort", "rasterize", "rasterize_backward", "preprocess_backward",
+ * "loss", "adam". Returns the mean milliseconds per launch of the last
+ * optimize() step's inputs re-run `repeats` times. */
+int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_t repeats, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -269,25 +269,36 @@ static double splat_power(const double* s, double dx, double dy) {
   return -0.5 * (s[4] * dx * dx + s[6] * dy * dy) - s[5] * dx * dy;
 }
 
-void oc_rasterize(const double* splats, const uint32_t* ranges, const uint32_t* pair_splats, int width,
-                  int height, const double* background, double* pixel_color, uint32_t* pixel_last) {
+/* `margin` (nullable) receives, per pixel, how close its blend came to one of the
+ * reference's discontinuities: the relative distance of any examined alpha to the
+ * 1/255 skip threshold or of the transmittance to the 1e-4 stop, or |power| at the
+ * power > 0 skip. A float32 renderer may decide those differently within rounding. */
+void oc_rasterize_margin(const double* splats, const uint32_t* ranges, const uint32_t* pair_splats, int width,
+                         int height, const double* background, double* pixel_color, uint32_t* pixel_last,
+                         double* margin) {
   int tx = (width + OC_TILE - 1) / OC_TILE;
 #pragma omp parallel for schedule(dynamic, 64)
   for (long p = 0; p < (long)width * height; p++) {
     int x = (int)(p % width), y = (int)(p / width);
     int tile = (y / OC_TILE) * tx + x / OC_TILE;
     uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
-    double T = 1, rgb[3] = {0, 0, 0};
+    double T = 1, rgb[3] = {0, 0, 0}, near = 1e300;
     uint32_t last = 0;
     for (uint32_t k = start; k < end; k++) {
       const double* s = splats + (long)pair_splats[k] * 12;
       double dx = x + 0.5 - s[0], dy = y + 0.5 - s[1];
       double power = splat_power(s, dx, dy);
+      double raw_alpha = s[11] * exp(power > 0 ? 0 : power);
+      if (raw_alpha >= 1.0 / 255.0 && fabs(power) < near) near = fabs(power);
       if (power > 0) continue;
-      double alpha = s[11] * exp(power);
+      double alpha = raw_alpha;
       if (alpha > 0.99) alpha = 0.99;
+      double da = fabs(alpha * 255.0 - 1.0);
+      if (da < near) near = da;
       if (alpha < 1.0 / 255.0) continue;
       double remaining = T * (1 - alpha);
+      double dr = fabs(remaining / 0.0001 - 1.0);
+      if (dr < near) near = dr;
       if (remaining < 0.0001) break;
       for (int ch = 0; ch < 3; ch++) rgb[ch] += s[8 + ch] * alpha * T;
       T = remaining;
@@ -296,7 +307,13 @@ void oc_rasterize(const double* splats, const uint32_t* ranges, const uint32_t* 
     for (int ch = 0; ch < 3; ch++) pixel_color[p * 4 + ch] = rgb[ch] + T * background[ch];
     pixel_color[p * 4 + 3] = T;
     pixel_last[p] = last;
+    if (margin) margin[p] = near;
   }
+}
+
+void oc_rasterize(const double* splats, const uint32_t* ranges, const uint32_t* pair_splats, int width,
+                  int height, const double* background, double* pixel_color, uint32_t* pixel_last) {
+  oc_rasterize_margin(splats, ranges, pair_splats, width, height, background, pixel_color, pixel_last, NULL);
 }
 
 /* rasterizeBackward.wgsl:49-163: the back-to-front replay, summed per splat. */

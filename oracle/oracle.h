@@ -34,6 +34,10 @@ void oc_rasterize(const double* splats, const uint32_t* ranges, const uint32_t* 
                   int width, int height, const double* background,
                   double* pixel_color, uint32_t* pixel_last);
 
+void oc_rasterize_margin(const double* splats, const uint32_t* ranges, const uint32_t* pair_splats,
+                         int width, int height, const double* background, double* pixel_color,
+                         uint32_t* pixel_last, double* margin);
+
 void oc_rasterize_backward(long n, const double* splats, const uint32_t* ranges,
                            const uint32_t* pair_splats, int width, int height,
                            const double* background, const double* pixel_color,

@@ -36,6 +36,7 @@ def lib():
         L.oc_tile_lists.argtypes = [c_long, _d, c_int, c_int, _u32, vp, c_long]
         L.oc_tile_lists.restype = c_long
         L.oc_rasterize.argtypes = [_d, _u32, _u32, c_int, c_int, _d, _d, _u32]
+        L.oc_rasterize_margin.argtypes = [_d, _u32, _u32, c_int, c_int, _d, _d, _u32, _d]
         L.oc_rasterize_backward.argtypes = [c_long, _d, _u32, _u32, c_int, c_int, _d, _d, _u32, _d, _d]
         L.oc_preprocess_backward.argtypes = [c_long, _d, _d, c_int, _d, _d, _d, _d, _d, _d, _d]
         L.oc_loss.argtypes = [c_int, c_int, _d, _u8, vp, vp]
@@ -92,15 +93,20 @@ def tile_lists(splats, width, height):
     return ranges.reshape(tiles, 2), pairs[:total]
 
 
-def rasterize(splats, ranges, pairs, width, height, background):
+def rasterize(splats, ranges, pairs, width, height, background, margin=False):
+    """(pixel_color[X,4], pixel_last[X]) and, with margin=True, each pixel's distance
+    to the nearest blend discontinuity (see oc_rasterize_margin)."""
     s = _f64(splats).reshape(-1)
     color = np.zeros(width * height * 4)
     last = np.zeros(width * height, dtype=np.uint32)
+    near = np.zeros(width * height)
     pr = np.ascontiguousarray(pairs, dtype=np.uint32)
     if pr.size == 0:
         pr = np.zeros(1, dtype=np.uint32)
-    lib().oc_rasterize(s, np.ascontiguousarray(ranges, dtype=np.uint32).reshape(-1), pr, width, height,
-                       _f64(background), color, last)
+    lib().oc_rasterize_margin(s, np.ascontiguousarray(ranges, dtype=np.uint32).reshape(-1), pr, width, height,
+                              _f64(background), color, last, near)
+    if margin:
+        return color.reshape(-1, 4), last, near
     return color.reshape(-1, 4), last
 
 

@@ -1,0 +1,165 @@
+"""PrefixSum and RadixSort on the GPU against closed forms: bit-exact.
+
+Cases follow src/checks/depthSort.ts:46-104 (300,000 values in [0, 9] from
+mulberry32(3); sort sizes 1, 1000, 1024, 100,003 with half full-range and half
+small keys) plus the microbenchmark inputs: geometric tile-touch counts (mean 6,
+capped at 256) for the scan and 64-bit (Zipf tile, depth bits) keys for the sort.
+"""
+import numpy as np
+import pytest
+
+import diorama_amd
+from diorama_amd import scene
+from tests.gpu_util import context, say
+
+pytestmark = pytest.mark.gpu
+
+
+def exclusive(values):
+    out = np.zeros_like(values, dtype=np.uint64)
+    if values.size:
+        out[1:] = np.cumsum(values[:-1].astype(np.uint64))
+    return (out & 0xFFFFFFFF).astype(np.uint32)
+
+
+def run_scan(values, capacity=None, count=None):
+    ctx = context()
+    capacity = values.size if capacity is None else capacity
+    data = ctx.buffer(max(capacity, 1) * 4)
+    if values.size:
+        data.upload(values)
+    count_buf = None
+    if count is not None:
+        count_buf = ctx.upload(np.array([count, 0, 0, 0], dtype=np.uint32))
+    scan = diorama_amd.PrefixSum(ctx, data, capacity, count_buf)
+    scan.encode()
+    out = data.download(np.uint32, max(capacity, 1))[:capacity]
+    total = int(scan.total.download(np.uint32, 1)[0])
+    scan.destroy()
+    return out, total
+
+
+def geometric_counts(n, seed):
+    rng = np.random.default_rng(seed)
+    return np.minimum(rng.geometric(1 / 6, size=n), 256).astype(np.uint32)
+
+
+@pytest.mark.parametrize("n", [0, 1, 511, 512, 4095, 4096, 4097, 100_003])
+def test_scan_sizes(n):
+    say("scan", n)
+    values = np.random.default_rng(n).integers(0, 1000, size=n).astype(np.uint32)
+    out, total = run_scan(values)
+    assert np.array_equal(out, exclusive(values))
+    assert total == int(values.astype(np.uint64).sum()) & 0xFFFFFFFF
+
+
+def test_scan_depth_sort_check_case():
+    random = scene.mulberry32(3)
+    values = np.array([int(random() * 10) for _ in range(300_000)], dtype=np.uint32)
+    say("scan", "mulberry32(3) 300000")
+    out, total = run_scan(values)
+    assert np.array_equal(out, exclusive(values))
+    assert total == int(values.sum())
+
+
+@pytest.mark.parametrize("n", [1 << 20, 16 << 20])
+def test_scan_geometric_microbench_inputs(n):
+    values = geometric_counts(n, 5)
+    say("scan geometric", n)
+    out, total = run_scan(values)
+    assert np.array_equal(out, exclusive(values))
+    assert total == int(values.astype(np.uint64).sum())
+
+
+def test_scan_count_from_device():
+    """Only min(count, capacity) values are scanned; the rest stay untouched."""
+    values = geometric_counts(50_000, 6)
+    say("scan device count")
+    out, total = run_scan(values, capacity=50_000, count=30_001)
+    assert np.array_equal(out[:30_001], exclusive(values[:30_001]))
+    assert np.array_equal(out[30_001:], values[30_001:])
+    assert total == int(values[:30_001].sum())
+
+
+def run_sort(keys, values, high_bits=0, capacity=None, count=None):
+    ctx = context()
+    n = values.size
+    capacity = n if capacity is None else capacity
+    count_buf = None if count is None else ctx.upload(np.array([count, 0, 0, 0], dtype=np.uint32))
+    sort = diorama_amd.RadixSort(ctx, capacity, high_bits, count_buf)
+    sort.keys.upload(keys)
+    sort.values.upload(values)
+    sort.encode()
+    width = 2 if high_bits else 1
+    k = sort.keys.download(np.uint32, capacity * width)
+    v = sort.values.download(np.uint32, capacity)
+    sort.destroy()
+    return k, v
+
+
+@pytest.mark.parametrize("n", [1, 1000, 1024, 100_003])
+def test_sort_32bit_depth_sort_check_case(n):
+    random = scene.mulberry32(n)
+    keys = np.array([int(random() * 2 ** 32) if i % 2 == 0 else int(random() * 16) for i in range(n)],
+                    dtype=np.uint32)
+    values = np.arange(n, dtype=np.uint32)
+    say("sort32", n)
+    k, v = run_sort(keys, values)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(v, order.astype(np.uint32))
+    assert np.array_equal(k, keys[order])
+
+
+def zipf_tiles(n, seed, tiles=62 * 35, s=1.1):
+    rng = np.random.default_rng(seed)
+    p = 1.0 / np.arange(1, tiles + 1) ** s
+    p /= p.sum()
+    perm = rng.permutation(tiles)
+    return perm[rng.choice(tiles, size=n, p=p)].astype(np.uint32)
+
+
+def microbench_keys(n, seed):
+    rng = np.random.default_rng(seed)
+    hi = zipf_tiles(n, seed)
+    lo = rng.uniform(0.2, 50.0, size=n).astype(np.float32).view(np.uint32)
+    keys = np.empty(2 * n, dtype=np.uint32)
+    keys[0::2] = lo
+    keys[1::2] = hi
+    return keys, lo, hi
+
+
+@pytest.mark.parametrize("n", [100_003, 8 << 20])
+def test_sort_64bit_microbench_keys(n):
+    keys, lo, hi = microbench_keys(n, 7)
+    values = np.random.default_rng(8).permutation(n).astype(np.uint32)  # arbitrary payloads
+    say("sort64", n)
+    k, v = run_sort(keys, values, high_bits=16)
+    order = np.lexsort((lo, hi & 0xFFFF))  # stable: ties keep input order
+    assert np.array_equal(v, values[order])
+    assert np.array_equal(k[0::2], lo[order]) and np.array_equal(k[1::2], hi[order])
+
+
+def test_sort_64bit_ignores_high_word_bits_past_high_bits():
+    """Only the low 16 bits of the high word are looked at (src/radixSort.ts:103-104)."""
+    n = 20_000
+    rng = np.random.default_rng(3)
+    lo = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+    hi = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+    keys = np.empty(2 * n, dtype=np.uint32)
+    keys[0::2], keys[1::2] = lo, hi
+    values = np.arange(n, dtype=np.uint32)
+    say("sort64 high bits")
+    k, v = run_sort(keys, values, high_bits=16)
+    assert np.array_equal(v, np.lexsort((lo, hi & 0xFFFF)).astype(np.uint32))
+
+
+def test_sort_count_from_device():
+    n, count = 30_000, 12_345
+    rng = np.random.default_rng(4)
+    keys = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+    values = np.arange(n, dtype=np.uint32)
+    say("sort device count")
+    k, v = run_sort(keys, values, count=count)
+    order = np.argsort(keys[:count], kind="stable")
+    assert np.array_equal(v[:count], order.astype(np.uint32))
+    assert np.array_equal(k[:count], keys[:count][order])

@@ -1,0 +1,205 @@
+"""The Trainer on the GPU: Adam, statistics, densification and opacity reset against
+the oracle (src/checks/densify.ts:28-185), and end-to-end learning
+(src/checks/training.ts:20-120)."""
+import math
+
+import numpy as np
+import pytest
+
+import diorama_amd
+from diorama_amd import scene
+from oracle import oracle
+from tests.gpu_util import config0_scene, context, say
+from tests.test_oracle import densify_case, rotation_columns
+
+pytestmark = pytest.mark.gpu
+
+
+def render_views(gaussians, cameras, sh=None, degree=0):
+    """Photos as rgba8 images rendered by the GPU renderer itself."""
+    ctx = context()
+    r = diorama_amd.GaussianRenderer(ctx)
+    r.set_gaussians(gaussians, sh, degree)
+    images = []
+    for cam in cameras:
+        w, h = int(cam["viewport"][0]), int(cam["viewport"][1])
+        target = ctx.buffer(w * h * 4)
+        r.encode(target, cam, [0, 0, 0])
+        images.append(target.download(np.uint32, w * h))
+    r.destroy()
+    return images
+
+
+def test_adam_and_stats_step_against_oracle():
+    params, sh, g, cam = config0_scene(seed=0, n=4096)
+    teacher_p, teacher_sh = scene.random_raw_gaussians(4096, 1, box=1.0)
+    images = render_views(scene.activate(teacher_p), [cam], teacher_sh, 3)
+    ctx = context()
+    settings = diorama_amd.default_settings(1.0)
+    say("adam step")
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], images, settings)
+    moments = np.random.default_rng(2).normal(size=(4096, 122)).astype(np.float32) * 1e-3
+    moments[:, 1::2] = np.abs(moments[:, 1::2])
+    t.buffers["moments"].upload(moments)
+    stats0 = np.abs(np.random.default_rng(3).normal(size=(4096, 4))).astype(np.float32)
+    t.buffers["stats"].upload(stats0)
+    t.optimize()
+    r = t.renderer
+    grads = r.gradients
+    gg = grads["gaussians"].download(np.float32).reshape(-1, 16)
+    shg = grads["sh"].download(np.float32).reshape(-1, 45)
+    sgrad = grads["splats"].download(np.float32).reshape(-1, 9)
+    splats = r.splat_buffer.download(np.float32).reshape(-1, 12)
+    p_after = t.buffers["params"].download(np.float32).reshape(-1, 16)
+    sh_after = r.sh_buffer.download(np.float32).reshape(-1, 45)
+    m_after = t.buffers["moments"].download(np.float32).reshape(-1, 122)
+    stats = t.buffers["stats"].download(np.float32).reshape(-1, 4)
+    act = r.gaussian_buffer.download(np.float32).reshape(-1, 16)
+    lr = [1 - 0.9, 1 - 0.999, 1.6e-4 * (1e-2) ** (1 / 30000), 0.025, 0.005, 0.001, 0.0025, 0.0025 / 20]
+    o_p, o_sh, o_m, o_act = oracle.adam_step(params, sh, moments, gg, shg, lr)
+    err_p = np.abs(p_after - o_p).max()
+    err_sh = np.abs(sh_after - o_sh).max()
+    err_m = np.abs(m_after - o_m).max() / np.abs(o_m).max()
+    print(f"adam: params err {err_p:.2e}, sh err {err_sh:.2e}, moments rel err {err_m:.2e}")
+    assert err_p < 1e-6 and err_sh < 1e-6 and err_m < 1e-5
+    cols = [k for k in range(16) if k not in (7, 15)]
+    assert np.allclose(act[:, cols], o_act[:, cols], rtol=1e-5, atol=1e-6)
+    o_stats = oracle.accumulate_stats(splats, sgrad, cam["viewport"], stats0)
+    assert np.allclose(stats, o_stats, rtol=1e-5, atol=1e-7)
+    t.destroy()
+
+
+@pytest.mark.parametrize("prune_large", [False, True])
+def test_densify_against_oracle(prune_large):
+    params, sh, moments, stats = densify_case(41 if prune_large else 43)
+    ctx = context()
+    cam = scene.camera(scene.IDENTITY, scene.IDENTITY, 16, 16)
+    image = np.zeros(256, dtype=np.uint32)
+    settings = diorama_amd.default_settings(1.0)
+    say("densify", prune_large)
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], [image], settings)
+    t.buffers["moments"].upload(moments)
+    t.buffers["stats"].upload(stats)
+    t.iteration = 700
+    t.densify(prune_large)
+    count, o_params, o_sh, o_m, tally = oracle.densify(params, sh, moments, stats,
+                                                       [0.0002, 0.01, 0.005, 0.1 if prune_large else 0.0], 700)
+    assert t.count == count
+    rec = t.densifications[0]
+    assert [rec["removed"], rec["kept"], rec["cloned"], rec["split"]] == list(tally)
+    raw = t.export_raw()
+    got_p = raw["params"]
+    child = got_p[:, 7] == 2
+    assert np.array_equal(child, o_params[:, 7] == 2)
+    err_pos = np.abs(got_p[child, 0:3] - o_params[child, 0:3]).max() if child.any() else 0.0
+    print(f"densify: {count} out; split position err {err_pos:.2e}")
+    assert err_pos < 1e-4
+    assert np.allclose(got_p[:, 3:], o_params[:, 3:], atol=1e-5)
+    assert np.allclose(got_p[~child, 0:3], o_params[~child, 0:3], atol=0)
+    assert np.array_equal(raw["sh"], o_sh.astype(np.float32))
+    m = t.buffers["moments"].download(np.float32).reshape(-1, 122)
+    assert np.array_equal(m, o_m.astype(np.float32))
+    assert not t.buffers["stats"].download(np.float32).any()
+    act = t.renderer.gaussian_buffer.download(np.float32).reshape(-1, 16)
+    assert np.allclose(act, scene.activate(got_p) * (np.arange(16) % 8 != 7), rtol=1e-5, atol=1e-6)
+    # The opacity reset right after: only opacity and its Adam history change.
+    t.reset_opacity()
+    p2 = t.buffers["params"].download(np.float32).reshape(-1, 16)
+    m2 = t.buffers["moments"].download(np.float32).reshape(-1, 122)
+    capped = math.log(0.01 / 0.99)
+    assert np.allclose(p2[:, 3], np.minimum(got_p[:, 3], capped), atol=1e-5)
+    assert np.all(m2[:, 6:8] == 0)
+    keep = np.ones(122, dtype=bool)
+    keep[6:8] = False
+    assert np.array_equal(m2[:, keep], m[:, keep])
+    t.destroy()
+
+
+def test_densify_removing_everything_is_an_error():
+    """src/trainer.ts:295."""
+    ctx = context()
+    params = np.zeros((8, 16), dtype=np.float32)
+    params[:, 3] = -20  # opacity ~0: all pruned
+    params[:, 8] = 1
+    cam = scene.camera(scene.IDENTITY, scene.IDENTITY, 16, 16)
+    t = diorama_amd.Trainer(ctx, params, np.zeros((8, 45), np.float32), [cam], [np.zeros(256, np.uint32)],
+                            diorama_amd.default_settings(1.0))
+    with pytest.raises(diorama_amd.DioramaError, match="removed every Gaussian"):
+        t.densify(False)
+    t.destroy()
+
+
+def test_uneven_views_are_an_error():
+    ctx = context()
+    cams = [scene.camera(scene.IDENTITY, scene.IDENTITY, 16, 16), scene.camera(scene.IDENTITY, scene.IDENTITY, 8, 8)]
+    params = np.zeros((4, 16), dtype=np.float32)
+    params[:, 8] = 1
+    with pytest.raises(diorama_amd.DioramaError, match="same size"):
+        diorama_amd.Trainer(ctx, params, np.zeros((4, 45), np.float32), cams,
+                            [np.zeros(256, np.uint32), np.zeros(64, np.uint32)], diorama_amd.default_settings(1.0))
+
+
+def training_scene():
+    """src/checks/training.ts:20-80: a 60-Gaussian truth seen from 8 ring views at 96x96."""
+    random = scene.mulberry32(17)
+    between = lambda lo, hi: lo + (hi - lo) * random()
+    truth = []
+    for _ in range(60):
+        position = [between(-1, 1), between(-1, 1), between(-1, 1)]
+        s = [between(0.08, 0.3), between(0.08, 0.3), between(0.08, 0.3)]
+        q = [between(-1, 1), between(-1, 1), between(-1, 1), between(-1, 1)]
+        color = [random(), random(), random()]
+        truth.append(position + [between(0.6, 0.95)] + s + [0] + q + color + [0])
+    truth = np.array(truth, dtype=np.float32)
+    cams = []
+    for k in range(8):
+        a = k / 8 * 2 * math.pi
+        eye = [3.5 * math.sin(a), 1.2, 3.5 * math.cos(a)]
+        cams.append(scene.camera(scene.look_at(eye, [0, 0, 0], [0, 1, 0]), scene.perspective(math.pi / 3.5, 1, 0.1, 100),
+                                 96, 96))
+    images = render_views(truth, cams)
+    scrambled = np.zeros((60, 16), dtype=np.float32)
+    for i in range(60):
+        for k in range(3):
+            scrambled[i, k] = truth[i, k] + between(-0.15, 0.15)
+            scrambled[i, 4 + k] = math.log(truth[i, 4 + k] * between(0.6, 1.6))
+            scrambled[i, 12 + k] = (random() - 0.5) / scene.SH_C0
+        scrambled[i, 3] = 0
+        scrambled[i, 8:12] = [1, 0, 0, 0]
+    sparse = np.zeros((12, 16), dtype=np.float32)
+    for i in range(12):
+        for k in range(3):
+            sparse[i, k] = between(-0.8, 0.8)
+            sparse[i, 4 + k] = math.log(0.25)
+        sparse[i, 3] = math.log(0.1 / 0.9)
+        sparse[i, 8:12] = [1, 0, 0, 0]
+    return cams, images, scrambled, sparse
+
+
+def train(params, cams, images, steps=400, **overrides):
+    ctx = context()
+    settings = diorama_amd.default_settings(3.85)
+    settings.update(overrides)
+    n = len(params)
+    t = diorama_amd.Trainer(ctx, params, np.zeros((n, 45), np.float32), cams, images, settings)
+    before = t.evaluate(cams, images, with_ssim=False)["psnr"]
+    for _ in range(steps):
+        t.step()
+    after = t.evaluate(cams, images, with_ssim=False)["psnr"]
+    count = t.count
+    loss = t.last_loss()
+    t.destroy()
+    return before, after, count, loss
+
+
+def test_training_learns_and_densifying_helps():
+    cams, images, scrambled, sparse = training_scene()
+    say("training fit")
+    before, after, _, loss = train(scrambled, cams, images)
+    print(f"training: PSNR {before:.1f} -> {after:.1f} dB in 400 steps, last loss {loss:.4f}")
+    assert after - before >= 6
+    say("training sparse")
+    _, plain, _, _ = train(sparse.copy(), cams, images, densify_until=0)
+    _, dens, count, _ = train(sparse.copy(), cams, images, densify_from=100, densify_interval=50, densify_until=351)
+    print(f"densify: {dens:.1f} dB with {count} Gaussians vs {plain:.1f} dB without")
+    assert count > 12 and dens > plain + 1
