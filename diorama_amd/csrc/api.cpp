@@ -431,7 +431,8 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
                              dev<uint32_t>(t->offsets), nullptr, t->count, dev<uint32_t>(t->pair_total),
                              dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
   dio::duplicate(s, splats, dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets), dev<uint32_t>(t->tile_counts),
-                 t->count, w, h, t->capacity, dev<uint32_t>(t->pair_tiles), dev<uint32_t>(t->pair_splats));
+                 t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total), dev<uint32_t>(t->pair_tiles),
+                 dev<uint32_t>(t->pair_splats));
   t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats), t->alt_tiles->ptr,
                                      dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total), t->capacity, 0,
                                      tile_sort_bits(tiles), dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity));

@@ -84,7 +84,7 @@ void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width,
 
 void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
                const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
-               uint32_t* pair_tiles, uint32_t* pair_splats);
+               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats);
 
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
                  uint32_t tiles, uint32_t* ranges);

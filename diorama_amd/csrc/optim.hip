@@ -29,16 +29,17 @@ __global__ __launch_bounds__(kThreads) void adam_kernel(uint32_t n, float* __res
                                                          const float* __restrict__ gaussian_grads,
                                                          const float* __restrict__ sh_grads, AdamArgs a,
                                                          float* __restrict__ gaussians) {
-  const size_t f = static_cast<size_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (f >= static_cast<size_t>(n) * kMomentSlots) return;
-  const uint32_t i = static_cast<uint32_t>(f / kMomentSlots);
-  const uint32_t slot = static_cast<uint32_t>(f - static_cast<size_t>(i) * kMomentSlots);
+  // 32-bit slot index: the division by 61 stays a multiply-high (n < 70M).
+  const uint32_t f = blockIdx.x * kThreads + threadIdx.x;
+  if (f >= n * kMomentSlots) return;
+  const uint32_t i = f / kMomentSlots;
+  const uint32_t slot = f - i * kMomentSlots;
   if (slot == 7 || slot == 15) return;  // padding: kind and birth stamps, never trained
   float* value;
   float grad, lr;
   if (slot < kFloatsPerGaussian) {
-    value = params + static_cast<size_t>(i) * kFloatsPerGaussian + slot;
-    grad = gaussian_grads[static_cast<size_t>(i) * kFloatsPerGaussian + slot];
+    value = params + i * kFloatsPerGaussian + slot;
+    grad = gaussian_grads[i * kFloatsPerGaussian + slot];
     if (slot < 3) {
       lr = a.position_lr;
     } else if (slot == 3) {
@@ -55,7 +56,7 @@ __global__ __launch_bounds__(kThreads) void adam_kernel(uint32_t n, float* __res
       lr = a.color_lr;
     }
   } else {
-    const size_t k = static_cast<size_t>(i) * kShRest + (slot - kFloatsPerGaussian);
+    const uint32_t k = i * kShRest + (slot - kFloatsPerGaussian);
     value = sh + k;
     grad = sh_grads[k];
     lr = a.sh_lr;
@@ -66,8 +67,7 @@ __global__ __launch_bounds__(kThreads) void adam_kernel(uint32_t n, float* __res
   moments[f] = m;
   const float updated = *value - lr * (m.x / a.bias_correction1) / (sqrtf(m.y / a.bias_correction2) + kEpsilon);
   *value = updated;
-  if (gaussians && slot < kFloatsPerGaussian)
-    gaussians[static_cast<size_t>(i) * kFloatsPerGaussian + slot] = activated(slot, updated);
+  if (gaussians && slot < kFloatsPerGaussian) gaussians[i * kFloatsPerGaussian + slot] = activated(slot, updated);
 }
 
 __global__ __launch_bounds__(kThreads) void activate_kernel(uint32_t n, const float* __restrict__ params,
@@ -216,6 +216,7 @@ void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const floa
 void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* moments, const float* gaussian_grads,
                const float* sh_grads, const AdamArgs& a, float* gaussians) {
   if (n == 0) return;
+  if (static_cast<uint64_t>(n) * kMomentSlots >= (1ull << 32)) throw Error(-2, "adam: too many Gaussians");
   const size_t slots = static_cast<size_t>(n) * kMomentSlots;
   adam_kernel<<<ceil_div(slots, kThreads), kThreads, 0, s>>>(n, params, sh, reinterpret_cast<float2*>(moments),
                                                              gaussian_grads, sh_grads, a, gaussians);

@@ -93,61 +93,80 @@ __global__ __launch_bounds__(kThreads) void count_tiles_kernel(const float4* __r
   depth_values[i] = i;
 }
 
-// duplicate (tiles.wgsl:36-59), one wavefront per 64 depth-ordered splats: the
-// wave's pairs are dealt out one per lane, so a splat covering many tiles does
-// not serialise on one thread, and consecutive lanes write consecutive slots.
+// duplicate (tiles.wgsl:36-59), load-balanced over pairs rather than splats: each
+// workgroup writes a fixed 2048-slot stretch of the pair buffers. It finds the
+// depth-ordered splat owning its first slot with a 64-ary search of the offsets
+// (one wavefront, a few rounds), then deals its slots out to threads from LDS
+// windows of 256 splats. The nearest splats cover the most tiles and sit together
+// in depth order, so splat-parallel work would pile onto a few waves.
+constexpr uint32_t kDupPairs = 2048;
+
 __global__ __launch_bounds__(kThreads) void duplicate_kernel(const float4* __restrict__ splats,
                                                               const uint32_t* __restrict__ order,
                                                               const uint32_t* __restrict__ offsets,
                                                               const uint32_t* __restrict__ tile_counts, uint32_t n,
                                                               uint32_t tiles_x, uint32_t tiles_y, uint32_t capacity,
+                                                              const uint32_t* __restrict__ pair_total,
                                                               uint32_t* __restrict__ pair_tiles,
                                                               uint32_t* __restrict__ pair_splats) {
-  __shared__ uint32_t s_end[kThreads];  // inclusive prefix of counts within the wave
-  __shared__ uint32_t s_x0[kThreads], s_y0[kThreads], s_w[kThreads], s_splat[kThreads];
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t j = blockIdx.x * kThreads + tid;
-  uint32_t count = 0, splat = 0, r[4] = {0, 0, 0, 0};
-  if (j < n) {
-    splat = order[j];
-    count = tile_counts[splat];
-    if (count) {
-      const float4 s0 = splats[splat * 3];
-      tile_rect(s0.x, s0.y, s0.w, tiles_x, tiles_y, r);
+  __shared__ uint32_t s_start[kThreads], s_count[kThreads], s_x0[kThreads], s_y0[kThreads], s_w[kThreads],
+      s_splat[kThreads];
+  __shared__ uint32_t s_first;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const uint32_t total = min(*pair_total, capacity);  // slots past the capacity are dropped
+  const uint32_t p0 = blockIdx.x * kDupPairs;
+  if (p0 >= total) return;
+  const uint32_t p_end = min(p0 + kDupPairs, total);
+  if (tid < kWave) {
+    // The last j with offsets[j] <= p0 (offsets never decrease; offsets[0] = 0).
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+      const uint32_t stride = ceil_div(hi - lo, kWave);
+      const uint32_t probe = lo + lane * stride;
+      const uint64_t ok = __ballot(probe < hi && offsets[probe] <= p0);
+      lo += static_cast<uint32_t>(kWave - 1 - __builtin_clzll(ok)) * stride;
+      hi = min(hi, lo + stride);
     }
+    if (lane == 0) s_first = lo;
   }
-  uint32_t inclusive = count;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    uint32_t y = __shfl_up(inclusive, o, kWave);
-    if (lane >= o) inclusive += y;
-  }
-  const uint32_t wave_total = __shfl(inclusive, kWave - 1, kWave);
-  s_end[tid] = inclusive;
-  s_x0[tid] = r[0];
-  s_y0[tid] = r[1];
-  s_w[tid] = r[2] - r[0];
-  s_splat[tid] = splat;
   __syncthreads();
-  if (wave_total == 0) return;
-  // Slots of this wave's pairs start at the offset of its first splat.
-  const uint32_t wave_base = offsets[blockIdx.x * kThreads + wave * kWave];
-  const int w0 = wave * kWave;
-  for (uint32_t q = lane; q < wave_total; q += kWave) {
-    int lo = 0, hi = kWave - 1;  // first lane whose inclusive end exceeds q
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (s_end[w0 + mid] > q) hi = mid; else lo = mid + 1;
+  uint32_t j_base = s_first, p_lo = p0;
+  while (p_lo < p_end && j_base < n) {
+    const uint32_t j = j_base + tid;
+    uint32_t start = 0xffffffffu, count = 0, splat = 0, r[4] = {0, 0, 0, 0};
+    if (j < n) {
+      splat = order[j];
+      start = offsets[j];
+      count = tile_counts[splat];
+      if (count) {
+        const float4 s0 = splats[splat * 3];
+        tile_rect(s0.x, s0.y, s0.w, tiles_x, tiles_y, r);
+      }
     }
-    const uint32_t own_count = (lo == 0 ? s_end[w0] : s_end[w0 + lo] - s_end[w0 + lo - 1]);
-    const uint32_t k = q - (s_end[w0 + lo] - own_count);
-    const uint32_t width = s_w[w0 + lo];
-    const uint32_t tx = s_x0[w0 + lo] + k % width, ty = s_y0[w0 + lo] + k / width;
-    const uint32_t slot = wave_base + q;
-    if (slot < capacity) {
-      pair_tiles[slot] = ty * tiles_x + tx;
-      pair_splats[slot] = s_splat[w0 + lo];
+    s_start[tid] = start;
+    s_count[tid] = count;
+    s_x0[tid] = r[0];
+    s_y0[tid] = r[1];
+    s_w[tid] = r[2] - r[0];
+    s_splat[tid] = splat;
+    __syncthreads();
+    const int valid = static_cast<int>(min(static_cast<uint32_t>(kThreads), n - j_base));
+    const uint32_t window_end = s_start[valid - 1] + s_count[valid - 1];
+    const uint32_t p_hi = min(p_end, window_end);
+    for (uint32_t p = p_lo + tid; p < p_hi; p += kThreads) {
+      int lo = 0, hi = valid - 1;  // the last entry starting at or before p
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_start[mid] <= p) lo = mid; else hi = mid - 1;
+      }
+      const uint32_t k = p - s_start[lo], width = s_w[lo];
+      const uint32_t row = k / width;
+      pair_tiles[p] = (s_y0[lo] + row) * tiles_x + s_x0[lo] + (k - row * width);
+      pair_splats[p] = s_splat[lo];
     }
+    p_lo = max(p_lo, p_hi);
+    j_base += kThreads;
+    __syncthreads();
   }
 }
 
@@ -173,64 +192,112 @@ __global__ __launch_bounds__(kThreads) void find_ranges_kernel(const uint32_t* _
 }
 
 // -------------------------------------------------------------- rasterize --
+//
+// Both rasterizer kernels run one workgroup per 16x16 tile as four independent
+// wavefronts, one per 8x8 quadrant (one pixel per lane). A wave stages 64 of the
+// tile's splats at a time in registers (one per lane) and broadcasts each to the
+// whole wave with v_readlane into scalar registers, so the blend loop reads no
+// LDS and the forward pass needs no workgroup barrier at all. Each staged splat
+// also gets the bounding box of its alpha >= 1/255 ellipse; a wave only visits
+// the splats whose box meets its quadrant (a ballot mask), which changes no
+// result: outside the box every pixel would skip the splat anyway.
 
-constexpr int kBatch = 256;
+constexpr int kChunk = kWave;
+constexpr int kGrads = 9;
 
-// rasterize.wgsl:31-100: one workgroup per 16x16 tile, splats staged 256 at a time
-// in LDS, front-to-back blend per pixel, and the whole tile stops once every
-// pixel is nearly opaque (one barrier-count per batch).
-__global__ __launch_bounds__(kThreads) void rasterize_kernel(const float4* __restrict__ splats,
-                                                              const uint32_t* __restrict__ pair_splats,
-                                                              const uint2* __restrict__ ranges, uint32_t width,
-                                                              uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
-                                                              float bg_b, uint32_t* __restrict__ target,
-                                                              float4* __restrict__ pixel_color,
-                                                              uint32_t* __restrict__ pixel_last) {
-  __shared__ float4 s_geom[kBatch];   // mean x, mean y, conic a, conic b
-  __shared__ float4 s_shade[kBatch];  // conic c, opacity, r, g
-  __shared__ float s_blue[kBatch];
-  const int tid = threadIdx.x;
+__device__ inline float lane_value(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+
+// One splat of a chunk, staged in a lane's registers.
+struct Staged {
+  float mx, my, ca, cb, cc, op, r, g, b;
+  float x0, x1, y0, y1;  // box holding every pixel centre where opacity * G >= 1/255
+  uint32_t index;
+};
+
+__device__ inline void stage(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, uint32_t k,
+                             Staged& s) {
+  s.index = pair_splats[k];
+  const float4 a = splats[s.index * 3], b = splats[s.index * 3 + 1], c = splats[s.index * 3 + 2];
+  s.mx = a.x; s.my = a.y;
+  s.ca = b.x; s.cb = b.y; s.cc = b.z;
+  s.r = c.x; s.g = c.y; s.b = c.z; s.op = c.w;
+  // opacity * exp(-q) >= 1/255 needs q <= t = ln(255 opacity), with
+  // q = (a dx² + c dy²)/2 + b dx dy. That ellipse's half-widths are sqrt(2t Σxx)
+  // and sqrt(2t Σyy), Σ = conic⁻¹. Widened by 1% and 0.01 px against rounding.
+  const float det = s.ca * s.cc - s.cb * s.cb;
+  const float t = __logf(255.f * s.op) * 1.01f + 0.01f;
+  float ex = sqrtf(2.f * t * s.cc / det) + 0.01f, ey = sqrtf(2.f * t * s.ca / det) + 0.01f;
+  ex = !(det > 0.f) ? INFINITY : (!(t > 0.f) ? -INFINITY : ex);  // degenerate: never cull; too faint: always
+  ey = !(det > 0.f) ? INFINITY : (!(t > 0.f) ? -INFINITY : ey);
+  s.x0 = s.mx - ex;
+  s.x1 = s.mx + ex;
+  s.y0 = s.my - ey;
+  s.y1 = s.my + ey;
+}
+
+__device__ inline bool meets(const Staged& s, float bx0, float bx1, float by0, float by1) {
+  return s.x1 >= bx0 && s.x0 <= bx1 && s.y1 >= by0 && s.y0 <= by1;
+}
+
+// rasterize.wgsl:31-100: front-to-back blend; a wave stops once all its pixels
+// are nearly opaque.
+__global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict__ splats,
+                                                         const uint32_t* __restrict__ pair_splats,
+                                                         const uint2* __restrict__ ranges, uint32_t width,
+                                                         uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
+                                                         float bg_b, uint32_t* __restrict__ target,
+                                                         float4* __restrict__ pixel_color,
+                                                         uint32_t* __restrict__ pixel_last) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   const uint32_t tile = blockIdx.x;
-  const uint32_t px = (tile % tiles_x) * kTile + (tid % kTile);
-  const uint32_t py = (tile / tiles_x) * kTile + (tid / kTile);
+  const uint32_t qx = (tile % tiles_x) * kTile + (wave & 1) * 8, qy = (tile / tiles_x) * kTile + (wave >> 1) * 8;
+  const uint32_t px = qx + (lane & 7), py = qy + (lane >> 3);
   const bool inside = px < width && py < height;
   const float cx = static_cast<float>(px) + 0.5f, cy = static_cast<float>(py) + 0.5f;
+  const float bx0 = static_cast<float>(qx) + 0.5f, bx1 = bx0 + 7.f;
+  const float by0 = static_cast<float>(qy) + 0.5f, by1 = by0 + 7.f;
   const uint2 run = ranges[tile];
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
   bool done = !inside;
   uint32_t last = 0;
-  for (uint32_t start = run.x; start < run.y; start += kBatch) {
-    if (__syncthreads_count(done) == kThreads) break;
-    const uint32_t k = start + tid;
-    if (k < run.y) {
-      const uint32_t idx = pair_splats[k];
-      const float4 a = splats[idx * 3], b = splats[idx * 3 + 1], c = splats[idx * 3 + 2];
-      s_geom[tid] = make_float4(a.x, a.y, b.x, b.y);
-      s_shade[tid] = make_float4(b.z, c.w, c.x, c.y);
-      s_blue[tid] = c.z;
+  for (uint32_t base = run.x; base < run.y; base += kChunk) {
+    if (__all(done)) break;
+    Staged s;
+    bool hit = false;
+    if (base + lane < run.y) {
+      stage(splats, pair_splats, base + lane, s);
+      hit = meets(s, bx0, bx1, by0, by1);
     }
-    __syncthreads();
-    const int batch = static_cast<int>(min(static_cast<uint32_t>(kBatch), run.y - start));
-    if (!done) {
-      for (int j = 0; j < batch; j++) {
-        const float4 ge = s_geom[j];
-        const float4 sh = s_shade[j];
-        const float dx = cx - ge.x, dy = cy - ge.y;
-        const float power = -0.5f * (ge.z * dx * dx + sh.x * dy * dy) - ge.w * dx * dy;
-        if (power > 0.f) continue;
-        const float alpha = fminf(0.99f, sh.y * __expf(power));
-        if (alpha < 1.f / 255.f) continue;
-        const float remaining = T * (1.f - alpha);
-        if (remaining < 0.0001f) {
-          done = true;
-          break;
+    uint64_t mask = __ballot(hit);
+    while (mask) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const float mx = lane_value(s.mx, j), my = lane_value(s.my, j);
+      const float ca = lane_value(s.ca, j), cbb = lane_value(s.cb, j), cc = lane_value(s.cc, j);
+      const float op = lane_value(s.op, j);
+      const float sr = lane_value(s.r, j), sg = lane_value(s.g, j), sb = lane_value(s.b, j);
+      if (!done) {
+        const float dx = cx - mx, dy = cy - my;
+        const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
+        if (power <= 0.f) {
+          const float alpha = fminf(0.99f, op * __expf(power));
+          if (alpha >= 1.f / 255.f) {
+            const float remaining = T * (1.f - alpha);
+            if (remaining < 0.0001f) {
+              done = true;
+            } else {
+              const float w = alpha * T;
+              cr += sr * w;
+              cg += sg * w;
+              cb += sb * w;
+              T = remaining;
+              last = base + j - run.x + 1;
+            }
+          }
         }
-        const float w = alpha * T;
-        cr += sh.z * w;
-        cg += sh.w * w;
-        cb += s_blue[j] * w;
-        T = remaining;
-        last = start + j - run.x + 1;
       }
     }
   }
@@ -248,37 +315,50 @@ __global__ __launch_bounds__(kThreads) void rasterize_kernel(const float4* __res
 
 // ------------------------------------------------------ rasterize backward --
 
-constexpr int kBackBatch = 64;
-constexpr int kGrads = 9;
+template <int kCtrl, int kRowMask = 0xf>
+__device__ inline float dpp_add(float v) {
+  return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, kRowMask,
+                                                                   0xf, false));
+}
 
-__device__ inline float wave_sum(float x) {
+// Sum over the 64 lanes with DPP row operations; the total lands in lane 63.
+__device__ inline float wave_total(float v) {
+  v = dpp_add<0xb1>(v);         // quad_perm [1,0,3,2]
+  v = dpp_add<0x4e>(v);         // quad_perm [2,3,0,1]: quad sums
+  v = dpp_add<0x141>(v);        // row_half_mirror: 8-lane sums
+  v = dpp_add<0x140>(v);        // row_mirror: 16-lane row sums
+  v = dpp_add<0x142, 0xa>(v);   // row_bcast:15 into rows 1, 3
+  v = dpp_add<0x143, 0xc>(v);   // row_bcast:31 into rows 2, 3
+  return lane_value(v, kWave - 1);
+}
+
+__device__ inline uint32_t wave_max(uint32_t v) {
 #pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
-  return x;
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, kWave)));
+  return v;
 }
 
 // rasterizeBackward.wgsl:49-163: each pixel replays its list back to front from
-// its last blended splat; a splat's 9 gradients are summed across each wavefront
-// with shuffles (skipped when no lane of the wave touched it), the 4 wave sums
-// are added in LDS once per 64-splat batch, and one hardware float atomic per
-// (splat, tile, gradient) goes to global memory.
-__global__ __launch_bounds__(kThreads) void rasterize_backward_kernel(
+// its last blended splat. Per splat, a wave sums its pixels' 9 gradients with DPP
+// and parks the total in the splat's staging lane; per 64-splat chunk the four
+// quadrant waves add their totals in LDS and one float atomic per (splat, tile,
+// gradient) goes out.
+__global__ __launch_bounds__(256) void rasterize_backward_kernel(
     const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
     uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g, float bg_b,
     const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
     const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads) {
-  __shared__ float4 s_geom[kBackBatch];   // mean x, mean y, conic a, conic b
-  __shared__ float4 s_shade[kBackBatch];  // conic c, opacity, r, g
-  __shared__ float s_blue[kBackBatch];
-  __shared__ uint32_t s_index[kBackBatch];
-  __shared__ float s_sums[kThreads / kWave][kBackBatch][kGrads];
-  __shared__ uint32_t s_max_last;
+  __shared__ float s_sums[4][kGrads][kChunk];
+  __shared__ uint32_t s_index[kChunk];
+  __shared__ uint32_t s_last[4];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const uint32_t tile = blockIdx.x;
-  const uint32_t px = (tile % tiles_x) * kTile + (tid % kTile);
-  const uint32_t py = (tile / tiles_x) * kTile + (tid / kTile);
+  const uint32_t qx = (tile % tiles_x) * kTile + (wave & 1) * 8, qy = (tile / tiles_x) * kTile + (wave >> 1) * 8;
+  const uint32_t px = qx + (lane & 7), py = qy + (lane >> 3);
   const bool inside = px < width && py < height;
   const float cx = static_cast<float>(px) + 0.5f, cy = static_cast<float>(py) + 0.5f;
+  const float bx0 = static_cast<float>(qx) + 0.5f, bx1 = bx0 + 7.f;
+  const float by0 = static_cast<float>(qy) + 0.5f, by1 = by0 + 7.f;
   const uint2 run = ranges[tile];
   float final_T = 0.f, gr = 0.f, gg = 0.f, gb = 0.f;
   uint32_t last = 0;
@@ -289,56 +369,59 @@ __global__ __launch_bounds__(kThreads) void rasterize_backward_kernel(
     const float4 pg = pixel_grads[index];
     gr = pg.x; gg = pg.y; gb = pg.z;
   }
-  if (tid == 0) s_max_last = 0;
+  const uint32_t my_last = wave_max(last);
+  if (lane == 0) s_last[wave] = my_last;
   __syncthreads();
-  if (last) atomicMax(&s_max_last, last);
-  __syncthreads();
-  const uint32_t end = run.x + s_max_last;
+  const uint32_t tile_last = max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
   const float bg_grad = bg_r * gr + bg_g * gg + bg_b * gb;
   float T = final_T;
   float behind_r = 0.f, behind_g = 0.f, behind_b = 0.f;
   float next_alpha = 0.f, next_r = 0.f, next_g = 0.f, next_b = 0.f;
 
-  for (uint32_t batch_end = end; batch_end > run.x;) {
-    const uint32_t batch_start = batch_end - run.x > kBackBatch ? batch_end - kBackBatch : run.x;
-    const int count = static_cast<int>(batch_end - batch_start);
-    __syncthreads();
-    if (tid < count) {
-      const uint32_t idx = pair_splats[batch_start + tid];
-      const float4 a = splats[idx * 3], b = splats[idx * 3 + 1], c = splats[idx * 3 + 2];
-      s_geom[tid] = make_float4(a.x, a.y, b.x, b.y);
-      s_shade[tid] = make_float4(b.z, c.w, c.x, c.y);
-      s_blue[tid] = c.z;
-      s_index[tid] = idx;
+  for (int c = static_cast<int>(ceil_div(tile_last, kChunk)) - 1; c >= 0; c--) {
+    const uint32_t base = run.x + static_cast<uint32_t>(c) * kChunk;
+    Staged s;
+    bool hit = false;
+    s.index = 0;
+    if (base + lane < run.y) {
+      stage(splats, pair_splats, base + lane, s);
+      hit = base + lane - run.x < my_last && meets(s, bx0, bx1, by0, by1);
     }
-    __syncthreads();
-    for (int slot = count - 1; slot >= 0; slot--) {
+    if (wave == 0) s_index[lane] = s.index;
+    float res[kGrads];
+#pragma unroll
+    for (int v = 0; v < kGrads; v++) res[v] = 0.f;
+    uint64_t mask = __ballot(hit);
+    while (mask) {
+      const int j = kWave - 1 - __builtin_clzll(mask);
+      mask &= ~(1ull << j);
+      const float mx = lane_value(s.mx, j), my = lane_value(s.my, j);
+      const float ca = lane_value(s.ca, j), cbb = lane_value(s.cb, j), cc = lane_value(s.cc, j);
+      const float op = lane_value(s.op, j);
+      const float sr = lane_value(s.r, j), sg = lane_value(s.g, j), sb = lane_value(s.b, j);
       float share[kGrads];
 #pragma unroll
       for (int v = 0; v < kGrads; v++) share[v] = 0.f;
       bool contributes = false;
-      if (batch_start + slot - run.x < last) {
-        const float4 ge = s_geom[slot];
-        const float4 sh = s_shade[slot];
-        const float dx = cx - ge.x, dy = cy - ge.y;
-        const float power = -0.5f * (ge.z * dx * dx + sh.x * dy * dy) - ge.w * dx * dy;
+      if (base + j - run.x < last) {
+        const float dx = cx - mx, dy = cy - my;
+        const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
         if (power <= 0.f) {
           const float G = __expf(power);
-          const float alpha = fminf(0.99f, sh.y * G);
+          const float alpha = fminf(0.99f, op * G);
           if (alpha >= 1.f / 255.f) {
             contributes = true;
             T /= 1.f - alpha;
-            const float cr = sh.z, cg = sh.w, cb = s_blue[slot];
             behind_r = next_alpha * next_r + (1.f - next_alpha) * behind_r;
             behind_g = next_alpha * next_g + (1.f - next_alpha) * behind_g;
             behind_b = next_alpha * next_b + (1.f - next_alpha) * behind_b;
             next_alpha = alpha;
-            next_r = cr; next_g = cg; next_b = cb;
-            const float alpha_grad = T * ((cr - behind_r) * gr + (cg - behind_g) * gg + (cb - behind_b) * gb) -
+            next_r = sr; next_g = sg; next_b = sb;
+            const float alpha_grad = T * ((sr - behind_r) * gr + (sg - behind_g) * gg + (sb - behind_b) * gb) -
                                      final_T / (1.f - alpha) * bg_grad;
-            const float power_grad = alpha_grad * sh.y * G;
-            share[0] = power_grad * (ge.z * dx + ge.w * dy);
-            share[1] = power_grad * (ge.w * dx + sh.x * dy);
+            const float power_grad = alpha_grad * op * G;
+            share[0] = power_grad * (ca * dx + cbb * dy);
+            share[1] = power_grad * (cbb * dx + cc * dy);
             share[2] = power_grad * (-0.5f * dx * dx);
             share[3] = power_grad * (-dx * dy);
             share[4] = power_grad * (-0.5f * dy * dy);
@@ -351,25 +434,23 @@ __global__ __launch_bounds__(kThreads) void rasterize_backward_kernel(
         }
       }
       if (__any(contributes)) {
+        const bool mine = lane == j;  // the splat's staging lane keeps the wave's total
 #pragma unroll
-        for (int v = 0; v < kGrads; v++) share[v] = wave_sum(share[v]);
-      }
-      if (lane < kGrads) {
-        float mine = 0.f;
-#pragma unroll
-        for (int v = 0; v < kGrads; v++) mine = lane == v ? share[v] : mine;
-        s_sums[wave][slot][lane] = mine;
+        for (int v = 0; v < kGrads; v++) {
+          const float total = wave_total(share[v]);
+          res[v] = mine ? total : res[v];
+        }
       }
     }
-    __syncthreads();
-    for (int e = tid; e < count * kGrads; e += kThreads) {
-      const int slot = e / kGrads, v = e % kGrads;
-      float total = 0.f;
 #pragma unroll
-      for (int w = 0; w < kThreads / kWave; w++) total += s_sums[w][slot][v];
+    for (int v = 0; v < kGrads; v++) s_sums[wave][v][lane] = res[v];
+    __syncthreads();
+    for (int e = tid; e < kGrads * kChunk; e += 256) {
+      const int v = e / kChunk, slot = e % kChunk;
+      const float total = s_sums[0][v][slot] + s_sums[1][v][slot] + s_sums[2][v][slot] + s_sums[3][v][slot];
       if (total != 0.f) unsafeAtomicAdd(splat_grads + static_cast<size_t>(s_index[slot]) * kGrads + v, total);
     }
-    batch_end = batch_start;
+    __syncthreads();
   }
 }
 
@@ -537,12 +618,11 @@ void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width,
 
 void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
                const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
-               uint32_t* pair_tiles, uint32_t* pair_splats) {
-  if (n == 0) return;
-  duplicate_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), order, offsets,
-                                                               tile_counts, n, ceil_div(width, kTile),
-                                                               ceil_div(height, kTile), capacity, pair_tiles,
-                                                               pair_splats);
+               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats) {
+  if (n == 0 || capacity == 0) return;
+  duplicate_kernel<<<ceil_div(capacity, kDupPairs), kThreads, 0, s>>>(
+      reinterpret_cast<const float4*>(splats), order, offsets, tile_counts, n, ceil_div(width, kTile),
+      ceil_div(height, kTile), capacity, pair_total, pair_tiles, pair_splats);
   DIO_LAUNCHED("duplicate");
 }
 
