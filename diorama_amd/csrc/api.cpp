@@ -985,6 +985,8 @@ void trainer_optimize(dio_trainer* t) {
   dio::AdamArgs a;
   a.bias_correction1 = static_cast<float>(1 - std::pow(0.9, t->adam_steps));
   a.bias_correction2 = static_cast<float>(1 - std::pow(0.999, t->adam_steps));
+  a.inv_bias_correction1 = static_cast<float>(1 / (1 - std::pow(0.9, t->adam_steps)));
+  a.inv_bias_correction2 = static_cast<float>(1 / (1 - std::pow(0.999, t->adam_steps)));
   a.position_lr = static_cast<float>(position_lr);
   a.opacity_lr = st.opacity_lr;
   a.scale_lr = st.scale_lr;

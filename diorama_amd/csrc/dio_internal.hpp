@@ -3,7 +3,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
+#include <utility>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -50,15 +52,15 @@ void scan_exclusive_gather(hipStream_t s, const uint32_t* in, const uint32_t* id
                            const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws);
 
 // ------------------------------------------------------------------ sort --
-// Stable LSD radix sort, 8-bit digits, of u32 keys or u64 keys stored as
+// Stable LSD onesweep radix sort, 8-bit digits, of u32 keys or u64 keys stored as
 // (low word, high word), each carrying a u32 value. Sorts key bits
 // [begin_bit, end_bit); the passes ping-pong between (keys, values) and the
 // alternate buffers. Returns true when the result ended in the alternate buffers.
 constexpr uint32_t kSortTile = 4096;  // 256 threads x 16 keys
 struct SortWorkspace {
-  uint32_t* counts = nullptr;      // 256 x tiles digit counts, digit-major
-  uint32_t* counts_len = nullptr;  // device: 256 x active tiles
-  ScanWorkspace scan;
+  uint32_t* hist = nullptr;                 // per pass: 256 digit counts, then run starts
+  uint32_t* state[2] = {nullptr, nullptr};  // per tile x digit look-back words, alternating passes
+  uint32_t* tickets = nullptr;              // per pass tile counter
   uint32_t tiles = 0;
 };
 size_t sort_workspace_bytes(uint32_t capacity);
@@ -108,6 +110,7 @@ void training_loss(hipStream_t s, uint32_t width, uint32_t height, const float* 
 // ----------------------------------------------------------- optimizer --
 struct AdamArgs {
   float bias_correction1, bias_correction2;
+  float inv_bias_correction1, inv_bias_correction2;
   float position_lr, opacity_lr, scale_lr, rotation_lr, color_lr, sh_lr;
   float viewport[2];
 };

@@ -15,7 +15,12 @@ constexpr int kThreads = 256;
 constexpr float kBeta1 = 0.9f, kBeta2 = 0.999f, kEpsilon = 1e-15f;
 constexpr float kResetLogit = -4.59511985013459f;  // logit(0.01)
 
-__device__ inline float sigmoid(float x) { return 1.f / (1.f + __expf(-x)); }
+// Hardware reciprocal and square root (about 1 ulp): Adam is a byte stream, and the
+// IEEE-exact division and root sequences would make it instruction-bound.
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ inline float sigmoid(float x) { return fast_rcp(1.f + __expf(-x)); }
 
 __device__ inline float activated(uint32_t slot, float raw) {
   if (slot == 3) return sigmoid(raw);
@@ -24,50 +29,90 @@ __device__ inline float activated(uint32_t slot, float raw) {
   return raw;
 }
 
+// Adam for 32 Gaussians per workgroup. Their parameters, SH coefficients and both
+// gradient arrays are contiguous runs (64 and 180 bytes per Gaussian), so the
+// workgroup moves them through LDS with 16-byte loads and stores; the 61 moment
+// slots per Gaussian are read and written straight from global memory, one slot
+// per lane (512 contiguous bytes per wave). Per slot the update is
+// adam.wgsl:51-57 with the chain rule of adam.wgsl:85-105.
+constexpr uint32_t kAdamGaussians = 32;
+
+__device__ inline void copy_in(float* __restrict__ dst, const float* __restrict__ src, uint32_t count) {
+  for (uint32_t q = threadIdx.x; q < count / 4; q += kThreads)
+    reinterpret_cast<float4*>(dst)[q] = reinterpret_cast<const float4*>(src)[q];
+  for (uint32_t k = count / 4 * 4 + threadIdx.x; k < count; k += kThreads) dst[k] = src[k];
+}
+
 __global__ __launch_bounds__(kThreads) void adam_kernel(uint32_t n, float* __restrict__ params, float* __restrict__ sh,
                                                          float2* __restrict__ moments,
                                                          const float* __restrict__ gaussian_grads,
                                                          const float* __restrict__ sh_grads, AdamArgs a,
                                                          float* __restrict__ gaussians) {
-  // 32-bit slot index: the division by 61 stays a multiply-high (n < 70M).
-  const uint32_t f = blockIdx.x * kThreads + threadIdx.x;
-  if (f >= n * kMomentSlots) return;
-  const uint32_t i = f / kMomentSlots;
-  const uint32_t slot = f - i * kMomentSlots;
-  if (slot == 7 || slot == 15) return;  // padding: kind and birth stamps, never trained
-  float* value;
-  float grad, lr;
-  if (slot < kFloatsPerGaussian) {
-    value = params + i * kFloatsPerGaussian + slot;
-    grad = gaussian_grads[i * kFloatsPerGaussian + slot];
-    if (slot < 3) {
-      lr = a.position_lr;
-    } else if (slot == 3) {
-      const float o = sigmoid(*value);
-      grad *= o * (1.f - o);
-      lr = a.opacity_lr;
-    } else if (slot < 7) {
-      grad *= __expf(*value);
-      lr = a.scale_lr;
-    } else if (slot < 12) {
-      lr = a.rotation_lr;
-    } else {
-      grad *= kShC0;
-      lr = a.color_lr;
+  __shared__ __align__(16) float s_p[kAdamGaussians * kFloatsPerGaussian];
+  __shared__ __align__(16) float s_gp[kAdamGaussians * kFloatsPerGaussian];
+  __shared__ __align__(16) float s_sh[kAdamGaussians * kShRest];
+  __shared__ __align__(16) float s_gsh[kAdamGaussians * kShRest];
+  __shared__ __align__(16) float s_act[kAdamGaussians * kFloatsPerGaussian];
+  const uint32_t first = blockIdx.x * kAdamGaussians;
+  const uint32_t count = min(kAdamGaussians, n - first);
+  copy_in(s_p, params + first * kFloatsPerGaussian, count * kFloatsPerGaussian);
+  copy_in(s_gp, gaussian_grads + first * kFloatsPerGaussian, count * kFloatsPerGaussian);
+  copy_in(s_sh, sh + first * kShRest, count * kShRest);
+  copy_in(s_gsh, sh_grads + first * kShRest, count * kShRest);
+  __syncthreads();
+  float2* m_base = moments + first * kMomentSlots;
+  for (uint32_t f = threadIdx.x; f < count * kMomentSlots; f += kThreads) {
+    const uint32_t i = f / kMomentSlots;
+    const uint32_t slot = f - i * kMomentSlots;
+    if (slot == 7 || slot == 15) {  // padding: kind and birth stamps, never trained
+      if (slot < kFloatsPerGaussian) s_act[i * kFloatsPerGaussian + slot] = 0.f;
+      continue;
     }
-  } else {
-    const uint32_t k = i * kShRest + (slot - kFloatsPerGaussian);
-    value = sh + k;
-    grad = sh_grads[k];
-    lr = a.sh_lr;
+    float* value;
+    float grad, lr;
+    if (slot < kFloatsPerGaussian) {
+      value = s_p + i * kFloatsPerGaussian + slot;
+      grad = s_gp[i * kFloatsPerGaussian + slot];
+      if (slot < 3) {
+        lr = a.position_lr;
+      } else if (slot == 3) {
+        const float o = sigmoid(*value);
+        grad *= o * (1.f - o);
+        lr = a.opacity_lr;
+      } else if (slot < 7) {
+        grad *= __expf(*value);
+        lr = a.scale_lr;
+      } else if (slot < 12) {
+        lr = a.rotation_lr;
+      } else {
+        grad *= kShC0;
+        lr = a.color_lr;
+      }
+    } else {
+      value = s_sh + i * kShRest + (slot - kFloatsPerGaussian);
+      grad = s_gsh[i * kShRest + (slot - kFloatsPerGaussian)];
+      lr = a.sh_lr;
+    }
+    float2 m = m_base[f];
+    m.x = kBeta1 * m.x + (1.f - kBeta1) * grad;
+    m.y = kBeta2 * m.y + (1.f - kBeta2) * grad * grad;
+    m_base[f] = m;
+    // lr (m / bc1) / (sqrt(v / bc2) + eps) with the two corrections folded on the host.
+    const float updated =
+        *value - lr * a.inv_bias_correction1 * m.x * fast_rcp(fast_sqrt(m.y * a.inv_bias_correction2) + kEpsilon);
+    *value = updated;
+    if (slot < kFloatsPerGaussian) s_act[i * kFloatsPerGaussian + slot] = activated(slot, updated);
   }
-  float2 m = moments[f];
-  m.x = kBeta1 * m.x + (1.f - kBeta1) * grad;
-  m.y = kBeta2 * m.y + (1.f - kBeta2) * grad * grad;
-  moments[f] = m;
-  const float updated = *value - lr * (m.x / a.bias_correction1) / (sqrtf(m.y / a.bias_correction2) + kEpsilon);
-  *value = updated;
-  if (gaussians && slot < kFloatsPerGaussian) gaussians[i * kFloatsPerGaussian + slot] = activated(slot, updated);
+  __syncthreads();
+  copy_in(params + first * kFloatsPerGaussian, s_p, count * kFloatsPerGaussian);
+  copy_in(sh + first * kShRest, s_sh, count * kShRest);
+  if (gaussians) {
+    // The pads (kind, birth) of the activated copy are left as they are.
+    for (uint32_t e = threadIdx.x; e < count * kFloatsPerGaussian; e += kThreads) {
+      const uint32_t slot = e % kFloatsPerGaussian;
+      if (slot != 7 && slot != 15) gaussians[first * kFloatsPerGaussian + e] = s_act[e];
+    }
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void activate_kernel(uint32_t n, const float* __restrict__ params,
@@ -218,8 +263,9 @@ void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* momen
   if (n == 0) return;
   if (static_cast<uint64_t>(n) * kMomentSlots >= (1ull << 32)) throw Error(-2, "adam: too many Gaussians");
   const size_t slots = static_cast<size_t>(n) * kMomentSlots;
-  adam_kernel<<<ceil_div(slots, kThreads), kThreads, 0, s>>>(n, params, sh, reinterpret_cast<float2*>(moments),
-                                                             gaussian_grads, sh_grads, a, gaussians);
+  (void)slots;
+  adam_kernel<<<ceil_div(n, kAdamGaussians), kThreads, 0, s>>>(n, params, sh, reinterpret_cast<float2*>(moments),
+                                                                gaussian_grads, sh_grads, a, gaussians);
   DIO_LAUNCHED("adam");
 }
 

@@ -15,31 +15,47 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Copies `count` floats between global memory and LDS with the whole workgroup,
+// 16 bytes per lane per step (both sides 16-byte aligned): the 45-float SH records
+// are 180 bytes apart, so per-thread access would touch a new cache line per lane.
+__device__ inline void block_copy(float* __restrict__ dst, const float* __restrict__ src, uint32_t count) {
+  const uint32_t quads = count / 4;
+  for (uint32_t q = threadIdx.x; q < quads; q += kThreads)
+    reinterpret_cast<float4*>(dst)[q] = reinterpret_cast<const float4*>(src)[q];
+  for (uint32_t k = quads * 4 + threadIdx.x; k < count; k += kThreads) dst[k] = src[k];
+}
+
 // ------------------------------------------------------------ preprocess --
 
 // preprocess.wgsl:20-87 plus count_tiles (tiles.wgsl:23-34) fused: one thread per
-// Gaussian writes its splat, its tile count and its depth-sort key.
+// Gaussian writes its splat, its tile count and its depth-sort key. A workgroup
+// stages its 256 Gaussians' SH records in LDS when any of them is visible.
 __global__ __launch_bounds__(kThreads) void preprocess_kernel(const float4* __restrict__ gaussians,
                                                                const float* __restrict__ sh, uint32_t sh_degree,
                                                                uint32_t n, CameraArgs cam, float4* __restrict__ splats,
                                                                uint32_t* __restrict__ tile_counts,
                                                                uint32_t* __restrict__ depth_keys,
                                                                uint32_t* __restrict__ depth_values) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
+  __shared__ __align__(16) float s_sh[kThreads * kShRest];
+  const uint32_t first = blockIdx.x * kThreads, i = first + threadIdx.x;
+  const bool active = i < n;
   float g[16];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    float4 v = gaussians[i * 4 + q];
+    float4 v = active ? gaussians[i * 4 + q] : make_float4(0.f, 0.f, 0.f, 1.f);
     g[4 * q] = v.x; g[4 * q + 1] = v.y; g[4 * q + 2] = v.z; g[4 * q + 3] = v.w;
   }
   Projection p;
-  const bool visible = project(g, cam, p);
+  const bool visible = active && project(g, cam, p);
+  const int count = sh ? sh_count(sh_degree) : 0;
+  if (count > 0 && __syncthreads_or(visible))
+    block_copy(s_sh, sh + static_cast<size_t>(first) * kShRest, min(static_cast<uint32_t>(kThreads), n - first) * kShRest);
+  __syncthreads();
+  if (!active) return;
   float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = s0;
   uint32_t touched = 0, key = 0xffffffffu;
   if (visible) {
     float color[3] = {g[12], g[13], g[14]};
-    const int count = sh ? sh_count(sh_degree) : 0;
     if (count > 0) {
       float cp[3];
       camera_position(cam, cp);
@@ -47,7 +63,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_kernel(const float4* __re
       float l = sqrtf(ox * ox + oy * oy + oz * oz);
       float b[15];
       sh_basis(ox / l, oy / l, oz / l, b);
-      const float* c = sh + static_cast<size_t>(i) * kShRest;
+      const float* c = s_sh + threadIdx.x * kShRest;
       for (int k = 0; k < count; k++) {
         color[0] += b[k] * c[3 * k];
         color[1] += b[k] * c[3 * k + 1];
@@ -315,21 +331,32 @@ __global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict
 
 // ------------------------------------------------------ rasterize backward --
 
-template <int kCtrl, int kRowMask = 0xf>
+template <int kCtrl>
 __device__ inline float dpp_add(float v) {
-  return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, kRowMask,
-                                                                   0xf, false));
+  return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, 0xf, 0xf,
+                                                                   false));
 }
 
-// Sum over the 64 lanes with DPP row operations; the total lands in lane 63.
-__device__ inline float wave_total(float v) {
-  v = dpp_add<0xb1>(v);         // quad_perm [1,0,3,2]
-  v = dpp_add<0x4e>(v);         // quad_perm [2,3,0,1]: quad sums
-  v = dpp_add<0x141>(v);        // row_half_mirror: 8-lane sums
-  v = dpp_add<0x140>(v);        // row_mirror: 16-lane row sums
-  v = dpp_add<0x142, 0xa>(v);   // row_bcast:15 into rows 1, 3
-  v = dpp_add<0x143, 0xc>(v);   // row_bcast:31 into rows 2, 3
-  return lane_value(v, kWave - 1);
+// Sum over each 16-lane row with DPP; every lane of the row gets the row total.
+__device__ inline float row_total(float v) {
+  v = dpp_add<0xb1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4e>(v);   // quad_perm [2,3,0,1]: quad sums
+  v = dpp_add<0x141>(v);  // row_half_mirror: 8-lane sums
+  v = dpp_add<0x140>(v);  // row_mirror: row sums
+  return v;
+}
+
+// Four splats' per-lane values (a, b, c, d) to four wave totals at once: a
+// butterfly across lane halves keeps a|c, b|d by half, one across 16-lane rows
+// keeps a|b|c|d by row, so one row reduction finishes all four: row r of the
+// wave ends with splat r's total.
+__device__ inline float four_totals(float a, float b, float c, float d) {
+  const int lane = __lane_id();
+  const bool lower = lane < 32;
+  const float lo = (lower ? a : c) + __shfl_xor(lower ? c : a, 32, kWave);
+  const float hi = (lower ? b : d) + __shfl_xor(lower ? d : b, 32, kWave);
+  const bool even_row = (lane & 16) == 0;
+  return row_total((even_row ? lo : hi) + __shfl_xor(even_row ? hi : lo, 16, kWave));
 }
 
 __device__ inline uint32_t wave_max(uint32_t v) {
@@ -338,11 +365,53 @@ __device__ inline uint32_t wave_max(uint32_t v) {
   return v;
 }
 
+// The per-pixel replay state of the backward pass (rasterizeBackward.wgsl:161-177).
+struct Replay {
+  float T, final_T, bg_grad, gr, gg, gb;
+  float behind_r, behind_g, behind_b;
+  float next_alpha, next_r, next_g, next_b;
+};
+
+// One splat's 9 gradient shares at one pixel (rasterizeBackward.wgsl:199-230).
+__device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy, float mx, float my, float ca,
+                                    float cbb, float cc, float op, float sr, float sg, float sb, float share[kGrads]) {
+#pragma unroll
+  for (int v = 0; v < kGrads; v++) share[v] = 0.f;
+  if (!blended) return false;
+  const float dx = cx - mx, dy = cy - my;
+  const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
+  if (power > 0.f) return false;
+  const float G = __expf(power);
+  const float alpha = fminf(0.99f, op * G);
+  if (alpha < 1.f / 255.f) return false;
+  const float keep = __builtin_amdgcn_rcpf(1.f - alpha);  // one hardware reciprocal for both divisions
+  st.T *= keep;
+  st.behind_r = st.next_alpha * st.next_r + (1.f - st.next_alpha) * st.behind_r;
+  st.behind_g = st.next_alpha * st.next_g + (1.f - st.next_alpha) * st.behind_g;
+  st.behind_b = st.next_alpha * st.next_b + (1.f - st.next_alpha) * st.behind_b;
+  st.next_alpha = alpha;
+  st.next_r = sr; st.next_g = sg; st.next_b = sb;
+  const float alpha_grad = st.T * ((sr - st.behind_r) * st.gr + (sg - st.behind_g) * st.gg + (sb - st.behind_b) * st.gb) -
+                           st.final_T * keep * st.bg_grad;
+  const float power_grad = alpha_grad * op * G;
+  share[0] = power_grad * (ca * dx + cbb * dy);
+  share[1] = power_grad * (cbb * dx + cc * dy);
+  share[2] = power_grad * (-0.5f * dx * dx);
+  share[3] = power_grad * (-dx * dy);
+  share[4] = power_grad * (-0.5f * dy * dy);
+  share[5] = alpha_grad * G;
+  const float at = alpha * st.T;
+  share[6] = at * st.gr;
+  share[7] = at * st.gg;
+  share[8] = at * st.gb;
+  return true;
+}
+
 // rasterizeBackward.wgsl:49-163: each pixel replays its list back to front from
-// its last blended splat. Per splat, a wave sums its pixels' 9 gradients with DPP
-// and parks the total in the splat's staging lane; per 64-splat chunk the four
-// quadrant waves add their totals in LDS and one float atomic per (splat, tile,
-// gradient) goes out.
+// its last blended splat. A wave takes the splats that touch its quadrant four at
+// a time, reduces their 9 gradients across its 64 pixels together, and its row
+// leaders park the totals in LDS; per 64-splat chunk the four quadrant waves add
+// theirs and one float atomic per (splat, tile, gradient) goes out.
 __global__ __launch_bounds__(256) void rasterize_backward_kernel(
     const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
     uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g, float bg_b,
@@ -360,23 +429,23 @@ __global__ __launch_bounds__(256) void rasterize_backward_kernel(
   const float bx0 = static_cast<float>(qx) + 0.5f, bx1 = bx0 + 7.f;
   const float by0 = static_cast<float>(qy) + 0.5f, by1 = by0 + 7.f;
   const uint2 run = ranges[tile];
-  float final_T = 0.f, gr = 0.f, gg = 0.f, gb = 0.f;
+  Replay st{};
   uint32_t last = 0;
   if (inside) {
     const uint32_t index = py * width + px;
-    final_T = pixel_color[index].w;
+    st.final_T = pixel_color[index].w;
     last = pixel_last[index];
     const float4 pg = pixel_grads[index];
-    gr = pg.x; gg = pg.y; gb = pg.z;
+    st.gr = pg.x; st.gg = pg.y; st.gb = pg.z;
   }
+  st.T = st.final_T;
+  st.bg_grad = bg_r * st.gr + bg_g * st.gg + bg_b * st.gb;
   const uint32_t my_last = wave_max(last);
   if (lane == 0) s_last[wave] = my_last;
   __syncthreads();
   const uint32_t tile_last = max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
-  const float bg_grad = bg_r * gr + bg_g * gg + bg_b * gb;
-  float T = final_T;
-  float behind_r = 0.f, behind_g = 0.f, behind_b = 0.f;
-  float next_alpha = 0.f, next_r = 0.f, next_g = 0.f, next_b = 0.f;
+  const int row = lane >> 4;
+  const bool row_leader = (lane & 15) == 0;
 
   for (int c = static_cast<int>(ceil_div(tile_last, kChunk)) - 1; c >= 0; c--) {
     const uint32_t base = run.x + static_cast<uint32_t>(c) * kChunk;
@@ -388,62 +457,39 @@ __global__ __launch_bounds__(256) void rasterize_backward_kernel(
       hit = base + lane - run.x < my_last && meets(s, bx0, bx1, by0, by1);
     }
     if (wave == 0) s_index[lane] = s.index;
-    float res[kGrads];
 #pragma unroll
-    for (int v = 0; v < kGrads; v++) res[v] = 0.f;
+    for (int v = 0; v < kGrads; v++) s_sums[wave][v][lane] = 0.f;
     uint64_t mask = __ballot(hit);
     while (mask) {
-      const int j = kWave - 1 - __builtin_clzll(mask);
-      mask &= ~(1ull << j);
-      const float mx = lane_value(s.mx, j), my = lane_value(s.my, j);
-      const float ca = lane_value(s.ca, j), cbb = lane_value(s.cb, j), cc = lane_value(s.cc, j);
-      const float op = lane_value(s.op, j);
-      const float sr = lane_value(s.r, j), sg = lane_value(s.g, j), sb = lane_value(s.b, j);
-      float share[kGrads];
+      // Up to four splats, back to front; missing ones contribute zeros.
+      float share[4][kGrads];
+      int js[4];
+      bool any = false;
 #pragma unroll
-      for (int v = 0; v < kGrads; v++) share[v] = 0.f;
-      bool contributes = false;
-      if (base + j - run.x < last) {
-        const float dx = cx - mx, dy = cy - my;
-        const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
-        if (power <= 0.f) {
-          const float G = __expf(power);
-          const float alpha = fminf(0.99f, op * G);
-          if (alpha >= 1.f / 255.f) {
-            contributes = true;
-            T /= 1.f - alpha;
-            behind_r = next_alpha * next_r + (1.f - next_alpha) * behind_r;
-            behind_g = next_alpha * next_g + (1.f - next_alpha) * behind_g;
-            behind_b = next_alpha * next_b + (1.f - next_alpha) * behind_b;
-            next_alpha = alpha;
-            next_r = sr; next_g = sg; next_b = sb;
-            const float alpha_grad = T * ((sr - behind_r) * gr + (sg - behind_g) * gg + (sb - behind_b) * gb) -
-                                     final_T / (1.f - alpha) * bg_grad;
-            const float power_grad = alpha_grad * op * G;
-            share[0] = power_grad * (ca * dx + cbb * dy);
-            share[1] = power_grad * (cbb * dx + cc * dy);
-            share[2] = power_grad * (-0.5f * dx * dx);
-            share[3] = power_grad * (-dx * dy);
-            share[4] = power_grad * (-0.5f * dy * dy);
-            share[5] = alpha_grad * G;
-            const float at = alpha * T;
-            share[6] = at * gr;
-            share[7] = at * gg;
-            share[8] = at * gb;
-          }
+      for (int g = 0; g < 4; g++) {
+        js[g] = -1;
+        bool blended = false;
+        float mx = 0.f, my = 0.f, ca = 0.f, cbb = 0.f, cc = 0.f, op = 0.f, sr = 0.f, sg = 0.f, sb = 0.f;
+        if (mask) {
+          const int j = kWave - 1 - __builtin_clzll(mask);
+          mask &= ~(1ull << j);
+          js[g] = j;
+          mx = lane_value(s.mx, j); my = lane_value(s.my, j);
+          ca = lane_value(s.ca, j); cbb = lane_value(s.cb, j); cc = lane_value(s.cc, j);
+          op = lane_value(s.op, j);
+          sr = lane_value(s.r, j); sg = lane_value(s.g, j); sb = lane_value(s.b, j);
+          blended = base + j - run.x < last;
         }
+        any |= replay_splat(st, blended, cx, cy, mx, my, ca, cbb, cc, op, sr, sg, sb, share[g]);
       }
-      if (__any(contributes)) {
-        const bool mine = lane == j;  // the splat's staging lane keeps the wave's total
+      if (!__any(any)) continue;
+      const int mine = row == 0 ? js[0] : (row == 1 ? js[1] : (row == 2 ? js[2] : js[3]));
 #pragma unroll
-        for (int v = 0; v < kGrads; v++) {
-          const float total = wave_total(share[v]);
-          res[v] = mine ? total : res[v];
-        }
+      for (int v = 0; v < kGrads; v++) {
+        const float total = four_totals(share[0][v], share[1][v], share[2][v], share[3][v]);
+        if (row_leader && mine >= 0) s_sums[wave][v][mine] = total;
       }
     }
-#pragma unroll
-    for (int v = 0; v < kGrads; v++) s_sums[wave][v][lane] = res[v];
     __syncthreads();
     for (int e = tid; e < kGrads * kChunk; e += 256) {
       const int v = e / kChunk, slot = e % kChunk;
@@ -462,16 +508,24 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
     const float4* __restrict__ gaussians, const float* __restrict__ sh, uint32_t sh_degree, uint32_t n,
     CameraArgs cam, const float4* __restrict__ splats, const float* __restrict__ splat_grads,
     float4* __restrict__ gaussian_grads, float* __restrict__ sh_grads) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  float* sg = sh_grads + static_cast<size_t>(i) * kShRest;
-  const float radius = splats[i * 3].w;
-  if (radius == 0.f) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) gaussian_grads[i * 4 + q] = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int k = 0; k < static_cast<int>(kShRest); k++) sg[k] = 0.f;
-    return;
+  // SH coefficients in, then their gradients out, staged through the same LDS block.
+  __shared__ __align__(16) float s_sh[kThreads * kShRest];
+  __shared__ __align__(16) float s_grad[kThreads * kGrads];
+  const uint32_t first = blockIdx.x * kThreads, i = first + threadIdx.x;
+  const uint32_t block_n = min(static_cast<uint32_t>(kThreads), n - first);
+  const bool active = i < n;
+  const bool visible = active && splats[i * 3].w != 0.f;
+  const int count = sh_count(sh_degree);
+  const bool any = __syncthreads_or(visible);
+  if (any) {
+    if (count > 0) block_copy(s_sh, sh + static_cast<size_t>(first) * kShRest, block_n * kShRest);
+    block_copy(s_grad, splat_grads + static_cast<size_t>(first) * kGrads, block_n * kGrads);
   }
+  __syncthreads();
+  // A thread only touches its own SH row, so gradients overwrite coefficients in place.
+  float* row = s_sh + threadIdx.x * kShRest;
+  float4 out0 = make_float4(0.f, 0.f, 0.f, 0.f), out1 = out0, out2 = out0, out3 = out0;
+  if (visible) {
   float g[16];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -480,7 +534,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
   }
   float gr[kGrads];
 #pragma unroll
-  for (int v = 0; v < kGrads; v++) gr[v] = splat_grads[static_cast<size_t>(i) * kGrads + v];
+  for (int v = 0; v < kGrads; v++) gr[v] = s_grad[threadIdx.x * kGrads + v];
   Projection p;
   project(g, cam, p);
   const float d = p.depth, d2 = d * d, d3 = d2 * d;
@@ -534,7 +588,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
   const int count = sh_count(sh_degree);
   float basis[15];
   sh_basis(dx, dy, dz, basis);
-  const float* coeff = sh + static_cast<size_t>(i) * kShRest;
+  float* coeff = row;
   float raw[3] = {g[12], g[13], g[14]};
   for (int k = 0; k < count; k++) {
     raw[0] += basis[k] * coeff[3 * k];
@@ -547,17 +601,15 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
   float dgx = 0.f, dgy = 0.f, dgz = 0.f;
   for (int k = 0; k < static_cast<int>(kShRest) / 3; k++) {
     if (k < count) {
-      sg[3 * k] = basis[k] * raw_grad[0];
-      sg[3 * k + 1] = basis[k] * raw_grad[1];
-      sg[3 * k + 2] = basis[k] * raw_grad[2];
       const float dot = coeff[3 * k] * raw_grad[0] + coeff[3 * k + 1] * raw_grad[1] + coeff[3 * k + 2] * raw_grad[2];
       dgx += bgx[k] * dot;
       dgy += bgy[k] * dot;
       dgz += bgz[k] * dot;
+      coeff[3 * k] = basis[k] * raw_grad[0];
+      coeff[3 * k + 1] = basis[k] * raw_grad[1];
+      coeff[3 * k + 2] = basis[k] * raw_grad[2];
     } else {
-      sg[3 * k] = 0.f;
-      sg[3 * k + 1] = 0.f;
-      sg[3 * k + 2] = 0.f;
+      coeff[3 * k] = coeff[3 * k + 1] = coeff[3 * k + 2] = 0.f;
     }
   }
   const float dd = dx * dgx + dy * dgy + dz * dgz;
@@ -588,11 +640,22 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
                  qy * rg[1][2] + qx * rg[2][0] + qy * rg[2][1]);
   const float qdot = qw * qg[0] + qx * qg[1] + qy * qg[2] + qz * qg[3];
   const float inv_l = 1.f / p.q_length;
-  gaussian_grads[i * 4] = make_float4(pos_grad[0], pos_grad[1], pos_grad[2], gr[5]);
-  gaussian_grads[i * 4 + 1] = make_float4(scale_grad[0], scale_grad[1], scale_grad[2], 0.f);
-  gaussian_grads[i * 4 + 2] = make_float4((qg[0] - qw * qdot) * inv_l, (qg[1] - qx * qdot) * inv_l,
-                                          (qg[2] - qy * qdot) * inv_l, (qg[3] - qz * qdot) * inv_l);
-  gaussian_grads[i * 4 + 3] = make_float4(raw_grad[0], raw_grad[1], raw_grad[2], 0.f);
+  out0 = make_float4(pos_grad[0], pos_grad[1], pos_grad[2], gr[5]);
+  out1 = make_float4(scale_grad[0], scale_grad[1], scale_grad[2], 0.f);
+  out2 = make_float4((qg[0] - qw * qdot) * inv_l, (qg[1] - qx * qdot) * inv_l, (qg[2] - qy * qdot) * inv_l,
+                     (qg[3] - qz * qdot) * inv_l);
+  out3 = make_float4(raw_grad[0], raw_grad[1], raw_grad[2], 0.f);
+  } else if (active) {
+    for (int k = 0; k < static_cast<int>(kShRest); k++) row[k] = 0.f;
+  }
+  if (active) {
+    gaussian_grads[i * 4] = out0;
+    gaussian_grads[i * 4 + 1] = out1;
+    gaussian_grads[i * 4 + 2] = out2;
+    gaussian_grads[i * 4 + 3] = out3;
+  }
+  __syncthreads();
+  block_copy(sh_grads + static_cast<size_t>(first) * kShRest, s_sh, block_n * kShRest);
 }
 
 }  // namespace

@@ -1,14 +1,15 @@
 // Stable LSD radix sort with 8-bit digits over u32 keys, or u64 keys stored as
-// (low word, high word) pairs, each key carrying a u32 value. The drop-in for
-// src/radixSort.ts:27-142 / src/shaders/radixSort.wgsl:44-120: the same passes
-// (digit-major block counts, an exclusive scan of them, a stable scatter), built
-// for wave64:
-// * a 4096-key tile per workgroup (256 threads x 16 keys, coalesced stripes);
-// * stable ranks from wavefront "match" masks (8 ballots per key) plus a
-//   per-digit prefix across the 4 waves, instead of 256-bit LDS masks;
-// * the tile is reordered in LDS first, so each digit's run is written out as one
-//   contiguous stretch of global memory;
-// * the count scan is the one-pass decoupled look-back scan.
+// (low word, high word) pairs, each key carrying a u32 value: the drop-in for
+// src/radixSort.ts:27-142 / src/shaders/radixSort.wgsl:44-120 (same result: a
+// stable sort by key bits [begin, end)), built as a onesweep sort for wave64:
+// * one histogram kernel counts every pass's digits at once (one read of the keys);
+// * then one kernel per pass: a 4096-key tile per workgroup (256 threads x 16
+//   keys, coalesced stripes) ranks its keys stably with wavefront "match" masks
+//   (8 ballots per key) plus a per-digit prefix across the 4 waves, learns where
+//   its run of each digit starts by per-digit decoupled look-back over earlier
+//   tiles, and scatters through LDS so each digit's run is written contiguously.
+// Tiles are handed out by an atomic ticket, so a tile only waits on tiles that
+// started before it. Per pass the keys are read and written once.
 #include "dio_internal.hpp"
 
 namespace dio {
@@ -17,6 +18,10 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kItems = kSortTile / kThreads;  // 16
 constexpr int kWaves = kThreads / kWave;
+constexpr uint32_t kMaxPasses = 8;
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1;
 
 template <bool kWide>
 struct KeyOps;
@@ -41,60 +46,110 @@ __device__ inline uint32_t key_count(const uint32_t* count, uint32_t capacity) {
   return count ? min(*count, capacity) : capacity;
 }
 
+// Counts the digits of every pass in one read of the keys (LDS histograms, one
+// per wave, added into the global ones), and clears the first pass's look-back
+// states and every pass ticket.
 template <bool kWide>
 __global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyOps<kWide>::T* __restrict__ keys,
                                                               const uint32_t* count, uint32_t capacity,
-                                                              uint32_t shift, uint32_t* __restrict__ counts,
-                                                              uint32_t* counts_len) {
-  __shared__ uint32_t hist[kWaves][256];
-  const int tid = threadIdx.x, wave = tid / kWave;
+                                                              uint32_t begin_bit, uint32_t passes,
+                                                              uint32_t* __restrict__ hist, uint32_t* __restrict__ state0,
+                                                              uint32_t* __restrict__ tickets) {
+  __shared__ uint32_t local[kMaxPasses][256];
+  const int tid = threadIdx.x;
   const uint32_t n = key_count(count, capacity);
   const uint32_t tiles = ceil_div(n, kSortTile);
-  if (blockIdx.x == 0 && tid == 0) *counts_len = 256u * tiles;
-  if (blockIdx.x >= tiles) return;
-#pragma unroll
-  for (int w = 0; w < kWaves; w++) hist[w][tid] = 0;
+  const uint32_t stride = gridDim.x * kThreads;
+  for (uint32_t k = blockIdx.x * kThreads + tid; k < tiles * 256u; k += stride) state0[k] = 0;
+  if (blockIdx.x == 0 && tid < static_cast<int>(kMaxPasses)) tickets[tid] = 0;
+  for (uint32_t p = 0; p < passes; p++) local[p][tid] = 0;
   __syncthreads();
-  const uint32_t base = blockIdx.x * kSortTile;
+  // Lanes holding the same digit first agree on one count (match masks), so a few
+  // hot digits (a float's exponent byte) do not serialise the LDS atomics.
+  const int lane = tid & (kWave - 1);
+  const uint64_t lower_lanes = (1ull << lane) - 1;
+  const uint32_t rounds = ceil_div(n, stride);
+  for (uint32_t r = 0; r < rounds; r++) {
+    const uint32_t i = r * stride + blockIdx.x * kThreads + tid;
+    const bool valid = i < n;
+    const auto key = valid ? keys[i] : KeyOps<kWide>::invalid();
+    for (uint32_t p = 0; p < passes; p++) {
+      const uint32_t d = KeyOps<kWide>::digit(key, begin_bit + 8 * p);
+      uint64_t peers = __ballot(valid);
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    uint32_t i = base + k * kThreads + tid;
-    if (i < n) atomicAdd(&hist[wave][KeyOps<kWide>::digit(keys[i], shift)], 1u);
+      for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t set = __ballot(bit);
+        peers &= bit ? set : ~set;
+      }
+      if (valid && (peers & lower_lanes) == 0) atomicAdd(&local[p][d], static_cast<uint32_t>(__popcll(peers)));
+    }
   }
   __syncthreads();
-  uint32_t total = 0;
+  for (uint32_t p = 0; p < passes; p++)
+    if (local[p][tid]) atomicAdd(&hist[p * 256 + tid], local[p][tid]);
+}
+
+// Turns each pass's digit counts into where that digit's run starts.
+__global__ __launch_bounds__(256) void digit_offsets_kernel(uint32_t* hist) {
+  __shared__ uint32_t wave_totals[kWaves];
+  uint32_t* h = hist + blockIdx.x * 256;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const uint32_t c = h[tid];
+  uint32_t x = c;
 #pragma unroll
-  for (int w = 0; w < kWaves; w++) total += hist[w][tid];
-  counts[tid * tiles + blockIdx.x] = total;
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) wave_totals[wave] = x;
+  __syncthreads();
+  uint32_t before = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) before += w < wave ? wave_totals[w] : 0u;
+  h[tid] = before + x - c;
+}
+
+__device__ inline void publish(uint32_t* state, uint32_t word) {
+  __hip_atomic_store(state, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint32_t peek(uint32_t* state) {
+  return __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <bool kWide>
-__global__ __launch_bounds__(kThreads) void scatter_kernel(const typename KeyOps<kWide>::T* __restrict__ keys_in,
-                                                            const uint32_t* __restrict__ values_in,
-                                                            typename KeyOps<kWide>::T* __restrict__ keys_out,
-                                                            uint32_t* __restrict__ values_out, const uint32_t* count,
-                                                            uint32_t capacity, uint32_t shift,
-                                                            const uint32_t* __restrict__ counts) {
+__global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOps<kWide>::T* __restrict__ keys_in,
+                                                             const uint32_t* __restrict__ values_in,
+                                                             typename KeyOps<kWide>::T* __restrict__ keys_out,
+                                                             uint32_t* __restrict__ values_out, const uint32_t* count,
+                                                             uint32_t capacity, uint32_t shift,
+                                                             const uint32_t* __restrict__ digit_start, uint32_t* state,
+                                                             uint32_t* __restrict__ next_state, uint32_t* ticket) {
   using Ops = KeyOps<kWide>;
   using Key = typename Ops::T;
   __shared__ uint32_t wave_count[kWaves][256];
   __shared__ uint32_t wave_offset[kWaves][256];
   __shared__ uint32_t running[256];
   __shared__ uint32_t local_start[256];
-  __shared__ uint32_t global_start[256];
+  __shared__ uint32_t run_start[256];
   __shared__ uint32_t wave_totals[kWaves];
+  __shared__ uint32_t s_tile;
   __shared__ Key staged_keys[kSortTile];
   __shared__ uint32_t staged_values[kSortTile];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t n = key_count(count, capacity);
-  const uint32_t tiles = ceil_div(n, kSortTile);
-  if (blockIdx.x >= tiles) return;
-  const uint32_t base = blockIdx.x * kSortTile;
-  global_start[tid] = counts[tid * tiles + blockIdx.x];
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   running[tid] = 0;
 #pragma unroll
   for (int w = 0; w < kWaves; w++) wave_count[w][tid] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t n = key_count(count, capacity);
+  const uint32_t tiles = ceil_div(n, kSortTile);
+  if (tile >= tiles) return;
+  if (next_state) next_state[tile * 256 + tid] = 0;  // the next pass's look-back states
+  const uint32_t base = tile * kSortTile;
   const uint64_t lower_lanes = (1ull << lane) - 1;
 
   Key key[kItems];
@@ -102,13 +157,14 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const typename KeyOps
   uint32_t rank[kItems];
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
-    uint32_t i = base + k * kThreads + tid;
-    bool valid = i < n;
+    const uint32_t i = base + k * kThreads + tid;
+    const bool valid = i < n;
     key[k] = valid ? keys_in[i] : Ops::invalid();
     value[k] = valid ? values_in[i] : 0u;
   }
-  __syncthreads();
 
+  // Stable ranks: within a stripe by lane (match masks), across stripes and waves
+  // by running per-digit counts.
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
     const bool valid = base + k * kThreads + tid < n;
@@ -127,7 +183,7 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const typename KeyOps
       uint32_t r = running[tid];
 #pragma unroll
       for (int w = 0; w < kWaves; w++) {
-        uint32_t c = wave_count[w][tid];
+        const uint32_t c = wave_count[w][tid];
         wave_offset[w][tid] = r;
         r += c;
         wave_count[w][tid] = 0;
@@ -138,10 +194,41 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const typename KeyOps
     rank[k] = wave_offset[wave][d] + __popcll(before);
   }
 
-  // Where each digit's run starts inside the tile: an exclusive scan of the counts.
+  // This tile's count of digit `tid`: publish it, then add up earlier tiles' counts
+  // back to the first one that knows its inclusive prefix.
+  const uint32_t mine = running[tid];
+  uint32_t exclusive = 0;
+  if (tile == 0) {
+    publish(state + tid, kFlagPrefix | mine);
+  } else {
+    publish(state + tile * 256 + tid, kFlagAggregate | mine);
+    // Sixteen predecessors per step, loaded together: every tile runs at once at
+    // these sizes, so the published prefixes advance a window per round trip.
+    constexpr int kWindow = 16;
+    for (int32_t t0 = static_cast<int32_t>(tile) - 1;; t0 -= kWindow) {
+      uint32_t word[kWindow];
+#pragma unroll
+      for (int k = 0; k < kWindow; k++) word[k] = t0 - k >= 0 ? peek(state + (t0 - k) * 256 + tid) : kFlagPrefix;
+#pragma unroll
+      for (int k = 0; k < kWindow; k++)
+        while ((word[k] >> 30) == 0) word[k] = peek(state + (t0 - k) * 256 + tid);
+      bool found = false;
+#pragma unroll
+      for (int k = 0; k < kWindow; k++) {
+        if (!found) {
+          exclusive += word[k] & kValueMask;
+          found = (word[k] >> 30) == 2;
+        }
+      }
+      if (found) break;
+    }
+    publish(state + tile * 256 + tid, kFlagPrefix | (exclusive + mine));
+  }
+  run_start[tid] = digit_start[tid] + exclusive;
+
+  // Where each digit's run starts inside the tile.
   {
-    const uint32_t c = running[tid];
-    uint32_t x = c;
+    uint32_t x = mine;
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
       uint32_t y = __shfl_up(x, o, kWave);
@@ -152,7 +239,7 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const typename KeyOps
     uint32_t before_wave = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; w++) before_wave += w < wave ? wave_totals[w] : 0u;
-    local_start[tid] = before_wave + x - c;
+    local_start[tid] = before_wave + x - mine;
   }
   __syncthreads();
 #pragma unroll
@@ -171,7 +258,7 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const typename KeyOps
     if (p < tile_n) {
       const Key kk = staged_keys[p];
       const uint32_t d = Ops::digit(kk, shift);
-      const uint32_t dst = global_start[d] + p - local_start[d];
+      const uint32_t dst = run_start[d] + p - local_start[d];
       keys_out[dst] = kk;
       values_out[dst] = staged_values[p];
     }
@@ -181,20 +268,22 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const typename KeyOps
 }  // namespace
 
 size_t sort_workspace_bytes(uint32_t capacity) {
-  uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kSortTile);
-  size_t counts = static_cast<size_t>(256) * tiles * 4;
-  return counts + 256 + scan_workspace_bytes(256u * tiles);
+  const uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kSortTile);
+  // digit offsets for every pass, two look-back state arrays, pass tickets
+  return kMaxPasses * 256 * 4 + 2 * static_cast<size_t>(tiles) * 256 * 4 + 256;
 }
 
 SortWorkspace sort_workspace(void* base, uint32_t capacity) {
   SortWorkspace ws;
   ws.tiles = capacity == 0 ? 1 : ceil_div(capacity, kSortTile);
   char* p = static_cast<char*>(base);
-  ws.counts = reinterpret_cast<uint32_t*>(p);
-  p += static_cast<size_t>(256) * ws.tiles * 4;
-  ws.counts_len = reinterpret_cast<uint32_t*>(p);
-  p += 256;
-  ws.scan = scan_workspace(p, 256u * ws.tiles);
+  ws.hist = reinterpret_cast<uint32_t*>(p);
+  p += kMaxPasses * 256 * 4;
+  ws.state[0] = reinterpret_cast<uint32_t*>(p);
+  p += static_cast<size_t>(ws.tiles) * 256 * 4;
+  ws.state[1] = reinterpret_cast<uint32_t*>(p);
+  p += static_cast<size_t>(ws.tiles) * 256 * 4;
+  ws.tickets = reinterpret_cast<uint32_t*>(p);
   return ws;
 }
 
@@ -203,42 +292,44 @@ bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* al
                 const SortWorkspace& ws) {
   if (begin_bit % 8 != 0) throw Error(-2, "radix sort: begin bit must be a multiple of 8");
   if (end_bit > (wide ? 64u : 32u)) throw Error(-2, "radix sort: end bit past the key");
-  if (capacity == 0) return false;
+  if (capacity >= (1u << 30)) throw Error(-2, "radix sort: at most 2^30 keys");
+  if (capacity == 0 || end_bit <= begin_bit) return false;
   const uint32_t tiles = ceil_div(capacity, kSortTile);
   if (tiles > ws.tiles) throw Error(-2, "sort workspace too small");
+  const uint32_t passes = (end_bit - begin_bit + 7) / 8;
+  if (passes > kMaxPasses) throw Error(-2, "radix sort: too many passes");
+  fill_u32(s, ws.hist, 0u, passes * 256);
+  const uint32_t hist_blocks = std::min<uint32_t>(tiles, 1024);
+  if (wide)
+    histogram_kernel<true><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint2*>(keys), count, capacity, begin_bit,
+                                                             passes, ws.hist, ws.state[0], ws.tickets);
+  else
+    histogram_kernel<false><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint32_t*>(keys), count, capacity,
+                                                              begin_bit, passes, ws.hist, ws.state[0], ws.tickets);
+  DIO_LAUNCHED("radix histogram");
+  digit_offsets_kernel<<<passes, 256, 0, s>>>(ws.hist);
+  DIO_LAUNCHED("radix digit offsets");
   void* kin = keys;
   uint32_t* vin = values;
   void* kout = alt_keys;
   uint32_t* vout = alt_values;
-  bool flipped = false;
-  for (uint32_t shift = begin_bit; shift < end_bit; shift += 8) {
-    if (wide) {
-      histogram_kernel<true><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), count, capacity, shift,
-                                                         ws.counts, ws.counts_len);
-    } else {
-      histogram_kernel<false><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), count, capacity, shift,
-                                                          ws.counts, ws.counts_len);
-    }
-    DIO_LAUNCHED("radix histogram");
-    scan_exclusive(s, ws.counts, ws.counts, ws.counts_len, 256u * tiles, ws.counts_len + 1, ws.scan);
-    if (wide) {
-      scatter_kernel<true><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), vin, static_cast<uint2*>(kout),
-                                                       vout, count, capacity, shift, ws.counts);
-    } else {
-      scatter_kernel<false><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), vin,
-                                                        static_cast<uint32_t*>(kout), vout, count, capacity, shift,
-                                                        ws.counts);
-    }
-    DIO_LAUNCHED("radix scatter");
-    void* tk = kin;
-    kin = kout;
-    kout = tk;
-    uint32_t* tv = vin;
-    vin = vout;
-    vout = tv;
-    flipped = !flipped;
+  for (uint32_t p = 0; p < passes; p++) {
+    const uint32_t shift = begin_bit + 8 * p;
+    uint32_t* state = ws.state[p % 2];
+    uint32_t* next = p + 1 < passes ? ws.state[(p + 1) % 2] : nullptr;
+    if (wide)
+      onesweep_kernel<true><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), vin, static_cast<uint2*>(kout),
+                                                        vout, count, capacity, shift, ws.hist + p * 256, state, next,
+                                                        ws.tickets + p);
+    else
+      onesweep_kernel<false><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), vin,
+                                                         static_cast<uint32_t*>(kout), vout, count, capacity, shift,
+                                                         ws.hist + p * 256, state, next, ws.tickets + p);
+    DIO_LAUNCHED("radix onesweep");
+    std::swap(kin, kout);
+    std::swap(vin, vout);
   }
-  return flipped;
+  return passes % 2 == 1;
 }
 
 }  // namespace dio
