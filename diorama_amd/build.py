@@ -54,7 +54,8 @@ def build_addon():
     if not os.path.exists(src):
         return None
     out = os.path.join(HERE, "node", "diorama.node")
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-I/usr/include/node", "-I", os.path.join(ROOT, "include"),
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-DNODE_GYP_MODULE_NAME=diorama",
+           "-I/usr/include/node", "-I", os.path.join(ROOT, "include"),
            src, "-o", out + ".tmp", "-L", HERE, "-ldiorama_hip", "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

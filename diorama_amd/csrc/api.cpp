@@ -1086,7 +1086,7 @@ extern "C" void dio_default_settings(float extent, dio_training_settings* out) {
   s.scale_lr = 0.005f;
   s.rotation_lr = 0.001f;
   s.color_lr = 0.0025f;
-  s.sh_lr = 0.0025f / 20.f;
+  s.sh_lr = static_cast<float>(0.0025 / 20.0);  // formed in double, as src/trainer.ts:72
   s.sh_degree_interval = 1000;
   s.densify_from = 500;
   s.densify_until = 15000;

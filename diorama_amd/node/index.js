@@ -1,0 +1,396 @@
+'use strict';
+// The reference's operator classes for Node, over the N-API addon and the C ABI
+// (include/diorama.h). Method and getter names follow src/prefixSum.ts,
+// src/radixSort.ts, src/tileRasterizer.ts, src/gaussianRenderer.ts,
+// src/trainingLoss.ts and src/trainer.ts. Where the reference records into a
+// GPUCommandEncoder, these enqueue on the context's HIP stream in call order;
+// methods the reference makes async (afterSubmit, read, step, ...) return Promises
+// that resolve once the GPU work they wait for is done. Plain ES2019.
+
+const path = require('path');
+const native = require(path.join(__dirname, 'diorama.node'));
+
+const FLOATS_PER_GAUSSIAN = 16; // struct Gaussian (src/gaussians.ts:32)
+const FLOATS_PER_SPLAT = 12; // struct Splat2D
+const SH_REST_FLOATS = 45;
+const SH_C0 = 0.28209479177387814;
+
+/** A device allocation; `owned` ones are freed by destroy(). */
+class Buffer {
+  constructor(handle, owned, keep) {
+    this.handle = handle;
+    this.owned = owned;
+    this.keep = keep;
+  }
+  get size() {
+    return native.buf_size(this.handle);
+  }
+  /** queue.writeBuffer: copies a TypedArray or ArrayBuffer in at `offset` bytes. */
+  write(data, offset) {
+    native.buf_upload(this.handle, offset || 0, data);
+  }
+  /** Reads `bytes` (default: all) from `offset`; waits for the GPU. */
+  read(offset, bytes) {
+    const from = offset || 0;
+    return native.buf_download(this.handle, from, bytes === undefined ? this.size - from : bytes);
+  }
+  /** encoder.clearBuffer */
+  clear() {
+    native.buf_clear(this.handle);
+  }
+  destroy() {
+    if (this.owned && this.handle) native.buf_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+function view(handle, keep) {
+  return handle ? new Buffer(handle, false, keep) : undefined;
+}
+
+/** One device and its in-order stream: the counterpart of initGpu (src/gpu.ts:8-47). */
+class Device {
+  constructor(index) {
+    this.handle = native.ctx_create(index || 0);
+  }
+  createBuffer(bytes) {
+    return new Buffer(native.buf_create(this.handle, bytes), true);
+  }
+  upload(data) {
+    const b = this.createBuffer(data.byteLength);
+    b.write(data);
+    return b;
+  }
+  synchronize() {
+    native.ctx_synchronize(this.handle);
+  }
+  onSubmittedWorkDone() {
+    this.synchronize();
+    return Promise.resolve();
+  }
+  destroy() {
+    if (this.handle) native.ctx_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+function initGpu(index) {
+  return new Device(index);
+}
+
+/** src/prefixSum.ts:10-103 */
+class PrefixSum {
+  constructor(device, data, capacity, count) {
+    this.handle = native.prefix_sum_create(device.handle, data.handle, capacity, count ? count.handle : null);
+    this.keep = [data, count];
+  }
+  get total() {
+    return view(native.prefix_sum_total(this.handle), this);
+  }
+  encode() {
+    native.prefix_sum_encode(this.handle);
+  }
+  destroy() {
+    if (this.handle) native.prefix_sum_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/** src/radixSort.ts:27-142 */
+class RadixSort {
+  constructor(device, capacity, options) {
+    const opts = options || {};
+    this.handle = native.radix_sort_create(device.handle, capacity, opts.highBits || 0,
+      opts.count ? opts.count.handle : null);
+    this.keep = opts.count;
+  }
+  get capacity() {
+    return native.radix_sort_capacity(this.handle);
+  }
+  get keys() {
+    return view(native.radix_sort_keys(this.handle), this);
+  }
+  get values() {
+    return view(native.radix_sort_values(this.handle), this);
+  }
+  encode() {
+    native.radix_sort_encode(this.handle);
+  }
+  destroy() {
+    if (this.handle) native.radix_sort_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/** src/tileRasterizer.ts:29-317, for splats already on the device. */
+class TileRasterizer {
+  constructor(device, splats, count, initialPairCapacity, handle, keep) {
+    if (handle) {
+      this.handle = handle;
+      this.owned = false;
+      this.keep = keep;
+      return;
+    }
+    this.handle = native.tiles_create(device.handle, splats.handle, count, initialPairCapacity || 0);
+    this.owned = true;
+    this.keep = splats;
+  }
+  get lastPairCount() {
+    return native.tiles_last_pair_count(this.handle);
+  }
+  get pairCapacity() {
+    return native.tiles_pair_capacity(this.handle);
+  }
+  get debugBuffers() {
+    return { pairSplats: view(native.tiles_pair_splats(this.handle), this), ranges: view(native.tiles_ranges(this.handle), this) };
+  }
+  get pixelColor() {
+    return view(native.tiles_pixel_color(this.handle), this);
+  }
+  get pixelLast() {
+    return view(native.tiles_pixel_last(this.handle), this);
+  }
+  encode(target, viewport, background) {
+    native.tiles_encode(this.handle, target ? target.handle : null, viewport[0], viewport[1], background);
+  }
+  encodeBackward(viewport, pixelGrads, splatGrads) {
+    native.tiles_encode_backward(this.handle, viewport[0], viewport[1], pixelGrads.handle, splatGrads.handle);
+  }
+  afterSubmit() {
+    native.tiles_after_submit(this.handle);
+    return Promise.resolve();
+  }
+  destroy() {
+    if (this.owned && this.handle) native.tiles_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/** src/gaussianRenderer.ts:55-311 */
+class GaussianRenderer {
+  constructor(device, options, handle, keep) {
+    this.device = device;
+    if (handle) {
+      this.handle = handle;
+      this.owned = false;
+      this.keep = keep;
+      return;
+    }
+    const opts = options || {};
+    this.handle = native.renderer_create(device.handle, opts.initialPairCapacity || 0);
+    this.owned = true;
+  }
+  /** setGaussians(Float32Array of 16 floats each, {rest, degree}) */
+  setGaussians(data, sh) {
+    native.renderer_set_gaussians(this.handle, data, sh ? sh.rest : null, sh ? sh.degree : 0);
+  }
+  adoptGaussians(count, gaussians, sh, degree) {
+    native.renderer_adopt_gaussians(this.handle, count, gaussians.handle, sh.handle, degree);
+    gaussians.owned = false;
+    sh.owned = false;
+  }
+  setShDegree(degree) {
+    native.renderer_set_sh_degree(this.handle, degree);
+  }
+  /** encode(target, camera, background, afterPreprocess): the hook runs between the preprocess and the tiles. */
+  encode(target, camera, background, afterPreprocess) {
+    native.renderer_encode_preprocess(this.handle, camera);
+    if (afterPreprocess) afterPreprocess(this);
+    native.renderer_encode_tiles(this.handle, target ? target.handle : null, background);
+  }
+  encodeDraw(target, splats, count, viewport, background) {
+    native.renderer_encode_draw(this.handle, target ? target.handle : null, splats.handle, count, viewport[0],
+      viewport[1], background);
+  }
+  encodeBackward(pixelGrads) {
+    native.renderer_encode_backward(this.handle, pixelGrads.handle);
+  }
+  afterSubmit() {
+    native.renderer_after_submit(this.handle);
+    return Promise.resolve();
+  }
+  get count() {
+    return native.renderer_count(this.handle);
+  }
+  get splatBuffer() {
+    return view(native.renderer_splat_buffer(this.handle), this);
+  }
+  get gaussianBuffer() {
+    return view(native.renderer_gaussian_buffer(this.handle), this);
+  }
+  get shBuffer() {
+    return view(native.renderer_sh_buffer(this.handle), this);
+  }
+  get tiles() {
+    const h = native.renderer_tiles(this.handle);
+    return h ? new TileRasterizer(null, null, 0, 0, h, this) : undefined;
+  }
+  get pixelBuffer() {
+    return view(native.renderer_pixel_buffer(this.handle), this);
+  }
+  get pixelLastBuffer() {
+    return view(native.renderer_pixel_last_buffer(this.handle), this);
+  }
+  get gradients() {
+    const g = native.renderer_gradient_gaussians(this.handle);
+    if (!g) return undefined;
+    return {
+      gaussians: view(g, this),
+      sh: view(native.renderer_gradient_sh(this.handle), this),
+      splats: view(native.renderer_gradient_splats(this.handle), this),
+    };
+  }
+  destroy() {
+    if (this.owned && this.handle) native.renderer_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/** src/trainingLoss.ts:16-130 */
+class TrainingLoss {
+  constructor(device, width, height) {
+    this.width = width;
+    this.height = height;
+    this.handle = native.loss_create(device.handle, width, height);
+  }
+  encode(rendered, target) {
+    native.loss_encode(this.handle, rendered.handle, target.handle);
+  }
+  read() {
+    return Promise.resolve(native.loss_read(this.handle));
+  }
+  get pixelGrads() {
+    return view(native.loss_pixel_grads(this.handle), this);
+  }
+  get pixelLoss() {
+    return view(native.loss_pixel_loss(this.handle), this);
+  }
+  destroy() {
+    if (this.handle) native.loss_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/** The shortest decimal that rounds to the same float32 (the C ABI stores floats). */
+function shortestFloat32(x) {
+  for (let p = 1; p <= 9; p++) {
+    const y = Number(x.toPrecision(p));
+    if (Math.fround(y) === Math.fround(x)) return y;
+  }
+  return x;
+}
+
+/** DEFAULT_SETTINGS (src/trainer.ts:65-82), without the extent. */
+const DEFAULT_SETTINGS = (() => {
+  const s = native.default_settings(1);
+  delete s.extent;
+  for (const key of Object.keys(s)) {
+    if (Array.isArray(s[key])) s[key] = s[key].map(shortestFloat32);
+    else s[key] = shortestFloat32(s[key]);
+  }
+  return Object.freeze(s);
+})();
+
+/**
+ * src/trainer.ts:103-446. `raw` = {params: Float32Array (16 per Gaussian), sh:
+ * Float32Array (45 per Gaussian)}; `views` = [{name, camera, image: Uint32Array rgba8}].
+ */
+class Trainer {
+  constructor(device, raw, views, settings) {
+    this.device = device;
+    this.settings = settings;
+    this.handle = native.trainer_create(device.handle, raw.params, raw.sh, views, settings);
+  }
+  get count() {
+    return native.trainer_count(this.handle);
+  }
+  get iteration() {
+    return native.trainer_iteration(this.handle);
+  }
+  set iteration(value) {
+    native.trainer_set_iteration(this.handle, value);
+  }
+  get shDegree() {
+    return native.trainer_sh_degree(this.handle);
+  }
+  set shDegree(value) {
+    native.trainer_set_sh_degree(this.handle, value);
+  }
+  get densifications() {
+    return native.trainer_densifications(this.handle);
+  }
+  get buffers() {
+    return {
+      params: view(native.trainer_params(this.handle), this),
+      moments: view(native.trainer_moments(this.handle), this),
+      stats: view(native.trainer_stats(this.handle), this),
+    };
+  }
+  get renderer() {
+    return new GaussianRenderer(this.device, null, native.trainer_renderer(this.handle), this);
+  }
+  step() {
+    native.trainer_step(this.handle);
+    return Promise.resolve();
+  }
+  optimize() {
+    native.trainer_optimize(this.handle);
+  }
+  maintain() {
+    native.trainer_maintain(this.handle);
+    return Promise.resolve();
+  }
+  densify(pruneLarge) {
+    native.trainer_densify(this.handle, !!pruneLarge);
+    return Promise.resolve();
+  }
+  resetOpacity() {
+    native.trainer_reset_opacity(this.handle);
+  }
+  exportRaw() {
+    return Promise.resolve(native.trainer_export_raw(this.handle));
+  }
+  lastLoss() {
+    return Promise.resolve(native.trainer_last_loss(this.handle));
+  }
+  evaluate(views, withSsim) {
+    return Promise.resolve(native.trainer_evaluate(this.handle, views, withSsim === undefined ? true : withSsim));
+  }
+  destroy() {
+    if (this.handle) native.trainer_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
+/** Raw parameters to the renderer's activated layout (src/trainer.ts:449-461). */
+function activate(params) {
+  const out = new Float32Array(params.length);
+  for (let at = 0; at < params.length; at += FLOATS_PER_GAUSSIAN) {
+    for (let k = 0; k < 3; k++) {
+      out[at + k] = params[at + k];
+      out[at + 4 + k] = Math.exp(params[at + 4 + k]);
+      out[at + 12 + k] = 0.5 + SH_C0 * params[at + 12 + k];
+    }
+    out[at + 3] = 1 / (1 + Math.exp(-params[at + 3]));
+    for (let k = 8; k < 12; k++) out[at + k] = params[at + k];
+  }
+  return out;
+}
+
+module.exports = {
+  version: native.version,
+  initGpu,
+  Device,
+  Buffer,
+  PrefixSum,
+  RadixSort,
+  TileRasterizer,
+  GaussianRenderer,
+  TrainingLoss,
+  Trainer,
+  DEFAULT_SETTINGS,
+  activate,
+  FLOATS_PER_GAUSSIAN,
+  FLOATS_PER_SPLAT,
+  SH_REST_FLOATS,
+};

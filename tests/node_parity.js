@@ -1,0 +1,202 @@
+'use strict';
+// Drives the Node classes (diorama_amd/node) the way the reference's checks drive
+// its operators: the prefix sum and radix sort against plain JS
+// (src/checks/prefixSum.ts, src/checks/radixSort.ts), the loss fixture
+// (src/checks/loss.ts:24-89) and the backward fixture (src/checks/backward.ts:49-114).
+// Prints one JSON line of results; tests/test_node.py applies the bars.
+// Usage: node tests/node_parity.js <cpu|gpu>
+
+const fs = require('fs');
+const path = require('path');
+const d = require(path.join(__dirname, '..', 'diorama_amd', 'node'));
+
+function fixture(name) {
+  return JSON.parse(fs.readFileSync(path.join(__dirname, 'golden', name), 'utf8'));
+}
+
+function rng(seed) {
+  let s = seed >>> 0;
+  return () => {
+    s = (s + 0x6d2b79f5) >>> 0;
+    let t = s;
+    t = Math.imul(t ^ (t >>> 15), t | 1);
+    t ^= t + Math.imul(t ^ (t >>> 7), t | 61);
+    return ((t ^ (t >>> 14)) >>> 0) / 4294967296;
+  };
+}
+
+function cpuChecks() {
+  const out = { version: d.version(), defaults: d.DEFAULT_SETTINGS };
+  const errors = [];
+  try {
+    d.initGpu(4096);
+  } catch (e) {
+    errors.push(e.message);
+  }
+  try {
+    new d.PrefixSum({ handle: null }, { handle: null }, 16);
+  } catch (e) {
+    errors.push(e.message);
+  }
+  out.errors = errors;
+  const raw = new Float32Array(16);
+  raw[3] = 0;
+  raw[4] = Math.log(2);
+  raw[12] = 1;
+  out.activated = Array.from(d.activate(raw));
+  return out;
+}
+
+async function gpuChecks() {
+  const out = {};
+  const device = d.initGpu(0);
+
+  // Prefix sum over a ragged count (src/checks/prefixSum.ts).
+  const random = rng(3);
+  const n = 1000003;
+  const values = new Uint32Array(n);
+  for (let i = 0; i < n; i++) values[i] = Math.floor(random() * 100);
+  const data = device.upload(values);
+  const scan = new d.PrefixSum(device, data, n);
+  scan.encode();
+  const scanned = new Uint32Array(data.read().buffer);
+  let running = 0;
+  let scanBad = 0;
+  for (let i = 0; i < n; i++) {
+    if (scanned[i] !== running) scanBad++;
+    running += values[i];
+  }
+  const total = new Uint32Array(scan.total.read(0, 4).buffer)[0];
+  out.scan = { mismatches: scanBad, total, expected: running };
+  scan.destroy();
+  data.destroy();
+
+  // Radix sort, 32-bit keys and 64-bit (low, high) keys with 16 high bits
+  // (src/checks/radixSort.ts); stable, values carried along.
+  out.sort = {};
+  for (const highBits of [0, 16]) {
+    const m = 300001;
+    const sort = new d.RadixSort(device, m, { highBits });
+    const words = highBits ? 2 : 1;
+    const keys = new Uint32Array(m * words);
+    const vals = new Uint32Array(m);
+    for (let i = 0; i < m; i++) {
+      keys[i * words] = Math.floor(random() * 4294967296) >>> 0;
+      if (highBits) keys[i * words + 1] = Math.floor(random() * 64);
+      vals[i] = i;
+    }
+    const high = (i) => (highBits ? keys[i * words + 1] : 0);
+    sort.keys.write(keys);
+    sort.values.write(vals);
+    sort.encode();
+    const sk = new Uint32Array(sort.keys.read(0, m * 4 * words).buffer);
+    const sv = new Uint32Array(sort.values.read(0, m * 4).buffer);
+    const order = Array.from(vals).sort((a, b) => high(a) - high(b) || keys[a * words] - keys[b * words] || a - b);
+    let bad = 0;
+    for (let i = 0; i < m; i++) {
+      const j = order[i];
+      if (sv[i] !== j || sk[i * words] !== keys[j * words] || (highBits && sk[i * words + 1] !== keys[j * words + 1])) bad++;
+    }
+    out.sort['high' + highBits] = bad;
+    sort.destroy();
+  }
+
+  // Loss fixture.
+  const lf = fixture('loss.json');
+  const [lw, lh] = lf.size;
+  const rendered = new Float32Array(lw * lh * 4);
+  const target = new Uint32Array(lw * lh);
+  for (let p = 0; p < lw * lh; p++) {
+    for (let c = 0; c < 3; c++) rendered[p * 4 + c] = lf.render[p * 3 + c];
+    target[p] = (lf.target[p * 3] | (lf.target[p * 3 + 1] << 8) | (lf.target[p * 3 + 2] << 16) | (255 << 24)) >>> 0;
+  }
+  const loss = new d.TrainingLoss(device, lw, lh);
+  const rb = device.upload(rendered);
+  const tb = device.upload(target);
+  loss.encode(rb, tb);
+  const value = await loss.read();
+  const lg = new Float32Array(loss.pixelGrads.read(0, lw * lh * 16).buffer);
+  let gmax = 0;
+  let gerr = 0;
+  for (let p = 0; p < lw * lh; p++) {
+    for (let c = 0; c < 3; c++) {
+      gmax = Math.max(gmax, Math.abs(lf.gradient[p * 3 + c]));
+      gerr = Math.max(gerr, Math.abs(lg[p * 4 + c] - lf.gradient[p * 3 + c]));
+    }
+  }
+  out.loss = { error: Math.abs(value - lf.loss), gradientError: gerr / gmax };
+  loss.destroy();
+  rb.destroy();
+  tb.destroy();
+
+  // Backward fixture through GaussianRenderer, with the afterPreprocess hook.
+  const bf = fixture('backward.json');
+  const count = bf.gaussians.length;
+  const g = new Float32Array(count * 16);
+  bf.gaussians.forEach((item, i) => {
+    g.set(item.position, i * 16);
+    g[i * 16 + 3] = item.opacity;
+    g.set(item.scale, i * 16 + 4);
+    g.set(item.rotation, i * 16 + 8);
+    g.set(item.color, i * 16 + 12);
+  });
+  const rest = new Float32Array([].concat.apply([], bf.shRest));
+  const renderer = new d.GaussianRenderer(device);
+  renderer.setGaussians(g, { rest, degree: bf.shDegree });
+  const [w, h] = bf.viewport;
+  let hookSaw = -1;
+  renderer.encode(null, { view: bf.view, proj: bf.proj, viewport: bf.viewport }, bf.background, (r) => {
+    hookSaw = r.splatBuffer.size / (d.FLOATS_PER_SPLAT * 4);
+  });
+  await renderer.afterSubmit();
+  const color = new Float32Array(renderer.pixelBuffer.read(0, w * h * 16).buffer);
+  let imageErr = 0;
+  for (let p = 0; p < w * h; p++) {
+    for (let c = 0; c < 3; c++) imageErr = Math.max(imageErr, Math.abs(color[p * 4 + c] - bf.image[p * 3 + c]));
+  }
+  const pg = new Float32Array(w * h * 4);
+  for (let p = 0; p < w * h; p++) for (let c = 0; c < 3; c++) pg[p * 4 + c] = bf.pixelGrads[p * 3 + c];
+  const pgb = device.upload(pg);
+  renderer.encodeBackward(pgb);
+  const grads = renderer.gradients;
+  const gg = new Float32Array(grads.gaussians.read(0, count * 64).buffer);
+  const shg = new Float32Array(grads.sh.read(0, count * 180).buffer);
+  const groups = { position: [0, 3], scale: [4, 3], rotation: [8, 4], opacity: [3, 1], color: [12, 3] };
+  const gradientErrors = {};
+  for (const name of Object.keys(groups)) {
+    const [off, size] = groups[name];
+    let ref = 0;
+    let err = 0;
+    bf.gradients.forEach((item, i) => {
+      const want = [].concat(item[name]);
+      for (let k = 0; k < size; k++) {
+        ref = Math.max(ref, Math.abs(want[k]));
+        err = Math.max(err, Math.abs(gg[i * 16 + off + k] - want[k]));
+      }
+    });
+    gradientErrors[name] = err / ref;
+  }
+  let shRef = 0;
+  let shErr = 0;
+  bf.gradients.forEach((item, i) => {
+    for (let k = 0; k < 45; k++) {
+      shRef = Math.max(shRef, Math.abs(item.sh[k]));
+      shErr = Math.max(shErr, Math.abs(shg[i * 45 + k] - item.sh[k]));
+    }
+  });
+  gradientErrors.sh = shErr / shRef;
+  out.render = { count: renderer.count, hookSaw, imageError: imageErr, gradientErrors };
+  pgb.destroy();
+  renderer.destroy();
+  device.destroy();
+  return out;
+}
+
+(async () => {
+  const mode = process.argv[2] || 'cpu';
+  const result = mode === 'gpu' ? await gpuChecks() : cpuChecks();
+  process.stdout.write(JSON.stringify(result) + '\n');
+})().catch((e) => {
+  process.stderr.write(String(e && e.stack ? e.stack : e) + '\n');
+  process.exit(1);
+});

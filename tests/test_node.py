@@ -1,0 +1,55 @@
+"""The Node boundary (diorama_amd/node): the N-API addon over the C ABI and the JS
+classes with the reference's operator names. tests/node_parity.js does the work;
+this file applies the bars."""
+import json
+import math
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ADDON = os.path.join(os.path.dirname(HERE), "diorama_amd", "node", "diorama.node")
+SCRIPT = os.path.join(HERE, "node_parity.js")
+
+needs_node = pytest.mark.skipif(shutil.which("node") is None or not os.path.exists(ADDON),
+                                reason="node or the built addon is absent")
+
+
+def run(mode, timeout):
+    r = subprocess.run(["node", SCRIPT, mode], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+@needs_node
+def test_node_module_loads_without_gpu():
+    out = run("cpu", 120)
+    assert out["version"].startswith("diorama-mi355x")
+    # DEFAULT_SETTINGS, src/trainer.ts:65-82, as float32 values.
+    want = {"positionLrSteps": 30000, "positionLr": [0.00016, 0.0000016], "opacityLr": 0.025, "scaleLr": 0.005,
+            "rotationLr": 0.001, "colorLr": 0.0025, "shLr": 0.0025 / 20, "shDegreeInterval": 1000,
+            "densifyFrom": 500, "densifyUntil": 15000, "densifyInterval": 100, "densifyGradThreshold": 0.0002,
+            "percentDense": 0.01, "minOpacity": 0.005, "opacityResetInterval": 3000, "background": [0, 0, 0]}
+    assert out["defaults"] == want
+    # Both error paths throw JS errors carrying the library's message.
+    assert len(out["errors"]) == 2
+    assert "ctx is null" in out["errors"][1]
+    a = out["activated"]
+    assert a[3] == 0.5 and a[4] == pytest.approx(2.0, rel=1e-6) and a[5] == 1.0
+    assert a[12] == pytest.approx(0.5 + 0.28209479177387814, rel=1e-6)
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_parity_on_gpu():
+    out = run("gpu", 110)
+    print(json.dumps(out))
+    assert out["scan"]["mismatches"] == 0 and out["scan"]["total"] == out["scan"]["expected"]
+    assert out["sort"] == {"high0": 0, "high16": 0}
+    assert out["loss"]["error"] < 1e-5 and out["loss"]["gradientError"] < 1e-3
+    r = out["render"]
+    assert r["count"] == 10 and r["hookSaw"] >= 10
+    assert r["imageError"] < 1e-4
+    assert max(r["gradientErrors"].values()) < 1e-3 and not any(math.isnan(v) for v in r["gradientErrors"].values())
