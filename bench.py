@@ -43,6 +43,12 @@ STAGE_BYTES = {
     "adam": (1708, 0, 0),
 }
 STEP_BYTES = (2612, 240, 88)
+# Stage name -> kernel name, for the PMC traffic lines.
+STAGE_KERNEL = {"preprocess": "preprocess_kernel", "rasterize": "rasterize_kernel",
+                "rasterize_backward": "rasterize_backward_kernel", "preprocess_backward": "preprocess_backward_kernel",
+                "adam": "adam_kernel"}
+TRAFFIC_FILE = "pmc_traffic_train_step_1.1M_980x545.txt"
+METRIC = "train steps/s @1.1M Gaussians 980x545 FP32; held-out PSNR delta vs plain"
 
 
 def log(*parts):
@@ -119,6 +125,34 @@ def cpu_baseline(params, sh, cam, image, sh_degree, budget_s=30.0):
                       + ("" if sample == n else f", time scaled by {n / sample:.2f} to the full count")}
 
 
+def copy_calibration():
+    """Three device-to-device copies of 256 MiB: a kernel whose HBM bytes are known
+    (256 MiB read + 256 MiB written each), profiled to scale the PMC byte counters."""
+    import torch
+
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": a.numel() * 4}), flush=True)
+    return 0
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE already scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 4 and parts[0] == "traffic" and parts[1] == kernel:
+                return int(parts[2]) + int(parts[3])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,7 +161,11 @@ def main():
     ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--repeats", type=int, default=20, help="launches per stage timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three plain 256 MiB device copies (the FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(multi-GPU sharding is a design, not an implementation)", file=sys.stderr)
@@ -175,7 +213,9 @@ def main():
     dominant = max(stages, key=lambda k: stages[k]["ms"])
     d = stages[dominant]
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(d["GB/s"], 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(d["GB/s"] / HBM_PEAK_GBS, 4), "traffic": None}
+                "unit": "GB/s", "frac": round(d["GB/s"] / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(STAGE_KERNEL.get(dominant, dominant)),
+                "algorithmic_bytes": d["bytes"]}
     step_bytes = STEP_BYTES[0] * n + STEP_BYTES[1] * pairs + STEP_BYTES[2] * x
 
     cpu = None
@@ -186,7 +226,7 @@ def main():
     trainer.destroy()
     ctx.close()
     line = {
-        "metric": "train steps/s @1.1M Gaussians 980x545 FP32",
+        "metric": METRIC,
         "value": round(steps_per_s, 2),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -205,6 +245,7 @@ def main():
                    "step_GBps_model": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
                    "parallelism": "single GPU"},
         "stages_us": {k: round(v["ms"] * 1000, 1) for k, v in stages.items()},
+        "stages_hbm_frac": {k: round(v["GB/s"] / HBM_PEAK_GBS, 4) for k, v in stages.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -29,7 +29,7 @@ class Buffer {
   write(data, offset) {
     native.buf_upload(this.handle, offset || 0, data);
   }
-  /** Reads `bytes` (default: all) from `offset`; waits for the GPU. */
+  /** Reads `bytes` (default: all) from `offset` into a new ArrayBuffer, as mapAsync + getMappedRange would; waits for the GPU. */
   read(offset, bytes) {
     const from = offset || 0;
     return native.buf_download(this.handle, from, bytes === undefined ? this.size - from : bytes);

@@ -59,14 +59,14 @@ async function gpuChecks() {
   const data = device.upload(values);
   const scan = new d.PrefixSum(device, data, n);
   scan.encode();
-  const scanned = new Uint32Array(data.read().buffer);
+  const scanned = new Uint32Array(data.read());
   let running = 0;
   let scanBad = 0;
   for (let i = 0; i < n; i++) {
     if (scanned[i] !== running) scanBad++;
     running += values[i];
   }
-  const total = new Uint32Array(scan.total.read(0, 4).buffer)[0];
+  const total = new Uint32Array(scan.total.read(0, 4))[0];
   out.scan = { mismatches: scanBad, total, expected: running };
   scan.destroy();
   data.destroy();
@@ -89,8 +89,8 @@ async function gpuChecks() {
     sort.keys.write(keys);
     sort.values.write(vals);
     sort.encode();
-    const sk = new Uint32Array(sort.keys.read(0, m * 4 * words).buffer);
-    const sv = new Uint32Array(sort.values.read(0, m * 4).buffer);
+    const sk = new Uint32Array(sort.keys.read(0, m * 4 * words));
+    const sv = new Uint32Array(sort.values.read(0, m * 4));
     const order = Array.from(vals).sort((a, b) => high(a) - high(b) || keys[a * words] - keys[b * words] || a - b);
     let bad = 0;
     for (let i = 0; i < m; i++) {
@@ -115,7 +115,7 @@ async function gpuChecks() {
   const tb = device.upload(target);
   loss.encode(rb, tb);
   const value = await loss.read();
-  const lg = new Float32Array(loss.pixelGrads.read(0, lw * lh * 16).buffer);
+  const lg = new Float32Array(loss.pixelGrads.read(0, lw * lh * 16));
   let gmax = 0;
   let gerr = 0;
   for (let p = 0; p < lw * lh; p++) {
@@ -149,7 +149,7 @@ async function gpuChecks() {
     hookSaw = r.splatBuffer.size / (d.FLOATS_PER_SPLAT * 4);
   });
   await renderer.afterSubmit();
-  const color = new Float32Array(renderer.pixelBuffer.read(0, w * h * 16).buffer);
+  const color = new Float32Array(renderer.pixelBuffer.read(0, w * h * 16));
   let imageErr = 0;
   for (let p = 0; p < w * h; p++) {
     for (let c = 0; c < 3; c++) imageErr = Math.max(imageErr, Math.abs(color[p * 4 + c] - bf.image[p * 3 + c]));
@@ -159,8 +159,8 @@ async function gpuChecks() {
   const pgb = device.upload(pg);
   renderer.encodeBackward(pgb);
   const grads = renderer.gradients;
-  const gg = new Float32Array(grads.gaussians.read(0, count * 64).buffer);
-  const shg = new Float32Array(grads.sh.read(0, count * 180).buffer);
+  const gg = new Float32Array(grads.gaussians.read(0, count * 64));
+  const shg = new Float32Array(grads.sh.read(0, count * 180));
   const groups = { position: [0, 3], scale: [4, 3], rotation: [8, 4], opacity: [3, 1], color: [12, 3] };
   const gradientErrors = {};
   for (const name of Object.keys(groups)) {
