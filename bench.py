@@ -139,6 +139,107 @@ def copy_calibration():
     return 0
 
 
+SCAN_SIZES = [1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20]
+SORT_SIZES = [8 << 20, 30_000_000]
+
+
+def scan_sort_bench(args):
+    """The scan/sort microbenchmark: exclusive scans of 1M-16M geometric tile-touch
+    counts and stable sorts of 8M and 30M 64-bit (Zipf tile, depth bits) keys with
+    u32 values, through PrefixSum and RadixSort (highBits 16, as the tile binning
+    of src/tileRasterizer.ts:279). Each launch is timed with timestamps on the
+    library's stream; the input is restored by a device copy before each one."""
+    import diorama_amd
+    from diorama_amd import scene
+
+    ctx = diorama_amd.Context(0)
+    results = {"scan": {}, "sort": {}}
+    cpu = {}
+    for n in SCAN_SIZES:
+        values = scene.tile_touch_counts(n, 5)
+        orig = ctx.upload(values)
+        data = ctx.buffer(n * 4)
+        scan = diorama_amd.PrefixSum(ctx, data, n)
+        times = []
+        for it in range(args.warmup + args.steps):
+            data.copy_from(orig)
+            ctx.timestamp(0)
+            scan.encode()
+            ctx.timestamp(1)
+            ms = ctx.elapsed_ms(0, 1)
+            if it >= args.warmup:
+                times.append(ms)
+        out = data.download(np.uint32, n)
+        ok = bool(out[-1] == np.uint32(int(values[:-1].astype(np.uint64).sum()) & 0xFFFFFFFF))
+        us = float(np.mean(times)) * 1e3
+        results["scan"][n] = {"us": round(us, 2), "GB/s": round(8 * n / us / 1e3, 1),
+                              "hbm_frac": round(8 * n / us / 1e3 / HBM_PEAK_GBS, 4), "last_ok": ok}
+        log(f"scan {n:>9d}: {us:8.1f} us  {8 * n / us / 1e3:7.1f} GB/s")
+        if n == SCAN_SIZES[-1] and not args.no_cpu:
+            t0 = time.perf_counter()
+            np.cumsum(values, dtype=np.uint32)
+            cpu["scan_16M_s"] = time.perf_counter() - t0
+        scan.destroy()
+        data.destroy()
+        orig.destroy()
+    for n in SORT_SIZES:
+        keys, lo, hi = scene.tile_depth_keys(n, 7)
+        payload = np.arange(n, dtype=np.uint32)
+        k_orig, v_orig = ctx.upload(keys), ctx.upload(payload)
+        sort = diorama_amd.RadixSort(ctx, n, 16)
+        times = []
+        for it in range(args.warmup + args.steps):
+            sort.keys.copy_from(k_orig)
+            sort.values.copy_from(v_orig)
+            ctx.timestamp(0)
+            sort.encode()
+            ctx.timestamp(1)
+            ms = ctx.elapsed_ms(0, 1)
+            if it >= args.warmup:
+                times.append(ms)
+        v = sort.values.download(np.uint32, n)
+        sorted_hi, sorted_lo = hi[v], lo[v]
+        ok = bool(np.all((sorted_hi[1:] > sorted_hi[:-1]) | ((sorted_hi[1:] == sorted_hi[:-1]) &
+                                                          ((sorted_lo[1:] > sorted_lo[:-1]) |
+                                                           ((sorted_lo[1:] == sorted_lo[:-1]) & (v[1:] > v[:-1]))))))
+        us = float(np.mean(times)) * 1e3
+        results["sort"][n] = {"us": round(us, 2), "Mkeys/s": round(n / us, 1),
+                              "GB/s_6pass_model": round(144 * n / us / 1e3, 1),
+                              "hbm_frac_6pass_model": round(144 * n / us / 1e3 / HBM_PEAK_GBS, 4),
+                              "GB/s_one_pass_bound": round(24 * n / us / 1e3, 1), "stable_sorted": ok}
+        log(f"sort {n:>9d}: {us:8.1f} us  {n / us:7.1f} Mkeys/s  {144 * n / us / 1e3:7.1f} GB/s (6-pass model)")
+        if n == SORT_SIZES[0] and not args.no_cpu:
+            t0 = time.perf_counter()
+            np.lexsort((lo, hi & 0xFFFF))
+            cpu["sort_8M_s"] = time.perf_counter() - t0
+        sort.destroy()
+        k_orig.destroy()
+        v_orig.destroy()
+    ctx.close()
+    big = results["sort"][SORT_SIZES[-1]]
+    line = {
+        "metric": "64-bit radix sort Mkeys/s @30M (Zipf tile, depth) keys",
+        "value": big["Mkeys/s"],
+        "unit": "Mkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "dtype": "u32/u64",
+        "data": "synthetic: Geometric(mean 6) counts capped at 256; Zipf(1.1) tiles over 62x35, depth U[0.2, 50]",
+        "config": {"workload": "scan 1M-16M u32 and stable sort of 8M/30M 64-bit keys (highBits 16) with u32 values"},
+        "results": results,
+        "roofline": {"kernel": "radix sort (onesweep passes)", "bound": "hbm", "achieved": big["GB/s_6pass_model"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": big["hbm_frac_6pass_model"], "traffic": None},
+        "cpu_baseline": None if args.no_cpu else {
+            "value": round(SORT_SIZES[0] / cpu["sort_8M_s"] / 1e6, 2), "unit": "Mkeys/s", "cores": 1, "kind": "port",
+            "sample": f"numpy lexsort of the 8M keys: {cpu['sort_8M_s']:.2f} s; numpy cumsum of 16M counts: "
+                      f"{cpu['scan_16M_s'] * 1e3:.1f} ms"},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE and
     WRITE_SIZE already scaled by the copy calibration), or None."""
@@ -161,15 +262,19 @@ def main():
     ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--repeats", type=int, default=20, help="launches per stage timing")
+    ap.add_argument("--workload", choices=["train", "scan-sort"], default="train",
+                    help="train: the training step (default); scan-sort: the scan/sort microbenchmark")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (the FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
-    if args.copy_calibration:
-        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(multi-GPU sharding is a design, not an implementation)", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration()
+    if args.workload == "scan-sort":
+        return scan_sort_bench(args)
 
     import diorama_amd
     from diorama_amd import scene

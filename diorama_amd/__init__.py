@@ -63,10 +63,13 @@ SIGNATURES = {
     "dio_ctx_create": (_I, [_I, ctypes.POINTER(_P)]),
     "dio_ctx_synchronize": (_I, [_P]),
     "dio_ctx_destroy": (None, [_P]),
+    "dio_ctx_timestamp": (_I, [_P, _U32]),
+    "dio_ctx_elapsed": (_I, [_P, _U32, _U32, ctypes.POINTER(ctypes.c_double)]),
     "dio_buf_create": (_I, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "dio_buf_upload": (_I, [_P, ctypes.c_size_t, _P, ctypes.c_size_t]),
     "dio_buf_download": (_I, [_P, ctypes.c_size_t, _P, ctypes.c_size_t]),
     "dio_buf_clear": (_I, [_P]),
+    "dio_buf_copy": (_I, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, ctypes.c_size_t]),
     "dio_buf_size": (ctypes.c_size_t, [_P]),
     "dio_buf_destroy": (None, [_P]),
     "dio_prefix_sum_create": (_I, [_P, _P, _U32, _P, ctypes.POINTER(_P)]),
@@ -192,6 +195,16 @@ class Context:
     def buffer(self, nbytes):
         return Buffer.create(self, nbytes)
 
+    def timestamp(self, slot):
+        """Records timestamp `slot` on the stream (a timestamp query)."""
+        _check(lib().dio_ctx_timestamp(self.handle, slot))
+
+    def elapsed_ms(self, from_slot, to_slot):
+        """Milliseconds between two recorded timestamps; waits for the later one."""
+        ms = ctypes.c_double()
+        _check(lib().dio_ctx_elapsed(self.handle, from_slot, to_slot, ctypes.byref(ms)))
+        return ms.value
+
     def upload(self, array):
         a = np.ascontiguousarray(array)
         b = Buffer.create(self, a.nbytes)
@@ -236,6 +249,11 @@ class Buffer:
 
     def clear(self):
         _check(lib().dio_buf_clear(self.handle))
+
+    def copy_from(self, src, nbytes=None, dst_offset=0, src_offset=0):
+        """encoder.copyBufferToBuffer on this buffer's stream."""
+        n = src.nbytes - src_offset if nbytes is None else nbytes
+        _check(lib().dio_buf_copy(self.handle, dst_offset, src.handle, src_offset, n))
 
     def destroy(self):
         if self.owned and self.handle:

@@ -48,6 +48,8 @@ void invalid(const std::string& message) { throw Error(DIO_ERR_INVALID, message)
 struct dio_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t stamps[DIO_TIMESTAMP_SLOTS] = {};
+  bool stamped[DIO_TIMESTAMP_SLOTS] = {};
 };
 
 struct dio_buf {
@@ -135,8 +137,33 @@ extern "C" int dio_ctx_synchronize(dio_ctx* ctx) {
 extern "C" void dio_ctx_destroy(dio_ctx* ctx) {
   if (!ctx) return;
   (void)hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t e : ctx->stamps)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+extern "C" int dio_ctx_timestamp(dio_ctx* ctx, uint32_t slot) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (slot >= DIO_TIMESTAMP_SLOTS) invalid("no such timestamp slot");
+    if (!ctx->stamps[slot]) DIO_HIP(hipEventCreate(&ctx->stamps[slot]));
+    DIO_HIP(hipEventRecord(ctx->stamps[slot], ctx->stream));
+    ctx->stamped[slot] = true;
+  });
+}
+
+extern "C" int dio_ctx_elapsed(dio_ctx* ctx, uint32_t from_slot, uint32_t to_slot, double* ms) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(ms, "ms");
+    if (from_slot >= DIO_TIMESTAMP_SLOTS || to_slot >= DIO_TIMESTAMP_SLOTS) invalid("no such timestamp slot");
+    if (!ctx->stamped[from_slot] || !ctx->stamped[to_slot]) invalid("timestamp slot not recorded");
+    DIO_HIP(hipEventSynchronize(ctx->stamps[to_slot]));
+    float elapsed = 0.f;
+    DIO_HIP(hipEventElapsedTime(&elapsed, ctx->stamps[from_slot], ctx->stamps[to_slot]));
+    *ms = elapsed;
+  });
 }
 
 extern "C" int dio_buf_create(dio_ctx* ctx, size_t bytes, dio_buf** out) {
@@ -174,6 +201,17 @@ extern "C" int dio_buf_clear(dio_buf* buf) {
   return guard([&] {
     need(buf, "buffer");
     dio::fill_u32(buf->ctx->stream, static_cast<uint32_t*>(buf->ptr), 0u, (buf->bytes + 3) / 4);
+  });
+}
+
+extern "C" int dio_buf_copy(dio_buf* dst, size_t dst_offset, const dio_buf* src, size_t src_offset, size_t bytes) {
+  return guard([&] {
+    need(dst, "destination");
+    need(src, "source");
+    if (dst_offset + bytes > dst->bytes || src_offset + bytes > src->bytes) invalid("copy past the end of a buffer");
+    if (bytes == 0) return;
+    DIO_HIP(hipMemcpyAsync(static_cast<char*>(dst->ptr) + dst_offset, static_cast<const char*>(src->ptr) + src_offset,
+                           bytes, hipMemcpyDeviceToDevice, dst->ctx->stream));
   });
 }
 

@@ -372,39 +372,43 @@ struct Replay {
   float next_alpha, next_r, next_g, next_b;
 };
 
-// One splat's 9 gradient shares at one pixel (rasterizeBackward.wgsl:199-230).
+// One splat's 9 gradient shares at one pixel (rasterizeBackward.wgsl:199-230),
+// written without branches: a pixel the splat does not blend into (power > 0,
+// alpha < 1/255, or past the pixel's last splat) runs the same instructions with
+// alpha = 0, which leaves T, the colour behind and the next splat as they were and
+// zeroes every share. The blended lanes compute exactly the branchy form.
 __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy, float mx, float my, float ca,
                                     float cbb, float cc, float op, float sr, float sg, float sb, float share[kGrads]) {
-#pragma unroll
-  for (int v = 0; v < kGrads; v++) share[v] = 0.f;
-  if (!blended) return false;
   const float dx = cx - mx, dy = cy - my;
   const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
-  if (power > 0.f) return false;
   const float G = __expf(power);
-  const float alpha = fminf(0.99f, op * G);
-  if (alpha < 1.f / 255.f) return false;
-  const float keep = __builtin_amdgcn_rcpf(1.f - alpha);  // one hardware reciprocal for both divisions
+  const float clamped = fminf(0.99f, op * G);
+  const bool valid = blended && power <= 0.f && clamped >= 1.f / 255.f;
+  const float alpha = valid ? clamped : 0.f;
+  const float keep = __builtin_amdgcn_rcpf(1.f - alpha);  // one hardware reciprocal for both divisions; 1 when idle
   st.T *= keep;
-  st.behind_r = st.next_alpha * st.next_r + (1.f - st.next_alpha) * st.behind_r;
-  st.behind_g = st.next_alpha * st.next_g + (1.f - st.next_alpha) * st.behind_g;
-  st.behind_b = st.next_alpha * st.next_b + (1.f - st.next_alpha) * st.behind_b;
-  st.next_alpha = alpha;
-  st.next_r = sr; st.next_g = sg; st.next_b = sb;
+  const float na = valid ? st.next_alpha : 0.f;
+  st.behind_r = na * st.next_r + (1.f - na) * st.behind_r;
+  st.behind_g = na * st.next_g + (1.f - na) * st.behind_g;
+  st.behind_b = na * st.next_b + (1.f - na) * st.behind_b;
+  st.next_alpha = valid ? alpha : st.next_alpha;
+  st.next_r = valid ? sr : st.next_r;
+  st.next_g = valid ? sg : st.next_g;
+  st.next_b = valid ? sb : st.next_b;
   const float alpha_grad = st.T * ((sr - st.behind_r) * st.gr + (sg - st.behind_g) * st.gg + (sb - st.behind_b) * st.gb) -
                            st.final_T * keep * st.bg_grad;
-  const float power_grad = alpha_grad * op * G;
+  const float power_grad = valid ? alpha_grad * op * G : 0.f;
   share[0] = power_grad * (ca * dx + cbb * dy);
   share[1] = power_grad * (cbb * dx + cc * dy);
   share[2] = power_grad * (-0.5f * dx * dx);
   share[3] = power_grad * (-dx * dy);
   share[4] = power_grad * (-0.5f * dy * dy);
-  share[5] = alpha_grad * G;
+  share[5] = valid ? alpha_grad * G : 0.f;
   const float at = alpha * st.T;
   share[6] = at * st.gr;
   share[7] = at * st.gg;
   share[8] = at * st.gb;
-  return true;
+  return valid;
 }
 
 // rasterizeBackward.wgsl:49-163: each pixel replays its list back to front from

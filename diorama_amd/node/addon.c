@@ -189,6 +189,23 @@ FN(buf_download) {
   if (!ok(env, dio_buf_download(handle(&c, 0), (size_t)f64(&c, 1), data, bytes))) return NULL;
   return ab;
 }
+FN(buf_copy) {
+  ARGS;
+  if (!ok(env, dio_buf_copy(handle(&c, 0), (size_t)f64(&c, 1), handle(&c, 2), (size_t)f64(&c, 3), (size_t)f64(&c, 4))))
+    return NULL;
+  return undefined(env);
+}
+FN(ctx_timestamp) {
+  ARGS;
+  if (!ok(env, dio_ctx_timestamp(handle(&c, 0), u32(&c, 1)))) return NULL;
+  return undefined(env);
+}
+FN(ctx_elapsed) {
+  ARGS;
+  double ms = 0;
+  if (!ok(env, dio_ctx_elapsed(handle(&c, 0), u32(&c, 1), u32(&c, 2), &ms))) return NULL;
+  return number(env, ms);
+}
 FN(buf_clear) {
   ARGS;
   if (!ok(env, dio_buf_clear(handle(&c, 0)))) return NULL;
@@ -671,7 +688,7 @@ FN(trainer_destroy) {
 static napi_value init(napi_env env, napi_value exports) {
   napi_property_descriptor d[] = {
       EXPORT(version), EXPORT(ctx_create), EXPORT(ctx_synchronize), EXPORT(ctx_destroy), EXPORT(buf_create),
-      EXPORT(buf_upload), EXPORT(buf_download), EXPORT(buf_clear), EXPORT(buf_size), EXPORT(buf_destroy),
+      EXPORT(buf_upload), EXPORT(buf_download), EXPORT(buf_clear), EXPORT(buf_copy), EXPORT(ctx_timestamp), EXPORT(ctx_elapsed), EXPORT(buf_size), EXPORT(buf_destroy),
       EXPORT(prefix_sum_create), EXPORT(prefix_sum_total), EXPORT(prefix_sum_encode), EXPORT(prefix_sum_destroy),
       EXPORT(radix_sort_create), EXPORT(radix_sort_keys), EXPORT(radix_sort_values), EXPORT(radix_sort_capacity),
       EXPORT(radix_sort_encode), EXPORT(radix_sort_destroy), EXPORT(tiles_create), EXPORT(tiles_encode),

@@ -38,6 +38,11 @@ class Buffer {
   clear() {
     native.buf_clear(this.handle);
   }
+  /** encoder.copyBufferToBuffer(src, srcOffset, this, dstOffset, bytes) */
+  copyFrom(src, srcOffset, dstOffset, bytes) {
+    const from = srcOffset || 0;
+    native.buf_copy(this.handle, dstOffset || 0, src.handle, from, bytes === undefined ? src.size - from : bytes);
+  }
   destroy() {
     if (this.owned && this.handle) native.buf_destroy(this.handle);
     this.handle = null;
@@ -63,6 +68,13 @@ class Device {
   }
   synchronize() {
     native.ctx_synchronize(this.handle);
+  }
+  /** Timestamp queries: record slot `slot` (0-63), then read the ms between two slots. */
+  timestamp(slot) {
+    native.ctx_timestamp(this.handle, slot);
+  }
+  elapsed(fromSlot, toSlot) {
+    return native.ctx_elapsed(this.handle, fromSlot, toSlot);
   }
   onSubmittedWorkDone() {
     this.synchronize();

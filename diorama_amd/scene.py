@@ -124,3 +124,32 @@ def ring_cameras(count, width=980, height=545, fov_deg=50.0, radius=6.0, seed=3)
         proj = perspective(math.radians(fov_deg), width / height, 0.01, 100.0)
         cams.append(camera(view, proj, width, height))
     return cams
+
+
+# Scan/sort microbenchmark inputs (shared bit-for-bit by the tests and the bench).
+
+def tile_touch_counts(n, seed):
+    """u32 tile-touch counts ~ Geometric(mean 6) capped at 256."""
+    rng = np.random.default_rng(seed)
+    return np.minimum(rng.geometric(1 / 6, size=n), 256).astype(np.uint32)
+
+
+def zipf_tiles(n, seed, tiles=62 * 35, s=1.1):
+    """Tile ids ~ Zipf(s) over `tiles` (the 62x35 tiles of 980x545), ranks shuffled."""
+    rng = np.random.default_rng(seed)
+    p = 1.0 / np.arange(1, tiles + 1) ** s
+    p /= p.sum()
+    perm = rng.permutation(tiles)
+    return perm[rng.choice(tiles, size=n, p=p)].astype(np.uint32)
+
+
+def tile_depth_keys(n, seed):
+    """64-bit (low = float32 bits of depth ~ U[0.2, 50], high = Zipf tile) keys as
+    interleaved u32 words, plus the two words separately."""
+    rng = np.random.default_rng(seed)
+    hi = zipf_tiles(n, seed)
+    lo = rng.uniform(0.2, 50.0, size=n).astype(np.float32).view(np.uint32)
+    keys = np.empty(2 * n, dtype=np.uint32)
+    keys[0::2] = lo
+    keys[1::2] = hi
+    return keys, lo, hi

@@ -63,12 +63,20 @@ const char* dio_version(void);
 int dio_ctx_create(int device, dio_ctx** out);
 int dio_ctx_synchronize(dio_ctx* ctx);
 void dio_ctx_destroy(dio_ctx* ctx);
+/* Timestamp queries (the 'timestamp-query' feature, src/main.ts:40): record a
+ * marker in one of DIO_TIMESTAMP_SLOTS slots on the context's stream, then read
+ * the milliseconds between two recorded slots (waits for the later one). */
+#define DIO_TIMESTAMP_SLOTS 64
+int dio_ctx_timestamp(dio_ctx* ctx, uint32_t slot);
+int dio_ctx_elapsed(dio_ctx* ctx, uint32_t from_slot, uint32_t to_slot, double* ms);
 
 /* Device buffers (GPUBuffer + queue.writeBuffer / readBuffer, src/readback.ts). */
 int dio_buf_create(dio_ctx* ctx, size_t bytes, dio_buf** out);
 int dio_buf_upload(dio_buf* buf, size_t offset, const void* src, size_t bytes);
 int dio_buf_download(const dio_buf* buf, size_t offset, void* dst, size_t bytes);
 int dio_buf_clear(dio_buf* buf); /* encoder.clearBuffer */
+/* encoder.copyBufferToBuffer (src/readback.ts:11), enqueued on dst's stream. */
+int dio_buf_copy(dio_buf* dst, size_t dst_offset, const dio_buf* src, size_t src_offset, size_t bytes);
 size_t dio_buf_size(const dio_buf* buf);
 void dio_buf_destroy(dio_buf* buf);
 

@@ -40,8 +40,7 @@ def run_scan(values, capacity=None, count=None):
 
 
 def geometric_counts(n, seed):
-    rng = np.random.default_rng(seed)
-    return np.minimum(rng.geometric(1 / 6, size=n), 256).astype(np.uint32)
+    return scene.tile_touch_counts(n, seed)
 
 
 @pytest.mark.parametrize("n", [0, 1, 511, 512, 4095, 4096, 4097, 100_003])
@@ -110,25 +109,11 @@ def test_sort_32bit_depth_sort_check_case(n):
     assert np.array_equal(k, keys[order])
 
 
-def zipf_tiles(n, seed, tiles=62 * 35, s=1.1):
-    rng = np.random.default_rng(seed)
-    p = 1.0 / np.arange(1, tiles + 1) ** s
-    p /= p.sum()
-    perm = rng.permutation(tiles)
-    return perm[rng.choice(tiles, size=n, p=p)].astype(np.uint32)
-
-
 def microbench_keys(n, seed):
-    rng = np.random.default_rng(seed)
-    hi = zipf_tiles(n, seed)
-    lo = rng.uniform(0.2, 50.0, size=n).astype(np.float32).view(np.uint32)
-    keys = np.empty(2 * n, dtype=np.uint32)
-    keys[0::2] = lo
-    keys[1::2] = hi
-    return keys, lo, hi
+    return scene.tile_depth_keys(n, seed)
 
 
-@pytest.mark.parametrize("n", [100_003, 8 << 20])
+@pytest.mark.parametrize("n", [100_003, 8 << 20, 30_000_000])
 def test_sort_64bit_microbench_keys(n):
     keys, lo, hi = microbench_keys(n, 7)
     values = np.random.default_rng(8).permutation(n).astype(np.uint32)  # arbitrary payloads
@@ -163,3 +148,21 @@ def test_sort_count_from_device():
     order = np.argsort(keys[:count], kind="stable")
     assert np.array_equal(v[:count], order.astype(np.uint32))
     assert np.array_equal(k[:count], keys[:count][order])
+
+
+def test_buffer_copy_and_timestamps():
+    """copyBufferToBuffer with offsets, and timestamp queries around it."""
+    ctx = context()
+    src = ctx.upload(np.arange(1000, dtype=np.uint32))
+    dst = ctx.upload(np.zeros(1000, dtype=np.uint32))
+    say("buffer copy")
+    ctx.timestamp(0)
+    dst.copy_from(src, nbytes=400, dst_offset=40, src_offset=800)
+    ctx.timestamp(1)
+    ms = ctx.elapsed_ms(0, 1)
+    out = dst.download(np.uint32)
+    assert np.array_equal(out[10:110], np.arange(200, 300))
+    assert not out[:10].any() and not out[110:].any()
+    assert 0 <= ms < 1000
+    with pytest.raises(diorama_amd.DioramaError):
+        dst.copy_from(src, nbytes=4000, dst_offset=4)
