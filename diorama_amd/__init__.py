@@ -123,6 +123,9 @@ SIGNATURES = {
     "dio_default_settings": (None, [ctypes.c_float, ctypes.POINTER(_Settings)]),
     "dio_trainer_create": (_I, [_P, _P, _P, _U32, ctypes.POINTER(_Camera), ctypes.POINTER(_P), _U32,
                                 ctypes.POINTER(_Settings), ctypes.POINTER(_P)]),
+    "dio_trainer_create_batched": (_I, [_P, _P, _P, _U32, ctypes.POINTER(_Camera), ctypes.POINTER(_P), _U32, _U32,
+                                        ctypes.POINTER(_Settings), ctypes.POINTER(_P)]),
+    "dio_trainer_batch_size": (_U32, [_P]),
     "dio_trainer_step": (_I, [_P]),
     "dio_trainer_optimize": (_I, [_P]),
     "dio_trainer_maintain": (_I, [_P]),
@@ -518,9 +521,12 @@ def _images(images):
 
 
 class Trainer:
-    """src/trainer.ts:103-446: one training view per step, plain 3DGS schedule."""
+    """src/trainer.ts:103-446: one training view per step, plain 3DGS schedule.
 
-    def __init__(self, ctx, params, sh, cameras, images, settings):
+    With `batch` > 0 it is a multi-view batch trainer: `batch` views of any sizes
+    per step, their gradients summed into one Adam step (one captured HIP graph)."""
+
+    def __init__(self, ctx, params, sh, cameras, images, settings, batch=0):
         p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
         s = np.ascontiguousarray(sh, dtype=np.float32).reshape(-1)
         n = p.size // 16
@@ -528,9 +534,17 @@ class Trainer:
         arrs, ptrs = _images(images)
         st = _settings(settings)
         h = _P()
-        _check(lib().dio_trainer_create(ctx.handle, p.ctypes.data_as(_P), s.ctypes.data_as(_P), n, cams, ptrs,
-                                        len(cameras), ctypes.byref(st), ctypes.byref(h)))
+        if batch:
+            _check(lib().dio_trainer_create_batched(ctx.handle, p.ctypes.data_as(_P), s.ctypes.data_as(_P), n, cams,
+                                                    ptrs, len(cameras), batch, ctypes.byref(st), ctypes.byref(h)))
+        else:
+            _check(lib().dio_trainer_create(ctx.handle, p.ctypes.data_as(_P), s.ctypes.data_as(_P), n, cams, ptrs,
+                                            len(cameras), ctypes.byref(st), ctypes.byref(h)))
         self.handle, self.ctx = h, ctx
+
+    @property
+    def batch_size(self):
+        return lib().dio_trainer_batch_size(self.handle)
 
     def step(self):
         _check(lib().dio_trainer_step(self.handle))

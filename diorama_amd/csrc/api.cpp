@@ -952,6 +952,44 @@ double ssim4(const std::vector<double>& a, const std::vector<double>& b, int w, 
 
 }  // namespace
 
+// The device state of a multi-view batch trainer (no counterpart in the
+// reference): each step takes `views` training views of any sizes, renders,
+// scores and back-propagates them in the same launches (batch.hip) and sums
+// their gradients into one Adam step. The whole step is one captured HIP graph;
+// the views, their cameras and the Adam arguments sit in device memory and are
+// rewritten before each replay.
+struct BatchState {
+  uint32_t views = 0;                   // views per step
+  uint32_t n = 0;                       // Gaussians the buffers hold
+  uint32_t max_tiles = 0, max_pixels = 0;
+  uint32_t capacity = 0;                // (splat, tile) pairs
+  dio_buf *desc = nullptr, *adam = nullptr;
+  dio_buf *splats = nullptr, *tile_counts = nullptr, *depth_keys = nullptr, *depth_values = nullptr,
+          *depth_alt_keys = nullptr, *depth_alt_values = nullptr, *offsets = nullptr, *scan_ws = nullptr,
+          *sort_ws = nullptr;
+  dio_buf *pair_total = nullptr, *pair_tiles = nullptr, *pair_splats = nullptr, *alt_tiles = nullptr,
+          *alt_splats = nullptr, *pair_sort_ws = nullptr, *ranges = nullptr;
+  dio_buf *pixel_color = nullptr, *pixel_last = nullptr, *partials = nullptr, *pixel_grads = nullptr,
+          *pixel_loss = nullptr;
+  dio_buf *splat_grads = nullptr, *grad_gaussians = nullptr, *grad_sh = nullptr;
+  char* host = nullptr;            // pinned staging: header, views, Adam arguments
+  uint32_t* readback = nullptr;    // pinned: the last step's pair count
+  hipEvent_t uploaded = nullptr, read = nullptr;
+  bool upload_pending = false, read_pending = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t graph_degree = ~0u, graph_generation = ~0u, frames_generation = ~0u;
+  bool sorted_in_alt = false;
+  std::vector<uint32_t> step_views;  // training views of the last step
+  uint32_t step_pixels = 0;
+  uint32_t last_pairs = 0;
+};
+
+constexpr size_t kBatchViewsOffset = 256;
+constexpr size_t kBatchDescBytes = kBatchViewsOffset + dio::kMaxBatchViews * sizeof(dio::ViewDesc);
+constexpr size_t kBatchHostBytes = kBatchDescBytes + sizeof(dio::AdamArgs);
+
+
 struct dio_trainer {
   dio_ctx* ctx = nullptr;
   dio_training_settings settings{};
@@ -971,13 +1009,43 @@ struct dio_trainer {
   bool due_densify = false, due_reset = false;
   dio::AdamArgs last_adam{};
   uint32_t last_view = 0;
+  BatchState* batch = nullptr;  // set for a multi-view batch trainer
+  uint32_t generation = 0;      // bumped whenever densification replaces the buffers
 };
 
 namespace {
 
+void batch_free_frames(BatchState* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (dio_buf** p : {&b->splats, &b->tile_counts, &b->depth_keys, &b->depth_values, &b->depth_alt_keys,
+                      &b->depth_alt_values, &b->offsets, &b->scan_ws, &b->sort_ws, &b->pair_total, &b->pair_tiles,
+                      &b->pair_splats, &b->alt_tiles, &b->alt_splats, &b->pair_sort_ws, &b->splat_grads,
+                      &b->grad_gaussians, &b->grad_sh})
+    release(*p);
+  b->n = 0;
+  b->capacity = 0;
+}
+
+void batch_destroy(BatchState* b) {
+  if (!b) return;
+  batch_free_frames(b);
+  for (dio_buf** p : {&b->desc, &b->adam, &b->ranges, &b->pixel_color, &b->pixel_last, &b->partials,
+                      &b->pixel_grads, &b->pixel_loss})
+    release(*p);
+  if (b->host) (void)hipHostFree(b->host);
+  if (b->readback) (void)hipHostFree(b->readback);
+  if (b->uploaded) (void)hipEventDestroy(b->uploaded);
+  if (b->read) (void)hipEventDestroy(b->read);
+  delete b;
+}
+
 void destroy_trainer(dio_trainer* t) {
   if (!t) return;
   (void)hipStreamSynchronize(t->ctx->stream);
+  batch_destroy(t->batch);
   dio_renderer_destroy(t->renderer);
   destroy_loss(t->loss);
   release(t->params); release(t->moments); release(t->stats);
@@ -998,7 +1066,13 @@ const dio_trainer::View& next_view(dio_trainer* t) {
   return t->views[t->last_view];
 }
 
+void batch_optimize(dio_trainer* t);
+
 void trainer_optimize(dio_trainer* t) {
+  if (t->batch) {
+    batch_optimize(t);
+    return;
+  }
   const uint32_t step = ++t->iteration;
   const auto& st = t->settings;
   if (st.sh_degree_interval && step % st.sh_degree_interval == 0 && t->sh_degree < 3) {
@@ -1102,6 +1176,231 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
   t->moments = moments;
   t->stats = stats;
   t->count = new_count;
+  t->generation++;
+}
+
+// ------------------------------------------------------ multi-view batches --
+
+// The tile space and pixel buffers any `views` of the training views fit in:
+// the sums of the largest tile and pixel counts.
+void batch_limits(dio_trainer* t, BatchState* b) {
+  std::vector<uint32_t> tiles, pixels;
+  for (const auto& v : t->views) {
+    const uint32_t w = static_cast<uint32_t>(v.camera.viewport[0]), h = static_cast<uint32_t>(v.camera.viewport[1]);
+    tiles.push_back(dio::ceil_div(w, dio::kTile) * dio::ceil_div(h, dio::kTile));
+    pixels.push_back(w * h);
+  }
+  std::sort(tiles.rbegin(), tiles.rend());
+  std::sort(pixels.rbegin(), pixels.rend());
+  uint64_t tile_sum = 0, pixel_sum = 0;
+  for (uint32_t k = 0; k < b->views; k++) {
+    tile_sum += tiles[k];
+    pixel_sum += pixels[k];
+  }
+  if (pixel_sum >= (1ull << 31)) invalid("the batch's views hold too many pixels");
+  b->max_tiles = static_cast<uint32_t>(tile_sum);
+  b->max_pixels = static_cast<uint32_t>(pixel_sum);
+}
+
+void batch_alloc_frames(dio_trainer* t, uint32_t capacity) {
+  BatchState* b = t->batch;
+  dio_ctx* ctx = t->ctx;
+  const uint32_t n = t->count;
+  const uint64_t total = static_cast<uint64_t>(n) * b->views;
+  if (total >= (1ull << 30)) invalid("a batch holds at most 2^30 splats (views x Gaussians)");
+  b->splats = alloc(ctx, total * dio::kFloatsPerSplat * 4, true);
+  b->tile_counts = alloc(ctx, total * 4, true);
+  b->depth_keys = alloc(ctx, total * 4);
+  b->depth_values = alloc(ctx, total * 4);
+  b->depth_alt_keys = alloc(ctx, total * 4);
+  b->depth_alt_values = alloc(ctx, total * 4);
+  b->offsets = alloc(ctx, total * 4);
+  b->scan_ws = alloc(ctx, dio::scan_workspace_bytes(static_cast<uint32_t>(total)), true);
+  b->sort_ws = alloc(ctx, dio::sort_workspace_bytes(static_cast<uint32_t>(total)), true);
+  b->pair_total = alloc(ctx, 16, true);
+  b->pair_tiles = alloc(ctx, static_cast<size_t>(capacity) * 4);
+  b->pair_splats = alloc(ctx, static_cast<size_t>(capacity) * 4, true);
+  b->alt_tiles = alloc(ctx, static_cast<size_t>(capacity) * 4);
+  b->alt_splats = alloc(ctx, static_cast<size_t>(capacity) * 4, true);
+  b->pair_sort_ws = alloc(ctx, dio::sort_workspace_bytes(capacity), true);
+  b->splat_grads = alloc(ctx, total * 9 * 4, true);
+  b->grad_gaussians = alloc(ctx, static_cast<size_t>(n) * dio::kFloatsPerGaussian * 4, true);
+  b->grad_sh = alloc(ctx, static_cast<size_t>(n) * dio::kShRest * 4, true);
+  b->n = n;
+  b->capacity = capacity;
+  b->frames_generation = t->generation;
+}
+
+dio::BatchArgs batch_args(const BatchState* b) {
+  dio::BatchArgs a;
+  a.header = static_cast<const dio::BatchHeader*>(b->desc->ptr);
+  a.views = reinterpret_cast<const dio::ViewDesc*>(static_cast<const char*>(b->desc->ptr) + kBatchViewsOffset);
+  a.view_count = b->views;
+  a.per_view = b->n;
+  a.max_tiles = b->max_tiles;
+  return a;
+}
+
+// One batch step's launches in stream order: the body of the captured graph.
+void batch_launch(dio_trainer* t, bool with_adam) {
+  BatchState* b = t->batch;
+  hipStream_t s = t->ctx->stream;
+  auto* sc = t->renderer->scene;
+  const dio::BatchArgs a = batch_args(b);
+  const uint32_t total = b->views * b->n;
+  const float* bg = t->settings.background;
+  dio::preprocess_batch(s, dev<float>(sc->gaussians), dev<float>(sc->sh), t->sh_degree, a, dev<float>(b->splats),
+                        dev<uint32_t>(b->tile_counts), dev<uint32_t>(b->depth_keys), dev<uint32_t>(b->depth_values));
+  // One depth order over every view's splats: the stable tile sort below keeps
+  // each tile's pairs front to back, and tiles of different views never mix.
+  if (dio::radix_sort(s, false, b->depth_keys->ptr, dev<uint32_t>(b->depth_values), b->depth_alt_keys->ptr,
+                      dev<uint32_t>(b->depth_alt_values), nullptr, total, 0, 32,
+                      dio::sort_workspace(b->sort_ws->ptr, total)))
+    throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
+  dio::scan_exclusive_gather(s, dev<uint32_t>(b->tile_counts), dev<uint32_t>(b->depth_values),
+                             dev<uint32_t>(b->offsets), nullptr, total, dev<uint32_t>(b->pair_total),
+                             dio::scan_workspace(b->scan_ws->ptr, total));
+  dio::duplicate_batch(s, dev<float>(b->splats), dev<uint32_t>(b->depth_values), dev<uint32_t>(b->offsets),
+                       dev<uint32_t>(b->tile_counts), a, b->capacity, dev<uint32_t>(b->pair_total),
+                       dev<uint32_t>(b->pair_tiles), dev<uint32_t>(b->pair_splats));
+  b->sorted_in_alt = dio::radix_sort(s, false, b->pair_tiles->ptr, dev<uint32_t>(b->pair_splats), b->alt_tiles->ptr,
+                                     dev<uint32_t>(b->alt_splats), dev<uint32_t>(b->pair_total), b->capacity, 0,
+                                     tile_sort_bits(b->max_tiles),
+                                     dio::sort_workspace(b->pair_sort_ws->ptr, b->capacity));
+  dio_buf* sorted_tiles = b->sorted_in_alt ? b->alt_tiles : b->pair_tiles;
+  dio_buf* sorted_splats = b->sorted_in_alt ? b->alt_splats : b->pair_splats;
+  dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(b->pair_total), b->capacity, b->max_tiles,
+                   dev<uint32_t>(b->ranges));
+  dio::rasterize_batch(s, dev<float>(b->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(b->ranges), a, bg,
+                       dev<float>(b->pixel_color), dev<uint32_t>(b->pixel_last));
+  dio::training_loss_batch(s, a, dev<float>(b->pixel_color), dev<float>(b->partials), dev<float>(b->pixel_grads),
+                           dev<float>(b->pixel_loss));
+  dio::fill_u32(s, dev<uint32_t>(b->splat_grads), 0u, static_cast<size_t>(total) * 9);  // the backward adds into it
+  dio::rasterize_backward_batch(s, dev<float>(b->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(b->ranges), a,
+                                bg, dev<float>(b->pixel_color), dev<uint32_t>(b->pixel_last),
+                                dev<float>(b->pixel_grads), dev<float>(b->splat_grads));
+  dio::preprocess_backward_batch(s, dev<float>(sc->gaussians), dev<float>(sc->sh), t->sh_degree, a,
+                                 dev<float>(b->splats), dev<float>(b->splat_grads), dev<float>(b->grad_gaussians),
+                                 dev<float>(b->grad_sh), dev<float>(t->stats));
+  if (with_adam)
+    dio::adam_step_device_args(s, b->n, dev<float>(t->params), dev<float>(sc->sh), dev<float>(t->moments),
+                               dev<float>(b->grad_gaussians), dev<float>(b->grad_sh),
+                               static_cast<const dio::AdamArgs*>(b->adam->ptr), dev<float>(sc->gaussians));
+}
+
+void batch_capture(dio_trainer* t) {
+  BatchState* b = t->batch;
+  hipStream_t s = t->ctx->stream;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  DIO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    batch_launch(t, true);
+  } catch (...) {
+    hipGraph_t broken = nullptr;
+    (void)hipStreamEndCapture(s, &broken);
+    if (broken) (void)hipGraphDestroy(broken);
+    throw;
+  }
+  DIO_HIP(hipStreamEndCapture(s, &b->graph));
+  DIO_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_degree = t->sh_degree;
+  b->graph_generation = t->generation;
+}
+
+// One optimize() step of a batch trainer: the plain schedule (src/trainer.ts:192-244)
+// with `views` training views per step and their gradients summed.
+void batch_optimize(dio_trainer* t) {
+  BatchState* b = t->batch;
+  hipStream_t s = t->ctx->stream;
+  const uint32_t step = ++t->iteration;
+  const auto& st = t->settings;
+  if (st.sh_degree_interval && step % st.sh_degree_interval == 0 && t->sh_degree < 3) {
+    t->sh_degree++;
+    if (t->renderer->scene) t->renderer->scene->degree = t->sh_degree;
+  }
+  const bool densifying = step < st.densify_until;
+  const bool densify_now = densifying && step > st.densify_from && st.densify_interval &&
+                           step % st.densify_interval == 0;
+  // The last step's pair count: grow the pair buffers after an overflow.
+  uint32_t capacity = b->capacity;
+  if (b->read_pending) {
+    DIO_HIP(hipEventSynchronize(b->read));
+    b->read_pending = false;
+    b->last_pairs = b->readback[0];
+    if (b->last_pairs > b->capacity) capacity = next_pow2_capacity(b->last_pairs);
+  }
+  if (b->n != t->count || capacity != b->capacity || b->frames_generation != t->generation) {
+    sync(t->ctx);
+    batch_free_frames(b);
+    batch_alloc_frames(t, capacity);
+  }
+  if (b->upload_pending) DIO_HIP(hipEventSynchronize(b->uploaded));  // the staging memory is free again
+  auto* header = reinterpret_cast<dio::BatchHeader*>(b->host);
+  auto* views = reinterpret_cast<dio::ViewDesc*>(b->host + kBatchViewsOffset);
+  uint32_t tile_base = 0, pixel_base = 0;
+  b->step_views.clear();
+  for (uint32_t k = 0; k < b->views; k++) {
+    const auto& view = next_view(t);
+    b->step_views.push_back(t->last_view);
+    dio::ViewDesc& d = views[k];
+    d = dio::ViewDesc{};
+    std::memcpy(d.cam.view, view.camera.view, sizeof(d.cam.view));
+    std::memcpy(d.cam.proj, view.camera.proj, sizeof(d.cam.proj));
+    d.cam.viewport[0] = view.camera.viewport[0];
+    d.cam.viewport[1] = view.camera.viewport[1];
+    d.width = static_cast<uint32_t>(view.camera.viewport[0]);
+    d.height = static_cast<uint32_t>(view.camera.viewport[1]);
+    d.tiles_x = dio::ceil_div(d.width, dio::kTile);
+    d.tiles_y = dio::ceil_div(d.height, dio::kTile);
+    d.tile_base = tile_base;
+    d.pixel_base = pixel_base;
+    d.loss_scale = static_cast<float>(1.0 / (3.0 * static_cast<double>(d.width) * d.height));
+    d.target = dev<uint32_t>(view.image);
+    tile_base += d.tiles_x * d.tiles_y;
+    pixel_base += d.width * d.height;
+  }
+  header->views = b->views;
+  header->tiles = tile_base;
+  header->pixels = pixel_base;
+  header->pad = 0;
+  if (!densify_now) t->adam_steps++;
+  const double progress = std::min(1.0, static_cast<double>(step) / st.position_lr_steps);
+  const double position_lr =
+      std::exp(std::log(static_cast<double>(st.position_lr[0])) * (1 - progress) +
+               std::log(static_cast<double>(st.position_lr[1])) * progress) * st.extent;
+  dio::AdamArgs a;
+  a.bias_correction1 = static_cast<float>(1 - std::pow(0.9, t->adam_steps));
+  a.bias_correction2 = static_cast<float>(1 - std::pow(0.999, t->adam_steps));
+  a.inv_bias_correction1 = static_cast<float>(1 / (1 - std::pow(0.9, t->adam_steps)));
+  a.inv_bias_correction2 = static_cast<float>(1 / (1 - std::pow(0.999, t->adam_steps)));
+  a.position_lr = static_cast<float>(position_lr);
+  a.opacity_lr = st.opacity_lr;
+  a.scale_lr = st.scale_lr;
+  a.rotation_lr = st.rotation_lr;
+  a.color_lr = st.color_lr;
+  a.sh_lr = st.sh_lr;
+  a.viewport[0] = a.viewport[1] = 0.f;  // statistics use each view's own viewport
+  std::memcpy(b->host + kBatchDescBytes, &a, sizeof(a));
+  t->last_adam = a;
+  DIO_HIP(hipMemcpyAsync(b->desc->ptr, b->host, kBatchDescBytes, hipMemcpyHostToDevice, s));
+  DIO_HIP(hipMemcpyAsync(b->adam->ptr, b->host + kBatchDescBytes, sizeof(a), hipMemcpyHostToDevice, s));
+  DIO_HIP(hipEventRecord(b->uploaded, s));
+  b->upload_pending = true;
+  if (densify_now) {
+    batch_launch(t, false);  // densification steps skip Adam (src/trainer.ts:236-239)
+  } else {
+    if (!b->exec || b->graph_degree != t->sh_degree || b->graph_generation != t->generation) batch_capture(t);
+    DIO_HIP(hipGraphLaunch(b->exec, s));
+  }
+  DIO_HIP(hipMemcpyAsync(b->readback, b->pair_total->ptr, 4, hipMemcpyDeviceToHost, s));
+  DIO_HIP(hipEventRecord(b->read, s));
+  b->read_pending = true;
+  b->step_pixels = pixel_base;
+  t->due_densify = densify_now;
+  t->due_reset = densifying && st.opacity_reset_interval && step % st.opacity_reset_interval == 0;
 }
 
 void trainer_maintain(dio_trainer* t) {
@@ -1137,10 +1436,15 @@ extern "C" void dio_default_settings(float extent, dio_training_settings* out) {
   *out = s;
 }
 
-extern "C" int dio_trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t count,
-                                  const dio_camera* cameras, const uint32_t* const* images, uint32_t view_count,
-                                  const dio_training_settings* settings, dio_trainer** out) {
-  return guard([&] {
+namespace {
+
+// The trainer both creation calls share. `batch` 0: the plain trainer, whose views
+// must share one size (src/trainer.ts:133-135); otherwise a batch trainer taking
+// `batch` views of any sizes per step.
+void trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t count, const dio_camera* cameras,
+                    const uint32_t* const* images, uint32_t view_count, uint32_t batch,
+                    const dio_training_settings* settings, dio_trainer** out) {
+  {
     need(ctx, "ctx");
     need(params, "params");
     need(sh, "sh");
@@ -1151,9 +1455,13 @@ extern "C" int dio_trainer_create(dio_ctx* ctx, const float* params, const float
     if (view_count == 0) invalid("Training needs at least one view.");
     if (count == 0) invalid("Training needs at least one Gaussian.");
     const float w = cameras[0].viewport[0], h = cameras[0].viewport[1];
-    for (uint32_t v = 0; v < view_count; v++)
-      if (cameras[v].viewport[0] != w || cameras[v].viewport[1] != h)
-        invalid("All training views must be the same size.");
+    if (batch == 0) {
+      for (uint32_t v = 0; v < view_count; v++)
+        if (cameras[v].viewport[0] != w || cameras[v].viewport[1] != h)
+          invalid("All training views must be the same size.");
+    } else if (batch > view_count || batch > dio::kMaxBatchViews) {
+      invalid("a batch takes between 1 and min(views, " + std::to_string(dio::kMaxBatchViews) + ") views");
+    }
     auto* t = new dio_trainer;
     try {
       t->ctx = ctx;
@@ -1161,9 +1469,9 @@ extern "C" int dio_trainer_create(dio_ctx* ctx, const float* params, const float
       t->count = count;
       t->width = static_cast<uint32_t>(w);
       t->height = static_cast<uint32_t>(h);
-      const size_t px = static_cast<size_t>(t->width) * t->height;
       for (uint32_t v = 0; v < view_count; v++) {
         need(images[v], "image");
+        const size_t px = static_cast<size_t>(cameras[v].viewport[0]) * static_cast<size_t>(cameras[v].viewport[1]);
         dio_buf* img = alloc(ctx, px * 4);
         t->views.push_back({cameras[v], img});
         DIO_HIP(hipMemcpyAsync(img->ptr, images[v], px * 4, hipMemcpyHostToDevice, ctx->stream));
@@ -1190,14 +1498,57 @@ extern "C" int dio_trainer_create(dio_ctx* ctx, const float* params, const float
       dio_training_loss* l = nullptr;
       if (dio_training_loss_create(ctx, t->width, t->height, &l) != DIO_OK) throw Error(DIO_ERR_MEMORY, g_error);
       t->loss = l;
+      if (batch) {
+        t->batch = new BatchState;
+        BatchState* b = t->batch;
+        b->views = batch;
+        batch_limits(t, b);
+        b->desc = alloc(ctx, kBatchDescBytes, true);
+        b->adam = alloc(ctx, sizeof(dio::AdamArgs), true);
+        b->ranges = alloc(ctx, static_cast<size_t>(b->max_tiles) * 8, true);
+        b->pixel_color = alloc(ctx, static_cast<size_t>(b->max_pixels) * 16, true);
+        b->pixel_last = alloc(ctx, static_cast<size_t>(b->max_pixels) * 4, true);
+        b->partials = alloc(ctx, static_cast<size_t>(b->max_pixels) * 48);
+        b->pixel_grads = alloc(ctx, static_cast<size_t>(b->max_pixels) * 16, true);
+        b->pixel_loss = alloc(ctx, static_cast<size_t>(b->max_pixels) * 4, true);
+        DIO_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->host), kBatchHostBytes, hipHostMallocDefault));
+        std::memset(b->host, 0, kBatchHostBytes);
+        DIO_HIP(hipHostMalloc(reinterpret_cast<void**>(&b->readback), 16, hipHostMallocDefault));
+        DIO_HIP(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming));
+        DIO_HIP(hipEventCreateWithFlags(&b->read, hipEventDisableTiming));
+        // Room for two pairs per splat to start with; an overflowing step drops
+        // its farthest pairs and the buffers grow for the next one.
+        const uint64_t guess = std::max<uint64_t>(1u << 20, 2ull * count * batch);
+        batch_alloc_frames(t, next_pow2_capacity(static_cast<uint32_t>(std::min<uint64_t>(guess, 1u << 30))));
+      }
       sync(ctx);
     } catch (...) {
       destroy_trainer(t);
       throw;
     }
     *out = t;
+  }
+}
+
+}  // namespace
+
+extern "C" int dio_trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t count,
+                                  const dio_camera* cameras, const uint32_t* const* images, uint32_t view_count,
+                                  const dio_training_settings* settings, dio_trainer** out) {
+  return guard([&] { trainer_create(ctx, params, sh, count, cameras, images, view_count, 0, settings, out); });
+}
+
+extern "C" int dio_trainer_create_batched(dio_ctx* ctx, const float* params, const float* sh, uint32_t count,
+                                          const dio_camera* cameras, const uint32_t* const* images,
+                                          uint32_t view_count, uint32_t batch, const dio_training_settings* settings,
+                                          dio_trainer** out) {
+  return guard([&] {
+    if (batch == 0) invalid("a batch takes at least one view");
+    trainer_create(ctx, params, sh, count, cameras, images, view_count, batch, settings, out);
   });
 }
+
+extern "C" uint32_t dio_trainer_batch_size(const dio_trainer* t) { return t && t->batch ? t->batch->views : 0; }
 
 extern "C" int dio_trainer_step(dio_trainer* t) {
   return guard([&] {
@@ -1251,7 +1602,20 @@ extern "C" int dio_trainer_last_loss(dio_trainer* t, double* out) {
   return guard([&] {
     need(t, "trainer");
     need(out, "out");
-    *out = loss_read(t->loss);
+    if (!t->batch) {
+      *out = loss_read(t->loss);
+      return;
+    }
+    // The mean of the batch's per-view losses, each read() of its own view.
+    BatchState* b = t->batch;
+    std::vector<float> shares(b->step_pixels);
+    if (!shares.empty())
+      DIO_HIP(hipMemcpyAsync(shares.data(), b->pixel_loss->ptr, shares.size() * 4, hipMemcpyDeviceToHost,
+                             t->ctx->stream));
+    sync(t->ctx);
+    double sum = 0;
+    for (float v : shares) sum += v;
+    *out = sum / b->views + 0.2;
   });
 }
 
@@ -1324,11 +1688,32 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
     need(stage, "stage");
     need(ms, "ms");
     if (repeats == 0) invalid("repeats must be positive");
+    const std::string name(stage);
+    if (t->batch) {
+      // A batch trainer times its whole captured step ("batch_step").
+      BatchState* b = t->batch;
+      if (name != "batch_step") invalid("a batch trainer times only the stage batch_step");
+      if (!b->exec) invalid("run an optimize() step before timing a stage");
+      hipStream_t s = t->ctx->stream;
+      hipEvent_t e0, e1;
+      DIO_HIP(hipEventCreate(&e0));
+      DIO_HIP(hipEventCreate(&e1));
+      DIO_HIP(hipGraphLaunch(b->exec, s));  // warm
+      DIO_HIP(hipEventRecord(e0, s));
+      for (uint32_t k = 0; k < repeats; k++) DIO_HIP(hipGraphLaunch(b->exec, s));
+      DIO_HIP(hipEventRecord(e1, s));
+      DIO_HIP(hipEventSynchronize(e1));
+      float elapsed = 0.f;
+      DIO_HIP(hipEventElapsedTime(&elapsed, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      *ms = elapsed / repeats;
+      return;
+    }
     auto* r = t->renderer;
     auto* sc = r->scene;
     auto* tr = sc->rasterizer;
     if (!tr->framed || !sc->grad_splats) invalid("run an optimize() step before timing a stage");
-    const std::string name(stage);
     hipStream_t s = t->ctx->stream;
     const auto& view = t->views[t->last_view];
     dio_buf* sorted_splats = tr->sorted_in_alt ? tr->alt_splats : tr->pair_splats;

@@ -103,6 +103,54 @@ void preprocess_backward(hipStream_t s, const float* gaussians, const float* sh,
                          const CameraArgs& cam, const float* splats, const float* splat_grads,
                          float* gaussian_grads, float* sh_grads);
 
+// ------------------------------------------------------ multi-view batch --
+// A batch renders several views of one Gaussian set in the same launches: view v's
+// splats sit at [v N, (v + 1) N), its 16x16 tiles at [tile_base, tile_base +
+// tiles_x tiles_y) of the batch's tile space and its pixels at pixel_base of the
+// batch's pixel buffers. The header and the views live in device memory, so a
+// captured graph replays with other views by rewriting them.
+constexpr uint32_t kMaxBatchViews = 64;
+
+struct ViewDesc {
+  CameraArgs cam;
+  uint32_t width, height, tiles_x, tiles_y;
+  uint32_t tile_base, pixel_base;
+  float loss_scale;  // 1 / (3 width height)
+  uint32_t pad;
+  const uint32_t* target;  // the view's rgba8 photo
+};
+
+struct BatchHeader {
+  uint32_t views, tiles, pixels, pad;
+};
+
+struct BatchArgs {
+  const BatchHeader* header;
+  const ViewDesc* views;
+  uint32_t view_count;  // the batch's views (fixed while a graph is replayed)
+  uint32_t per_view;    // N: Gaussians per view
+  uint32_t max_tiles;   // the tile space the launches cover
+};
+
+void preprocess_batch(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, const BatchArgs& b,
+                      float* splats, uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values);
+void duplicate_batch(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
+                     const uint32_t* tile_counts, const BatchArgs& b, uint32_t capacity, const uint32_t* pair_total,
+                     uint32_t* pair_tiles, uint32_t* pair_splats);
+void rasterize_batch(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
+                     const BatchArgs& b, const float* background, float* pixel_color, uint32_t* pixel_last);
+void training_loss_batch(hipStream_t s, const BatchArgs& b, const float* rendered, float* partials,
+                         float* pixel_grads, float* pixel_loss);
+void rasterize_backward_batch(hipStream_t s, const float* splats, const uint32_t* pair_splats,
+                              const uint32_t* ranges, const BatchArgs& b, const float* background,
+                              const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
+                              float* splat_grads);
+// Gradients summed over the batch's views, and the densification statistics of
+// every view (adam.wgsl:125-134) in the same pass.
+void preprocess_backward_batch(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree,
+                               const BatchArgs& b, const float* splats, const float* splat_grads,
+                               float* gaussian_grads, float* sh_grads, float* stats);
+
 // ---------------------------------------------------------------- loss --
 void training_loss(hipStream_t s, uint32_t width, uint32_t height, const float* rendered, const uint32_t* target,
                    float* partials, float* pixel_grads, float* pixel_loss);
@@ -118,6 +166,9 @@ void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const floa
                       const AdamArgs& a, float* stats);
 void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* moments, const float* gaussian_grads,
                const float* sh_grads, const AdamArgs& a, float* gaussians);
+// The same step with its arguments read from device memory (for graph replays).
+void adam_step_device_args(hipStream_t s, uint32_t n, float* params, float* sh, float* moments,
+                           const float* gaussian_grads, const float* sh_grads, const AdamArgs* a, float* gaussians);
 void activate(hipStream_t s, uint32_t n, const float* params, float* gaussians);
 void reset_opacity(hipStream_t s, uint32_t n, float* params, float* moments, float* gaussians);
 

@@ -43,11 +43,15 @@ __device__ inline void copy_in(float* __restrict__ dst, const float* __restrict_
   for (uint32_t k = count / 4 * 4 + threadIdx.x; k < count; k += kThreads) dst[k] = src[k];
 }
 
+// kDeviceArgs: the arguments come from device memory (`args`) instead of `a`.
+template <bool kDeviceArgs>
 __global__ __launch_bounds__(kThreads) void adam_kernel(uint32_t n, float* __restrict__ params, float* __restrict__ sh,
                                                          float2* __restrict__ moments,
                                                          const float* __restrict__ gaussian_grads,
-                                                         const float* __restrict__ sh_grads, AdamArgs a,
+                                                         const float* __restrict__ sh_grads, AdamArgs a_value,
+                                                         const AdamArgs* __restrict__ args,
                                                          float* __restrict__ gaussians) {
+  const AdamArgs a = kDeviceArgs ? *args : a_value;
   __shared__ __align__(16) float s_p[kAdamGaussians * kFloatsPerGaussian];
   __shared__ __align__(16) float s_gp[kAdamGaussians * kFloatsPerGaussian];
   __shared__ __align__(16) float s_sh[kAdamGaussians * kShRest];
@@ -262,10 +266,17 @@ void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* momen
                const float* sh_grads, const AdamArgs& a, float* gaussians) {
   if (n == 0) return;
   if (static_cast<uint64_t>(n) * kMomentSlots >= (1ull << 32)) throw Error(-2, "adam: too many Gaussians");
-  const size_t slots = static_cast<size_t>(n) * kMomentSlots;
-  (void)slots;
-  adam_kernel<<<ceil_div(n, kAdamGaussians), kThreads, 0, s>>>(n, params, sh, reinterpret_cast<float2*>(moments),
-                                                                gaussian_grads, sh_grads, a, gaussians);
+  adam_kernel<false><<<ceil_div(n, kAdamGaussians), kThreads, 0, s>>>(
+      n, params, sh, reinterpret_cast<float2*>(moments), gaussian_grads, sh_grads, a, nullptr, gaussians);
+  DIO_LAUNCHED("adam");
+}
+
+void adam_step_device_args(hipStream_t s, uint32_t n, float* params, float* sh, float* moments,
+                           const float* gaussian_grads, const float* sh_grads, const AdamArgs* a, float* gaussians) {
+  if (n == 0) return;
+  if (static_cast<uint64_t>(n) * kMomentSlots >= (1ull << 32)) throw Error(-2, "adam: too many Gaussians");
+  adam_kernel<true><<<ceil_div(n, kAdamGaussians), kThreads, 0, s>>>(
+      n, params, sh, reinterpret_cast<float2*>(moments), gaussian_grads, sh_grads, AdamArgs{}, a, gaussians);
   DIO_LAUNCHED("adam");
 }
 

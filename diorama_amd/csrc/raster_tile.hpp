@@ -1,0 +1,311 @@
+// The per-tile bodies of the rasterizer and its backward pass, shared by the
+// single-frame kernels (render.hip) and the multi-view batch kernels (batch.hip).
+#pragma once
+#include "dio_internal.hpp"
+
+namespace dio {
+namespace raster {
+
+// Both rasterizer kernels run one workgroup per 16x16 tile as four independent
+// wavefronts, one per 8x8 quadrant (one pixel per lane). A wave stages 64 of the
+// tile's splats at a time in registers (one per lane) and broadcasts each to the
+// whole wave with v_readlane into scalar registers, so the blend loop reads no
+// LDS and the forward pass needs no workgroup barrier at all. Each staged splat
+// also gets the bounding box of its alpha >= 1/255 ellipse; a wave only visits
+// the splats whose box meets its quadrant (a ballot mask), which changes no
+// result: outside the box every pixel would skip the splat anyway.
+
+constexpr int kChunk = kWave;
+constexpr int kGrads = 9;
+
+__device__ inline float lane_value(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// One splat of a chunk, staged in a lane's registers.
+struct Staged {
+  float mx, my, ca, cb, cc, op, r, g, b;
+  float x0, x1, y0, y1;  // box holding every pixel centre where opacity * G >= 1/255
+  uint32_t index;
+};
+
+__device__ inline void stage(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, uint32_t k,
+                             Staged& s) {
+  s.index = pair_splats[k];
+  const float4 a = splats[s.index * 3], b = splats[s.index * 3 + 1], c = splats[s.index * 3 + 2];
+  s.mx = a.x; s.my = a.y;
+  s.ca = b.x; s.cb = b.y; s.cc = b.z;
+  s.r = c.x; s.g = c.y; s.b = c.z; s.op = c.w;
+  // opacity * exp(-q) >= 1/255 needs q <= t = ln(255 opacity), with
+  // q = (a dx² + c dy²)/2 + b dx dy. That ellipse's half-widths are sqrt(2t Σxx)
+  // and sqrt(2t Σyy), Σ = conic⁻¹. Widened by 1% and 0.01 px against rounding.
+  const float det = s.ca * s.cc - s.cb * s.cb;
+  const float t = __logf(255.f * s.op) * 1.01f + 0.01f;
+  float ex = sqrtf(2.f * t * s.cc / det) + 0.01f, ey = sqrtf(2.f * t * s.ca / det) + 0.01f;
+  ex = !(det > 0.f) ? INFINITY : (!(t > 0.f) ? -INFINITY : ex);  // degenerate: never cull; too faint: always
+  ey = !(det > 0.f) ? INFINITY : (!(t > 0.f) ? -INFINITY : ey);
+  s.x0 = s.mx - ex;
+  s.x1 = s.mx + ex;
+  s.y0 = s.my - ey;
+  s.y1 = s.my + ey;
+}
+
+__device__ inline bool meets(const Staged& s, float bx0, float bx1, float by0, float by1) {
+  return s.x1 >= bx0 && s.x0 <= bx1 && s.y1 >= by0 && s.y0 <= by1;
+}
+
+// rasterize.wgsl:31-100: front-to-back blend; a wave stops once all its pixels
+// are nearly opaque.
+// One 16x16 tile (tile_x, tile_y) of a width x height frame whose pair list is
+// `run`; the per-pixel outputs are indexed by the frame's own row-major pixel index.
+__device__ inline void rasterize_tile(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
+                                      uint2 run, uint32_t tile_x, uint32_t tile_y, uint32_t width, uint32_t height,
+                                      float bg_r, float bg_g, float bg_b, uint32_t* __restrict__ target,
+                                      float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const uint32_t qx = tile_x * kTile + (wave & 1) * 8, qy = tile_y * kTile + (wave >> 1) * 8;
+  const uint32_t px = qx + (lane & 7), py = qy + (lane >> 3);
+  const bool inside = px < width && py < height;
+  const float cx = static_cast<float>(px) + 0.5f, cy = static_cast<float>(py) + 0.5f;
+  const float bx0 = static_cast<float>(qx) + 0.5f, bx1 = bx0 + 7.f;
+  const float by0 = static_cast<float>(qy) + 0.5f, by1 = by0 + 7.f;
+  float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
+  bool done = !inside;
+  uint32_t last = 0;
+  for (uint32_t base = run.x; base < run.y; base += kChunk) {
+    if (__all(done)) break;
+    Staged s;
+    bool hit = false;
+    if (base + lane < run.y) {
+      stage(splats, pair_splats, base + lane, s);
+      hit = meets(s, bx0, bx1, by0, by1);
+    }
+    uint64_t mask = __ballot(hit);
+    while (mask) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const float mx = lane_value(s.mx, j), my = lane_value(s.my, j);
+      const float ca = lane_value(s.ca, j), cbb = lane_value(s.cb, j), cc = lane_value(s.cc, j);
+      const float op = lane_value(s.op, j);
+      const float sr = lane_value(s.r, j), sg = lane_value(s.g, j), sb = lane_value(s.b, j);
+      if (!done) {
+        const float dx = cx - mx, dy = cy - my;
+        const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
+        if (power <= 0.f) {
+          const float alpha = fminf(0.99f, op * __expf(power));
+          if (alpha >= 1.f / 255.f) {
+            const float remaining = T * (1.f - alpha);
+            if (remaining < 0.0001f) {
+              done = true;
+            } else {
+              const float w = alpha * T;
+              cr += sr * w;
+              cg += sg * w;
+              cb += sb * w;
+              T = remaining;
+              last = base + j - run.x + 1;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (inside) {
+    const float r = cr + T * bg_r, g = cg + T * bg_g, b = cb + T * bg_b;
+    const uint32_t index = py * width + px;
+    pixel_color[index] = make_float4(r, g, b, T);
+    pixel_last[index] = last;
+    if (target) {
+      auto u8 = [](float v) { return static_cast<uint32_t>(fminf(fmaxf(v, 0.f), 1.f) * 255.f + 0.5f); };
+      target[index] = u8(r) | (u8(g) << 8) | (u8(b) << 16) | (255u << 24);
+    }
+  }
+}
+
+// ------------------------------------------------------ rasterize backward --
+
+template <int kCtrl>
+__device__ inline float dpp_add(float v) {
+  return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, 0xf, 0xf,
+                                                                   false));
+}
+
+// Sum over each 16-lane row with DPP; every lane of the row gets the row total.
+__device__ inline float row_total(float v) {
+  v = dpp_add<0xb1>(v);   // quad_perm [1,0,3,2]
+  v = dpp_add<0x4e>(v);   // quad_perm [2,3,0,1]: quad sums
+  v = dpp_add<0x141>(v);  // row_half_mirror: 8-lane sums
+  v = dpp_add<0x140>(v);  // row_mirror: row sums
+  return v;
+}
+
+// Four splats' per-lane values (a, b, c, d) to four wave totals at once: a
+// butterfly across lane halves keeps a|c, b|d by half, one across 16-lane rows
+// keeps a|b|c|d by row, so one row reduction finishes all four: row r of the
+// wave ends with splat r's total.
+__device__ inline float four_totals(float a, float b, float c, float d) {
+  const int lane = __lane_id();
+  const bool lower = lane < 32;
+  const float lo = (lower ? a : c) + __shfl_xor(lower ? c : a, 32, kWave);
+  const float hi = (lower ? b : d) + __shfl_xor(lower ? d : b, 32, kWave);
+  const bool even_row = (lane & 16) == 0;
+  return row_total((even_row ? lo : hi) + __shfl_xor(even_row ? hi : lo, 16, kWave));
+}
+
+__device__ inline uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, kWave)));
+  return v;
+}
+
+// The per-pixel replay state of the backward pass (rasterizeBackward.wgsl:161-177).
+struct Replay {
+  float T, final_T, bg_grad, gr, gg, gb;
+  float behind_r, behind_g, behind_b;
+  float next_alpha, next_r, next_g, next_b;
+};
+
+// One splat's 9 gradient shares at one pixel (rasterizeBackward.wgsl:199-230),
+// written without branches: a pixel the splat does not blend into (power > 0,
+// alpha < 1/255, or past the pixel's last splat) runs the same instructions with
+// alpha = 0, which leaves T, the colour behind and the next splat as they were and
+// zeroes every share. The blended lanes compute exactly the branchy form.
+__device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy, float mx, float my, float ca,
+                                    float cbb, float cc, float op, float sr, float sg, float sb, float share[kGrads]) {
+  const float dx = cx - mx, dy = cy - my;
+  const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
+  const float G = __expf(power);
+  const float clamped = fminf(0.99f, op * G);
+  const bool valid = blended && power <= 0.f && clamped >= 1.f / 255.f;
+  const float alpha = valid ? clamped : 0.f;
+  const float keep = __builtin_amdgcn_rcpf(1.f - alpha);  // one hardware reciprocal for both divisions; 1 when idle
+  st.T *= keep;
+  const float na = valid ? st.next_alpha : 0.f;
+  st.behind_r = na * st.next_r + (1.f - na) * st.behind_r;
+  st.behind_g = na * st.next_g + (1.f - na) * st.behind_g;
+  st.behind_b = na * st.next_b + (1.f - na) * st.behind_b;
+  st.next_alpha = valid ? alpha : st.next_alpha;
+  st.next_r = valid ? sr : st.next_r;
+  st.next_g = valid ? sg : st.next_g;
+  st.next_b = valid ? sb : st.next_b;
+  const float alpha_grad = st.T * ((sr - st.behind_r) * st.gr + (sg - st.behind_g) * st.gg + (sb - st.behind_b) * st.gb) -
+                           st.final_T * keep * st.bg_grad;
+  const float power_grad = valid ? alpha_grad * op * G : 0.f;
+  share[0] = power_grad * (ca * dx + cbb * dy);
+  share[1] = power_grad * (cbb * dx + cc * dy);
+  share[2] = power_grad * (-0.5f * dx * dx);
+  share[3] = power_grad * (-dx * dy);
+  share[4] = power_grad * (-0.5f * dy * dy);
+  share[5] = valid ? alpha_grad * G : 0.f;
+  const float at = alpha * st.T;
+  share[6] = at * st.gr;
+  share[7] = at * st.gg;
+  share[8] = at * st.gb;
+  return valid;
+}
+
+// rasterizeBackward.wgsl:49-163: each pixel replays its list back to front from
+// its last blended splat. A wave takes the splats that touch its quadrant four at
+// a time, reduces their 9 gradients across its 64 pixels together, and its row
+// leaders park the totals in LDS; per 64-splat chunk the four quadrant waves add
+// theirs and one float atomic per (splat, tile, gradient) goes out.
+// The workgroup's LDS for one backward tile.
+struct BackwardShared {
+  float sums[4][kGrads][kChunk];
+  uint32_t index[kChunk];
+  uint32_t last[4];
+};
+
+// One 16x16 tile of the backward pass; the workgroup may call it again for another
+// tile once it returns (it ends on a barrier).
+__device__ inline void rasterize_backward_tile(const float4* __restrict__ splats,
+                                               const uint32_t* __restrict__ pair_splats, uint2 run, uint32_t tile_x,
+                                               uint32_t tile_y, uint32_t width, uint32_t height, float bg_r,
+                                               float bg_g, float bg_b, const float4* __restrict__ pixel_color,
+                                               const uint32_t* __restrict__ pixel_last,
+                                               const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads,
+                                               BackwardShared& sh) {
+  auto& s_sums = sh.sums;
+  auto& s_index = sh.index;
+  auto& s_last = sh.last;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const uint32_t qx = tile_x * kTile + (wave & 1) * 8, qy = tile_y * kTile + (wave >> 1) * 8;
+  const uint32_t px = qx + (lane & 7), py = qy + (lane >> 3);
+  const bool inside = px < width && py < height;
+  const float cx = static_cast<float>(px) + 0.5f, cy = static_cast<float>(py) + 0.5f;
+  const float bx0 = static_cast<float>(qx) + 0.5f, bx1 = bx0 + 7.f;
+  const float by0 = static_cast<float>(qy) + 0.5f, by1 = by0 + 7.f;
+  Replay st{};
+  uint32_t last = 0;
+  if (inside) {
+    const uint32_t index = py * width + px;
+    st.final_T = pixel_color[index].w;
+    last = pixel_last[index];
+    const float4 pg = pixel_grads[index];
+    st.gr = pg.x; st.gg = pg.y; st.gb = pg.z;
+  }
+  st.T = st.final_T;
+  st.bg_grad = bg_r * st.gr + bg_g * st.gg + bg_b * st.gb;
+  const uint32_t my_last = wave_max(last);
+  if (lane == 0) s_last[wave] = my_last;
+  __syncthreads();
+  const uint32_t tile_last = max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
+  const int row = lane >> 4;
+  const bool row_leader = (lane & 15) == 0;
+
+  for (int c = static_cast<int>(ceil_div(tile_last, kChunk)) - 1; c >= 0; c--) {
+    const uint32_t base = run.x + static_cast<uint32_t>(c) * kChunk;
+    Staged s;
+    bool hit = false;
+    s.index = 0;
+    if (base + lane < run.y) {
+      stage(splats, pair_splats, base + lane, s);
+      hit = base + lane - run.x < my_last && meets(s, bx0, bx1, by0, by1);
+    }
+    if (wave == 0) s_index[lane] = s.index;
+#pragma unroll
+    for (int v = 0; v < kGrads; v++) s_sums[wave][v][lane] = 0.f;
+    uint64_t mask = __ballot(hit);
+    while (mask) {
+      // Up to four splats, back to front; missing ones contribute zeros.
+      float share[4][kGrads];
+      int js[4];
+      bool any = false;
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        js[g] = -1;
+        bool blended = false;
+        float mx = 0.f, my = 0.f, ca = 0.f, cbb = 0.f, cc = 0.f, op = 0.f, sr = 0.f, sg = 0.f, sb = 0.f;
+        if (mask) {
+          const int j = kWave - 1 - __builtin_clzll(mask);
+          mask &= ~(1ull << j);
+          js[g] = j;
+          mx = lane_value(s.mx, j); my = lane_value(s.my, j);
+          ca = lane_value(s.ca, j); cbb = lane_value(s.cb, j); cc = lane_value(s.cc, j);
+          op = lane_value(s.op, j);
+          sr = lane_value(s.r, j); sg = lane_value(s.g, j); sb = lane_value(s.b, j);
+          blended = base + j - run.x < last;
+        }
+        any |= replay_splat(st, blended, cx, cy, mx, my, ca, cbb, cc, op, sr, sg, sb, share[g]);
+      }
+      if (!__any(any)) continue;
+      const int mine = row == 0 ? js[0] : (row == 1 ? js[1] : (row == 2 ? js[2] : js[3]));
+#pragma unroll
+      for (int v = 0; v < kGrads; v++) {
+        const float total = four_totals(share[0][v], share[1][v], share[2][v], share[3][v]);
+        if (row_leader && mine >= 0) s_sums[wave][v][mine] = total;
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < kGrads * kChunk; e += 256) {
+      const int v = e / kChunk, slot = e % kChunk;
+      const float total = s_sums[0][v][slot] + s_sums[1][v][slot] + s_sums[2][v][slot] + s_sums[3][v][slot];
+      if (total != 0.f) unsafeAtomicAdd(splat_grads + static_cast<size_t>(s_index[slot]) * kGrads + v, total);
+    }
+    __syncthreads();
+  }
+  __syncthreads();  // s_last is rewritten by the next tile
+}
+
+}  // namespace raster
+}  // namespace dio

@@ -558,9 +558,12 @@ FN(trainer_create) {
     fail(env, "settings are incomplete");
   } else {
     dio_trainer* t = NULL;
-    if (ok(env, dio_trainer_create(handle(&c, 0), (const float*)params, (const float*)sh, (uint32_t)(plen / 64), cams,
-                                   images, views, &s, &t)))
-      result = wrap(env, t);
+    const uint32_t batch = u32(&c, 5); /* optional: views per step of a batch trainer */
+    const int status = batch ? dio_trainer_create_batched(handle(&c, 0), (const float*)params, (const float*)sh,
+                                                          (uint32_t)(plen / 64), cams, images, views, batch, &s, &t)
+                             : dio_trainer_create(handle(&c, 0), (const float*)params, (const float*)sh,
+                                                  (uint32_t)(plen / 64), cams, images, views, &s, &t);
+    if (ok(env, status)) result = wrap(env, t);
   }
   free(cams);
   free((void*)images);
@@ -671,6 +674,10 @@ FN(trainer_densifications) {
   }
   return arr;
 }
+FN(trainer_batch_size) {
+  ARGS;
+  return number(env, dio_trainer_batch_size(handle(&c, 0)));
+}
 GETTER(trainer_params, dio_trainer_params)
 GETTER(trainer_moments, dio_trainer_moments)
 GETTER(trainer_stats, dio_trainer_stats)
@@ -707,7 +714,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT(trainer_last_loss), EXPORT(trainer_evaluate), EXPORT(trainer_count), EXPORT(trainer_iteration),
       EXPORT(trainer_set_iteration), EXPORT(trainer_sh_degree), EXPORT(trainer_set_sh_degree),
       EXPORT(trainer_densifications), EXPORT(trainer_params), EXPORT(trainer_moments), EXPORT(trainer_stats),
-      EXPORT(trainer_renderer), EXPORT(trainer_destroy),
+      EXPORT(trainer_renderer), EXPORT(trainer_destroy), EXPORT(trainer_batch_size),
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

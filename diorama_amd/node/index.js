@@ -306,12 +306,18 @@ const DEFAULT_SETTINGS = (() => {
 /**
  * src/trainer.ts:103-446. `raw` = {params: Float32Array (16 per Gaussian), sh:
  * Float32Array (45 per Gaussian)}; `views` = [{name, camera, image: Uint32Array rgba8}].
+ * options.batch > 0 makes a multi-view batch trainer: that many views (of any
+ * sizes) per step, their gradients summed into one Adam step.
  */
 class Trainer {
-  constructor(device, raw, views, settings) {
+  constructor(device, raw, views, settings, options) {
     this.device = device;
     this.settings = settings;
-    this.handle = native.trainer_create(device.handle, raw.params, raw.sh, views, settings);
+    const batch = options && options.batch ? options.batch : 0;
+    this.handle = native.trainer_create(device.handle, raw.params, raw.sh, views, settings, batch);
+  }
+  get batchSize() {
+    return native.trainer_batch_size(this.handle);
   }
   get count() {
     return native.trainer_count(this.handle);

@@ -198,6 +198,17 @@ void dio_default_settings(float extent, dio_training_settings* out);
 int dio_trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t count, const dio_camera* cameras,
                        const uint32_t* const* images, uint32_t view_count, const dio_training_settings* settings,
                        dio_trainer** out);
+/* Multi-view batching (no counterpart in the reference; for callers training on
+ * many views per step): `batch` training views of any sizes per step are rendered,
+ * scored and back-propagated in the same launches and their gradients summed into
+ * one Adam step, the step being one captured HIP graph. Every other dio_trainer_*
+ * call works as for dio_trainer_create; step counts, learning-rate schedule and
+ * densification cadence count batch steps; dio_trainer_last_loss is the mean of
+ * the batch's per-view losses. */
+int dio_trainer_create_batched(dio_ctx* ctx, const float* params, const float* sh, uint32_t count,
+                               const dio_camera* cameras, const uint32_t* const* images, uint32_t view_count,
+                               uint32_t batch, const dio_training_settings* settings, dio_trainer** out);
+uint32_t dio_trainer_batch_size(const dio_trainer* t); /* 0 for a plain trainer */
 int dio_trainer_step(dio_trainer* t);      /* optimize() then maintain() */
 int dio_trainer_optimize(dio_trainer* t);  /* render, loss, backward, stats, Adam; no wait */
 int dio_trainer_maintain(dio_trainer* t);  /* densify / reset when due; waits when densifying */
@@ -226,8 +237,9 @@ void dio_trainer_destroy(dio_trainer* t);
 /* Times `repeats` launches of one named stage between HIP events recorded on the
  * context's stream, for the benchmark's roofline figures. Stages: "preprocess",
  * "scan", "sort", "rasterize", "rasterize_backward", "preprocess_backward",
- * "loss", "adam". Returns the mean milliseconds per launch of the last
- * optimize() step's inputs re-run `repeats` times. */
+ * "loss", "adam"; a batch trainer has the one stage "batch_step" (its captured
+ * step graph). Returns the mean milliseconds per launch of the last optimize()
+ * step's inputs re-run `repeats` times. */
 int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_t repeats, double* ms);
 
 #ifdef __cplusplus
