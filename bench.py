@@ -80,7 +80,8 @@ def teacher_targets(ctx, cams, n):
     images = []
     for cam in cams:
         r.encode(None, cam, [0, 0, 0])
-        color = r.pixel_buffer.download(np.float32, WIDTH * HEIGHT * 4).reshape(-1, 4)[:, :3]
+        px = int(cam["viewport"][0]) * int(cam["viewport"][1])
+        color = r.pixel_buffer.download(np.float32, px * 4).reshape(-1, 4)[:, :3]
         noisy = color + rng.standard_normal(color.shape, dtype=np.float32) * np.float32(0.01)
         q = np.rint(np.clip(noisy, 0, 1) * 255).astype(np.uint32)
         images.append(q[:, 0] | (q[:, 1] << 8) | (q[:, 2] << 16) | np.uint32(255 << 24))
@@ -240,6 +241,93 @@ def scan_sort_bench(args):
     return 0
 
 
+BATCH_SIZES = [(320, 240), (640, 360), (980, 545), (1280, 720), (1600, 900)]
+BATCH_VIEWS = 16
+
+
+def batch_poses():
+    """The 263 training poses of the ring scene, each at a resolution drawn uniformly
+    (seeded) from BATCH_SIZES, fov 50 degrees."""
+    from diorama_amd import scene
+
+    rng = np.random.default_rng(11)
+    base = ring_poses(TRAIN_VIEWS + HELD_OUT)
+    train, _ = split_views(base)
+    out = []
+    for cam in train:
+        w, h = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        proj = scene.perspective(math.radians(50.0), w / h, 0.01, 100.0)
+        out.append(scene.camera(cam["view"], proj, w, h))
+    return out
+
+
+def batch_bench(args):
+    """Batched multi-view training: 16 views of uneven resolutions per step through the
+    batch trainer (one captured graph per step, gradients summed into one Adam step),
+    against the same views one per step (a batch of 1) on the same code path. The
+    figure compared is training pixels per second."""
+    import diorama_amd
+    from diorama_amd import scene
+
+    ctx = diorama_amd.Context(0)
+    cams = batch_poses()
+    t0 = time.perf_counter()
+    images = teacher_targets(ctx, cams, args.gaussians)
+    log(f"teacher targets: {len(images)} views of mixed sizes in {time.perf_counter() - t0:.1f} s")
+    params, sh = scene.random_raw_gaussians(args.gaussians, 0, box=5.0)
+    extent = 1.1 * max(math.sqrt(sum(v * v for v in _eye(c))) for c in cams)
+    settings = diorama_amd.default_settings(extent)
+    results = {}
+    for batch in (BATCH_VIEWS, 1):
+        trainer = diorama_amd.Trainer(ctx, params, sh, cams, images, settings, batch=batch)
+        trainer.iteration = 15_000
+        trainer.sh_degree = 3
+        steps = args.steps if batch > 1 else args.steps * BATCH_VIEWS // 4
+        for _ in range(args.warmup):
+            trainer.step()
+        ctx.synchronize()
+        pixels = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            trainer.step()
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        # pixels per step: the mean view size times the views per step
+        mean_px = float(np.mean([c["viewport"][0] * c["viewport"][1] for c in cams]))
+        pixels = mean_px * batch * steps
+        graph_ms = trainer.time_stage("batch_step", 5)
+        results[batch] = {"steps_per_s": steps / elapsed, "views_per_s": steps * batch / elapsed,
+                          "Mpx_per_s": pixels / elapsed / 1e6, "ms_per_step": elapsed * 1e3 / steps,
+                          "graph_ms": graph_ms}
+        log(f"batch {batch:2d}: {elapsed * 1e3 / steps:8.2f} ms/step, {steps * batch / elapsed:8.1f} views/s, "
+            f"{pixels / elapsed / 1e6:8.1f} Mpx/s (graph replay {graph_ms:.2f} ms)")
+        trainer.destroy()
+    ctx.close()
+    b, one = results[BATCH_VIEWS], results[1]
+    line = {
+        "metric": "batched multi-view training views/s @1.1M Gaussians, 16 views of uneven sizes per step",
+        "value": round(b["views_per_s"], 2),
+        "unit": "views/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(b["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: seed-7 teacher renders + N(0,0.01) noise on 263 ring poses at sizes drawn from "
+                "320x240..1600x900; seed-0 student",
+        "config": {"workload": "batch trainer, 16 views per step, one graph replay + one Adam step per batch",
+                   "gaussians": args.gaussians, "sizes": BATCH_SIZES, "views_per_step": BATCH_VIEWS},
+        "batch16": {k: round(v, 3) for k, v in b.items()},
+        "batch1": {k: round(v, 3) for k, v in one.items()},
+        "speedup_px_per_s_vs_batch1": round(b["Mpx_per_s"] / one["Mpx_per_s"], 3),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE and
     WRITE_SIZE already scaled by the copy calibration), or None."""
@@ -262,8 +350,9 @@ def main():
     ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--repeats", type=int, default=20, help="launches per stage timing")
-    ap.add_argument("--workload", choices=["train", "scan-sort"], default="train",
-                    help="train: the training step (default); scan-sort: the scan/sort microbenchmark")
+    ap.add_argument("--workload", choices=["train", "scan-sort", "batch"], default="train",
+                    help="train: the training step (default); scan-sort: the scan/sort microbenchmark; "
+                         "batch: the multi-view batch trainer")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (the FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -275,6 +364,8 @@ def main():
         return copy_calibration()
     if args.workload == "scan-sort":
         return scan_sort_bench(args)
+    if args.workload == "batch":
+        return batch_bench(args)
 
     import diorama_amd
     from diorama_amd import scene
