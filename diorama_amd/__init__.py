@@ -126,6 +126,17 @@ SIGNATURES = {
     "dio_trainer_create_batched": (_I, [_P, _P, _P, _U32, ctypes.POINTER(_Camera), ctypes.POINTER(_P), _U32, _U32,
                                         ctypes.POINTER(_Settings), ctypes.POINTER(_P)]),
     "dio_trainer_batch_size": (_U32, [_P]),
+    "dio_mlp_create": (_I, [_P, _U32, ctypes.POINTER(_P)]),
+    "dio_mlp_get_weights": (_I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "dio_mlp_set_weights": (_I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "dio_mlp_forward": (_I, [_P, _P, _U32, _P]),
+    "dio_mlp_gradients": (_I, [_P, _P, _P, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]),
+    "dio_mlp_train": (_I, [_P, _P, _P, _U32, _U32, _U32, ctypes.c_float, _U32, ctypes.POINTER(ctypes.c_double)]),
+    "dio_mlp_destroy": (None, [_P]),
+    "dio_trainer_set_densify_mode": (_I, [_P, _U32, _P, _U32, ctypes.c_float, _U32]),
+    "dio_trainer_samples": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_U32)]),
+    "dio_trainer_features": (_I, [_P, ctypes.POINTER(_P)]),
+    "dio_trainer_extra_stats": (_P, [_P]),
     "dio_trainer_step": (_I, [_P]),
     "dio_trainer_optimize": (_I, [_P]),
     "dio_trainer_maintain": (_I, [_P]),
@@ -520,6 +531,55 @@ def _images(images):
     return arrs, ptrs
 
 
+MLP_PARAMS = 5252
+MLP_INPUTS = 12
+MLP_CLASSES = ("keep", "clone", "split", "prune")
+
+
+class DensifyMLP:
+    """The learned densification proposal: 12 -> 64 -> 64 -> 4, BF16 weights, FP32
+    accumulation (no counterpart in the reference)."""
+
+    def __init__(self, ctx, seed=1):
+        h = _P()
+        _check(lib().dio_mlp_create(ctx.handle, seed, ctypes.byref(h)))
+        self.handle, self.ctx = h, ctx
+
+    @property
+    def weights(self):
+        out = np.empty(MLP_PARAMS, dtype=np.float32)
+        _check(lib().dio_mlp_get_weights(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    @weights.setter
+    def weights(self, w):
+        a = np.ascontiguousarray(w, dtype=np.float32).reshape(-1)
+        if a.size != MLP_PARAMS:
+            raise ValueError(f"the MLP has {MLP_PARAMS} parameters")
+        _check(lib().dio_mlp_set_weights(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def forward(self, features, count, logits):
+        _check(lib().dio_mlp_forward(self.handle, features.handle, count, logits.handle))
+
+    def gradients(self, features, labels, count):
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().dio_mlp_gradients(self.handle, features.handle, labels.handle, count,
+                                       g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, features, labels, count, steps, batch=4096, lr=1e-3, seed=1):
+        loss = ctypes.c_double()
+        _check(lib().dio_mlp_train(self.handle, features.handle, labels.handle, count, steps, batch, lr, seed,
+                                   ctypes.byref(loss)))
+        return loss.value
+
+    def destroy(self):
+        if self.handle:
+            lib().dio_mlp_destroy(self.handle)
+            self.handle = None
+
+
 class Trainer:
     """src/trainer.ts:103-446: one training view per step, plain 3DGS schedule.
 
@@ -545,6 +605,27 @@ class Trainer:
     @property
     def batch_size(self):
         return lib().dio_trainer_batch_size(self.handle)
+
+    def set_densify_mode(self, mode, mlp=None, budget=0, confidence=0.5, samples_per_densify=80000):
+        """mode 0 plain rule, 1 plain rule + sample logging, 2 learned (see dio_trainer_set_densify_mode)."""
+        modes = {"plain": 0, "log": 1, "learned": 2}
+        m = modes.get(mode, mode)
+        _check(lib().dio_trainer_set_densify_mode(self.handle, m, mlp.handle if mlp else None, budget, confidence,
+                                                  samples_per_densify))
+
+    def samples(self):
+        f, l, n = _P(), _P(), _U32()
+        _check(lib().dio_trainer_samples(self.handle, ctypes.byref(f), ctypes.byref(l), ctypes.byref(n)))
+        return _view(f.value, self), _view(l.value, self), n.value
+
+    def features(self):
+        f = _P()
+        _check(lib().dio_trainer_features(self.handle, ctypes.byref(f)))
+        return _view(f.value, self)
+
+    @property
+    def extra_stats(self):
+        return _view(lib().dio_trainer_extra_stats(self.handle), self)
 
     def step(self):
         _check(lib().dio_trainer_step(self.handle))

@@ -990,6 +990,33 @@ constexpr size_t kBatchDescBytes = kBatchViewsOffset + dio::kMaxBatchViews * siz
 constexpr size_t kBatchHostBytes = kBatchDescBytes + sizeof(dio::AdamArgs);
 
 
+// The learned densification proposal (no counterpart in the reference): the MLP's
+// BF16 weights with their FP32 master copy and Adam state (mlp.hip).
+struct dio_densify_mlp {
+  dio_ctx* ctx = nullptr;
+  dio_buf *weights = nullptr, *master = nullptr, *moments = nullptr, *grads = nullptr, *loss = nullptr;
+  uint32_t adam_steps = 0;
+};
+
+// A trainer's learned-densification state. mode 0: the plain rule; 1: the plain
+// rule, logging (inputs, action) samples whose outcome held; 2: the MLP's action
+// where confident, else the rule's. A budget caps the count after densifying.
+struct LearnedState {
+  uint32_t mode = 0;
+  dio_densify_mlp* mlp = nullptr;  // not owned
+  uint32_t budget = 0;
+  float confidence = 0.5f;
+  uint32_t samples_per_densify = 80000;
+  dio_buf* extra = nullptr;        // per Gaussian: largest gradient, summed depth, summed gradient
+  dio_buf *features = nullptr, *logits = nullptr;
+  uint32_t features_count = 0;
+  dio_buf *pending = nullptr, *pending_count = nullptr;
+  uint32_t pending_capacity = 0;
+  bool pending_valid = false;
+  dio_buf *store_features = nullptr, *store_labels = nullptr, *store_count = nullptr;
+  uint32_t store_capacity = 0;
+};
+
 struct dio_trainer {
   dio_ctx* ctx = nullptr;
   dio_training_settings settings{};
@@ -1011,6 +1038,7 @@ struct dio_trainer {
   uint32_t last_view = 0;
   BatchState* batch = nullptr;  // set for a multi-view batch trainer
   uint32_t generation = 0;      // bumped whenever densification replaces the buffers
+  LearnedState learned;
 };
 
 namespace {
@@ -1046,6 +1074,10 @@ void destroy_trainer(dio_trainer* t) {
   if (!t) return;
   (void)hipStreamSynchronize(t->ctx->stream);
   batch_destroy(t->batch);
+  for (dio_buf** p : {&t->learned.extra, &t->learned.features, &t->learned.logits, &t->learned.pending,
+                      &t->learned.pending_count, &t->learned.store_features, &t->learned.store_labels,
+                      &t->learned.store_count})
+    release(*p);
   dio_renderer_destroy(t->renderer);
   destroy_loss(t->loss);
   release(t->params); release(t->moments); release(t->stats);
@@ -1110,7 +1142,7 @@ void trainer_optimize(dio_trainer* t) {
   t->last_adam = a;
   auto* sc = t->renderer->scene;
   dio::accumulate_stats(t->ctx->stream, t->count, dev<float>(sc->splats), dev<float>(sc->grad_splats), a,
-                        dev<float>(t->stats));
+                        dev<float>(t->stats), dev<float>(t->learned.extra));
   if (!densify_now)
     dio::adam_step(t->ctx->stream, t->count, dev<float>(t->params), dev<float>(sc->sh), dev<float>(t->moments),
                    dev<float>(sc->grad_gaussians), dev<float>(sc->grad_sh), a, dev<float>(sc->gaussians));
@@ -1124,9 +1156,82 @@ void trainer_reset_opacity(dio_trainer* t) {
                      dev<float>(t->renderer->scene->gaussians));
 }
 
+// The per-Gaussian buffers of the learned densification, sized for the count.
+void learned_resize(dio_trainer* t) {
+  LearnedState& L = t->learned;
+  if (L.features_count == t->count && L.features) return;
+  release(L.features);
+  release(L.logits);
+  L.features = alloc(t->ctx, static_cast<size_t>(t->count) * dio::kMlpInputs * 4, true);
+  L.logits = alloc(t->ctx, static_cast<size_t>(t->count) * 4 * 4, true);
+  L.features_count = t->count;
+}
+
+// The plain rule's actions, or the MLP's (mode 2), and the growth budget: fills
+// `codes` and returns true when the densification must follow them rather than
+// the rule (mode 0 without a budget returns false and leaves codes alone).
+bool densify_codes(dio_trainer* t, const dio::DensifyArgs& a, dio_buf* codes, dio_buf* priority,
+                   uint32_t* counters) {
+  LearnedState& L = t->learned;
+  const uint32_t n = t->count;
+  hipStream_t s = t->ctx->stream;
+  if (L.mode == 0 && L.budget == 0) return false;
+  dio::densify_rule_codes(s, n, dev<float>(t->params), dev<float>(t->stats), a, dev<uint32_t>(codes),
+                          dev<float>(priority));
+  if (L.mode != 0) {
+    learned_resize(t);
+    dio::mlp_features(s, n, dev<float>(t->params), dev<float>(t->stats), dev<float>(L.extra), t->settings.extent,
+                      t->iteration, dev<float>(L.features));
+    if (L.mode == 2) {
+      dio::mlp_forward(s, n, dev<float>(L.features), dev<uint16_t>(L.mlp->weights), dev<float>(L.logits));
+      dio::mlp_learned_codes(s, n, dev<float>(L.logits), L.confidence, dev<uint32_t>(codes), dev<float>(priority));
+    }
+    if (L.mode == 1 && L.pending_valid)  // the last logging step's samples, judged on these statistics
+      dio::mlp_resolve(s, dev<uint32_t>(L.pending_count), L.pending_capacity, dev<float>(L.pending),
+                       dev<float>(t->stats), n, L.store_capacity, dev<float>(L.store_features),
+                       dev<uint32_t>(L.store_labels), dev<uint32_t>(L.store_count));
+  }
+  if (L.budget) {
+    dio_buf *scratch = nullptr, *keys = nullptr, *values = nullptr, *alt_keys = nullptr, *alt_values = nullptr,
+            *ws = nullptr;
+    try {
+      scratch = alloc(t->ctx, static_cast<size_t>(n) * 4);
+      dio::densify_count(s, n, dev<float>(t->params), dev<float>(t->stats), a, dev<uint32_t>(codes),
+                         dev<uint32_t>(scratch), counters);
+      uint32_t tally[4];
+      DIO_HIP(hipMemcpyAsync(tally, counters, sizeof(tally), hipMemcpyDeviceToHost, s));
+      sync(t->ctx);
+      const uint64_t kept = static_cast<uint64_t>(n) - tally[0];
+      const uint64_t grown = kept + tally[2] + tally[3];
+      if (grown > L.budget) {
+        const uint32_t allowed = static_cast<uint32_t>(L.budget > kept ? L.budget - kept : 0);
+        keys = alloc(t->ctx, static_cast<size_t>(n) * 4);
+        values = alloc(t->ctx, static_cast<size_t>(n) * 4);
+        alt_keys = alloc(t->ctx, static_cast<size_t>(n) * 4);
+        alt_values = alloc(t->ctx, static_cast<size_t>(n) * 4);
+        ws = alloc(t->ctx, dio::sort_workspace_bytes(n), true);
+        dio::densify_growth_keys(s, n, dev<uint32_t>(codes), dev<float>(priority), dev<uint32_t>(keys),
+                                 dev<uint32_t>(values));
+        if (dio::radix_sort(s, false, keys->ptr, dev<uint32_t>(values), alt_keys->ptr, dev<uint32_t>(alt_values),
+                            nullptr, n, 0, 32, dio::sort_workspace(ws->ptr, n)))
+          throw Error(DIO_ERR_INTERNAL, "budget sort ended in its scratch buffers");
+        dio::densify_demote(s, n, dev<uint32_t>(keys), dev<uint32_t>(values), allowed, dev<float>(t->params), a,
+                            dev<uint32_t>(codes));
+      }
+      sync(t->ctx);
+    } catch (...) {
+      release(scratch); release(keys); release(values); release(alt_keys); release(alt_values); release(ws);
+      throw;
+    }
+    release(scratch); release(keys); release(values); release(alt_keys); release(alt_values); release(ws);
+  }
+  return true;
+}
+
 void trainer_densify(dio_trainer* t, bool prune_large) {
   const uint32_t n = t->count;
   const auto& st = t->settings;
+  LearnedState& L = t->learned;
   dio::DensifyArgs a;
   a.grad_threshold = st.densify_grad_threshold;
   a.clone_max_size = st.percent_dense * st.extent;
@@ -1135,14 +1240,27 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
   a.seed = t->iteration;
   hipStream_t s = t->ctx->stream;
   dio_buf* outputs = alloc(t->ctx, static_cast<size_t>(n) * 4);
-  dio_buf* scratch = nullptr;
+  dio_buf *scratch = nullptr, *codes = nullptr, *priority = nullptr, *extra = nullptr;
   dio_buf *params = nullptr, *sh = nullptr, *moments = nullptr, *stats = nullptr, *gaussians = nullptr;
   try {
     scratch = alloc(t->ctx, 64 + dio::scan_workspace_bytes(n), true);
     uint32_t* counters = dev<uint32_t>(scratch);  // [0..3] tally, [4] total
-    dio::densify_count(s, n, dev<float>(t->params), dev<float>(t->stats), a, dev<uint32_t>(outputs), counters);
+    codes = alloc(t->ctx, static_cast<size_t>(n) * 4);
+    priority = alloc(t->ctx, static_cast<size_t>(n) * 4);
+    const bool forced = densify_codes(t, a, codes, priority, counters);
+    const uint32_t* forced_codes = forced ? dev<uint32_t>(codes) : nullptr;
+    dio::densify_count(s, n, dev<float>(t->params), dev<float>(t->stats), a, forced_codes, dev<uint32_t>(outputs),
+                       counters);
     dio::scan_exclusive(s, dev<uint32_t>(outputs), dev<uint32_t>(outputs), nullptr, n, counters + 4,
                         dio::scan_workspace(static_cast<char*>(scratch->ptr) + 64, n));
+    if (L.mode == 1) {  // log this step's samples; their outcome is read at the next densification
+      const double fraction = std::min(1.0, static_cast<double>(L.samples_per_densify) / std::max(n, 1u));
+      const uint32_t threshold = static_cast<uint32_t>(std::min(4294967295.0, fraction * 4294967296.0));
+      dio::fill_u32(s, dev<uint32_t>(L.pending_count), 0u, 1);
+      dio::mlp_log_pending(s, n, dev<float>(L.features), forced_codes, dev<uint32_t>(outputs), t->iteration,
+                           threshold, L.pending_capacity, dev<float>(L.pending), dev<uint32_t>(L.pending_count));
+      L.pending_valid = true;
+    }
     uint32_t host[5];
     DIO_HIP(hipMemcpyAsync(host, counters, sizeof(host), hipMemcpyDeviceToHost, s));
     sync(t->ctx);
@@ -1154,18 +1272,21 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
     moments = alloc(t->ctx, static_cast<size_t>(new_count) * dio::kMomentSlots * 8, true);
     stats = alloc(t->ctx, static_cast<size_t>(new_count) * 16, true);
     gaussians = alloc(t->ctx, static_cast<size_t>(new_count) * dio::kFloatsPerGaussian * 4, true);
+    if (L.extra) extra = alloc(t->ctx, static_cast<size_t>(new_count) * 16, true);
     dio::densify_scatter(s, n, dev<float>(t->params), dev<float>(t->stats), dev<uint32_t>(outputs),
-                         dev<float>(t->renderer->scene->sh), dev<float>(t->moments), a, dev<float>(params),
-                         dev<float>(sh), dev<float>(moments));
+                         dev<float>(t->renderer->scene->sh), dev<float>(t->moments), a, forced_codes,
+                         dev<float>(params), dev<float>(sh), dev<float>(moments));
     dio::activate(s, new_count, dev<float>(params), dev<float>(gaussians));
   } catch (...) {
-    release(outputs); release(scratch); release(params); release(sh); release(moments); release(stats);
-    release(gaussians);
+    release(outputs); release(scratch); release(codes); release(priority); release(params); release(sh);
+    release(moments); release(stats); release(gaussians); release(extra);
     throw;
   }
   sync(t->ctx);
   release(outputs);
   release(scratch);
+  release(codes);
+  release(priority);
   const uint32_t new_count = t->densifications.back().after;
   dio_renderer* r = t->renderer;
   const uint32_t capacity = r->scene->rasterizer->capacity;
@@ -1176,6 +1297,10 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
   t->moments = moments;
   t->stats = stats;
   t->count = new_count;
+  if (L.extra) {
+    release(L.extra);
+    L.extra = extra;
+  }
   t->generation++;
 }
 
@@ -1281,7 +1406,7 @@ void batch_launch(dio_trainer* t, bool with_adam) {
                                 dev<float>(b->pixel_grads), dev<float>(b->splat_grads));
   dio::preprocess_backward_batch(s, dev<float>(sc->gaussians), dev<float>(sc->sh), t->sh_degree, a,
                                  dev<float>(b->splats), dev<float>(b->splat_grads), dev<float>(b->grad_gaussians),
-                                 dev<float>(b->grad_sh), dev<float>(t->stats));
+                                 dev<float>(b->grad_sh), dev<float>(t->stats), dev<float>(t->learned.extra));
   if (with_adam)
     dio::adam_step_device_args(s, b->n, dev<float>(t->params), dev<float>(sc->sh), dev<float>(t->moments),
                                dev<float>(b->grad_gaussians), dev<float>(b->grad_sh),
@@ -1791,3 +1916,205 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
     *ms = elapsed / repeats;
   });
 }
+
+// ================================================== learned densification
+
+namespace {
+
+void destroy_mlp(dio_densify_mlp* m) {
+  if (!m) return;
+  (void)hipStreamSynchronize(m->ctx->stream);
+  release(m->weights); release(m->master); release(m->moments); release(m->grads); release(m->loss);
+  delete m;
+}
+
+void mlp_upload_master(dio_densify_mlp* m, const float* w) {
+  hipStream_t s = m->ctx->stream;
+  DIO_HIP(hipMemcpyAsync(m->master->ptr, w, dio::kMlpParams * 4, hipMemcpyHostToDevice, s));
+  dio::mlp_round(s, dev<float>(m->master), dev<uint16_t>(m->weights));
+  dio::fill_u32(s, dev<uint32_t>(m->moments), 0u, static_cast<size_t>(dio::kMlpParams) * 2);
+  dio::fill_u32(s, dev<uint32_t>(m->grads), 0u, dio::kMlpParams);
+  m->adam_steps = 0;
+  sync(m->ctx);
+}
+
+}  // namespace
+
+extern "C" int dio_mlp_create(dio_ctx* ctx, uint32_t seed, dio_densify_mlp** out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(out, "out");
+    auto* m = new dio_densify_mlp;
+    m->ctx = ctx;
+    try {
+      m->weights = alloc(ctx, dio::kMlpParams * 2, true);
+      m->master = alloc(ctx, dio::kMlpParams * 4, true);
+      m->moments = alloc(ctx, dio::kMlpParams * 8, true);
+      m->grads = alloc(ctx, dio::kMlpParams * 4, true);
+      m->loss = alloc(ctx, 16, true);
+      // He-uniform layers, zero biases, from mulberry32(seed).
+      std::vector<float> w(dio::kMlpParams, 0.f);
+      Mulberry32 random(seed);
+      const int fan_in[3] = {static_cast<int>(dio::kMlpInputs), 64, 64};
+      const int offset[3] = {0, 12 * 64 + 64, 12 * 64 + 64 + 64 * 64 + 64};
+      const int size[3] = {12 * 64, 64 * 64, 64 * 4};
+      for (int l = 0; l < 3; l++) {
+        const double bound = std::sqrt(6.0 / fan_in[l]);
+        for (int k = 0; k < size[l]; k++) w[offset[l] + k] = static_cast<float>((2 * random.next() - 1) * bound);
+      }
+      mlp_upload_master(m, w.data());
+    } catch (...) {
+      destroy_mlp(m);
+      throw;
+    }
+    *out = m;
+  });
+}
+
+extern "C" int dio_mlp_get_weights(dio_densify_mlp* m, float* out) {
+  return guard([&] {
+    need(m, "mlp");
+    need(out, "out");
+    std::vector<uint16_t> bf(dio::kMlpParams);
+    DIO_HIP(hipMemcpyAsync(bf.data(), m->weights->ptr, bf.size() * 2, hipMemcpyDeviceToHost, m->ctx->stream));
+    sync(m->ctx);
+    for (uint32_t k = 0; k < dio::kMlpParams; k++) {
+      const uint32_t bits = static_cast<uint32_t>(bf[k]) << 16;
+      std::memcpy(&out[k], &bits, 4);
+    }
+  });
+}
+
+extern "C" int dio_mlp_set_weights(dio_densify_mlp* m, const float* weights) {
+  return guard([&] {
+    need(m, "mlp");
+    need(weights, "weights");
+    mlp_upload_master(m, weights);
+  });
+}
+
+extern "C" int dio_mlp_forward(dio_densify_mlp* m, dio_buf* features, uint32_t count, dio_buf* logits) {
+  return guard([&] {
+    need(m, "mlp");
+    need(features, "features");
+    need(logits, "logits");
+    if (features->bytes < static_cast<size_t>(count) * dio::kMlpInputs * 4 || logits->bytes < static_cast<size_t>(count) * 16)
+      invalid("the feature or logit buffer is smaller than count samples");
+    dio::mlp_forward(m->ctx->stream, count, dev<float>(features), dev<uint16_t>(m->weights), dev<float>(logits));
+  });
+}
+
+extern "C" int dio_mlp_gradients(dio_densify_mlp* m, dio_buf* features, dio_buf* labels, uint32_t count,
+                                 float* grads, double* loss) {
+  return guard([&] {
+    need(m, "mlp");
+    need(features, "features");
+    need(labels, "labels");
+    need(grads, "grads");
+    need(loss, "loss");
+    if (features->bytes < static_cast<size_t>(count) * dio::kMlpInputs * 4 || labels->bytes < static_cast<size_t>(count) * 4)
+      invalid("the feature or label buffer is smaller than count samples");
+    hipStream_t s = m->ctx->stream;
+    dio::fill_u32(s, dev<uint32_t>(m->loss), 0u, 1);
+    dio::mlp_gradients(s, count, count, 0, dev<float>(features), dev<uint32_t>(labels), dev<uint16_t>(m->weights),
+                       dev<float>(m->grads), dev<float>(m->loss));
+    float total = 0.f;
+    DIO_HIP(hipMemcpyAsync(grads, m->grads->ptr, dio::kMlpParams * 4, hipMemcpyDeviceToHost, s));
+    DIO_HIP(hipMemcpyAsync(&total, m->loss->ptr, 4, hipMemcpyDeviceToHost, s));
+    dio::fill_u32(s, dev<uint32_t>(m->grads), 0u, dio::kMlpParams);
+    sync(m->ctx);
+    *loss = count ? total / count : 0.0;
+  });
+}
+
+extern "C" int dio_mlp_train(dio_densify_mlp* m, dio_buf* features, dio_buf* labels, uint32_t count, uint32_t steps,
+                             uint32_t batch, float lr, uint32_t seed, double* last_loss) {
+  return guard([&] {
+    need(m, "mlp");
+    need(features, "features");
+    need(labels, "labels");
+    if (count == 0 || batch == 0) invalid("training needs samples and a batch size");
+    if (features->bytes < static_cast<size_t>(count) * dio::kMlpInputs * 4 || labels->bytes < static_cast<size_t>(count) * 4)
+      invalid("the feature or label buffer is smaller than count samples");
+    hipStream_t s = m->ctx->stream;
+    for (uint32_t k = 0; k < steps; k++) {
+      dio::fill_u32(s, dev<uint32_t>(m->loss), 0u, 1);
+      // a nonzero seed per step: `batch` hashed draws from the samples
+      const uint32_t step_seed = (seed * 2654435761u) ^ (k * 40503u + 1u) | 1u;
+      dio::mlp_gradients(s, batch, count, step_seed, dev<float>(features), dev<uint32_t>(labels),
+                         dev<uint16_t>(m->weights), dev<float>(m->grads), dev<float>(m->loss));
+      m->adam_steps++;
+      const float inv1 = static_cast<float>(1.0 / (1.0 - std::pow(0.9, m->adam_steps)));
+      const float inv2 = static_cast<float>(1.0 / (1.0 - std::pow(0.999, m->adam_steps)));
+      dio::mlp_adam(s, dev<float>(m->master), dev<float>(m->moments), dev<float>(m->grads), dev<uint16_t>(m->weights),
+                    lr, inv1, inv2);
+    }
+    float total = 0.f;
+    DIO_HIP(hipMemcpyAsync(&total, m->loss->ptr, 4, hipMemcpyDeviceToHost, s));
+    sync(m->ctx);
+    if (last_loss) *last_loss = total / batch;
+  });
+}
+
+extern "C" void dio_mlp_destroy(dio_densify_mlp* m) { destroy_mlp(m); }
+
+extern "C" int dio_trainer_set_densify_mode(dio_trainer* t, uint32_t mode, dio_densify_mlp* mlp, uint32_t budget,
+                                            float confidence, uint32_t samples_per_densify) {
+  return guard([&] {
+    need(t, "trainer");
+    if (mode > 2) invalid("densify mode is 0 (rule), 1 (rule, logging samples) or 2 (learned)");
+    if (mode == 2) need(mlp, "mlp");
+    LearnedState& L = t->learned;
+    L.mode = mode;
+    L.mlp = mlp;
+    L.budget = budget;
+    L.confidence = confidence;
+    L.samples_per_densify = samples_per_densify ? samples_per_densify : 80000;
+    if (mode != 0 && !L.extra) L.extra = alloc(t->ctx, static_cast<size_t>(t->count) * 16, true);
+    if (mode == 1 && !L.pending) {
+      L.pending_capacity = 2 * L.samples_per_densify + 1024;
+      L.pending = alloc(t->ctx, static_cast<size_t>(L.pending_capacity) * dio::kMlpPendingWords * 4, true);
+      L.pending_count = alloc(t->ctx, 16, true);
+      L.store_capacity = 8u << 20;
+      L.store_features = alloc(t->ctx, static_cast<size_t>(L.store_capacity) * dio::kMlpInputs * 4, true);
+      L.store_labels = alloc(t->ctx, static_cast<size_t>(L.store_capacity) * 4, true);
+      L.store_count = alloc(t->ctx, 16, true);
+    }
+    if (mode != 1) L.pending_valid = false;
+    learned_resize(t);
+    t->generation++;  // a captured batch graph must pick up the statistics buffer
+  });
+}
+
+extern "C" int dio_trainer_samples(dio_trainer* t, dio_buf** features, dio_buf** labels, uint32_t* count) {
+  return guard([&] {
+    need(t, "trainer");
+    need(features, "features");
+    need(labels, "labels");
+    need(count, "count");
+    LearnedState& L = t->learned;
+    uint32_t n = 0;
+    if (L.store_count) {
+      DIO_HIP(hipMemcpyAsync(&n, L.store_count->ptr, 4, hipMemcpyDeviceToHost, t->ctx->stream));
+      sync(t->ctx);
+    }
+    *features = L.store_features;
+    *labels = L.store_labels;
+    *count = std::min(n, L.store_capacity);
+  });
+}
+
+extern "C" int dio_trainer_features(dio_trainer* t, dio_buf** features) {
+  return guard([&] {
+    need(t, "trainer");
+    need(features, "features");
+    LearnedState& L = t->learned;
+    if (!L.extra) invalid("set a densify mode first: the inputs need its statistics");
+    learned_resize(t);
+    dio::mlp_features(t->ctx->stream, t->count, dev<float>(t->params), dev<float>(t->stats), dev<float>(L.extra),
+                      t->settings.extent, t->iteration, dev<float>(L.features));
+    *features = L.features;
+  });
+}
+
+extern "C" dio_buf* dio_trainer_extra_stats(dio_trainer* t) { return t ? t->learned.extra : nullptr; }

@@ -133,12 +133,13 @@ __global__ __launch_bounds__(256) void loss_gradient_batch_kernel(BatchArgs b, c
 }
 
 // preprocessBackward.wgsl:27-181 summed over the views, with accumulate_stats
-// (adam.wgsl:125-134) for every view that sees the Gaussian: a thread keeps its
+// (adam.wgsl:125-134) for every view that sees the Gaussian (and, when `extra` is
+// set, the learned densification's statistics): a thread keeps its
 // Gaussian's 16 + 45 gradient sums in registers across the views.
 __global__ __launch_bounds__(kThreads) void preprocess_backward_batch_kernel(
     const float4* __restrict__ gaussians, const float* __restrict__ sh, uint32_t sh_degree, BatchArgs b,
     const float4* __restrict__ splats, const float* __restrict__ splat_grads, float4* __restrict__ gaussian_grads,
-    float* __restrict__ sh_grads, float4* __restrict__ stats) {
+    float* __restrict__ sh_grads, float4* __restrict__ stats, float4* __restrict__ extra) {
   __shared__ __align__(16) float s_sh[kThreads * kShRest];
   const uint32_t n = b.per_view;
   const uint32_t first = blockIdx.x * kThreads, i = first + threadIdx.x;
@@ -160,9 +161,11 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_batch_kernel(
       g[4 * q] = v.x; g[4 * q + 1] = v.y; g[4 * q + 2] = v.z; g[4 * q + 3] = v.w;
     }
     float4 st = stats[i];
+    float4 ex = extra ? extra[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     for (uint32_t v = 0; v < b.view_count; v++) {
       const size_t o = static_cast<size_t>(v) * n + i;
-      const float radius = splats[o * 3].w;
+      const float4 s0 = splats[o * 3];
+      const float radius = s0.w;
       if (radius == 0.f) continue;  // not visible in this view
       const ViewDesc& view = b.views[v];
       float gr[kGrads];
@@ -185,9 +188,12 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_batch_kernel(
         }
       }
       const float gx = gr[0] * view.cam.viewport[0] * 0.5f, gy = gr[1] * view.cam.viewport[1] * 0.5f;
-      st = make_float4(st.x + sqrtf(gx * gx + gy * gy), st.y + 1.f, fmaxf(st.z, radius), 0.f);
+      const float length = sqrtf(gx * gx + gy * gy);
+      st = make_float4(st.x + length, st.y + 1.f, fmaxf(st.z, radius), 0.f);
+      ex = make_float4(fmaxf(ex.x, length), ex.y + s0.z, ex.z + gx, ex.w + gy);
     }
     stats[i] = st;
+    if (extra) extra[i] = ex;
     gaussian_grads[i * 4] = out0;
     gaussian_grads[i * 4 + 1] = out1;
     gaussian_grads[i * 4 + 2] = out2;
@@ -248,11 +254,12 @@ void rasterize_backward_batch(hipStream_t s, const float* splats, const uint32_t
 
 void preprocess_backward_batch(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree,
                                const BatchArgs& b, const float* splats, const float* splat_grads,
-                               float* gaussian_grads, float* sh_grads, float* stats) {
+                               float* gaussian_grads, float* sh_grads, float* stats, float* extra) {
   if (b.per_view == 0 || b.view_count == 0) return;
   preprocess_backward_batch_kernel<<<ceil_div(b.per_view, kThreads), kThreads, 0, s>>>(
       reinterpret_cast<const float4*>(gaussians), sh, sh_degree, b, reinterpret_cast<const float4*>(splats),
-      splat_grads, reinterpret_cast<float4*>(gaussian_grads), sh_grads, reinterpret_cast<float4*>(stats));
+      splat_grads, reinterpret_cast<float4*>(gaussian_grads), sh_grads, reinterpret_cast<float4*>(stats),
+      reinterpret_cast<float4*>(extra));
   DIO_LAUNCHED("preprocess_backward_batch");
 }
 

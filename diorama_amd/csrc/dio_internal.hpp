@@ -149,7 +149,7 @@ void rasterize_backward_batch(hipStream_t s, const float* splats, const uint32_t
 // every view (adam.wgsl:125-134) in the same pass.
 void preprocess_backward_batch(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree,
                                const BatchArgs& b, const float* splats, const float* splat_grads,
-                               float* gaussian_grads, float* sh_grads, float* stats);
+                               float* gaussian_grads, float* sh_grads, float* stats, float* extra = nullptr);
 
 // ---------------------------------------------------------------- loss --
 void training_loss(hipStream_t s, uint32_t width, uint32_t height, const float* rendered, const uint32_t* target,
@@ -162,8 +162,10 @@ struct AdamArgs {
   float position_lr, opacity_lr, scale_lr, rotation_lr, color_lr, sh_lr;
   float viewport[2];
 };
+// `extra` (nullable): per Gaussian (largest gradient length, summed depth,
+// summed gradient x, summed gradient y), for the learned densification.
 void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const float* splat_grads,
-                      const AdamArgs& a, float* stats);
+                      const AdamArgs& a, float* stats, float* extra = nullptr);
 void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* moments, const float* gaussian_grads,
                const float* sh_grads, const AdamArgs& a, float* gaussians);
 // The same step with its arguments read from device memory (for graph replays).
@@ -176,11 +178,50 @@ struct DensifyArgs {
   float grad_threshold, clone_max_size, min_opacity, max_size;
   uint32_t seed;
 };
+// Action codes: 0 prune, 1 keep, 2 clone, 3 split. `forced` (nullable) gives
+// every Gaussian's action instead of the rule.
 void densify_count(hipStream_t s, uint32_t n, const float* params, const float* stats, const DensifyArgs& a,
-                   uint32_t* outputs, uint32_t* counters);
+                   const uint32_t* forced, uint32_t* outputs, uint32_t* counters);
 void densify_scatter(hipStream_t s, uint32_t n, const float* params, const float* stats, const uint32_t* outputs,
-                     const float* sh, const float* moments, const DensifyArgs& a, float* new_params, float* new_sh,
-                     float* new_moments);
+                     const float* sh, const float* moments, const DensifyArgs& a, const uint32_t* forced,
+                     float* new_params, float* new_sh, float* new_moments);
+// The rule's codes and each Gaussian's mean view-space gradient as its growth priority.
+void densify_rule_codes(hipStream_t s, uint32_t n, const float* params, const float* stats, const DensifyArgs& a,
+                        uint32_t* codes, float* priority);
+// Budget: sort keys ranking clone/split candidates by priority, then demotion of
+// the candidates past the first `allowed`.
+void densify_growth_keys(hipStream_t s, uint32_t n, const uint32_t* codes, const float* priority, uint32_t* keys,
+                         uint32_t* values);
+void densify_demote(hipStream_t s, uint32_t n, const uint32_t* sorted_keys, const uint32_t* sorted_values,
+                    uint32_t allowed, const float* params, const DensifyArgs& a, uint32_t* codes);
+
+// ------------------------------------------------ learned densification --
+constexpr uint32_t kMlpInputs = 12;
+constexpr uint32_t kMlpParams = 12 * 64 + 64 + 64 * 64 + 64 + 64 * 4 + 4;  // 5,252
+void mlp_features(hipStream_t s, uint32_t n, const float* params, const float* stats, const float* extra,
+                  float extent, uint32_t iteration, float* features);
+void mlp_forward(hipStream_t s, uint32_t n, const float* features, const uint16_t* weights, float* logits);
+// Mean softmax cross-entropy gradients over `batch` samples added into `grads`
+// (cleared by the caller), the loss sum into loss[0]. seed 0: samples 0..batch-1;
+// else `batch` hashed draws from `count` samples.
+void mlp_gradients(hipStream_t s, uint32_t batch, uint32_t count, uint32_t seed, const float* features,
+                   const uint32_t* labels, const uint16_t* weights, float* grads, float* loss);
+void mlp_adam(hipStream_t s, float* master, float* moments, float* grads, uint16_t* weights, float lr,
+              float inv_bias1, float inv_bias2);
+void mlp_round(hipStream_t s, const float* master, uint16_t* weights);
+// The MLP's action where its top probability reaches `confidence` (codes holds
+// the rule's codes on entry), and P(clone) + P(split) as the growth priority.
+void mlp_learned_codes(hipStream_t s, uint32_t n, const float* logits, float confidence, uint32_t* codes,
+                       float* priority);
+// Sample logging: Gaussians drawn with probability (threshold + 1) / 2^32 become
+// pending samples (16 words each); at the next densification the ones whose
+// outcome held join the training set.
+constexpr uint32_t kMlpPendingWords = 16;
+void mlp_log_pending(hipStream_t s, uint32_t n, const float* features, const uint32_t* codes, const uint32_t* outputs,
+                     uint32_t step, uint32_t threshold, uint32_t capacity, float* pending, uint32_t* pending_count);
+void mlp_resolve(hipStream_t s, const uint32_t* pending_count, uint32_t pending_capacity, const float* pending,
+                 const float* stats, uint32_t n, uint32_t store_capacity, float* store_features,
+                 uint32_t* store_labels, uint32_t* store_count);
 
 void fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count);
 

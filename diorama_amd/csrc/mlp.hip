@@ -1,0 +1,415 @@
+// The learned densification proposal: a 12 -> 64 -> 64 -> 4 MLP over per-Gaussian
+// statistics whose outputs are keep / clone / split / prune logits. It has no
+// counterpart in the reference; the plain threshold rule (densify.wgsl:72-85)
+// supplies its training labels and stays its fallback.
+//
+// Weights are stored in BF16 and every product accumulates in FP32. Training keeps
+// an FP32 master copy for Adam and re-rounds the BF16 copy after each step. The
+// layer is small (5,252 parameters, 10.5 KB in BF16), so every workgroup holds all
+// of it in LDS as FP32.
+//
+// Layout of the 5,252 parameters: W1 [12][64], b1 [64], W2 [64][64], b2 [64],
+// W3 [64][4], b3 [4], row-major with the input index first.
+#include "dio_internal.hpp"
+
+namespace dio {
+namespace {
+
+constexpr int kIn = kMlpInputs, kHidden = 64, kOut = 4;
+constexpr int kW1 = 0, kB1 = kW1 + kIn * kHidden, kW2 = kB1 + kHidden, kB2 = kW2 + kHidden * kHidden,
+              kW3 = kB2 + kHidden, kB3 = kW3 + kHidden * kOut, kParams = kB3 + kOut;
+static_assert(kParams == static_cast<int>(kMlpParams), "the MLP layout");
+constexpr int kThreads = 256;
+
+__device__ inline float bf16_to_float(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// Round to nearest even (finite inputs).
+__device__ inline uint16_t float_to_bf16(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ inline void load_weights(float* __restrict__ w, const uint16_t* __restrict__ weights) {
+  for (int k = threadIdx.x; k < kParams; k += blockDim.x) w[k] = bf16_to_float(weights[k]);
+}
+
+// One sample's forward pass: both hidden layers (after ReLU) and the logits.
+__device__ inline void forward(const float* __restrict__ w, const float x[kIn], float h1[kHidden], float h2[kHidden],
+                               float z[kOut]) {
+#pragma unroll
+  for (int j = 0; j < kHidden; j++) {
+    float a = w[kB1 + j];
+#pragma unroll
+    for (int i = 0; i < kIn; i++) a += x[i] * w[kW1 + i * kHidden + j];
+    h1[j] = fmaxf(a, 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < kHidden; k++) {
+    float a = w[kB2 + k];
+#pragma unroll
+    for (int j = 0; j < kHidden; j++) a += h1[j] * w[kW2 + j * kHidden + k];
+    h2[k] = fmaxf(a, 0.f);
+  }
+#pragma unroll
+  for (int c = 0; c < kOut; c++) {
+    float a = w[kB3 + c];
+#pragma unroll
+    for (int k = 0; k < kHidden; k++) a += h2[k] * w[kW3 + k * kOut + c];
+    z[c] = a;
+  }
+}
+
+// The 12 inputs of one Gaussian, each brought to order one:
+//  0 log(1 + mean view-space gradient / 2e-4)   1 the same of the largest one
+//  2 opacity                                     3 log(largest scale / extent)
+//  4 log(smallest scale / extent)                5 log(largest / smallest scale)
+//  6 luminance of the SH DC colour               7 log(1 + views that saw it)
+//  8 log(1 + age in steps)                       9 log(1 + largest screen radius)
+// 10 log(mean depth)                            11 gradient-direction variance,
+//    1 - |sum of gradients| / sum of their lengths.
+__global__ __launch_bounds__(kThreads) void features_kernel(uint32_t n, const float* __restrict__ params,
+                                                             const float4* __restrict__ stats,
+                                                             const float4* __restrict__ extra, float extent,
+                                                             uint32_t iteration, float* __restrict__ features) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const float* p = params + static_cast<size_t>(i) * kFloatsPerGaussian;
+  const float4 s = stats[i], e = extra[i];
+  const float views = s.y;
+  const float mean_grad = views > 0.f ? s.x / views : 0.f;
+  const float sx = __expf(p[4]), sy = __expf(p[5]), sz = __expf(p[6]);
+  const float big = fmaxf(sx, fmaxf(sy, sz)), small = fminf(sx, fminf(sy, sz));
+  const float c0 = 0.5f + 0.28209479177387814f * p[12], c1 = 0.5f + 0.28209479177387814f * p[13],
+              c2 = 0.5f + 0.28209479177387814f * p[14];
+  const float age = fmaxf(static_cast<float>(iteration) - p[15], 0.f);
+  const float depth = views > 0.f ? e.y / views : 1.f;
+  const float pull = sqrtf(e.z * e.z + e.w * e.w);
+  float f[kIn];
+  f[0] = logf(1.f + mean_grad / 2e-4f);
+  f[1] = logf(1.f + e.x / 2e-4f);
+  f[2] = 1.f / (1.f + __expf(-p[3]));
+  f[3] = logf(big / extent);
+  f[4] = logf(small / extent);
+  f[5] = logf(big / small);
+  f[6] = 0.299f * c0 + 0.587f * c1 + 0.114f * c2;
+  f[7] = logf(1.f + views);
+  f[8] = logf(1.f + age);
+  f[9] = logf(1.f + s.z);
+  f[10] = logf(fmaxf(depth, 1e-3f));
+  f[11] = s.x > 0.f ? fmaxf(1.f - pull / s.x, 0.f) : 0.f;
+#pragma unroll
+  for (int k = 0; k < kIn; k++) features[static_cast<size_t>(i) * kIn + k] = f[k];
+}
+
+__global__ __launch_bounds__(kThreads) void mlp_forward_kernel(uint32_t n, const float* __restrict__ features,
+                                                                const uint16_t* __restrict__ weights,
+                                                                float* __restrict__ logits) {
+  __shared__ float w[kParams];
+  load_weights(w, weights);
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  float x[kIn], h1[kHidden], h2[kHidden], z[kOut];
+#pragma unroll
+  for (int k = 0; k < kIn; k++) x[k] = features[static_cast<size_t>(i) * kIn + k];
+  forward(w, x, h1, h2, z);
+#pragma unroll
+  for (int c = 0; c < kOut; c++) logits[static_cast<size_t>(i) * kOut + c] = z[c];
+}
+
+// The sample a minibatch slot reads: every sample when `seed` is 0 (slot s reads
+// sample s), else a hashed draw from the `count` samples.
+__device__ inline uint32_t pcg_hash(uint32_t v) {
+  const uint32_t state = v * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+__device__ inline uint32_t sample_of(uint32_t slot, uint32_t count, uint32_t seed) {
+  if (seed == 0) return slot;
+  return pcg_hash(pcg_hash(seed) ^ slot) % count;
+}
+
+// Gradients of the mean softmax cross-entropy over `batch` samples, added into
+// `grads` (which the caller cleared), and the loss sum into loss[0]. A workgroup
+// takes 32 samples: threads 0..31 run one sample each forward and backward and
+// leave its activations and their gradients in LDS (55 KB with the weights); then
+// all 256 threads form the weight gradients as sums of outer products over the 32
+// samples and add them out with one atomic per parameter.
+constexpr int kTrainSamples = 32;
+
+struct TrainShared {
+  float w[kParams];
+  float x[kTrainSamples][kIn];
+  float h1[kTrainSamples][kHidden];
+  float d1[kTrainSamples][kHidden];  // dL/d(pre-activation of layer 1)
+  float h2[kTrainSamples][kHidden];
+  float d2[kTrainSamples][kHidden];  // dL/d(pre-activation of layer 2)
+  float dz[kTrainSamples][kOut];
+};
+
+__global__ __launch_bounds__(kThreads) void mlp_train_kernel(uint32_t batch, uint32_t count, uint32_t seed,
+                                                              const float* __restrict__ features,
+                                                              const uint32_t* __restrict__ labels,
+                                                              const uint16_t* __restrict__ weights,
+                                                              float* __restrict__ grads, float* __restrict__ loss) {
+  __shared__ TrainShared sh;
+  load_weights(sh.w, weights);
+  __syncthreads();
+  const uint32_t first = blockIdx.x * kTrainSamples;
+  const uint32_t live = min(static_cast<uint32_t>(kTrainSamples), batch - first);
+  const int t = threadIdx.x;
+  const float inv_batch = 1.f / static_cast<float>(batch);
+  if (t < kTrainSamples) {
+    float x[kIn], h1[kHidden], h2[kHidden], z[kOut], dz[kOut];
+    const bool active = static_cast<uint32_t>(t) < live;
+    uint32_t label = 0;
+    if (active) {
+      const uint32_t sample = sample_of(first + t, count, seed);
+#pragma unroll
+      for (int k = 0; k < kIn; k++) x[k] = features[static_cast<size_t>(sample) * kIn + k];
+      label = labels[sample];
+    } else {
+#pragma unroll
+      for (int k = 0; k < kIn; k++) x[k] = 0.f;
+    }
+    forward(sh.w, x, h1, h2, z);
+    float top = z[0];
+#pragma unroll
+    for (int c = 1; c < kOut; c++) top = fmaxf(top, z[c]);
+    float total = 0.f;
+#pragma unroll
+    for (int c = 0; c < kOut; c++) total += __expf(z[c] - top);
+    const float log_total = logf(total);
+#pragma unroll
+    for (int c = 0; c < kOut; c++) {
+      const float prob = __expf(z[c] - top - log_total);
+      dz[c] = active ? (prob - (static_cast<uint32_t>(c) == label ? 1.f : 0.f)) * inv_batch : 0.f;
+    }
+    if (active) {
+      float zl = z[0];
+#pragma unroll
+      for (int c = 1; c < kOut; c++) zl = static_cast<uint32_t>(c) == label ? z[c] : zl;
+      atomicAdd(loss, top + log_total - zl);
+    }
+    float d2[kHidden];
+#pragma unroll
+    for (int k = 0; k < kHidden; k++) {
+      float a = 0.f;
+#pragma unroll
+      for (int c = 0; c < kOut; c++) a += dz[c] * sh.w[kW3 + k * kOut + c];
+      d2[k] = h2[k] > 0.f ? a : 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < kHidden; j++) {
+      float a = 0.f;
+#pragma unroll
+      for (int k = 0; k < kHidden; k++) a += d2[k] * sh.w[kW2 + j * kHidden + k];
+      sh.d1[t][j] = h1[j] > 0.f ? a : 0.f;
+      sh.h1[t][j] = h1[j];
+      sh.h2[t][j] = h2[j];
+      sh.d2[t][j] = d2[j];
+    }
+#pragma unroll
+    for (int k = 0; k < kIn; k++) sh.x[t][k] = x[k];
+#pragma unroll
+    for (int c = 0; c < kOut; c++) sh.dz[t][c] = dz[c];
+  }
+  __syncthreads();
+  // dW1 = Σ x ⊗ d1, db1 = Σ d1, dW2 = Σ h1 ⊗ d2, db2 = Σ d2, dW3 = Σ h2 ⊗ dz, db3 = Σ dz.
+  for (int e = t; e < kParams; e += kThreads) {
+    float g = 0.f;
+    if (e < kB1) {
+      const int i = e / kHidden, j = e % kHidden;
+      for (uint32_t s = 0; s < live; s++) g += sh.x[s][i] * sh.d1[s][j];
+    } else if (e < kW2) {
+      const int j = e - kB1;
+      for (uint32_t s = 0; s < live; s++) g += sh.d1[s][j];
+    } else if (e < kB2) {
+      const int j = (e - kW2) / kHidden, k = (e - kW2) % kHidden;
+      for (uint32_t s = 0; s < live; s++) g += sh.h1[s][j] * sh.d2[s][k];
+    } else if (e < kW3) {
+      const int k = e - kB2;
+      for (uint32_t s = 0; s < live; s++) g += sh.d2[s][k];
+    } else if (e < kB3) {
+      const int k = (e - kW3) / kOut, c = (e - kW3) % kOut;
+      for (uint32_t s = 0; s < live; s++) g += sh.h2[s][k] * sh.dz[s][c];
+    } else {
+      const int c = e - kB3;
+      for (uint32_t s = 0; s < live; s++) g += sh.dz[s][c];
+    }
+    atomicAdd(grads + e, g);
+  }
+}
+
+// Adam on the FP32 master weights (then their BF16 copy), clearing the gradients
+// for the next step.
+__global__ __launch_bounds__(kThreads) void mlp_adam_kernel(float* __restrict__ master, float2* __restrict__ moments,
+                                                             float* __restrict__ grads, uint16_t* __restrict__ weights,
+                                                             float lr, float inv_bias1, float inv_bias2) {
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= kParams) return;
+  const float g = grads[e];
+  float2 m = moments[e];
+  m.x = 0.9f * m.x + 0.1f * g;
+  m.y = 0.999f * m.y + 0.001f * g * g;
+  moments[e] = m;
+  const float w = master[e] - lr * (m.x * inv_bias1) / (sqrtf(m.y * inv_bias2) + 1e-8f);
+  master[e] = w;
+  weights[e] = float_to_bf16(w);
+  grads[e] = 0.f;
+}
+
+__global__ __launch_bounds__(kThreads) void mlp_round_kernel(const float* __restrict__ master,
+                                                              uint16_t* __restrict__ weights) {
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e < kParams) weights[e] = float_to_bf16(master[e]);
+}
+
+// Learned actions from the logits (keep, clone, split, prune): the most likely
+// action where its probability reaches `confidence`, else the rule's code already
+// in `codes`. The growth priority is P(clone) + P(split).
+__global__ __launch_bounds__(kThreads) void learned_codes_kernel(uint32_t n, const float* __restrict__ logits,
+                                                                  float confidence, uint32_t* __restrict__ codes,
+                                                                  float* __restrict__ priority) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  float z[kOut];
+#pragma unroll
+  for (int c = 0; c < kOut; c++) z[c] = logits[static_cast<size_t>(i) * kOut + c];
+  float top = z[0];
+  int best = 0;
+#pragma unroll
+  for (int c = 1; c < kOut; c++) {
+    if (z[c] > top) {
+      top = z[c];
+      best = c;
+    }
+  }
+  float total = 0.f, p[kOut];
+#pragma unroll
+  for (int c = 0; c < kOut; c++) total += p[c] = __expf(z[c] - top);
+  const float inv = 1.f / total;
+  constexpr uint32_t kCode[kOut] = {1u, 2u, 3u, 0u};  // class -> action code
+  if (p[best] * inv >= confidence) codes[i] = kCode[best];
+  priority[i] = (p[1] + p[2]) * inv;
+}
+
+// A pending sample: the 12 inputs, the rule's class, the Gaussian's first index
+// after the densification, and its mean view-space gradient before it.
+constexpr int kPendingWords = 16;
+
+__device__ inline uint32_t class_of_code(uint32_t code) { return code == 0u ? 3u : code - 1u; }
+
+__global__ __launch_bounds__(kThreads) void log_pending_kernel(uint32_t n, const float* __restrict__ features,
+                                                                const uint32_t* __restrict__ codes,
+                                                                const uint32_t* __restrict__ outputs,
+                                                                uint32_t step, uint32_t threshold,
+                                                                uint32_t capacity, float* __restrict__ pending,
+                                                                uint32_t* __restrict__ pending_count) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n || pcg_hash(pcg_hash(step) ^ i) > threshold) return;
+  const uint32_t slot = atomicAdd(pending_count, 1u);
+  if (slot >= capacity) return;
+  float* out = pending + static_cast<size_t>(slot) * kPendingWords;
+#pragma unroll
+  for (int k = 0; k < kIn; k++) out[k] = features[static_cast<size_t>(i) * kIn + k];
+  out[12] = __uint_as_float(class_of_code(codes[i]));
+  out[13] = __uint_as_float(outputs[i]);
+  out[14] = features[static_cast<size_t>(i) * kIn];  // log(1 + mean gradient / 2e-4)
+  out[15] = 0.f;
+}
+
+// A pending sample enters the training set when its outcome held: the Gaussian
+// (its first descendant) was pruned, or its mean view-space gradient over the
+// steps since fell below what it was.
+__global__ __launch_bounds__(kThreads) void resolve_kernel(const uint32_t* __restrict__ pending_count,
+                                                            uint32_t pending_capacity,
+                                                            const float* __restrict__ pending,
+                                                            const float4* __restrict__ stats, uint32_t n,
+                                                            uint32_t store_capacity, float* __restrict__ store_features,
+                                                            uint32_t* __restrict__ store_labels,
+                                                            uint32_t* __restrict__ store_count) {
+  const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= min(*pending_count, pending_capacity)) return;
+  const float* in = pending + static_cast<size_t>(j) * kPendingWords;
+  const uint32_t label = __float_as_uint(in[12]);
+  bool held = label == 3u;
+  if (!held) {
+    const uint32_t o = __float_as_uint(in[13]);
+    if (o >= n) return;
+    const float4 s = stats[o];
+    const float after = logf(1.f + (s.y > 0.f ? s.x / s.y : 0.f) / 2e-4f);
+    held = after < in[14];
+  }
+  if (!held) return;
+  const uint32_t slot = atomicAdd(store_count, 1u);
+  if (slot >= store_capacity) return;
+#pragma unroll
+  for (int k = 0; k < kIn; k++) store_features[static_cast<size_t>(slot) * kIn + k] = in[k];
+  store_labels[slot] = label;
+}
+
+}  // namespace
+
+void mlp_learned_codes(hipStream_t s, uint32_t n, const float* logits, float confidence, uint32_t* codes,
+                       float* priority) {
+  if (n == 0) return;
+  learned_codes_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, logits, confidence, codes, priority);
+  DIO_LAUNCHED("mlp_learned_codes");
+}
+
+void mlp_log_pending(hipStream_t s, uint32_t n, const float* features, const uint32_t* codes, const uint32_t* outputs,
+                     uint32_t step, uint32_t threshold, uint32_t capacity, float* pending, uint32_t* pending_count) {
+  if (n == 0) return;
+  log_pending_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, features, codes, outputs, step, threshold, capacity,
+                                                                pending, pending_count);
+  DIO_LAUNCHED("mlp_log_pending");
+}
+
+void mlp_resolve(hipStream_t s, const uint32_t* pending_count, uint32_t pending_capacity, const float* pending,
+                 const float* stats, uint32_t n, uint32_t store_capacity, float* store_features,
+                 uint32_t* store_labels, uint32_t* store_count) {
+  if (pending_capacity == 0) return;
+  resolve_kernel<<<ceil_div(pending_capacity, kThreads), kThreads, 0, s>>>(
+      pending_count, pending_capacity, pending, reinterpret_cast<const float4*>(stats), n, store_capacity,
+      store_features, store_labels, store_count);
+  DIO_LAUNCHED("mlp_resolve");
+}
+
+void mlp_features(hipStream_t s, uint32_t n, const float* params, const float* stats, const float* extra,
+                  float extent, uint32_t iteration, float* features) {
+  if (n == 0) return;
+  features_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, params, reinterpret_cast<const float4*>(stats),
+                                                              reinterpret_cast<const float4*>(extra), extent,
+                                                              iteration, features);
+  DIO_LAUNCHED("mlp_features");
+}
+
+void mlp_forward(hipStream_t s, uint32_t n, const float* features, const uint16_t* weights, float* logits) {
+  if (n == 0) return;
+  mlp_forward_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, features, weights, logits);
+  DIO_LAUNCHED("mlp_forward");
+}
+
+void mlp_gradients(hipStream_t s, uint32_t batch, uint32_t count, uint32_t seed, const float* features,
+                   const uint32_t* labels, const uint16_t* weights, float* grads, float* loss) {
+  if (batch == 0) return;
+  mlp_train_kernel<<<ceil_div(batch, kTrainSamples), kThreads, 0, s>>>(
+      batch, count, seed, features, labels, weights, grads, loss);
+  DIO_LAUNCHED("mlp_train");
+}
+
+void mlp_adam(hipStream_t s, float* master, float* moments, float* grads, uint16_t* weights, float lr,
+              float inv_bias1, float inv_bias2) {
+  mlp_adam_kernel<<<ceil_div(kParams, kThreads), kThreads, 0, s>>>(master, reinterpret_cast<float2*>(moments), grads,
+                                                                     weights, lr, inv_bias1, inv_bias2);
+  DIO_LAUNCHED("mlp_adam");
+}
+
+void mlp_round(hipStream_t s, const float* master, uint16_t* weights) {
+  mlp_round_kernel<<<ceil_div(kParams, kThreads), kThreads, 0, s>>>(master, weights);
+  DIO_LAUNCHED("mlp_round");
+}
+
+}  // namespace dio

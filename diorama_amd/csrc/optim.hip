@@ -128,17 +128,25 @@ __global__ __launch_bounds__(kThreads) void activate_kernel(uint32_t n, const fl
   gaussians[f] = activated(slot, params[f]);
 }
 
+// adam.wgsl:125-134. `extra` (optional, for the learned densification) gathers
+// (largest gradient length, summed depth, summed gradient x, summed gradient y).
 __global__ __launch_bounds__(kThreads) void stats_kernel(uint32_t n, const float4* __restrict__ splats,
                                                           const float* __restrict__ splat_grads, float vx, float vy,
-                                                          float4* __restrict__ stats) {
+                                                          float4* __restrict__ stats, float4* __restrict__ extra) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
-  const float radius = splats[i * 3].w;
+  const float4 s0 = splats[i * 3];
+  const float radius = s0.w;
   if (radius == 0.f) return;
   const float gx = splat_grads[static_cast<size_t>(i) * 9] * vx * 0.5f;
   const float gy = splat_grads[static_cast<size_t>(i) * 9 + 1] * vy * 0.5f;
+  const float length = sqrtf(gx * gx + gy * gy);
   const float4 s = stats[i];
-  stats[i] = make_float4(s.x + sqrtf(gx * gx + gy * gy), s.y + 1.f, fmaxf(s.z, radius), 0.f);
+  stats[i] = make_float4(s.x + length, s.y + 1.f, fmaxf(s.z, radius), 0.f);
+  if (extra) {
+    const float4 e = extra[i];
+    extra[i] = make_float4(fmaxf(e.x, length), e.y + s0.z, e.z + gx, e.w + gy);
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void reset_opacity_kernel(uint32_t n, float* __restrict__ params,
@@ -161,7 +169,7 @@ __device__ inline bool removed(const DensifyArgs& a, float opacity, float size) 
 }
 
 // densify.wgsl:72-85: the reference clones, then splits, then prunes the result.
-__device__ inline uint32_t action(const float* params, const float4* stats, const DensifyArgs& a, uint32_t i) {
+__device__ inline uint32_t rule_action(const float* params, const float4* stats, const DensifyArgs& a, uint32_t i) {
   const float* p = params + static_cast<size_t>(i) * kFloatsPerGaussian;
   const float4 s = stats[i];
   const float grad = s.y > 0.f ? s.x / s.y : 0.f;
@@ -174,8 +182,56 @@ __device__ inline uint32_t action(const float* params, const float4* stats, cons
   return removed(a, opacity, size) ? kDrop : kKeep;
 }
 
+// The rule's action, or the one chosen for Gaussian i beforehand (`forced`).
+__device__ inline uint32_t action(const float* params, const float4* stats, const DensifyArgs& a,
+                                  const uint32_t* forced, uint32_t i) {
+  return forced ? forced[i] : rule_action(params, stats, a, i);
+}
+
+// The rule's actions, and each Gaussian's claim to grow (its mean view-space
+// gradient) for a budget that cannot take every clone and split.
+__global__ __launch_bounds__(kThreads) void rule_codes_kernel(uint32_t n, const float* __restrict__ params,
+                                                               const float4* __restrict__ stats, DensifyArgs a,
+                                                               uint32_t* __restrict__ codes,
+                                                               float* __restrict__ priority) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  codes[i] = rule_action(params, stats, a, i);
+  const float4 st = stats[i];
+  priority[i] = st.y > 0.f ? st.x / st.y : 0.f;
+}
+
+// Growth candidates (clone or split) in order of priority, highest first, as sort
+// keys (the other Gaussians sort last).
+__global__ __launch_bounds__(kThreads) void growth_keys_kernel(uint32_t n, const uint32_t* __restrict__ codes,
+                                                                const float* __restrict__ priority,
+                                                                uint32_t* __restrict__ keys,
+                                                                uint32_t* __restrict__ values) {
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = codes[i];
+  const bool grows = c == kClone || c == kSplit;
+  keys[i] = grows ? ~__float_as_uint(fmaxf(priority[i], 0.f)) : 0xffffffffu;
+  values[i] = i;
+}
+
+// Growth past the first `allowed` candidates of that order falls back to keeping
+// (or pruning, where the rule prunes the unchanged Gaussian).
+__global__ __launch_bounds__(kThreads) void demote_kernel(uint32_t n, const uint32_t* __restrict__ sorted_keys,
+                                                           const uint32_t* __restrict__ sorted_values,
+                                                           uint32_t allowed, const float* __restrict__ params,
+                                                           DensifyArgs a, uint32_t* __restrict__ codes) {
+  const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= n || k < allowed || sorted_keys[k] == 0xffffffffu) return;
+  const uint32_t i = sorted_values[k];
+  const float* p = params + static_cast<size_t>(i) * kFloatsPerGaussian;
+  const float size = __expf(fmaxf(p[4], fmaxf(p[5], p[6])));
+  codes[i] = removed(a, sigmoid(p[3]), size) ? kDrop : kKeep;
+}
+
 __global__ __launch_bounds__(kThreads) void densify_count_kernel(uint32_t n, const float* __restrict__ params,
                                                                   const float4* __restrict__ stats, DensifyArgs a,
+                                                                  const uint32_t* __restrict__ forced,
                                                                   uint32_t* __restrict__ outputs,
                                                                   uint32_t* __restrict__ counters) {
   __shared__ uint32_t tally[4];
@@ -183,7 +239,7 @@ __global__ __launch_bounds__(kThreads) void densify_count_kernel(uint32_t n, con
   __syncthreads();
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i < n) {
-    const uint32_t act = action(params, stats, a, i);
+    const uint32_t act = action(params, stats, a, forced, i);
     outputs[i] = act == kDrop ? 0u : (act == kKeep ? 1u : 2u);
     atomicAdd(&tally[act], 1u);
   }
@@ -213,10 +269,11 @@ __device__ inline void copy_gaussian(uint32_t i, uint32_t o, bool keep_moments, 
 __global__ __launch_bounds__(kThreads) void densify_scatter_kernel(
     uint32_t n, const float* __restrict__ params, const float4* __restrict__ stats,
     const uint32_t* __restrict__ outputs, const float* __restrict__ sh, const float2* __restrict__ moments,
-    DensifyArgs a, float* __restrict__ new_params, float* __restrict__ new_sh, float2* __restrict__ new_moments) {
+    DensifyArgs a, const uint32_t* __restrict__ forced, float* __restrict__ new_params, float* __restrict__ new_sh,
+    float2* __restrict__ new_moments) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
-  const uint32_t act = action(params, stats, a, i);
+  const uint32_t act = action(params, stats, a, forced, i);
   const uint32_t o = outputs[i];
   if (act == kKeep || act == kClone) copy_gaussian(i, o, true, params, sh, moments, new_params, new_sh, new_moments);
   if (act == kClone) {
@@ -254,11 +311,12 @@ __global__ __launch_bounds__(kThreads) void densify_scatter_kernel(
 }  // namespace
 
 void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const float* splat_grads, const AdamArgs& a,
-                      float* stats) {
+                      float* stats, float* extra) {
   if (n == 0) return;
   stats_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, reinterpret_cast<const float4*>(splats), splat_grads,
                                                           a.viewport[0], a.viewport[1],
-                                                          reinterpret_cast<float4*>(stats));
+                                                          reinterpret_cast<float4*>(stats),
+                                                          reinterpret_cast<float4*>(extra));
   DIO_LAUNCHED("accumulate_stats");
 }
 
@@ -295,22 +353,44 @@ void reset_opacity(hipStream_t s, uint32_t n, float* params, float* moments, flo
 }
 
 void densify_count(hipStream_t s, uint32_t n, const float* params, const float* stats, const DensifyArgs& a,
-                   uint32_t* outputs, uint32_t* counters) {
+                   const uint32_t* forced, uint32_t* outputs, uint32_t* counters) {
   fill_u32(s, counters, 0u, 4);
   if (n == 0) return;
   densify_count_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, params, reinterpret_cast<const float4*>(stats), a,
-                                                                  outputs, counters);
+                                                                  forced, outputs, counters);
   DIO_LAUNCHED("densify_count");
 }
 
 void densify_scatter(hipStream_t s, uint32_t n, const float* params, const float* stats, const uint32_t* outputs,
-                     const float* sh, const float* moments, const DensifyArgs& a, float* new_params, float* new_sh,
-                     float* new_moments) {
+                     const float* sh, const float* moments, const DensifyArgs& a, const uint32_t* forced,
+                     float* new_params, float* new_sh, float* new_moments) {
   if (n == 0) return;
   densify_scatter_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(
       n, params, reinterpret_cast<const float4*>(stats), outputs, sh, reinterpret_cast<const float2*>(moments), a,
-      new_params, new_sh, reinterpret_cast<float2*>(new_moments));
+      forced, new_params, new_sh, reinterpret_cast<float2*>(new_moments));
   DIO_LAUNCHED("densify_scatter");
+}
+
+void densify_rule_codes(hipStream_t s, uint32_t n, const float* params, const float* stats, const DensifyArgs& a,
+                        uint32_t* codes, float* priority) {
+  if (n == 0) return;
+  rule_codes_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, params, reinterpret_cast<const float4*>(stats), a,
+                                                               codes, priority);
+  DIO_LAUNCHED("densify_rule_codes");
+}
+
+void densify_growth_keys(hipStream_t s, uint32_t n, const uint32_t* codes, const float* priority, uint32_t* keys,
+                         uint32_t* values) {
+  if (n == 0) return;
+  growth_keys_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, codes, priority, keys, values);
+  DIO_LAUNCHED("densify_growth_keys");
+}
+
+void densify_demote(hipStream_t s, uint32_t n, const uint32_t* sorted_keys, const uint32_t* sorted_values,
+                    uint32_t allowed, const float* params, const DensifyArgs& a, uint32_t* codes) {
+  if (n == 0) return;
+  demote_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, sorted_keys, sorted_values, allowed, params, a, codes);
+  DIO_LAUNCHED("densify_demote");
 }
 
 }  // namespace dio

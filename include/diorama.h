@@ -234,6 +234,45 @@ dio_renderer* dio_trainer_renderer(dio_trainer* t);
 void dio_trainer_destroy(dio_trainer* t);
 
 /* --------------------------------------------------------- measurement -- */
+/* ------------------------------------------------ learned densification -- */
+/* No counterpart in the reference; for callers that densify with a learned
+ * proposal. A 12 -> 64 -> 64 -> 4 MLP (5,252 parameters, BF16 weights, FP32
+ * accumulation) reads 12 statistics per Gaussian and scores keep / clone / split
+ * / prune. Weights cross the boundary as 5,252 floats: W1[12][64], b1[64],
+ * W2[64][64], b2[64], W3[64][4], b3[4], input index first. */
+typedef struct dio_densify_mlp dio_densify_mlp;
+#define DIO_MLP_PARAMS 5252
+#define DIO_MLP_INPUTS 12
+int dio_mlp_create(dio_ctx* ctx, uint32_t seed, dio_densify_mlp** out); /* He-uniform, zero biases */
+int dio_mlp_get_weights(dio_densify_mlp* mlp, float* weights);        /* the BF16 weights as floats */
+int dio_mlp_set_weights(dio_densify_mlp* mlp, const float* weights);  /* rounded to BF16; resets Adam */
+/* features: count x 12 floats; logits: count x 4 floats (keep, clone, split, prune). */
+int dio_mlp_forward(dio_densify_mlp* mlp, dio_buf* features, uint32_t count, dio_buf* logits);
+/* Gradients of the mean softmax cross-entropy over all `count` samples (labels:
+ * u32 class 0 keep, 1 clone, 2 split, 3 prune) and that mean loss. */
+int dio_mlp_gradients(dio_densify_mlp* mlp, dio_buf* features, dio_buf* labels, uint32_t count, float* grads,
+                      double* loss);
+/* `steps` Adam steps, each on `batch` samples drawn (hashed from `seed`) from the
+ * `count` samples; last_loss (nullable) gets the last step's mean loss. */
+int dio_mlp_train(dio_densify_mlp* mlp, dio_buf* features, dio_buf* labels, uint32_t count, uint32_t steps,
+                  uint32_t batch, float lr, uint32_t seed, double* last_loss);
+void dio_mlp_destroy(dio_densify_mlp* mlp);
+/* mode 0: the plain rule (the default); 1: the plain rule, logging samples (the
+ * 12 inputs and the rule's class of Gaussians drawn at each densification,
+ * kept when at the next densification the Gaussian was pruned or its mean
+ * view-space gradient fell); 2: the MLP's class where its probability reaches
+ * `confidence`, else the rule's. budget > 0 caps the count after densifying,
+ * growth ranked by mean gradient (rule) or P(clone) + P(split) (MLP). */
+int dio_trainer_set_densify_mode(dio_trainer* t, uint32_t mode, dio_densify_mlp* mlp, uint32_t budget,
+                                 float confidence, uint32_t samples_per_densify);
+/* The logged samples (non-owning views) and how many there are. */
+int dio_trainer_samples(dio_trainer* t, dio_buf** features, dio_buf** labels, uint32_t* count);
+/* The 12 inputs of the current Gaussians (a trainer-owned count x 12 buffer). */
+int dio_trainer_features(dio_trainer* t, dio_buf** features);
+/* Per Gaussian (largest gradient length, summed depth, summed gradient x, y)
+ * since the last densification; null in mode 0. */
+dio_buf* dio_trainer_extra_stats(dio_trainer* t);
+
 /* Times `repeats` launches of one named stage between HIP events recorded on the
  * context's stream, for the benchmark's roofline figures. Stages: "preprocess",
  * "scan", "sort", "rasterize", "rasterize_backward", "preprocess_backward",
