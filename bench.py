@@ -328,6 +328,143 @@ def batch_bench(args):
     return 0
 
 
+def scaled_settings(settings, factor):
+    """The schedule of `settings` with every step count divided by `factor` (a batch
+    step sees `factor` views)."""
+    out = dict(settings)
+    for key in ("position_lr_steps", "sh_degree_interval", "densify_from", "densify_until", "densify_interval",
+                "opacity_reset_interval"):
+        out[key] = max(1, int(round(settings[key] / factor)))
+    return out
+
+
+def learned_bench(args):
+    """Learned densification on the ring scene at a fixed Gaussian budget.
+
+    A: the plain rule from 100k Gaussians for --train-steps steps, logging samples
+       (inputs and outcome-filtered rule actions) from step 500 to --log-until;
+    the MLP then trains on them (2,000 Adam steps of 4,096 samples);
+    B: the same run densified by the MLP (the rule where it is unsure);
+    C: B batched, 16 views per step with the schedule divided by 16.
+    Every run holds the 1.1M budget; all are scored on the 33 held-out views."""
+    import diorama_amd
+    from diorama_amd import scene
+
+    ctx = diorama_amd.Context(0)
+    cams = ring_poses(TRAIN_VIEWS + HELD_OUT)
+    train, held = split_views(cams)
+    t0 = time.perf_counter()
+    images = teacher_targets(ctx, train + held, N_GAUSSIANS)
+    train_images, held_images = images[:len(train)], images[len(train):]
+    log(f"teacher targets: {len(images)} views in {time.perf_counter() - t0:.1f} s")
+    params, sh = scene.random_raw_gaussians(args.initial, 0, box=5.0)
+    extent = 1.1 * max(math.sqrt(sum(v * v for v in _eye(c))) for c in train)
+    settings = diorama_amd.default_settings(extent)
+    budget = N_GAUSSIANS
+    steps = args.train_steps
+    per_densify = max(1000, 5_000_000 // max(1, (min(args.log_until, settings["densify_until"]) - 500) // 100))
+
+    def run(name, trainer, n_steps, on_step=None):
+        t0 = time.perf_counter()
+        for k in range(1, n_steps + 1):
+            trainer.step()
+            if on_step:
+                on_step(k)
+            if k % 2000 == 0:
+                log(f"{name}: step {k}/{n_steps}, {trainer.count} Gaussians, {time.perf_counter() - t0:.1f} s")
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        psnr = trainer.evaluate(held, held_images, with_ssim=False)["psnr"]
+        log(f"{name}: {n_steps} steps in {elapsed:.1f} s, {trainer.count} Gaussians, held-out PSNR {psnr:.3f} dB")
+        return {"steps": n_steps, "seconds": round(elapsed, 2), "steps_per_s": round(n_steps / elapsed, 2),
+                "views_per_s": round(n_steps * max(1, trainer.batch_size) / elapsed, 2), "gaussians": trainer.count,
+                "held_out_psnr": round(psnr, 4), "densifications": len(trainer.densifications)}
+
+    out = {}
+    a = diorama_amd.Trainer(ctx, params, sh, train, train_images, settings)
+    a.set_densify_mode("log", budget=budget, samples_per_densify=per_densify)
+
+    def stop_logging(k):
+        if k == args.log_until:
+            a.set_densify_mode("plain", budget=budget)
+
+    out["plain"] = run("plain", a, steps, stop_logging)
+    feats, labels, count = a.samples()
+    classes = np.bincount(labels.download(np.uint32, count), minlength=4).tolist() if count else [0, 0, 0, 0]
+    net = diorama_amd.DensifyMLP(ctx, seed=1)
+    if count:  # standardise the inputs over the logged samples
+        x = feats.download(np.float32, count * 12).reshape(count, 12).astype(np.float64)
+        bad = ~np.isfinite(x)
+        if bad.any():
+            rows = np.nonzero(bad.any(axis=1))[0]
+            log(f"non-finite inputs: {bad.sum(axis=0).tolist()} per input, {len(rows)} samples; first: "
+                f"{x[rows[:3]].tolist()} labels {labels.download(np.uint32, count)[rows[:3]].tolist()}")
+        good = x[~bad.any(axis=1)]
+        log(f"inputs min {np.round(good.min(axis=0), 3).tolist()} max {np.round(good.max(axis=0), 3).tolist()}")
+        net.set_input_norm(good.mean(axis=0), 1.0 / np.maximum(good.std(axis=0), 1e-6))
+    # square-root inverse class frequency: the rule keeps almost every Gaussian
+    weights = [min(30.0, math.sqrt(max(count, 1) / (4.0 * max(c, 1)))) for c in classes]
+    net.set_class_weights(weights)
+    t0 = time.perf_counter()
+    mlp_loss = net.train(feats, labels, count, steps=2000, batch=4096, lr=2e-3, seed=7) if count else float("nan")
+    mlp_s = time.perf_counter() - t0
+    log(f"mlp: {count} samples {classes}, 2000 Adam steps in {mlp_s:.2f} s, loss {mlp_loss:.4f}")
+    confusion = None
+    if count:
+        logits = ctx.buffer(count * 16)
+        net.forward(feats, count, logits)
+        pred = np.argmax(logits.download(np.float32, count * 4).reshape(count, 4), axis=1)
+        lab = labels.download(np.uint32, count)
+        confusion = np.zeros((4, 4), dtype=np.int64)
+        np.add.at(confusion, (lab, pred), 1)
+        x = feats.download(np.float32, count * 12).reshape(count, 12)
+        for c in range(4):
+            if (lab == c).any():
+                log(f"class {c}: mean inputs {np.round(x[lab == c].mean(axis=0), 3).tolist()}")
+        log(f"mlp confusion (rows: label, cols: predicted): {confusion.tolist()}")
+        logits.destroy()
+    a.destroy()
+    b = diorama_amd.Trainer(ctx, params, sh, train, train_images, settings)
+    b.set_densify_mode("learned", net, budget=budget)
+    out["learned"] = run("learned", b, steps)
+    b.destroy()
+    if not args.no_batch:
+        c = diorama_amd.Trainer(ctx, params, sh, train, train_images, scaled_settings(settings, BATCH_VIEWS),
+                                batch=BATCH_VIEWS)
+        c.set_densify_mode("learned", net, budget=budget)
+        out["batched_learned"] = run("batched+learned", c, steps // BATCH_VIEWS)
+        c.destroy()
+    net.destroy()
+    ctx.close()
+    delta = out["learned"]["held_out_psnr"] - out["plain"]["held_out_psnr"]
+    line = {
+        "metric": "held-out PSNR delta vs plain (learned densification, 1.1M budget)",
+        "value": round(delta, 4),
+        "unit": "dB",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "higher_is_better": True,
+        "dtype": "fp32 (MLP: bf16 weights, fp32 accumulation)",
+        "data": "synthetic: seed-7 1.1M teacher renders + N(0,0.01) noise, 263 train / 33 held-out ring poses "
+                "at 980x545; seed-0 student",
+        "config": {"workload": "plain rule vs learned densification vs batched+learned", "initial": args.initial,
+                   "budget": budget, "train_steps": steps, "log_until": args.log_until,
+                   "samples": count, "sample_classes": classes, "class_weights": [round(w, 3) for w in weights],
+                   "mlp_loss": round(mlp_loss, 4),
+                   "mlp_confusion": confusion.tolist() if confusion is not None else None,
+                   "mlp_train_s": round(mlp_s, 3)},
+        "runs": out,
+    }
+    if "batched_learned" in out:
+        line["batched_learned_psnr_delta"] = round(out["batched_learned"]["held_out_psnr"] -
+                                                   out["plain"]["held_out_psnr"], 4)
+        line["batched_learned_views_per_s_ratio"] = round(out["batched_learned"]["views_per_s"] /
+                                                          out["plain"]["views_per_s"], 3)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE and
     WRITE_SIZE already scaled by the copy calibration), or None."""
@@ -350,9 +487,13 @@ def main():
     ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--repeats", type=int, default=20, help="launches per stage timing")
-    ap.add_argument("--workload", choices=["train", "scan-sort", "batch"], default="train",
+    ap.add_argument("--workload", choices=["train", "scan-sort", "batch", "learned"], default="train",
                     help="train: the training step (default); scan-sort: the scan/sort microbenchmark; "
-                         "batch: the multi-view batch trainer")
+                         "batch: the multi-view batch trainer; learned: learned vs plain densification")
+    ap.add_argument("--train-steps", type=int, default=30000, help="learned: training steps per run")
+    ap.add_argument("--log-until", type=int, default=7000, help="learned: last step that logs samples")
+    ap.add_argument("--initial", type=int, default=100_000, help="learned: initial Gaussians")
+    ap.add_argument("--no-batch", action="store_true", help="learned: skip the batched run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (the FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -366,6 +507,8 @@ def main():
         return scan_sort_bench(args)
     if args.workload == "batch":
         return batch_bench(args)
+    if args.workload == "learned":
+        return learned_bench(args)
 
     import diorama_amd
     from diorama_amd import scene

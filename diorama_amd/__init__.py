@@ -132,6 +132,8 @@ SIGNATURES = {
     "dio_mlp_forward": (_I, [_P, _P, _U32, _P]),
     "dio_mlp_gradients": (_I, [_P, _P, _P, _U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]),
     "dio_mlp_train": (_I, [_P, _P, _P, _U32, _U32, _U32, ctypes.c_float, _U32, ctypes.POINTER(ctypes.c_double)]),
+    "dio_mlp_set_class_weights": (_I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "dio_mlp_set_input_norm": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "dio_mlp_destroy": (None, [_P]),
     "dio_trainer_set_densify_mode": (_I, [_P, _U32, _P, _U32, ctypes.c_float, _U32]),
     "dio_trainer_samples": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_U32)]),
@@ -567,6 +569,16 @@ class DensifyMLP:
         _check(lib().dio_mlp_gradients(self.handle, features.handle, labels.handle, count,
                                        g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(loss)))
         return g, loss.value
+
+    def set_input_norm(self, mean, scale):
+        m = np.ascontiguousarray(mean, dtype=np.float32).reshape(MLP_INPUTS)
+        sc = np.ascontiguousarray(scale, dtype=np.float32).reshape(MLP_INPUTS)
+        _check(lib().dio_mlp_set_input_norm(self.handle, m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                            sc.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def set_class_weights(self, weights):
+        a = np.ascontiguousarray(weights, dtype=np.float32).reshape(4)
+        _check(lib().dio_mlp_set_class_weights(self.handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
     def train(self, features, labels, count, steps, batch=4096, lr=1e-3, seed=1):
         loss = ctypes.c_double()

@@ -995,7 +995,9 @@ constexpr size_t kBatchHostBytes = kBatchDescBytes + sizeof(dio::AdamArgs);
 struct dio_densify_mlp {
   dio_ctx* ctx = nullptr;
   dio_buf *weights = nullptr, *master = nullptr, *moments = nullptr, *grads = nullptr, *loss = nullptr;
+  dio_buf* norm = nullptr;  // input standardisation: mean[12], scale[12]
   uint32_t adam_steps = 0;
+  float class_weights[4] = {1.f, 1.f, 1.f, 1.f};
 };
 
 // A trainer's learned-densification state. mode 0: the plain rule; 1: the plain
@@ -1183,7 +1185,8 @@ bool densify_codes(dio_trainer* t, const dio::DensifyArgs& a, dio_buf* codes, di
     dio::mlp_features(s, n, dev<float>(t->params), dev<float>(t->stats), dev<float>(L.extra), t->settings.extent,
                       t->iteration, dev<float>(L.features));
     if (L.mode == 2) {
-      dio::mlp_forward(s, n, dev<float>(L.features), dev<uint16_t>(L.mlp->weights), dev<float>(L.logits));
+      dio::mlp_forward(s, n, dev<float>(L.features), dev<uint16_t>(L.mlp->weights), dev<float>(L.mlp->norm),
+                       dev<float>(L.logits));
       dio::mlp_learned_codes(s, n, dev<float>(L.logits), L.confidence, dev<uint32_t>(codes), dev<float>(priority));
     }
     if (L.mode == 1 && L.pending_valid)  // the last logging step's samples, judged on these statistics
@@ -1925,6 +1928,7 @@ void destroy_mlp(dio_densify_mlp* m) {
   if (!m) return;
   (void)hipStreamSynchronize(m->ctx->stream);
   release(m->weights); release(m->master); release(m->moments); release(m->grads); release(m->loss);
+  release(m->norm);
   delete m;
 }
 
@@ -1952,6 +1956,10 @@ extern "C" int dio_mlp_create(dio_ctx* ctx, uint32_t seed, dio_densify_mlp** out
       m->moments = alloc(ctx, dio::kMlpParams * 8, true);
       m->grads = alloc(ctx, dio::kMlpParams * 4, true);
       m->loss = alloc(ctx, 16, true);
+      m->norm = alloc(ctx, 2 * dio::kMlpInputs * 4, true);
+      std::vector<float> identity(2 * dio::kMlpInputs, 0.f);
+      for (uint32_t k = 0; k < dio::kMlpInputs; k++) identity[dio::kMlpInputs + k] = 1.f;
+      DIO_HIP(hipMemcpyAsync(m->norm->ptr, identity.data(), identity.size() * 4, hipMemcpyHostToDevice, ctx->stream));
       // He-uniform layers, zero biases, from mulberry32(seed).
       std::vector<float> w(dio::kMlpParams, 0.f);
       Mulberry32 random(seed);
@@ -2000,7 +2008,8 @@ extern "C" int dio_mlp_forward(dio_densify_mlp* m, dio_buf* features, uint32_t c
     need(logits, "logits");
     if (features->bytes < static_cast<size_t>(count) * dio::kMlpInputs * 4 || logits->bytes < static_cast<size_t>(count) * 16)
       invalid("the feature or logit buffer is smaller than count samples");
-    dio::mlp_forward(m->ctx->stream, count, dev<float>(features), dev<uint16_t>(m->weights), dev<float>(logits));
+    dio::mlp_forward(m->ctx->stream, count, dev<float>(features), dev<uint16_t>(m->weights), dev<float>(m->norm),
+                     dev<float>(logits));
   });
 }
 
@@ -2017,7 +2026,7 @@ extern "C" int dio_mlp_gradients(dio_densify_mlp* m, dio_buf* features, dio_buf*
     hipStream_t s = m->ctx->stream;
     dio::fill_u32(s, dev<uint32_t>(m->loss), 0u, 1);
     dio::mlp_gradients(s, count, count, 0, dev<float>(features), dev<uint32_t>(labels), dev<uint16_t>(m->weights),
-                       dev<float>(m->grads), dev<float>(m->loss));
+                       dev<float>(m->norm), m->class_weights, dev<float>(m->grads), dev<float>(m->loss));
     float total = 0.f;
     DIO_HIP(hipMemcpyAsync(grads, m->grads->ptr, dio::kMlpParams * 4, hipMemcpyDeviceToHost, s));
     DIO_HIP(hipMemcpyAsync(&total, m->loss->ptr, 4, hipMemcpyDeviceToHost, s));
@@ -2042,7 +2051,8 @@ extern "C" int dio_mlp_train(dio_densify_mlp* m, dio_buf* features, dio_buf* lab
       // a nonzero seed per step: `batch` hashed draws from the samples
       const uint32_t step_seed = (seed * 2654435761u) ^ (k * 40503u + 1u) | 1u;
       dio::mlp_gradients(s, batch, count, step_seed, dev<float>(features), dev<uint32_t>(labels),
-                         dev<uint16_t>(m->weights), dev<float>(m->grads), dev<float>(m->loss));
+                         dev<uint16_t>(m->weights), dev<float>(m->norm), m->class_weights, dev<float>(m->grads),
+                         dev<float>(m->loss));
       m->adam_steps++;
       const float inv1 = static_cast<float>(1.0 / (1.0 - std::pow(0.9, m->adam_steps)));
       const float inv2 = static_cast<float>(1.0 / (1.0 - std::pow(0.999, m->adam_steps)));
@@ -2053,6 +2063,33 @@ extern "C" int dio_mlp_train(dio_densify_mlp* m, dio_buf* features, dio_buf* lab
     DIO_HIP(hipMemcpyAsync(&total, m->loss->ptr, 4, hipMemcpyDeviceToHost, s));
     sync(m->ctx);
     if (last_loss) *last_loss = total / batch;
+  });
+}
+
+extern "C" int dio_mlp_set_input_norm(dio_densify_mlp* m, const float mean[12], const float scale[12]) {
+  return guard([&] {
+    need(m, "mlp");
+    need(mean, "mean");
+    need(scale, "scale");
+    float host[2 * dio::kMlpInputs];
+    for (uint32_t k = 0; k < dio::kMlpInputs; k++) {
+      if (!std::isfinite(mean[k]) || !std::isfinite(scale[k])) invalid("the input standardisation must be finite");
+      host[k] = mean[k];
+      host[dio::kMlpInputs + k] = scale[k];
+    }
+    DIO_HIP(hipMemcpyAsync(m->norm->ptr, host, sizeof(host), hipMemcpyHostToDevice, m->ctx->stream));
+    sync(m->ctx);
+  });
+}
+
+extern "C" int dio_mlp_set_class_weights(dio_densify_mlp* m, const float weights[4]) {
+  return guard([&] {
+    need(m, "mlp");
+    need(weights, "weights");
+    for (int c = 0; c < 4; c++) {
+      if (!(weights[c] >= 0.f)) invalid("class weights must be non-negative");
+      m->class_weights[c] = weights[c];
+    }
   });
 }
 

@@ -200,12 +200,15 @@ constexpr uint32_t kMlpInputs = 12;
 constexpr uint32_t kMlpParams = 12 * 64 + 64 + 64 * 64 + 64 + 64 * 4 + 4;  // 5,252
 void mlp_features(hipStream_t s, uint32_t n, const float* params, const float* stats, const float* extra,
                   float extent, uint32_t iteration, float* features);
-void mlp_forward(hipStream_t s, uint32_t n, const float* features, const uint16_t* weights, float* logits);
-// Mean softmax cross-entropy gradients over `batch` samples added into `grads`
-// (cleared by the caller), the loss sum into loss[0]. seed 0: samples 0..batch-1;
-// else `batch` hashed draws from `count` samples.
+// `norm`: the input standardisation, mean[12] then scale[12].
+void mlp_forward(hipStream_t s, uint32_t n, const float* features, const uint16_t* weights, const float* norm,
+                 float* logits);
+// Mean class-weighted softmax cross-entropy gradients over `batch` samples added
+// into `grads` (cleared by the caller), the loss sum into loss[0]. seed 0: samples
+// 0..batch-1; else `batch` hashed draws from `count` samples.
 void mlp_gradients(hipStream_t s, uint32_t batch, uint32_t count, uint32_t seed, const float* features,
-                   const uint32_t* labels, const uint16_t* weights, float* grads, float* loss);
+                   const uint32_t* labels, const uint16_t* weights, const float* norm, const float class_weights[4],
+                   float* grads, float* loss);
 void mlp_adam(hipStream_t s, float* master, float* moments, float* grads, uint16_t* weights, float lr,
               float inv_bias1, float inv_bias2);
 void mlp_round(hipStream_t s, const float* master, uint16_t* weights);

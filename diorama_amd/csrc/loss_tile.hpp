@@ -93,10 +93,13 @@ __device__ inline void ssim_partials_tile(uint32_t tile_x, uint32_t tile_y, uint
     const float vx = mu[2] - mx * mx, vy = mu[3] - my * my, cxy = mu[4] - mx * my;
     const float a1 = 2.f * mx * my + kC1, a2 = 2.f * cxy + kC2;
     const float b1 = mx * mx + my * my + kC1, b2 = vx + vy + kC2;
-    const float s = a1 * a2 / (b1 * b2);
-    out[ch][0] = s * (2.f * my / a1 - 2.f * my / a2 - 2.f * mx / b1 + 2.f * mx / b2);
+    // s / a2 taken as a1 / (b1 b2): the reference's s (1 / a2) (loss.wgsl:103-106) is
+    // 0 * inf = NaN where a2 = 2 σxy + C2 rounds to exactly 0; a1, b1, b2 stay >= C1.
+    const float s_over_a2 = a1 / (b1 * b2);
+    const float s = s_over_a2 * a2;
+    out[ch][0] = s * (2.f * my / a1 - 2.f * mx / b1 + 2.f * mx / b2) - 2.f * my * s_over_a2;
     out[ch][1] = -s / b2;
-    out[ch][2] = 2.f * s / a2;
+    out[ch][2] = 2.f * s_over_a2;
     out[ch][3] = s;
     __syncthreads();
   }

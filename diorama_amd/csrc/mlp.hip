@@ -29,8 +29,19 @@ __device__ inline uint16_t float_to_bf16(float f) {
   return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-__device__ inline void load_weights(float* __restrict__ w, const uint16_t* __restrict__ weights) {
+// The weights as FP32 in LDS, followed by the input standardisation (mean[12],
+// scale[12]: x' = (x - mean) scale), which is not trained.
+constexpr int kNorm = kParams, kLds = kParams + 2 * kIn;
+
+__device__ inline void load_weights(float* __restrict__ w, const uint16_t* __restrict__ weights,
+                                    const float* __restrict__ norm) {
   for (int k = threadIdx.x; k < kParams; k += blockDim.x) w[k] = bf16_to_float(weights[k]);
+  for (int k = threadIdx.x; k < 2 * kIn; k += blockDim.x) w[kNorm + k] = norm[k];
+}
+
+__device__ inline void standardise(const float* __restrict__ w, float x[kIn]) {
+#pragma unroll
+  for (int i = 0; i < kIn; i++) x[i] = (x[i] - w[kNorm + i]) * w[kNorm + kIn + i];
 }
 
 // One sample's forward pass: both hidden layers (after ReLU) and the logits.
@@ -103,15 +114,17 @@ __global__ __launch_bounds__(kThreads) void features_kernel(uint32_t n, const fl
 
 __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(uint32_t n, const float* __restrict__ features,
                                                                 const uint16_t* __restrict__ weights,
+                                                                const float* __restrict__ norm,
                                                                 float* __restrict__ logits) {
-  __shared__ float w[kParams];
-  load_weights(w, weights);
+  __shared__ float w[kLds];
+  load_weights(w, weights, norm);
   __syncthreads();
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   float x[kIn], h1[kHidden], h2[kHidden], z[kOut];
 #pragma unroll
   for (int k = 0; k < kIn; k++) x[k] = features[static_cast<size_t>(i) * kIn + k];
+  standardise(w, x);
   forward(w, x, h1, h2, z);
 #pragma unroll
   for (int c = 0; c < kOut; c++) logits[static_cast<size_t>(i) * kOut + c] = z[c];
@@ -130,8 +143,9 @@ __device__ inline uint32_t sample_of(uint32_t slot, uint32_t count, uint32_t see
   return pcg_hash(pcg_hash(seed) ^ slot) % count;
 }
 
-// Gradients of the mean softmax cross-entropy over `batch` samples, added into
-// `grads` (which the caller cleared), and the loss sum into loss[0]. A workgroup
+// Gradients of the mean class-weighted softmax cross-entropy over `batch`
+// samples, added into `grads` (which the caller cleared), and the loss sum into
+// loss[0]. A workgroup
 // takes 32 samples: threads 0..31 run one sample each forward and backward and
 // leave its activations and their gradients in LDS (55 KB with the weights); then
 // all 256 threads form the weight gradients as sums of outer products over the 32
@@ -139,7 +153,7 @@ __device__ inline uint32_t sample_of(uint32_t slot, uint32_t count, uint32_t see
 constexpr int kTrainSamples = 32;
 
 struct TrainShared {
-  float w[kParams];
+  float w[kLds];
   float x[kTrainSamples][kIn];
   float h1[kTrainSamples][kHidden];
   float d1[kTrainSamples][kHidden];  // dL/d(pre-activation of layer 1)
@@ -152,9 +166,10 @@ __global__ __launch_bounds__(kThreads) void mlp_train_kernel(uint32_t batch, uin
                                                               const float* __restrict__ features,
                                                               const uint32_t* __restrict__ labels,
                                                               const uint16_t* __restrict__ weights,
+                                                              const float* __restrict__ norm, float4 class_weight,
                                                               float* __restrict__ grads, float* __restrict__ loss) {
   __shared__ TrainShared sh;
-  load_weights(sh.w, weights);
+  load_weights(sh.w, weights, norm);
   __syncthreads();
   const uint32_t first = blockIdx.x * kTrainSamples;
   const uint32_t live = min(static_cast<uint32_t>(kTrainSamples), batch - first);
@@ -168,6 +183,7 @@ __global__ __launch_bounds__(kThreads) void mlp_train_kernel(uint32_t batch, uin
       const uint32_t sample = sample_of(first + t, count, seed);
 #pragma unroll
       for (int k = 0; k < kIn; k++) x[k] = features[static_cast<size_t>(sample) * kIn + k];
+      standardise(sh.w, x);
       label = labels[sample];
     } else {
 #pragma unroll
@@ -181,16 +197,18 @@ __global__ __launch_bounds__(kThreads) void mlp_train_kernel(uint32_t batch, uin
 #pragma unroll
     for (int c = 0; c < kOut; c++) total += __expf(z[c] - top);
     const float log_total = logf(total);
+    const float cw = label == 0u ? class_weight.x : (label == 1u ? class_weight.y : (label == 2u ? class_weight.z
+                                                                                                  : class_weight.w));
 #pragma unroll
     for (int c = 0; c < kOut; c++) {
       const float prob = __expf(z[c] - top - log_total);
-      dz[c] = active ? (prob - (static_cast<uint32_t>(c) == label ? 1.f : 0.f)) * inv_batch : 0.f;
+      dz[c] = active ? (prob - (static_cast<uint32_t>(c) == label ? 1.f : 0.f)) * cw * inv_batch : 0.f;
     }
     if (active) {
       float zl = z[0];
 #pragma unroll
       for (int c = 1; c < kOut; c++) zl = static_cast<uint32_t>(c) == label ? z[c] : zl;
-      atomicAdd(loss, top + log_total - zl);
+      atomicAdd(loss, cw * (top + log_total - zl));
     }
     float d2[kHidden];
 #pragma unroll
@@ -386,17 +404,20 @@ void mlp_features(hipStream_t s, uint32_t n, const float* params, const float* s
   DIO_LAUNCHED("mlp_features");
 }
 
-void mlp_forward(hipStream_t s, uint32_t n, const float* features, const uint16_t* weights, float* logits) {
+void mlp_forward(hipStream_t s, uint32_t n, const float* features, const uint16_t* weights, const float* norm,
+                 float* logits) {
   if (n == 0) return;
-  mlp_forward_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, features, weights, logits);
+  mlp_forward_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, features, weights, norm, logits);
   DIO_LAUNCHED("mlp_forward");
 }
 
 void mlp_gradients(hipStream_t s, uint32_t batch, uint32_t count, uint32_t seed, const float* features,
-                   const uint32_t* labels, const uint16_t* weights, float* grads, float* loss) {
+                   const uint32_t* labels, const uint16_t* weights, const float* norm, const float class_weights[4],
+                   float* grads, float* loss) {
   if (batch == 0) return;
+  const float4 cw = make_float4(class_weights[0], class_weights[1], class_weights[2], class_weights[3]);
   mlp_train_kernel<<<ceil_div(batch, kTrainSamples), kThreads, 0, s>>>(
-      batch, count, seed, features, labels, weights, grads, loss);
+      batch, count, seed, features, labels, weights, norm, cw, grads, loss);
   DIO_LAUNCHED("mlp_train");
 }
 

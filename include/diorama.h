@@ -248,7 +248,7 @@ int dio_mlp_get_weights(dio_densify_mlp* mlp, float* weights);        /* the BF1
 int dio_mlp_set_weights(dio_densify_mlp* mlp, const float* weights);  /* rounded to BF16; resets Adam */
 /* features: count x 12 floats; logits: count x 4 floats (keep, clone, split, prune). */
 int dio_mlp_forward(dio_densify_mlp* mlp, dio_buf* features, uint32_t count, dio_buf* logits);
-/* Gradients of the mean softmax cross-entropy over all `count` samples (labels:
+/* Gradients of the mean class-weighted softmax cross-entropy over all `count` samples (labels:
  * u32 class 0 keep, 1 clone, 2 split, 3 prune) and that mean loss. */
 int dio_mlp_gradients(dio_densify_mlp* mlp, dio_buf* features, dio_buf* labels, uint32_t count, float* grads,
                       double* loss);
@@ -256,6 +256,12 @@ int dio_mlp_gradients(dio_densify_mlp* mlp, dio_buf* features, dio_buf* labels, 
  * `count` samples; last_loss (nullable) gets the last step's mean loss. */
 int dio_mlp_train(dio_densify_mlp* mlp, dio_buf* features, dio_buf* labels, uint32_t count, uint32_t steps,
                   uint32_t batch, float lr, uint32_t seed, double* last_loss);
+/* The inputs' standardisation, applied before the first layer in inference and
+ * training: x' = (x - mean) * scale (mean 0, scale 1 by default). */
+int dio_mlp_set_input_norm(dio_densify_mlp* mlp, const float mean[12], const float scale[12]);
+/* Per-class loss weights (keep, clone, split, prune) for dio_mlp_gradients and
+ * dio_mlp_train; all 1 by default. */
+int dio_mlp_set_class_weights(dio_densify_mlp* mlp, const float weights[4]);
 void dio_mlp_destroy(dio_densify_mlp* mlp);
 /* mode 0: the plain rule (the default); 1: the plain rule, logging samples (the
  * 12 inputs and the rule's class of Gaussians drawn at each densification,

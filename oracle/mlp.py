@@ -3,7 +3,7 @@ inputs), for the tests only.
 
 TEST INFRASTRUCTURE: only the tests load this module, as the checker. The MLP has
 no counterpart in the reference; it is pinned by central finite differences of
-its own loss (tests/test_oracle.py) and by the BF16 rounding rule.
+its own loss (tests/test_oracle_mlp.py) and by the BF16 rounding rule.
 
 Layout of the 5,252 parameters: W1 [12][64], b1 [64], W2 [64][64], b2 [64],
 W3 [64][4], b3 [4], row-major, input index first. Classes: 0 keep, 1 clone,
@@ -34,10 +34,12 @@ def bf16(w):
     return r.astype(np.uint32).view(np.float32)
 
 
-def forward(w, x):
-    """Logits (n, 4) and the hidden activations."""
+def forward(w, x, mean=None, scale=None):
+    """Logits (n, 4) and the hidden activations; inputs standardised first."""
     w1, b1, w2, b2, w3, b3 = unpack(w)
     x = np.asarray(x, dtype=np.float64)
+    if mean is not None:
+        x = (x - np.asarray(mean, dtype=np.float64)) * np.asarray(scale, dtype=np.float64)
     a1 = x @ w1 + b1
     h1 = np.maximum(a1, 0.0)
     a2 = h1 @ w2 + b2
@@ -45,18 +47,20 @@ def forward(w, x):
     return h2 @ w3 + b3, (x, a1, h1, a2, h2)
 
 
-def loss_and_grads(w, x, labels):
-    """Mean softmax cross-entropy over the samples and its gradient (PARAMS,)."""
-    z, (x, a1, h1, a2, h2) = forward(w, x)
+def loss_and_grads(w, x, labels, class_weights=None, mean=None, scale=None):
+    """Mean class-weighted softmax cross-entropy over the samples and its gradient (PARAMS,)."""
+    z, (x, a1, h1, a2, h2) = forward(w, x, mean, scale)
     n = len(x)
     labels = np.asarray(labels, dtype=np.int64)
     zmax = z.max(axis=1, keepdims=True)
     logsum = np.log(np.exp(z - zmax).sum(axis=1, keepdims=True)) + zmax
-    loss = float(np.mean(logsum[:, 0] - z[np.arange(n), labels]))
+    cw = np.ones(CLASSES) if class_weights is None else np.asarray(class_weights, dtype=np.float64)
+    per = cw[labels]
+    loss = float(np.mean(per * (logsum[:, 0] - z[np.arange(n), labels])))
     p = np.exp(z - logsum)
     dz = p.copy()
     dz[np.arange(n), labels] -= 1.0
-    dz /= n
+    dz *= per[:, None] / n
     _, _, w2, _, w3, _ = unpack(w)
     d2 = (dz @ w3.T) * (a2 > 0)
     d1 = (d2 @ w2.T) * (a1 > 0)

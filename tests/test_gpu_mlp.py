@@ -43,14 +43,19 @@ def test_forward_and_gradients_against_oracle():
     labels = rng.integers(0, 4, size=n).astype(np.uint32)
     fx, lb = ctx.upload(x), ctx.upload(labels)
     logits = ctx.buffer(n * 16)
+    mean = rng.normal(0, 0.5, size=12).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, size=12).astype(np.float32)
+    net.set_input_norm(mean, scale)
     say("mlp forward")
     net.forward(fx, n, logits)
     z = logits.download(np.float32).reshape(n, 4)
-    oz, _ = omlp.forward(wb, x)
+    oz, _ = omlp.forward(wb, x, mean, scale)
     err = np.abs(z - oz).max() / np.abs(oz).max()
     say("mlp gradients")
+    cw = [0.5, 2.0, 3.0, 1.5]
+    net.set_class_weights(cw)
     g, loss = net.gradients(fx, lb, n)
-    oloss, og = omlp.loss_and_grads(wb, x, labels)
+    oloss, og = omlp.loss_and_grads(wb, x, labels, cw, mean, scale)
     gerr = np.abs(g - og).max() / np.abs(og).max()
     print(f"mlp: logits rel err {err:.2e}, loss {loss:.6f} vs {oloss:.6f}, gradient rel err {gerr:.2e}")
     assert err < 1e-5

@@ -17,7 +17,8 @@ def test_gradients_match_central_differences():
     w = rng.normal(0, 0.3, size=mlp.PARAMS)
     x = rng.normal(0, 1, size=(24, mlp.INPUTS))
     labels = rng.integers(0, 4, size=24)
-    _, g = mlp.loss_and_grads(w, x, labels)
+    cw = [0.7, 1.3, 2.0, 0.4]
+    _, g = mlp.loss_and_grads(w, x, labels, cw)
     picks = np.concatenate([rng.choice(mlp.PARAMS, 60, replace=False), np.arange(mlp.PARAMS - 4, mlp.PARAMS)])
     h = 1e-6
     worst = 0.0
@@ -25,7 +26,7 @@ def test_gradients_match_central_differences():
         wp, wm = w.copy(), w.copy()
         wp[k] += h
         wm[k] -= h
-        fd = (mlp.loss_and_grads(wp, x, labels)[0] - mlp.loss_and_grads(wm, x, labels)[0]) / (2 * h)
+        fd = (mlp.loss_and_grads(wp, x, labels, cw)[0] - mlp.loss_and_grads(wm, x, labels, cw)[0]) / (2 * h)
         worst = max(worst, abs(fd - g[k]))
     assert worst < 1e-7 * max(1.0, np.abs(g).max()), worst
 
