@@ -329,12 +329,17 @@ def batch_bench(args):
 
 
 def scaled_settings(settings, factor):
-    """The schedule of `settings` with every step count divided by `factor` (a batch
-    step sees `factor` views)."""
+    """The schedule of `settings` for batch steps of `factor` views: every step count
+    divided by `factor`, the learning rates multiplied by sqrt(factor) (16 times
+    fewer Adam steps, each on the summed gradient of 16 views)."""
     out = dict(settings)
     for key in ("position_lr_steps", "sh_degree_interval", "densify_from", "densify_until", "densify_interval",
                 "opacity_reset_interval"):
         out[key] = max(1, int(round(settings[key] / factor)))
+    scale = math.sqrt(factor)
+    out["position_lr"] = [v * scale for v in settings["position_lr"]]
+    for key in ("opacity_lr", "scale_lr", "rotation_lr", "color_lr", "sh_lr"):
+        out[key] = settings[key] * scale
     return out
 
 

@@ -1183,7 +1183,7 @@ bool densify_codes(dio_trainer* t, const dio::DensifyArgs& a, dio_buf* codes, di
   if (L.mode != 0) {
     learned_resize(t);
     dio::mlp_features(s, n, dev<float>(t->params), dev<float>(t->stats), dev<float>(L.extra), t->settings.extent,
-                      t->iteration, dev<float>(L.features));
+                      t->iteration, t->batch ? t->batch->views : 1u, dev<float>(L.features));
     if (L.mode == 2) {
       dio::mlp_forward(s, n, dev<float>(L.features), dev<uint16_t>(L.mlp->weights), dev<float>(L.mlp->norm),
                        dev<float>(L.logits));
@@ -2149,7 +2149,7 @@ extern "C" int dio_trainer_features(dio_trainer* t, dio_buf** features) {
     if (!L.extra) invalid("set a densify mode first: the inputs need its statistics");
     learned_resize(t);
     dio::mlp_features(t->ctx->stream, t->count, dev<float>(t->params), dev<float>(t->stats), dev<float>(L.extra),
-                      t->settings.extent, t->iteration, dev<float>(L.features));
+                      t->settings.extent, t->iteration, t->batch ? t->batch->views : 1u, dev<float>(L.features));
     *features = L.features;
   });
 }

@@ -198,8 +198,9 @@ void densify_demote(hipStream_t s, uint32_t n, const uint32_t* sorted_keys, cons
 // ------------------------------------------------ learned densification --
 constexpr uint32_t kMlpInputs = 12;
 constexpr uint32_t kMlpParams = 12 * 64 + 64 + 64 * 64 + 64 + 64 * 4 + 4;  // 5,252
+// `views_per_step` turns ages in steps into ages in views seen.
 void mlp_features(hipStream_t s, uint32_t n, const float* params, const float* stats, const float* extra,
-                  float extent, uint32_t iteration, float* features);
+                  float extent, uint32_t iteration, uint32_t views_per_step, float* features);
 // `norm`: the input standardisation, mean[12] then scale[12].
 void mlp_forward(hipStream_t s, uint32_t n, const float* features, const uint16_t* weights, const float* norm,
                  float* logits);

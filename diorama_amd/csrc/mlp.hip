@@ -75,13 +75,14 @@ __device__ inline void forward(const float* __restrict__ w, const float x[kIn], 
 //  2 opacity                                     3 log(largest scale / extent)
 //  4 log(smallest scale / extent)                5 log(largest / smallest scale)
 //  6 luminance of the SH DC colour               7 log(1 + views that saw it)
-//  8 log(1 + age in steps)                       9 log(1 + largest screen radius)
+//  8 log(1 + age in views seen)                 9 log(1 + largest screen radius)
 // 10 log(mean depth)                            11 gradient-direction variance,
 //    1 - |sum of gradients| / sum of their lengths.
 __global__ __launch_bounds__(kThreads) void features_kernel(uint32_t n, const float* __restrict__ params,
                                                              const float4* __restrict__ stats,
                                                              const float4* __restrict__ extra, float extent,
-                                                             uint32_t iteration, float* __restrict__ features) {
+                                                             uint32_t iteration, float views_per_step,
+                                                             float* __restrict__ features) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const float* p = params + static_cast<size_t>(i) * kFloatsPerGaussian;
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void features_kernel(uint32_t n, const fl
   const float big = fmaxf(sx, fmaxf(sy, sz)), small = fminf(sx, fminf(sy, sz));
   const float c0 = 0.5f + 0.28209479177387814f * p[12], c1 = 0.5f + 0.28209479177387814f * p[13],
               c2 = 0.5f + 0.28209479177387814f * p[14];
-  const float age = fmaxf(static_cast<float>(iteration) - p[15], 0.f);
+  const float age = fmaxf(static_cast<float>(iteration) - p[15], 0.f) * views_per_step;
   const float depth = views > 0.f ? e.y / views : 1.f;
   const float pull = sqrtf(e.z * e.z + e.w * e.w);
   float f[kIn];
@@ -396,11 +397,12 @@ void mlp_resolve(hipStream_t s, const uint32_t* pending_count, uint32_t pending_
 }
 
 void mlp_features(hipStream_t s, uint32_t n, const float* params, const float* stats, const float* extra,
-                  float extent, uint32_t iteration, float* features) {
+                  float extent, uint32_t iteration, uint32_t views_per_step, float* features) {
   if (n == 0) return;
   features_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, params, reinterpret_cast<const float4*>(stats),
                                                               reinterpret_cast<const float4*>(extra), extent,
-                                                              iteration, features);
+                                                              iteration, static_cast<float>(views_per_step),
+                                                              features);
   DIO_LAUNCHED("mlp_features");
 }
 

@@ -273,7 +273,11 @@ int dio_trainer_set_densify_mode(dio_trainer* t, uint32_t mode, dio_densify_mlp*
                                  float confidence, uint32_t samples_per_densify);
 /* The logged samples (non-owning views) and how many there are. */
 int dio_trainer_samples(dio_trainer* t, dio_buf** features, dio_buf** labels, uint32_t* count);
-/* The 12 inputs of the current Gaussians (a trainer-owned count x 12 buffer). */
+/* The 12 inputs of the current Gaussians (a trainer-owned count x 12 buffer):
+ * log(1 + mean and largest view-space gradient / 2e-4), opacity, log(largest and
+ * smallest scale / extent), log(largest / smallest scale), SH DC luminance,
+ * log(1 + views), log(1 + age in views seen), log(1 + largest radius),
+ * log(mean depth), gradient-direction variance. */
 int dio_trainer_features(dio_trainer* t, dio_buf** features);
 /* Per Gaussian (largest gradient length, summed depth, summed gradient x, y)
  * since the last densification; null in mode 0. */

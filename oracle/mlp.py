@@ -68,7 +68,7 @@ def loss_and_grads(w, x, labels, class_weights=None, mean=None, scale=None):
     return loss, np.concatenate([g.reshape(-1) for g in grads])
 
 
-def features(params, stats, extra, extent, iteration):
+def features(params, stats, extra, extent, iteration, views_per_step=1):
     """The 12 inputs per Gaussian (n, 12) from raw parameters (n, 16), statistics
     (n, 4: summed gradient length, views, largest radius, -) and the extra
     statistics (n, 4: largest gradient length, summed depth, summed gradient x, y)."""
@@ -92,7 +92,7 @@ def features(params, stats, extra, extent, iteration):
     f[:, 5] = np.log(big / small)
     f[:, 6] = colour @ np.array([0.299, 0.587, 0.114])
     f[:, 7] = np.log1p(views)
-    f[:, 8] = np.log1p(np.maximum(iteration - p[:, 15], 0.0))
+    f[:, 8] = np.log1p(np.maximum(iteration - p[:, 15], 0.0) * views_per_step)
     f[:, 9] = np.log1p(s[:, 2])
     f[:, 10] = np.log(np.maximum(depth, 1e-3))
     f[:, 11] = np.where(s[:, 0] > 0, np.maximum(1.0 - pull / np.where(s[:, 0] > 0, s[:, 0], 1.0), 0.0), 0.0)
