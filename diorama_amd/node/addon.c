@@ -688,6 +688,110 @@ FN(trainer_destroy) {
   return undefined(env);
 }
 
+
+/* ------------------------------------------------ learned densification -- */
+
+static napi_value float_array(napi_env env, const float* src, size_t n) {
+  void* data;
+  napi_value ab, ta;
+  napi_create_arraybuffer(env, n * 4, &data, &ab);
+  memcpy(data, src, n * 4);
+  napi_create_typedarray(env, napi_float32_array, n, ab, 0, &ta);
+  return ta;
+}
+
+FN(mlp_create) {
+  ARGS;
+  dio_densify_mlp* m = NULL;
+  if (!ok(env, dio_mlp_create(handle(&c, 0), u32(&c, 1), &m))) return NULL;
+  return wrap(env, m);
+}
+FN(mlp_get_weights) {
+  ARGS;
+  float w[DIO_MLP_PARAMS];
+  if (!ok(env, dio_mlp_get_weights(handle(&c, 0), w))) return NULL;
+  return float_array(env, w, DIO_MLP_PARAMS);
+}
+FN(mlp_set_weights) {
+  ARGS;
+  void* w = NULL;
+  size_t len = 0;
+  if (!bytes_of(&c, 1, &w, &len) || len != DIO_MLP_PARAMS * 4) return fail(env, "weights must be 5252 floats");
+  if (!ok(env, dio_mlp_set_weights(handle(&c, 0), (const float*)w))) return NULL;
+  return undefined(env);
+}
+FN(mlp_set_input_norm) {
+  ARGS;
+  void *mean = NULL, *scale = NULL;
+  size_t ml = 0, sl = 0;
+  if (!bytes_of(&c, 1, &mean, &ml) || !bytes_of(&c, 2, &scale, &sl) || ml != 48 || sl != 48)
+    return fail(env, "mean and scale must be 12 floats each");
+  if (!ok(env, dio_mlp_set_input_norm(handle(&c, 0), (const float*)mean, (const float*)scale))) return NULL;
+  return undefined(env);
+}
+FN(mlp_set_class_weights) {
+  ARGS;
+  float w[4];
+  if (!floats(&c, c.argv[1], w, 4)) return fail(env, "class weights must be 4 numbers");
+  if (!ok(env, dio_mlp_set_class_weights(handle(&c, 0), w))) return NULL;
+  return undefined(env);
+}
+FN(mlp_forward) {
+  ARGS;
+  if (!ok(env, dio_mlp_forward(handle(&c, 0), handle(&c, 1), u32(&c, 2), handle(&c, 3)))) return NULL;
+  return undefined(env);
+}
+FN(mlp_gradients) {
+  ARGS;
+  float g[DIO_MLP_PARAMS];
+  double loss = 0;
+  napi_value o;
+  if (!ok(env, dio_mlp_gradients(handle(&c, 0), handle(&c, 1), handle(&c, 2), u32(&c, 3), g, &loss))) return NULL;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "grads", float_array(env, g, DIO_MLP_PARAMS));
+  napi_set_named_property(env, o, "loss", number(env, loss));
+  return o;
+}
+FN(mlp_train) {
+  ARGS;
+  double loss = 0;
+  if (!ok(env, dio_mlp_train(handle(&c, 0), handle(&c, 1), handle(&c, 2), u32(&c, 3), u32(&c, 4), u32(&c, 5),
+                             (float)f64(&c, 6), u32(&c, 7), &loss)))
+    return NULL;
+  return number(env, loss);
+}
+FN(mlp_destroy) {
+  ARGS;
+  dio_mlp_destroy(handle(&c, 0));
+  return undefined(env);
+}
+FN(trainer_set_densify_mode) {
+  ARGS;
+  if (!ok(env, dio_trainer_set_densify_mode(handle(&c, 0), u32(&c, 1), handle(&c, 2), u32(&c, 3), (float)f64(&c, 4),
+                                            u32(&c, 5))))
+    return NULL;
+  return undefined(env);
+}
+FN(trainer_samples) {
+  ARGS;
+  dio_buf *f = NULL, *l = NULL;
+  uint32_t n = 0;
+  napi_value o;
+  if (!ok(env, dio_trainer_samples(handle(&c, 0), &f, &l, &n))) return NULL;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "features", wrap(env, f));
+  napi_set_named_property(env, o, "labels", wrap(env, l));
+  napi_set_named_property(env, o, "count", number(env, n));
+  return o;
+}
+FN(trainer_features) {
+  ARGS;
+  dio_buf* f = NULL;
+  if (!ok(env, dio_trainer_features(handle(&c, 0), &f))) return NULL;
+  return wrap(env, f);
+}
+GETTER(trainer_extra_stats, dio_trainer_extra_stats)
+
 /* ---------------------------------------------------------------- init -- */
 
 #define EXPORT(name) {#name, NULL, name, NULL, NULL, NULL, napi_default, NULL}
@@ -715,6 +819,10 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT(trainer_set_iteration), EXPORT(trainer_sh_degree), EXPORT(trainer_set_sh_degree),
       EXPORT(trainer_densifications), EXPORT(trainer_params), EXPORT(trainer_moments), EXPORT(trainer_stats),
       EXPORT(trainer_renderer), EXPORT(trainer_destroy), EXPORT(trainer_batch_size),
+      EXPORT(mlp_create), EXPORT(mlp_get_weights), EXPORT(mlp_set_weights), EXPORT(mlp_set_input_norm),
+      EXPORT(mlp_set_class_weights), EXPORT(mlp_forward), EXPORT(mlp_gradients), EXPORT(mlp_train),
+      EXPORT(mlp_destroy), EXPORT(trainer_set_densify_mode), EXPORT(trainer_samples), EXPORT(trainer_features),
+      EXPORT(trainer_extra_stats),
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

@@ -319,6 +319,20 @@ class Trainer {
   get batchSize() {
     return native.trainer_batch_size(this.handle);
   }
+  /** mode 'plain' | 'log' | 'learned' (see dio_trainer_set_densify_mode). */
+  setDensifyMode(mode, mlp, options) {
+    const o = options || {};
+    const code = { plain: 0, log: 1, learned: 2 }[mode];
+    native.trainer_set_densify_mode(this.handle, code === undefined ? mode : code, mlp ? mlp.handle : null,
+      o.budget || 0, o.confidence === undefined ? 0.5 : o.confidence, o.samplesPerDensify || 80000);
+  }
+  samples() {
+    const s = native.trainer_samples(this.handle);
+    return { features: view(s.features, this), labels: view(s.labels, this), count: s.count };
+  }
+  features() {
+    return view(native.trainer_features(this.handle), this);
+  }
   get count() {
     return native.trainer_count(this.handle);
   }
@@ -380,6 +394,42 @@ class Trainer {
   }
 }
 
+/**
+ * The learned densification proposal (no counterpart in the reference): a
+ * 12 -> 64 -> 64 -> 4 MLP with BF16 weights and FP32 accumulation.
+ */
+class DensifyMLP {
+  constructor(device, seed) {
+    this.handle = native.mlp_create(device.handle, seed || 1);
+  }
+  get weights() {
+    return native.mlp_get_weights(this.handle);
+  }
+  set weights(w) {
+    native.mlp_set_weights(this.handle, w);
+  }
+  setInputNorm(mean, scale) {
+    native.mlp_set_input_norm(this.handle, mean, scale);
+  }
+  setClassWeights(weights) {
+    native.mlp_set_class_weights(this.handle, weights);
+  }
+  forward(features, count, logits) {
+    native.mlp_forward(this.handle, features.handle, count, logits.handle);
+  }
+  gradients(features, labels, count) {
+    return native.mlp_gradients(this.handle, features.handle, labels.handle, count);
+  }
+  train(features, labels, count, steps, batch, lr, seed) {
+    return native.mlp_train(this.handle, features.handle, labels.handle, count, steps, batch || 4096, lr || 1e-3,
+      seed || 1);
+  }
+  destroy() {
+    if (this.handle) native.mlp_destroy(this.handle);
+    this.handle = null;
+  }
+}
+
 /** Raw parameters to the renderer's activated layout (src/trainer.ts:449-461). */
 function activate(params) {
   const out = new Float32Array(params.length);
@@ -406,6 +456,7 @@ module.exports = {
   GaussianRenderer,
   TrainingLoss,
   Trainer,
+  DensifyMLP,
   DEFAULT_SETTINGS,
   activate,
   FLOATS_PER_GAUSSIAN,

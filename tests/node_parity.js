@@ -25,6 +25,22 @@ function rng(seed) {
   };
 }
 
+function lookAt(eye) {
+  const len = Math.hypot(eye[0], eye[1], eye[2]);
+  const z = eye.map((v) => v / len);
+  const xl = Math.hypot(z[2], z[0]);
+  const x = [z[2] / xl, 0, -z[0] / xl];
+  const y = [z[1] * x[2] - z[2] * x[1], z[2] * x[0] - z[0] * x[2], z[0] * x[1] - z[1] * x[0]];
+  const dot = (a, b) => a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+  return [x[0], y[0], z[0], 0, x[1], y[1], z[1], 0, x[2], y[2], z[2], 0, -dot(x, eye), -dot(y, eye), -dot(z, eye), 1];
+}
+
+function perspective(fov, aspect) {
+  const f = 1 / Math.tan(fov / 2);
+  const near = 0.01, far = 100, depth = 1 / (near - far);
+  return [f / aspect, 0, 0, 0, 0, f, 0, 0, 0, 0, far * depth, -1, 0, 0, near * far * depth, 0];
+}
+
 function cpuChecks() {
   const out = { version: d.version(), defaults: d.DEFAULT_SETTINGS };
   const errors = [];
@@ -188,6 +204,35 @@ async function gpuChecks() {
   out.render = { count: renderer.count, hookSaw, imageError: imageErr, gradientErrors };
   pgb.destroy();
   renderer.destroy();
+
+  // The MLP with only output biases: every sample's logits are those biases.
+  const net = new d.DensifyMLP(device, 3);
+  const biasOnly = new Float32Array(5252);
+  biasOnly.set([0.5, -1.25, 2.0, 3.75], 5248);
+  net.weights = biasOnly;
+  const feats = device.upload(new Float32Array(12 * 5).map((_, k) => k * 0.1));
+  const logits = device.createBuffer(5 * 16);
+  net.forward(feats, 5, logits);
+  out.mlp = { logits: Array.from(new Float32Array(logits.read())), weightsBack: net.weights[5251] };
+  net.destroy();
+  feats.destroy();
+  logits.destroy();
+
+  // A batch trainer over views of two sizes: one step.
+  const views = [[64, 48], [48, 64], [64, 48]].map(([vw, vh], k) => {
+    const a = (2 * Math.PI * k) / 3;
+    const eye = [4 * Math.sin(a), 0, 4 * Math.cos(a)];
+    return { camera: { view: lookAt(eye), proj: perspective(Math.PI / 3, vw / vh), viewport: [vw, vh] },
+      image: new Uint32Array(vw * vh).fill(0xff808080) };
+  });
+  const raw = { params: new Float32Array(16 * 64), sh: new Float32Array(45 * 64) };
+  for (let i = 0; i < 64; i++) {
+    raw.params.set([random() - 0.5, random() - 0.5, random() - 0.5, 0, -3, -3, -3, 0, 1, 0, 0, 0, 0, 0, 0, 0], i * 16);
+  }
+  const trainer = new d.Trainer(device, raw, views, Object.assign({}, d.DEFAULT_SETTINGS, { extent: 1 }), { batch: 2 });
+  await trainer.step();
+  out.batch = { size: trainer.batchSize, loss: await trainer.lastLoss(), iteration: trainer.iteration };
+  trainer.destroy();
   device.destroy();
   return out;
 }

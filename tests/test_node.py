@@ -53,3 +53,5 @@ def test_node_parity_on_gpu():
     assert r["count"] == 10 and r["hookSaw"] >= 10
     assert r["imageError"] < 1e-4
     assert max(r["gradientErrors"].values()) < 1e-3 and not any(math.isnan(v) for v in r["gradientErrors"].values())
+    assert out["mlp"]["logits"] == [0.5, -1.25, 2.0, 3.75] * 5 and out["mlp"]["weightsBack"] == 3.75
+    assert out["batch"]["size"] == 2 and out["batch"]["iteration"] == 1 and math.isfinite(out["batch"]["loss"])
