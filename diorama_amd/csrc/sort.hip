@@ -3,9 +3,10 @@
 // src/radixSort.ts:27-142 / src/shaders/radixSort.wgsl:44-120 (same result: a
 // stable sort by key bits [begin, end)), built as a onesweep sort for wave64:
 // * one histogram kernel counts every pass's digits at once (one read of the keys);
-// * then one kernel per pass: a 4096-key tile per workgroup (256 threads x 16
-//   keys, coalesced stripes) ranks its keys stably with wavefront "match" masks
-//   (8 ballots per key) plus a per-digit prefix across the 4 waves, learns where
+// * then one kernel per pass: a 4096-key tile per workgroup (4 waves, each owning
+//   1,024 consecutive keys) ranks its keys stably with wavefront "match" masks
+//   (8 ballots per key) and per-wave digit counts, then a per-digit prefix across
+//   the 4 waves (two barriers per tile), learns where
 //   its run of each digit starts by per-digit decoupled look-back over earlier
 //   tiles, and scatters through LDS so each digit's run is written contiguously.
 // Tiles are handed out by an atomic ticket, so a tile only waits on tiles that
@@ -152,22 +153,26 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
   const uint32_t base = tile * kSortTile;
   const uint64_t lower_lanes = (1ull << lane) - 1;
 
+  // Each wave owns 1,024 consecutive keys of the tile (16 rows of 64), so index
+  // order is (wave, row, lane) and a wave can rank its keys on its own.
   Key key[kItems];
   uint32_t value[kItems];
   uint32_t rank[kItems];
+  const uint32_t wave_base = base + wave * (kItems * kWave);
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
-    const uint32_t i = base + k * kThreads + tid;
+    const uint32_t i = wave_base + k * kWave + lane;
     const bool valid = i < n;
     key[k] = valid ? keys_in[i] : Ops::invalid();
     value[k] = valid ? values_in[i] : 0u;
   }
 
-  // Stable ranks: within a stripe by lane (match masks), across stripes and waves
-  // by running per-digit counts.
+  // Stable ranks within the wave: lanes with the same digit agree through match
+  // masks, and the wave's own per-digit counts carry across its rows (one wave
+  // reads and writes its row of LDS in program order, so no barrier is needed).
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
-    const bool valid = base + k * kThreads + tid < n;
+    const bool valid = wave_base + k * kWave + lane < n;
     const uint32_t d = Ops::digit(key[k], shift);
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -177,22 +182,27 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
       peers &= bit ? set : ~set;
     }
     const uint64_t before = peers & lower_lanes;
-    if (valid && before == 0) wave_count[wave][d] = __popcll(peers);
-    __syncthreads();
-    {
-      uint32_t r = running[tid];
-#pragma unroll
-      for (int w = 0; w < kWaves; w++) {
-        const uint32_t c = wave_count[w][tid];
-        wave_offset[w][tid] = r;
-        r += c;
-        wave_count[w][tid] = 0;
-      }
-      running[tid] = r;
-    }
-    __syncthreads();
-    rank[k] = wave_offset[wave][d] + __popcll(before);
+    const uint32_t so_far = valid ? wave_count[wave][d] : 0u;
+    rank[k] = so_far + __popcll(before);
+    __builtin_amdgcn_wave_barrier();
+    if (valid && before == 0) wave_count[wave][d] = so_far + __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
   }
+  __syncthreads();
+  // Per digit: the earlier waves' counts offset each wave's ranks.
+  {
+    uint32_t r = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+      const uint32_t c = wave_count[w][tid];
+      wave_offset[w][tid] = r;
+      r += c;
+    }
+    running[tid] = r;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; k++) rank[k] += wave_offset[wave][Ops::digit(key[k], shift)];
 
   // This tile's count of digit `tid`: publish it, then add up earlier tiles' counts
   // back to the first one that knows its inclusive prefix.
@@ -244,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
-    if (base + k * kThreads + tid < n) {
+    if (wave_base + k * kWave + lane < n) {
       const uint32_t pos = local_start[Ops::digit(key[k], shift)] + rank[k];
       staged_keys[pos] = key[k];
       staged_values[pos] = value[k];
