@@ -26,6 +26,14 @@ class _Camera(ctypes.Structure):
     _fields_ = [("view", ctypes.c_float * 16), ("proj", ctypes.c_float * 16), ("viewport", ctypes.c_float * 2)]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("tile_size", ctypes.c_uint32), ("splat_batch", ctypes.c_uint32), ("digit_bits", ctypes.c_uint32)]
+
+
+class _TuningResult(ctypes.Structure):
+    _fields_ = [("tuning", _Tuning), ("ms_per_step", ctypes.c_double)]
+
+
 class _Settings(ctypes.Structure):
     _fields_ = [
         ("extent", ctypes.c_float),
@@ -136,6 +144,9 @@ SIGNATURES = {
     "dio_mlp_set_input_norm": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "dio_mlp_destroy": (None, [_P]),
     "dio_trainer_set_densify_mode": (_I, [_P, _U32, _P, _U32, ctypes.c_float, _U32]),
+    "dio_trainer_set_tuning": (_I, [_P, ctypes.POINTER(_Tuning)]),
+    "dio_trainer_get_tuning": (_I, [_P, ctypes.POINTER(_Tuning)]),
+    "dio_trainer_autotune": (_I, [_P, _U32, ctypes.POINTER(_TuningResult), _U32, ctypes.POINTER(_U32)]),
     "dio_trainer_samples": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_U32)]),
     "dio_trainer_features": (_I, [_P, ctypes.POINTER(_P)]),
     "dio_trainer_extra_stats": (_P, [_P]),
@@ -624,6 +635,26 @@ class Trainer:
         m = modes.get(mode, mode)
         _check(lib().dio_trainer_set_densify_mode(self.handle, m, mlp.handle if mlp else None, budget, confidence,
                                                   samples_per_densify))
+
+    @property
+    def tuning(self):
+        k = _Tuning()
+        _check(lib().dio_trainer_get_tuning(self.handle, ctypes.byref(k)))
+        return {"tile_size": k.tile_size, "splat_batch": k.splat_batch, "digit_bits": k.digit_bits}
+
+    @tuning.setter
+    def tuning(self, value):
+        k = _Tuning(value.get("tile_size", 16), value.get("splat_batch", 64), value.get("digit_bits", 8))
+        _check(lib().dio_trainer_set_tuning(self.handle, ctypes.byref(k)))
+
+    def autotune(self, steps=3):
+        """Times every candidate setting over `steps` training steps, keeps the fastest;
+        returns [(tuning, ms per step)]."""
+        out = (_TuningResult * 16)()
+        n = _U32()
+        _check(lib().dio_trainer_autotune(self.handle, steps, out, 16, ctypes.byref(n)))
+        return [({"tile_size": r.tuning.tile_size, "splat_batch": r.tuning.splat_batch,
+                  "digit_bits": r.tuning.digit_bits}, r.ms_per_step) for r in out[:n.value]]
 
     def samples(self):
         f, l, n = _P(), _P(), _U32()

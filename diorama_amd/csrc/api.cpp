@@ -363,6 +363,7 @@ struct dio_tile_rasterizer {
   bool framed = false;
   uint32_t frame_w = 0, frame_h = 0;
   float frame_bg[3] = {0.f, 0.f, 0.f};
+  uint32_t splat_batch = 64, digit_bits = 8;  // tuning; results do not depend on them
 };
 
 namespace {
@@ -437,11 +438,11 @@ void ensure_frame_buffers(dio_tile_rasterizer* t, uint32_t tiles, uint32_t pixel
   }
 }
 
-// Bits of the tile index the tile sort must look at, in whole 8-bit digits.
-uint32_t tile_sort_bits(uint32_t tiles) {
+// Bits of the tile index the tile sort must look at, in whole digits.
+uint32_t tile_sort_bits(uint32_t tiles, uint32_t digit_bits = 8) {
   uint32_t bits = 0;
   while ((1u << bits) < tiles && bits < 32) bits++;
-  return std::max<uint32_t>(8u, (bits + 7) / 8 * 8);
+  return std::max(digit_bits, (bits + digit_bits - 1) / digit_bits * digit_bits);
 }
 
 // The tile stage of TileRasterizer.encode (src/tileRasterizer.ts:127-192) after
@@ -463,7 +464,7 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   // Depth order of the splats (culled ones last), then each splat's first pair slot.
   if (dio::radix_sort(s, false, t->depth_keys->ptr, dev<uint32_t>(t->depth_values), t->depth_alt_keys->ptr,
                       dev<uint32_t>(t->depth_alt_values), nullptr, t->count, 0, 32,
-                      dio::sort_workspace(t->splat_sort_ws->ptr, t->count)))
+                      dio::sort_workspace(t->splat_sort_ws->ptr, t->count), t->digit_bits))
     throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
   dio::scan_exclusive_gather(s, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_values),
                              dev<uint32_t>(t->offsets), nullptr, t->count, dev<uint32_t>(t->pair_total),
@@ -473,13 +474,14 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
                  dev<uint32_t>(t->pair_splats));
   t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats), t->alt_tiles->ptr,
                                      dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total), t->capacity, 0,
-                                     tile_sort_bits(tiles), dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity));
+                                     tile_sort_bits(tiles, t->digit_bits),
+                                     dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity), t->digit_bits);
   dio_buf* sorted_tiles = t->sorted_in_alt ? t->alt_tiles : t->pair_tiles;
   dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
   dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(t->pair_total), t->capacity, tiles,
                    dev<uint32_t>(t->ranges));
   dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), w, h, bg, dev<uint32_t>(target),
-                 dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last));
+                 dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last), t->splat_batch);
   t->framed = true;
   t->frame_w = w;
   t->frame_h = h;
@@ -517,7 +519,7 @@ void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pix
   dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
   dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), w, h,
                           t->frame_bg, dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
-                          dev<float>(pixel_grads), dev<float>(splat_grads));
+                          dev<float>(pixel_grads), dev<float>(splat_grads), t->splat_batch);
 }
 
 }  // namespace
@@ -1041,6 +1043,7 @@ struct dio_trainer {
   BatchState* batch = nullptr;  // set for a multi-view batch trainer
   uint32_t generation = 0;      // bumped whenever densification replaces the buffers
   LearnedState learned;
+  dio_tuning tuning{16, 64, 8};
 };
 
 namespace {
@@ -1117,6 +1120,8 @@ void trainer_optimize(dio_trainer* t) {
   const bool densify_now = densifying && step > st.densify_from && st.densify_interval &&
                            step % st.densify_interval == 0;
   const auto& view = next_view(t);
+  t->renderer->scene->rasterizer->splat_batch = t->tuning.splat_batch;
+  t->renderer->scene->rasterizer->digit_bits = t->tuning.digit_bits;
   renderer_preprocess(t->renderer, view.camera);
   renderer_tiles(t->renderer, nullptr, st.background);
   dio::training_loss(t->ctx->stream, t->width, t->height, dev<float>(t->renderer->scene->rasterizer->pixel_color),
@@ -1381,9 +1386,10 @@ void batch_launch(dio_trainer* t, bool with_adam) {
                         dev<uint32_t>(b->tile_counts), dev<uint32_t>(b->depth_keys), dev<uint32_t>(b->depth_values));
   // One depth order over every view's splats: the stable tile sort below keeps
   // each tile's pairs front to back, and tiles of different views never mix.
+  const dio_tuning& tune = t->tuning;
   if (dio::radix_sort(s, false, b->depth_keys->ptr, dev<uint32_t>(b->depth_values), b->depth_alt_keys->ptr,
                       dev<uint32_t>(b->depth_alt_values), nullptr, total, 0, 32,
-                      dio::sort_workspace(b->sort_ws->ptr, total)))
+                      dio::sort_workspace(b->sort_ws->ptr, total), tune.digit_bits))
     throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
   dio::scan_exclusive_gather(s, dev<uint32_t>(b->tile_counts), dev<uint32_t>(b->depth_values),
                              dev<uint32_t>(b->offsets), nullptr, total, dev<uint32_t>(b->pair_total),
@@ -1393,20 +1399,20 @@ void batch_launch(dio_trainer* t, bool with_adam) {
                        dev<uint32_t>(b->pair_tiles), dev<uint32_t>(b->pair_splats));
   b->sorted_in_alt = dio::radix_sort(s, false, b->pair_tiles->ptr, dev<uint32_t>(b->pair_splats), b->alt_tiles->ptr,
                                      dev<uint32_t>(b->alt_splats), dev<uint32_t>(b->pair_total), b->capacity, 0,
-                                     tile_sort_bits(b->max_tiles),
-                                     dio::sort_workspace(b->pair_sort_ws->ptr, b->capacity));
+                                     tile_sort_bits(b->max_tiles, tune.digit_bits),
+                                     dio::sort_workspace(b->pair_sort_ws->ptr, b->capacity), tune.digit_bits);
   dio_buf* sorted_tiles = b->sorted_in_alt ? b->alt_tiles : b->pair_tiles;
   dio_buf* sorted_splats = b->sorted_in_alt ? b->alt_splats : b->pair_splats;
   dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(b->pair_total), b->capacity, b->max_tiles,
                    dev<uint32_t>(b->ranges));
   dio::rasterize_batch(s, dev<float>(b->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(b->ranges), a, bg,
-                       dev<float>(b->pixel_color), dev<uint32_t>(b->pixel_last));
+                       dev<float>(b->pixel_color), dev<uint32_t>(b->pixel_last), tune.splat_batch);
   dio::training_loss_batch(s, a, dev<float>(b->pixel_color), dev<float>(b->partials), dev<float>(b->pixel_grads),
                            dev<float>(b->pixel_loss));
   dio::fill_u32(s, dev<uint32_t>(b->splat_grads), 0u, static_cast<size_t>(total) * 9);  // the backward adds into it
   dio::rasterize_backward_batch(s, dev<float>(b->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(b->ranges), a,
                                 bg, dev<float>(b->pixel_color), dev<uint32_t>(b->pixel_last),
-                                dev<float>(b->pixel_grads), dev<float>(b->splat_grads));
+                                dev<float>(b->pixel_grads), dev<float>(b->splat_grads), tune.splat_batch);
   dio::preprocess_backward_batch(s, dev<float>(sc->gaussians), dev<float>(sc->sh), t->sh_degree, a,
                                  dev<float>(b->splats), dev<float>(b->splat_grads), dev<float>(b->grad_gaussians),
                                  dev<float>(b->grad_sh), dev<float>(t->stats), dev<float>(t->learned.extra));
@@ -1859,30 +1865,65 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
                                    dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
       };
     } else if (name == "sort") {
-      // The whole binning sort: Gaussians by depth, then pairs by tile.
-      run = [&] {
-        const uint32_t tiles = dio::ceil_div(tr->frame_w, dio::kTile) * dio::ceil_div(tr->frame_h, dio::kTile);
-        dio::radix_sort(s, false, tr->depth_keys->ptr, dev<uint32_t>(tr->depth_values), tr->depth_alt_keys->ptr,
-                        dev<uint32_t>(tr->depth_alt_values), nullptr, tr->count, 0, 32,
-                        dio::sort_workspace(tr->splat_sort_ws->ptr, tr->count));
-        dio::radix_sort(s, false, sorted_splats == tr->alt_splats ? tr->alt_tiles->ptr : tr->pair_tiles->ptr,
-                        dev<uint32_t>(sorted_splats),
-                        sorted_splats == tr->alt_splats ? tr->pair_tiles->ptr : tr->alt_tiles->ptr,
-                        dev<uint32_t>(sorted_splats == tr->alt_splats ? tr->pair_splats : tr->alt_splats),
-                        dev<uint32_t>(tr->pair_total), tr->capacity, 0, tile_sort_bits(tiles),
-                        dio::sort_workspace(tr->pair_sort_ws->ptr, tr->capacity));
+      // The whole binning sort (Gaussians by depth, then pairs by tile) on the data a
+      // step really sorts: the chain preprocess -> depth sort -> scan -> duplicate ->
+      // tile sort, less the same chain without the two sorts (timed apart below).
+      const uint32_t tiles = dio::ceil_div(tr->frame_w, dio::kTile) * dio::ceil_div(tr->frame_h, dio::kTile);
+      auto chain = [&](bool sorts) {
+        dio::preprocess(s, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
+                        dev<float>(sc->splats), dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_keys),
+                        dev<uint32_t>(tr->depth_values));
+        if (sorts)
+          dio::radix_sort(s, false, tr->depth_keys->ptr, dev<uint32_t>(tr->depth_values), tr->depth_alt_keys->ptr,
+                          dev<uint32_t>(tr->depth_alt_values), nullptr, tr->count, 0, 32,
+                          dio::sort_workspace(tr->splat_sort_ws->ptr, tr->count), tr->digit_bits);
+        dio::scan_exclusive_gather(s, dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_values),
+                                   dev<uint32_t>(tr->offsets), nullptr, tr->count, dev<uint32_t>(tr->pair_total),
+                                   dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
+        dio::duplicate(s, dev<float>(sc->splats), dev<uint32_t>(tr->depth_values), dev<uint32_t>(tr->offsets),
+                       dev<uint32_t>(tr->tile_counts), tr->count, tr->frame_w, tr->frame_h, tr->capacity,
+                       dev<uint32_t>(tr->pair_total), dev<uint32_t>(tr->pair_tiles), dev<uint32_t>(tr->pair_splats));
+        if (sorts)
+          dio::radix_sort(s, false, tr->pair_tiles->ptr, dev<uint32_t>(tr->pair_splats), tr->alt_tiles->ptr,
+                          dev<uint32_t>(tr->alt_splats), dev<uint32_t>(tr->pair_total), tr->capacity, 0,
+                          tile_sort_bits(tiles, tr->digit_bits), dio::sort_workspace(tr->pair_sort_ws->ptr, tr->capacity),
+                          tr->digit_bits);
       };
+      hipEvent_t e0, e1, e2;
+      DIO_HIP(hipEventCreate(&e0));
+      DIO_HIP(hipEventCreate(&e1));
+      DIO_HIP(hipEventCreate(&e2));
+      chain(true);  // warm
+      DIO_HIP(hipEventRecord(e0, s));
+      for (uint32_t k = 0; k < repeats; k++) chain(true);
+      DIO_HIP(hipEventRecord(e1, s));
+      for (uint32_t k = 0; k < repeats; k++) chain(false);
+      DIO_HIP(hipEventRecord(e2, s));
+      DIO_HIP(hipEventSynchronize(e2));
+      float with_sorts = 0.f, without = 0.f;
+      DIO_HIP(hipEventElapsedTime(&with_sorts, e0, e1));
+      DIO_HIP(hipEventElapsedTime(&without, e1, e2));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipEventDestroy(e2);
+      // Leave the frame as a step leaves it.
+      chain(true);
+      tr->sorted_in_alt = (tile_sort_bits(tiles, tr->digit_bits) / tr->digit_bits) % 2 == 1;
+      sync(t->ctx);
+      *ms = std::max(0.0, static_cast<double>(with_sorts - without)) / repeats;
+      return;
     } else if (name == "rasterize") {
       run = [&] {
         dio::rasterize(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges), tr->frame_w,
-                       tr->frame_h, tr->frame_bg, nullptr, dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last));
+                       tr->frame_h, tr->frame_bg, nullptr, dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last),
+                       tr->splat_batch);
       };
     } else if (name == "rasterize_backward") {
       run = [&] {
         dio::rasterize_backward(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges),
                                 tr->frame_w, tr->frame_h, tr->frame_bg, dev<float>(tr->pixel_color),
                                 dev<uint32_t>(tr->pixel_last), dev<float>(t->loss->pixel_grads),
-                                dev<float>(sc->grad_splats));
+                                dev<float>(sc->grad_splats), tr->splat_batch);
       };
     } else if (name == "preprocess_backward") {
       run = [&] {
@@ -2155,3 +2196,83 @@ extern "C" int dio_trainer_features(dio_trainer* t, dio_buf** features) {
 }
 
 extern "C" dio_buf* dio_trainer_extra_stats(dio_trainer* t) { return t ? t->learned.extra : nullptr; }
+
+// ================================================================= tuning
+
+namespace {
+
+void check_tuning(const dio_tuning& k) {
+  if (k.tile_size != 16) invalid("the tile size is 16 pixels (the reference's tiling decides which pixels a splat touches)");
+  if (k.splat_batch != 64 && k.splat_batch != 128 && k.splat_batch != 256) invalid("splat batch must be 64, 128 or 256");
+  if (k.digit_bits != 4 && k.digit_bits != 8) invalid("digit width must be 4 or 8 bits");
+}
+
+void apply_tuning(dio_trainer* t, const dio_tuning& k) {
+  check_tuning(k);
+  const bool changed = k.splat_batch != t->tuning.splat_batch || k.digit_bits != t->tuning.digit_bits;
+  t->tuning = k;
+  if (changed) t->generation++;  // a captured batch graph bakes the launches in
+}
+
+}  // namespace
+
+extern "C" int dio_trainer_set_tuning(dio_trainer* t, const dio_tuning* tuning) {
+  return guard([&] {
+    need(t, "trainer");
+    need(tuning, "tuning");
+    apply_tuning(t, *tuning);
+  });
+}
+
+extern "C" int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out) {
+  return guard([&] {
+    need(t, "trainer");
+    need(out, "out");
+    *out = t->tuning;
+  });
+}
+
+extern "C" int dio_trainer_autotune(dio_trainer* t, uint32_t steps, dio_tuning_result* results, uint32_t capacity,
+                                    uint32_t* count) {
+  return guard([&] {
+    need(t, "trainer");
+    if (steps == 0) invalid("the autotuner needs at least one timed step per candidate");
+    hipStream_t s = t->ctx->stream;
+    hipEvent_t e0, e1;
+    DIO_HIP(hipEventCreate(&e0));
+    DIO_HIP(hipEventCreate(&e1));
+    std::vector<dio_tuning_result> found;
+    try {
+      for (uint32_t bits : {8u, 4u})
+        for (uint32_t batch : {64u, 128u, 256u}) {
+          apply_tuning(t, dio_tuning{16, batch, bits});
+          trainer_optimize(t);  // untimed: captures a batch graph, warms the launches
+          trainer_maintain(t);
+          DIO_HIP(hipEventRecord(e0, s));
+          for (uint32_t k = 0; k < steps; k++) {
+            trainer_optimize(t);
+            trainer_maintain(t);
+          }
+          DIO_HIP(hipEventRecord(e1, s));
+          DIO_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          DIO_HIP(hipEventElapsedTime(&ms, e0, e1));
+          found.push_back({t->tuning, static_cast<double>(ms) / steps});
+        }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const auto best = std::min_element(found.begin(), found.end(),
+                                       [](const dio_tuning_result& a, const dio_tuning_result& b) {
+                                         return a.ms_per_step < b.ms_per_step;
+                                       });
+    apply_tuning(t, best->tuning);
+    if (count) *count = static_cast<uint32_t>(found.size());
+    if (results)
+      for (uint32_t k = 0; k < capacity && k < found.size(); k++) results[k] = found[k];
+  });
+}

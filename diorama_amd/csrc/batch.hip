@@ -75,6 +75,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_batch_kernel(const float4
   }
 }
 
+template <int kPer>
 __global__ __launch_bounds__(256) void rasterize_batch_kernel(const float4* __restrict__ splats,
                                                                const uint32_t* __restrict__ pair_splats,
                                                                const uint2* __restrict__ ranges, BatchArgs b,
@@ -85,21 +86,22 @@ __global__ __launch_bounds__(256) void rasterize_batch_kernel(const float4* __re
   if (tile >= b.header->tiles) return;
   const ViewDesc& view = view_of_tile(b, tile);
   const uint32_t local = tile - view.tile_base;
-  raster::rasterize_tile(splats, pair_splats, ranges[tile], local % view.tiles_x, local / view.tiles_x, view.width,
-                         view.height, bg_r, bg_g, bg_b, nullptr, pixel_color + view.pixel_base,
-                         pixel_last + view.pixel_base);
+  raster::rasterize_tile<kPer>(splats, pair_splats, ranges[tile], local % view.tiles_x, local / view.tiles_x,
+                               view.width, view.height, bg_r, bg_g, bg_b, nullptr, pixel_color + view.pixel_base,
+                               pixel_last + view.pixel_base);
 }
 
+template <int kPer>
 __global__ __launch_bounds__(256) void rasterize_backward_batch_kernel(
     const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
     BatchArgs b, float bg_r, float bg_g, float bg_b, const float4* __restrict__ pixel_color,
     const uint32_t* __restrict__ pixel_last, const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads) {
-  __shared__ raster::BackwardShared shared;
+  __shared__ raster::BackwardShared<kPer> shared;
   const uint32_t tile = blockIdx.x;
   if (tile >= b.header->tiles) return;
   const ViewDesc& view = view_of_tile(b, tile);
   const uint32_t local = tile - view.tile_base;
-  raster::rasterize_backward_tile(splats, pair_splats, ranges[tile], local % view.tiles_x, local / view.tiles_x,
+  raster::rasterize_backward_tile<kPer>(splats, pair_splats, ranges[tile], local % view.tiles_x, local / view.tiles_x,
                                   view.width, view.height, bg_r, bg_g, bg_b, pixel_color + view.pixel_base,
                                   pixel_last + view.pixel_base, pixel_grads + view.pixel_base, splat_grads, shared);
 }
@@ -218,12 +220,20 @@ void preprocess_batch(hipStream_t s, const float* gaussians, const float* sh, ui
 }
 
 void rasterize_batch(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
-                     const BatchArgs& b, const float* background, float* pixel_color, uint32_t* pixel_last) {
+                     const BatchArgs& b, const float* background, float* pixel_color, uint32_t* pixel_last,
+                     uint32_t splat_batch) {
   if (b.max_tiles == 0) return;
-  rasterize_batch_kernel<<<b.max_tiles, 256, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
-                                                      reinterpret_cast<const uint2*>(ranges), b, background[0],
-                                                      background[1], background[2],
-                                                      reinterpret_cast<float4*>(pixel_color), pixel_last);
+  auto launch = [&](auto kernel) {
+    kernel<<<b.max_tiles, 256, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                       reinterpret_cast<const uint2*>(ranges), b, background[0], background[1],
+                                       background[2], reinterpret_cast<float4*>(pixel_color), pixel_last);
+  };
+  switch (splat_batch) {
+    case 64: launch(rasterize_batch_kernel<1>); break;
+    case 128: launch(rasterize_batch_kernel<2>); break;
+    case 256: launch(rasterize_batch_kernel<4>); break;
+    default: throw Error(-1, "splat batch must be 64, 128 or 256");
+  }
   DIO_LAUNCHED("rasterize_batch");
 }
 
@@ -243,12 +253,20 @@ void training_loss_batch(hipStream_t s, const BatchArgs& b, const float* rendere
 void rasterize_backward_batch(hipStream_t s, const float* splats, const uint32_t* pair_splats,
                               const uint32_t* ranges, const BatchArgs& b, const float* background,
                               const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
-                              float* splat_grads) {
+                              float* splat_grads, uint32_t splat_batch) {
   if (b.max_tiles == 0) return;
-  rasterize_backward_batch_kernel<<<b.max_tiles, 256, 0, s>>>(
-      reinterpret_cast<const float4*>(splats), pair_splats, reinterpret_cast<const uint2*>(ranges), b, background[0],
-      background[1], background[2], reinterpret_cast<const float4*>(pixel_color), pixel_last,
-      reinterpret_cast<const float4*>(pixel_grads), splat_grads);
+  auto launch = [&](auto kernel) {
+    kernel<<<b.max_tiles, 256, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                       reinterpret_cast<const uint2*>(ranges), b, background[0], background[1],
+                                       background[2], reinterpret_cast<const float4*>(pixel_color), pixel_last,
+                                       reinterpret_cast<const float4*>(pixel_grads), splat_grads);
+  };
+  switch (splat_batch) {
+    case 64: launch(rasterize_backward_batch_kernel<1>); break;
+    case 128: launch(rasterize_backward_batch_kernel<2>); break;
+    case 256: launch(rasterize_backward_batch_kernel<4>); break;
+    default: throw Error(-1, "splat batch must be 64, 128 or 256");
+  }
   DIO_LAUNCHED("rasterize_backward_batch");
 }
 

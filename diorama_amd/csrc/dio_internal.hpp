@@ -52,7 +52,7 @@ void scan_exclusive_gather(hipStream_t s, const uint32_t* in, const uint32_t* id
                            const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws);
 
 // ------------------------------------------------------------------ sort --
-// Stable LSD onesweep radix sort, 8-bit digits, of u32 keys or u64 keys stored as
+// Stable LSD onesweep radix sort, 8- or 4-bit digits, of u32 keys or u64 keys stored as
 // (low word, high word), each carrying a u32 value. Sorts key bits
 // [begin_bit, end_bit); the passes ping-pong between (keys, values) and the
 // alternate buffers. Returns true when the result ended in the alternate buffers.
@@ -65,9 +65,10 @@ struct SortWorkspace {
 };
 size_t sort_workspace_bytes(uint32_t capacity);
 SortWorkspace sort_workspace(void* base, uint32_t capacity);
+// digit_bits: 8 or 4 (more passes of fewer buckets); the result is the same.
 bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
                 const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
-                const SortWorkspace& ws);
+                const SortWorkspace& ws, uint32_t digit_bits = 8);
 
 // ------------------------------------------------------------ renderer --
 struct CameraArgs {
@@ -91,13 +92,16 @@ void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const 
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
                  uint32_t tiles, uint32_t* ranges);
 
+// splat_batch: pairs a wave stages per round (64, 128 or 256); results do not
+// depend on it.
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                uint32_t width, uint32_t height, const float* background, uint32_t* target, float* pixel_color,
-               uint32_t* pixel_last);
+               uint32_t* pixel_last, uint32_t splat_batch = 64);
 
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                         uint32_t width, uint32_t height, const float* background, const float* pixel_color,
-                        const uint32_t* pixel_last, const float* pixel_grads, float* splat_grads);
+                        const uint32_t* pixel_last, const float* pixel_grads, float* splat_grads,
+                        uint32_t splat_batch = 64);
 
 void preprocess_backward(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                          const CameraArgs& cam, const float* splats, const float* splat_grads,
@@ -138,13 +142,14 @@ void duplicate_batch(hipStream_t s, const float* splats, const uint32_t* order, 
                      const uint32_t* tile_counts, const BatchArgs& b, uint32_t capacity, const uint32_t* pair_total,
                      uint32_t* pair_tiles, uint32_t* pair_splats);
 void rasterize_batch(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
-                     const BatchArgs& b, const float* background, float* pixel_color, uint32_t* pixel_last);
+                     const BatchArgs& b, const float* background, float* pixel_color, uint32_t* pixel_last,
+                     uint32_t splat_batch = 64);
 void training_loss_batch(hipStream_t s, const BatchArgs& b, const float* rendered, float* partials,
                          float* pixel_grads, float* pixel_loss);
 void rasterize_backward_batch(hipStream_t s, const float* splats, const uint32_t* pair_splats,
                               const uint32_t* ranges, const BatchArgs& b, const float* background,
                               const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
-                              float* splat_grads);
+                              float* splat_grads, uint32_t splat_batch = 64);
 // Gradients summed over the batch's views, and the densification statistics of
 // every view (adam.wgsl:125-134) in the same pass.
 void preprocess_backward_batch(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree,

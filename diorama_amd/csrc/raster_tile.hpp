@@ -58,6 +58,9 @@ __device__ inline bool meets(const Staged& s, float bx0, float bx1, float by0, f
 // are nearly opaque.
 // One 16x16 tile (tile_x, tile_y) of a width x height frame whose pair list is
 // `run`; the per-pixel outputs are indexed by the frame's own row-major pixel index.
+// A wave stages kPer x 64 pairs per round (kPer per lane); the per-pixel order and
+// every result are the same for any kPer.
+template <int kPer>
 __device__ inline void rasterize_tile(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
                                       uint2 run, uint32_t tile_x, uint32_t tile_y, uint32_t width, uint32_t height,
                                       float bg_r, float bg_g, float bg_b, uint32_t* __restrict__ target,
@@ -72,38 +75,48 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
   bool done = !inside;
   uint32_t last = 0;
-  for (uint32_t base = run.x; base < run.y; base += kChunk) {
+  for (uint32_t round = run.x; round < run.y; round += kPer * kChunk) {
     if (__all(done)) break;
-    Staged s;
-    bool hit = false;
-    if (base + lane < run.y) {
-      stage(splats, pair_splats, base + lane, s);
-      hit = meets(s, bx0, bx1, by0, by1);
+    Staged st[kPer];
+    bool hits[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      hits[q] = false;
+      const uint32_t k = round + q * kChunk + lane;
+      if (k < run.y) {
+        stage(splats, pair_splats, k, st[q]);
+        hits[q] = meets(st[q], bx0, bx1, by0, by1);
+      }
     }
-    uint64_t mask = __ballot(hit);
-    while (mask) {
-      const int j = __builtin_ctzll(mask);
-      mask &= mask - 1;
-      const float mx = lane_value(s.mx, j), my = lane_value(s.my, j);
-      const float ca = lane_value(s.ca, j), cbb = lane_value(s.cb, j), cc = lane_value(s.cc, j);
-      const float op = lane_value(s.op, j);
-      const float sr = lane_value(s.r, j), sg = lane_value(s.g, j), sb = lane_value(s.b, j);
-      if (!done) {
-        const float dx = cx - mx, dy = cy - my;
-        const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
-        if (power <= 0.f) {
-          const float alpha = fminf(0.99f, op * __expf(power));
-          if (alpha >= 1.f / 255.f) {
-            const float remaining = T * (1.f - alpha);
-            if (remaining < 0.0001f) {
-              done = true;
-            } else {
-              const float w = alpha * T;
-              cr += sr * w;
-              cg += sg * w;
-              cb += sb * w;
-              T = remaining;
-              last = base + j - run.x + 1;
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const Staged& s = st[q];
+      const uint32_t base = round + q * kChunk;
+      uint64_t mask = __ballot(hits[q]);
+      while (mask) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const float mx = lane_value(s.mx, j), my = lane_value(s.my, j);
+        const float ca = lane_value(s.ca, j), cbb = lane_value(s.cb, j), cc = lane_value(s.cc, j);
+        const float op = lane_value(s.op, j);
+        const float sr = lane_value(s.r, j), sg = lane_value(s.g, j), sb = lane_value(s.b, j);
+        if (!done) {
+          const float dx = cx - mx, dy = cy - my;
+          const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
+          if (power <= 0.f) {
+            const float alpha = fminf(0.99f, op * __expf(power));
+            if (alpha >= 1.f / 255.f) {
+              const float remaining = T * (1.f - alpha);
+              if (remaining < 0.0001f) {
+                done = true;
+              } else {
+                const float w = alpha * T;
+                cr += sr * w;
+                cg += sg * w;
+                cb += sb * w;
+                T = remaining;
+                last = base + j - run.x + 1;
+              }
             }
           }
         }
@@ -121,8 +134,6 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
     }
   }
 }
-
-// ------------------------------------------------------ rasterize backward --
 
 template <int kCtrl>
 __device__ inline float dpp_add(float v) {
@@ -210,21 +221,25 @@ __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy
 // leaders park the totals in LDS; per 64-splat chunk the four quadrant waves add
 // theirs and one float atomic per (splat, tile, gradient) goes out.
 // The workgroup's LDS for one backward tile.
+// The workgroup's LDS for one backward tile with kPer x 64 pairs per round.
+template <int kPer>
 struct BackwardShared {
-  float sums[4][kGrads][kChunk];
-  uint32_t index[kChunk];
+  float sums[4][kGrads][kPer * kChunk];
+  uint32_t index[kPer * kChunk];
   uint32_t last[4];
 };
 
-// One 16x16 tile of the backward pass; the workgroup may call it again for another
-// tile once it returns (it ends on a barrier).
+// One 16x16 tile of the backward pass, kPer x 64 pairs per round; the workgroup
+// may call it again for another tile once it returns (it ends on a barrier).
+template <int kPer>
 __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats,
                                                const uint32_t* __restrict__ pair_splats, uint2 run, uint32_t tile_x,
                                                uint32_t tile_y, uint32_t width, uint32_t height, float bg_r,
                                                float bg_g, float bg_b, const float4* __restrict__ pixel_color,
                                                const uint32_t* __restrict__ pixel_last,
                                                const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads,
-                                               BackwardShared& sh) {
+                                               BackwardShared<kPer>& sh) {
+  constexpr int kRound = kPer * kChunk;
   auto& s_sums = sh.sums;
   auto& s_index = sh.index;
   auto& s_last = sh.last;
@@ -253,52 +268,63 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
   const int row = lane >> 4;
   const bool row_leader = (lane & 15) == 0;
 
-  for (int c = static_cast<int>(ceil_div(tile_last, kChunk)) - 1; c >= 0; c--) {
-    const uint32_t base = run.x + static_cast<uint32_t>(c) * kChunk;
-    Staged s;
-    bool hit = false;
-    s.index = 0;
-    if (base + lane < run.y) {
-      stage(splats, pair_splats, base + lane, s);
-      hit = base + lane - run.x < my_last && meets(s, bx0, bx1, by0, by1);
-    }
-    if (wave == 0) s_index[lane] = s.index;
+  for (int c = static_cast<int>(ceil_div(tile_last, kRound)) - 1; c >= 0; c--) {
+    const uint32_t round = run.x + static_cast<uint32_t>(c) * kRound;
+    Staged sts[kPer];
+    bool hits[kPer];
 #pragma unroll
-    for (int v = 0; v < kGrads; v++) s_sums[wave][v][lane] = 0.f;
-    uint64_t mask = __ballot(hit);
-    while (mask) {
-      // Up to four splats, back to front; missing ones contribute zeros.
-      float share[4][kGrads];
-      int js[4];
-      bool any = false;
-#pragma unroll
-      for (int g = 0; g < 4; g++) {
-        js[g] = -1;
-        bool blended = false;
-        float mx = 0.f, my = 0.f, ca = 0.f, cbb = 0.f, cc = 0.f, op = 0.f, sr = 0.f, sg = 0.f, sb = 0.f;
-        if (mask) {
-          const int j = kWave - 1 - __builtin_clzll(mask);
-          mask &= ~(1ull << j);
-          js[g] = j;
-          mx = lane_value(s.mx, j); my = lane_value(s.my, j);
-          ca = lane_value(s.ca, j); cbb = lane_value(s.cb, j); cc = lane_value(s.cc, j);
-          op = lane_value(s.op, j);
-          sr = lane_value(s.r, j); sg = lane_value(s.g, j); sb = lane_value(s.b, j);
-          blended = base + j - run.x < last;
-        }
-        any |= replay_splat(st, blended, cx, cy, mx, my, ca, cbb, cc, op, sr, sg, sb, share[g]);
+    for (int q = 0; q < kPer; q++) {
+      const uint32_t k = round + q * kChunk + lane;
+      hits[q] = false;
+      sts[q].index = 0;
+      if (k < run.y) {
+        stage(splats, pair_splats, k, sts[q]);
+        hits[q] = k - run.x < my_last && meets(sts[q], bx0, bx1, by0, by1);
       }
-      if (!__any(any)) continue;
-      const int mine = row == 0 ? js[0] : (row == 1 ? js[1] : (row == 2 ? js[2] : js[3]));
+      if (wave == 0) s_index[q * kChunk + lane] = sts[q].index;
 #pragma unroll
-      for (int v = 0; v < kGrads; v++) {
-        const float total = four_totals(share[0][v], share[1][v], share[2][v], share[3][v]);
-        if (row_leader && mine >= 0) s_sums[wave][v][mine] = total;
+      for (int v = 0; v < kGrads; v++) s_sums[wave][v][q * kChunk + lane] = 0.f;
+    }
+    // Back to front: the last 64 of the round first.
+#pragma unroll
+    for (int q = kPer - 1; q >= 0; q--) {
+      const Staged& s = sts[q];
+      const uint32_t base = round + q * kChunk;
+      uint64_t mask = __ballot(hits[q]);
+      while (mask) {
+        // Up to four splats, back to front; missing ones contribute zeros.
+        float share[4][kGrads];
+        int js[4];
+        bool any = false;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          js[g] = -1;
+          bool blended = false;
+          float mx = 0.f, my = 0.f, ca = 0.f, cbb = 0.f, cc = 0.f, op = 0.f, sr = 0.f, sg = 0.f, sb = 0.f;
+          if (mask) {
+            const int j = kWave - 1 - __builtin_clzll(mask);
+            mask &= ~(1ull << j);
+            js[g] = q * kChunk + j;
+            mx = lane_value(s.mx, j); my = lane_value(s.my, j);
+            ca = lane_value(s.ca, j); cbb = lane_value(s.cb, j); cc = lane_value(s.cc, j);
+            op = lane_value(s.op, j);
+            sr = lane_value(s.r, j); sg = lane_value(s.g, j); sb = lane_value(s.b, j);
+            blended = base + j - run.x < last;
+          }
+          any |= replay_splat(st, blended, cx, cy, mx, my, ca, cbb, cc, op, sr, sg, sb, share[g]);
+        }
+        if (!__any(any)) continue;
+        const int mine = row == 0 ? js[0] : (row == 1 ? js[1] : (row == 2 ? js[2] : js[3]));
+#pragma unroll
+        for (int v = 0; v < kGrads; v++) {
+          const float total = four_totals(share[0][v], share[1][v], share[2][v], share[3][v]);
+          if (row_leader && mine >= 0) s_sums[wave][v][mine] = total;
+        }
       }
     }
     __syncthreads();
-    for (int e = tid; e < kGrads * kChunk; e += 256) {
-      const int v = e / kChunk, slot = e % kChunk;
+    for (int e = tid; e < kGrads * kRound; e += 256) {
+      const int v = e / kRound, slot = e % kRound;
       const float total = s_sums[0][v][slot] + s_sums[1][v][slot] + s_sums[2][v][slot] + s_sums[3][v][slot];
       if (total != 0.f) unsafeAtomicAdd(splat_grads + static_cast<size_t>(s_index[slot]) * kGrads + v, total);
     }

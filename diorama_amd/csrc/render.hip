@@ -201,6 +201,7 @@ __global__ __launch_bounds__(kThreads) void find_ranges_kernel(const uint32_t* _
 // -------------------------------------------------------------- rasterize --
 
 // rasterize.wgsl:31-100: front-to-back blend, one workgroup per tile.
+template <int kPer>
 __global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict__ splats,
                                                          const uint32_t* __restrict__ pair_splats,
                                                          const uint2* __restrict__ ranges, uint32_t width,
@@ -209,20 +210,22 @@ __global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict
                                                          float4* __restrict__ pixel_color,
                                                          uint32_t* __restrict__ pixel_last) {
   const uint32_t tile = blockIdx.x;
-  raster::rasterize_tile(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width, height, bg_r, bg_g,
-                         bg_b, target, pixel_color, pixel_last);
+  raster::rasterize_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width, height, bg_r,
+                               bg_g, bg_b, target, pixel_color, pixel_last);
 }
 
 // rasterizeBackward.wgsl:49-163, one workgroup per tile.
+template <int kPer>
 __global__ __launch_bounds__(256) void rasterize_backward_kernel(
     const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
     uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g, float bg_b,
     const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
     const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads) {
-  __shared__ raster::BackwardShared shared;
+  __shared__ raster::BackwardShared<kPer> shared;
   const uint32_t tile = blockIdx.x;
-  raster::rasterize_backward_tile(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width, height,
-                                  bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads, shared);
+  raster::rasterize_backward_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width,
+                                        height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads,
+                                        shared);
 }
 
 // ----------------------------------------------------- preprocess backward --
@@ -341,25 +344,42 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
 
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                uint32_t width, uint32_t height, const float* background, uint32_t* target, float* pixel_color,
-               uint32_t* pixel_last) {
+               uint32_t* pixel_last, uint32_t splat_batch) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
-  rasterize_kernel<<<tx * ty, kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
-                                                reinterpret_cast<const uint2*>(ranges), width, height, tx,
-                                                background[0], background[1], background[2], target,
-                                                reinterpret_cast<float4*>(pixel_color), pixel_last);
+  auto launch = [&](auto kernel) {
+    kernel<<<tx * ty, kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                        reinterpret_cast<const uint2*>(ranges), width, height, tx, background[0],
+                                        background[1], background[2], target, reinterpret_cast<float4*>(pixel_color),
+                                        pixel_last);
+  };
+  switch (splat_batch) {
+    case 64: launch(rasterize_kernel<1>); break;
+    case 128: launch(rasterize_kernel<2>); break;
+    case 256: launch(rasterize_kernel<4>); break;
+    default: throw Error(-1, "splat batch must be 64, 128 or 256");
+  }
   DIO_LAUNCHED("rasterize");
 }
 
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                         uint32_t width, uint32_t height, const float* background, const float* pixel_color,
-                        const uint32_t* pixel_last, const float* pixel_grads, float* splat_grads) {
+                        const uint32_t* pixel_last, const float* pixel_grads, float* splat_grads,
+                        uint32_t splat_batch) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
-  rasterize_backward_kernel<<<tx * ty, kThreads, 0, s>>>(
-      reinterpret_cast<const float4*>(splats), pair_splats, reinterpret_cast<const uint2*>(ranges), width, height, tx,
-      background[0], background[1], background[2], reinterpret_cast<const float4*>(pixel_color), pixel_last,
-      reinterpret_cast<const float4*>(pixel_grads), splat_grads);
+  auto launch = [&](auto kernel) {
+    kernel<<<tx * ty, kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                        reinterpret_cast<const uint2*>(ranges), width, height, tx, background[0],
+                                        background[1], background[2], reinterpret_cast<const float4*>(pixel_color),
+                                        pixel_last, reinterpret_cast<const float4*>(pixel_grads), splat_grads);
+  };
+  switch (splat_batch) {
+    case 64: launch(rasterize_backward_kernel<1>); break;
+    case 128: launch(rasterize_backward_kernel<2>); break;
+    case 256: launch(rasterize_backward_kernel<4>); break;
+    default: throw Error(-1, "splat batch must be 64, 128 or 256");
+  }
   DIO_LAUNCHED("rasterize_backward");
 }
 

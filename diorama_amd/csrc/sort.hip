@@ -1,4 +1,4 @@
-// Stable LSD radix sort with 8-bit digits over u32 keys, or u64 keys stored as
+// Stable LSD radix sort with 8-bit (or 4-bit) digits over u32 keys, or u64 keys stored as
 // (low word, high word) pairs, each key carrying a u32 value: the drop-in for
 // src/radixSort.ts:27-142 / src/shaders/radixSort.wgsl:44-120 (same result: a
 // stable sort by key bits [begin, end)), built as a onesweep sort for wave64:
@@ -19,26 +19,28 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kItems = kSortTile / kThreads;  // 16
 constexpr int kWaves = kThreads / kWave;
-constexpr uint32_t kMaxPasses = 8;
+constexpr uint32_t kMaxPasses = 16;  // 64 key bits in 4-bit digits
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1;
 
-template <bool kWide>
+// Keys of one width with digits of kBits (8 or 4) bits; a digit never straddles
+// the two words of a 64-bit key.
+template <bool kWide, int kBits>
 struct KeyOps;
 
-template <>
-struct KeyOps<false> {
+template <int kBits>
+struct KeyOps<false, kBits> {
   using T = uint32_t;
-  __device__ static uint32_t digit(T k, uint32_t shift) { return (k >> shift) & 0xffu; }
+  __device__ static uint32_t digit(T k, uint32_t shift) { return (k >> shift) & ((1u << kBits) - 1u); }
   __device__ static T invalid() { return 0xffffffffu; }
 };
 
-template <>
-struct KeyOps<true> {
+template <int kBits>
+struct KeyOps<true, kBits> {
   using T = uint2;  // (low word, high word)
   __device__ static uint32_t digit(T k, uint32_t shift) {
-    return shift >= 32 ? (k.y >> (shift - 32)) & 0xffu : (k.x >> shift) & 0xffu;
+    return (shift >= 32 ? k.y >> (shift - 32) : k.x >> shift) & ((1u << kBits) - 1u);
   }
   __device__ static T invalid() { return make_uint2(0xffffffffu, 0xffffffffu); }
 };
@@ -50,20 +52,23 @@ __device__ inline uint32_t key_count(const uint32_t* count, uint32_t capacity) {
 // Counts the digits of every pass in one read of the keys (LDS histograms, one
 // per wave, added into the global ones), and clears the first pass's look-back
 // states and every pass ticket.
-template <bool kWide>
-__global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyOps<kWide>::T* __restrict__ keys,
+template <bool kWide, int kBits>
+__global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyOps<kWide, kBits>::T* __restrict__ keys,
                                                               const uint32_t* count, uint32_t capacity,
                                                               uint32_t begin_bit, uint32_t passes,
                                                               uint32_t* __restrict__ hist, uint32_t* __restrict__ state0,
                                                               uint32_t* __restrict__ tickets) {
-  __shared__ uint32_t local[kMaxPasses][256];
+  using Ops = KeyOps<kWide, kBits>;
+  constexpr int kRadix = 1 << kBits;
+  __shared__ uint32_t local[kMaxPasses][kRadix];
   const int tid = threadIdx.x;
   const uint32_t n = key_count(count, capacity);
   const uint32_t tiles = ceil_div(n, kSortTile);
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t k = blockIdx.x * kThreads + tid; k < tiles * 256u; k += stride) state0[k] = 0;
   if (blockIdx.x == 0 && tid < static_cast<int>(kMaxPasses)) tickets[tid] = 0;
-  for (uint32_t p = 0; p < passes; p++) local[p][tid] = 0;
+  if (tid < kRadix)
+    for (uint32_t p = 0; p < passes; p++) local[p][tid] = 0;
   __syncthreads();
   // Lanes holding the same digit first agree on one count (match masks), so a few
   // hot digits (a float's exponent byte) do not serialise the LDS atomics.
@@ -73,12 +78,12 @@ __global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyO
   for (uint32_t r = 0; r < rounds; r++) {
     const uint32_t i = r * stride + blockIdx.x * kThreads + tid;
     const bool valid = i < n;
-    const auto key = valid ? keys[i] : KeyOps<kWide>::invalid();
+    const auto key = valid ? keys[i] : Ops::invalid();
     for (uint32_t p = 0; p < passes; p++) {
-      const uint32_t d = KeyOps<kWide>::digit(key, begin_bit + 8 * p);
+      const uint32_t d = Ops::digit(key, begin_bit + kBits * p);
       uint64_t peers = __ballot(valid);
 #pragma unroll
-      for (int b = 0; b < 8; b++) {
+      for (int b = 0; b < kBits; b++) {
         const bool bit = (d >> b) & 1u;
         const uint64_t set = __ballot(bit);
         peers &= bit ? set : ~set;
@@ -87,8 +92,9 @@ __global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyO
     }
   }
   __syncthreads();
-  for (uint32_t p = 0; p < passes; p++)
-    if (local[p][tid]) atomicAdd(&hist[p * 256 + tid], local[p][tid]);
+  if (tid < kRadix)
+    for (uint32_t p = 0; p < passes; p++)
+      if (local[p][tid]) atomicAdd(&hist[p * 256 + tid], local[p][tid]);
 }
 
 // Turns each pass's digit counts into where that digit's run starts.
@@ -119,21 +125,22 @@ __device__ inline uint32_t peek(uint32_t* state) {
   return __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <bool kWide>
-__global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOps<kWide>::T* __restrict__ keys_in,
+template <bool kWide, int kBits>
+__global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOps<kWide, kBits>::T* __restrict__ keys_in,
                                                              const uint32_t* __restrict__ values_in,
-                                                             typename KeyOps<kWide>::T* __restrict__ keys_out,
+                                                             typename KeyOps<kWide, kBits>::T* __restrict__ keys_out,
                                                              uint32_t* __restrict__ values_out, const uint32_t* count,
                                                              uint32_t capacity, uint32_t shift,
                                                              const uint32_t* __restrict__ digit_start, uint32_t* state,
                                                              uint32_t* __restrict__ next_state, uint32_t* ticket) {
-  using Ops = KeyOps<kWide>;
+  using Ops = KeyOps<kWide, kBits>;
   using Key = typename Ops::T;
-  __shared__ uint32_t wave_count[kWaves][256];
-  __shared__ uint32_t wave_offset[kWaves][256];
-  __shared__ uint32_t running[256];
-  __shared__ uint32_t local_start[256];
-  __shared__ uint32_t run_start[256];
+  constexpr int kRadix = 1 << kBits;  // thread d < kRadix owns digit d
+  __shared__ uint32_t wave_count[kWaves][kRadix];
+  __shared__ uint32_t wave_offset[kWaves][kRadix];
+  __shared__ uint32_t running[kRadix];
+  __shared__ uint32_t local_start[kRadix];
+  __shared__ uint32_t run_start[kRadix];
   __shared__ uint32_t wave_totals[kWaves];
   __shared__ uint32_t s_tile;
   __shared__ Key staged_keys[kSortTile];
@@ -141,9 +148,11 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-  running[tid] = 0;
+  if (tid < kRadix) {
+    running[tid] = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; w++) wave_count[w][tid] = 0;
+    for (int w = 0; w < kWaves; w++) wave_count[w][tid] = 0;
+  }
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t n = key_count(count, capacity);
@@ -176,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
     const uint32_t d = Ops::digit(key[k], shift);
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; b++) {
+    for (int b = 0; b < kBits; b++) {
       const bool bit = (d >> b) & 1u;
       const uint64_t set = __ballot(bit);
       peers &= bit ? set : ~set;
@@ -190,7 +199,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
   }
   __syncthreads();
   // Per digit: the earlier waves' counts offset each wave's ranks.
-  {
+  if (tid < kRadix) {
     uint32_t r = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; w++) {
@@ -206,35 +215,37 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
 
   // This tile's count of digit `tid`: publish it, then add up earlier tiles' counts
   // back to the first one that knows its inclusive prefix.
-  const uint32_t mine = running[tid];
+  const uint32_t mine = tid < kRadix ? running[tid] : 0u;
   uint32_t exclusive = 0;
-  if (tile == 0) {
-    publish(state + tid, kFlagPrefix | mine);
-  } else {
-    publish(state + tile * 256 + tid, kFlagAggregate | mine);
-    // Sixteen predecessors per step, loaded together: every tile runs at once at
-    // these sizes, so the published prefixes advance a window per round trip.
-    constexpr int kWindow = 16;
-    for (int32_t t0 = static_cast<int32_t>(tile) - 1;; t0 -= kWindow) {
-      uint32_t word[kWindow];
+  if (tid < kRadix) {
+    if (tile == 0) {
+      publish(state + tid, kFlagPrefix | mine);
+    } else {
+      publish(state + tile * 256 + tid, kFlagAggregate | mine);
+      // Sixteen predecessors per step, loaded together: every tile runs at once at
+      // these sizes, so the published prefixes advance a window per round trip.
+      constexpr int kWindow = 16;
+      for (int32_t t0 = static_cast<int32_t>(tile) - 1;; t0 -= kWindow) {
+        uint32_t word[kWindow];
 #pragma unroll
-      for (int k = 0; k < kWindow; k++) word[k] = t0 - k >= 0 ? peek(state + (t0 - k) * 256 + tid) : kFlagPrefix;
+        for (int k = 0; k < kWindow; k++) word[k] = t0 - k >= 0 ? peek(state + (t0 - k) * 256 + tid) : kFlagPrefix;
 #pragma unroll
-      for (int k = 0; k < kWindow; k++)
-        while ((word[k] >> 30) == 0) word[k] = peek(state + (t0 - k) * 256 + tid);
-      bool found = false;
+        for (int k = 0; k < kWindow; k++)
+          while ((word[k] >> 30) == 0) word[k] = peek(state + (t0 - k) * 256 + tid);
+        bool found = false;
 #pragma unroll
-      for (int k = 0; k < kWindow; k++) {
-        if (!found) {
-          exclusive += word[k] & kValueMask;
-          found = (word[k] >> 30) == 2;
+        for (int k = 0; k < kWindow; k++) {
+          if (!found) {
+            exclusive += word[k] & kValueMask;
+            found = (word[k] >> 30) == 2;
+          }
         }
+        if (found) break;
       }
-      if (found) break;
+      publish(state + tile * 256 + tid, kFlagPrefix | (exclusive + mine));
     }
-    publish(state + tile * 256 + tid, kFlagPrefix | (exclusive + mine));
+    run_start[tid] = digit_start[tid] + exclusive;
   }
-  run_start[tid] = digit_start[tid] + exclusive;
 
   // Where each digit's run starts inside the tile.
   {
@@ -249,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
     uint32_t before_wave = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; w++) before_wave += w < wave ? wave_totals[w] : 0u;
-    local_start[tid] = before_wave + x - mine;
+    if (tid < kRadix) local_start[tid] = before_wave + x - mine;
   }
   __syncthreads();
 #pragma unroll
@@ -297,25 +308,25 @@ SortWorkspace sort_workspace(void* base, uint32_t capacity) {
   return ws;
 }
 
-bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
-                const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
-                const SortWorkspace& ws) {
-  if (begin_bit % 8 != 0) throw Error(-2, "radix sort: begin bit must be a multiple of 8");
-  if (end_bit > (wide ? 64u : 32u)) throw Error(-2, "radix sort: end bit past the key");
-  if (capacity >= (1u << 30)) throw Error(-2, "radix sort: at most 2^30 keys");
-  if (capacity == 0 || end_bit <= begin_bit) return false;
+namespace {
+
+template <int kBits>
+bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
+                     const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
+                     const SortWorkspace& ws) {
   const uint32_t tiles = ceil_div(capacity, kSortTile);
-  if (tiles > ws.tiles) throw Error(-2, "sort workspace too small");
-  const uint32_t passes = (end_bit - begin_bit + 7) / 8;
+  const uint32_t passes = (end_bit - begin_bit + kBits - 1) / kBits;
   if (passes > kMaxPasses) throw Error(-2, "radix sort: too many passes");
   fill_u32(s, ws.hist, 0u, passes * 256);
   const uint32_t hist_blocks = std::min<uint32_t>(tiles, 1024);
   if (wide)
-    histogram_kernel<true><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint2*>(keys), count, capacity, begin_bit,
-                                                             passes, ws.hist, ws.state[0], ws.tickets);
+    histogram_kernel<true, kBits><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint2*>(keys), count, capacity,
+                                                                    begin_bit, passes, ws.hist, ws.state[0],
+                                                                    ws.tickets);
   else
-    histogram_kernel<false><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint32_t*>(keys), count, capacity,
-                                                              begin_bit, passes, ws.hist, ws.state[0], ws.tickets);
+    histogram_kernel<false, kBits><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint32_t*>(keys), count,
+                                                                     capacity, begin_bit, passes, ws.hist, ws.state[0],
+                                                                     ws.tickets);
   DIO_LAUNCHED("radix histogram");
   digit_offsets_kernel<<<passes, 256, 0, s>>>(ws.hist);
   DIO_LAUNCHED("radix digit offsets");
@@ -324,22 +335,38 @@ bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* al
   void* kout = alt_keys;
   uint32_t* vout = alt_values;
   for (uint32_t p = 0; p < passes; p++) {
-    const uint32_t shift = begin_bit + 8 * p;
+    const uint32_t shift = begin_bit + kBits * p;
     uint32_t* state = ws.state[p % 2];
     uint32_t* next = p + 1 < passes ? ws.state[(p + 1) % 2] : nullptr;
     if (wide)
-      onesweep_kernel<true><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), vin, static_cast<uint2*>(kout),
-                                                        vout, count, capacity, shift, ws.hist + p * 256, state, next,
-                                                        ws.tickets + p);
+      onesweep_kernel<true, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), vin,
+                                                               static_cast<uint2*>(kout), vout, count, capacity, shift,
+                                                               ws.hist + p * 256, state, next, ws.tickets + p);
     else
-      onesweep_kernel<false><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), vin,
-                                                         static_cast<uint32_t*>(kout), vout, count, capacity, shift,
-                                                         ws.hist + p * 256, state, next, ws.tickets + p);
+      onesweep_kernel<false, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), vin,
+                                                                static_cast<uint32_t*>(kout), vout, count, capacity,
+                                                                shift, ws.hist + p * 256, state, next, ws.tickets + p);
     DIO_LAUNCHED("radix onesweep");
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
   return passes % 2 == 1;
+}
+
+}  // namespace
+
+bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
+                const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
+                const SortWorkspace& ws, uint32_t digit_bits) {
+  if (digit_bits != 8 && digit_bits != 4) throw Error(-1, "radix digit width must be 4 or 8 bits");
+  if (begin_bit % digit_bits != 0) throw Error(-2, "radix sort: begin bit must be a multiple of the digit width");
+  if (end_bit > (wide ? 64u : 32u)) throw Error(-2, "radix sort: end bit past the key");
+  if (capacity >= (1u << 30)) throw Error(-2, "radix sort: at most 2^30 keys");
+  if (capacity == 0 || end_bit <= begin_bit) return false;
+  if (ceil_div(capacity, kSortTile) > ws.tiles) throw Error(-2, "sort workspace too small");
+  return digit_bits == 8
+             ? radix_sort_bits<8>(s, wide, keys, values, alt_keys, alt_values, count, capacity, begin_bit, end_bit, ws)
+             : radix_sort_bits<4>(s, wide, keys, values, alt_keys, alt_values, count, capacity, begin_bit, end_bit, ws);
 }
 
 }  // namespace dio

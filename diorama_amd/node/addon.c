@@ -792,6 +792,50 @@ FN(trainer_features) {
 }
 GETTER(trainer_extra_stats, dio_trainer_extra_stats)
 
+
+/* --------------------------------------------------------------- tuning -- */
+
+static napi_value tuning_object(napi_env env, const dio_tuning* k) {
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "tileSize", number(env, k->tile_size));
+  napi_set_named_property(env, o, "splatBatch", number(env, k->splat_batch));
+  napi_set_named_property(env, o, "digitBits", number(env, k->digit_bits));
+  return o;
+}
+
+FN(trainer_get_tuning) {
+  ARGS;
+  dio_tuning k;
+  if (!ok(env, dio_trainer_get_tuning(handle(&c, 0), &k))) return NULL;
+  return tuning_object(env, &k);
+}
+FN(trainer_set_tuning) {
+  ARGS;
+  dio_tuning k;
+  float v[3];
+  if (!floats(&c, c.argv[1], v, 3)) return fail(env, "tuning must be [tileSize, splatBatch, digitBits]");
+  k.tile_size = (uint32_t)v[0];
+  k.splat_batch = (uint32_t)v[1];
+  k.digit_bits = (uint32_t)v[2];
+  if (!ok(env, dio_trainer_set_tuning(handle(&c, 0), &k))) return NULL;
+  return undefined(env);
+}
+FN(trainer_autotune) {
+  ARGS;
+  dio_tuning_result r[16];
+  uint32_t n = 0;
+  napi_value arr;
+  if (!ok(env, dio_trainer_autotune(handle(&c, 0), u32(&c, 1), r, 16, &n))) return NULL;
+  napi_create_array_with_length(env, n, &arr);
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value o = tuning_object(env, &r[k].tuning);
+    napi_set_named_property(env, o, "msPerStep", number(env, r[k].ms_per_step));
+    napi_set_element(env, arr, k, o);
+  }
+  return arr;
+}
+
 /* ---------------------------------------------------------------- init -- */
 
 #define EXPORT(name) {#name, NULL, name, NULL, NULL, NULL, napi_default, NULL}
@@ -822,7 +866,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT(mlp_create), EXPORT(mlp_get_weights), EXPORT(mlp_set_weights), EXPORT(mlp_set_input_norm),
       EXPORT(mlp_set_class_weights), EXPORT(mlp_forward), EXPORT(mlp_gradients), EXPORT(mlp_train),
       EXPORT(mlp_destroy), EXPORT(trainer_set_densify_mode), EXPORT(trainer_samples), EXPORT(trainer_features),
-      EXPORT(trainer_extra_stats),
+      EXPORT(trainer_extra_stats), EXPORT(trainer_get_tuning), EXPORT(trainer_set_tuning), EXPORT(trainer_autotune),
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

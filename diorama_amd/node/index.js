@@ -326,6 +326,16 @@ class Trainer {
     native.trainer_set_densify_mode(this.handle, code === undefined ? mode : code, mlp ? mlp.handle : null,
       o.budget || 0, o.confidence === undefined ? 0.5 : o.confidence, o.samplesPerDensify || 80000);
   }
+  get tuning() {
+    return native.trainer_get_tuning(this.handle);
+  }
+  set tuning(k) {
+    native.trainer_set_tuning(this.handle, [k.tileSize || 16, k.splatBatch || 64, k.digitBits || 8]);
+  }
+  /** Times every candidate setting over `steps` training steps and keeps the fastest. */
+  autotune(steps) {
+    return native.trainer_autotune(this.handle, steps || 3);
+  }
   samples() {
     const s = native.trainer_samples(this.handle);
     return { features: view(s.features, this), labels: view(s.labels, this), count: s.count };

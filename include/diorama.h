@@ -50,6 +50,18 @@ typedef struct dio_training_loss dio_training_loss;
 typedef struct dio_trainer dio_trainer;
 
 /* CameraData (src/gaussianRenderer.ts:11-15). */
+/* Launch settings that change speed, never results (see dio_trainer_autotune). */
+typedef struct dio_tuning {
+  uint32_t tile_size;   /* pixels per tile side: 16 (the reference's, src/tileRasterizer.ts) */
+  uint32_t splat_batch; /* pairs a wave stages per round: 64, 128 or 256 */
+  uint32_t digit_bits;  /* radix digit width of the binning sorts: 4 or 8 */
+} dio_tuning;
+
+typedef struct dio_tuning_result {
+  dio_tuning tuning;
+  double ms_per_step;
+} dio_tuning_result;
+
 typedef struct dio_camera {
   float view[16];
   float proj[16];
@@ -234,6 +246,17 @@ dio_renderer* dio_trainer_renderer(dio_trainer* t);
 void dio_trainer_destroy(dio_trainer* t);
 
 /* --------------------------------------------------------- measurement -- */
+/* ------------------------------------------------------------- tuning -- */
+/* No counterpart in the reference (the reference's settings are fixed); for
+ * callers that tune a trainer to their GPU and views. */
+int dio_trainer_set_tuning(dio_trainer* t, const dio_tuning* tuning);
+int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out);
+/* The autotuner: for every candidate (tile size 16; splat batch 64, 128, 256;
+ * digit width 4, 8) run one untimed and `steps` timed optimize() steps (ordinary
+ * training steps), keep the fastest setting, and list up to `capacity` results. */
+int dio_trainer_autotune(dio_trainer* t, uint32_t steps, dio_tuning_result* results, uint32_t capacity,
+                         uint32_t* count);
+
 /* ------------------------------------------------ learned densification -- */
 /* No counterpart in the reference; for callers that densify with a learned
  * proposal. A 12 -> 64 -> 64 -> 4 MLP (5,252 parameters, BF16 weights, FP32
