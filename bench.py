@@ -285,6 +285,10 @@ def batch_bench(args):
         steps = args.steps if batch > 1 else args.steps * BATCH_VIEWS // 4
         for _ in range(args.warmup):
             trainer.step()
+        tuned = None if args.no_autotune else trainer.autotune(steps=4)
+        if tuned:
+            log(f"batch {batch} autotune: " + ", ".join(f"{k['splat_batch']}/{k['digit_bits']}: {ms:.3f} ms"
+                                                        for k, ms in tuned))
         ctx.synchronize()
         pixels = 0
         t0 = time.perf_counter()
@@ -298,7 +302,8 @@ def batch_bench(args):
         graph_ms = trainer.time_stage("batch_step", 5)
         results[batch] = {"steps_per_s": steps / elapsed, "views_per_s": steps * batch / elapsed,
                           "Mpx_per_s": pixels / elapsed / 1e6, "ms_per_step": elapsed * 1e3 / steps,
-                          "graph_ms": graph_ms}
+                          "graph_ms": graph_ms, "splat_batch": trainer.tuning["splat_batch"],
+                          "digit_bits": trainer.tuning["digit_bits"]}
         log(f"batch {batch:2d}: {elapsed * 1e3 / steps:8.2f} ms/step, {steps * batch / elapsed:8.1f} views/s, "
             f"{pixels / elapsed / 1e6:8.1f} Mpx/s (graph replay {graph_ms:.2f} ms)")
         trainer.destroy()
@@ -491,6 +496,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
     ap.add_argument("--repeats", type=int, default=20, help="launches per stage timing")
     ap.add_argument("--workload", choices=["train", "scan-sort", "batch", "learned"], default="train",
                     help="train: the training step (default); scan-sort: the scan/sort microbenchmark; "
@@ -532,8 +538,13 @@ def main():
     trainer.sh_degree = 3
     for _ in range(args.warmup):
         trainer.step()
+    tuned = None
+    if not args.no_autotune:
+        tuned = trainer.autotune(steps=3)
+        log("autotune: " + ", ".join(f"batch {k['splat_batch']} digits {k['digit_bits']}: {ms:.3f} ms"
+                                     for k, ms in tuned))
     ctx.synchronize()
-    log(f"warmup {args.warmup} steps done")
+    log(f"warmup {args.warmup} steps done; tuning {trainer.tuning}")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         trainer.step()
@@ -567,6 +578,8 @@ def main():
         t0 = time.perf_counter()
         cpu = cpu_baseline(params, sh, train[0], images[0], 3)
         log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads ({time.perf_counter() - t0:.1f} s)")
+    trainer_tuning = trainer.tuning
+    autotune_table = None if tuned is None else [[k["splat_batch"], k["digit_bits"], round(ms, 4)] for k, ms in tuned]
     trainer.destroy()
     ctx.close()
     line = {
@@ -584,6 +597,7 @@ def main():
         "data": "synthetic: seed-7 teacher renders + N(0,0.01) noise on 263 ring poses; seed-0 student",
         "config": {"workload": "plain 3DGS training step (Trainer.step: render, L1+D-SSIM, backward, stats, Adam)",
                    "gaussians": n, "width": WIDTH, "height": HEIGHT, "sh_degree": 3, "train_views": len(train),
+                   "tuning": trainer_tuning, "autotune_ms": autotune_table,
                    "pairs": pairs, "views_per_s": round(steps_per_s, 2),
                    "step_bytes_model": step_bytes,
                    "step_GBps_model": round(step_bytes / (elapsed / args.steps) / 1e9, 1),

@@ -2248,10 +2248,17 @@ extern "C" int dio_trainer_autotune(dio_trainer* t, uint32_t steps, dio_tuning_r
           apply_tuning(t, dio_tuning{16, batch, bits});
           trainer_optimize(t);  // untimed: captures a batch graph, warms the launches
           trainer_maintain(t);
+          const bool replay = t->batch && t->batch->exec && !t->due_densify;
           DIO_HIP(hipEventRecord(e0, s));
           for (uint32_t k = 0; k < steps; k++) {
-            trainer_optimize(t);
-            trainer_maintain(t);
+            if (replay) {
+              // A batch trainer replays its step graph on the same views, so every
+              // candidate is timed on the same work.
+              DIO_HIP(hipGraphLaunch(t->batch->exec, s));
+            } else {
+              trainer_optimize(t);
+              trainer_maintain(t);
+            }
           }
           DIO_HIP(hipEventRecord(e1, s));
           DIO_HIP(hipEventSynchronize(e1));

@@ -252,8 +252,9 @@ void dio_trainer_destroy(dio_trainer* t);
 int dio_trainer_set_tuning(dio_trainer* t, const dio_tuning* tuning);
 int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out);
 /* The autotuner: for every candidate (tile size 16; splat batch 64, 128, 256;
- * digit width 4, 8) run one untimed and `steps` timed optimize() steps (ordinary
- * training steps), keep the fastest setting, and list up to `capacity` results. */
+ * digit width 4, 8) run one untimed optimize() step, then time `steps` more
+ * (ordinary training steps; a batch trainer replays its step graph on the same
+ * views), keep the fastest setting, and list up to `capacity` results. */
 int dio_trainer_autotune(dio_trainer* t, uint32_t steps, dio_tuning_result* results, uint32_t capacity,
                          uint32_t* count);
 
