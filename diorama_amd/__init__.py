@@ -172,6 +172,11 @@ SIGNATURES = {
     "dio_trainer_renderer": (_P, [_P]),
     "dio_trainer_destroy": (None, [_P]),
     "dio_trainer_time_stage": (_I, [_P, ctypes.c_char_p, _U32, ctypes.POINTER(ctypes.c_double)]),
+    "dio_mean_neighbor_distance2": (_I, [_P, _P, _U32, _P]),
+    "dio_initial_gaussians": (_I, [_P, _P, _P, _U32, _P, _P]),
+    "dio_ply_write": (_I, [_P, _P, _U32, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "dio_ply_parse": (_I, [_P, _P, ctypes.c_size_t, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_U32),
+                           ctypes.POINTER(_U32)]),
 }
 
 
@@ -417,6 +422,12 @@ class GaussianRenderer:
             raise DioramaError(f"Expected {n * 45} spherical-harmonic floats, got {s.size}.")
         _check(lib().dio_renderer_set_gaussians(self.handle, g.ctypes.data_as(_P), n,
                                                 None if s is None else s.ctypes.data_as(_P), degree))
+
+    def adopt_gaussians(self, count, gaussians, sh, degree):
+        """adoptGaussians (src/gaussianRenderer.ts:118-122): the renderer takes ownership
+        of the two device buffers and destroys the old scene."""
+        _check(lib().dio_renderer_adopt_gaussians(self.handle, count, gaussians.handle, sh.handle, degree))
+        gaussians.owned = sh.owned = False
 
     def set_sh_degree(self, degree):
         _check(lib().dio_renderer_set_sh_degree(self.handle, degree))
@@ -745,6 +756,12 @@ class Trainer:
                 "moments": _view(lib().dio_trainer_moments(self.handle), self),
                 "stats": _view(lib().dio_trainer_stats(self.handle), self)}
 
+    def write_ply(self):
+        """writeGaussianPly(trainer.exportRaw()) without the host round trip: the
+        .ply packed from the trainer's buffers in device memory."""
+        raw = {"params": self.buffers["params"], "sh": self.renderer.sh_buffer}
+        return write_gaussian_ply(raw, count=self.count)
+
     @property
     def renderer(self):
         return GaussianRenderer(self.ctx, handle=ctypes.c_void_p(lib().dio_trainer_renderer(self.handle)), keep=self)
@@ -753,3 +770,76 @@ class Trainer:
         if self.handle:
             lib().dio_trainer_destroy(self.handle)
             self.handle = None
+
+
+def write_gaussian_ply(raw, ctx=None, count=None):
+    """writeGaussianPly (src/plyWriter.ts:14-43): the .ply file of raw Gaussians as bytes.
+
+    raw: {"params": count x 16, "sh": count x 45} as host arrays (uploaded on `ctx`)
+    or device Buffers (then `count` may be given; the body is packed on the device)."""
+    params, sh = raw["params"], raw["sh"]
+    keep = []
+    if not isinstance(params, Buffer):
+        if ctx is None:
+            raise DioramaError("write_gaussian_ply needs a Context for host arrays")
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        s = np.ascontiguousarray(sh, dtype=np.float32).reshape(-1)
+        count = p.size // 16 if count is None else count
+        if s.size < count * 45:
+            raise DioramaError("sh holds fewer than 45 floats per Gaussian")
+        params, sh = ctx.upload(p), ctx.upload(s)
+        keep = [params, sh]
+    elif count is None:
+        count = params.nbytes // 64
+    length = ctypes.c_size_t()
+    _check(lib().dio_ply_write(None, None, count, None, 0, ctypes.byref(length)))
+    out = ctypes.create_string_buffer(length.value)
+    _check(lib().dio_ply_write(params.handle, sh.handle, count, out, length.value, ctypes.byref(length)))
+    del keep
+    return out.raw
+
+
+def parse_gaussian_ply(data, ctx, on_device=False):
+    """parseGaussianPly (src/plyLoader.ts:21-98): {"count", "gaussians" (count x 16,
+    activated), "sh_rest" (count x 45), "sh_degree"}; host arrays, or device Buffers
+    (ready for GaussianRenderer.adopt_gaussians) with on_device."""
+    buf = bytes(data)
+    g, s, n, d = _P(), _P(), _U32(), _U32()
+    _check(lib().dio_ply_parse(ctx.handle if ctx else None, buf, len(buf), ctypes.byref(g), ctypes.byref(s),
+                               ctypes.byref(n), ctypes.byref(d)))
+    gb, sb = Buffer(g, True), Buffer(s, True)
+    out = {"count": n.value, "sh_degree": d.value}
+    if on_device:
+        out.update(gaussians=gb, sh_rest=sb)
+    else:
+        out.update(gaussians=gb.download(np.float32, n.value * 16).reshape(-1, 16),
+                   sh_rest=sb.download(np.float32, n.value * 45).reshape(-1, 45))
+        gb.destroy()
+        sb.destroy()
+    return out
+
+
+def mean_neighbor_distance2(ctx, positions):
+    """meanNeighborDistance2 (src/loadDataset.ts:126-149): per point the mean squared
+    distance to its 3 nearest other points, computed on the device."""
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+    n = p.size // 3
+    pts = ctx.upload(p)
+    out = ctx.buffer(max(n, 1) * 4)
+    _check(lib().dio_mean_neighbor_distance2(ctx.handle, pts.handle, n, out.handle))
+    return out.download(np.float32, n)
+
+
+def initial_gaussians(ctx, positions, colors):
+    """initialGaussians (src/loadDataset.ts:87-110): {"params": n x 16, "sh": n x 45}
+    raw Gaussians for structure-from-motion points and their 0-255 colours."""
+    p = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+    c = np.ascontiguousarray(colors, dtype=np.uint8).reshape(-1)
+    n = p.size // 3
+    if c.size < 3 * n:
+        raise DioramaError("colors hold fewer than 3 bytes per point")
+    pts, cols = ctx.upload(p), ctx.upload(c)
+    params, sh = ctx.buffer(max(n, 1) * 64), ctx.buffer(max(n, 1) * 180)
+    _check(lib().dio_initial_gaussians(ctx.handle, pts.handle, cols.handle, n, params.handle, sh.handle))
+    return {"params": params.download(np.float32, n * 16).reshape(n, 16),
+            "sh": sh.download(np.float32, n * 45).reshape(n, 45)}

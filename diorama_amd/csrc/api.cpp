@@ -11,8 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/diorama.h"
-#include "dio_internal.hpp"
+#include "runtime.hpp"
 
 namespace dio {
 void check_hip(hipError_t e, const char* what) {
@@ -21,77 +20,22 @@ void check_hip(hipError_t e, const char* what) {
 }  // namespace dio
 
 using dio::Error;
+using dio::alloc;
+using dio::dev;
+using dio::guard;
+using dio::invalid;
+using dio::need;
+using dio::release;
+using dio::sync;
+
+namespace dio {
+std::string& last_error() {
+  thread_local std::string message;
+  return message;
+}
+}  // namespace dio
 
 namespace {
-thread_local std::string g_error;
-
-template <class F>
-int guard(F&& f) {
-  try {
-    f();
-    return DIO_OK;
-  } catch (const Error& e) {
-    g_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_error = "host allocation failed";
-    return DIO_ERR_MEMORY;
-  } catch (const std::exception& e) {
-    g_error = e.what();
-    return DIO_ERR_INTERNAL;
-  }
-}
-
-void invalid(const std::string& message) { throw Error(DIO_ERR_INVALID, message); }
-}  // namespace
-
-struct dio_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t stamps[DIO_TIMESTAMP_SLOTS] = {};
-  bool stamped[DIO_TIMESTAMP_SLOTS] = {};
-};
-
-struct dio_buf {
-  dio_ctx* ctx = nullptr;
-  void* ptr = nullptr;
-  size_t bytes = 0;
-};
-
-namespace {
-
-dio_buf* alloc(dio_ctx* ctx, size_t bytes, bool zero = false) {
-  auto* b = new dio_buf;
-  b->ctx = ctx;
-  b->bytes = bytes;
-  hipError_t e = hipMalloc(&b->ptr, std::max<size_t>(bytes, 256));
-  if (e != hipSuccess) {
-    delete b;
-    (void)hipGetLastError();
-    throw Error(DIO_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
-  }
-  if (zero) dio::fill_u32(ctx->stream, static_cast<uint32_t*>(b->ptr), 0u, (std::max<size_t>(bytes, 256) + 3) / 4);
-  return b;
-}
-
-void release(dio_buf*& b) {
-  if (b) {
-    (void)hipFree(b->ptr);
-    delete b;
-    b = nullptr;
-  }
-}
-
-template <class T>
-T* dev(dio_buf* b) {
-  return b ? static_cast<T*>(b->ptr) : nullptr;
-}
-
-void need(const void* p, const char* what) {
-  if (!p) invalid(std::string(what) + " is null");
-}
-
-void sync(dio_ctx* ctx) { DIO_HIP(hipStreamSynchronize(ctx->stream)); }
 
 void to_camera(const dio_camera& c, dio::CameraArgs& out) {
   std::memcpy(out.view, c.view, sizeof(out.view));
@@ -109,7 +53,7 @@ uint32_t next_pow2_capacity(uint32_t count) {
 
 // =================================================================== runtime
 
-extern "C" const char* dio_last_error(void) { return g_error.c_str(); }
+extern "C" const char* dio_last_error(void) { return dio::last_error().c_str(); }
 extern "C" const char* dio_version(void) { return "diorama-mi355x 0.1 gfx950"; }
 
 extern "C" int dio_ctx_create(int device, dio_ctx** out) {
@@ -1630,7 +1574,7 @@ void trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t
       }
       create_scene(t->renderer, count, g, s, 0, 0);
       dio_training_loss* l = nullptr;
-      if (dio_training_loss_create(ctx, t->width, t->height, &l) != DIO_OK) throw Error(DIO_ERR_MEMORY, g_error);
+      if (dio_training_loss_create(ctx, t->width, t->height, &l) != DIO_OK) throw Error(DIO_ERR_MEMORY, dio::last_error());
       t->loss = l;
       if (batch) {
         t->batch = new BatchState;
@@ -1725,7 +1669,8 @@ extern "C" int dio_trainer_export_raw(dio_trainer* t, float* params, float* sh) 
     need(t, "trainer");
     need(params, "params");
     need(sh, "sh");
-    DIO_HIP(hipMemcpyAsync(params, t->params->ptr, t->params->bytes, hipMemcpyDeviceToHost, t->ctx->stream));
+    DIO_HIP(hipMemcpyAsync(params, t->params->ptr, static_cast<size_t>(t->count) * dio::kFloatsPerGaussian * 4,
+                           hipMemcpyDeviceToHost, t->ctx->stream));
     DIO_HIP(hipMemcpyAsync(sh, t->renderer->scene->sh->ptr, static_cast<size_t>(t->count) * dio::kShRest * 4,
                            hipMemcpyDeviceToHost, t->ctx->stream));
     sync(t->ctx);
@@ -2090,7 +2035,7 @@ extern "C" int dio_mlp_train(dio_densify_mlp* m, dio_buf* features, dio_buf* lab
     for (uint32_t k = 0; k < steps; k++) {
       dio::fill_u32(s, dev<uint32_t>(m->loss), 0u, 1);
       // a nonzero seed per step: `batch` hashed draws from the samples
-      const uint32_t step_seed = (seed * 2654435761u) ^ (k * 40503u + 1u) | 1u;
+      const uint32_t step_seed = ((seed * 2654435761u) ^ (k * 40503u + 1u)) | 1u;
       dio::mlp_gradients(s, batch, count, step_seed, dev<float>(features), dev<uint32_t>(labels),
                          dev<uint16_t>(m->weights), dev<float>(m->norm), m->class_weights, dev<float>(m->grads),
                          dev<float>(m->loss));

@@ -232,6 +232,37 @@ void mlp_resolve(hipStream_t s, const uint32_t* pending_count, uint32_t pending_
                  const float* stats, uint32_t n, uint32_t store_capacity, float* store_features,
                  uint32_t* store_labels, uint32_t* store_count);
 
+// ------------------------------------------------------------------- ply --
+// The 3DGS .ply body: 62 float columns per vertex (ply.hip).
+constexpr uint32_t kPlyColumns = 62;
+constexpr uint32_t kPlyLdsFloats = 16384;  // rows staged per unpack workgroup x stride stay within 64 KiB
+// Where each field sits in a file's rows of `stride` floats; `per_channel`
+// higher-order coefficients per colour channel (0, 3, 8 or 15).
+struct PlyColumns {
+  uint32_t stride;
+  uint32_t per_channel;
+  uint32_t position[3];
+  uint32_t dc[3];
+  uint32_t opacity;
+  uint32_t scale[3];
+  uint32_t rotation[4];
+  uint32_t rest[kShRest];
+};
+// Raw params (16 floats) + SH (45, coefficient-major) -> count x 62 body floats.
+void ply_pack(hipStream_t s, const float* params, const float* sh, uint32_t count, float* body);
+// count rows of cols.stride floats -> activated Gaussians (16 floats) + SH (45).
+void ply_unpack(hipStream_t s, const float* body, uint32_t count, const PlyColumns& cols, float* gaussians, float* sh);
+uint32_t ply_unpack_rows(uint32_t stride);
+
+// ------------------------------------------------------------- neighbors --
+// Mean squared distance to the 3 nearest other points of n points (n x 3
+// floats), bit-identical to the brute-force scan (knn.hip).
+size_t neighbors_workspace_bytes(uint32_t n);
+void mean_neighbor_distance2(hipStream_t s, const float* pts, uint32_t n, float* out, void* workspace);
+// Starting raw parameters (n x 16) and zero SH (n x 45) from points, rgb bytes and those distances.
+void initial_gaussians(hipStream_t s, const float* pts, const uint8_t* colors, const float* d2, uint32_t n,
+                       float* params, float* sh);
+
 void fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count);
 
 }  // namespace dio

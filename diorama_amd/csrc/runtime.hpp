@@ -1,0 +1,85 @@
+// The host runtime shared by the C-ABI translation units (api.cpp, io.cpp): the
+// structs behind the dio_ctx / dio_buf handles, the conversion of thrown errors
+// into status codes, and device allocation.
+#pragma once
+#include <algorithm>
+#include <new>
+#include <string>
+
+#include "../../include/diorama.h"
+#include "dio_internal.hpp"
+
+struct dio_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t stamps[DIO_TIMESTAMP_SLOTS] = {};
+  bool stamped[DIO_TIMESTAMP_SLOTS] = {};
+};
+
+struct dio_buf {
+  dio_ctx* ctx = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+namespace dio {
+
+// The calling thread's message behind dio_last_error().
+std::string& last_error();
+
+// Runs f, turning what it throws into a status code and the thread's message.
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return DIO_OK;
+  } catch (const Error& e) {
+    last_error() = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    last_error() = "host allocation failed";
+    return DIO_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    last_error() = e.what();
+    return DIO_ERR_INTERNAL;
+  }
+}
+
+[[noreturn]] inline void invalid(const std::string& message) { throw Error(DIO_ERR_INVALID, message); }
+
+inline void need(const void* p, const char* what) {
+  if (!p) invalid(std::string(what) + " is null");
+}
+
+// A device buffer of `bytes` (at least 256 are reserved), zero-filled on the
+// context's stream when asked.
+inline dio_buf* alloc(dio_ctx* ctx, size_t bytes, bool zero = false) {
+  auto* b = new dio_buf;
+  b->ctx = ctx;
+  b->bytes = bytes;
+  hipError_t e = hipMalloc(&b->ptr, std::max<size_t>(bytes, 256));
+  if (e != hipSuccess) {
+    delete b;
+    (void)hipGetLastError();
+    throw Error(DIO_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  if (zero) fill_u32(ctx->stream, static_cast<uint32_t*>(b->ptr), 0u, (std::max<size_t>(bytes, 256) + 3) / 4);
+  return b;
+}
+
+inline void release(dio_buf*& b) {
+  if (b) {
+    (void)hipFree(b->ptr);
+    delete b;
+    b = nullptr;
+  }
+}
+
+template <class T>
+T* dev(dio_buf* b) {
+  return b ? static_cast<T*>(b->ptr) : nullptr;
+}
+
+inline void sync(dio_ctx* ctx) { DIO_HIP(hipStreamSynchronize(ctx->stream)); }
+
+}  // namespace dio

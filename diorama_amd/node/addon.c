@@ -792,6 +792,47 @@ FN(trainer_features) {
 }
 GETTER(trainer_extra_stats, dio_trainer_extra_stats)
 
+FN(mean_neighbor_distance2) {
+  ARGS;
+  if (!ok(env, dio_mean_neighbor_distance2(handle(&c, 0), handle(&c, 1), u32(&c, 2), handle(&c, 3)))) return NULL;
+  return undefined(env);
+}
+FN(initial_gaussians) {
+  ARGS;
+  if (!ok(env, dio_initial_gaussians(handle(&c, 0), handle(&c, 1), handle(&c, 2), u32(&c, 3), handle(&c, 4),
+                                     handle(&c, 5))))
+    return NULL;
+  return undefined(env);
+}
+/* writeGaussianPly: (params, sh, count) -> ArrayBuffer holding the file. */
+FN(ply_write) {
+  ARGS;
+  size_t length = 0;
+  void* data = NULL;
+  napi_value ab;
+  if (!ok(env, dio_ply_write(NULL, NULL, u32(&c, 2), NULL, 0, &length))) return NULL;
+  if (napi_create_arraybuffer(env, length, &data, &ab) != napi_ok) return fail(env, "could not allocate the .ply");
+  if (!ok(env, dio_ply_write(handle(&c, 0), handle(&c, 1), u32(&c, 2), data, length, &length))) return NULL;
+  return ab;
+}
+/* parseGaussianPly: (ctx, ArrayBuffer | TypedArray) -> {gaussians, shRest, count, shDegree} with owned buffers. */
+FN(ply_parse) {
+  ARGS;
+  void* data = NULL;
+  size_t length = 0;
+  dio_buf *g = NULL, *s = NULL;
+  uint32_t n = 0, d = 0;
+  napi_value o;
+  if (!bytes_of(&c, 1, &data, &length)) return fail(env, "parseGaussianPly needs an ArrayBuffer or TypedArray");
+  if (!ok(env, dio_ply_parse(handle(&c, 0), data, length, &g, &s, &n, &d))) return NULL;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "gaussians", wrap(env, g));
+  napi_set_named_property(env, o, "shRest", wrap(env, s));
+  napi_set_named_property(env, o, "count", number(env, n));
+  napi_set_named_property(env, o, "shDegree", number(env, d));
+  return o;
+}
+
 
 /* --------------------------------------------------------------- tuning -- */
 
@@ -867,6 +908,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT(mlp_set_class_weights), EXPORT(mlp_forward), EXPORT(mlp_gradients), EXPORT(mlp_train),
       EXPORT(mlp_destroy), EXPORT(trainer_set_densify_mode), EXPORT(trainer_samples), EXPORT(trainer_features),
       EXPORT(trainer_extra_stats), EXPORT(trainer_get_tuning), EXPORT(trainer_set_tuning), EXPORT(trainer_autotune),
+      EXPORT(ply_write), EXPORT(ply_parse), EXPORT(mean_neighbor_distance2), EXPORT(initial_gaussians),
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

@@ -371,6 +371,10 @@ class Trainer {
   get renderer() {
     return new GaussianRenderer(this.device, null, native.trainer_renderer(this.handle), this);
   }
+  /** writeGaussianPly(await trainer.exportRaw()) without the host round trip. */
+  writePly() {
+    return writeGaussianPly({ params: this.buffers.params, sh: this.renderer.shBuffer }, null, this.count);
+  }
   step() {
     native.trainer_step(this.handle);
     return Promise.resolve();
@@ -440,6 +444,93 @@ class DensifyMLP {
   }
 }
 
+/**
+ * meanNeighborDistance2 (src/loadDataset.ts:126-149): per point (Float32Array of
+ * xyz) the mean squared distance to its 3 nearest other points.
+ */
+function meanNeighborDistance2(device, positions) {
+  const n = positions.length / 3;
+  const points = device.upload(positions);
+  const out = device.createBuffer(Math.max(n, 1) * 4);
+  try {
+    native.mean_neighbor_distance2(device.handle, points.handle, n, out.handle);
+    return Promise.resolve(new Float32Array(out.read(0, n * 4)));
+  } finally {
+    points.destroy();
+    out.destroy();
+  }
+}
+
+/**
+ * initialGaussians (src/loadDataset.ts:87-110): RawGaussians {params, sh} for
+ * structure-from-motion points (Float32Array xyz) and colours (Uint8Array rgb).
+ */
+function initialGaussians(device, positions, colors) {
+  const n = positions.length / 3;
+  const points = device.upload(positions);
+  const rgb = device.upload(colors);
+  const params = device.createBuffer(Math.max(n, 1) * FLOATS_PER_GAUSSIAN * 4);
+  const sh = device.createBuffer(Math.max(n, 1) * SH_REST_FLOATS * 4);
+  try {
+    native.initial_gaussians(device.handle, points.handle, rgb.handle, n, params.handle, sh.handle);
+    return Promise.resolve({
+      params: new Float32Array(params.read(0, n * FLOATS_PER_GAUSSIAN * 4)),
+      sh: new Float32Array(sh.read(0, n * SH_REST_FLOATS * 4)),
+    });
+  } finally {
+    for (const b of [points, rgb, params, sh]) b.destroy();
+  }
+}
+
+/**
+ * writeGaussianPly (src/plyWriter.ts:14-43). `raw` is {params, sh}: Float32Arrays
+ * (uploaded on `device`) or device Buffers (then `count` Gaussians; the body is
+ * packed where they live). Returns a Blob-like {size, type, arrayBuffer()}, as the
+ * reference's Blob is read with `await blob.arrayBuffer()`.
+ */
+function writeGaussianPly(raw, device, count) {
+  let params = raw.params;
+  let sh = raw.sh;
+  const staged = [];
+  let n = count;
+  if (!(params instanceof Buffer)) {
+    if (!device) throw new Error('writeGaussianPly needs a Device for host arrays');
+    n = n === undefined ? params.length / FLOATS_PER_GAUSSIAN : n;
+    params = device.upload(raw.params);
+    sh = device.upload(raw.sh);
+    staged.push(params, sh);
+  } else if (n === undefined) {
+    n = params.size / (FLOATS_PER_GAUSSIAN * 4);
+  }
+  try {
+    const file = native.ply_write(params.handle, sh.handle, n);
+    return { size: file.byteLength, type: 'application/octet-stream', arrayBuffer: () => Promise.resolve(file) };
+  } finally {
+    for (const b of staged) b.destroy();
+  }
+}
+
+/**
+ * parseGaussianPly (src/plyLoader.ts:21-98): {count, gaussians, shRest, shDegree}
+ * with Float32Arrays, as the reference returns; with {onDevice: true} the two
+ * arrays stay device Buffers, ready for GaussianRenderer.adoptGaussians.
+ */
+function parseGaussianPly(buffer, device, options) {
+  const o = native.ply_parse(device.handle, buffer);
+  const gaussians = new Buffer(o.gaussians, true);
+  const shRest = new Buffer(o.shRest, true);
+  if (options && options.onDevice) return { count: o.count, gaussians, shRest, shDegree: o.shDegree };
+  const out = {
+    count: o.count,
+    gaussians: new Float32Array(gaussians.read(0, o.count * FLOATS_PER_GAUSSIAN * 4)),
+    shRest: new Float32Array(shRest.read(0, o.count * SH_REST_FLOATS * 4)),
+    shDegree: o.shDegree,
+  };
+  gaussians.destroy();
+  shRest.destroy();
+  return out;
+}
+
 /** Raw parameters to the renderer's activated layout (src/trainer.ts:449-461). */
 function activate(params) {
   const out = new Float32Array(params.length);
@@ -468,6 +559,10 @@ module.exports = {
   Trainer,
   DensifyMLP,
   DEFAULT_SETTINGS,
+  writeGaussianPly,
+  parseGaussianPly,
+  meanNeighborDistance2,
+  initialGaussians,
   activate,
   FLOATS_PER_GAUSSIAN,
   FLOATS_PER_SPLAT,

@@ -315,6 +315,40 @@ dio_buf* dio_trainer_extra_stats(dio_trainer* t);
  * step's inputs re-run `repeats` times. */
 int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_t repeats, double* ms);
 
+/* ---------------------------------------------------- starting Gaussians --
+ * meanNeighborDistance2 (src/loadDataset.ts:126-149, nearestNeighbors.wgsl:12-44):
+ * for each of `count` points (positions: count x 3 floats on the device) the
+ * mean squared distance to its 3 nearest other points, into distance2 (count
+ * floats). Bit-identical to the reference's brute-force scan with the distance
+ * fma(dz, dz, fma(dy, dy, dx * dx)); non-finite points get +inf. */
+int dio_mean_neighbor_distance2(dio_ctx* ctx, const dio_buf* positions, uint32_t count, dio_buf* distance2);
+/* initialGaussians (src/loadDataset.ts:87-110): raw params (count x 16) and zero
+ * SH (count x 45) for structure-from-motion points (positions count x 3 floats,
+ * colors count x 3 bytes): the point's position and colour, opacity 0.1, no
+ * rotation, log scale 0.5 log(max(mean 3-nearest squared distance, 1e-7)). */
+int dio_initial_gaussians(dio_ctx* ctx, const dio_buf* positions, const dio_buf* colors, uint32_t count,
+                          dio_buf* params, dio_buf* sh);
+
+/* ----------------------------------------------------------- .ply files --
+ * The 3DGS .ply format of trained Gaussians; the body is packed and unpacked on
+ * the device.
+ *
+ * writeGaussianPly (src/plyWriter.ts:14-43): the file for `count` raw Gaussians
+ * (params: count x 16 floats as the trainer keeps them; sh: count x 45,
+ * coefficient-major) written into `out` (host memory of `capacity` bytes): the
+ * text header, then count x 62 little-endian floats, packed on the stream of
+ * params' context. *length receives the file's size; with out null nothing else
+ * happens, so a caller can size `out` first. */
+int dio_ply_write(const dio_buf* params, const dio_buf* sh, uint32_t count, void* out, size_t capacity, size_t* length);
+/* parseGaussianPly (src/plyLoader.ts:21-98): a binary little-endian .ply of float
+ * vertex properties in any order, SH degree 0-3, into two new device buffers the
+ * caller owns (dio_buf_destroy, or hand them to dio_renderer_adopt_gaussians):
+ * *gaussians count x 16 activated floats (exp scales, sigmoid opacity, 0.5 +
+ * SH_C0 dc), *sh_rest count x 45 coefficient-major, zero past the file's degree.
+ * Malformed files fail with DIO_ERR_INVALID and the loader's messages. */
+int dio_ply_parse(dio_ctx* ctx, const void* file, size_t bytes, dio_buf** gaussians, dio_buf** sh_rest,
+                  uint32_t* count, uint32_t* sh_degree);
+
 #ifdef __cplusplus
 }
 #endif

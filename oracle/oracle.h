@@ -78,6 +78,10 @@ double oc_train_step(long n, double* params, double* sh, double* moments, double
                      int width, int height, const double* background, const uint8_t* target,
                      const double* lr, int adam, long* pair_count);
 
+/* Mean squared distance to the 3 nearest other points of pts (n x 3 float32),
+ * for queries[0..m) (null: points 0..m-1), into out[m]; float32 arithmetic. */
+void oc_mean_neighbor_distance2(long n, const float* pts, long m, const long* queries, float* out);
+
 int oc_threads(void);
 
 #ifdef __cplusplus

@@ -55,6 +55,8 @@ def lib():
                                     c_int, ctypes.POINTER(c_long)]
         L.oc_train_step.restype = ctypes.c_double
         L.oc_threads.restype = c_int
+        _f32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.oc_mean_neighbor_distance2.argtypes = [c_long, _f32, c_long, vp, _f32]
         _LIB = L
     return _LIB
 
@@ -233,3 +235,35 @@ def train_step(params, sh, moments, stats, gaussians, degree, camera, background
 
 def threads():
     return lib().oc_threads()
+
+
+def mean_neighbor_distance2(points, queries=None):
+    """Brute-force float32 mean squared distance to the 3 nearest other points
+    (src/shaders/nearestNeighbors.wgsl:12-44), for every point or for `queries`."""
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1)
+    n = pts.size // 3
+    if queries is None:
+        out = np.empty(n, np.float32)
+        lib().oc_mean_neighbor_distance2(n, pts, n, None, out)
+        return out
+    q = np.ascontiguousarray(queries, dtype=np.int64)
+    out = np.empty(q.size, np.float32)
+    lib().oc_mean_neighbor_distance2(n, pts, q.size, q.ctypes.data_as(ctypes.c_void_p), out)
+    return out
+
+
+def initial_gaussians(points, colors, distance2):
+    """src/loadDataset.ts:93-110: raw params (n x 16 float32) and zero SH (n x 45),
+    the arithmetic in float64 rounded once."""
+    p = np.asarray(points, np.float32).reshape(-1, 3)
+    c = np.asarray(colors, np.uint8).reshape(-1, 3)
+    d2 = np.asarray(distance2, np.float32).astype(np.float64)
+    n = len(p)
+    params = np.zeros((n, 16), np.float32)
+    params[:, 0:3] = p
+    params[:, 3] = np.float32(np.log(0.1 / 0.9))
+    with np.errstate(over="ignore", invalid="ignore"):
+        params[:, 4:7] = (0.5 * np.log(np.maximum(d2, 1e-7)))[:, None].astype(np.float32)
+    params[:, 8] = 1.0
+    params[:, 12:15] = ((c / 255.0 - 0.5) / 0.28209479177387814).astype(np.float32)
+    return params, np.zeros((n, 45), np.float32)

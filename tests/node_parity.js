@@ -54,6 +54,11 @@ function cpuChecks() {
   } catch (e) {
     errors.push(e.message);
   }
+  try {
+    d.parseGaussianPly(new ArrayBuffer(8), { handle: null });
+  } catch (e) {
+    errors.push(e.message);
+  }
   out.errors = errors;
   const raw = new Float32Array(16);
   raw[3] = 0;
@@ -232,7 +237,22 @@ async function gpuChecks() {
   const trainer = new d.Trainer(device, raw, views, Object.assign({}, d.DEFAULT_SETTINGS, { extent: 1 }), { batch: 2 });
   await trainer.step();
   out.batch = { size: trainer.batchSize, loss: await trainer.lastLoss(), iteration: trainer.iteration };
+
+  // The .ply round trip of src/checks/training.ts:122-136 on the trained Gaussians.
+  const file = await trainer.writePly().arrayBuffer();
+  const scene = d.parseGaussianPly(file, device);
+  const back = await trainer.exportRaw();
+  const expected = d.activate(back.params);
+  const worst = (a, b) => a.reduce((most, value, k) => Math.max(most, Math.abs(value - b[k])), 0);
+  out.ply = { bytes: file.byteLength, count: scene.count, degree: scene.shDegree,
+    paramsError: worst(scene.gaussians, expected), shError: worst(scene.shRest, back.sh) };
   trainer.destroy();
+
+  // Starting Gaussians from points: a line 0, 1, 3, 7 (src/loadDataset.ts:87-149).
+  const line = new Float32Array([0, 0, 0, 1, 0, 0, 3, 0, 0, 7, 0, 0]);
+  out.neighbors = Array.from(await d.meanNeighborDistance2(device, line));
+  const start = await d.initialGaussians(device, line, new Uint8Array([255, 0, 128, 0, 0, 0, 0, 0, 0, 0, 0, 0]));
+  out.initial = Array.from(start.params.subarray(0, 16));
   device.destroy();
   return out;
 }

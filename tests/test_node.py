@@ -7,6 +7,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -34,8 +35,9 @@ def test_node_module_loads_without_gpu():
             "percentDense": 0.01, "minOpacity": 0.005, "opacityResetInterval": 3000, "background": [0, 0, 0]}
     assert out["defaults"] == want
     # Both error paths throw JS errors carrying the library's message.
-    assert len(out["errors"]) == 2
+    assert len(out["errors"]) == 3
     assert "ctx is null" in out["errors"][1]
+    assert out["errors"][2] == "No .ply header found."
     a = out["activated"]
     assert a[3] == 0.5 and a[4] == pytest.approx(2.0, rel=1e-6) and a[5] == 1.0
     assert a[12] == pytest.approx(0.5 + 0.28209479177387814, rel=1e-6)
@@ -55,3 +57,13 @@ def test_node_parity_on_gpu():
     assert max(r["gradientErrors"].values()) < 1e-3 and not any(math.isnan(v) for v in r["gradientErrors"].values())
     assert out["mlp"]["logits"] == [0.5, -1.25, 2.0, 3.75] * 5 and out["mlp"]["weightsBack"] == 3.75
     assert out["batch"]["size"] == 2 and out["batch"]["iteration"] == 1 and math.isfinite(out["batch"]["loss"])
+    p = out["ply"]
+    assert p["count"] == 64 and p["degree"] == 3 and p["bytes"] == 1527 + 64 * 248  # header of "element vertex 64"
+    assert p["paramsError"] < 1e-6 and p["shError"] == 0
+    f = np.float32
+    assert out["neighbors"] == [float(f(59) / f(3)), float(f(41) / f(3)), float(f(29) / f(3)), float(f(101) / f(3))]
+    g = out["initial"]
+    assert g[0:4] == [0, 0, 0, float(f(math.log(0.1 / 0.9)))] and g[8:12] == [1, 0, 0, 0]
+    assert g[4] == g[5] == g[6] == float(f(0.5 * math.log(float(f(59) / f(3)))))
+    assert g[12:15] == [float(f(0.5 / 0.28209479177387814)), float(f(-0.5 / 0.28209479177387814)),
+                        float(f((128 / 255 - 0.5) / 0.28209479177387814))]
