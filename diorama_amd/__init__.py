@@ -172,6 +172,8 @@ SIGNATURES = {
     "dio_trainer_renderer": (_P, [_P]),
     "dio_trainer_destroy": (None, [_P]),
     "dio_trainer_time_stage": (_I, [_P, ctypes.c_char_p, _U32, ctypes.POINTER(ctypes.c_double)]),
+    "dio_trainer_measure_psnr": (_I, [_P, ctypes.POINTER(_Camera), ctypes.POINTER(_P), _U32,
+                                     ctypes.POINTER(ctypes.c_double)]),
     "dio_mean_neighbor_distance2": (_I, [_P, _P, _U32, _P]),
     "dio_initial_gaussians": (_I, [_P, _P, _P, _U32, _P, _P]),
     "dio_ply_write": (_I, [_P, _P, _U32, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
@@ -715,6 +717,15 @@ class Trainer:
         _check(lib().dio_trainer_evaluate(self.handle, cams, ptrs, len(cameras), 1 if with_ssim else 0,
                                           ctypes.byref(p), ctypes.byref(s)))
         return {"psnr": p.value, "ssim": s.value}
+
+    def measure_psnr(self, cameras, images):
+        """measurePsnr (src/insideView.ts:233-267): the mean PSNR of views whose photos
+        are device Buffers (rgba8)."""
+        cams = (_Camera * len(cameras))(*[_camera(c) for c in cameras])
+        ptrs = (_P * len(images))(*[b.handle for b in images])
+        p = ctypes.c_double()
+        _check(lib().dio_trainer_measure_psnr(self.handle, cams, ptrs, len(cameras), ctypes.byref(p)))
+        return p.value
 
     def time_stage(self, stage, repeats=10):
         v = ctypes.c_double()

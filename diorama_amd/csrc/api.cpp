@@ -848,53 +848,6 @@ struct Mulberry32 {
   }
 };
 
-// CPU image metrics on 4-value pixels (src/metrics.ts:5-72).
-double psnr4(const std::vector<double>& a, const std::vector<double>& b) {
-  double se = 0;
-  for (size_t i = 0; i < a.size(); i += 4)
-    for (int c = 0; c < 3; c++) {
-      double d = a[i + c] - b[i + c];
-      se += d * d;
-    }
-  return 10.0 * std::log10(1.0 / (se / (3.0 * a.size() / 4)));
-}
-
-double ssim4(const std::vector<double>& a, const std::vector<double>& b, int w, int h) {
-  double window[11], sum = 0;
-  for (int k = 0; k < 11; k++) sum += window[k] = std::exp(-((k - 5) * (k - 5)) / (2 * 1.5 * 1.5));
-  for (double& v : window) v /= sum;
-  const size_t n = static_cast<size_t>(w) * h;
-  std::vector<double> maps[5], scratch(n);
-  for (auto& m : maps) m.resize(n);
-  auto blur = [&](const std::vector<double>& src, std::vector<double>& dst, int dx, int dy) {
-    for (int y = 0; y < h; y++)
-      for (int x = 0; x < w; x++) {
-        double s = 0;
-        for (int k = -5; k <= 5; k++) {
-          int sx = x + k * dx, sy = y + k * dy;
-          if (sx >= 0 && sx < w && sy >= 0 && sy < h) s += window[k + 5] * src[static_cast<size_t>(sy) * w + sx];
-        }
-        dst[static_cast<size_t>(y) * w + x] = s;
-      }
-  };
-  double total = 0;
-  for (int c = 0; c < 3; c++) {
-    for (size_t p = 0; p < n; p++) {
-      double x = a[p * 4 + c], y = b[p * 4 + c];
-      maps[0][p] = x; maps[1][p] = y; maps[2][p] = x * x; maps[3][p] = y * y; maps[4][p] = x * y;
-    }
-    for (auto& m : maps) {
-      blur(m, scratch, 1, 0);
-      blur(scratch, m, 0, 1);
-    }
-    for (size_t p = 0; p < n; p++) {
-      double mx = maps[0][p], my = maps[1][p];
-      double vx = maps[2][p] - mx * mx, vy = maps[3][p] - my * my, cxy = maps[4][p] - mx * my;
-      total += ((2 * mx * my + 1e-4) * (2 * cxy + 9e-4)) / ((mx * mx + my * my + 1e-4) * (vx + vy + 9e-4));
-    }
-  }
-  return total / (3.0 * n);
-}
 
 }  // namespace
 
@@ -1698,33 +1651,107 @@ extern "C" int dio_trainer_last_loss(dio_trainer* t, double* out) {
   });
 }
 
+namespace {
+
+// Renders each view and scores it on the device (metrics.hip); one download of
+// every tile's sums at the end. Photos come from the host (uploaded through one
+// staging buffer) or are already device buffers.
+void trainer_score(dio_trainer* t, const dio_camera* cameras, const uint32_t* const* host_images,
+                   const dio_buf* const* device_images, uint32_t view_count, bool with_ssim, double* psnr,
+                   double* ssim) {
+  need(cameras, "cameras");
+  if (!host_images && !device_images) invalid("images is null");
+  if (view_count == 0) invalid("no views to evaluate");
+  std::vector<uint32_t> first(view_count + 1, 0);
+  size_t largest = 0;
+  for (uint32_t v = 0; v < view_count; v++) {
+    const uint32_t w = static_cast<uint32_t>(cameras[v].viewport[0]), h = static_cast<uint32_t>(cameras[v].viewport[1]);
+    if (w == 0 || h == 0) invalid("a view has an empty viewport");
+    first[v + 1] = first[v] + dio::evaluate_tiles(w, h);
+    largest = std::max<size_t>(largest, static_cast<size_t>(w) * h);
+    if (device_images) {
+      need(device_images[v], "image");
+      if (device_images[v]->bytes < static_cast<size_t>(w) * h * 4) invalid("an image is smaller than its viewport");
+    } else {
+      need(host_images[v], "image");
+    }
+  }
+  const uint32_t tiles = first[view_count];
+  dio_buf* staging = nullptr;
+  dio_buf* sse = nullptr;
+  dio_buf* structure = nullptr;
+  std::vector<uint32_t> squared(tiles);
+  std::vector<double> sums(with_ssim ? tiles : 0);
+  try {
+    if (!device_images) staging = alloc(t->ctx, largest * 4);
+    sse = alloc(t->ctx, static_cast<size_t>(tiles) * 4);
+    structure = alloc(t->ctx, with_ssim ? static_cast<size_t>(tiles) * 8 : 0);
+    for (uint32_t v = 0; v < view_count; v++) {
+      const uint32_t w = static_cast<uint32_t>(cameras[v].viewport[0]),
+                     h = static_cast<uint32_t>(cameras[v].viewport[1]);
+      const uint32_t* photo = nullptr;
+      if (device_images) {
+        photo = static_cast<const uint32_t*>(device_images[v]->ptr);
+      } else {
+        DIO_HIP(hipMemcpyAsync(staging->ptr, host_images[v], static_cast<size_t>(w) * h * 4, hipMemcpyHostToDevice,
+                               t->ctx->stream));
+        photo = static_cast<const uint32_t*>(staging->ptr);
+      }
+      renderer_preprocess(t->renderer, cameras[v]);
+      renderer_tiles(t->renderer, nullptr, t->settings.background);
+      dio::evaluate_image(t->ctx->stream, static_cast<const float4*>(t->renderer->scene->rasterizer->pixel_color->ptr),
+                          photo, w, h, with_ssim, dev<uint32_t>(sse) + first[v],
+                          with_ssim ? dev<double>(structure) + first[v] : nullptr);
+    }
+    DIO_HIP(hipMemcpyAsync(squared.data(), sse->ptr, squared.size() * 4, hipMemcpyDeviceToHost, t->ctx->stream));
+    if (with_ssim)
+      DIO_HIP(hipMemcpyAsync(sums.data(), structure->ptr, sums.size() * 8, hipMemcpyDeviceToHost, t->ctx->stream));
+    sync(t->ctx);
+  } catch (...) {
+    release(staging);
+    release(sse);
+    release(structure);
+    throw;
+  }
+  release(staging);
+  release(sse);
+  release(structure);
+  // src/metrics.ts:5-15 per view: 10 log10(1 / (SSE / 3 pixels)) with SSE over
+  // 0-1 values, here the exact 8-bit sum / 255^2; the views' mean.
+  double psnr_total = 0, ssim_total = 0;
+  for (uint32_t v = 0; v < view_count; v++) {
+    const double pixels = cameras[v].viewport[0] * static_cast<double>(cameras[v].viewport[1]);
+    uint64_t se = 0;
+    double st = 0;
+    for (uint32_t k = first[v]; k < first[v + 1]; k++) {
+      se += squared[k];
+      if (with_ssim) st += sums[k];
+    }
+    psnr_total += 10.0 * std::log10(3.0 * pixels * 65025.0 / static_cast<double>(se));
+    ssim_total += st / (3.0 * pixels);
+  }
+  if (psnr) *psnr = psnr_total / view_count;
+  if (ssim) *ssim = with_ssim ? ssim_total / view_count : NAN;
+}
+
+}  // namespace
+
 extern "C" int dio_trainer_evaluate(dio_trainer* t, const dio_camera* cameras, const uint32_t* const* images,
                                     uint32_t view_count, int with_ssim, double* psnr, double* ssim) {
   return guard([&] {
     need(t, "trainer");
-    need(cameras, "cameras");
     need(images, "images");
-    double psnr_total = 0, ssim_total = 0;
-    for (uint32_t v = 0; v < view_count; v++) {
-      const uint32_t w = static_cast<uint32_t>(cameras[v].viewport[0]), h = static_cast<uint32_t>(cameras[v].viewport[1]);
-      renderer_preprocess(t->renderer, cameras[v]);
-      renderer_tiles(t->renderer, nullptr, t->settings.background);
-      const size_t n = static_cast<size_t>(w) * h;
-      std::vector<float> pixels(n * 4);
-      DIO_HIP(hipMemcpyAsync(pixels.data(), t->renderer->scene->rasterizer->pixel_color->ptr, n * 16,
-                             hipMemcpyDeviceToHost, t->ctx->stream));
-      sync(t->ctx);
-      std::vector<double> rendered(n * 4), photo(n * 4);
-      for (size_t i = 0; i < n * 4; i++) {
-        double x = std::min(1.0, std::max(0.0, static_cast<double>(pixels[i])));
-        rendered[i] = std::round(x * 255) / 255;
-        photo[i] = ((images[v][i / 4] >> (8 * (i % 4))) & 0xffu) / 255.0;
-      }
-      psnr_total += psnr4(rendered, photo);
-      if (with_ssim) ssim_total += ssim4(rendered, photo, static_cast<int>(w), static_cast<int>(h));
-    }
-    if (psnr) *psnr = psnr_total / view_count;
-    if (ssim) *ssim = with_ssim ? ssim_total / view_count : NAN;
+    trainer_score(t, cameras, images, nullptr, view_count, with_ssim != 0, psnr, ssim);
+  });
+}
+
+extern "C" int dio_trainer_measure_psnr(dio_trainer* t, const dio_camera* cameras, const dio_buf* const* images,
+                                        uint32_t view_count, double* psnr) {
+  return guard([&] {
+    need(t, "trainer");
+    need(images, "images");
+    need(psnr, "psnr");
+    trainer_score(t, cameras, nullptr, images, view_count, false, psnr, nullptr);
   });
 }
 

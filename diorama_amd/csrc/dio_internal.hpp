@@ -263,6 +263,13 @@ void mean_neighbor_distance2(hipStream_t s, const float* pts, uint32_t n, float*
 void initial_gaussians(hipStream_t s, const float* pts, const uint8_t* colors, const float* d2, uint32_t n,
                        float* params, float* sh);
 
+// --------------------------------------------------------------- metrics --
+// Per 16x16 tile of a render against an rgba8 photo (metrics.hip): the exact sum
+// of squared 8-bit errors over rgb and, with_ssim, the sum of per-pixel SSIM.
+uint32_t evaluate_tiles(uint32_t width, uint32_t height);
+void evaluate_image(hipStream_t s, const float4* rendered, const uint32_t* target, uint32_t width, uint32_t height,
+                    bool with_ssim, uint32_t* sse, double* ssim);
+
 void fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count);
 
 }  // namespace dio

@@ -630,6 +630,36 @@ FN(trainer_evaluate) {
   free((void*)images);
   return result;
 }
+/* measurePsnr: (trainer, [camera], [image buffer handle]) -> mean PSNR */
+FN(trainer_measure_psnr) {
+  ARGS;
+  uint32_t views = 0, handles = 0;
+  napi_get_array_length(env, c.argv[1], &views);
+  napi_get_array_length(env, c.argv[2], &handles);
+  if (views != handles) return fail(env, "one image per camera");
+  dio_camera* cams = (dio_camera*)calloc(views ? views : 1, sizeof(dio_camera));
+  const dio_buf** images = (const dio_buf**)calloc(views ? views : 1, sizeof(dio_buf*));
+  napi_value result = NULL;
+  int good = 1;
+  for (uint32_t k = 0; k < views && good; k++) {
+    napi_value v;
+    napi_get_element(env, c.argv[1], k, &v);
+    good = camera_of(&c, v, &cams[k]);
+    napi_get_element(env, c.argv[2], k, &v);
+    void* h = NULL;
+    if (good) good = napi_get_value_external(env, v, &h) == napi_ok;
+    images[k] = (const dio_buf*)h;
+  }
+  double psnr = 0;
+  if (!good) {
+    fail(env, "views must be [{camera, image: Buffer}]");
+  } else if (ok(env, dio_trainer_measure_psnr(handle(&c, 0), cams, images, views, &psnr))) {
+    result = number(env, psnr);
+  }
+  free(cams);
+  free((void*)images);
+  return result;
+}
 FN(trainer_count) {
   ARGS;
   return number(env, dio_trainer_count(handle(&c, 0)));
@@ -909,6 +939,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT(mlp_destroy), EXPORT(trainer_set_densify_mode), EXPORT(trainer_samples), EXPORT(trainer_features),
       EXPORT(trainer_extra_stats), EXPORT(trainer_get_tuning), EXPORT(trainer_set_tuning), EXPORT(trainer_autotune),
       EXPORT(ply_write), EXPORT(ply_parse), EXPORT(mean_neighbor_distance2), EXPORT(initial_gaussians),
+      EXPORT(trainer_measure_psnr),
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

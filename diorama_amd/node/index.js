@@ -402,6 +402,11 @@ class Trainer {
   evaluate(views, withSsim) {
     return Promise.resolve(native.trainer_evaluate(this.handle, views, withSsim === undefined ? true : withSsim));
   }
+  /** measurePsnr (src/insideView.ts:233-267): views [{camera, image}] whose images are device Buffers. */
+  measurePsnr(views) {
+    return Promise.resolve(native.trainer_measure_psnr(this.handle, views.map((v) => v.camera),
+      views.map((v) => v.image.handle)));
+  }
   destroy() {
     if (this.handle) native.trainer_destroy(this.handle);
     this.handle = null;

@@ -228,9 +228,16 @@ int dio_trainer_densify(dio_trainer* t, int prune_large);
 int dio_trainer_reset_opacity(dio_trainer* t);
 int dio_trainer_export_raw(dio_trainer* t, float* params, float* sh); /* count x 16, count x 45 */
 int dio_trainer_last_loss(dio_trainer* t, double* out);
-/* evaluate (:371-386): mean PSNR/SSIM of renders rounded to 8 bits. */
+/* evaluate (:371-386): mean PSNR/SSIM of renders clamped and rounded to 8 bits
+ * against rgba8 photos in host memory, scored on the device (src/metrics.ts:5-72);
+ * ssim is NaN without with_ssim. */
 int dio_trainer_evaluate(dio_trainer* t, const dio_camera* cameras, const uint32_t* const* images, uint32_t view_count,
                          int with_ssim, double* psnr, double* ssim);
+/* measurePsnr (src/insideView.ts:233-267): the mean PSNR of `view_count` views
+ * whose photos are already device buffers (rgba8, at least width x height x 4
+ * bytes), for training curves without host transfers. */
+int dio_trainer_measure_psnr(dio_trainer* t, const dio_camera* cameras, const dio_buf* const* images,
+                             uint32_t view_count, double* psnr);
 uint32_t dio_trainer_count(const dio_trainer* t);
 uint32_t dio_trainer_iteration(const dio_trainer* t);
 void dio_trainer_set_iteration(dio_trainer* t, uint32_t iteration);
