@@ -57,11 +57,14 @@ void scan_exclusive_gather(hipStream_t s, const uint32_t* in, const uint32_t* id
 // [begin_bit, end_bit); the passes ping-pong between (keys, values) and the
 // alternate buffers. Returns true when the result ended in the alternate buffers.
 constexpr uint32_t kSortTile = 4096;  // 256 threads x 16 keys
+constexpr uint32_t kSortGroup = 16;  // tiles per look-back group
 struct SortWorkspace {
   uint32_t* hist = nullptr;                 // per pass: 256 digit counts, then run starts
   uint32_t* state[2] = {nullptr, nullptr};  // per tile x digit look-back words, alternating passes
+  uint32_t* group_sum[2] = {nullptr, nullptr};     // per group x digit: tiles done << 24 | their count
+  uint32_t* group_prefix[2] = {nullptr, nullptr};  // per group x digit: inclusive prefix once known
   uint32_t* tickets = nullptr;              // per pass tile counter
-  uint32_t tiles = 0;
+  uint32_t tiles = 0, groups = 0;
 };
 size_t sort_workspace_bytes(uint32_t capacity);
 SortWorkspace sort_workspace(void* base, uint32_t capacity);

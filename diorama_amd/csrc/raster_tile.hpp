@@ -22,26 +22,41 @@ __device__ inline float lane_value(float v, int lane) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
-// One splat of a chunk, staged in a lane's registers.
+// One splat of a chunk, staged in a lane's registers. The conic (a, b, c) is
+// kept as the coefficients of the Gaussian's exponent in log2 units,
+// ha = -a/2 log2(e), hb = -b log2(e), hc = -c/2 log2(e), so G = exp2(power2).
 struct Staged {
-  float mx, my, ca, cb, cc, op, r, g, b;
+  float mx, my, ha, hb, hc, op, r, g, b;
   float x0, x1, y0, y1;  // box holding every pixel centre where opacity * G >= 1/255
   uint32_t index;
 };
 
+constexpr float kLog2e = 1.4426950408889634f;
+
+// The exponent of G at (dx, dy) in log2 units: the forward pass and the backward
+// replay both use exactly this, so they agree on every alpha.
+__device__ inline float splat_power2(float dx, float dy, float ha, float hb, float hc) {
+  return __builtin_fmaf(dx, __builtin_fmaf(ha, dx, hb * dy), (hc * dy) * dy);
+}
+
+// Stages pair k's splat; `conic` (optional) receives its raw (a, b, c).
 __device__ inline void stage(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, uint32_t k,
-                             Staged& s) {
+                             Staged& s, float4* conic = nullptr) {
   s.index = pair_splats[k];
   const float4 a = splats[s.index * 3], b = splats[s.index * 3 + 1], c = splats[s.index * 3 + 2];
   s.mx = a.x; s.my = a.y;
-  s.ca = b.x; s.cb = b.y; s.cc = b.z;
+  const float ca = b.x, cb = b.y, cc = b.z;
+  if (conic) *conic = make_float4(ca, cb, cc, 0.f);
+  s.ha = -0.5f * kLog2e * ca;
+  s.hb = -kLog2e * cb;
+  s.hc = -0.5f * kLog2e * cc;
   s.r = c.x; s.g = c.y; s.b = c.z; s.op = c.w;
   // opacity * exp(-q) >= 1/255 needs q <= t = ln(255 opacity), with
   // q = (a dx² + c dy²)/2 + b dx dy. That ellipse's half-widths are sqrt(2t Σxx)
   // and sqrt(2t Σyy), Σ = conic⁻¹. Widened by 1% and 0.01 px against rounding.
-  const float det = s.ca * s.cc - s.cb * s.cb;
+  const float det = ca * cc - cb * cb;
   const float t = __logf(255.f * s.op) * 1.01f + 0.01f;
-  float ex = sqrtf(2.f * t * s.cc / det) + 0.01f, ey = sqrtf(2.f * t * s.ca / det) + 0.01f;
+  float ex = sqrtf(2.f * t * cc / det) + 0.01f, ey = sqrtf(2.f * t * ca / det) + 0.01f;
   ex = !(det > 0.f) ? INFINITY : (!(t > 0.f) ? -INFINITY : ex);  // degenerate: never cull; too faint: always
   ey = !(det > 0.f) ? INFINITY : (!(t > 0.f) ? -INFINITY : ey);
   s.x0 = s.mx - ex;
@@ -97,14 +112,13 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
         const int j = __builtin_ctzll(mask);
         mask &= mask - 1;
         const float mx = lane_value(s.mx, j), my = lane_value(s.my, j);
-        const float ca = lane_value(s.ca, j), cbb = lane_value(s.cb, j), cc = lane_value(s.cc, j);
+        const float ha = lane_value(s.ha, j), hb = lane_value(s.hb, j), hc = lane_value(s.hc, j);
         const float op = lane_value(s.op, j);
         const float sr = lane_value(s.r, j), sg = lane_value(s.g, j), sb = lane_value(s.b, j);
         if (!done) {
-          const float dx = cx - mx, dy = cy - my;
-          const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
-          if (power <= 0.f) {
-            const float alpha = fminf(0.99f, op * __expf(power));
+          const float power2 = splat_power2(cx - mx, cy - my, ha, hb, hc);
+          if (power2 <= 0.f) {
+            const float alpha = fminf(0.99f, op * __builtin_amdgcn_exp2f(power2));
             if (alpha >= 1.f / 255.f) {
               const float remaining = T * (1.f - alpha);
               if (remaining < 0.0001f) {
@@ -150,17 +164,28 @@ __device__ inline float row_total(float v) {
   return v;
 }
 
-// Four splats' per-lane values (a, b, c, d) to four wave totals at once: a
-// butterfly across lane halves keeps a|c, b|d by half, one across 16-lane rows
-// keeps a|b|c|d by row, so one row reduction finishes all four: row r of the
-// wave ends with splat r's total.
+// The sum of the two results of a permlane swap. hipcc (ROCm 7.2) folds a float
+// add of the swap's two results into twice the first; an empty asm barrier
+// between them keeps them distinct (it emits no instruction).
+__device__ inline float swap_sum(decltype(__builtin_amdgcn_permlane32_swap(0u, 0u, false, false)) r) {
+  uint32_t r0 = r[0], r1 = r[1];
+  __asm__ volatile("" : "+v"(r0), "+v"(r1));
+  return __builtin_bit_cast(float, r0) + __builtin_bit_cast(float, r1);
+}
+
+// Four splats' per-lane values (a, b, c, d) to four wave totals at once.
+// v_permlane32_swap(x, y) returns (x.lo y.lo, x.hi y.hi) by 32-lane halves and
+// v_permlane16_swap(x, y) (x.r0 y.r0 x.r2 y.r2, x.r1 y.r1 x.r3 y.r3) by 16-lane
+// rows: swapping halves and adding leaves a|c and b|d by half, swapping rows and
+// adding leaves a|b|c|d by row, and one DPP row reduction finishes all four, so
+// row r of the wave ends with splat r's total.
 __device__ inline float four_totals(float a, float b, float c, float d) {
-  const int lane = __lane_id();
-  const bool lower = lane < 32;
-  const float lo = (lower ? a : c) + __shfl_xor(lower ? c : a, 32, kWave);
-  const float hi = (lower ? b : d) + __shfl_xor(lower ? d : b, 32, kWave);
-  const bool even_row = (lane & 16) == 0;
-  return row_total((even_row ? lo : hi) + __shfl_xor(even_row ? hi : lo, 16, kWave));
+  const float lo = swap_sum(__builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a),
+                                                             __builtin_bit_cast(uint32_t, c), false, false));
+  const float hi = swap_sum(__builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, b),
+                                                             __builtin_bit_cast(uint32_t, d), false, false));
+  return row_total(swap_sum(__builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, lo),
+                                                             __builtin_bit_cast(uint32_t, hi), false, false)));
 }
 
 __device__ inline uint32_t wave_max(uint32_t v) {
@@ -171,43 +196,49 @@ __device__ inline uint32_t wave_max(uint32_t v) {
 
 // The per-pixel replay state of the backward pass (rasterizeBackward.wgsl:161-177).
 struct Replay {
-  float T, final_T, bg_grad, gr, gg, gb;
+  float T, final_bg, gr, gg, gb;  // final_bg = final T x (background . dL/dpixel)
   float behind_r, behind_g, behind_b;
   float next_alpha, next_r, next_g, next_b;
 };
 
-// One splat's 9 gradient shares at one pixel (rasterizeBackward.wgsl:199-230),
+// One splat's 9 gradient terms at one pixel (rasterizeBackward.wgsl:199-230),
 // written without branches: a pixel the splat does not blend into (power > 0,
 // alpha < 1/255, or past the pixel's last splat) runs the same instructions with
 // alpha = 0, which leaves T, the colour behind and the next splat as they were and
-// zeroes every share. The blended lanes compute exactly the branchy form.
-__device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy, float mx, float my, float ca,
-                                    float cbb, float cc, float op, float sr, float sg, float sb, float share[kGrads]) {
+// zeroes every term. The terms are linear in the splat's constants, so the conic
+// is applied once per splat after the reduction: with g = dL/dpower they are
+// g dx, g dy (mean2: a, b, c times these), g dx², g dx dy, g dy² (conic: times
+// -1/2, -1, -1/2), dL/dopacity and dL/drgb.
+__device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy, float mx, float my, float ha,
+                                    float hb, float hc, float op, float sr, float sg, float sb, float share[kGrads]) {
   const float dx = cx - mx, dy = cy - my;
-  const float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cbb * dx * dy;
-  const float G = __expf(power);
+  const float power2 = splat_power2(dx, dy, ha, hb, hc);
+  const float G = __builtin_amdgcn_exp2f(power2);
   const float clamped = fminf(0.99f, op * G);
-  const bool valid = blended && power <= 0.f && clamped >= 1.f / 255.f;
+  const bool valid = blended && power2 <= 0.f && clamped >= 1.f / 255.f;
   const float alpha = valid ? clamped : 0.f;
   const float keep = __builtin_amdgcn_rcpf(1.f - alpha);  // one hardware reciprocal for both divisions; 1 when idle
   st.T *= keep;
   const float na = valid ? st.next_alpha : 0.f;
-  st.behind_r = na * st.next_r + (1.f - na) * st.behind_r;
-  st.behind_g = na * st.next_g + (1.f - na) * st.behind_g;
-  st.behind_b = na * st.next_b + (1.f - na) * st.behind_b;
+  st.behind_r += na * (st.next_r - st.behind_r);
+  st.behind_g += na * (st.next_g - st.behind_g);
+  st.behind_b += na * (st.next_b - st.behind_b);
   st.next_alpha = valid ? alpha : st.next_alpha;
   st.next_r = valid ? sr : st.next_r;
   st.next_g = valid ? sg : st.next_g;
   st.next_b = valid ? sb : st.next_b;
-  const float alpha_grad = st.T * ((sr - st.behind_r) * st.gr + (sg - st.behind_g) * st.gg + (sb - st.behind_b) * st.gb) -
-                           st.final_T * keep * st.bg_grad;
-  const float power_grad = valid ? alpha_grad * op * G : 0.f;
-  share[0] = power_grad * (ca * dx + cbb * dy);
-  share[1] = power_grad * (cbb * dx + cc * dy);
-  share[2] = power_grad * (-0.5f * dx * dx);
-  share[3] = power_grad * (-dx * dy);
-  share[4] = power_grad * (-0.5f * dy * dy);
-  share[5] = valid ? alpha_grad * G : 0.f;
+  const float alpha_grad =
+      st.T * ((sr - st.behind_r) * st.gr + (sg - st.behind_g) * st.gg + (sb - st.behind_b) * st.gb) -
+      st.final_bg * keep;
+  const float opacity_grad = valid ? alpha_grad * G : 0.f;
+  const float g = opacity_grad * op;  // dL/dpower
+  const float gdx = g * dx, gdy = g * dy;
+  share[0] = gdx;
+  share[1] = gdy;
+  share[2] = gdx * dx;
+  share[3] = gdx * dy;
+  share[4] = gdy * dy;
+  share[5] = opacity_grad;
   const float at = alpha * st.T;
   share[6] = at * st.gr;
   share[7] = at * st.gg;
@@ -225,6 +256,7 @@ __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy
 template <int kPer>
 struct BackwardShared {
   float sums[4][kGrads][kPer * kChunk];
+  float4 conic[kPer * kChunk];
   uint32_t index[kPer * kChunk];
   uint32_t last[4];
 };
@@ -252,15 +284,16 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
   const float by0 = static_cast<float>(qy) + 0.5f, by1 = by0 + 7.f;
   Replay st{};
   uint32_t last = 0;
+  float final_T = 0.f;
   if (inside) {
     const uint32_t index = py * width + px;
-    st.final_T = pixel_color[index].w;
+    final_T = pixel_color[index].w;
     last = pixel_last[index];
     const float4 pg = pixel_grads[index];
     st.gr = pg.x; st.gg = pg.y; st.gb = pg.z;
   }
-  st.T = st.final_T;
-  st.bg_grad = bg_r * st.gr + bg_g * st.gg + bg_b * st.gb;
+  st.T = final_T;
+  st.final_bg = final_T * (bg_r * st.gr + bg_g * st.gg + bg_b * st.gb);
   const uint32_t my_last = wave_max(last);
   if (lane == 0) s_last[wave] = my_last;
   __syncthreads();
@@ -277,11 +310,15 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
       const uint32_t k = round + q * kChunk + lane;
       hits[q] = false;
       sts[q].index = 0;
+      float4 conic = make_float4(0.f, 0.f, 0.f, 0.f);
       if (k < run.y) {
-        stage(splats, pair_splats, k, sts[q]);
+        stage(splats, pair_splats, k, sts[q], &conic);
         hits[q] = k - run.x < my_last && meets(sts[q], bx0, bx1, by0, by1);
       }
-      if (wave == 0) s_index[q * kChunk + lane] = sts[q].index;
+      if (wave == 0) {
+        s_index[q * kChunk + lane] = sts[q].index;
+        sh.conic[q * kChunk + lane] = conic;
+      }
 #pragma unroll
       for (int v = 0; v < kGrads; v++) s_sums[wave][v][q * kChunk + lane] = 0.f;
     }
@@ -300,18 +337,18 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
         for (int g = 0; g < 4; g++) {
           js[g] = -1;
           bool blended = false;
-          float mx = 0.f, my = 0.f, ca = 0.f, cbb = 0.f, cc = 0.f, op = 0.f, sr = 0.f, sg = 0.f, sb = 0.f;
+          float mx = 0.f, my = 0.f, ha = 0.f, hb = 0.f, hc = 0.f, op = 0.f, sr = 0.f, sg = 0.f, sb = 0.f;
           if (mask) {
             const int j = kWave - 1 - __builtin_clzll(mask);
             mask &= ~(1ull << j);
             js[g] = q * kChunk + j;
             mx = lane_value(s.mx, j); my = lane_value(s.my, j);
-            ca = lane_value(s.ca, j); cbb = lane_value(s.cb, j); cc = lane_value(s.cc, j);
+            ha = lane_value(s.ha, j); hb = lane_value(s.hb, j); hc = lane_value(s.hc, j);
             op = lane_value(s.op, j);
             sr = lane_value(s.r, j); sg = lane_value(s.g, j); sb = lane_value(s.b, j);
             blended = base + j - run.x < last;
           }
-          any |= replay_splat(st, blended, cx, cy, mx, my, ca, cbb, cc, op, sr, sg, sb, share[g]);
+          any |= replay_splat(st, blended, cx, cy, mx, my, ha, hb, hc, op, sr, sg, sb, share[g]);
         }
         if (!__any(any)) continue;
         const int mine = row == 0 ? js[0] : (row == 1 ? js[1] : (row == 2 ? js[2] : js[3]));
@@ -323,10 +360,36 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
       }
     }
     __syncthreads();
-    for (int e = tid; e < kGrads * kRound; e += 256) {
-      const int v = e / kRound, slot = e % kRound;
-      const float total = s_sums[0][v][slot] + s_sums[1][v][slot] + s_sums[2][v][slot] + s_sums[3][v][slot];
-      if (total != 0.f) unsafeAtomicAdd(splat_grads + static_cast<size_t>(s_index[slot]) * kGrads + v, total);
+    // Per (slot, gradient pair): the four quadrants' sums, then the splat's
+    // constants: mean2 from the (g dx, g dy) sums through the conic, the conic
+    // terms scaled by -1/2, -1, -1/2; the rest as summed.
+    for (int e = tid; e < 5 * kRound; e += 256) {
+      const int part = e / kRound, slot = e % kRound;
+      auto sum = [&](int v) { return s_sums[0][v][slot] + s_sums[1][v][slot] + s_sums[2][v][slot] + s_sums[3][v][slot]; };
+      float* out = splat_grads + static_cast<size_t>(s_index[slot]) * kGrads;
+      float t0, t1;
+      int v0, v1;
+      if (part == 0) {
+        const float4 k = sh.conic[slot];
+        const float u = sum(0), w = sum(1);
+        t0 = k.x * u + k.y * w;
+        t1 = k.y * u + k.z * w;
+        v0 = 0; v1 = 1;
+      } else if (part == 1) {
+        t0 = -0.5f * sum(2); t1 = -sum(3);
+        v0 = 2; v1 = 3;
+      } else if (part == 2) {
+        t0 = -0.5f * sum(4); t1 = sum(5);
+        v0 = 4; v1 = 5;
+      } else if (part == 3) {
+        t0 = sum(6); t1 = sum(7);
+        v0 = 6; v1 = 7;
+      } else {
+        t0 = sum(8); t1 = 0.f;
+        v0 = 8; v1 = -1;
+      }
+      if (t0 != 0.f) unsafeAtomicAdd(out + v0, t0);
+      if (v1 >= 0 && t1 != 0.f) unsafeAtomicAdd(out + v1, t1);
     }
     __syncthreads();
   }

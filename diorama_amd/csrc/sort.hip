@@ -6,9 +6,15 @@
 // * then one kernel per pass: a 4096-key tile per workgroup (4 waves, each owning
 //   1,024 consecutive keys) ranks its keys stably with wavefront "match" masks
 //   (8 ballots per key) and per-wave digit counts, then a per-digit prefix across
-//   the 4 waves (two barriers per tile), learns where
-//   its run of each digit starts by per-digit decoupled look-back over earlier
-//   tiles, and scatters through LDS so each digit's run is written contiguously.
+//   the 4 waves (two barriers per tile), learns where its run of each digit
+//   starts by per-digit look-back over earlier tiles, and scatters through LDS so
+//   each digit's run is written contiguously.
+// The look-back is two-level. Tiles form groups of 16; each tile adds its counts
+// into its group's sum (with a count of tiles added in the high bits), reads its
+// own group's earlier tiles, then whole earlier groups (their inclusive prefix
+// once a group's last tile has published it, else their completed sum). When
+// every tile of a pass runs at once (a million keys is 269 tiles) that is a few
+// round trips, where a window over single tiles needs one per 16 tiles.
 // Tiles are handed out by an atomic ticket, so a tile only waits on tiles that
 // started before it. Per pass the keys are read and written once.
 #include "dio_internal.hpp"
@@ -23,6 +29,16 @@ constexpr uint32_t kMaxPasses = 16;  // 64 key bits in 4-bit digits
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1;
+// Group sums: tiles added so far above bit 24, their counts (at most 16 x 4096) below.
+constexpr uint32_t kGroupShift = 24;
+constexpr uint32_t kGroupSumMask = (1u << kGroupShift) - 1;
+
+// One pass's look-back words: per tile, per group sum, per group prefix.
+struct PassState {
+  uint32_t* state;
+  uint32_t* group_sum;
+  uint32_t* group_prefix;
+};
 
 // Keys of one width with digits of kBits (8 or 4) bits; a digit never straddles
 // the two words of a 64-bit key.
@@ -49,46 +65,49 @@ __device__ inline uint32_t key_count(const uint32_t* count, uint32_t capacity) {
   return count ? min(*count, capacity) : capacity;
 }
 
-// Counts the digits of every pass in one read of the keys (LDS histograms, one
-// per wave, added into the global ones), and clears the first pass's look-back
-// states and every pass ticket.
+// Counts the digits of every pass in one read of the keys (LDS histograms added
+// into the global ones), a 4096-key tile at a time with all 16 loads per thread
+// in flight together, and clears the first pass's look-back words and every
+// pass ticket.
 template <bool kWide, int kBits>
 __global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyOps<kWide, kBits>::T* __restrict__ keys,
                                                               const uint32_t* count, uint32_t capacity,
                                                               uint32_t begin_bit, uint32_t passes,
                                                               uint32_t* __restrict__ hist, uint32_t* __restrict__ state0,
+                                                              uint32_t* __restrict__ group_sum0,
+                                                              uint32_t* __restrict__ group_prefix0,
                                                               uint32_t* __restrict__ tickets) {
   using Ops = KeyOps<kWide, kBits>;
+  using Key = typename Ops::T;
   constexpr int kRadix = 1 << kBits;
   __shared__ uint32_t local[kMaxPasses][kRadix];
   const int tid = threadIdx.x;
   const uint32_t n = key_count(count, capacity);
-  const uint32_t tiles = ceil_div(n, kSortTile);
+  const uint32_t tiles = ceil_div(n, kSortTile), groups = ceil_div(tiles, kSortGroup);
   const uint32_t stride = gridDim.x * kThreads;
   for (uint32_t k = blockIdx.x * kThreads + tid; k < tiles * 256u; k += stride) state0[k] = 0;
+  for (uint32_t k = blockIdx.x * kThreads + tid; k < groups * 256u; k += stride) {
+    group_sum0[k] = 0;
+    group_prefix0[k] = 0;
+  }
   if (blockIdx.x == 0 && tid < static_cast<int>(kMaxPasses)) tickets[tid] = 0;
   if (tid < kRadix)
     for (uint32_t p = 0; p < passes; p++) local[p][tid] = 0;
   __syncthreads();
-  // Lanes holding the same digit first agree on one count (match masks), so a few
-  // hot digits (a float's exponent byte) do not serialise the LDS atomics.
-  const int lane = tid & (kWave - 1);
-  const uint64_t lower_lanes = (1ull << lane) - 1;
-  const uint32_t rounds = ceil_div(n, stride);
-  for (uint32_t r = 0; r < rounds; r++) {
-    const uint32_t i = r * stride + blockIdx.x * kThreads + tid;
-    const bool valid = i < n;
-    const auto key = valid ? keys[i] : Ops::invalid();
-    for (uint32_t p = 0; p < passes; p++) {
-      const uint32_t d = Ops::digit(key, begin_bit + kBits * p);
-      uint64_t peers = __ballot(valid);
+  // Plain LDS atomics: lanes hitting the same digit serialise in the LDS (a hot
+  // exponent byte costs up to 64 cycles an instruction), which is still far
+  // cheaper than agreeing on one count per digit with 8 ballots per key first.
+  for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    Key key[kItems];
 #pragma unroll
-      for (int b = 0; b < kBits; b++) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t set = __ballot(bit);
-        peers &= bit ? set : ~set;
-      }
-      if (valid && (peers & lower_lanes) == 0) atomicAdd(&local[p][d], static_cast<uint32_t>(__popcll(peers)));
+    for (int k = 0; k < kItems; k++) {
+      const uint32_t i = tile * kSortTile + k * kThreads + tid;
+      key[k] = i < n ? keys[i] : Ops::invalid();
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+      if (tile * kSortTile + k * kThreads + tid < n)
+        for (uint32_t p = 0; p < passes; p++) atomicAdd(&local[p][Ops::digit(key[k], begin_bit + kBits * p)], 1u);
     }
   }
   __syncthreads();
@@ -131,8 +150,8 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
                                                              typename KeyOps<kWide, kBits>::T* __restrict__ keys_out,
                                                              uint32_t* __restrict__ values_out, const uint32_t* count,
                                                              uint32_t capacity, uint32_t shift,
-                                                             const uint32_t* __restrict__ digit_start, uint32_t* state,
-                                                             uint32_t* __restrict__ next_state, uint32_t* ticket) {
+                                                             const uint32_t* __restrict__ digit_start, PassState now,
+                                                             PassState next, uint32_t* ticket) {
   using Ops = KeyOps<kWide, kBits>;
   using Key = typename Ops::T;
   constexpr int kRadix = 1 << kBits;  // thread d < kRadix owns digit d
@@ -158,7 +177,14 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
   const uint32_t n = key_count(count, capacity);
   const uint32_t tiles = ceil_div(n, kSortTile);
   if (tile >= tiles) return;
-  if (next_state) next_state[tile * 256 + tid] = 0;  // the next pass's look-back states
+  const uint32_t groups = ceil_div(tiles, kSortGroup);
+  if (next.state) {  // the next pass's look-back words
+    next.state[tile * 256 + tid] = 0;
+    if (tile < groups) {
+      next.group_sum[tile * 256 + tid] = 0;
+      next.group_prefix[tile * 256 + tid] = 0;
+    }
+  }
   const uint32_t base = tile * kSortTile;
   const uint64_t lower_lanes = (1ull << lane) - 1;
 
@@ -213,37 +239,66 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
 #pragma unroll
   for (int k = 0; k < kItems; k++) rank[k] += wave_offset[wave][Ops::digit(key[k], shift)];
 
-  // This tile's count of digit `tid`: publish it, then add up earlier tiles' counts
-  // back to the first one that knows its inclusive prefix.
+  // This tile's count of digit `tid`: publish it and add it to its group's sum,
+  // then add up what came before. Tiles come in groups of 16: the tile reads its
+  // own group's earlier tiles (nearest first, back to one that knows its
+  // inclusive prefix) and then whole earlier groups (a group's inclusive prefix
+  // where its last tile has published one, else its completed sum). When every
+  // tile runs at once this takes a few round trips instead of one per 16 tiles.
   const uint32_t mine = tid < kRadix ? running[tid] : 0u;
   uint32_t exclusive = 0;
   if (tid < kRadix) {
-    if (tile == 0) {
-      publish(state + tid, kFlagPrefix | mine);
-    } else {
-      publish(state + tile * 256 + tid, kFlagAggregate | mine);
-      // Sixteen predecessors per step, loaded together: every tile runs at once at
-      // these sizes, so the published prefixes advance a window per round trip.
-      constexpr int kWindow = 16;
-      for (int32_t t0 = static_cast<int32_t>(tile) - 1;; t0 -= kWindow) {
-        uint32_t word[kWindow];
+    const uint32_t group = tile / kSortGroup, first = group * kSortGroup;
+    uint32_t* const state = now.state;
+    publish(state + tile * 256 + tid, (tile == 0 ? kFlagPrefix : kFlagAggregate) | mine);
+    atomicAdd(now.group_sum + group * 256 + tid, (1u << kGroupShift) | mine);
+    bool found = tile == 0;
+    if (!found) {
+      const uint32_t m = tile - first;  // earlier tiles of the own group
+      uint32_t word[kSortGroup - 1];
 #pragma unroll
-        for (int k = 0; k < kWindow; k++) word[k] = t0 - k >= 0 ? peek(state + (t0 - k) * 256 + tid) : kFlagPrefix;
+      for (uint32_t k = 0; k < kSortGroup - 1; k++) word[k] = k < m ? peek(state + (tile - 1 - k) * 256 + tid) : 0u;
 #pragma unroll
-        for (int k = 0; k < kWindow; k++)
-          while ((word[k] >> 30) == 0) word[k] = peek(state + (t0 - k) * 256 + tid);
-        bool found = false;
+      for (uint32_t k = 0; k < kSortGroup - 1; k++)
+        if (k < m)
+          while ((word[k] >> 30) == 0) word[k] = peek(state + (tile - 1 - k) * 256 + tid);
 #pragma unroll
-        for (int k = 0; k < kWindow; k++) {
-          if (!found) {
-            exclusive += word[k] & kValueMask;
-            found = (word[k] >> 30) == 2;
-          }
+      for (uint32_t k = 0; k < kSortGroup - 1; k++) {
+        if (k < m && !found) {
+          exclusive += word[k] & kValueMask;
+          found = (word[k] >> 30) == 2;
         }
-        if (found) break;
       }
-      publish(state + tile * 256 + tid, kFlagPrefix | (exclusive + mine));
     }
+    constexpr int kWindow = 8;  // earlier groups read per round trip
+    for (int32_t g0 = static_cast<int32_t>(group) - 1; !found && g0 >= 0; g0 -= kWindow) {
+      uint32_t inclusive[kWindow], sum[kWindow];
+#pragma unroll
+      for (int k = 0; k < kWindow; k++) {
+        const int32_t g = g0 - k;
+        inclusive[k] = g >= 0 ? peek(now.group_prefix + g * 256 + tid) : 0u;
+        sum[k] = g >= 0 ? peek(now.group_sum + g * 256 + tid) : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < kWindow; k++) {
+        const int32_t g = g0 - k;
+        if (found || g < 0) continue;
+        // an earlier group is whole (16 tiles): wait until all of them have added in
+        while ((inclusive[k] >> 30) != 2 && (sum[k] >> kGroupShift) < kSortGroup) {
+          inclusive[k] = peek(now.group_prefix + g * 256 + tid);
+          sum[k] = peek(now.group_sum + g * 256 + tid);
+        }
+        if ((inclusive[k] >> 30) == 2) {
+          exclusive += inclusive[k] & kValueMask;
+          found = true;
+        } else {
+          exclusive += sum[k] & kGroupSumMask;
+        }
+      }
+    }
+    if (tile != 0) publish(state + tile * 256 + tid, kFlagPrefix | (exclusive + mine));
+    if (tile == min(first + kSortGroup, tiles) - 1)
+      publish(now.group_prefix + group * 256 + tid, kFlagPrefix | (exclusive + mine));
     run_start[tid] = digit_start[tid] + exclusive;
   }
 
@@ -290,21 +345,27 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
 
 size_t sort_workspace_bytes(uint32_t capacity) {
   const uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kSortTile);
-  // digit offsets for every pass, two look-back state arrays, pass tickets
-  return kMaxPasses * 256 * 4 + 2 * static_cast<size_t>(tiles) * 256 * 4 + 256;
+  const uint32_t groups = ceil_div(tiles, kSortGroup);
+  // digit offsets for every pass, two tile and two pairs of group look-back arrays, pass tickets
+  return kMaxPasses * 256 * 4 + 2 * static_cast<size_t>(tiles) * 256 * 4 +
+         4 * static_cast<size_t>(groups) * 256 * 4 + 256;
 }
 
 SortWorkspace sort_workspace(void* base, uint32_t capacity) {
   SortWorkspace ws;
   ws.tiles = capacity == 0 ? 1 : ceil_div(capacity, kSortTile);
+  ws.groups = ceil_div(ws.tiles, kSortGroup);
   char* p = static_cast<char*>(base);
-  ws.hist = reinterpret_cast<uint32_t*>(p);
-  p += kMaxPasses * 256 * 4;
-  ws.state[0] = reinterpret_cast<uint32_t*>(p);
-  p += static_cast<size_t>(ws.tiles) * 256 * 4;
-  ws.state[1] = reinterpret_cast<uint32_t*>(p);
-  p += static_cast<size_t>(ws.tiles) * 256 * 4;
-  ws.tickets = reinterpret_cast<uint32_t*>(p);
+  auto take = [&p](size_t words) {
+    uint32_t* q = reinterpret_cast<uint32_t*>(p);
+    p += words * 4;
+    return q;
+  };
+  ws.hist = take(kMaxPasses * 256);
+  for (int k = 0; k < 2; k++) ws.state[k] = take(static_cast<size_t>(ws.tiles) * 256);
+  for (int k = 0; k < 2; k++) ws.group_sum[k] = take(static_cast<size_t>(ws.groups) * 256);
+  for (int k = 0; k < 2; k++) ws.group_prefix[k] = take(static_cast<size_t>(ws.groups) * 256);
+  ws.tickets = take(64);
   return ws;
 }
 
@@ -322,11 +383,11 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
   if (wide)
     histogram_kernel<true, kBits><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint2*>(keys), count, capacity,
                                                                     begin_bit, passes, ws.hist, ws.state[0],
-                                                                    ws.tickets);
+                                                                    ws.group_sum[0], ws.group_prefix[0], ws.tickets);
   else
     histogram_kernel<false, kBits><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint32_t*>(keys), count,
                                                                      capacity, begin_bit, passes, ws.hist, ws.state[0],
-                                                                     ws.tickets);
+                                                                     ws.group_sum[0], ws.group_prefix[0], ws.tickets);
   DIO_LAUNCHED("radix histogram");
   digit_offsets_kernel<<<passes, 256, 0, s>>>(ws.hist);
   DIO_LAUNCHED("radix digit offsets");
@@ -336,16 +397,18 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
   uint32_t* vout = alt_values;
   for (uint32_t p = 0; p < passes; p++) {
     const uint32_t shift = begin_bit + kBits * p;
-    uint32_t* state = ws.state[p % 2];
-    uint32_t* next = p + 1 < passes ? ws.state[(p + 1) % 2] : nullptr;
+    const PassState now{ws.state[p % 2], ws.group_sum[p % 2], ws.group_prefix[p % 2]};
+    const PassState next = p + 1 < passes
+                               ? PassState{ws.state[(p + 1) % 2], ws.group_sum[(p + 1) % 2], ws.group_prefix[(p + 1) % 2]}
+                               : PassState{nullptr, nullptr, nullptr};
     if (wide)
       onesweep_kernel<true, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), vin,
                                                                static_cast<uint2*>(kout), vout, count, capacity, shift,
-                                                               ws.hist + p * 256, state, next, ws.tickets + p);
+                                                               ws.hist + p * 256, now, next, ws.tickets + p);
     else
       onesweep_kernel<false, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), vin,
                                                                 static_cast<uint32_t*>(kout), vout, count, capacity,
-                                                                shift, ws.hist + p * 256, state, next, ws.tickets + p);
+                                                                shift, ws.hist + p * 256, now, next, ws.tickets + p);
     DIO_LAUNCHED("radix onesweep");
     std::swap(kin, kout);
     std::swap(vin, vout);

@@ -96,7 +96,7 @@ def run_sort(keys, values, high_bits=0, capacity=None, count=None):
     return k, v
 
 
-@pytest.mark.parametrize("n", [1, 1000, 1024, 100_003])
+@pytest.mark.parametrize("n", [1, 1000, 1024, 100_003, 1_100_000])
 def test_sort_32bit_depth_sort_check_case(n):
     random = scene.mulberry32(n)
     keys = np.array([int(random() * 2 ** 32) if i % 2 == 0 else int(random() * 16) for i in range(n)],
