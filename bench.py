@@ -39,14 +39,18 @@ STAGE_BYTES = {
     "rasterize": (0, 40, 24),
     "loss": (0, 0, 32),
     "rasterize_backward": (36, 40, 32),
-    "preprocess_backward": (512, 0, 0),
-    "adam": (1708, 0, 0),
+    # preprocess backward + statistics + Adam in one pass: R activated 56, raw 56, SH 180,
+    # splat 4 + 36 grads, moments 472, stats 16; W raw 56, SH 180, moments 472,
+    # activated 56, stats 16 (the reference's separate passes move 512 + 44 + 1708)
+    "backward_adam": (1600, 0, 0),
 }
+# The reference's per-step byte model (separate preprocess backward, stats and Adam).
 STEP_BYTES = (2612, 240, 88)
-# Stage name -> kernel name, for the PMC traffic lines.
+# Stage name -> kernel name, for the PMC traffic and VALU lines.
 STAGE_KERNEL = {"preprocess": "preprocess_kernel", "rasterize": "rasterize_kernel",
-                "rasterize_backward": "rasterize_backward_kernel", "preprocess_backward": "preprocess_backward_kernel",
-                "adam": "adam_kernel"}
+                "rasterize_backward": "rasterize_backward_kernel", "backward_adam": "backward_adam_kernel"}
+# VALU issue: a wave64 instruction occupies a SIMD for 4 cycles; 1024 SIMDs at 2.4 GHz.
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 TRAFFIC_FILE = "pmc_traffic_train_step_1.1M_980x545.txt"
 METRIC = "train steps/s @1.1M Gaussians 980x545 FP32; held-out PSNR delta vs plain"
 
@@ -475,6 +479,19 @@ def learned_bench(args):
     return 0
 
 
+def measured_valu(kernel):
+    """VALU instructions per launch of `kernel` from the committed PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == "valu" and parts[1] == kernel:
+                return int(parts[2])
+    return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE and
     WRITE_SIZE already scaled by the copy calibration), or None."""
@@ -571,6 +588,11 @@ def main():
                 "unit": "GB/s", "frac": round(d["GB/s"] / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(STAGE_KERNEL.get(dominant, dominant)),
                 "algorithmic_bytes": d["bytes"]}
+    valu = measured_valu(STAGE_KERNEL.get(dominant, dominant))
+    if valu:
+        # The rasterizer passes are bound by VALU issue, not HBM: the share of the
+        # issue limit their PMC-counted instructions take at the measured time.
+        roofline["valu_issue_frac"] = round(valu / (d["ms"] * 1e-3) / VALU_ISSUE_PER_S, 4)
     step_bytes = STEP_BYTES[0] * n + STEP_BYTES[1] * pairs + STEP_BYTES[2] * x
 
     cpu = None

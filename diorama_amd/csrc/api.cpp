@@ -594,9 +594,9 @@ void renderer_tiles(dio_renderer* r, dio_buf* target, const float bg[3]) {
   encode_tiles(r->scene->rasterizer, target, w, h, bg, true);
 }
 
-void renderer_backward(dio_renderer* r, dio_buf* pixel_grads) {
+// The rasterizer's half of encodeBackward: per-splat gradients into grad_splats.
+void renderer_backward_splats(dio_renderer* r, dio_buf* pixel_grads) {
   auto* sc = r->scene;
-  if (!sc) return;
   if (!sc->grad_splats) {
     sc->grad_splats = alloc(r->ctx, static_cast<size_t>(sc->count) * 9 * 4, true);
     sc->grad_gaussians = alloc(r->ctx, static_cast<size_t>(sc->count) * dio::kFloatsPerGaussian * 4, true);
@@ -605,6 +605,12 @@ void renderer_backward(dio_renderer* r, dio_buf* pixel_grads) {
   auto* t = sc->rasterizer;
   if (!t->framed) invalid("Encode a frame before its backward pass.");
   backward_tiles(t, t->frame_w, t->frame_h, pixel_grads, sc->grad_splats);
+}
+
+void renderer_backward(dio_renderer* r, dio_buf* pixel_grads) {
+  auto* sc = r->scene;
+  if (!sc) return;
+  renderer_backward_splats(r, pixel_grads);
   dio::preprocess_backward(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count,
                            r->camera, dev<float>(sc->splats), dev<float>(sc->grad_splats),
                            dev<float>(sc->grad_gaussians), dev<float>(sc->grad_sh));
@@ -1024,7 +1030,6 @@ void trainer_optimize(dio_trainer* t) {
   dio::training_loss(t->ctx->stream, t->width, t->height, dev<float>(t->renderer->scene->rasterizer->pixel_color),
                      dev<uint32_t>(view.image), dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
                      dev<float>(t->loss->pixel_loss));
-  renderer_backward(t->renderer, t->loss->pixel_grads);
   if (!densify_now) t->adam_steps++;
   const double progress = std::min(1.0, static_cast<double>(step) / st.position_lr_steps);
   const double position_lr =
@@ -1045,11 +1050,20 @@ void trainer_optimize(dio_trainer* t) {
   a.viewport[1] = view.camera.viewport[1];
   t->last_adam = a;
   auto* sc = t->renderer->scene;
-  dio::accumulate_stats(t->ctx->stream, t->count, dev<float>(sc->splats), dev<float>(sc->grad_splats), a,
-                        dev<float>(t->stats), dev<float>(t->learned.extra));
-  if (!densify_now)
-    dio::adam_step(t->ctx->stream, t->count, dev<float>(t->params), dev<float>(sc->sh), dev<float>(t->moments),
-                   dev<float>(sc->grad_gaussians), dev<float>(sc->grad_sh), a, dev<float>(sc->gaussians));
+  if (densify_now) {
+    // No Adam step on a densification step: the gradients feed only the statistics.
+    renderer_backward(t->renderer, t->loss->pixel_grads);
+    dio::accumulate_stats(t->ctx->stream, t->count, dev<float>(sc->splats), dev<float>(sc->grad_splats), a,
+                          dev<float>(t->stats), dev<float>(t->learned.extra));
+  } else {
+    // The rasterizer's backward, then preprocess backward + statistics + Adam in
+    // one pass; the renderer's per-Gaussian gradient buffers are not written.
+    renderer_backward_splats(t->renderer, t->loss->pixel_grads);
+    dio::backward_adam(t->ctx->stream, t->count, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree,
+                       t->renderer->camera, dev<float>(sc->splats), dev<float>(sc->grad_splats),
+                       dev<float>(t->params), dev<float>(t->moments), a, dev<float>(t->stats),
+                       dev<float>(t->learned.extra));
+  }
   tiles_after_submit(sc->rasterizer, false);
   t->due_densify = densify_now;
   t->due_reset = densifying && st.opacity_reset_interval && step % st.opacity_reset_interval == 0;
@@ -1908,6 +1922,12 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
         dio::training_loss(s, t->width, t->height, dev<float>(tr->pixel_color), dev<uint32_t>(view.image),
                            dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
                            dev<float>(t->loss->pixel_loss));
+      };
+    } else if (name == "backward_adam") {
+      run = [&] {
+        dio::backward_adam(s, t->count, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, r->camera,
+                           dev<float>(sc->splats), dev<float>(sc->grad_splats), dev<float>(t->params),
+                           dev<float>(t->moments), t->last_adam, dev<float>(t->stats), dev<float>(t->learned.extra));
       };
     } else if (name == "adam") {
       run = [&] {

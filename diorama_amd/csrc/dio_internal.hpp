@@ -174,6 +174,13 @@ struct AdamArgs {
 // summed gradient x, summed gradient y), for the learned densification.
 void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const float* splat_grads,
                       const AdamArgs& a, float* stats, float* extra = nullptr);
+// The plain trainer's preprocess backward, statistics and Adam step in one pass
+// (the per-Gaussian gradients stay on chip): reads the activated Gaussians and SH
+// before the step, the splats and their gradients; updates params, SH, moments,
+// the activated copy and the statistics (and `extra`, optional).
+void backward_adam(hipStream_t s, uint32_t n, float* gaussians, float* sh, uint32_t sh_degree, const CameraArgs& cam,
+                   const float* splats, const float* splat_grads, float* params, float* moments, const AdamArgs& a,
+                   float* stats, float* extra);
 void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* moments, const float* gaussian_grads,
                const float* sh_grads, const AdamArgs& a, float* gaussians);
 // The same step with its arguments read from device memory (for graph replays).

@@ -316,9 +316,10 @@ dio_buf* dio_trainer_extra_stats(dio_trainer* t);
 
 /* Times `repeats` launches of one named stage between HIP events recorded on the
  * context's stream, for the benchmark's roofline figures. Stages: "preprocess",
- * "scan", "sort", "rasterize", "rasterize_backward", "preprocess_backward",
- * "loss", "adam"; a batch trainer has the one stage "batch_step" (its captured
- * step graph). Returns the mean milliseconds per launch of the last optimize()
+ * "scan", "sort", "rasterize", "rasterize_backward", "loss", "backward_adam" (the
+ * step's fused preprocess backward + statistics + Adam), and the separate
+ * "preprocess_backward" and "adam" of densification steps; a batch trainer has
+ * the one stage "batch_step" (its captured step graph). Returns the mean milliseconds per launch of the last optimize()
  * step's inputs re-run `repeats` times. */
 int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_t repeats, double* ms);
 

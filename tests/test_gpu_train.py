@@ -43,13 +43,15 @@ def test_adam_and_stats_step_against_oracle():
     t.buffers["moments"].upload(moments)
     stats0 = np.abs(np.random.default_rng(3).normal(size=(4096, 4))).astype(np.float32)
     t.buffers["stats"].upload(stats0)
-    t.optimize()
     r = t.renderer
-    grads = r.gradients
-    gg = grads["gaussians"].download(np.float32).reshape(-1, 16)
-    shg = grads["sh"].download(np.float32).reshape(-1, 45)
-    sgrad = grads["splats"].download(np.float32).reshape(-1, 9)
+    act_before = r.gaussian_buffer.download(np.float32).reshape(-1, 16)
+    t.optimize()
+    # The step's preprocess backward, statistics and Adam run fused (the per-Gaussian
+    # gradients stay on chip), so the expected gradients come from the oracle's
+    # preprocess backward of the step's own splat gradients.
+    sgrad = r.gradients["splats"].download(np.float32).reshape(-1, 9)
     splats = r.splat_buffer.download(np.float32).reshape(-1, 12)
+    gg, shg = oracle.preprocess_backward(act_before, sh, t.sh_degree, cam, splats, sgrad)
     p_after = t.buffers["params"].download(np.float32).reshape(-1, 16)
     sh_after = r.sh_buffer.download(np.float32).reshape(-1, 45)
     m_after = t.buffers["moments"].download(np.float32).reshape(-1, 122)
