@@ -34,6 +34,12 @@ class _TuningResult(ctypes.Structure):
     _fields_ = [("tuning", _Tuning), ("ms_per_step", ctypes.c_double)]
 
 
+class _ColmapCamera(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("position", ctypes.c_double * 3),
+                ("right", ctypes.c_double * 3), ("down", ctypes.c_double * 3), ("forward", ctypes.c_double * 3)]
+
+
 class _Settings(ctypes.Structure):
     _fields_ = [
         ("extent", ctypes.c_float),
@@ -174,6 +180,10 @@ SIGNATURES = {
     "dio_trainer_time_stage": (_I, [_P, ctypes.c_char_p, _U32, ctypes.POINTER(ctypes.c_double)]),
     "dio_trainer_measure_psnr": (_I, [_P, ctypes.POINTER(_Camera), ctypes.POINTER(_P), _U32,
                                      ctypes.POINTER(ctypes.c_double)]),
+    "dio_colmap_parse": (_I, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t,
+                              ctypes.POINTER(_ColmapCamera), _U32, ctypes.POINTER(_U32), _P, _P, _U32,
+                              ctypes.POINTER(_U32)]),
+    "dio_colmap_training_camera": (_I, [ctypes.POINTER(_ColmapCamera), _U32, _U32, ctypes.POINTER(_Camera)]),
     "dio_mean_neighbor_distance2": (_I, [_P, _P, _U32, _P]),
     "dio_initial_gaussians": (_I, [_P, _P, _P, _U32, _P, _P]),
     "dio_ply_write": (_I, [_P, _P, _U32, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
@@ -854,3 +864,52 @@ def initial_gaussians(ctx, positions, colors):
     _check(lib().dio_initial_gaussians(ctx.handle, pts.handle, cols.handle, n, params.handle, sh.handle))
     return {"params": params.download(np.float32, n * 16).reshape(n, 16),
             "sh": sh.download(np.float32, n * 45).reshape(n, 45)}
+
+
+def _colmap_camera(c):
+    out = _ColmapCamera()
+    out.name = c["name"].encode()
+    out.width, out.height = int(c["width"]), int(c["height"])
+    out.fx, out.fy = float(c["fx"]), float(c["fy"])
+    for key in ("position", "right", "down", "forward"):
+        getattr(out, key)[:] = [float(v) for v in c[key]]
+    return out
+
+
+def parse_colmap(cameras_bin, images_bin, points_bin):
+    """parseColmap (src/colmap.ts:18-129): {"cameras": [TrainingCamera dicts sorted by
+    name], "positions": n x 3 float32, "colors": n x 3 uint8}."""
+    files = [bytes(cameras_bin), bytes(images_bin), bytes(points_bin)]
+    nc, npts = _U32(), _U32()
+    args = []
+    for f in files:
+        args += [f, len(f)]
+    _check(lib().dio_colmap_parse(*args, None, 0, ctypes.byref(nc), None, None, 0, ctypes.byref(npts)))
+    cams = (_ColmapCamera * max(nc.value, 1))()
+    positions = np.empty((npts.value, 3), np.float32)
+    colors = np.empty((npts.value, 3), np.uint8)
+    _check(lib().dio_colmap_parse(*args, cams, nc.value, ctypes.byref(nc), positions.ctypes.data_as(_P),
+                                  colors.ctypes.data_as(_P), npts.value, ctypes.byref(npts)))
+    cameras = [{"name": c.name.decode(), "width": c.width, "height": c.height, "fx": c.fx, "fy": c.fy,
+                "position": list(c.position), "right": list(c.right), "down": list(c.down),
+                "forward": list(c.forward)} for c in cams[:nc.value]]
+    return {"cameras": cameras, "positions": positions, "colors": colors}
+
+
+def training_camera_data(camera, viewport):
+    """trainingCameraData (src/loadDataset.ts:80-86): the renderer's camera for a
+    TrainingCamera rendered at viewport (width, height)."""
+    out = _Camera()
+    _check(lib().dio_colmap_training_camera(ctypes.byref(_colmap_camera(camera)), int(viewport[0]),
+                                            int(viewport[1]), ctypes.byref(out)))
+    return {"view": list(out.view), "proj": list(out.proj), "viewport": list(out.viewport)}
+
+
+def split_dataset(cameras, test_every=8):
+    """loadDataset's split and extent (src/loadDataset.ts:10-43): every 8th photo by
+    name is held out; extent = 1.1 x the farthest training camera from their mean."""
+    train = [c for i, c in enumerate(cameras) if i % test_every != 0]
+    test = [c for i, c in enumerate(cameras) if i % test_every == 0]
+    centers = np.array([c["position"] for c in train], np.float64).reshape(-1, 3)
+    extent = 1.1 * float(np.linalg.norm(centers - centers.mean(axis=0), axis=1).max()) if len(centers) else 0.0
+    return {"train": train, "test": test, "extent": extent}

@@ -1,9 +1,12 @@
-// What sits either side of the training path, over the C ABI: the starting
-// Gaussians from structure-from-motion points (src/loadDataset.ts:87-149, the
-// kernels in knn.hip) and the 3DGS .ply writer and loader (src/plyWriter.ts:14-43,
-// src/plyLoader.ts:21-110). The .ply header is text handled here; the body is
+// What sits either side of the training path, over the C ABI: a COLMAP sparse
+// reconstruction and its cameras (src/colmap.ts:18-129, src/sceneCameras.ts:53-67,
+// src/loadDataset.ts:80-86), the starting Gaussians from its points
+// (src/loadDataset.ts:87-149, the kernels in knn.hip) and the 3DGS .ply writer
+// and loader (src/plyWriter.ts:14-43, src/plyLoader.ts:21-110). The .ply header is text handled here; the body is
 // packed and unpacked by ply.hip on the device, so a trainer's Gaussians go to a
 // file straight from HBM and a loaded scene lands in HBM in the renderer's layout.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -280,5 +283,181 @@ extern "C" int dio_initial_gaussians(dio_ctx* ctx, const dio_buf* positions, con
       throw;
     }
     release(d2);
+  });
+}
+
+// ================================================================= COLMAP
+
+namespace {
+
+// Little-endian reads over one COLMAP file, failing on a short file.
+struct ColmapReader {
+  const uint8_t* p;
+  size_t size, at = 0;
+  const char* what;
+  void need_bytes(size_t n) {
+    if (at + n > size || at + n < at) invalid(std::string("The COLMAP file ") + what + " is shorter than its contents say.");
+  }
+  template <class T>
+  T read() {
+    need_bytes(sizeof(T));
+    T v;
+    std::memcpy(&v, p + at, sizeof(T));
+    at += sizeof(T);
+    return v;
+  }
+  void skip(uint64_t n) {
+    if (n > size) invalid(std::string("The COLMAP file ") + what + " is shorter than its contents say.");
+    need_bytes(static_cast<size_t>(n));
+    at += static_cast<size_t>(n);
+  }
+  std::string text() {
+    const size_t start = at;
+    while (true) {
+      need_bytes(1);
+      if (p[at++] == 0) break;
+    }
+    return std::string(reinterpret_cast<const char*>(p + start), at - 1 - start);
+  }
+};
+
+struct Intrinsics {
+  uint64_t width, height;
+  double fx, fy;
+};
+
+}  // namespace
+
+extern "C" int dio_colmap_parse(const void* cameras_bin, size_t cameras_bytes, const void* images_bin,
+                                size_t images_bytes, const void* points_bin, size_t points_bytes,
+                                dio_colmap_camera* cameras, uint32_t camera_capacity, uint32_t* camera_count,
+                                float* positions, uint8_t* colors, uint32_t point_capacity, uint32_t* point_count) {
+  return guard([&] {
+    need(cameras_bin, "cameras_bin");
+    need(images_bin, "images_bin");
+    need(points_bin, "points_bin");
+    need(camera_count, "camera_count");
+    need(point_count, "point_count");
+    // cameras.bin: the intrinsics of undistorted pinhole models (src/colmap.ts:72-94)
+    ColmapReader rc{static_cast<const uint8_t*>(cameras_bin), cameras_bytes, 0, "cameras.bin"};
+    std::vector<std::pair<int32_t, Intrinsics>> intrinsics;
+    for (uint64_t n = rc.read<uint64_t>(); n > 0; n--) {
+      const int32_t id = rc.read<int32_t>(), model = rc.read<int32_t>();
+      Intrinsics k{rc.read<uint64_t>(), rc.read<uint64_t>(), 0, 0};
+      if (model == 0) {  // SIMPLE_PINHOLE: f, cx, cy
+        k.fx = k.fy = rc.read<double>();
+        rc.skip(16);
+      } else if (model == 1) {  // PINHOLE: fx, fy, cx, cy
+        k.fx = rc.read<double>();
+        k.fy = rc.read<double>();
+        rc.skip(16);
+      } else {
+        invalid("Camera " + std::to_string(id) + " uses COLMAP model " + std::to_string(model) +
+                "; only undistorted pinhole cameras are supported.");
+      }
+      intrinsics.emplace_back(id, k);
+    }
+    // images.bin: each photo's pose, world to camera x_c = R x_w + t (src/colmap.ts:96-109)
+    ColmapReader ri{static_cast<const uint8_t*>(images_bin), images_bytes, 0, "images.bin"};
+    std::vector<dio_colmap_camera> photos;
+    for (uint64_t n = ri.read<uint64_t>(); n > 0; n--) {
+      ri.read<int32_t>();  // image id
+      double q[4], t[3];
+      for (double& v : q) v = ri.read<double>();  // (w, x, y, z)
+      for (double& v : t) v = ri.read<double>();
+      const int32_t camera_id = ri.read<int32_t>();
+      const std::string name = ri.text();
+      const uint64_t points2d = ri.read<uint64_t>();
+      if (points2d > ri.size / 24) invalid("The COLMAP file images.bin is shorter than its contents say.");
+      ri.skip(points2d * 24);
+      const Intrinsics* k = nullptr;
+      for (const auto& c : intrinsics)
+        if (c.first == camera_id) k = &c.second;
+      if (!k) invalid("Photo " + name + " uses camera " + std::to_string(camera_id) + ", which isn't in cameras.bin.");
+      if (name.size() >= sizeof(dio_colmap_camera::name))
+        invalid("Photo name " + name + " is longer than " + std::to_string(sizeof(dio_colmap_camera::name) - 1) +
+                " bytes.");
+      dio_colmap_camera c{};
+      std::memcpy(c.name, name.data(), name.size());
+      c.width = static_cast<uint32_t>(k->width);
+      c.height = static_cast<uint32_t>(k->height);
+      c.fx = k->fx;
+      c.fy = k->fy;
+      // R of the (w, x, y, z) quaternion; its rows are the camera's axes in world
+      // space (x right, y down, z forward) and the centre is -R^T t.
+      const double w = q[0], x = q[1], y = q[2], z = q[3];
+      const double r[3][3] = {{1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)},
+                              {2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)},
+                              {2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)}};
+      for (int col = 0; col < 3; col++) {
+        c.position[col] = -(r[0][col] * t[0] + r[1][col] * t[1] + r[2][col] * t[2]);
+        c.right[col] = r[0][col];
+        c.down[col] = r[1][col];
+        c.forward[col] = r[2][col];
+      }
+      photos.push_back(c);
+    }
+    // sorted by file name, as the reference (src/colmap.ts:31)
+    std::stable_sort(photos.begin(), photos.end(), [](const dio_colmap_camera& a, const dio_colmap_camera& b) {
+      return std::strcmp(a.name, b.name) < 0;
+    });
+    // points3D.bin: xyz as float32 and rgb (src/colmap.ts:111-124)
+    ColmapReader rp{static_cast<const uint8_t*>(points_bin), points_bytes, 0, "points3D.bin"};
+    const uint64_t points = rp.read<uint64_t>();
+    if (points > rp.size / 43) invalid("The COLMAP file points3D.bin is shorter than its contents say.");
+    *camera_count = static_cast<uint32_t>(photos.size());
+    *point_count = static_cast<uint32_t>(points);
+    if (!cameras && !positions && !colors) return;  // counts only
+    if (cameras) {
+      if (camera_capacity < photos.size()) invalid("cameras holds fewer than the reconstruction's photos");
+      std::copy(photos.begin(), photos.end(), cameras);
+    }
+    if ((positions || colors) && point_capacity < points) invalid("positions / colors hold fewer than its points");
+    for (uint64_t i = 0; i < points; i++) {
+      rp.read<uint64_t>();  // point id
+      for (int k = 0; k < 3; k++) {
+        const float v = static_cast<float>(rp.read<double>());
+        if (positions) positions[3 * i + k] = v;
+      }
+      for (int k = 0; k < 3; k++) {
+        const uint8_t v = rp.read<uint8_t>();
+        if (colors) colors[3 * i + k] = v;
+      }
+      rp.read<double>();  // reprojection error
+      const uint64_t track = rp.read<uint64_t>();
+      if (track > rp.size / 8) invalid("The COLMAP file points3D.bin is shorter than its contents say.");
+      rp.skip(track * 8);
+    }
+  });
+}
+
+extern "C" int dio_colmap_training_camera(const dio_colmap_camera* camera, uint32_t width, uint32_t height,
+                                          dio_camera* out) {
+  return guard([&] {
+    need(camera, "camera");
+    need(out, "out");
+    if (width == 0 || height == 0) invalid("the viewport is empty");
+    if (!(camera->fy > 0) || camera->height == 0) invalid("the camera has no focal length");
+    const double* p = camera->position;
+    const double up[3] = {-camera->down[0], -camera->down[1], -camera->down[2]};
+    const double back[3] = {-camera->forward[0], -camera->forward[1], -camera->forward[2]};
+    auto dot = [p](const double* a) { return a[0] * p[0] + a[1] * p[1] + a[2] * p[2]; };
+    // cameraView (src/sceneCameras.ts:54-67): x right, y up, looking down -z; column-major
+    const double view[16] = {camera->right[0], up[0], back[0], 0,
+                             camera->right[1], up[1], back[1], 0,
+                             camera->right[2], up[2], back[2], 0,
+                             -dot(camera->right), -dot(up), -dot(back), 1};
+    // perspective(verticalFov, aspect, 0.01, 1000) (src/loadDataset.ts:80-86, src/mat4.ts:39-48),
+    // WebGPU's clip depth 0..1
+    const double fov = 2 * std::atan(camera->height / (2 * camera->fy));
+    const double f = 1 / std::tan(fov / 2), aspect = static_cast<double>(width) / height;
+    const double near = 0.01, far = 1000, depth = 1 / (near - far);
+    const double proj[16] = {f / aspect, 0, 0, 0, 0, f, 0, 0, 0, 0, far * depth, -1, 0, 0, near * far * depth, 0};
+    for (int k = 0; k < 16; k++) {
+      out->view[k] = static_cast<float>(view[k]);
+      out->proj[k] = static_cast<float>(proj[k]);
+    }
+    out->viewport[0] = static_cast<float>(width);
+    out->viewport[1] = static_cast<float>(height);
   });
 }

@@ -834,6 +834,104 @@ FN(initial_gaussians) {
     return NULL;
   return undefined(env);
 }
+/* ------------------------------------------------------------- COLMAP */
+
+static napi_value double_list(napi_env env, const double* v, uint32_t n) {
+  napi_value arr;
+  napi_create_array_with_length(env, n, &arr);
+  for (uint32_t k = 0; k < n; k++) napi_set_element(env, arr, k, number(env, v[k]));
+  return arr;
+}
+
+static int doubles(call_t* c, napi_value arr, double* out, uint32_t n) {
+  for (uint32_t k = 0; k < n; k++) {
+    napi_value e;
+    if (napi_get_element(c->env, arr, k, &e) != napi_ok) return 0;
+    if (napi_get_value_double(c->env, e, &out[k]) != napi_ok) return 0;
+  }
+  return 1;
+}
+
+/* parseColmap: (camerasBin, imagesBin, pointsBin) -> {cameras, positions, colors} */
+FN(colmap_parse) {
+  ARGS;
+  void *cb = NULL, *ib = NULL, *pb = NULL;
+  size_t cn = 0, in = 0, pn = 0;
+  if (!bytes_of(&c, 0, &cb, &cn) || !bytes_of(&c, 1, &ib, &in) || !bytes_of(&c, 2, &pb, &pn))
+    return fail(env, "parseColmap needs three ArrayBuffers");
+  uint32_t ncam = 0, npts = 0;
+  if (!ok(env, dio_colmap_parse(cb, cn, ib, in, pb, pn, NULL, 0, &ncam, NULL, NULL, 0, &npts))) return NULL;
+  dio_colmap_camera* cams = (dio_colmap_camera*)calloc(ncam ? ncam : 1, sizeof(dio_colmap_camera));
+  void *pos_data = NULL, *col_data = NULL;
+  napi_value pos_ab, col_ab, positions, colors, result = NULL;
+  napi_create_arraybuffer(env, (size_t)npts * 12, &pos_data, &pos_ab);
+  napi_create_arraybuffer(env, (size_t)npts * 3, &col_data, &col_ab);
+  if (ok(env, dio_colmap_parse(cb, cn, ib, in, pb, pn, cams, ncam, &ncam, (float*)pos_data, (uint8_t*)col_data, npts,
+                               &npts))) {
+    napi_create_typedarray(env, napi_float32_array, (size_t)npts * 3, pos_ab, 0, &positions);
+    napi_create_typedarray(env, napi_uint8_array, (size_t)npts * 3, col_ab, 0, &colors);
+    napi_value list;
+    napi_create_array_with_length(env, ncam, &list);
+    for (uint32_t k = 0; k < ncam; k++) {
+      napi_value o, name;
+      napi_create_object(env, &o);
+      napi_create_string_utf8(env, cams[k].name, NAPI_AUTO_LENGTH, &name);
+      napi_set_named_property(env, o, "name", name);
+      napi_set_named_property(env, o, "width", number(env, cams[k].width));
+      napi_set_named_property(env, o, "height", number(env, cams[k].height));
+      napi_set_named_property(env, o, "position", double_list(env, cams[k].position, 3));
+      napi_set_named_property(env, o, "right", double_list(env, cams[k].right, 3));
+      napi_set_named_property(env, o, "down", double_list(env, cams[k].down, 3));
+      napi_set_named_property(env, o, "forward", double_list(env, cams[k].forward, 3));
+      napi_set_named_property(env, o, "fx", number(env, cams[k].fx));
+      napi_set_named_property(env, o, "fy", number(env, cams[k].fy));
+      napi_set_element(env, list, k, o);
+    }
+    napi_create_object(env, &result);
+    napi_set_named_property(env, result, "cameras", list);
+    napi_set_named_property(env, result, "positions", positions);
+    napi_set_named_property(env, result, "colors", colors);
+  }
+  free(cams);
+  return result;
+}
+
+/* trainingCameraData: (TrainingCamera, width, height) -> {view, proj, viewport} */
+FN(colmap_training_camera) {
+  ARGS;
+  dio_colmap_camera cam;
+  memset(&cam, 0, sizeof cam);
+  napi_value o = c.argv[0];
+  double w = 0, h = 0;
+  if (!doubles(&c, property(env, o, "position"), cam.position, 3) ||
+      !doubles(&c, property(env, o, "right"), cam.right, 3) || !doubles(&c, property(env, o, "down"), cam.down, 3) ||
+      !doubles(&c, property(env, o, "forward"), cam.forward, 3))
+    return fail(env, "a TrainingCamera needs position, right, down and forward");
+  napi_get_value_double(env, property(env, o, "fx"), &cam.fx);
+  napi_get_value_double(env, property(env, o, "fy"), &cam.fy);
+  napi_get_value_double(env, property(env, o, "width"), &w);
+  napi_get_value_double(env, property(env, o, "height"), &h);
+  cam.width = (uint32_t)w;
+  cam.height = (uint32_t)h;
+  dio_camera out;
+  if (!ok(env, dio_colmap_training_camera(&cam, u32(&c, 1), u32(&c, 2), &out))) return NULL;
+  napi_value r, view, proj, viewport;
+  napi_create_object(env, &r);
+  napi_create_array_with_length(env, 16, &view);
+  napi_create_array_with_length(env, 16, &proj);
+  napi_create_array_with_length(env, 2, &viewport);
+  for (uint32_t k = 0; k < 16; k++) {
+    napi_set_element(env, view, k, number(env, out.view[k]));
+    napi_set_element(env, proj, k, number(env, out.proj[k]));
+  }
+  napi_set_element(env, viewport, 0, number(env, out.viewport[0]));
+  napi_set_element(env, viewport, 1, number(env, out.viewport[1]));
+  napi_set_named_property(env, r, "view", view);
+  napi_set_named_property(env, r, "proj", proj);
+  napi_set_named_property(env, r, "viewport", viewport);
+  return r;
+}
+
 /* writeGaussianPly: (params, sh, count) -> ArrayBuffer holding the file. */
 FN(ply_write) {
   ARGS;
@@ -939,7 +1037,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT(mlp_destroy), EXPORT(trainer_set_densify_mode), EXPORT(trainer_samples), EXPORT(trainer_features),
       EXPORT(trainer_extra_stats), EXPORT(trainer_get_tuning), EXPORT(trainer_set_tuning), EXPORT(trainer_autotune),
       EXPORT(ply_write), EXPORT(ply_parse), EXPORT(mean_neighbor_distance2), EXPORT(initial_gaussians),
-      EXPORT(trainer_measure_psnr),
+      EXPORT(trainer_measure_psnr), EXPORT(colmap_parse), EXPORT(colmap_training_camera),
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

@@ -450,6 +450,20 @@ class DensifyMLP {
 }
 
 /**
+ * parseColmap (src/colmap.ts:18-32): {cameras (TrainingCamera objects sorted by
+ * name), positions (Float32Array xyz), colors (Uint8Array rgb)} from the three
+ * binary files' ArrayBuffers.
+ */
+function parseColmap(camerasBin, imagesBin, pointsBin) {
+  return native.colmap_parse(camerasBin, imagesBin, pointsBin);
+}
+
+/** trainingCameraData (src/loadDataset.ts:80-86): the renderer's camera for a TrainingCamera at `viewport`. */
+function trainingCameraData(camera, viewport) {
+  return native.colmap_training_camera(camera, viewport[0], viewport[1]);
+}
+
+/**
  * meanNeighborDistance2 (src/loadDataset.ts:126-149): per point (Float32Array of
  * xyz) the mean squared distance to its 3 nearest other points.
  */
@@ -568,6 +582,8 @@ module.exports = {
   parseGaussianPly,
   meanNeighborDistance2,
   initialGaussians,
+  parseColmap,
+  trainingCameraData,
   activate,
   FLOATS_PER_GAUSSIAN,
   FLOATS_PER_SPLAT,

@@ -323,6 +323,33 @@ dio_buf* dio_trainer_extra_stats(dio_trainer* t);
  * step's inputs re-run `repeats` times. */
 int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_t repeats, double* ms);
 
+/* --------------------------------------------------- COLMAP reconstructions --
+ * One registered photo of a COLMAP sparse reconstruction: the TrainingCamera of
+ * src/sceneCameras.ts:6-19 (OpenCV axes in world space: x right, y down, z
+ * forward; focal lengths in pixels). */
+typedef struct dio_colmap_camera {
+  char name[256];
+  uint32_t width, height;
+  double fx, fy;
+  double position[3];
+  double right[3], down[3], forward[3];
+} dio_colmap_camera;
+/* parseColmap (src/colmap.ts:18-129): cameras.bin, images.bin and points3D.bin
+ * (binary, little-endian; undistorted SIMPLE_PINHOLE or PINHOLE cameras) into the
+ * photos sorted by name and the points' float32 xyz and rgb bytes. *camera_count
+ * and *point_count receive the counts; with cameras, positions and colors all
+ * null nothing else is written, so a caller can size its arrays first. Unknown
+ * camera models, photos of missing cameras and short files fail with
+ * DIO_ERR_INVALID and the reader's messages. */
+int dio_colmap_parse(const void* cameras_bin, size_t cameras_bytes, const void* images_bin, size_t images_bytes,
+                     const void* points_bin, size_t points_bytes, dio_colmap_camera* cameras, uint32_t camera_capacity,
+                     uint32_t* camera_count, float* positions, uint8_t* colors, uint32_t point_capacity,
+                     uint32_t* point_count);
+/* trainingCameraData (src/loadDataset.ts:80-86): the renderer's camera for a photo
+ * rendered at width x height: cameraView (src/sceneCameras.ts:54-67) and
+ * perspective(verticalFov, width / height, 0.01, 1000). */
+int dio_colmap_training_camera(const dio_colmap_camera* camera, uint32_t width, uint32_t height, dio_camera* out);
+
 /* ---------------------------------------------------- starting Gaussians --
  * meanNeighborDistance2 (src/loadDataset.ts:126-149, nearestNeighbors.wgsl:12-44):
  * for each of `count` points (positions: count x 3 floats on the device) the

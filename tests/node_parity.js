@@ -60,12 +60,62 @@ function cpuChecks() {
     errors.push(e.message);
   }
   out.errors = errors;
+  out.colmap = colmapCheck();
   const raw = new Float32Array(16);
   raw[3] = 0;
   raw[4] = Math.log(2);
   raw[12] = 1;
   out.activated = Array.from(d.activate(raw));
   return out;
+}
+
+// A two-photo reconstruction written by hand: camera 1 SIMPLE_PINHOLE f = 500 at
+// 640 x 480; photo "b.jpg" at the identity pose moved by t = (0, 0, 2), photo
+// "a.jpg" turned 90 degrees about y; three points.
+function colmapCheck() {
+  const bytes = (n, fill) => {
+    const v = new DataView(new ArrayBuffer(n));
+    fill(v);
+    return v.buffer;
+  };
+  const cams = bytes(8 + 24 + 24, (v) => {
+    v.setBigUint64(0, 1n, true);
+    v.setInt32(8, 1, true); v.setInt32(12, 0, true);
+    v.setBigUint64(16, 640n, true); v.setBigUint64(24, 480n, true);
+    v.setFloat64(32, 500, true); v.setFloat64(40, 320, true); v.setFloat64(48, 240, true);
+  });
+  const image = (v, at, id, q, t, name) => {
+    v.setInt32(at, id, true);
+    q.forEach((x, k) => v.setFloat64(at + 4 + 8 * k, x, true));
+    t.forEach((x, k) => v.setFloat64(at + 36 + 8 * k, x, true));
+    v.setInt32(at + 60, 1, true);
+    for (let k = 0; k < name.length; k++) v.setUint8(at + 64 + k, name.charCodeAt(k));
+    v.setUint8(at + 64 + name.length, 0);
+    v.setBigUint64(at + 65 + name.length, 0n, true);
+    return at + 73 + name.length;
+  };
+  const s = Math.SQRT1_2;
+  const imgs = bytes(8 + 2 * 78, (v) => {
+    v.setBigUint64(0, 2n, true);
+    const at = image(v, 8, 1, [1, 0, 0, 0], [0, 0, 2], 'b.jpg');
+    image(v, at, 2, [s, 0, s, 0], [0, 0, 0], 'a.jpg');
+  });
+  const pts = bytes(8 + 3 * 51, (v) => {
+    v.setBigUint64(0, 3n, true);
+    for (let i = 0; i < 3; i++) {
+      const at = 8 + 51 * i;
+      v.setBigUint64(at, BigInt(i), true);
+      [i, 2 * i, -i].forEach((x, k) => v.setFloat64(at + 8 + 8 * k, x, true));
+      [10 * i, 20 * i, 30 * i].forEach((x, k) => v.setUint8(at + 32 + k, x));
+      v.setFloat64(at + 35, 0.1, true);
+      v.setBigUint64(at + 43, 0n, true);
+    }
+  });
+  const r = d.parseColmap(cams, imgs, pts);
+  const cam = r.cameras.find((c) => c.name === 'b.jpg');
+  return { names: r.cameras.map((c) => c.name), position: cam.position, forwardA: r.cameras[0].forward,
+    positions: Array.from(r.positions), colors: Array.from(r.colors),
+    view: d.trainingCameraData(cam, [640, 480]).view, fy: cam.fy };
 }
 
 async function gpuChecks() {

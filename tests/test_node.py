@@ -38,6 +38,14 @@ def test_node_module_loads_without_gpu():
     assert len(out["errors"]) == 3
     assert "ctx is null" in out["errors"][1]
     assert out["errors"][2] == "No .ply header found."
+    cm = out["colmap"]
+    assert cm["names"] == ["a.jpg", "b.jpg"] and cm["fy"] == 500
+    assert cm["position"] == [0, 0, -2]               # -R^T t with R = I, t = (0, 0, 2)
+    # photo a: 90 degrees about y, so its forward axis (R's third row) is (-1, 0, 0)
+    assert cm["forwardA"] == pytest.approx([-1, 0, 0], abs=1e-12)
+    assert cm["positions"] == [0, 0, 0, 1, 2, -1, 2, 4, -2] and cm["colors"] == [0, 0, 0, 10, 20, 30, 20, 40, 60]
+    # cameraView at the identity pose: x right, y up (= -down), z back (= -forward), eye at (0, 0, -2)
+    assert cm["view"] == pytest.approx([1, 0, 0, 0, 0, -1, 0, 0, 0, 0, -1, 0, 0, 0, -2, 1], abs=1e-6)
     a = out["activated"]
     assert a[3] == 0.5 and a[4] == pytest.approx(2.0, rel=1e-6) and a[5] == 1.0
     assert a[12] == pytest.approx(0.5 + 0.28209479177387814, rel=1e-6)
