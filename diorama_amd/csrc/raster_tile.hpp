@@ -8,19 +8,16 @@ namespace raster {
 
 // Both rasterizer kernels run one workgroup per 16x16 tile as four independent
 // wavefronts, one per 8x8 quadrant (one pixel per lane). A wave stages 64 of the
-// tile's splats at a time in registers (one per lane) and broadcasts each to the
-// whole wave with v_readlane into scalar registers, so the blend loop reads no
-// LDS and the forward pass needs no workgroup barrier at all. Each staged splat
+// tile's splats at a time (one per lane) and parks their blend constants in LDS,
+// from where each is read back by the whole wave with a uniform address (a
+// broadcast on the LDS pipe; nine v_readlane per splat took VALU issue slots),
+// and the forward pass needs no workgroup barrier at all. Each staged splat
 // also gets the bounding box of its alpha >= 1/255 ellipse; a wave only visits
 // the splats whose box meets its quadrant (a ballot mask), which changes no
 // result: outside the box every pixel would skip the splat anyway.
 
 constexpr int kChunk = kWave;
 constexpr int kGrads = 9;
-
-__device__ inline float lane_value(float v, int lane) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
-}
 
 // One splat of a chunk, staged in a lane's registers. The conic (a, b, c) is
 // kept as the coefficients of the Gaussian's exponent in log2 units,
@@ -69,6 +66,27 @@ __device__ inline bool meets(const Staged& s, float bx0, float bx1, float by0, f
   return s.x1 >= bx0 && s.x0 <= bx1 && s.y1 >= by0 && s.y0 <= by1;
 }
 
+// A staged splat's blend constants as LDS records: (mean x, mean y, ha, hb) and
+// (hc, opacity, r, g), with b apart. A wave reads one back with a uniform
+// address (a broadcast on the LDS pipe) instead of nine v_readlane on the VALU.
+struct SplatRecords {
+  float4 a[kChunk], c[kChunk];
+  float b[kChunk];
+};
+
+__device__ inline void put_record(SplatRecords& r, int slot, const Staged& s) {
+  r.a[slot] = make_float4(s.mx, s.my, s.ha, s.hb);
+  r.c[slot] = make_float4(s.hc, s.op, s.r, s.g);
+  r.b[slot] = s.b;
+}
+
+// Orders a wave's LDS writes before its later reads of other lanes' words.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // rasterize.wgsl:31-100: front-to-back blend; a wave stops once all its pixels
 // are nearly opaque.
 // One 16x16 tile (tile_x, tile_y) of a width x height frame whose pair list is
@@ -90,31 +108,33 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
   bool done = !inside;
   uint32_t last = 0;
+  __shared__ SplatRecords s_records[4][kPer];  // one slot per wave: waves never share records
   for (uint32_t round = run.x; round < run.y; round += kPer * kChunk) {
     if (__all(done)) break;
-    Staged st[kPer];
     bool hits[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; q++) {
       hits[q] = false;
       const uint32_t k = round + q * kChunk + lane;
+      Staged st{};
       if (k < run.y) {
-        stage(splats, pair_splats, k, st[q]);
-        hits[q] = meets(st[q], bx0, bx1, by0, by1);
+        stage(splats, pair_splats, k, st);
+        hits[q] = meets(st, bx0, bx1, by0, by1);
       }
+      put_record(s_records[wave][q], lane, st);
     }
+    wave_sync();
 #pragma unroll
     for (int q = 0; q < kPer; q++) {
-      const Staged& s = st[q];
+      const SplatRecords& rec = s_records[wave][q];
       const uint32_t base = round + q * kChunk;
       uint64_t mask = __ballot(hits[q]);
       while (mask) {
         const int j = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const float mx = lane_value(s.mx, j), my = lane_value(s.my, j);
-        const float ha = lane_value(s.ha, j), hb = lane_value(s.hb, j), hc = lane_value(s.hc, j);
-        const float op = lane_value(s.op, j);
-        const float sr = lane_value(s.r, j), sg = lane_value(s.g, j), sb = lane_value(s.b, j);
+        const float4 ra = rec.a[j], rc = rec.c[j];
+        const float mx = ra.x, my = ra.y, ha = ra.z, hb = ra.w;
+        const float hc = rc.x, op = rc.y, sr = rc.z, sg = rc.w, sb = rec.b[j];
         if (!done) {
           const float power2 = splat_power2(cx - mx, cy - my, ha, hb, hc);
           if (power2 <= 0.f) {
@@ -136,6 +156,7 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
         }
       }
     }
+    wave_sync();  // every lane has read this round's records before the next round writes
   }
   if (inside) {
     const float r = cr + T * bg_r, g = cg + T * bg_g, b = cb + T * bg_b;
@@ -257,6 +278,7 @@ template <int kPer>
 struct BackwardShared {
   float sums[4][kGrads][kPer * kChunk];
   float4 conic[kPer * kChunk];
+  SplatRecords records[kPer];  // the round's splats, shared by the four waves
   uint32_t index[kPer * kChunk];
   uint32_t last[4];
 };
@@ -318,14 +340,16 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
       if (wave == 0) {
         s_index[q * kChunk + lane] = sts[q].index;
         sh.conic[q * kChunk + lane] = conic;
+        put_record(sh.records[q], lane, sts[q]);
       }
 #pragma unroll
       for (int v = 0; v < kGrads; v++) s_sums[wave][v][q * kChunk + lane] = 0.f;
     }
+    __syncthreads();  // wave 0's records are read by every wave
     // Back to front: the last 64 of the round first.
 #pragma unroll
     for (int q = kPer - 1; q >= 0; q--) {
-      const Staged& s = sts[q];
+      const SplatRecords& rec = sh.records[q];
       const uint32_t base = round + q * kChunk;
       uint64_t mask = __ballot(hits[q]);
       while (mask) {
@@ -342,10 +366,9 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
             const int j = kWave - 1 - __builtin_clzll(mask);
             mask &= ~(1ull << j);
             js[g] = q * kChunk + j;
-            mx = lane_value(s.mx, j); my = lane_value(s.my, j);
-            ha = lane_value(s.ha, j); hb = lane_value(s.hb, j); hc = lane_value(s.hc, j);
-            op = lane_value(s.op, j);
-            sr = lane_value(s.r, j); sg = lane_value(s.g, j); sb = lane_value(s.b, j);
+            const float4 ra = rec.a[j], rc = rec.c[j];
+            mx = ra.x; my = ra.y; ha = ra.z; hb = ra.w;
+            hc = rc.x; op = rc.y; sr = rc.z; sg = rc.w; sb = rec.b[j];
             blended = base + j - run.x < last;
           }
           any |= replay_splat(st, blended, cx, cy, mx, my, ha, hb, hc, op, sr, sg, sb, share[g]);
