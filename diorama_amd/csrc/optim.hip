@@ -189,12 +189,14 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(uint32_t n, const float
 // passes); every Gaussian, visible or not, gets its Adam step, as the reference.
 // (Measured against variants: 32 Gaussians with the moments staged in LDS, and
 // waves 1-3 loading their moments ahead while wave 0 runs the backward, were
-// both slower.)
+// both slower. The backward phase wants 106 VGPRs, which allows 4 waves per
+// SIMD; asking for 5, which the 31 KB of LDS also allows, costs a 36-byte spill
+// in that phase and gains about 6% on the kernel.)
 constexpr uint32_t kFusedGaussians = 64;
 constexpr uint32_t kFusedRow = 35;  // 16 + 15 + 3, odd so lanes' rows start in different banks
 
 template <bool kExtra>
-__global__ __launch_bounds__(kThreads) void backward_adam_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void backward_adam_kernel(
     uint32_t n, float* __restrict__ gaussians, float* __restrict__ sh, uint32_t sh_degree, CameraArgs cam,
     const float4* __restrict__ splats, const float* __restrict__ splat_grads, float* __restrict__ params,
     float2* __restrict__ moments, AdamArgs a, float4* __restrict__ stats, float4* __restrict__ extra) {
