@@ -278,7 +278,8 @@ template <int kPer>
 struct BackwardShared {
   float sums[4][kGrads][kPer * kChunk];
   float4 conic[kPer * kChunk];
-  SplatRecords records[kPer];  // the round's splats, shared by the four waves
+  float4 box[kPer * kChunk];   // each splat's alpha >= 1/255 box (x0, x1, y0, y1)
+  SplatRecords records[kPer];  // the round's splats, staged once by wave 0 for all four
   uint32_t index[kPer * kChunk];
   uint32_t last[4];
 };
@@ -325,27 +326,32 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
 
   for (int c = static_cast<int>(ceil_div(tile_last, kRound)) - 1; c >= 0; c--) {
     const uint32_t round = run.x + static_cast<uint32_t>(c) * kRound;
-    Staged sts[kPer];
+    // Wave 0 stages the round's splats for all four waves.
+    if (wave == 0) {
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const uint32_t k = round + q * kChunk + lane;
+        Staged st{};
+        float4 conic = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (k < run.y) stage(splats, pair_splats, k, st, &conic);
+        s_index[q * kChunk + lane] = st.index;
+        sh.conic[q * kChunk + lane] = conic;
+        sh.box[q * kChunk + lane] = make_float4(st.x0, st.x1, st.y0, st.y1);
+        put_record(sh.records[q], lane, st);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; q++)
+#pragma unroll
+      for (int v = 0; v < kGrads; v++) s_sums[wave][v][q * kChunk + lane] = 0.f;
+    __syncthreads();  // wave 0's records are read by every wave
     bool hits[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; q++) {
       const uint32_t k = round + q * kChunk + lane;
-      hits[q] = false;
-      sts[q].index = 0;
-      float4 conic = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (k < run.y) {
-        stage(splats, pair_splats, k, sts[q], &conic);
-        hits[q] = k - run.x < my_last && meets(sts[q], bx0, bx1, by0, by1);
-      }
-      if (wave == 0) {
-        s_index[q * kChunk + lane] = sts[q].index;
-        sh.conic[q * kChunk + lane] = conic;
-        put_record(sh.records[q], lane, sts[q]);
-      }
-#pragma unroll
-      for (int v = 0; v < kGrads; v++) s_sums[wave][v][q * kChunk + lane] = 0.f;
+      const float4 b = sh.box[q * kChunk + lane];
+      hits[q] = k < run.y && k - run.x < my_last && b.y >= bx0 && b.x <= bx1 && b.w >= by0 && b.z <= by1;
     }
-    __syncthreads();  // wave 0's records are read by every wave
     // Back to front: the last 64 of the round first.
 #pragma unroll
     for (int q = kPer - 1; q >= 0; q--) {
