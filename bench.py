@@ -8,9 +8,12 @@ with N(0, 0.01) pixel noise. The student is a seed-0 set of the same size and
 distribution in the 10-unit box, trained from iteration 15,000 (past the
 densification window, SH degree 3), so every timed step is a full optimizer step.
 
-Prints one JSON line. `roofline` is the dominant kernel's algorithmic bytes over
-its mean launch time (HIP events on the library's stream); `cpu_baseline` times
-the float64 CPU restatement of the same step (oracle/, OpenMP) on a bounded sample.
+Prints one JSON line. `value` is steps/s over the timed steps. The same number of
+steps then runs again with the trainer's stage clock on: HIP events recorded
+between the step's launches on the library's stream give each stage's mean time
+inside real steps (`stages_us`), and `roofline` is the dominant kernel's
+algorithmic bytes over that time. `cpu_baseline` times the float64 CPU
+restatement of the same step (oracle/, OpenMP) on a bounded sample.
 """
 import argparse
 import json
@@ -30,20 +33,25 @@ TRAIN_VIEWS, HELD_OUT = 263, 33
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per unit (useful bytes only): per Gaussian, per (splat, tile)
-# pair, per pixel. Sort is charged at the reference's 6 x 8-bit pass model over the
-# 48 significant key bits.
+# pair, per pixel, per tile. Sort (the depth sort and the tile sort together) is
+# charged at the reference's 6 x 8-bit pass model over the 48 significant key bits.
 STAGE_BYTES = {
-    "preprocess": (284, 0, 0),
-    "scan": (8, 0, 0),
-    "sort": (0, 144, 0),
-    "rasterize": (0, 40, 24),
-    "loss": (0, 0, 32),
-    "rasterize_backward": (36, 40, 32),
+    "preprocess": (284, 0, 0, 0),
+    "scan": (8, 0, 0, 0),
+    "sort": (0, 144, 0, 0),
+    "duplicate": (20, 12, 0, 0),
+    "ranges": (0, 4, 0, 8),
+    "rasterize": (0, 40, 24, 0),
+    "loss": (0, 0, 32, 0),
+    "clear_grads": (36, 0, 0, 0),
+    "rasterize_backward": (36, 40, 32, 0),
     # preprocess backward + statistics + Adam in one pass: R activated 56, raw 56, SH 180,
     # splat 4 + 36 grads, moments 472, stats 16; W raw 56, SH 180, moments 472,
     # activated 56, stats 16 (the reference's separate passes move 512 + 44 + 1708)
-    "backward_adam": (1600, 0, 0),
+    "backward_adam": (1600, 0, 0, 0),
 }
+# Clocked stages summed into a reported one.
+STAGE_PARTS = {"sort": ("depth_sort", "tile_sort")}
 # The reference's per-step byte model (separate preprocess backward, stats and Adam).
 STEP_BYTES = (2612, 240, 88)
 # Stage name -> kernel name, for the PMC traffic and VALU lines.
@@ -514,7 +522,6 @@ def main():
     ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
-    ap.add_argument("--repeats", type=int, default=20, help="launches per stage timing")
     ap.add_argument("--workload", choices=["train", "scan-sort", "batch", "learned"], default="train",
                     help="train: the training step (default); scan-sort: the scan/sort microbenchmark; "
                          "batch: the multi-view batch trainer; learned: learned vs plain densification")
@@ -570,19 +577,32 @@ def main():
     steps_per_s = args.steps / elapsed
     log(f"timed {args.steps} steps: {elapsed * 1000 / args.steps:.3f} ms/step")
 
-    renderer = trainer.renderer
-    tiles = renderer.tiles
-    tiles.after_submit()
-    pairs = tiles.last_pair_count
+    # The same steps again with the stage clock on, for the per-stage times inside
+    # real steps; P is the mean of those steps' own pair counts.
+    trainer.stage_clock(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        trainer.step()
+    ctx.synchronize()
+    clocked_elapsed = time.perf_counter() - t0
+    clock = trainer.stage_totals()
+    trainer.stage_clock(False)
+    totals, clocked = clock["ms"], clock["steps"]
+    if clocked != args.steps:
+        raise RuntimeError(f"the stage clock counted {clocked} of {args.steps} steps")
+    log(f"clocked {clocked} steps: {clocked_elapsed * 1000 / clocked:.3f} ms/step")
+    pairs = int(round(clock["pairs"] / clocked))
     n = trainer.count
     x = WIDTH * HEIGHT
+    n_tiles = math.ceil(WIDTH / 16) * math.ceil(HEIGHT / 16)
     stages = {}
-    for name, (bg, bp, bx) in STAGE_BYTES.items():
-        ms = trainer.time_stage(name, args.repeats)
-        nbytes = bg * n + bp * pairs + bx * x
+    for name, (bg, bp, bx, bt) in STAGE_BYTES.items():
+        ms = sum(totals[p] for p in STAGE_PARTS.get(name, (name,))) / clocked
+        nbytes = bg * n + bp * pairs + bx * x + bt * n_tiles
         stages[name] = {"ms": ms, "bytes": nbytes, "GB/s": nbytes / (ms * 1e-3) / 1e9}
         log(f"{name:20s} {ms * 1000:9.1f} us  {stages[name]['GB/s']:8.1f} GB/s algorithmic")
-    dominant = max(stages, key=lambda k: stages[k]["ms"])
+    step_gpu_ms = sum(totals.values()) / clocked
+    dominant = max(STAGE_KERNEL, key=lambda k: stages[k]["ms"])
     d = stages[dominant]
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(d["GB/s"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(d["GB/s"] / HBM_PEAK_GBS, 4),
@@ -625,6 +645,9 @@ def main():
                    "step_GBps_model": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
                    "parallelism": "single GPU"},
         "stages_us": {k: round(v["ms"] * 1000, 1) for k, v in stages.items()},
+        "stage_clock": {"steps": clocked, "ms_per_step": round(clocked_elapsed * 1000 / clocked, 4),
+                        "gpu_ms_per_step": round(step_gpu_ms, 4),
+                        "parts_us": {k: round(totals[k] / clocked * 1000, 1) for k in totals}},
         "stages_hbm_frac": {k: round(v["GB/s"] / HBM_PEAK_GBS, 4) for k, v in stages.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -178,6 +178,10 @@ SIGNATURES = {
     "dio_trainer_renderer": (_P, [_P]),
     "dio_trainer_destroy": (None, [_P]),
     "dio_trainer_time_stage": (_I, [_P, ctypes.c_char_p, _U32, ctypes.POINTER(ctypes.c_double)]),
+    "dio_trainer_stage_clock": (_I, [_P, ctypes.c_int]),
+    "dio_trainer_stage_totals": (_I, [_P, ctypes.POINTER(ctypes.c_double), _U32, ctypes.POINTER(_U32),
+                                      ctypes.POINTER(_U32), ctypes.POINTER(ctypes.c_uint64)]),
+    "dio_trainer_stage_name": (ctypes.c_char_p, [_U32]),
     "dio_trainer_measure_psnr": (_I, [_P, ctypes.POINTER(_Camera), ctypes.POINTER(_P), _U32,
                                      ctypes.POINTER(ctypes.c_double)]),
     "dio_colmap_parse": (_I, [_P, ctypes.c_size_t, _P, ctypes.c_size_t, _P, ctypes.c_size_t,
@@ -741,6 +745,22 @@ class Trainer:
         v = ctypes.c_double()
         _check(lib().dio_trainer_time_stage(self.handle, stage.encode(), repeats, ctypes.byref(v)))
         return v.value
+
+    def stage_clock(self, enable=True):
+        """Starts (or stops) summing HIP-event times per stage over the plain steps
+        that follow; either way the totals start from zero."""
+        _check(lib().dio_trainer_stage_clock(self.handle, 1 if enable else 0))
+
+    def stage_totals(self):
+        """Since the clock started: {"ms": milliseconds summed per stage name, "steps":
+        steps clocked, "pairs": their summed (splat, tile) pair counts}."""
+        stages, steps, pairs = _U32(), _U32(), ctypes.c_uint64()
+        _check(lib().dio_trainer_stage_totals(self.handle, None, 0, ctypes.byref(stages), None, None))
+        ms = (ctypes.c_double * stages.value)()
+        _check(lib().dio_trainer_stage_totals(self.handle, ms, stages.value, None, ctypes.byref(steps),
+                                              ctypes.byref(pairs)))
+        names = [lib().dio_trainer_stage_name(k).decode() for k in range(stages.value)]
+        return {"ms": dict(zip(names, ms)), "steps": steps.value, "pairs": pairs.value}
 
     @property
     def count(self):

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <memory>
 #include <new>
@@ -283,6 +284,106 @@ extern "C" void dio_radix_sort_destroy(dio_radix_sort* s) {
 
 // ============================================================ TileRasterizer
 
+namespace {
+
+// Stage boundaries of a plain training step, in launch order: event k closes stage
+// k - 1 (kStageNames) and opens stage k.
+enum StageMark : uint32_t {
+  kMarkStart, kMarkPreprocess, kMarkDepthSort, kMarkScan, kMarkDuplicate, kMarkTileSort, kMarkRanges,
+  kMarkRasterize, kMarkLoss, kMarkClearGrads, kMarkBackward, kMarkBackwardAdam, kMarks
+};
+constexpr const char* kStageNames[kMarks - 1] = {
+    "preprocess", "depth_sort", "scan", "duplicate", "tile_sort", "ranges",
+    "rasterize", "loss", "clear_grads", "rasterize_backward", "backward_adam"};
+
+// HIP events between the stages of clocked steps, summed per stage, and each step's
+// pair count. A step's records are read once they are old (kKeep steps behind) or
+// when the totals are asked for, so the clock never waits on the step being enqueued.
+struct StageClock {
+  struct Step {
+    hipEvent_t marks[kMarks] = {};
+    hipEvent_t read = nullptr;   // after the pair-count copy
+    uint32_t* pairs = nullptr;   // pinned host word
+    void init() {
+      // The stage marks only time: without the system-scope fence a mark does not
+      // write back and invalidate the caches under the next stage.
+      for (auto& e : marks) DIO_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+      DIO_HIP(hipEventCreate(&read));
+      DIO_HIP(hipHostMalloc(reinterpret_cast<void**>(&pairs), 4, hipHostMallocDefault));
+    }
+    ~Step() {
+      for (auto e : marks)
+        if (e) (void)hipEventDestroy(e);
+      if (read) (void)hipEventDestroy(read);
+      if (pairs) (void)hipHostFree(pairs);
+    }
+  };
+  static constexpr size_t kKeep = 16;
+  bool enabled = false;
+  std::deque<std::unique_ptr<Step>> pending;
+  std::vector<std::unique_ptr<Step>> spare;
+  std::unique_ptr<Step> current;
+  double totals_ms[kMarks - 1] = {};
+  uint64_t pairs = 0;
+  uint32_t steps = 0;
+
+  void mark(hipStream_t s, StageMark k) {
+    if (current) DIO_HIP(hipEventRecord(current->marks[k], s));
+  }
+  void begin(hipStream_t s) {
+    drop();  // a step that an error cut short
+    while (pending.size() >= kKeep) harvest_oldest();
+    if (!spare.empty()) {
+      current = std::move(spare.back());
+      spare.pop_back();
+    } else {
+      current = std::make_unique<Step>();
+      current->init();
+    }
+    mark(s, kMarkStart);
+  }
+  // Closes a step that ran every stage: its pair count is copied after the last mark,
+  // outside every timed stage.
+  void finish(hipStream_t s, const void* pair_total) {
+    if (!current) return;
+    DIO_HIP(hipMemcpyAsync(current->pairs, pair_total, 4, hipMemcpyDeviceToHost, s));
+    DIO_HIP(hipEventRecord(current->read, s));
+    pending.push_back(std::move(current));
+  }
+  // Drops a step that left the plain stage order (a densification step) unread.
+  void drop() {
+    if (current) spare.push_back(std::move(current));
+  }
+  void harvest_oldest() {
+    Step& m = *pending.front();
+    DIO_HIP(hipEventSynchronize(m.read));
+    for (uint32_t k = 0; k + 1 < kMarks; k++) {
+      float ms = 0.f;
+      DIO_HIP(hipEventElapsedTime(&ms, m.marks[k], m.marks[k + 1]));
+      totals_ms[k] += ms;
+    }
+    pairs += *m.pairs;
+    steps++;
+    spare.push_back(std::move(pending.front()));
+    pending.pop_front();
+  }
+  void harvest_all() {
+    while (!pending.empty()) harvest_oldest();
+  }
+  void reset() {
+    for (double& v : totals_ms) v = 0;
+    pairs = 0;
+    steps = 0;
+  }
+  void destroy() {
+    pending.clear();
+    spare.clear();
+    current.reset();
+  }
+};
+
+}  // namespace
+
 struct dio_tile_rasterizer {
   dio_ctx* ctx = nullptr;
   dio_buf* splats = nullptr;  // not owned
@@ -308,6 +409,7 @@ struct dio_tile_rasterizer {
   uint32_t frame_w = 0, frame_h = 0;
   float frame_bg[3] = {0.f, 0.f, 0.f};
   uint32_t splat_batch = 64, digit_bits = 8;  // tuning; results do not depend on them
+  StageClock* clock = nullptr;  // set by a trainer only while it runs a clocked step
 };
 
 namespace {
@@ -410,22 +512,31 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
                       dev<uint32_t>(t->depth_alt_values), nullptr, t->count, 0, 32,
                       dio::sort_workspace(t->splat_sort_ws->ptr, t->count), t->digit_bits))
     throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
+  auto mark = [&](StageMark k) {
+    if (t->clock) t->clock->mark(s, k);
+  };
+  mark(kMarkDepthSort);
   dio::scan_exclusive_gather(s, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_values),
                              dev<uint32_t>(t->offsets), nullptr, t->count, dev<uint32_t>(t->pair_total),
                              dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
+  mark(kMarkScan);
   dio::duplicate(s, splats, dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets), dev<uint32_t>(t->tile_counts),
                  t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total), dev<uint32_t>(t->pair_tiles),
                  dev<uint32_t>(t->pair_splats));
+  mark(kMarkDuplicate);
   t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats), t->alt_tiles->ptr,
                                      dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total), t->capacity, 0,
                                      tile_sort_bits(tiles, t->digit_bits),
                                      dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity), t->digit_bits);
+  mark(kMarkTileSort);
   dio_buf* sorted_tiles = t->sorted_in_alt ? t->alt_tiles : t->pair_tiles;
   dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
   dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(t->pair_total), t->capacity, tiles,
                    dev<uint32_t>(t->ranges));
+  mark(kMarkRanges);
   dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), w, h, bg, dev<uint32_t>(target),
                  dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last), t->splat_batch);
+  mark(kMarkRasterize);
   t->framed = true;
   t->frame_w = w;
   t->frame_h = h;
@@ -460,10 +571,12 @@ void tiles_after_submit(dio_tile_rasterizer* t, bool wait) {
 void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pixel_grads, dio_buf* splat_grads) {
   hipStream_t s = t->ctx->stream;
   dio::fill_u32(s, dev<uint32_t>(splat_grads), 0u, static_cast<size_t>(t->count) * 9);  // the pass adds into it
+  if (t->clock) t->clock->mark(s, kMarkClearGrads);
   dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
   dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), w, h,
                           t->frame_bg, dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
                           dev<float>(pixel_grads), dev<float>(splat_grads), t->splat_batch);
+  if (t->clock) t->clock->mark(s, kMarkBackward);
 }
 
 }  // namespace
@@ -580,6 +693,7 @@ void renderer_preprocess(dio_renderer* r, const dio_camera& camera) {
   dio::preprocess(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
                   dev<float>(sc->splats), dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
                   dev<uint32_t>(t->depth_values));
+  if (t->clock) t->clock->mark(r->ctx->stream, kMarkPreprocess);
   r->preprocessed = true;
 }
 
@@ -947,6 +1061,7 @@ struct dio_trainer {
   uint32_t generation = 0;      // bumped whenever densification replaces the buffers
   LearnedState learned;
   dio_tuning tuning{16, 64, 8};
+  StageClock clock;
 };
 
 namespace {
@@ -981,6 +1096,7 @@ void batch_destroy(BatchState* b) {
 void destroy_trainer(dio_trainer* t) {
   if (!t) return;
   (void)hipStreamSynchronize(t->ctx->stream);
+  t->clock.destroy();
   batch_destroy(t->batch);
   for (dio_buf** p : {&t->learned.extra, &t->learned.features, &t->learned.logits, &t->learned.pending,
                       &t->learned.pending_count, &t->learned.store_features, &t->learned.store_labels,
@@ -1023,13 +1139,19 @@ void trainer_optimize(dio_trainer* t) {
   const bool densify_now = densifying && step > st.densify_from && st.densify_interval &&
                            step % st.densify_interval == 0;
   const auto& view = next_view(t);
-  t->renderer->scene->rasterizer->splat_batch = t->tuning.splat_batch;
-  t->renderer->scene->rasterizer->digit_bits = t->tuning.digit_bits;
+  dio_tile_rasterizer* tiles = t->renderer->scene->rasterizer;
+  tiles->splat_batch = t->tuning.splat_batch;
+  tiles->digit_bits = t->tuning.digit_bits;
+  if (t->clock.enabled) {
+    t->clock.begin(t->ctx->stream);
+    tiles->clock = &t->clock;
+  }
   renderer_preprocess(t->renderer, view.camera);
   renderer_tiles(t->renderer, nullptr, st.background);
-  dio::training_loss(t->ctx->stream, t->width, t->height, dev<float>(t->renderer->scene->rasterizer->pixel_color),
+  dio::training_loss(t->ctx->stream, t->width, t->height, dev<float>(tiles->pixel_color),
                      dev<uint32_t>(view.image), dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
                      dev<float>(t->loss->pixel_loss));
+  if (tiles->clock) tiles->clock->mark(t->ctx->stream, kMarkLoss);
   if (!densify_now) t->adam_steps++;
   const double progress = std::min(1.0, static_cast<double>(step) / st.position_lr_steps);
   const double position_lr =
@@ -1063,6 +1185,14 @@ void trainer_optimize(dio_trainer* t) {
                        t->renderer->camera, dev<float>(sc->splats), dev<float>(sc->grad_splats),
                        dev<float>(t->params), dev<float>(t->moments), a, dev<float>(t->stats),
                        dev<float>(t->learned.extra));
+    if (tiles->clock) tiles->clock->mark(t->ctx->stream, kMarkBackwardAdam);
+  }
+  if (tiles->clock) {
+    if (densify_now)
+      t->clock.drop();
+    else
+      t->clock.finish(t->ctx->stream, tiles->pair_total->ptr);
+    tiles->clock = nullptr;
   }
   tiles_after_submit(sc->rasterizer, false);
   t->due_densify = densify_now;
@@ -1951,6 +2081,35 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
     (void)hipEventDestroy(stop);
     *ms = elapsed / repeats;
   });
+}
+
+extern "C" int dio_trainer_stage_clock(dio_trainer* t, int enable) {
+  return guard([&] {
+    need(t, "trainer");
+    if (t->batch) invalid("the stage clock times the plain trainer's step; a batch trainer has time_stage(\"batch_step\")");
+    t->clock.harvest_all();
+    t->clock.reset();
+    t->clock.enabled = enable != 0;
+  });
+}
+
+extern "C" int dio_trainer_stage_totals(dio_trainer* t, double* ms, uint32_t capacity, uint32_t* stages,
+                                        uint32_t* steps, uint64_t* pairs) {
+  return guard([&] {
+    need(t, "trainer");
+    if (ms && capacity < kMarks - 1) invalid("ms holds " + std::to_string(capacity) + " stages; the step has " +
+                                             std::to_string(kMarks - 1));
+    t->clock.harvest_all();
+    if (ms)
+      for (uint32_t k = 0; k + 1 < kMarks; k++) ms[k] = t->clock.totals_ms[k];
+    if (stages) *stages = kMarks - 1;
+    if (steps) *steps = t->clock.steps;
+    if (pairs) *pairs = t->clock.pairs;
+  });
+}
+
+extern "C" const char* dio_trainer_stage_name(uint32_t stage) {
+  return stage + 1 < kMarks ? kStageNames[stage] : nullptr;
 }
 
 // ================================================== learned densification

@@ -323,6 +323,28 @@ dio_buf* dio_trainer_extra_stats(dio_trainer* t);
  * step's inputs re-run `repeats` times. */
 int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_t repeats, double* ms);
 
+/* Stage clock of the plain trainer. The reference times no stage: it requests
+ * 'timestamp-query' (src/main.ts:40) but records none, and its checks wall-clock
+ * whole submissions with performance.now() (e.g. src/checks/backward.ts:160-163).
+ * While enabled, every plain optimize() step records HIP
+ * events between its launches on the context's stream, and their elapsed times are
+ * summed per stage: the stages run in their real order, on a real step's data.
+ * Densification steps are not counted. Enabling or disabling clears the totals;
+ * a batch trainer is an error. */
+int dio_trainer_stage_clock(dio_trainer* t, int enable);
+/* Waits for the clocked steps and returns the summed milliseconds per stage in
+ * the order of dio_trainer_stage_name (ms may be NULL; otherwise it holds
+ * `capacity` >= *stages values), the number of stages, of steps clocked, and
+ * their summed pair counts (each step's count is copied out after its last
+ * stage). Any output pointer may be NULL. */
+int dio_trainer_stage_totals(dio_trainer* t, double* ms, uint32_t capacity, uint32_t* stages, uint32_t* steps,
+                             uint64_t* pairs);
+/* Name of stage k of a clocked step, or NULL past the last: "preprocess",
+ * "depth_sort", "scan", "duplicate", "tile_sort", "ranges", "rasterize", "loss"
+ * (with the pair-count readback copy), "clear_grads", "rasterize_backward",
+ * "backward_adam". */
+const char* dio_trainer_stage_name(uint32_t stage);
+
 /* --------------------------------------------------- COLMAP reconstructions --
  * One registered photo of a COLMAP sparse reconstruction: the TrainingCamera of
  * src/sceneCameras.ts:6-19 (OpenCV axes in world space: x right, y down, z

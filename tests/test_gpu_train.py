@@ -205,3 +205,40 @@ def test_training_learns_and_densifying_helps():
     _, dens, count, _ = train(sparse.copy(), cams, images, densify_from=100, densify_interval=50, densify_until=351)
     print(f"densify: {dens:.1f} dB with {count} Gaussians vs {plain:.1f} dB without")
     assert count > 12 and dens > plain + 1
+
+
+STAGES = ["preprocess", "depth_sort", "scan", "duplicate", "tile_sort", "ranges", "rasterize", "loss",
+          "clear_grads", "rasterize_backward", "backward_adam"]
+
+
+def test_stage_clock_times_plain_steps():
+    """The stage clock: one entry per stage of every plain step, each step's own pair
+    count, densification steps left out, and the totals start over when re-enabled."""
+    params, sh, g, cam = config0_scene(seed=4, n=4096)
+    images = render_views(g, [cam], sh, 3)
+    ctx = context()
+    settings = diorama_amd.default_settings(1.0)
+    settings.update(densify_from=0, densify_interval=4, densify_until=10)
+    say("stage clock")
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], images, settings)
+    tiles = t.renderer.tiles
+    t.stage_clock(True)
+    pairs = 0
+    for _ in range(3):  # steps 1-3: plain
+        t.step()
+        tiles.after_submit()
+        pairs += tiles.last_pair_count
+    clock = t.stage_totals()
+    assert list(clock["ms"]) == STAGES
+    assert clock["steps"] == 3 and clock["pairs"] == pairs > 0
+    assert all(ms > 0 for ms in clock["ms"].values())
+    t.step()  # step 4 densifies: not clocked
+    assert t.stage_totals()["steps"] == 3
+    t.stage_clock(True)
+    assert t.stage_totals()["steps"] == 0 and t.stage_totals()["pairs"] == 0
+    t.step()
+    assert t.stage_totals()["steps"] == 1
+    t.stage_clock(False)
+    t.step()
+    assert t.stage_totals()["steps"] == 0
+    t.destroy()
