@@ -195,6 +195,17 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(uint32_t n, const float
 constexpr uint32_t kFusedGaussians = 64;
 constexpr uint32_t kFusedRow = 35;  // 16 + 15 + 3, odd so lanes' rows start in different banks
 
+// A full block's four inputs as one run of float4s (SH, raw, activated, splat
+// gradients), staged in kStageRounds rounds of the workgroup's threads with every
+// load issued before the first LDS store. In the last round the lanes past the run
+// wrap to its start and store a value that round 0 also stores: same data, same
+// address. No load or store is conditional, so none can be sunk into a branch.
+constexpr uint32_t kStageSh = kFusedGaussians * kShRest / 4, kStageRaw = kStageSh + kFusedGaussians * 4,
+                   kStageAct = kStageRaw + kFusedGaussians * 4, kStageVec = kStageAct + kFusedGaussians * 9 / 4;
+constexpr uint32_t kStageRounds = (kStageVec + kThreads - 1) / kThreads;
+static_assert(kFusedGaussians * kShRest % 4 == 0 && kFusedGaussians * 9 % 4 == 0, "whole float4s per block");
+static_assert(kStageRounds * kThreads - kStageVec <= kStageVec, "the wrapped lanes stay inside the run");
+
 template <bool kExtra>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void backward_adam_kernel(
     uint32_t n, float* __restrict__ gaussians, float* __restrict__ sh, uint32_t sh_degree, CameraArgs cam,
@@ -205,19 +216,59 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   __shared__ __align__(16) float s_act[kFusedGaussians * kFloatsPerGaussian];
   __shared__ __align__(16) float s_sg[kFusedGaussians * 9];
   __shared__ float s_rows[kFusedGaussians * kFusedRow];
+  __shared__ float2 s_splat[kFusedGaussians];  // depth, radius
+  __shared__ float4 s_stats[kFusedGaussians];
+  __shared__ float4 s_extra[kExtra ? kFusedGaussians : 1];
   const uint32_t first = blockIdx.x * kFusedGaussians;
   const uint32_t count = min(kFusedGaussians, n - first);
-  copy_in(s_sh, sh + static_cast<size_t>(first) * kShRest, count * kShRest);
-  copy_in(s_raw, params + static_cast<size_t>(first) * kFloatsPerGaussian, count * kFloatsPerGaussian);
-  copy_in(s_act, gaussians + static_cast<size_t>(first) * kFloatsPerGaussian, count * kFloatsPerGaussian);
-  copy_in(s_sg, splat_grads + static_cast<size_t>(first) * 9, count * 9);
+  // Stage the block's inputs with all loads in flight together; wave 1 fetches the
+  // splats' depth and radius, wave 2 the statistics, wave 3 the extra statistics.
+  const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t mine = first + min(lane, count - 1);
+  float4 side = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (wave == 1) side = splats[static_cast<size_t>(mine) * 3];
+  if (wave == 2) side = stats[mine];
+  if (kExtra && wave == 3) side = extra[mine];
+  if (count == kFusedGaussians) {
+    const float4* src_sh = reinterpret_cast<const float4*>(sh + static_cast<size_t>(first) * kShRest);
+    const float4* src_raw = reinterpret_cast<const float4*>(params + static_cast<size_t>(first) * kFloatsPerGaussian);
+    const float4* src_act =
+        reinterpret_cast<const float4*>(gaussians + static_cast<size_t>(first) * kFloatsPerGaussian);
+    const float4* src_sg = reinterpret_cast<const float4*>(splat_grads + static_cast<size_t>(first) * 9);
+    float4 v[kStageRounds];
+    float4* dst[kStageRounds];
+#pragma unroll
+    for (uint32_t k = 0; k < kStageRounds; k++) {
+      uint32_t q = threadIdx.x + k * kThreads;
+      if (k + 1 == kStageRounds) q = q < kStageVec ? q : q - kStageVec;
+      const float4* src = q < kStageSh    ? src_sh + q
+                          : q < kStageRaw ? src_raw + (q - kStageSh)
+                          : q < kStageAct ? src_act + (q - kStageRaw)
+                                          : src_sg + (q - kStageAct);
+      dst[k] = q < kStageSh    ? reinterpret_cast<float4*>(s_sh) + q
+               : q < kStageRaw ? reinterpret_cast<float4*>(s_raw) + (q - kStageSh)
+               : q < kStageAct ? reinterpret_cast<float4*>(s_act) + (q - kStageRaw)
+                               : reinterpret_cast<float4*>(s_sg) + (q - kStageAct);
+      v[k] = *src;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kStageRounds; k++) *dst[k] = v[k];
+  } else {
+    copy_in(s_sh, sh + static_cast<size_t>(first) * kShRest, count * kShRest);
+    copy_in(s_raw, params + static_cast<size_t>(first) * kFloatsPerGaussian, count * kFloatsPerGaussian);
+    copy_in(s_act, gaussians + static_cast<size_t>(first) * kFloatsPerGaussian, count * kFloatsPerGaussian);
+    copy_in(s_sg, splat_grads + static_cast<size_t>(first) * 9, count * 9);
+  }
+  if (wave == 1) s_splat[lane] = make_float2(side.z, side.w);
+  if (wave == 2) s_stats[lane] = side;
+  if (kExtra && wave == 3) s_extra[lane] = side;
   __syncthreads();
   if (threadIdx.x < kFusedGaussians) {
     const uint32_t t = threadIdx.x, i = first + t;
     float* row = s_rows + t * kFusedRow;
     for (uint32_t k = 0; k < kFusedRow; k++) row[k] = 0.f;
-    const float4 s0 = t < count ? splats[static_cast<size_t>(i) * 3] : make_float4(0.f, 0.f, 0.f, 0.f);
-    if (s0.w != 0.f) {
+    const float2 splat = t < count ? s_splat[t] : make_float2(0.f, 0.f);
+    if (splat.y != 0.f) {
       const int used = sh_count(sh_degree);
       float g[16], gr[9];
 #pragma unroll
@@ -240,37 +291,48 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
       // the statistics, as stats_kernel
       const float gx = gr[0] * a.viewport[0] * 0.5f, gy = gr[1] * a.viewport[1] * 0.5f;
       const float length = sqrtf(gx * gx + gy * gy);
-      const float4 st = stats[i];
-      stats[i] = make_float4(st.x + length, st.y + 1.f, fmaxf(st.z, s0.w), 0.f);
+      const float4 st = s_stats[t];
+      stats[i] = make_float4(st.x + length, st.y + 1.f, fmaxf(st.z, splat.y), 0.f);
       if (kExtra) {
-        const float4 e = extra[i];
-        extra[i] = make_float4(fmaxf(e.x, length), e.y + s0.z, e.z + gx, e.w + gy);
+        const float4 e = s_extra[t];
+        extra[i] = make_float4(fmaxf(e.x, length), e.y + splat.x, e.z + gx, e.w + gy);
       }
     }
   }
   __syncthreads();
+  // Adam over every (Gaussian, slot): a thread's moments are all loaded before the
+  // first update. The padding slots (7: kind, 15: birth stamp) are never trained; their
+  // moments go back unchanged so that every written line is whole.
+  constexpr uint32_t kRounds = (kFusedGaussians * kMomentSlots + kThreads - 1) / kThreads;
+  const uint32_t total = count * kMomentSlots;
   float2* m_base = moments + static_cast<size_t>(first) * kMomentSlots;
+  float2 m[kRounds];
+#pragma unroll
+  for (uint32_t k = 0; k < kRounds; k++) m[k] = m_base[min(threadIdx.x + k * kThreads, total - 1)];
   SlotWalk w(threadIdx.x);
-  for (uint32_t f = threadIdx.x; f < count * kMomentSlots; f += kThreads, w.next()) {
+#pragma unroll
+  for (uint32_t k = 0; k < kRounds; k++, w.next()) {
+    const uint32_t f = threadIdx.x + k * kThreads;
+    if (f >= total) break;
     const uint32_t il = w.il, slot = w.slot;
-    if (slot == 7 || slot == 15) continue;  // padding: kind and birth stamps, never trained
-    const float* r = s_rows + il * kFusedRow;
-    float* value;
-    float grad;
-    if (slot < kFloatsPerGaussian) {
-      value = s_raw + il * kFloatsPerGaussian + slot;
-      grad = r[slot];
-    } else {
-      const uint32_t j = slot - kFloatsPerGaussian, c = (j * 43u) >> 7;  // j / 3 for j < 45
-      value = s_sh + il * kShRest + j;
-      grad = r[16 + c] * r[31 + (j - 3 * c)];
+    if (slot != 7 && slot != 15) {
+      const float* r = s_rows + il * kFusedRow;
+      float* value;
+      float grad;
+      if (slot < kFloatsPerGaussian) {
+        value = s_raw + il * kFloatsPerGaussian + slot;
+        grad = r[slot];
+      } else {
+        const uint32_t j = slot - kFloatsPerGaussian, c = (j * 43u) >> 7;  // j / 3 for j < 45
+        value = s_sh + il * kShRest + j;
+        grad = r[16 + c] * r[31 + (j - 3 * c)];
+      }
+      float act;
+      const float updated = adam_slot(slot, *value, grad, m[k], a, act);
+      *value = updated;
+      if (slot < kFloatsPerGaussian) s_act[il * kFloatsPerGaussian + slot] = act;
     }
-    float2 m = m_base[f];
-    float act;
-    const float updated = adam_slot(slot, *value, grad, m, a, act);
-    m_base[f] = m;
-    *value = updated;
-    if (slot < kFloatsPerGaussian) s_act[il * kFloatsPerGaussian + slot] = act;
+    m_base[f] = m[k];
   }
   __syncthreads();
   copy_in(params + static_cast<size_t>(first) * kFloatsPerGaussian, s_raw, count * kFloatsPerGaussian);

@@ -30,18 +30,19 @@ def render_views(gaussians, cameras, sh=None, degree=0):
     return images
 
 
-def test_adam_and_stats_step_against_oracle():
-    params, sh, g, cam = config0_scene(seed=0, n=4096)
-    teacher_p, teacher_sh = scene.random_raw_gaussians(4096, 1, box=1.0)
+@pytest.mark.parametrize("n", [4096, 4133])  # whole 64-Gaussian blocks; a partial last block
+def test_adam_and_stats_step_against_oracle(n):
+    params, sh, g, cam = config0_scene(seed=0, n=n)
+    teacher_p, teacher_sh = scene.random_raw_gaussians(n, 1, box=1.0)
     images = render_views(scene.activate(teacher_p), [cam], teacher_sh, 3)
     ctx = context()
     settings = diorama_amd.default_settings(1.0)
-    say("adam step")
+    say("adam step", n)
     t = diorama_amd.Trainer(ctx, params, sh, [cam], images, settings)
-    moments = np.random.default_rng(2).normal(size=(4096, 122)).astype(np.float32) * 1e-3
+    moments = np.random.default_rng(2).normal(size=(n, 122)).astype(np.float32) * 1e-3
     moments[:, 1::2] = np.abs(moments[:, 1::2])
     t.buffers["moments"].upload(moments)
-    stats0 = np.abs(np.random.default_rng(3).normal(size=(4096, 4))).astype(np.float32)
+    stats0 = np.abs(np.random.default_rng(3).normal(size=(n, 4))).astype(np.float32)
     t.buffers["stats"].upload(stats0)
     r = t.renderer
     act_before = r.gaussian_buffer.download(np.float32).reshape(-1, 16)
