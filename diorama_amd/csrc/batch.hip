@@ -11,6 +11,7 @@
 // tile space), never on which views it holds: the view table sits in device
 // memory and tiles past the batch's tile count return at once, so one captured
 // graph serves every step.
+#include "block_copy.hpp"
 #include "gaussian_math.hpp"
 #include "loss_tile.hpp"
 #include "raster_tile.hpp"
@@ -21,11 +22,16 @@ namespace {
 constexpr int kThreads = 256;
 using raster::kGrads;
 
-__device__ inline void stage_block(float* __restrict__ dst, const float* __restrict__ src, uint32_t count) {
-  const uint32_t quads = count / 4;
-  for (uint32_t q = threadIdx.x; q < quads; q += kThreads)
-    reinterpret_cast<float4*>(dst)[q] = reinterpret_cast<const float4*>(src)[q];
-  for (uint32_t k = quads * 4 + threadIdx.x; k < count; k += kThreads) dst[k] = src[k];
+// A block's 256 SH records between global memory and LDS: every load in flight at
+// once for a full block, one at a time for the partial last block.
+__device__ inline void stage_sh(float* __restrict__ dst, const float* __restrict__ src, uint32_t block_n) {
+  if (block_n == kThreads) {
+    RecordCopy<kThreads, kShRest, kThreads> c;
+    c.load(src);
+    c.store(dst);
+  } else {
+    block_copy<kThreads>(dst, src, block_n * kShRest);
+  }
 }
 
 // The view owning a tile of the batch's tile space (views are in tile order).
@@ -55,7 +61,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_batch_kernel(const float4
     g[4 * q] = v.x; g[4 * q + 1] = v.y; g[4 * q + 2] = v.z; g[4 * q + 3] = v.w;
   }
   const int count = sh ? sh_count(sh_degree) : 0;
-  if (count > 0) stage_block(s_sh, sh + static_cast<size_t>(first) * kShRest, min(static_cast<uint32_t>(kThreads), n - first) * kShRest);
+  if (count > 0) stage_sh(s_sh, sh + static_cast<size_t>(first) * kShRest, min(static_cast<uint32_t>(kThreads), n - first));
   __syncthreads();
   if (!active) return;
   const float* row = s_sh + threadIdx.x * kShRest;
@@ -148,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_batch_kernel(
   const uint32_t block_n = min(static_cast<uint32_t>(kThreads), n - first);
   const bool active = i < n;
   const int count = sh_count(sh_degree);
-  if (count > 0) stage_block(s_sh, sh + static_cast<size_t>(first) * kShRest, block_n * kShRest);
+  if (count > 0) stage_sh(s_sh, sh + static_cast<size_t>(first) * kShRest, block_n);
   __syncthreads();
   float* row = s_sh + threadIdx.x * kShRest;
   float4 out0 = make_float4(0.f, 0.f, 0.f, 0.f), out1 = out0, out2 = out0, out3 = out0;
@@ -205,7 +211,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_batch_kernel(
 #pragma unroll
   for (int k = 0; k < static_cast<int>(kShRest); k++) row[k] = shg[k];
   __syncthreads();
-  stage_block(sh_grads + static_cast<size_t>(first) * kShRest, s_sh, block_n * kShRest);
+  stage_sh(sh_grads + static_cast<size_t>(first) * kShRest, s_sh, block_n);
 }
 
 }  // namespace

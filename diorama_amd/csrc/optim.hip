@@ -6,6 +6,7 @@
 // Gaussian: the 61 float2 moments of a Gaussian are contiguous, so a flat index
 // over slots reads and writes them fully coalesced, which is what bounds this
 // kernel (1,708 bytes per Gaussian per step).
+#include "block_copy.hpp"
 #include "gaussian_math.hpp"
 
 namespace dio {
@@ -81,12 +82,6 @@ struct SlotWalk {
   }
 };
 
-__device__ inline void copy_in(float* __restrict__ dst, const float* __restrict__ src, uint32_t count) {
-  for (uint32_t q = threadIdx.x; q < count / 4; q += kThreads)
-    reinterpret_cast<float4*>(dst)[q] = reinterpret_cast<const float4*>(src)[q];
-  for (uint32_t k = count / 4 * 4 + threadIdx.x; k < count; k += kThreads) dst[k] = src[k];
-}
-
 // kDeviceArgs: the arguments come from device memory (`args`) instead of `a`.
 template <bool kDeviceArgs>
 __global__ __launch_bounds__(kThreads) void adam_kernel(uint32_t n, float* __restrict__ params, float* __restrict__ sh,
@@ -103,38 +98,73 @@ __global__ __launch_bounds__(kThreads) void adam_kernel(uint32_t n, float* __res
   __shared__ __align__(16) float s_act[kAdamGaussians * kFloatsPerGaussian];
   const uint32_t first = blockIdx.x * kAdamGaussians;
   const uint32_t count = min(kAdamGaussians, n - first);
-  copy_in(s_p, params + first * kFloatsPerGaussian, count * kFloatsPerGaussian);
-  copy_in(s_gp, gaussian_grads + first * kFloatsPerGaussian, count * kFloatsPerGaussian);
-  copy_in(s_sh, sh + first * kShRest, count * kShRest);
-  copy_in(s_gsh, sh_grads + first * kShRest, count * kShRest);
-  __syncthreads();
-  float2* m_base = moments + first * kMomentSlots;
-  SlotWalk w(threadIdx.x);
-  for (uint32_t f = threadIdx.x; f < count * kMomentSlots; f += kThreads, w.next()) {
-    const uint32_t i = w.il, slot = w.slot;
-    if (slot == 7 || slot == 15) {  // padding: kind and birth stamps, never trained
-      s_act[i * kFloatsPerGaussian + slot] = 0.f;
-      continue;
-    }
-    float* value;
-    float grad;
-    if (slot < kFloatsPerGaussian) {
-      value = s_p + i * kFloatsPerGaussian + slot;
-      grad = s_gp[i * kFloatsPerGaussian + slot];
-    } else {
-      value = s_sh + i * kShRest + (slot - kFloatsPerGaussian);
-      grad = s_gsh[i * kShRest + (slot - kFloatsPerGaussian)];
-    }
-    float2 m = m_base[f];
-    float act;
-    const float updated = adam_slot(slot, *value, grad, m, a, act);
-    m_base[f] = m;
-    *value = updated;
-    if (slot < kFloatsPerGaussian) s_act[i * kFloatsPerGaussian + slot] = act;
+  const float* src_p = params + first * kFloatsPerGaussian;
+  const float* src_gp = gaussian_grads + first * kFloatsPerGaussian;
+  const float* src_sh = sh + first * kShRest;
+  const float* src_gsh = sh_grads + first * kShRest;
+  if (count == kAdamGaussians) {
+    RecordCopy<kAdamGaussians, kFloatsPerGaussian, kThreads> c_p, c_gp;
+    RecordCopy<kAdamGaussians, kShRest, kThreads> c_sh, c_gsh;
+    c_p.load(src_p);
+    c_gp.load(src_gp);
+    c_sh.load(src_sh);
+    c_gsh.load(src_gsh);
+    c_p.store(s_p);
+    c_gp.store(s_gp);
+    c_sh.store(s_sh);
+    c_gsh.store(s_gsh);
+  } else {
+    block_copy<kThreads>(s_p, src_p, count * kFloatsPerGaussian);
+    block_copy<kThreads>(s_gp, src_gp, count * kFloatsPerGaussian);
+    block_copy<kThreads>(s_sh, src_sh, count * kShRest);
+    block_copy<kThreads>(s_gsh, src_gsh, count * kShRest);
   }
   __syncthreads();
-  copy_in(params + first * kFloatsPerGaussian, s_p, count * kFloatsPerGaussian);
-  copy_in(sh + first * kShRest, s_sh, count * kShRest);
+  // A thread's moments are all loaded before the first update; the padding slots
+  // (7: kind, 15: birth stamp) are never trained and go back unchanged.
+  constexpr uint32_t kRounds = (kAdamGaussians * kMomentSlots + kThreads - 1) / kThreads;
+  const uint32_t total = count * kMomentSlots;
+  float2* m_base = moments + first * kMomentSlots;
+  float2 m[kRounds];
+#pragma unroll
+  for (uint32_t k = 0; k < kRounds; k++) m[k] = m_base[min(threadIdx.x + k * kThreads, total - 1)];
+  SlotWalk w(threadIdx.x);
+#pragma unroll
+  for (uint32_t k = 0; k < kRounds; k++, w.next()) {
+    const uint32_t f = threadIdx.x + k * kThreads;
+    if (f >= total) break;
+    const uint32_t i = w.il, slot = w.slot;
+    if (slot == 7 || slot == 15) {
+      s_act[i * kFloatsPerGaussian + slot] = 0.f;
+    } else {
+      float* value;
+      float grad;
+      if (slot < kFloatsPerGaussian) {
+        value = s_p + i * kFloatsPerGaussian + slot;
+        grad = s_gp[i * kFloatsPerGaussian + slot];
+      } else {
+        value = s_sh + i * kShRest + (slot - kFloatsPerGaussian);
+        grad = s_gsh[i * kShRest + (slot - kFloatsPerGaussian)];
+      }
+      float act;
+      const float updated = adam_slot(slot, *value, grad, m[k], a, act);
+      *value = updated;
+      if (slot < kFloatsPerGaussian) s_act[i * kFloatsPerGaussian + slot] = act;
+    }
+    m_base[f] = m[k];
+  }
+  __syncthreads();
+  if (count == kAdamGaussians) {
+    RecordCopy<kAdamGaussians, kFloatsPerGaussian, kThreads> c_p;
+    RecordCopy<kAdamGaussians, kShRest, kThreads> c_sh;
+    c_p.load(s_p);
+    c_sh.load(s_sh);
+    c_p.store(params + first * kFloatsPerGaussian);
+    c_sh.store(sh + first * kShRest);
+  } else {
+    block_copy<kThreads>(params + first * kFloatsPerGaussian, s_p, count * kFloatsPerGaussian);
+    block_copy<kThreads>(sh + first * kShRest, s_sh, count * kShRest);
+  }
   if (gaussians) {
     // The pads (kind, birth) of the activated copy are left as they are.
     for (uint32_t e = threadIdx.x; e < count * kFloatsPerGaussian; e += kThreads) {
@@ -195,17 +225,6 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(uint32_t n, const float
 constexpr uint32_t kFusedGaussians = 64;
 constexpr uint32_t kFusedRow = 35;  // 16 + 15 + 3, odd so lanes' rows start in different banks
 
-// A full block's four inputs as one run of float4s (SH, raw, activated, splat
-// gradients), staged in kStageRounds rounds of the workgroup's threads with every
-// load issued before the first LDS store. In the last round the lanes past the run
-// wrap to its start and store a value that round 0 also stores: same data, same
-// address. No load or store is conditional, so none can be sunk into a branch.
-constexpr uint32_t kStageSh = kFusedGaussians * kShRest / 4, kStageRaw = kStageSh + kFusedGaussians * 4,
-                   kStageAct = kStageRaw + kFusedGaussians * 4, kStageVec = kStageAct + kFusedGaussians * 9 / 4;
-constexpr uint32_t kStageRounds = (kStageVec + kThreads - 1) / kThreads;
-static_assert(kFusedGaussians * kShRest % 4 == 0 && kFusedGaussians * 9 % 4 == 0, "whole float4s per block");
-static_assert(kStageRounds * kThreads - kStageVec <= kStageVec, "the wrapped lanes stay inside the run");
-
 template <bool kExtra>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void backward_adam_kernel(
     uint32_t n, float* __restrict__ gaussians, float* __restrict__ sh, uint32_t sh_degree, CameraArgs cam,
@@ -229,35 +248,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   if (wave == 1) side = splats[static_cast<size_t>(mine) * 3];
   if (wave == 2) side = stats[mine];
   if (kExtra && wave == 3) side = extra[mine];
+  float* g_sh = sh + static_cast<size_t>(first) * kShRest;
+  float* g_raw = params + static_cast<size_t>(first) * kFloatsPerGaussian;
+  float* g_act = gaussians + static_cast<size_t>(first) * kFloatsPerGaussian;
+  const float* g_sg = splat_grads + static_cast<size_t>(first) * 9;
   if (count == kFusedGaussians) {
-    const float4* src_sh = reinterpret_cast<const float4*>(sh + static_cast<size_t>(first) * kShRest);
-    const float4* src_raw = reinterpret_cast<const float4*>(params + static_cast<size_t>(first) * kFloatsPerGaussian);
-    const float4* src_act =
-        reinterpret_cast<const float4*>(gaussians + static_cast<size_t>(first) * kFloatsPerGaussian);
-    const float4* src_sg = reinterpret_cast<const float4*>(splat_grads + static_cast<size_t>(first) * 9);
-    float4 v[kStageRounds];
-    float4* dst[kStageRounds];
-#pragma unroll
-    for (uint32_t k = 0; k < kStageRounds; k++) {
-      uint32_t q = threadIdx.x + k * kThreads;
-      if (k + 1 == kStageRounds) q = q < kStageVec ? q : q - kStageVec;
-      const float4* src = q < kStageSh    ? src_sh + q
-                          : q < kStageRaw ? src_raw + (q - kStageSh)
-                          : q < kStageAct ? src_act + (q - kStageRaw)
-                                          : src_sg + (q - kStageAct);
-      dst[k] = q < kStageSh    ? reinterpret_cast<float4*>(s_sh) + q
-               : q < kStageRaw ? reinterpret_cast<float4*>(s_raw) + (q - kStageSh)
-               : q < kStageAct ? reinterpret_cast<float4*>(s_act) + (q - kStageRaw)
-                               : reinterpret_cast<float4*>(s_sg) + (q - kStageAct);
-      v[k] = *src;
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kStageRounds; k++) *dst[k] = v[k];
+    RecordCopy<kFusedGaussians, kShRest, kThreads> c_sh;
+    RecordCopy<kFusedGaussians, kFloatsPerGaussian, kThreads> c_raw, c_act;
+    RecordCopy<kFusedGaussians, 9, kThreads> c_sg;
+    c_sh.load(g_sh);
+    c_raw.load(g_raw);
+    c_act.load(g_act);
+    c_sg.load(g_sg);
+    c_sh.store(s_sh);
+    c_raw.store(s_raw);
+    c_act.store(s_act);
+    c_sg.store(s_sg);
   } else {
-    copy_in(s_sh, sh + static_cast<size_t>(first) * kShRest, count * kShRest);
-    copy_in(s_raw, params + static_cast<size_t>(first) * kFloatsPerGaussian, count * kFloatsPerGaussian);
-    copy_in(s_act, gaussians + static_cast<size_t>(first) * kFloatsPerGaussian, count * kFloatsPerGaussian);
-    copy_in(s_sg, splat_grads + static_cast<size_t>(first) * 9, count * 9);
+    block_copy<kThreads>(s_sh, g_sh, count * kShRest);
+    block_copy<kThreads>(s_raw, g_raw, count * kFloatsPerGaussian);
+    block_copy<kThreads>(s_act, g_act, count * kFloatsPerGaussian);
+    block_copy<kThreads>(s_sg, g_sg, count * 9);
   }
   if (wave == 1) s_splat[lane] = make_float2(side.z, side.w);
   if (wave == 2) s_stats[lane] = side;
@@ -335,9 +346,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
     m_base[f] = m[k];
   }
   __syncthreads();
-  copy_in(params + static_cast<size_t>(first) * kFloatsPerGaussian, s_raw, count * kFloatsPerGaussian);
-  copy_in(sh + static_cast<size_t>(first) * kShRest, s_sh, count * kShRest);
-  copy_in(gaussians + static_cast<size_t>(first) * kFloatsPerGaussian, s_act, count * kFloatsPerGaussian);
+  if (count == kFusedGaussians) {
+    RecordCopy<kFusedGaussians, kShRest, kThreads> c_sh;
+    RecordCopy<kFusedGaussians, kFloatsPerGaussian, kThreads> c_raw, c_act;
+    c_sh.load(s_sh);
+    c_raw.load(s_raw);
+    c_act.load(s_act);
+    c_sh.store(g_sh);
+    c_raw.store(g_raw);
+    c_act.store(g_act);
+  } else {
+    block_copy<kThreads>(g_raw, s_raw, count * kFloatsPerGaussian);
+    block_copy<kThreads>(g_sh, s_sh, count * kShRest);
+    block_copy<kThreads>(g_act, s_act, count * kFloatsPerGaussian);
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void reset_opacity_kernel(uint32_t n, float* __restrict__ params,

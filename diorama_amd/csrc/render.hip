@@ -9,6 +9,7 @@
 // (src/shaders/tiles.wgsl:36-59, src/radixSort.ts:103-104), so the lists and
 // ranges are bit-identical while the pair traffic drops from 6 passes to 1-2.
 #include "gaussian_math.hpp"
+#include "block_copy.hpp"
 #include "raster_tile.hpp"
 
 namespace dio {
@@ -17,15 +18,8 @@ namespace {
 constexpr int kThreads = 256;
 using raster::kGrads;
 
-// Copies `count` floats between global memory and LDS with the whole workgroup,
-// 16 bytes per lane per step (both sides 16-byte aligned): the 45-float SH records
-// are 180 bytes apart, so per-thread access would touch a new cache line per lane.
-__device__ inline void block_copy(float* __restrict__ dst, const float* __restrict__ src, uint32_t count) {
-  const uint32_t quads = count / 4;
-  for (uint32_t q = threadIdx.x; q < quads; q += kThreads)
-    reinterpret_cast<float4*>(dst)[q] = reinterpret_cast<const float4*>(src)[q];
-  for (uint32_t k = quads * 4 + threadIdx.x; k < count; k += kThreads) dst[k] = src[k];
-}
+using ShBlock = RecordCopy<kThreads, kShRest, kThreads>;
+using GradBlock = RecordCopy<kThreads, kGrads, kThreads>;
 
 // ------------------------------------------------------------ preprocess --
 
@@ -50,8 +44,16 @@ __global__ __launch_bounds__(kThreads) void preprocess_kernel(const float4* __re
   Projection p;
   const bool visible = active && project(g, cam, p);
   const int count = sh ? sh_count(sh_degree) : 0;
-  if (count > 0 && __syncthreads_or(visible))
-    block_copy(s_sh, sh + static_cast<size_t>(first) * kShRest, min(static_cast<uint32_t>(kThreads), n - first) * kShRest);
+  if (count > 0 && __syncthreads_or(visible)) {
+    const float* src = sh + static_cast<size_t>(first) * kShRest;
+    if (n - first >= kThreads) {
+      ShBlock c;
+      c.load(src);
+      c.store(s_sh);
+    } else {
+      block_copy<kThreads>(s_sh, src, (n - first) * kShRest);
+    }
+  }
   __syncthreads();
   if (!active) return;
   SplatRecord rec;
@@ -246,8 +248,19 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
   const int count = sh_count(sh_degree);
   const bool any = __syncthreads_or(visible);
   if (any) {
-    if (count > 0) block_copy(s_sh, sh + static_cast<size_t>(first) * kShRest, block_n * kShRest);
-    block_copy(s_grad, splat_grads + static_cast<size_t>(first) * kGrads, block_n * kGrads);
+    const float* src_sh = sh + static_cast<size_t>(first) * kShRest;
+    const float* src_grad = splat_grads + static_cast<size_t>(first) * kGrads;
+    if (block_n == kThreads) {
+      ShBlock c_sh;
+      GradBlock c_grad;
+      if (count > 0) c_sh.load(src_sh);
+      c_grad.load(src_grad);
+      if (count > 0) c_sh.store(s_sh);
+      c_grad.store(s_grad);
+    } else {
+      if (count > 0) block_copy<kThreads>(s_sh, src_sh, block_n * kShRest);
+      block_copy<kThreads>(s_grad, src_grad, block_n * kGrads);
+    }
   }
   __syncthreads();
   // A thread only touches its own SH row, so gradients overwrite coefficients in place.
@@ -288,7 +301,14 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
     gaussian_grads[i * 4 + 3] = out3;
   }
   __syncthreads();
-  block_copy(sh_grads + static_cast<size_t>(first) * kShRest, s_sh, block_n * kShRest);
+  float* dst = sh_grads + static_cast<size_t>(first) * kShRest;
+  if (block_n == kThreads) {
+    ShBlock c;
+    c.load(s_sh);
+    c.store(dst);
+  } else {
+    block_copy<kThreads>(dst, s_sh, block_n * kShRest);
+  }
 }
 
 }  // namespace
