@@ -398,6 +398,7 @@ struct dio_tile_rasterizer {
           *pair_sort_ws = nullptr;
   bool sorted_in_alt = false;
   dio_buf* ranges = nullptr;
+  dio_buf* tile_order = nullptr;  // the rasterizer's launch order, heaviest tiles first
   uint32_t ranges_tiles = 0;
   dio_buf *pixel_color = nullptr, *pixel_last = nullptr;
   uint32_t pixel_count = 0;
@@ -436,6 +437,7 @@ void destroy_tiles(dio_tile_rasterizer* t) {
   release(t->pair_total);
   free_pairs(t);
   release(t->ranges);
+  release(t->tile_order);
   release(t->pixel_color);
   release(t->pixel_last);
   if (t->readback_host) (void)hipHostFree(t->readback_host);
@@ -472,7 +474,9 @@ dio_tile_rasterizer* create_tiles(dio_ctx* ctx, dio_buf* splats, uint32_t count,
 void ensure_frame_buffers(dio_tile_rasterizer* t, uint32_t tiles, uint32_t pixels) {
   if (!t->ranges || t->ranges_tiles < tiles) {
     release(t->ranges);
+    release(t->tile_order);
     t->ranges = alloc(t->ctx, static_cast<size_t>(tiles) * 8, true);
+    t->tile_order = alloc(t->ctx, static_cast<size_t>(tiles) * 4, true);
     t->ranges_tiles = tiles;
   }
   if (!t->pixel_color || t->pixel_count < pixels) {
@@ -532,10 +536,11 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   dio_buf* sorted_tiles = t->sorted_in_alt ? t->alt_tiles : t->pair_tiles;
   dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
   dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(t->pair_total), t->capacity, tiles,
-                   dev<uint32_t>(t->ranges));
+                   dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order));
   mark(kMarkRanges);
-  dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), w, h, bg, dev<uint32_t>(target),
-                 dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last), t->splat_batch);
+  dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), w,
+                 h, bg, dev<uint32_t>(target), dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
+                 t->splat_batch);
   mark(kMarkRasterize);
   t->framed = true;
   t->frame_w = w;
@@ -573,9 +578,10 @@ void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pix
   dio::fill_u32(s, dev<uint32_t>(splat_grads), 0u, static_cast<size_t>(t->count) * 9);  // the pass adds into it
   if (t->clock) t->clock->mark(s, kMarkClearGrads);
   dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
-  dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), w, h,
-                          t->frame_bg, dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
-                          dev<float>(pixel_grads), dev<float>(splat_grads), t->splat_batch);
+  dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges),
+                          dev<uint32_t>(t->tile_order), w, h, t->frame_bg, dev<float>(t->pixel_color),
+                          dev<uint32_t>(t->pixel_last), dev<float>(pixel_grads), dev<float>(splat_grads),
+                          t->splat_batch);
   if (t->clock) t->clock->mark(s, kMarkBackward);
 }
 
@@ -2030,14 +2036,15 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       return;
     } else if (name == "rasterize") {
       run = [&] {
-        dio::rasterize(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges), tr->frame_w,
-                       tr->frame_h, tr->frame_bg, nullptr, dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last),
-                       tr->splat_batch);
+        dio::rasterize(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges),
+                       dev<uint32_t>(tr->tile_order), tr->frame_w, tr->frame_h, tr->frame_bg, nullptr,
+                       dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last), tr->splat_batch);
       };
     } else if (name == "rasterize_backward") {
       run = [&] {
         dio::rasterize_backward(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges),
-                                tr->frame_w, tr->frame_h, tr->frame_bg, dev<float>(tr->pixel_color),
+                                dev<uint32_t>(tr->tile_order), tr->frame_w, tr->frame_h, tr->frame_bg,
+                                dev<float>(tr->pixel_color),
                                 dev<uint32_t>(tr->pixel_last), dev<float>(t->loss->pixel_grads),
                                 dev<float>(sc->grad_splats), tr->splat_batch);
       };

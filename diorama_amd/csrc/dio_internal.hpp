@@ -92,19 +92,21 @@ void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const 
                const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
                const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats);
 
+// Each tile's [start, end) run of the sorted pairs; with `order` (nullable, one
+// word per tile) also the rasterizer's launch order, heaviest tiles first.
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
-                 uint32_t tiles, uint32_t* ranges);
+                 uint32_t tiles, uint32_t* ranges, uint32_t* order = nullptr);
 
-// splat_batch: pairs a wave stages per round (64, 128 or 256); results do not
-// depend on it.
+// splat_batch: pairs a wave stages per round (64, 128 or 256); `order` (nullable)
+// maps workgroups to tiles. Results depend on neither.
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
-               uint32_t width, uint32_t height, const float* background, uint32_t* target, float* pixel_color,
-               uint32_t* pixel_last, uint32_t splat_batch = 64);
+               const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
+               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch = 64);
 
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
-                        uint32_t width, uint32_t height, const float* background, const float* pixel_color,
-                        const uint32_t* pixel_last, const float* pixel_grads, float* splat_grads,
-                        uint32_t splat_batch = 64);
+                        const uint32_t* order, uint32_t width, uint32_t height, const float* background,
+                        const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
+                        float* splat_grads, uint32_t splat_batch = 64);
 
 void preprocess_backward(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                          const CameraArgs& cam, const float* splats, const float* splat_grads,

@@ -200,31 +200,70 @@ __global__ __launch_bounds__(kThreads) void find_ranges_kernel(const uint32_t* _
   if (i == count - 1) ranges[tile].y = count;
 }
 
+// The rasterizer's launch order: tiles by falling pair count (in steps of
+// kOrderStep pairs; ties in any order), so that the heaviest tiles start first and
+// the light ones fill in behind them instead of the centre tiles of a view, which
+// row-major order dispatches halfway through, finishing last. One workgroup
+// counting-sorts the tiles.
+constexpr uint32_t kOrderThreads = 1024, kOrderStep = 8;
+
+__global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint2* __restrict__ ranges, uint32_t tiles,
+                                                                   uint32_t* __restrict__ order) {
+  __shared__ uint32_t s_next[kOrderThreads];
+  __shared__ uint32_t s_wave[kOrderThreads / 64];
+  const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  auto bucket = [&](uint32_t tile) {  // 0 for the heaviest
+    const uint2 r = ranges[tile];
+    return kOrderThreads - 1 - min((r.y - r.x) / kOrderStep, kOrderThreads - 1);
+  };
+  s_next[tid] = 0;
+  __syncthreads();
+  for (uint32_t t = tid; t < tiles; t += kOrderThreads) atomicAdd(&s_next[bucket(t)], 1u);
+  __syncthreads();
+  // Exclusive scan of the bucket sizes: within each wave, then over the wave totals.
+  const uint32_t mine = s_next[tid];
+  uint32_t x = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= static_cast<uint32_t>(o)) x += y;
+  }
+  if (lane == 63) s_wave[wave] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < wave; w++) before += s_wave[w];
+  s_next[tid] = before + x - mine;
+  __syncthreads();
+  for (uint32_t t = tid; t < tiles; t += kOrderThreads) order[atomicAdd(&s_next[bucket(t)], 1u)] = t;
+}
+
 // -------------------------------------------------------------- rasterize --
 
-// rasterize.wgsl:31-100: front-to-back blend, one workgroup per tile.
+// rasterize.wgsl:31-100: front-to-back blend, one workgroup per tile (`order`, if
+// given, maps workgroups to tiles).
 template <int kPer>
 __global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict__ splats,
                                                          const uint32_t* __restrict__ pair_splats,
-                                                         const uint2* __restrict__ ranges, uint32_t width,
+                                                         const uint2* __restrict__ ranges,
+                                                         const uint32_t* __restrict__ order, uint32_t width,
                                                          uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
                                                          float bg_b, uint32_t* __restrict__ target,
                                                          float4* __restrict__ pixel_color,
                                                          uint32_t* __restrict__ pixel_last) {
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
   raster::rasterize_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width, height, bg_r,
                                bg_g, bg_b, target, pixel_color, pixel_last);
 }
 
-// rasterizeBackward.wgsl:49-163, one workgroup per tile.
+// rasterizeBackward.wgsl:49-163, one workgroup per tile (`order` as above).
 template <int kPer>
 __global__ __launch_bounds__(256) void rasterize_backward_kernel(
     const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
-    uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g, float bg_b,
-    const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
+    const uint32_t* __restrict__ order, uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
+    float bg_b, const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
     const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads) {
   __shared__ raster::BackwardShared<kPer> shared;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
   raster::rasterize_backward_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width,
                                         height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads,
                                         shared);
@@ -354,22 +393,27 @@ void duplicate_batch(hipStream_t s, const float* splats, const uint32_t* order, 
 }
 
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
-                 uint32_t tiles, uint32_t* ranges) {
+                 uint32_t tiles, uint32_t* ranges, uint32_t* order) {
   fill_u32(s, ranges, 0u, static_cast<size_t>(tiles) * 2);
-  if (capacity == 0) return;
-  find_ranges_kernel<<<ceil_div(capacity, kThreads), kThreads, 0, s>>>(pair_tiles, pair_total, capacity,
-                                                                        reinterpret_cast<uint2*>(ranges));
-  DIO_LAUNCHED("find_ranges");
+  if (capacity > 0) {
+    find_ranges_kernel<<<ceil_div(capacity, kThreads), kThreads, 0, s>>>(pair_tiles, pair_total, capacity,
+                                                                          reinterpret_cast<uint2*>(ranges));
+    DIO_LAUNCHED("find_ranges");
+  }
+  if (order && tiles > 0) {
+    tile_order_kernel<<<1, kOrderThreads, 0, s>>>(reinterpret_cast<const uint2*>(ranges), tiles, order);
+    DIO_LAUNCHED("tile_order");
+  }
 }
 
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
-               uint32_t width, uint32_t height, const float* background, uint32_t* target, float* pixel_color,
-               uint32_t* pixel_last, uint32_t splat_batch) {
+               const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
+               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
   auto launch = [&](auto kernel) {
     kernel<<<tx * ty, kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
-                                        reinterpret_cast<const uint2*>(ranges), width, height, tx, background[0],
+                                        reinterpret_cast<const uint2*>(ranges), order, width, height, tx, background[0],
                                         background[1], background[2], target, reinterpret_cast<float4*>(pixel_color),
                                         pixel_last);
   };
@@ -383,14 +427,14 @@ void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, 
 }
 
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
-                        uint32_t width, uint32_t height, const float* background, const float* pixel_color,
-                        const uint32_t* pixel_last, const float* pixel_grads, float* splat_grads,
-                        uint32_t splat_batch) {
+                        const uint32_t* order, uint32_t width, uint32_t height, const float* background,
+                        const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
+                        float* splat_grads, uint32_t splat_batch) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
   auto launch = [&](auto kernel) {
     kernel<<<tx * ty, kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
-                                        reinterpret_cast<const uint2*>(ranges), width, height, tx, background[0],
+                                        reinterpret_cast<const uint2*>(ranges), order, width, height, tx, background[0],
                                         background[1], background[2], reinterpret_cast<const float4*>(pixel_color),
                                         pixel_last, reinterpret_cast<const float4*>(pixel_grads), splat_grads);
   };
