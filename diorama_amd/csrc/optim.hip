@@ -238,7 +238,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   __shared__ float2 s_splat[kFusedGaussians];  // depth, radius
   __shared__ float4 s_stats[kFusedGaussians];
   __shared__ float4 s_extra[kExtra ? kFusedGaussians : 1];
-  const uint32_t first = blockIdx.x * kFusedGaussians;
+  // Blocks run from the last Gaussians to the first: the next step's preprocess reads
+  // the activated Gaussians and SH from the first ones on, so it starts on the lines
+  // written last, which the memory-side cache still holds.
+  const uint32_t first = (gridDim.x - 1 - blockIdx.x) * kFusedGaussians;
   const uint32_t count = min(kFusedGaussians, n - first);
   // Stage the block's inputs with all loads in flight together; wave 1 fetches the
   // splats' depth and radius, wave 2 the statistics, wave 3 the extra statistics.
