@@ -26,6 +26,10 @@ constexpr int kThreads = 256;
 constexpr int kItems = kSortTile / kThreads;  // 16
 constexpr int kWaves = kThreads / kWave;
 constexpr uint32_t kMaxPasses = 16;  // 64 key bits in 4-bit digits
+// The global digit counts are kept in one copy per XCD (workgroup b runs on XCD
+// b mod 8): every workgroup adds its counts with device-scope atomics, and one
+// shared copy made each counter take an atomic from every workgroup in turn.
+constexpr uint32_t kHistCopies = 8, kHistStride = kMaxPasses * 256;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1;
@@ -111,17 +115,21 @@ __global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyO
     }
   }
   __syncthreads();
+  uint32_t* mine = hist + (blockIdx.x % kHistCopies) * kHistStride;
   if (tid < kRadix)
     for (uint32_t p = 0; p < passes; p++)
-      if (local[p][tid]) atomicAdd(&hist[p * 256 + tid], local[p][tid]);
+      if (local[p][tid]) atomicAdd(&mine[p * 256 + tid], local[p][tid]);
 }
 
-// Turns each pass's digit counts into where that digit's run starts.
+// Sums each pass's digit counts over the copies and turns them into where that
+// digit's run starts (written over the first copy).
 __global__ __launch_bounds__(256) void digit_offsets_kernel(uint32_t* hist) {
   __shared__ uint32_t wave_totals[kWaves];
   uint32_t* h = hist + blockIdx.x * 256;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t c = h[tid];
+  uint32_t c = 0;
+#pragma unroll
+  for (uint32_t x = 0; x < kHistCopies; x++) c += h[x * kHistStride + tid];
   uint32_t x = c;
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) {
@@ -347,7 +355,7 @@ size_t sort_workspace_bytes(uint32_t capacity) {
   const uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kSortTile);
   const uint32_t groups = ceil_div(tiles, kSortGroup);
   // digit offsets for every pass, two tile and two pairs of group look-back arrays, pass tickets
-  return kMaxPasses * 256 * 4 + 2 * static_cast<size_t>(tiles) * 256 * 4 +
+  return kHistCopies * kHistStride * 4 + 2 * static_cast<size_t>(tiles) * 256 * 4 +
          4 * static_cast<size_t>(groups) * 256 * 4 + 256;
 }
 
@@ -361,7 +369,7 @@ SortWorkspace sort_workspace(void* base, uint32_t capacity) {
     p += words * 4;
     return q;
   };
-  ws.hist = take(kMaxPasses * 256);
+  ws.hist = take(kHistCopies * kHistStride);
   for (int k = 0; k < 2; k++) ws.state[k] = take(static_cast<size_t>(ws.tiles) * 256);
   for (int k = 0; k < 2; k++) ws.group_sum[k] = take(static_cast<size_t>(ws.groups) * 256);
   for (int k = 0; k < 2; k++) ws.group_prefix[k] = take(static_cast<size_t>(ws.groups) * 256);
@@ -378,7 +386,7 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
   const uint32_t tiles = ceil_div(capacity, kSortTile);
   const uint32_t passes = (end_bit - begin_bit + kBits - 1) / kBits;
   if (passes > kMaxPasses) throw Error(-2, "radix sort: too many passes");
-  fill_u32(s, ws.hist, 0u, passes * 256);
+  fill_u32(s, ws.hist, 0u, kHistCopies * kHistStride);
   const uint32_t hist_blocks = std::min<uint32_t>(tiles, 1024);
   if (wide)
     histogram_kernel<true, kBits><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint2*>(keys), count, capacity,
