@@ -36,9 +36,25 @@ __host__ __device__ inline uint32_t ceil_div(uint64_t a, uint64_t b) { return st
 // Exclusive prefix sum of u32 values (decoupled look-back, one pass over HBM).
 // `count` (device, element 0) caps how many of `capacity` values are scanned;
 // null scans all of them. `total` (device) receives the grand total.
+// Workspaces of the scan and the radix sort start zeroed (allocate them cleared)
+// and every call leaves them zeroed again: the last workgroup to finish clears what
+// the call used, so no clearing launch precedes a call.
+//
+// wait_memory() waits until this wave's vector memory operations, stores and
+// atomics included, have completed (vmcnt counts all of them on this GPU family).
+// A workgroup calls it before the barrier that precedes counting itself finished,
+// so the last workgroup sees every other workgroup's device-coherent writes; an
+// agent-scope fence would also order them but writes back the whole L2 in every
+// workgroup.
+__device__ inline void wait_memory() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#endif
+}
 struct ScanWorkspace {
   uint64_t* tile_state = nullptr;  // one (flag, value) word per tile
   uint32_t* ticket = nullptr;      // dynamic tile counter
+  uint32_t* done = nullptr;        // workgroups finished
   uint32_t tiles = 0;
 };
 constexpr uint32_t kScanTile = 4096;  // 256 threads x 16 values
@@ -59,11 +75,12 @@ void scan_exclusive_gather(hipStream_t s, const uint32_t* in, const uint32_t* id
 constexpr uint32_t kSortTile = 4096;  // 256 threads x 16 keys
 constexpr uint32_t kSortGroup = 16;  // tiles per look-back group
 struct SortWorkspace {
-  uint32_t* hist = nullptr;                 // per pass: 256 digit counts, then run starts
+  uint32_t* hist = nullptr;                 // per XCD copy, per pass: 256 digit counts
+  uint32_t* offsets = nullptr;              // per pass: where each digit's run starts
   uint32_t* state[2] = {nullptr, nullptr};  // per tile x digit look-back words, alternating passes
   uint32_t* group_sum[2] = {nullptr, nullptr};     // per group x digit: tiles done << 24 | their count
   uint32_t* group_prefix[2] = {nullptr, nullptr};  // per group x digit: inclusive prefix once known
-  uint32_t* tickets = nullptr;              // per pass tile counter
+  uint32_t* tickets = nullptr;              // per pass tile counter; the histogram's finish counter
   uint32_t tiles = 0, groups = 0;
 };
 size_t sort_workspace_bytes(uint32_t capacity);

@@ -234,7 +234,7 @@ void check_points(const dio_buf* positions, uint32_t count) {
 void neighbor_distances(dio_ctx* ctx, const dio_buf* positions, uint32_t count, float* out) {
   if (count == 0) return;
   if (count >= (1u << 30)) invalid("at most 2^30 points");
-  dio_buf* ws = alloc(ctx, dio::neighbors_workspace_bytes(count));
+  dio_buf* ws = alloc(ctx, dio::neighbors_workspace_bytes(count), true);  // the sort's part starts zeroed
   try {
     dio::mean_neighbor_distance2(ctx->stream, static_cast<const float*>(positions->ptr), count, out, ws->ptr);
     dio::sync(ctx);

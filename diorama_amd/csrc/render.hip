@@ -181,30 +181,46 @@ __global__ __launch_bounds__(kThreads) void duplicate_kernel(const float4* __res
 
 // find_ranges (tiles.wgsl:64-84): tile boundaries in the sorted pairs. Ranges
 // are cleared first, so tiles without pairs read [0, 0).
+// find_ranges (tiles.wgsl:61-79): each tile's [start, end) run of the sorted pairs,
+// (0, 0) for a tile without pairs, one thread per pair. The pair at a change of
+// tile writes the end of the previous tile, the start of its own and the empty
+// tiles between them (the first and last pairs also the tiles before and after
+// them), so no clearing pass is needed.
 __global__ __launch_bounds__(kThreads) void find_ranges_kernel(const uint32_t* __restrict__ pair_tiles,
                                                                 const uint32_t* __restrict__ pair_total,
-                                                                uint32_t capacity, uint2* __restrict__ ranges) {
+                                                                uint32_t capacity, uint32_t tiles,
+                                                                uint2* __restrict__ ranges) {
   const uint32_t count = min(*pair_total, capacity);
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const uint2 empty = make_uint2(0u, 0u);
+  if (count == 0) {
+    for (uint32_t t = i; t < tiles; t += gridDim.x * kThreads) ranges[t] = empty;
+    return;
+  }
   if (i >= count) return;
-  const uint32_t tile = pair_tiles[i];
+  const uint32_t tile = min(pair_tiles[i], tiles - 1);
   if (i == 0) {
+    for (uint32_t t = 0; t < tile; t++) ranges[t] = empty;
     ranges[tile].x = 0;
   } else {
-    const uint32_t previous = pair_tiles[i - 1];
+    const uint32_t previous = min(pair_tiles[i - 1], tiles - 1);
     if (tile != previous) {
       ranges[previous].y = i;
+      for (uint32_t t = previous + 1; t < tile; t++) ranges[t] = empty;
       ranges[tile].x = i;
     }
   }
-  if (i == count - 1) ranges[tile].y = count;
+  if (i == count - 1) {
+    ranges[tile].y = count;
+    for (uint32_t t = tile + 1; t < tiles; t++) ranges[t] = empty;
+  }
 }
 
 // The rasterizer's launch order: tiles by falling pair count (in steps of
 // kOrderStep pairs; ties in any order), so that the heaviest tiles start first and
 // the light ones fill in behind them instead of the centre tiles of a view, which
 // row-major order dispatches halfway through, finishing last. One workgroup
-// counting-sorts the tiles.
+// counting-sorts the tiles, a thread per bucket.
 constexpr uint32_t kOrderThreads = 1024, kOrderStep = 8;
 
 __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint2* __restrict__ ranges, uint32_t tiles,
@@ -394,13 +410,11 @@ void duplicate_batch(hipStream_t s, const float* splats, const uint32_t* order, 
 
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
                  uint32_t tiles, uint32_t* ranges, uint32_t* order) {
-  fill_u32(s, ranges, 0u, static_cast<size_t>(tiles) * 2);
-  if (capacity > 0) {
-    find_ranges_kernel<<<ceil_div(capacity, kThreads), kThreads, 0, s>>>(pair_tiles, pair_total, capacity,
-                                                                          reinterpret_cast<uint2*>(ranges));
-    DIO_LAUNCHED("find_ranges");
-  }
-  if (order && tiles > 0) {
+  if (tiles == 0) return;
+  find_ranges_kernel<<<std::max(1u, ceil_div(capacity, static_cast<uint32_t>(kThreads))), kThreads, 0, s>>>(
+      pair_tiles, pair_total, capacity, tiles, reinterpret_cast<uint2*>(ranges));
+  DIO_LAUNCHED("find_ranges");
+  if (order) {
     tile_order_kernel<<<1, kOrderThreads, 0, s>>>(reinterpret_cast<const uint2*>(ranges), tiles, order);
     DIO_LAUNCHED("tile_order");
   }

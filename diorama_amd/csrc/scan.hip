@@ -4,7 +4,8 @@
 // a tile publishes its own sum ("aggregate") at once and its inclusive prefix
 // once known, and one wavefront reads up to 64 predecessors per step.
 // Tiles are handed out by an atomic ticket, so a tile only ever waits for tiles
-// that started before it. Replaces the Blelloch block scan + recursive levels of
+// that started before it. The last workgroup to finish clears the tile states,
+// the ticket and the finish counter for the next call. Replaces the Blelloch block scan + recursive levels of
 // src/shaders/prefixSum.wgsl:30-91 / src/prefixSum.ts:59-98 (same results: an
 // exclusive scan of min(count, capacity) values, total in element 0 of `total`).
 #include "dio_internal.hpp"
@@ -62,22 +63,13 @@ __device__ uint32_t look_back(const uint64_t* state, uint32_t tile, int lane) {
   }
 }
 
+// One tile of the scan (tile < tiles).
 template <bool kGather>
-__global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in,
-                                                         const uint32_t* __restrict__ idx, uint32_t* out,
-                                                         const uint32_t* count, uint32_t capacity, uint32_t* total,
-                                                         uint64_t* state, uint32_t* ticket) {
-  __shared__ uint32_t s_tile;
+__device__ void scan_tile(const uint32_t* in, const uint32_t* __restrict__ idx, uint32_t* out, uint32_t n,
+                          uint32_t tiles, uint32_t tile, uint32_t* total, uint64_t* state) {
   __shared__ uint32_t s_wave[kThreads / kWave];
   __shared__ uint32_t s_prefix;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t n = count ? min(*count, capacity) : capacity;
-  const uint32_t tiles = n == 0 ? 1 : ceil_div(n, kScanTile);
-  if (tile >= tiles) return;
-
   const uint32_t base = tile * kScanTile + tid * kItems;
   uint32_t v[kItems];
   if (!kGather && base + kItems <= n) {
@@ -141,6 +133,35 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in,
   }
 }
 
+template <bool kGather>
+__global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, const uint32_t* __restrict__ idx,
+                                                         uint32_t* out, const uint32_t* count, uint32_t capacity,
+                                                         uint32_t* total, uint64_t* state, uint32_t* ticket,
+                                                         uint32_t* done) {
+  __shared__ uint32_t s_tile;
+  __shared__ bool s_last;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t n = count ? min(*count, capacity) : capacity;
+  const uint32_t tiles = n == 0 ? 1 : ceil_div(n, kScanTile);
+  if (tile < tiles) scan_tile<kGather>(in, idx, out, n, tiles, tile, total, state);
+  // The states only see device-scope atomics, and a workgroup counts as finished
+  // after its look-back reads and once its own publishes have completed, so the
+  // last one can clear them.
+  wait_memory();
+  __syncthreads();
+  if (tid == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  for (uint32_t k = tid; k < tiles; k += kThreads) publish(state, k, 0);
+  if (tid == 0) {
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 __global__ void fill_kernel(uint32_t* p, uint32_t value, size_t count) {
   size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -167,6 +188,7 @@ ScanWorkspace scan_workspace(void* base, uint32_t capacity) {
   ws.tiles = capacity == 0 ? 1 : ceil_div(capacity, kScanTile);
   ws.tile_state = static_cast<uint64_t*>(base);
   ws.ticket = reinterpret_cast<uint32_t*>(static_cast<char*>(base) + static_cast<size_t>(ws.tiles) * 8);
+  ws.done = ws.ticket + 1;
   return ws;
 }
 
@@ -174,12 +196,12 @@ static void scan_launch(hipStream_t s, bool gather, const uint32_t* in, const ui
                         const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws) {
   uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kScanTile);
   if (tiles > ws.tiles) throw Error(-2, "scan workspace too small");
-  // Tile states and the ticket are one contiguous run of words.
-  fill_u32(s, reinterpret_cast<uint32_t*>(ws.tile_state), 0u, static_cast<size_t>(ws.tiles) * 2 + 1);
   if (gather)
-    scan_kernel<true><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket);
+    scan_kernel<true><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket,
+                                                 ws.done);
   else
-    scan_kernel<false><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket);
+    scan_kernel<false><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket,
+                                                  ws.done);
   DIO_LAUNCHED("scan");
 }
 

@@ -30,6 +30,7 @@ constexpr uint32_t kMaxPasses = 16;  // 64 key bits in 4-bit digits
 // b mod 8): every workgroup adds its counts with device-scope atomics, and one
 // shared copy made each counter take an atomic from every workgroup in turn.
 constexpr uint32_t kHistCopies = 8, kHistStride = kMaxPasses * 256;
+constexpr uint32_t kHistDone = 32;  // the histogram's finish counter among the tickets
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1;
@@ -69,15 +70,25 @@ __device__ inline uint32_t key_count(const uint32_t* count, uint32_t capacity) {
   return count ? min(*count, capacity) : capacity;
 }
 
+// Device-coherent word access (agent-scope atomics bypass the per-XCD L2).
+__device__ inline void publish(uint32_t* state, uint32_t word) {
+  __hip_atomic_store(state, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint32_t peek(uint32_t* state) {
+  return __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Counts the digits of every pass in one read of the keys (LDS histograms added
 // into the global ones), a 4096-key tile at a time with all 16 loads per thread
-// in flight together, and clears the first pass's look-back words and every
-// pass ticket.
+// in flight together, clears the first pass's look-back words and every pass
+// ticket, and ends with the digit offsets of every pass.
 template <bool kWide, int kBits>
 __global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyOps<kWide, kBits>::T* __restrict__ keys,
                                                               const uint32_t* count, uint32_t capacity,
                                                               uint32_t begin_bit, uint32_t passes,
-                                                              uint32_t* __restrict__ hist, uint32_t* __restrict__ state0,
+                                                              uint32_t* __restrict__ hist, uint32_t* __restrict__ offsets,
+                                                              uint32_t* __restrict__ state0,
                                                               uint32_t* __restrict__ group_sum0,
                                                               uint32_t* __restrict__ group_prefix0,
                                                               uint32_t* __restrict__ tickets) {
@@ -119,37 +130,40 @@ __global__ __launch_bounds__(kThreads) void histogram_kernel(const typename KeyO
   if (tid < kRadix)
     for (uint32_t p = 0; p < passes; p++)
       if (local[p][tid]) atomicAdd(&mine[p * 256 + tid], local[p][tid]);
-}
-
-// Sums each pass's digit counts over the copies and turns them into where that
-// digit's run starts (written over the first copy).
-__global__ __launch_bounds__(256) void digit_offsets_kernel(uint32_t* hist) {
+  // The last workgroup to finish sums the copies into each pass's digit offsets
+  // (exclusive scan over the digits) and zeroes the copies and its counter. The
+  // copies only ever see device-scope atomics, and every workgroup's have completed
+  // before its finish is counted.
+  __shared__ bool s_last;
   __shared__ uint32_t wave_totals[kWaves];
-  uint32_t* h = hist + blockIdx.x * 256;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  uint32_t c = 0;
-#pragma unroll
-  for (uint32_t x = 0; x < kHistCopies; x++) c += h[x * kHistStride + tid];
-  uint32_t x = c;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    uint32_t y = __shfl_up(x, o, kWave);
-    if (lane >= o) x += y;
-  }
-  if (lane == kWave - 1) wave_totals[wave] = x;
+  wait_memory();
   __syncthreads();
-  uint32_t before = 0;
+  if (tid == 0) s_last = atomicAdd(&tickets[kHistDone], 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!s_last) return;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  for (uint32_t p = 0; p < passes; p++) {
+    uint32_t c = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; w++) before += w < wave ? wave_totals[w] : 0u;
-  h[tid] = before + x - c;
-}
-
-__device__ inline void publish(uint32_t* state, uint32_t word) {
-  __hip_atomic_store(state, word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline uint32_t peek(uint32_t* state) {
-  return __hip_atomic_load(state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t x = 0; x < kHistCopies; x++)
+      c += peek(&hist[x * kHistStride + p * 256 + tid]);
+    uint32_t inclusive = c;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(inclusive, o, kWave);
+      if (lane >= o) inclusive += y;
+    }
+    if (lane == kWave - 1) wave_totals[wave] = inclusive;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) before += w < wave ? wave_totals[w] : 0u;
+    offsets[p * 256 + tid] = before + inclusive - c;
+#pragma unroll
+    for (uint32_t x = 0; x < kHistCopies; x++) publish(&hist[x * kHistStride + p * 256 + tid], 0u);
+    __syncthreads();  // wave_totals is rewritten by the next pass
+  }
+  if (tid == 0) publish(&tickets[kHistDone], 0u);
 }
 
 template <bool kWide, int kBits>
@@ -354,8 +368,9 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
 size_t sort_workspace_bytes(uint32_t capacity) {
   const uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kSortTile);
   const uint32_t groups = ceil_div(tiles, kSortGroup);
-  // digit offsets for every pass, two tile and two pairs of group look-back arrays, pass tickets
-  return kHistCopies * kHistStride * 4 + 2 * static_cast<size_t>(tiles) * 256 * 4 +
+  // digit counts per XCD, digit offsets for every pass, two tile and two pairs of group
+  // look-back arrays, pass tickets
+  return kHistCopies * kHistStride * 4 + kHistStride * 4 + 2 * static_cast<size_t>(tiles) * 256 * 4 +
          4 * static_cast<size_t>(groups) * 256 * 4 + 256;
 }
 
@@ -370,6 +385,7 @@ SortWorkspace sort_workspace(void* base, uint32_t capacity) {
     return q;
   };
   ws.hist = take(kHistCopies * kHistStride);
+  ws.offsets = take(kHistStride);
   for (int k = 0; k < 2; k++) ws.state[k] = take(static_cast<size_t>(ws.tiles) * 256);
   for (int k = 0; k < 2; k++) ws.group_sum[k] = take(static_cast<size_t>(ws.groups) * 256);
   for (int k = 0; k < 2; k++) ws.group_prefix[k] = take(static_cast<size_t>(ws.groups) * 256);
@@ -386,19 +402,17 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
   const uint32_t tiles = ceil_div(capacity, kSortTile);
   const uint32_t passes = (end_bit - begin_bit + kBits - 1) / kBits;
   if (passes > kMaxPasses) throw Error(-2, "radix sort: too many passes");
-  fill_u32(s, ws.hist, 0u, kHistCopies * kHistStride);
   const uint32_t hist_blocks = std::min<uint32_t>(tiles, 1024);
   if (wide)
     histogram_kernel<true, kBits><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint2*>(keys), count, capacity,
-                                                                    begin_bit, passes, ws.hist, ws.state[0],
+                                                                    begin_bit, passes, ws.hist, ws.offsets, ws.state[0],
                                                                     ws.group_sum[0], ws.group_prefix[0], ws.tickets);
   else
     histogram_kernel<false, kBits><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint32_t*>(keys), count,
-                                                                     capacity, begin_bit, passes, ws.hist, ws.state[0],
+                                                                     capacity, begin_bit, passes, ws.hist, ws.offsets,
+                                                                     ws.state[0],
                                                                      ws.group_sum[0], ws.group_prefix[0], ws.tickets);
   DIO_LAUNCHED("radix histogram");
-  digit_offsets_kernel<<<passes, 256, 0, s>>>(ws.hist);
-  DIO_LAUNCHED("radix digit offsets");
   void* kin = keys;
   uint32_t* vin = values;
   void* kout = alt_keys;
@@ -412,11 +426,11 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
     if (wide)
       onesweep_kernel<true, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), vin,
                                                                static_cast<uint2*>(kout), vout, count, capacity, shift,
-                                                               ws.hist + p * 256, now, next, ws.tickets + p);
+                                                               ws.offsets + p * 256, now, next, ws.tickets + p);
     else
       onesweep_kernel<false, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), vin,
                                                                 static_cast<uint32_t*>(kout), vout, count, capacity,
-                                                                shift, ws.hist + p * 256, now, next, ws.tickets + p);
+                                                                shift, ws.offsets + p * 256, now, next, ws.tickets + p);
     DIO_LAUNCHED("radix onesweep");
     std::swap(kin, kout);
     std::swap(vin, vout);

@@ -166,3 +166,41 @@ def test_buffer_copy_and_timestamps():
     assert 0 <= ms < 1000
     with pytest.raises(diorama_amd.DioramaError):
         dst.copy_from(src, nbytes=4000, dst_offset=4)
+
+
+def test_back_to_back_calls_reuse_their_workspace():
+    """A scan or sort leaves its workspace cleared for the next call (no clearing
+    launch precedes a call): 12 calls in a row on one PrefixSum and one RadixSort,
+    each with its own device count, every result checked."""
+    ctx = context()
+    rng = np.random.default_rng(11)
+    capacity = 300_000
+    data = ctx.buffer(capacity * 4)
+    count_buf = ctx.buffer(16)
+    scan = diorama_amd.PrefixSum(ctx, data, capacity, count_buf)
+    sort = diorama_amd.RadixSort(ctx, capacity, 16, count_buf)
+    counts = [capacity, 1, 4097, 123_457, 0, 299_999, 4096, 70_001, capacity, 2, 150_000, 9]
+    say("back-to-back scans and sorts")
+    for n in counts:
+        count_buf.upload(np.array([n, 0, 0, 0], dtype=np.uint32))
+        values = rng.integers(0, 300, size=capacity).astype(np.uint32)
+        data.upload(values)
+        scan.encode()
+        out = data.download(np.uint32, capacity)
+        assert np.array_equal(out[:n], exclusive(values[:n])), n
+        assert int(scan.total.download(np.uint32, 1)[0]) == int(values[:n].sum()), n
+        low = rng.integers(0, 1 << 32, size=capacity, dtype=np.uint64).astype(np.uint32)
+        high = rng.integers(0, 1 << 12, size=capacity).astype(np.uint32)
+        keys = np.stack([low, high], axis=1).reshape(-1)
+        idx = np.arange(capacity, dtype=np.uint32)
+        sort.keys.upload(keys)
+        sort.values.upload(idx)
+        sort.encode()
+        k = sort.keys.download(np.uint32, 2 * capacity).reshape(-1, 2)
+        v = sort.values.download(np.uint32, capacity)
+        full = (high[:n].astype(np.uint64) << 32) | low[:n]
+        want = np.argsort(full, kind="stable").astype(np.uint32)
+        assert np.array_equal(v[:n], want), n
+        assert np.array_equal(k[:n, 0], low[want]) and np.array_equal(k[:n, 1], high[want]), n
+    scan.destroy()
+    sort.destroy()
