@@ -31,9 +31,30 @@ __device__ inline float unorm8(uint32_t v, int channel) {
   return static_cast<float>((v >> (8 * channel)) & 0xffu) / 255.f;
 }
 
+// A tile's halo as LDS rows of kSpan x kSpan values, padded to whole rounds of the
+// 256 threads: every lane of every load round stores somewhere, so the loads carry
+// no condition that would hold them back until their stores.
+constexpr int kHalo = kSpan * kSpan;                 // 676
+constexpr int kHaloRounds = (kHalo + 255) / 256;     // 3
+constexpr int kHaloPad = kHaloRounds * 256;          // 768
+
+// Where halo element e of a tile whose halo starts at (x0, y0) reads from (clamped
+// into the image) and whether it lies inside it (outside, the value is zero).
+struct HaloPixel {
+  size_t index;
+  bool inside;
+};
+__device__ inline HaloPixel halo_pixel(int e, int x0, int y0, uint32_t width, uint32_t height) {
+  const int ec = min(e, kHalo - 1);
+  const int gx = x0 + ec % kSpan, gy = y0 + ec / kSpan;
+  const bool inside = gx >= 0 && gy >= 0 && gx < static_cast<int>(width) && gy < static_cast<int>(height);
+  const int cx = min(max(gx, 0), static_cast<int>(width) - 1), cy = min(max(gy, 0), static_cast<int>(height) - 1);
+  return {static_cast<size_t>(cy) * width + cx, inside};
+}
+
 struct PartialsShared {
-  float x[kSpan][kSpan];
-  float y[kSpan][kSpan];
+  float x[3][kHaloPad];  // per channel, row-major kSpan x kSpan
+  float y[3][kHaloPad];
   float h[5][kSpan][kT];
 };
 
@@ -48,22 +69,31 @@ __device__ inline void ssim_partials_tile(uint32_t tile_x, uint32_t tile_y, uint
   const int lx = tid % kT, ly = tid / kT;
   const int x0 = static_cast<int>(tile_x) * kT - kR, y0 = static_cast<int>(tile_y) * kT - kR;
   const int px = static_cast<int>(tile_x) * kT + lx, py = static_cast<int>(tile_y) * kT + ly;
+  // The halo of all three channels, every load in flight at once.
+  {
+    float4 r[kHaloRounds];
+    uint32_t t[kHaloRounds];
+    bool in[kHaloRounds];
+#pragma unroll
+    for (int k = 0; k < kHaloRounds; k++) {
+      const HaloPixel hp = halo_pixel(tid + k * 256, x0, y0, width, height);
+      r[k] = rendered[hp.index];
+      t[k] = target[hp.index];
+      in[k] = hp.inside;
+    }
+#pragma unroll
+    for (int k = 0; k < kHaloRounds; k++) {
+      const int e = tid + k * 256;
+      s_x[0][e] = in[k] ? r[k].x : 0.f;
+      s_x[1][e] = in[k] ? r[k].y : 0.f;
+      s_x[2][e] = in[k] ? r[k].z : 0.f;
+#pragma unroll
+      for (int ch = 0; ch < 3; ch++) s_y[ch][e] = in[k] ? unorm8(t[k], ch) : 0.f;
+    }
+  }
+  __syncthreads();
   float out[3][4];
   for (int ch = 0; ch < 3; ch++) {
-    for (int e = tid; e < kSpan * kSpan; e += 256) {
-      const int rx = e % kSpan, ry = e / kSpan;
-      const int gx = x0 + rx, gy = y0 + ry;
-      float x = 0.f, y = 0.f;
-      if (gx >= 0 && gy >= 0 && gx < static_cast<int>(width) && gy < static_cast<int>(height)) {
-        const size_t p = static_cast<size_t>(gy) * width + gx;
-        const float4 r = rendered[p];
-        x = ch == 0 ? r.x : (ch == 1 ? r.y : r.z);
-        y = unorm8(target[p], ch);
-      }
-      s_x[ry][rx] = x;
-      s_y[ry][rx] = y;
-    }
-    __syncthreads();
     // Horizontal blur of x, y, x², y², xy for every halo row and the tile's columns.
     for (int e = tid; e < kSpan * kT; e += 256) {
       const int col = e % kT, row = e / kT;
@@ -71,7 +101,7 @@ __device__ inline void ssim_partials_tile(uint32_t tile_x, uint32_t tile_y, uint
 #pragma unroll
       for (int k = -kR; k <= kR; k++) {
         const float w = win.w[k < 0 ? -k : k];
-        const float x = s_x[row][col + kR + k], y = s_y[row][col + kR + k];
+        const float x = s_x[ch][row * kSpan + col + kR + k], y = s_y[ch][row * kSpan + col + kR + k];
         m[0] += w * x;
         m[1] += w * y;
         m[2] += w * (x * x);
@@ -113,7 +143,7 @@ __device__ inline void ssim_partials_tile(uint32_t tile_x, uint32_t tile_y, uint
 }
 
 struct GradientShared {
-  float p[3][kSpan][kSpan];
+  float p[9][kHaloPad];  // (∂μ, ∂E[x²], ∂E[xy]) x channel, row-major kSpan x kSpan
   float h[3][kSpan][kT];
 };
 
@@ -139,24 +169,31 @@ __device__ inline void loss_gradient_tile(uint32_t tile_x, uint32_t tile_y, uint
     t8 = target[me];
     s4 = make_float4(partials[me * 3].w, partials[me * 3 + 1].w, partials[me * 3 + 2].w, 0.f);
   }
+  // The partials' halo for all three channels, every load in flight at once.
+  {
+    float4 v[kHaloRounds][3];
+    bool in[kHaloRounds];
+#pragma unroll
+    for (int k = 0; k < kHaloRounds; k++) {
+      const HaloPixel hp = halo_pixel(tid + k * 256, x0, y0, width, height);
+#pragma unroll
+      for (int q = 0; q < 3; q++) v[k][q] = partials[hp.index * 3 + q];
+      in[k] = hp.inside;
+    }
+#pragma unroll
+    for (int k = 0; k < kHaloRounds; k++) {
+      const int e = tid + k * 256;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        s_p[q * 3][e] = in[k] ? v[k][q].x : 0.f;
+        s_p[q * 3 + 1][e] = in[k] ? v[k][q].y : 0.f;
+        s_p[q * 3 + 2][e] = in[k] ? v[k][q].z : 0.f;
+      }
+    }
+  }
+  __syncthreads();
   float grad[3], share = 0.f;
   for (int ch = 0; ch < 3; ch++) {
-    for (int e = tid; e < kSpan * kSpan; e += 256) {
-      const int rx = e % kSpan, ry = e / kSpan;
-      const int gx = x0 + rx, gy = y0 + ry;
-      float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-      if (gx >= 0 && gy >= 0 && gx < static_cast<int>(width) && gy < static_cast<int>(height)) {
-        const size_t p = (static_cast<size_t>(gy) * width + gx) * 3;
-        const float4 a = partials[p], b = partials[p + 1], c = partials[p + 2];
-        v0 = ch == 0 ? a.x : (ch == 1 ? a.y : a.z);
-        v1 = ch == 0 ? b.x : (ch == 1 ? b.y : b.z);
-        v2 = ch == 0 ? c.x : (ch == 1 ? c.y : c.z);
-      }
-      s_p[0][ry][rx] = v0;
-      s_p[1][ry][rx] = v1;
-      s_p[2][ry][rx] = v2;
-    }
-    __syncthreads();
     for (int e = tid; e < kSpan * kT; e += 256) {
       const int col = e % kT, row = e / kT;
       float m[3] = {0.f, 0.f, 0.f};
@@ -164,7 +201,7 @@ __device__ inline void loss_gradient_tile(uint32_t tile_x, uint32_t tile_y, uint
       for (int k = -kR; k <= kR; k++) {
         const float w = win.w[k < 0 ? -k : k];
 #pragma unroll
-        for (int q = 0; q < 3; q++) m[q] += w * s_p[q][row][col + kR + k];
+        for (int q = 0; q < 3; q++) m[q] += w * s_p[q * 3 + ch][row * kSpan + col + kR + k];
       }
 #pragma unroll
       for (int q = 0; q < 3; q++) s_h[q][row][col] = m[q];
