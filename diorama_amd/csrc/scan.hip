@@ -17,6 +17,7 @@ constexpr int kThreads = 256;
 constexpr int kItems = kScanTile / kThreads;  // 16
 constexpr uint64_t kFlagAggregate = 1ull << 32;
 constexpr uint64_t kFlagPrefix = 2ull << 32;
+constexpr uint32_t kSelfCleanTiles = 1024;
 
 __device__ inline uint32_t wave_inclusive_scan(uint32_t x, int lane) {
 #pragma unroll
@@ -63,10 +64,23 @@ __device__ uint32_t look_back(const uint64_t* state, uint32_t tile, int lane) {
   }
 }
 
-// One tile of the scan (tile < tiles).
+// Counts this workgroup finished with the tile states (lane 0 of a wave); true for
+// the last one of the grid. Without a counter (a large grid, cleared by the host
+// after the kernel instead) it is never the last.
+__device__ inline bool finish(uint32_t* done) {
+  if (!done) return false;
+  wait_memory();  // this wave's publishes have completed
+  return atomicAdd(done, 1u) == gridDim.x - 1;
+}
+
+// One tile of the scan (tile < tiles). Wave 0 counts the workgroup finished with
+// the tile states once it has published its inclusive prefix, before the outputs
+// are written; `last` tells whether it was the last of the grid.
 template <bool kGather>
 __device__ void scan_tile(const uint32_t* in, const uint32_t* __restrict__ idx, uint32_t* out, uint32_t n,
-                          uint32_t tiles, uint32_t tile, uint32_t* total, uint64_t* state) {
+                          uint32_t tiles, uint32_t tile, uint32_t* total, uint64_t* state, uint32_t* done,
+                          bool& last) {
+  __shared__ bool s_last;
   __shared__ uint32_t s_wave[kThreads / kWave];
   __shared__ uint32_t s_prefix;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -117,9 +131,11 @@ __device__ void scan_tile(const uint32_t* in, const uint32_t* __restrict__ idx, 
     if (lane == 0) {
       s_prefix = exclusive;
       if (tile == tiles - 1) *total = exclusive + aggregate;
+      s_last = finish(done);
     }
   }
   __syncthreads();
+  last = s_last;
   const uint32_t add = s_prefix + thread_prefix;
   if (base + kItems <= n) {
     uint4* dst = reinterpret_cast<uint4*>(out + base);
@@ -139,22 +155,25 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, cons
                                                          uint32_t* total, uint64_t* state, uint32_t* ticket,
                                                          uint32_t* done) {
   __shared__ uint32_t s_tile;
-  __shared__ bool s_last;
+  __shared__ bool s_idle_last;
   const int tid = threadIdx.x;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t n = count ? min(*count, capacity) : capacity;
   const uint32_t tiles = n == 0 ? 1 : ceil_div(n, kScanTile);
-  if (tile < tiles) scan_tile<kGather>(in, idx, out, n, tiles, tile, total, state);
   // The states only see device-scope atomics, and a workgroup counts as finished
-  // after its look-back reads and once its own publishes have completed, so the
-  // last one can clear them.
-  wait_memory();
-  __syncthreads();
-  if (tid == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!s_last) return;
+  // after its look-back reads and once its publishes have completed, so the last
+  // one can clear them.
+  bool last;
+  if (tile < tiles) {
+    scan_tile<kGather>(in, idx, out, n, tiles, tile, total, state, done, last);
+  } else {
+    if (tid == 0) s_idle_last = finish(done);
+    __syncthreads();
+    last = s_idle_last;
+  }
+  if (!last) return;
   for (uint32_t k = tid; k < tiles; k += kThreads) publish(state, k, 0);
   if (tid == 0) {
     __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -196,13 +215,19 @@ static void scan_launch(hipStream_t s, bool gather, const uint32_t* in, const ui
                         const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws) {
   uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kScanTile);
   if (tiles > ws.tiles) throw Error(-2, "scan workspace too small");
+  // Up to kSelfCleanTiles workgroups the last one clears the workspace; past that,
+  // one more same-address atomic per workgroup costs more than a clearing launch.
+  const bool self_clean = tiles <= kSelfCleanTiles;
+  uint32_t* done = self_clean ? ws.done : nullptr;
   if (gather)
     scan_kernel<true><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket,
-                                                 ws.done);
+                                                 done);
   else
     scan_kernel<false><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket,
-                                                  ws.done);
+                                                  done);
   DIO_LAUNCHED("scan");
+  // The states, the ticket and the finish counter are one run of words.
+  if (!self_clean) fill_u32(s, reinterpret_cast<uint32_t*>(ws.tile_state), 0u, static_cast<size_t>(ws.tiles) * 2 + 2);
 }
 
 void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,

@@ -204,3 +204,24 @@ def test_back_to_back_calls_reuse_their_workspace():
         assert np.array_equal(k[:n, 0], low[want]) and np.array_equal(k[:n, 1], high[want]), n
     scan.destroy()
     sort.destroy()
+
+
+def test_back_to_back_large_scans_reuse_their_workspace():
+    """Past 1024 tiles the scan's workspace is cleared by a launch after the kernel
+    instead of by its last workgroup; the next call still starts from a clean one."""
+    ctx = context()
+    rng = np.random.default_rng(12)
+    capacity = 5_000_000
+    data = ctx.buffer(capacity * 4)
+    count_buf = ctx.buffer(16)
+    scan = diorama_amd.PrefixSum(ctx, data, capacity, count_buf)
+    say("back-to-back large scans")
+    for n in [capacity, 1000, 4_200_001, 0, capacity]:
+        count_buf.upload(np.array([n, 0, 0, 0], dtype=np.uint32))
+        values = rng.integers(0, 100, size=capacity).astype(np.uint32)
+        data.upload(values)
+        scan.encode()
+        out = data.download(np.uint32, capacity)
+        assert np.array_equal(out[:n], exclusive(values[:n])), n
+        assert int(scan.total.download(np.uint32, 1)[0]) == int(values[:n].astype(np.uint64).sum()), n
+    scan.destroy()
