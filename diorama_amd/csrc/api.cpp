@@ -691,14 +691,15 @@ void fill_background(dio_ctx* ctx, dio_buf* target, uint32_t w, uint32_t h, cons
                 std::min<size_t>(static_cast<size_t>(w) * h, target->bytes / 4));
 }
 
-void renderer_preprocess(dio_renderer* r, const dio_camera& camera) {
+// `vis` (nullable): each Gaussian's (depth, radius) for the trainer's fused backward.
+void renderer_preprocess(dio_renderer* r, const dio_camera& camera, dio_buf* vis = nullptr) {
   to_camera(camera, r->camera);
   auto* sc = r->scene;
   if (!sc) return;
   auto* t = sc->rasterizer;
   dio::preprocess(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
                   dev<float>(sc->splats), dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
-                  dev<uint32_t>(t->depth_values));
+                  dev<uint32_t>(t->depth_values), vis ? dev<float>(vis) : nullptr);
   if (t->clock) t->clock->mark(r->ctx->stream, kMarkPreprocess);
   r->preprocessed = true;
 }
@@ -1062,6 +1063,7 @@ struct dio_trainer {
   std::vector<dio_densification> densifications;
   bool due_densify = false, due_reset = false;
   dio::AdamArgs last_adam{};
+  dio_buf* vis = nullptr;  // each Gaussian's (depth, radius) from the step's preprocess
   uint32_t last_view = 0;
   BatchState* batch = nullptr;  // set for a multi-view batch trainer
   uint32_t generation = 0;      // bumped whenever densification replaces the buffers
@@ -1110,7 +1112,7 @@ void destroy_trainer(dio_trainer* t) {
     release(*p);
   dio_renderer_destroy(t->renderer);
   destroy_loss(t->loss);
-  release(t->params); release(t->moments); release(t->stats);
+  release(t->params); release(t->moments); release(t->stats); release(t->vis);
   for (auto& v : t->views) release(v.image);
   delete t;
 }
@@ -1152,7 +1154,11 @@ void trainer_optimize(dio_trainer* t) {
     t->clock.begin(t->ctx->stream);
     tiles->clock = &t->clock;
   }
-  renderer_preprocess(t->renderer, view.camera);
+  if (!t->vis || t->vis->bytes < static_cast<size_t>(t->count) * 8) {
+    release(t->vis);
+    t->vis = alloc(t->ctx, static_cast<size_t>(t->count) * 8);
+  }
+  renderer_preprocess(t->renderer, view.camera, t->vis);
   renderer_tiles(t->renderer, nullptr, st.background);
   dio::training_loss(t->ctx->stream, t->width, t->height, dev<float>(tiles->pixel_color),
                      dev<uint32_t>(view.image), dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
@@ -1188,7 +1194,7 @@ void trainer_optimize(dio_trainer* t) {
     // one pass; the renderer's per-Gaussian gradient buffers are not written.
     renderer_backward_splats(t->renderer, t->loss->pixel_grads);
     dio::backward_adam(t->ctx->stream, t->count, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree,
-                       t->renderer->camera, dev<float>(sc->splats), dev<float>(sc->grad_splats),
+                       t->renderer->camera, dev<float>(t->vis), dev<float>(sc->grad_splats),
                        dev<float>(t->params), dev<float>(t->moments), a, dev<float>(t->stats),
                        dev<float>(t->learned.extra));
     if (tiles->clock) tiles->clock->mark(t->ctx->stream, kMarkBackwardAdam);
@@ -2063,7 +2069,7 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
     } else if (name == "backward_adam") {
       run = [&] {
         dio::backward_adam(s, t->count, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, r->camera,
-                           dev<float>(sc->splats), dev<float>(sc->grad_splats), dev<float>(t->params),
+                           dev<float>(t->vis), dev<float>(sc->grad_splats), dev<float>(t->params),
                            dev<float>(t->moments), t->last_adam, dev<float>(t->stats), dev<float>(t->learned.extra));
       };
     } else if (name == "adam") {

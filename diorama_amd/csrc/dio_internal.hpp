@@ -97,9 +97,10 @@ struct CameraArgs {
   float viewport[2];
 };
 
+// `vis` (nullable): each Gaussian's (depth, radius) as two floats, radius 0 when culled.
 void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                 const CameraArgs& cam, float* splats, uint32_t* tile_counts, uint32_t* depth_keys,
-                uint32_t* depth_values);
+                uint32_t* depth_values, float* vis = nullptr);
 
 // The tile stage from existing splats: counts (+ depth keys) only.
 void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width, uint32_t height,
@@ -195,10 +196,11 @@ void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const floa
                       const AdamArgs& a, float* stats, float* extra = nullptr);
 // The plain trainer's preprocess backward, statistics and Adam step in one pass
 // (the per-Gaussian gradients stay on chip): reads the activated Gaussians and SH
-// before the step, the splats and their gradients; updates params, SH, moments,
-// the activated copy and the statistics (and `extra`, optional).
+// before the step, each splat's (depth, radius) from the preprocess's `vis` and the
+// splat gradients; updates params, SH, moments, the activated copy and the
+// statistics (and `extra`, optional).
 void backward_adam(hipStream_t s, uint32_t n, float* gaussians, float* sh, uint32_t sh_degree, const CameraArgs& cam,
-                   const float* splats, const float* splat_grads, float* params, float* moments, const AdamArgs& a,
+                   const float* vis, const float* splat_grads, float* params, float* moments, const AdamArgs& a,
                    float* stats, float* extra);
 void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* moments, const float* gaussian_grads,
                const float* sh_grads, const AdamArgs& a, float* gaussians);

@@ -228,7 +228,7 @@ constexpr uint32_t kFusedRow = 35;  // 16 + 15 + 3, odd so lanes' rows start in 
 template <bool kExtra>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void backward_adam_kernel(
     uint32_t n, float* __restrict__ gaussians, float* __restrict__ sh, uint32_t sh_degree, CameraArgs cam,
-    const float4* __restrict__ splats, const float* __restrict__ splat_grads, float* __restrict__ params,
+    const float2* __restrict__ vis, const float* __restrict__ splat_grads, float* __restrict__ params,
     float2* __restrict__ moments, AdamArgs a, float4* __restrict__ stats, float4* __restrict__ extra) {
   __shared__ __align__(16) float s_sh[kFusedGaussians * kShRest];
   __shared__ __align__(16) float s_raw[kFusedGaussians * kFloatsPerGaussian];
@@ -244,11 +244,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   const uint32_t first = (gridDim.x - 1 - blockIdx.x) * kFusedGaussians;
   const uint32_t count = min(kFusedGaussians, n - first);
   // Stage the block's inputs with all loads in flight together; wave 1 fetches the
-  // splats' depth and radius, wave 2 the statistics, wave 3 the extra statistics.
+  // splats' depth and radius (8 bytes each rather than their 48-byte records), wave 2
+  // the statistics, wave 3 the extra statistics.
   const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const uint32_t mine = first + min(lane, count - 1);
   float4 side = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (wave == 1) side = splats[static_cast<size_t>(mine) * 3];
+  if (wave == 1) {
+    const float2 v = vis[mine];
+    side = make_float4(0.f, 0.f, v.x, v.y);
+  }
   if (wave == 2) side = stats[mine];
   if (kExtra && wave == 3) side = extra[mine];
   float* g_sh = sh + static_cast<size_t>(first) * kShRest;
@@ -610,16 +614,16 @@ void densify_demote(hipStream_t s, uint32_t n, const uint32_t* sorted_keys, cons
 }
 
 void backward_adam(hipStream_t s, uint32_t n, float* gaussians, float* sh, uint32_t sh_degree, const CameraArgs& cam,
-                   const float* splats, const float* splat_grads, float* params, float* moments, const AdamArgs& a,
+                   const float* vis, const float* splat_grads, float* params, float* moments, const AdamArgs& a,
                    float* stats, float* extra) {
   if (n == 0) return;
   if (extra)
     backward_adam_kernel<true><<<ceil_div(n, kFusedGaussians), kThreads, 0, s>>>(
-        n, gaussians, sh, sh_degree, cam, reinterpret_cast<const float4*>(splats), splat_grads, params,
+        n, gaussians, sh, sh_degree, cam, reinterpret_cast<const float2*>(vis), splat_grads, params,
         reinterpret_cast<float2*>(moments), a, reinterpret_cast<float4*>(stats), reinterpret_cast<float4*>(extra));
   else
     backward_adam_kernel<false><<<ceil_div(n, kFusedGaussians), kThreads, 0, s>>>(
-        n, gaussians, sh, sh_degree, cam, reinterpret_cast<const float4*>(splats), splat_grads, params,
+        n, gaussians, sh, sh_degree, cam, reinterpret_cast<const float2*>(vis), splat_grads, params,
         reinterpret_cast<float2*>(moments), a, reinterpret_cast<float4*>(stats), nullptr);
   DIO_LAUNCHED("backward_adam");
 }

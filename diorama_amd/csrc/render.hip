@@ -31,7 +31,8 @@ __global__ __launch_bounds__(kThreads) void preprocess_kernel(const float4* __re
                                                                uint32_t n, CameraArgs cam, float4* __restrict__ splats,
                                                                uint32_t* __restrict__ tile_counts,
                                                                uint32_t* __restrict__ depth_keys,
-                                                               uint32_t* __restrict__ depth_values) {
+                                                               uint32_t* __restrict__ depth_values,
+                                                               float2* __restrict__ vis) {
   __shared__ __align__(16) float s_sh[kThreads * kShRest];
   const uint32_t first = blockIdx.x * kThreads, i = first + threadIdx.x;
   const bool active = i < n;
@@ -64,6 +65,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_kernel(const float4* __re
   tile_counts[i] = rec.touched;
   depth_keys[i] = rec.key;
   depth_values[i] = i;
+  if (vis) vis[i] = make_float2(rec.s0.z, rec.s0.w);
 }
 
 // count_tiles (tiles.wgsl:23-34) for splats that did not come from the preprocess.
@@ -370,11 +372,11 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
 
 void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                 const CameraArgs& cam, float* splats, uint32_t* tile_counts, uint32_t* depth_keys,
-                uint32_t* depth_values) {
+                uint32_t* depth_values, float* vis) {
   if (n == 0) return;
   preprocess_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(
       reinterpret_cast<const float4*>(gaussians), sh, sh_degree, n, cam, reinterpret_cast<float4*>(splats),
-      tile_counts, depth_keys, depth_values);
+      tile_counts, depth_keys, depth_values, reinterpret_cast<float2*>(vis));
   DIO_LAUNCHED("preprocess");
 }
 
