@@ -24,6 +24,7 @@ using dio::Error;
 using dio::alloc;
 using dio::dev;
 using dio::guard;
+using dio::guard_read;
 using dio::invalid;
 using dio::need;
 using dio::release;
@@ -73,7 +74,7 @@ extern "C" int dio_ctx_create(int device, dio_ctx** out) {
 }
 
 extern "C" int dio_ctx_synchronize(dio_ctx* ctx) {
-  return guard([&] {
+  return guard_read([&] {
     need(ctx, "ctx");
     sync(ctx);
   });
@@ -131,7 +132,7 @@ extern "C" int dio_buf_upload(dio_buf* buf, size_t offset, const void* src, size
 }
 
 extern "C" int dio_buf_download(const dio_buf* buf, size_t offset, void* dst, size_t bytes) {
-  return guard([&] {
+  return guard_read([&] {
     need(buf, "buffer");
     if (offset + bytes > buf->bytes) invalid("download past the end of the buffer");
     if (bytes == 0) return;
@@ -622,7 +623,7 @@ extern "C" int dio_tile_rasterizer_encode_backward(dio_tile_rasterizer* t, uint3
 }
 
 extern "C" int dio_tile_rasterizer_after_submit(dio_tile_rasterizer* t) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "tile rasterizer");
     tiles_after_submit(t, true);
   });
@@ -849,7 +850,7 @@ extern "C" int dio_renderer_encode_backward(dio_renderer* r, dio_buf* pixel_grad
 }
 
 extern "C" int dio_renderer_after_submit(dio_renderer* r) {
-  return guard([&] {
+  return guard_read([&] {
     need(r, "renderer");
     if (r->scene) tiles_after_submit(r->scene->rasterizer, true);
   });
@@ -1070,6 +1071,23 @@ struct dio_trainer {
   LearnedState learned;
   dio_tuning tuning{16, 64, 8};
   StageClock clock;
+  // The next step's frame, prepared at the end of a plain step by the fused
+  // backward (its preprocess for the next view), and what it was prepared for. The
+  // next step swaps these buffers in instead of running its preprocess while they
+  // are still valid: same iteration, view and SH degree, the same tile rasterizer,
+  // and no mutating call through the C ABI since (mutation_epoch).
+  struct Prepared {
+    dio_buf *splats = nullptr, *tile_counts = nullptr, *depth_keys = nullptr, *depth_values = nullptr,
+            *vis = nullptr;
+    uint32_t count = 0;  // Gaussians the buffers are sized for
+    bool ready = false;
+    uint32_t iteration = 0, view = 0, sh_degree = 0;
+    uint64_t epoch = 0;
+    const dio_tile_rasterizer* tiles = nullptr;
+  } prepared;
+  // A view drawn ahead for the next step (the draw order is the reference's either way).
+  bool has_pending_view = false;
+  uint32_t pending_view = 0;
 };
 
 namespace {
@@ -1101,6 +1119,12 @@ void batch_destroy(BatchState* b) {
   delete b;
 }
 
+void release_prepared(dio_trainer::Prepared& p) {
+  release(p.splats); release(p.tile_counts); release(p.depth_keys); release(p.depth_values); release(p.vis);
+  p.count = 0;
+  p.ready = false;
+}
+
 void destroy_trainer(dio_trainer* t) {
   if (!t) return;
   (void)hipStreamSynchronize(t->ctx->stream);
@@ -1113,11 +1137,13 @@ void destroy_trainer(dio_trainer* t) {
   dio_renderer_destroy(t->renderer);
   destroy_loss(t->loss);
   release(t->params); release(t->moments); release(t->stats); release(t->vis);
+  release_prepared(t->prepared);
   for (auto& v : t->views) release(v.image);
   delete t;
 }
 
-const dio_trainer::View& next_view(dio_trainer* t) {
+// The next view of the shuffled queue (src/trainer.ts:188-199).
+uint32_t draw_view(dio_trainer* t) {
   if (t->queue.empty()) {
     for (uint32_t i = 0; i < t->views.size(); i++) t->queue.push_back(i);
     for (size_t i = t->queue.size() - 1; i > 0; i--) {
@@ -1125,8 +1151,15 @@ const dio_trainer::View& next_view(dio_trainer* t) {
       std::swap(t->queue[i], t->queue[j]);
     }
   }
-  t->last_view = t->queue.back();
+  const uint32_t v = t->queue.back();
   t->queue.pop_back();
+  return v;
+}
+
+// This step's view: the one drawn ahead for it, if any, else the next of the queue.
+const dio_trainer::View& next_view(dio_trainer* t) {
+  t->last_view = t->has_pending_view ? t->pending_view : draw_view(t);
+  t->has_pending_view = false;
   return t->views[t->last_view];
 }
 
@@ -1154,11 +1187,31 @@ void trainer_optimize(dio_trainer* t) {
     t->clock.begin(t->ctx->stream);
     tiles->clock = &t->clock;
   }
-  if (!t->vis || t->vis->bytes < static_cast<size_t>(t->count) * 8) {
-    release(t->vis);
-    t->vis = alloc(t->ctx, static_cast<size_t>(t->count) * 8);
+  auto& prep = t->prepared;
+  const bool use_prepared = prep.ready && prep.iteration == step && prep.view == t->last_view &&
+                            prep.sh_degree == t->sh_degree && prep.tiles == tiles && prep.count == t->count &&
+                            prep.epoch == dio::mutation_epoch().load(std::memory_order_relaxed);
+  prep.ready = false;
+  if (use_prepared) {
+    // The previous step's fused backward already ran this preprocess: swap its
+    // outputs in (the previous frame's buffers take their place for the next one).
+    dio_renderer::Scene* sc = t->renderer->scene;
+    std::swap(sc->splats, prep.splats);
+    tiles->splats = sc->splats;
+    std::swap(tiles->tile_counts, prep.tile_counts);
+    std::swap(tiles->depth_keys, prep.depth_keys);
+    std::swap(tiles->depth_values, prep.depth_values);
+    std::swap(t->vis, prep.vis);
+    to_camera(view.camera, t->renderer->camera);
+    t->renderer->preprocessed = true;
+    if (tiles->clock) tiles->clock->mark(t->ctx->stream, kMarkPreprocess);
+  } else {
+    if (!t->vis || t->vis->bytes < static_cast<size_t>(t->count) * 8) {
+      release(t->vis);
+      t->vis = alloc(t->ctx, static_cast<size_t>(t->count) * 8);
+    }
+    renderer_preprocess(t->renderer, view.camera, t->vis);
   }
-  renderer_preprocess(t->renderer, view.camera, t->vis);
   renderer_tiles(t->renderer, nullptr, st.background);
   dio::training_loss(t->ctx->stream, t->width, t->height, dev<float>(tiles->pixel_color),
                      dev<uint32_t>(view.image), dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
@@ -1191,12 +1244,50 @@ void trainer_optimize(dio_trainer* t) {
                           dev<float>(t->stats), dev<float>(t->learned.extra));
   } else {
     // The rasterizer's backward, then preprocess backward + statistics + Adam in
-    // one pass; the renderer's per-Gaussian gradient buffers are not written.
+    // one pass; the renderer's per-Gaussian gradient buffers are not written. Unless
+    // this step's maintenance resets the opacities, the same pass also runs the next
+    // step's preprocess (next view drawn now, SH degree as that step will have it).
     renderer_backward_splats(t->renderer, t->loss->pixel_grads);
+    const bool reset_next = densifying && st.opacity_reset_interval && step % st.opacity_reset_interval == 0;
+    dio::PrepArgs next;
+    const bool prepare = !reset_next;
+    if (prepare) {
+      if (!t->has_pending_view) {
+        t->pending_view = draw_view(t);
+        t->has_pending_view = true;
+      }
+      uint32_t next_degree = t->sh_degree;
+      if (st.sh_degree_interval && (step + 1) % st.sh_degree_interval == 0 && next_degree < 3) next_degree++;
+      if (prep.count != t->count || !prep.splats) {
+        release_prepared(prep);
+        const size_t n = t->count;
+        prep.splats = alloc(t->ctx, n * dio::kFloatsPerSplat * 4);
+        prep.tile_counts = alloc(t->ctx, n * 4);
+        prep.depth_keys = alloc(t->ctx, n * 4);
+        prep.depth_values = alloc(t->ctx, n * 4);
+        prep.vis = alloc(t->ctx, n * 8);
+        prep.count = t->count;
+      }
+      to_camera(t->views[t->pending_view].camera, next.cam);
+      next.sh_degree = next_degree;
+      next.splats = dev<float>(prep.splats);
+      next.tile_counts = dev<uint32_t>(prep.tile_counts);
+      next.depth_keys = dev<uint32_t>(prep.depth_keys);
+      next.depth_values = dev<uint32_t>(prep.depth_values);
+      next.vis = dev<float>(prep.vis);
+      prep.iteration = step + 1;
+      prep.view = t->pending_view;
+      prep.sh_degree = next_degree;
+      prep.tiles = tiles;
+    }
     dio::backward_adam(t->ctx->stream, t->count, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree,
                        t->renderer->camera, dev<float>(t->vis), dev<float>(sc->grad_splats),
                        dev<float>(t->params), dev<float>(t->moments), a, dev<float>(t->stats),
-                       dev<float>(t->learned.extra));
+                       dev<float>(t->learned.extra), prepare ? &next : nullptr);
+    if (prepare) {
+      prep.epoch = dio::mutation_epoch().load(std::memory_order_relaxed);
+      prep.ready = true;
+    }
     if (tiles->clock) tiles->clock->mark(t->ctx->stream, kMarkBackwardAdam);
   }
   if (tiles->clock) {
@@ -1593,6 +1684,7 @@ void batch_optimize(dio_trainer* t) {
 void trainer_maintain(dio_trainer* t) {
   const bool densify = t->due_densify, reset = t->due_reset;
   t->due_densify = t->due_reset = false;
+  if (densify || reset) t->prepared.ready = false;  // the Gaussians change under the prepared frame
   if (densify) trainer_densify(t, t->iteration > t->settings.opacity_reset_interval);
   if (reset) trainer_reset_opacity(t);
 }
@@ -1738,7 +1830,7 @@ extern "C" int dio_trainer_create_batched(dio_ctx* ctx, const float* params, con
 extern "C" uint32_t dio_trainer_batch_size(const dio_trainer* t) { return t && t->batch ? t->batch->views : 0; }
 
 extern "C" int dio_trainer_step(dio_trainer* t) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     trainer_optimize(t);
     trainer_maintain(t);
@@ -1746,14 +1838,14 @@ extern "C" int dio_trainer_step(dio_trainer* t) {
 }
 
 extern "C" int dio_trainer_optimize(dio_trainer* t) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     trainer_optimize(t);
   });
 }
 
 extern "C" int dio_trainer_maintain(dio_trainer* t) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     trainer_maintain(t);
   });
@@ -1787,7 +1879,7 @@ extern "C" int dio_trainer_export_raw(dio_trainer* t, float* params, float* sh) 
 }
 
 extern "C" int dio_trainer_last_loss(dio_trainer* t, double* out) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     need(out, "out");
     if (!t->batch) {
@@ -2097,7 +2189,7 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
 }
 
 extern "C" int dio_trainer_stage_clock(dio_trainer* t, int enable) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     if (t->batch) invalid("the stage clock times the plain trainer's step; a batch trainer has time_stage(\"batch_step\")");
     t->clock.harvest_all();
@@ -2108,7 +2200,7 @@ extern "C" int dio_trainer_stage_clock(dio_trainer* t, int enable) {
 
 extern "C" int dio_trainer_stage_totals(dio_trainer* t, double* ms, uint32_t capacity, uint32_t* stages,
                                         uint32_t* steps, uint64_t* pairs) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     if (ms && capacity < kMarks - 1) invalid("ms holds " + std::to_string(capacity) + " stages; the step has " +
                                              std::to_string(kMarks - 1));

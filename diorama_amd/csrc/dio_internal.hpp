@@ -199,9 +199,23 @@ void accumulate_stats(hipStream_t s, uint32_t n, const float* splats, const floa
 // before the step, each splat's (depth, radius) from the preprocess's `vis` and the
 // splat gradients; updates params, SH, moments, the activated copy and the
 // statistics (and `extra`, optional).
+// With `prep`, the kernel also runs the next step's preprocess (preprocess with
+// that step's camera and SH degree) on the updated Gaussians it still holds on chip,
+// writing that step's splats, tile counts, depth keys and values and (depth, radius)
+// where `prep` says: the next step then skips its own preprocess and the re-read of
+// the activated Gaussians and SH.
+struct PrepArgs {
+  CameraArgs cam;
+  uint32_t sh_degree = 0;
+  float* splats = nullptr;
+  uint32_t* tile_counts = nullptr;
+  uint32_t* depth_keys = nullptr;
+  uint32_t* depth_values = nullptr;
+  float* vis = nullptr;
+};
 void backward_adam(hipStream_t s, uint32_t n, float* gaussians, float* sh, uint32_t sh_degree, const CameraArgs& cam,
                    const float* vis, const float* splat_grads, float* params, float* moments, const AdamArgs& a,
-                   float* stats, float* extra);
+                   float* stats, float* extra, const PrepArgs* prep = nullptr);
 void adam_step(hipStream_t s, uint32_t n, float* params, float* sh, float* moments, const float* gaussian_grads,
                const float* sh_grads, const AdamArgs& a, float* gaussians);
 // The same step with its arguments read from device memory (for graph replays).

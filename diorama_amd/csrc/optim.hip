@@ -225,11 +225,12 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(uint32_t n, const float
 constexpr uint32_t kFusedGaussians = 64;
 constexpr uint32_t kFusedRow = 35;  // 16 + 15 + 3, odd so lanes' rows start in different banks
 
-template <bool kExtra>
+template <bool kExtra, bool kPrep>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void backward_adam_kernel(
     uint32_t n, float* __restrict__ gaussians, float* __restrict__ sh, uint32_t sh_degree, CameraArgs cam,
     const float2* __restrict__ vis, const float* __restrict__ splat_grads, float* __restrict__ params,
-    float2* __restrict__ moments, AdamArgs a, float4* __restrict__ stats, float4* __restrict__ extra) {
+    float2* __restrict__ moments, AdamArgs a, float4* __restrict__ stats, float4* __restrict__ extra,
+    PrepArgs prep) {
   __shared__ __align__(16) float s_sh[kFusedGaussians * kShRest];
   __shared__ __align__(16) float s_raw[kFusedGaussians * kFloatsPerGaussian];
   __shared__ __align__(16) float s_act[kFusedGaussians * kFloatsPerGaussian];
@@ -366,6 +367,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
     block_copy<kThreads>(g_raw, s_raw, count * kFloatsPerGaussian);
     block_copy<kThreads>(g_sh, s_sh, count * kShRest);
     block_copy<kThreads>(g_act, s_act, count * kFloatsPerGaussian);
+  }
+  if (kPrep && threadIdx.x < count) {
+    // The next step's preprocess, as preprocess_kernel computes it (project, then
+    // splat_record), from the updated activated Gaussian and SH still in LDS: the
+    // same values the step writes out above.
+    const uint32_t t = threadIdx.x, i = first + t;
+    float g[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) g[k] = s_act[t * kFloatsPerGaussian + k];
+    Projection p;
+    const bool visible = project(g, prep.cam, p);
+    SplatRecord rec;
+    splat_record(g, p, visible, prep.cam, s_sh + t * kShRest, sh_count(prep.sh_degree), rec);
+    float4* out = reinterpret_cast<float4*>(prep.splats) + static_cast<size_t>(i) * 3;
+    out[0] = rec.s0;
+    out[1] = rec.s1;
+    out[2] = rec.s2;
+    prep.tile_counts[i] = rec.touched;
+    prep.depth_keys[i] = rec.key;
+    prep.depth_values[i] = i;
+    reinterpret_cast<float2*>(prep.vis)[i] = make_float2(rec.s0.z, rec.s0.w);
   }
 }
 
@@ -615,16 +637,18 @@ void densify_demote(hipStream_t s, uint32_t n, const uint32_t* sorted_keys, cons
 
 void backward_adam(hipStream_t s, uint32_t n, float* gaussians, float* sh, uint32_t sh_degree, const CameraArgs& cam,
                    const float* vis, const float* splat_grads, float* params, float* moments, const AdamArgs& a,
-                   float* stats, float* extra) {
+                   float* stats, float* extra, const PrepArgs* prep) {
   if (n == 0) return;
+  auto launch = [&](auto kernel) {
+    kernel<<<ceil_div(n, kFusedGaussians), kThreads, 0, s>>>(
+        n, gaussians, sh, sh_degree, cam, reinterpret_cast<const float2*>(vis), splat_grads, params,
+        reinterpret_cast<float2*>(moments), a, reinterpret_cast<float4*>(stats), reinterpret_cast<float4*>(extra),
+        prep ? *prep : PrepArgs{});
+  };
   if (extra)
-    backward_adam_kernel<true><<<ceil_div(n, kFusedGaussians), kThreads, 0, s>>>(
-        n, gaussians, sh, sh_degree, cam, reinterpret_cast<const float2*>(vis), splat_grads, params,
-        reinterpret_cast<float2*>(moments), a, reinterpret_cast<float4*>(stats), reinterpret_cast<float4*>(extra));
+    prep ? launch(backward_adam_kernel<true, true>) : launch(backward_adam_kernel<true, false>);
   else
-    backward_adam_kernel<false><<<ceil_div(n, kFusedGaussians), kThreads, 0, s>>>(
-        n, gaussians, sh, sh_degree, cam, reinterpret_cast<const float2*>(vis), splat_grads, params,
-        reinterpret_cast<float2*>(moments), a, reinterpret_cast<float4*>(stats), nullptr);
+    prep ? launch(backward_adam_kernel<false, true>) : launch(backward_adam_kernel<false, false>);
   DIO_LAUNCHED("backward_adam");
 }
 

@@ -2,6 +2,8 @@
 // structs behind the dio_ctx / dio_buf handles, the conversion of thrown errors
 // into status codes, and device allocation.
 #pragma once
+#include <atomic>
+#include <utility>
 #include <algorithm>
 #include <new>
 #include <string>
@@ -27,9 +29,20 @@ namespace dio {
 // The calling thread's message behind dio_last_error().
 std::string& last_error();
 
-// Runs f, turning what it throws into a status code and the thread's message.
+// Counts the calls through the C ABI that may write device memory or change what a
+// trainer computes next: every guarded call but the few that use guard_read. A
+// trainer's frame prepared ahead of its next step is only used while this count
+// has not moved since it was prepared.
+inline std::atomic<uint64_t>& mutation_epoch() {
+  static std::atomic<uint64_t> epoch{0};
+  return epoch;
+}
+
+// Runs f, turning what it throws into a status code and the thread's message,
+// without counting the call as a mutation: for reads, and for the trainer's step
+// calls, which keep their prepared frame consistent themselves.
 template <class F>
-int guard(F&& f) {
+int guard_read(F&& f) {
   try {
     f();
     return DIO_OK;
@@ -43,6 +56,13 @@ int guard(F&& f) {
     last_error() = e.what();
     return DIO_ERR_INTERNAL;
   }
+}
+
+// guard_read for a call that may mutate device memory.
+template <class F>
+int guard(F&& f) {
+  mutation_epoch().fetch_add(1, std::memory_order_relaxed);
+  return guard_read(std::forward<F>(f));
 }
 
 [[noreturn]] inline void invalid(const std::string& message) { throw Error(DIO_ERR_INVALID, message); }

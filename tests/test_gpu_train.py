@@ -243,3 +243,64 @@ def test_stage_clock_times_plain_steps():
     t.step()
     assert t.stage_totals()["steps"] == 0
     t.destroy()
+
+
+def fresh_splats(gaussians, sh, degree, cam):
+    """The splats a fresh preprocess of these activated Gaussians gives for `cam`."""
+    ctx = context()
+    r = diorama_amd.GaussianRenderer(ctx)
+    r.set_gaussians(gaussians, sh, degree)
+    w, h = int(cam["viewport"][0]), int(cam["viewport"][1])
+    target = ctx.buffer(w * h * 4)
+    r.encode(target, cam, [0, 0, 0])
+    out = r.splat_buffer.download(np.float32).reshape(-1, 12)
+    r.destroy()
+    return out
+
+
+def test_prepared_next_frame_equals_a_fresh_preprocess():
+    """A plain step's fused backward also runs the next step's preprocess; that step
+    renders from exactly what a fresh preprocess of the updated Gaussians gives."""
+    params, sh, g, cam = config0_scene(seed=6, n=5000)
+    images = render_views(g, [cam], sh, 3)
+    ctx = context()
+    settings = diorama_amd.default_settings(1.0)
+    settings.update(densify_until=0)
+    say("prepared frame")
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], images, settings)
+    t.iteration = 5000
+    t.sh_degree = 3
+    t.step()  # preprocesses itself, prepares the next frame
+    r = t.renderer
+    g1 = r.gaussian_buffer.download(np.float32).reshape(-1, 16)
+    sh1 = r.sh_buffer.download(np.float32).reshape(-1, 45)
+    t.step()  # renders from the prepared frame
+    got = r.splat_buffer.download(np.float32).reshape(-1, 12)
+    want = fresh_splats(g1, sh1, 3, cam)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    t.destroy()
+
+
+def test_prepared_frame_is_dropped_after_a_buffer_write():
+    """A device write through the C ABI between steps (here new activated positions)
+    invalidates the prepared frame: the next step preprocesses what is there now."""
+    params, sh, g, cam = config0_scene(seed=7, n=3000)
+    images = render_views(g, [cam], sh, 3)
+    ctx = context()
+    settings = diorama_amd.default_settings(1.0)
+    settings.update(densify_until=0)
+    say("prepared frame dropped")
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], images, settings)
+    t.iteration = 5000
+    t.sh_degree = 3
+    t.step()
+    r = t.renderer
+    g1 = r.gaussian_buffer.download(np.float32).reshape(-1, 16)
+    sh1 = r.sh_buffer.download(np.float32).reshape(-1, 45)
+    g1[:, 0] += 0.05
+    r.gaussian_buffer.upload(g1)
+    t.step()
+    got = r.splat_buffer.download(np.float32).reshape(-1, 12)
+    want = fresh_splats(g1, sh1, 3, cam)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    t.destroy()
