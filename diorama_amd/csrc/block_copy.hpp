@@ -23,12 +23,13 @@ __device__ inline void block_copy(float* __restrict__ dst, const float* __restri
 // at once. In the last round the lanes past the run take element q mod kVec, which
 // round 0 also copies (same data, same address), so no load or store is conditional
 // and none can be sunk into a branch that waits for it.
-template <uint32_t kVec, uint32_t kBlock>
+// Threads kFirst .. kFirst + kBlock - 1 of the workgroup do the copy.
+template <uint32_t kVec, uint32_t kBlock, uint32_t kFirst = 0>
 struct RunCopy {
   static constexpr uint32_t kRounds = (kVec + kBlock - 1) / kBlock;
   float4 v[kRounds];
   __device__ static uint32_t index(uint32_t k) {
-    const uint32_t q = threadIdx.x + k * kBlock;
+    const uint32_t q = (threadIdx.x - kFirst) + k * kBlock;
     return k + 1 == kRounds && kRounds * kBlock > kVec ? q % kVec : q;
   }
   __device__ void load(const float* __restrict__ src) {
@@ -42,8 +43,8 @@ struct RunCopy {
 };
 
 // A full block of kRecords records of `kFloats` floats each.
-template <uint32_t kRecords, uint32_t kFloats, uint32_t kBlock>
-struct RecordCopy : RunCopy<kRecords * kFloats / 4, kBlock> {
+template <uint32_t kRecords, uint32_t kFloats, uint32_t kBlock, uint32_t kFirst = 0>
+struct RecordCopy : RunCopy<kRecords * kFloats / 4, kBlock, kFirst> {
   static_assert(kRecords * kFloats % 4 == 0, "a full block's records are whole float4s");
 };
 

@@ -354,7 +354,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
     m_base[f] = m[k];
   }
   __syncthreads();
-  if (count == kFusedGaussians) {
+  if (kPrep && count == kFusedGaussians) {
+    // Waves 1-3 write the block back while wave 0 runs the next step's preprocess
+    // below; both only read LDS.
+    if (threadIdx.x >= kWave) {
+      RecordCopy<kFusedGaussians, kShRest, kThreads - kWave, kWave> c_sh;
+      RecordCopy<kFusedGaussians, kFloatsPerGaussian, kThreads - kWave, kWave> c_raw, c_act;
+      c_sh.load(s_sh);
+      c_raw.load(s_raw);
+      c_act.load(s_act);
+      c_sh.store(g_sh);
+      c_raw.store(g_raw);
+      c_act.store(g_act);
+    }
+  } else if (count == kFusedGaussians) {
     RecordCopy<kFusedGaussians, kShRest, kThreads> c_sh;
     RecordCopy<kFusedGaussians, kFloatsPerGaussian, kThreads> c_raw, c_act;
     c_sh.load(s_sh);
