@@ -36,7 +36,10 @@ HBM_PEAK_GBS = 8000.0
 # pair, per pixel, per tile. Sort (the depth sort and the tile sort together) is
 # charged at the reference's 6 x 8-bit pass model over the 48 significant key bits.
 STAGE_BYTES = {
-    "preprocess": (284, 0, 0, 0),
+    # Runs inside the previous step's backward_adam (below) while that step prepared
+    # this one's frame, which every timed step after the first does; the stage then
+    # holds only its marker.
+    "preprocess": (0, 0, 0, 0),
     "scan": (8, 0, 0, 0),
     "sort": (0, 144, 0, 0),
     "duplicate": (20, 12, 0, 0),
@@ -47,8 +50,11 @@ STAGE_BYTES = {
     "rasterize_backward": (36, 40, 32, 0),
     # preprocess backward + statistics + Adam in one pass: R activated 56, raw 56, SH 180,
     # splat 4 + 36 grads, moments 472, stats 16; W raw 56, SH 180, moments 472,
-    # activated 56, stats 16 (the reference's separate passes move 512 + 44 + 1708)
-    "backward_adam": (1600, 0, 0, 0),
+    # activated 56, stats 16 (the reference's separate passes move 512 + 44 + 1708);
+    # plus the next step's preprocess from the same on-chip values: W splat 48, tile
+    # count 4, depth key 4 and value 4, (depth, radius) 8 (its reads of the activated
+    # Gaussians and SH are the ones above)
+    "backward_adam": (1668, 0, 0, 0),
 }
 # Clocked stages summed into a reported one.
 STAGE_PARTS = {"sort": ("depth_sort", "tile_sort")}
