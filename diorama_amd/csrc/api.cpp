@@ -81,12 +81,17 @@ extern "C" int dio_ctx_synchronize(dio_ctx* ctx) {
 }
 
 extern "C" void dio_ctx_destroy(dio_ctx* ctx) {
-  if (!ctx) return;
+  if (!ctx || ctx->closed) return;
   (void)hipStreamSynchronize(ctx->stream);
-  for (hipEvent_t e : ctx->stamps)
-    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t& e : ctx->stamps)
+    if (e) {
+      (void)hipEventDestroy(e);
+      e = nullptr;
+    }
   (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  ctx->stream = nullptr;  // later syncs of buffers or objects that outlive it touch no stream
+  ctx->closed = true;
+  if (ctx->refs == 0) delete ctx;
 }
 
 extern "C" int dio_ctx_timestamp(dio_ctx* ctx, uint32_t slot) {
@@ -165,7 +170,7 @@ extern "C" size_t dio_buf_size(const dio_buf* buf) { return buf ? buf->bytes : 0
 
 extern "C" void dio_buf_destroy(dio_buf* buf) {
   if (!buf) return;
-  (void)hipStreamSynchronize(buf->ctx->stream);
+  if (!buf->ctx->closed) (void)hipStreamSynchronize(buf->ctx->stream);
   release(buf);
 }
 
@@ -189,6 +194,7 @@ extern "C" int dio_prefix_sum_create(dio_ctx* ctx, dio_buf* data, uint32_t capac
     if (static_cast<size_t>(capacity) * 4 > data->bytes) invalid("the data buffer is smaller than the capacity");
     auto* s = new dio_prefix_sum;
     s->ctx = ctx;
+    dio::hold(s->ctx);
     s->data = data;
     s->count = count;
     s->capacity = capacity;
@@ -219,7 +225,9 @@ extern "C" void dio_prefix_sum_destroy(dio_prefix_sum* s) {
   (void)hipStreamSynchronize(s->ctx->stream);
   release(s->total);
   release(s->ws);
+  dio_ctx* owner = s->ctx;
   delete s;
+  dio::drop(owner);
 }
 
 // ================================================================= RadixSort
@@ -240,6 +248,7 @@ extern "C" int dio_radix_sort_create(dio_ctx* ctx, uint32_t capacity, uint32_t h
     if (high_bits > 32) invalid("highBits must be at most 32");
     std::unique_ptr<dio_radix_sort> s(new dio_radix_sort);
     s->ctx = ctx;
+    dio::hold(s->ctx);
     s->capacity = capacity;
     s->high_bits = high_bits;
     s->wide = high_bits > 0;
@@ -280,7 +289,9 @@ extern "C" void dio_radix_sort_destroy(dio_radix_sort* s) {
   if (!s) return;
   (void)hipStreamSynchronize(s->ctx->stream);
   release(s->keys); release(s->values); release(s->alt_keys); release(s->alt_values); release(s->ws);
+  dio_ctx* owner = s->ctx;
   delete s;
+  dio::drop(owner);
 }
 
 // ============================================================ TileRasterizer
@@ -443,12 +454,15 @@ void destroy_tiles(dio_tile_rasterizer* t) {
   release(t->pixel_last);
   if (t->readback_host) (void)hipHostFree(t->readback_host);
   if (t->readback_done) (void)hipEventDestroy(t->readback_done);
+  dio_ctx* owner = t->ctx;
   delete t;
+  dio::drop(owner);
 }
 
 dio_tile_rasterizer* create_tiles(dio_ctx* ctx, dio_buf* splats, uint32_t count, uint32_t initial) {
   auto* t = new dio_tile_rasterizer;
   t->ctx = ctx;
+  dio::hold(t->ctx);
   t->splats = splats;
   t->count = count;
   try {
@@ -746,6 +760,7 @@ extern "C" int dio_renderer_create(dio_ctx* ctx, uint32_t initial_pair_capacity,
     need(out, "out");
     auto* r = new dio_renderer;
     r->ctx = ctx;
+    dio::hold(r->ctx);
     r->initial_pair_capacity = initial_pair_capacity;
     *out = r;
   });
@@ -885,7 +900,9 @@ extern "C" void dio_renderer_destroy(dio_renderer* r) {
   if (!r) return;
   destroy_scene(r);
   destroy_tiles(r->draw_rasterizer);
+  dio_ctx* owner = r->ctx;
   delete r;
+  dio::drop(owner);
 }
 
 // ============================================================== TrainingLoss
@@ -901,7 +918,9 @@ void destroy_loss(dio_training_loss* l) {
   if (!l) return;
   (void)hipStreamSynchronize(l->ctx->stream);
   release(l->partials); release(l->pixel_grads); release(l->pixel_loss);
+  dio_ctx* owner = l->ctx;
   delete l;
+  dio::drop(owner);
 }
 
 double loss_read(dio_training_loss* l) {
@@ -920,6 +939,7 @@ extern "C" int dio_training_loss_create(dio_ctx* ctx, uint32_t width, uint32_t h
     need(out, "out");
     auto* l = new dio_training_loss;
     l->ctx = ctx;
+    dio::hold(l->ctx);
     l->width = width;
     l->height = height;
     const size_t px = std::max<size_t>(static_cast<size_t>(width) * height, 1);
@@ -1139,7 +1159,9 @@ void destroy_trainer(dio_trainer* t) {
   release(t->params); release(t->moments); release(t->stats); release(t->vis);
   release_prepared(t->prepared);
   for (auto& v : t->views) release(v.image);
+  dio_ctx* owner = t->ctx;
   delete t;
+  dio::drop(owner);
 }
 
 // The next view of the shuffled queue (src/trainer.ts:188-199).
@@ -1744,6 +1766,7 @@ void trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t
     auto* t = new dio_trainer;
     try {
       t->ctx = ctx;
+      dio::hold(t->ctx);
       t->settings = *settings;
       t->count = count;
       t->width = static_cast<uint32_t>(w);
@@ -1761,6 +1784,7 @@ void trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t
       t->stats = alloc(ctx, static_cast<size_t>(count) * 16, true);
       t->renderer = new dio_renderer;
       t->renderer->ctx = ctx;
+      dio::hold(t->renderer->ctx);
       // The renderer gets the activated copy; from here on Adam keeps it in step.
       dio_buf* g = alloc(ctx, static_cast<size_t>(count) * dio::kFloatsPerGaussian * 4, true);
       dio_buf* s = nullptr;
@@ -2226,7 +2250,9 @@ void destroy_mlp(dio_densify_mlp* m) {
   (void)hipStreamSynchronize(m->ctx->stream);
   release(m->weights); release(m->master); release(m->moments); release(m->grads); release(m->loss);
   release(m->norm);
+  dio_ctx* owner = m->ctx;
   delete m;
+  dio::drop(owner);
 }
 
 void mlp_upload_master(dio_densify_mlp* m, const float* w) {
@@ -2247,6 +2273,7 @@ extern "C" int dio_mlp_create(dio_ctx* ctx, uint32_t seed, dio_densify_mlp** out
     need(out, "out");
     auto* m = new dio_densify_mlp;
     m->ctx = ctx;
+    dio::hold(m->ctx);
     try {
       m->weights = alloc(ctx, dio::kMlpParams * 2, true);
       m->master = alloc(ctx, dio::kMlpParams * 4, true);

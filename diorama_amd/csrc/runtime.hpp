@@ -16,6 +16,11 @@ struct dio_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t stamps[DIO_TIMESTAMP_SLOTS] = {};
   bool stamped[DIO_TIMESTAMP_SLOTS] = {};
+  // Buffers and objects of this context still alive. A destroyed context that they
+  // outlive (a caller may free them afterwards) stays allocated, with no stream,
+  // until the last one is released.
+  uint64_t refs = 0;
+  bool closed = false;
 };
 
 struct dio_buf {
@@ -73,6 +78,12 @@ inline void need(const void* p, const char* what) {
 
 // A device buffer of `bytes` (at least 256 are reserved), zero-filled on the
 // context's stream when asked.
+// A buffer or object of `ctx` comes into being / goes away.
+inline void hold(dio_ctx* ctx) { ctx->refs++; }
+inline void drop(dio_ctx* ctx) {
+  if (ctx && --ctx->refs == 0 && ctx->closed) delete ctx;
+}
+
 inline dio_buf* alloc(dio_ctx* ctx, size_t bytes, bool zero = false) {
   auto* b = new dio_buf;
   b->ctx = ctx;
@@ -83,6 +94,7 @@ inline dio_buf* alloc(dio_ctx* ctx, size_t bytes, bool zero = false) {
     (void)hipGetLastError();
     throw Error(DIO_ERR_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
   }
+  hold(ctx);
   if (zero) fill_u32(ctx->stream, static_cast<uint32_t*>(b->ptr), 0u, (std::max<size_t>(bytes, 256) + 3) / 4);
   return b;
 }
@@ -90,8 +102,10 @@ inline dio_buf* alloc(dio_ctx* ctx, size_t bytes, bool zero = false) {
 inline void release(dio_buf*& b) {
   if (b) {
     (void)hipFree(b->ptr);
+    dio_ctx* ctx = b->ctx;
     delete b;
     b = nullptr;
+    drop(ctx);
   }
 }
 

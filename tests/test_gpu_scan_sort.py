@@ -225,3 +225,18 @@ def test_back_to_back_large_scans_reuse_their_workspace():
         assert np.array_equal(out[:n], exclusive(values[:n])), n
         assert int(scan.total.download(np.uint32, 1)[0]) == int(values[:n].astype(np.uint64).sum()), n
     scan.destroy()
+
+
+def test_buffers_and_objects_outlive_their_context():
+    """Buffers and objects destroyed after their context was closed (as a garbage
+    collector may do at exit) free cleanly; the context goes with the last of them."""
+    ctx = diorama_amd.Context(0)
+    b = ctx.buffer(1024)
+    b.upload(np.arange(256, dtype=np.uint32))
+    r = diorama_amd.GaussianRenderer(ctx)
+    s = diorama_amd.PrefixSum(ctx, b, 256, None)
+    s.encode()
+    ctx.close()
+    s.destroy()
+    r.destroy()
+    b.destroy()
