@@ -74,6 +74,9 @@ const char* dio_last_error(void);
 const char* dio_version(void);
 int dio_ctx_create(int device, dio_ctx** out);
 int dio_ctx_synchronize(dio_ctx* ctx);
+/* Waits for the stream and destroys it. Buffers and objects of the context may
+ * still be destroyed afterwards (nothing else may be done with them); the
+ * context's own memory goes with the last of them. */
 void dio_ctx_destroy(dio_ctx* ctx);
 /* Timestamp queries (the 'timestamp-query' feature, src/main.ts:40): record a
  * marker in one of DIO_TIMESTAMP_SLOTS slots on the context's stream, then read
