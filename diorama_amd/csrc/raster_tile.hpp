@@ -129,31 +129,41 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
       const SplatRecords& rec = s_records[wave][q];
       const uint32_t base = round + q * kChunk;
       uint64_t mask = __ballot(hits[q]);
-      while (mask) {
-        const int j = __builtin_ctzll(mask);
-        mask &= mask - 1;
+      // Two splats per iteration: both alphas first (they do not depend on T), then
+      // the two blends in order, exactly as one at a time.
+      auto alpha_of = [&](int j, float& alpha, float& sr, float& sg, float& sb) {
         const float4 ra = rec.a[j], rc = rec.c[j];
-        const float mx = ra.x, my = ra.y, ha = ra.z, hb = ra.w;
-        const float hc = rc.x, op = rc.y, sr = rc.z, sg = rc.w, sb = rec.b[j];
-        if (!done) {
-          const float power2 = splat_power2(cx - mx, cy - my, ha, hb, hc);
-          if (power2 <= 0.f) {
-            const float alpha = fminf(0.99f, op * __builtin_amdgcn_exp2f(power2));
-            if (alpha >= 1.f / 255.f) {
-              const float remaining = T * (1.f - alpha);
-              if (remaining < 0.0001f) {
-                done = true;
-              } else {
-                const float w = alpha * T;
-                cr += sr * w;
-                cg += sg * w;
-                cb += sb * w;
-                T = remaining;
-                last = base + j - run.x + 1;
-              }
-            }
-          }
+        sr = rc.z;
+        sg = rc.w;
+        sb = rec.b[j];
+        const float power2 = splat_power2(cx - ra.x, cy - ra.y, ra.z, ra.w, rc.x);
+        const float a = fminf(0.99f, rc.y * __builtin_amdgcn_exp2f(power2));
+        return power2 <= 0.f && a >= 1.f / 255.f ? (alpha = a, true) : (alpha = 0.f, false);
+      };
+      auto blend = [&](int j, float alpha, float sr, float sg, float sb) {
+        const float remaining = T * (1.f - alpha);
+        if (remaining < 0.0001f) {
+          done = true;
+        } else {
+          const float w = alpha * T;
+          cr += sr * w;
+          cg += sg * w;
+          cb += sb * w;
+          T = remaining;
+          last = base + j - run.x + 1;
         }
+      };
+      while (mask) {
+        const int j0 = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int j1 = mask ? __builtin_ctzll(mask) : j0;
+        const bool second = mask != 0;
+        mask &= mask - 1;
+        float a0, r0, g0, b0, a1, r1, g1, b1;
+        const bool v0 = alpha_of(j0, a0, r0, g0, b0);
+        const bool v1 = alpha_of(j1, a1, r1, g1, b1) && second;
+        if (!done && v0) blend(j0, a0, r0, g0, b0);
+        if (!done && v1) blend(j1, a1, r1, g1, b1);
       }
     }
     wave_sync();  // every lane has read this round's records before the next round writes
