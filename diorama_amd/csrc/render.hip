@@ -181,9 +181,7 @@ __global__ __launch_bounds__(kThreads) void duplicate_kernel(const float4* __res
   }
 }
 
-// find_ranges (tiles.wgsl:64-84): tile boundaries in the sorted pairs. Ranges
-// are cleared first, so tiles without pairs read [0, 0).
-// find_ranges (tiles.wgsl:61-79): each tile's [start, end) run of the sorted pairs,
+// find_ranges (tiles.wgsl:61-84): each tile's [start, end) run of the sorted pairs,
 // (0, 0) for a tile without pairs, one thread per pair. The pair at a change of
 // tile writes the end of the previous tile, the start of its own and the empty
 // tiles between them (the first and last pairs also the tiles before and after
