@@ -401,9 +401,11 @@ struct dio_tile_rasterizer {
   dio_buf* splats = nullptr;  // not owned
   uint32_t count = 0;
   uint32_t capacity = 0;
-  // Per splat: tile counts, depth sort keys/values (+ scratch), pair offsets.
+  // Per splat: tile counts, depth sort keys/values (+ scratch), the tile counts in
+  // depth order (written by the depth sort's last pass), pair offsets.
   dio_buf *tile_counts = nullptr, *depth_keys = nullptr, *depth_values = nullptr, *depth_alt_keys = nullptr,
-          *depth_alt_values = nullptr, *offsets = nullptr, *splat_scan_ws = nullptr, *splat_sort_ws = nullptr;
+          *depth_alt_values = nullptr, *sorted_counts = nullptr, *offsets = nullptr, *splat_scan_ws = nullptr,
+          *splat_sort_ws = nullptr;
   dio_buf* pair_total = nullptr;  // element 0: pairs this frame needs
   // Pairs: tile index and splat, ping-pong buffers for the tile sort.
   dio_buf *pair_tiles = nullptr, *pair_splats = nullptr, *alt_tiles = nullptr, *alt_splats = nullptr,
@@ -445,7 +447,8 @@ void destroy_tiles(dio_tile_rasterizer* t) {
   if (!t) return;
   (void)hipStreamSynchronize(t->ctx->stream);
   release(t->tile_counts); release(t->depth_keys); release(t->depth_values); release(t->depth_alt_keys);
-  release(t->depth_alt_values); release(t->offsets); release(t->splat_scan_ws); release(t->splat_sort_ws);
+  release(t->depth_alt_values); release(t->sorted_counts); release(t->offsets); release(t->splat_scan_ws);
+  release(t->splat_sort_ws);
   release(t->pair_total);
   free_pairs(t);
   release(t->ranges);
@@ -472,6 +475,7 @@ dio_tile_rasterizer* create_tiles(dio_ctx* ctx, dio_buf* splats, uint32_t count,
     t->depth_values = alloc(ctx, n * 4);
     t->depth_alt_keys = alloc(ctx, n * 4);
     t->depth_alt_values = alloc(ctx, n * 4);
+    t->sorted_counts = alloc(ctx, n * 4);
     t->offsets = alloc(ctx, n * 4);
     t->splat_scan_ws = alloc(ctx, dio::scan_workspace_bytes(count), true);
     t->splat_sort_ws = alloc(ctx, dio::sort_workspace_bytes(count), true);
@@ -526,18 +530,19 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   if (!counts_ready)
     dio::count_tiles(s, splats, t->count, w, h, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
                      dev<uint32_t>(t->depth_values));
-  // Depth order of the splats (culled ones last), then each splat's first pair slot.
+  // Depth order of the splats (culled ones last), with their tile counts in that
+  // order, then each splat's first pair slot.
   if (dio::radix_sort(s, false, t->depth_keys->ptr, dev<uint32_t>(t->depth_values), t->depth_alt_keys->ptr,
                       dev<uint32_t>(t->depth_alt_values), nullptr, t->count, 0, 32,
-                      dio::sort_workspace(t->splat_sort_ws->ptr, t->count), t->digit_bits))
+                      dio::sort_workspace(t->splat_sort_ws->ptr, t->count), t->digit_bits,
+                      dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->sorted_counts)))
     throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
   auto mark = [&](StageMark k) {
     if (t->clock) t->clock->mark(s, k);
   };
   mark(kMarkDepthSort);
-  dio::scan_exclusive_gather(s, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_values),
-                             dev<uint32_t>(t->offsets), nullptr, t->count, dev<uint32_t>(t->pair_total),
-                             dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
+  dio::scan_exclusive(s, dev<uint32_t>(t->sorted_counts), dev<uint32_t>(t->offsets), nullptr, t->count,
+                      dev<uint32_t>(t->pair_total), dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
   mark(kMarkScan);
   dio::duplicate(s, splats, dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets), dev<uint32_t>(t->tile_counts),
                  t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total), dev<uint32_t>(t->pair_tiles),
@@ -1012,7 +1017,8 @@ struct BatchState {
   uint32_t capacity = 0;                // (splat, tile) pairs
   dio_buf *desc = nullptr, *adam = nullptr;
   dio_buf *splats = nullptr, *tile_counts = nullptr, *depth_keys = nullptr, *depth_values = nullptr,
-          *depth_alt_keys = nullptr, *depth_alt_values = nullptr, *offsets = nullptr, *scan_ws = nullptr,
+          *depth_alt_keys = nullptr, *depth_alt_values = nullptr, *sorted_counts = nullptr, *offsets = nullptr,
+          *scan_ws = nullptr,
           *sort_ws = nullptr;
   dio_buf *pair_total = nullptr, *pair_tiles = nullptr, *pair_splats = nullptr, *alt_tiles = nullptr,
           *alt_splats = nullptr, *pair_sort_ws = nullptr, *ranges = nullptr;
@@ -1118,7 +1124,7 @@ void batch_free_frames(BatchState* b) {
   b->exec = nullptr;
   b->graph = nullptr;
   for (dio_buf** p : {&b->splats, &b->tile_counts, &b->depth_keys, &b->depth_values, &b->depth_alt_keys,
-                      &b->depth_alt_values, &b->offsets, &b->scan_ws, &b->sort_ws, &b->pair_total, &b->pair_tiles,
+                      &b->depth_alt_values, &b->sorted_counts, &b->offsets, &b->scan_ws, &b->sort_ws, &b->pair_total, &b->pair_tiles,
                       &b->pair_splats, &b->alt_tiles, &b->alt_splats, &b->pair_sort_ws, &b->splat_grads,
                       &b->grad_gaussians, &b->grad_sh})
     release(*p);
@@ -1513,6 +1519,7 @@ void batch_alloc_frames(dio_trainer* t, uint32_t capacity) {
   b->depth_values = alloc(ctx, total * 4);
   b->depth_alt_keys = alloc(ctx, total * 4);
   b->depth_alt_values = alloc(ctx, total * 4);
+  b->sorted_counts = alloc(ctx, total * 4);
   b->offsets = alloc(ctx, total * 4);
   b->scan_ws = alloc(ctx, dio::scan_workspace_bytes(static_cast<uint32_t>(total)), true);
   b->sort_ws = alloc(ctx, dio::sort_workspace_bytes(static_cast<uint32_t>(total)), true);
@@ -1555,11 +1562,11 @@ void batch_launch(dio_trainer* t, bool with_adam) {
   const dio_tuning& tune = t->tuning;
   if (dio::radix_sort(s, false, b->depth_keys->ptr, dev<uint32_t>(b->depth_values), b->depth_alt_keys->ptr,
                       dev<uint32_t>(b->depth_alt_values), nullptr, total, 0, 32,
-                      dio::sort_workspace(b->sort_ws->ptr, total), tune.digit_bits))
+                      dio::sort_workspace(b->sort_ws->ptr, total), tune.digit_bits, dev<uint32_t>(b->tile_counts),
+                      dev<uint32_t>(b->sorted_counts)))
     throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
-  dio::scan_exclusive_gather(s, dev<uint32_t>(b->tile_counts), dev<uint32_t>(b->depth_values),
-                             dev<uint32_t>(b->offsets), nullptr, total, dev<uint32_t>(b->pair_total),
-                             dio::scan_workspace(b->scan_ws->ptr, total));
+  dio::scan_exclusive(s, dev<uint32_t>(b->sorted_counts), dev<uint32_t>(b->offsets), nullptr, total,
+                      dev<uint32_t>(b->pair_total), dio::scan_workspace(b->scan_ws->ptr, total));
   dio::duplicate_batch(s, dev<float>(b->splats), dev<uint32_t>(b->depth_values), dev<uint32_t>(b->offsets),
                        dev<uint32_t>(b->tile_counts), a, b->capacity, dev<uint32_t>(b->pair_total),
                        dev<uint32_t>(b->pair_tiles), dev<uint32_t>(b->pair_splats));
@@ -2104,9 +2111,8 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       };
     } else if (name == "scan") {
       run = [&] {
-        dio::scan_exclusive_gather(s, dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_values),
-                                   dev<uint32_t>(tr->offsets), nullptr, tr->count, dev<uint32_t>(tr->pair_total),
-                                   dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
+        dio::scan_exclusive(s, dev<uint32_t>(tr->sorted_counts), dev<uint32_t>(tr->offsets), nullptr, tr->count,
+                            dev<uint32_t>(tr->pair_total), dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
       };
     } else if (name == "sort") {
       // The whole binning sort (Gaussians by depth, then pairs by tile) on the data a
@@ -2120,10 +2126,10 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
         if (sorts)
           dio::radix_sort(s, false, tr->depth_keys->ptr, dev<uint32_t>(tr->depth_values), tr->depth_alt_keys->ptr,
                           dev<uint32_t>(tr->depth_alt_values), nullptr, tr->count, 0, 32,
-                          dio::sort_workspace(tr->splat_sort_ws->ptr, tr->count), tr->digit_bits);
-        dio::scan_exclusive_gather(s, dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_values),
-                                   dev<uint32_t>(tr->offsets), nullptr, tr->count, dev<uint32_t>(tr->pair_total),
-                                   dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
+                          dio::sort_workspace(tr->splat_sort_ws->ptr, tr->count), tr->digit_bits,
+                          dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->sorted_counts));
+        dio::scan_exclusive(s, dev<uint32_t>(tr->sorted_counts), dev<uint32_t>(tr->offsets), nullptr, tr->count,
+                            dev<uint32_t>(tr->pair_total), dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
         dio::duplicate(s, dev<float>(sc->splats), dev<uint32_t>(tr->depth_values), dev<uint32_t>(tr->offsets),
                        dev<uint32_t>(tr->tile_counts), tr->count, tr->frame_w, tr->frame_h, tr->capacity,
                        dev<uint32_t>(tr->pair_total), dev<uint32_t>(tr->pair_tiles), dev<uint32_t>(tr->pair_splats));

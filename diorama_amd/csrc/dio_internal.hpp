@@ -63,10 +63,6 @@ ScanWorkspace scan_workspace(void* base, uint32_t capacity);
 void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
                     uint32_t* total, const ScanWorkspace& ws);
 
-// Exclusive scan of in[idx[j]] for j < count: the counts gathered in another order.
-void scan_exclusive_gather(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t* out,
-                           const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws);
-
 // ------------------------------------------------------------------ sort --
 // Stable LSD onesweep radix sort, 8- or 4-bit digits, of u32 keys or u64 keys stored as
 // (low word, high word), each carrying a u32 value. Sorts key bits
@@ -86,9 +82,12 @@ struct SortWorkspace {
 size_t sort_workspace_bytes(uint32_t capacity);
 SortWorkspace sort_workspace(void* base, uint32_t capacity);
 // digit_bits: 8 or 4 (more passes of fewer buckets); the result is the same.
+// With gather_src and gather_out, the last pass also writes gather_out[i] =
+// gather_src[sorted value i] (e.g. each depth-sorted splat's tile count).
 bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
                 const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
-                const SortWorkspace& ws, uint32_t digit_bits = 8);
+                const SortWorkspace& ws, uint32_t digit_bits = 8, const uint32_t* gather_src = nullptr,
+                uint32_t* gather_out = nullptr);
 
 // ------------------------------------------------------------ renderer --
 struct CameraArgs {

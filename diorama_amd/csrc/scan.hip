@@ -76,8 +76,7 @@ __device__ inline bool finish(uint32_t* done) {
 // One tile of the scan (tile < tiles). Wave 0 counts the workgroup finished with
 // the tile states once it has published its inclusive prefix, before the outputs
 // are written; `last` tells whether it was the last of the grid.
-template <bool kGather>
-__device__ void scan_tile(const uint32_t* in, const uint32_t* __restrict__ idx, uint32_t* out, uint32_t n,
+__device__ void scan_tile(const uint32_t* in, uint32_t* out, uint32_t n,
                           uint32_t tiles, uint32_t tile, uint32_t* total, uint64_t* state, uint32_t* done,
                           bool& last) {
   __shared__ bool s_last;
@@ -86,7 +85,7 @@ __device__ void scan_tile(const uint32_t* in, const uint32_t* __restrict__ idx, 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const uint32_t base = tile * kScanTile + tid * kItems;
   uint32_t v[kItems];
-  if (!kGather && base + kItems <= n) {
+  if (base + kItems <= n) {
     const uint4* src = reinterpret_cast<const uint4*>(in + base);
 #pragma unroll
     for (int q = 0; q < kItems / 4; q++) {
@@ -97,7 +96,7 @@ __device__ void scan_tile(const uint32_t* in, const uint32_t* __restrict__ idx, 
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
       uint32_t i = base + k;
-      v[k] = i < n ? (kGather ? in[idx[i]] : in[i]) : 0u;
+      v[k] = i < n ? in[i] : 0u;
     }
   }
   uint32_t sum = 0;
@@ -149,9 +148,7 @@ __device__ void scan_tile(const uint32_t* in, const uint32_t* __restrict__ idx, 
   }
 }
 
-template <bool kGather>
-__global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, const uint32_t* __restrict__ idx,
-                                                         uint32_t* out, const uint32_t* count, uint32_t capacity,
+__global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
                                                          uint32_t* total, uint64_t* state, uint32_t* ticket,
                                                          uint32_t* done) {
   __shared__ uint32_t s_tile;
@@ -167,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, cons
   // one can clear them.
   bool last;
   if (tile < tiles) {
-    scan_tile<kGather>(in, idx, out, n, tiles, tile, total, state, done, last);
+    scan_tile(in, out, n, tiles, tile, total, state, done, last);
   } else {
     if (tid == 0) s_idle_last = finish(done);
     __syncthreads();
@@ -211,33 +208,18 @@ ScanWorkspace scan_workspace(void* base, uint32_t capacity) {
   return ws;
 }
 
-static void scan_launch(hipStream_t s, bool gather, const uint32_t* in, const uint32_t* idx, uint32_t* out,
-                        const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws) {
+void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
+                    uint32_t* total, const ScanWorkspace& ws) {
   uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kScanTile);
   if (tiles > ws.tiles) throw Error(-2, "scan workspace too small");
   // Up to kSelfCleanTiles workgroups the last one clears the workspace; past that,
   // one more same-address atomic per workgroup costs more than a clearing launch.
   const bool self_clean = tiles <= kSelfCleanTiles;
   uint32_t* done = self_clean ? ws.done : nullptr;
-  if (gather)
-    scan_kernel<true><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket,
-                                                 done);
-  else
-    scan_kernel<false><<<tiles, kThreads, 0, s>>>(in, idx, out, count, capacity, total, ws.tile_state, ws.ticket,
-                                                  done);
+  scan_kernel<<<tiles, kThreads, 0, s>>>(in, out, count, capacity, total, ws.tile_state, ws.ticket, done);
   DIO_LAUNCHED("scan");
   // The states, the ticket and the finish counter are one run of words.
   if (!self_clean) fill_u32(s, reinterpret_cast<uint32_t*>(ws.tile_state), 0u, static_cast<size_t>(ws.tiles) * 2 + 2);
-}
-
-void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
-                    uint32_t* total, const ScanWorkspace& ws) {
-  scan_launch(s, false, in, nullptr, out, count, capacity, total, ws);
-}
-
-void scan_exclusive_gather(hipStream_t s, const uint32_t* in, const uint32_t* idx, uint32_t* out,
-                           const uint32_t* count, uint32_t capacity, uint32_t* total, const ScanWorkspace& ws) {
-  scan_launch(s, true, in, idx, out, count, capacity, total, ws);
 }
 
 }  // namespace dio

@@ -173,7 +173,9 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
                                                              uint32_t* __restrict__ values_out, const uint32_t* count,
                                                              uint32_t capacity, uint32_t shift,
                                                              const uint32_t* __restrict__ digit_start, PassState now,
-                                                             PassState next, uint32_t* ticket) {
+                                                             PassState next, uint32_t* ticket,
+                                                             const uint32_t* __restrict__ gather_src,
+                                                             uint32_t* __restrict__ gather_out) {
   using Ops = KeyOps<kWide, kBits>;
   using Key = typename Ops::T;
   constexpr int kRadix = 1 << kBits;  // thread d < kRadix owns digit d
@@ -350,6 +352,26 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
   }
   __syncthreads();
   const uint32_t tile_n = min(static_cast<uint32_t>(kSortTile), n - base);
+  if (gather_out && tile_n == kSortTile) {
+    // A full tile of a pass that also writes gather_src[value] beside each value:
+    // every gather load of a thread is issued before its first store, and nothing
+    // is conditional, so no load waits inside a branch of its own store.
+    uint32_t dst[kItems], gathered[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+      const uint32_t p = k * kThreads + tid;
+      const Key kk = staged_keys[p];
+      const uint32_t d = Ops::digit(kk, shift);
+      const uint32_t v = staged_values[p];
+      dst[k] = run_start[d] + p - local_start[d];
+      gathered[k] = gather_src[v];
+      keys_out[dst[k]] = kk;
+      values_out[dst[k]] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; k++) gather_out[dst[k]] = gathered[k];
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < kItems; k++) {
     const uint32_t p = k * kThreads + tid;
@@ -357,8 +379,10 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
       const Key kk = staged_keys[p];
       const uint32_t d = Ops::digit(kk, shift);
       const uint32_t dst = run_start[d] + p - local_start[d];
+      const uint32_t v = staged_values[p];
       keys_out[dst] = kk;
-      values_out[dst] = staged_values[p];
+      values_out[dst] = v;
+      if (gather_out) gather_out[dst] = gather_src[v];
     }
   }
 }
@@ -398,7 +422,7 @@ namespace {
 template <int kBits>
 bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
                      const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
-                     const SortWorkspace& ws) {
+                     const SortWorkspace& ws, const uint32_t* gather_src, uint32_t* gather_out) {
   const uint32_t tiles = ceil_div(capacity, kSortTile);
   const uint32_t passes = (end_bit - begin_bit + kBits - 1) / kBits;
   if (passes > kMaxPasses) throw Error(-2, "radix sort: too many passes");
@@ -423,14 +447,18 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
     const PassState next = p + 1 < passes
                                ? PassState{ws.state[(p + 1) % 2], ws.group_sum[(p + 1) % 2], ws.group_prefix[(p + 1) % 2]}
                                : PassState{nullptr, nullptr, nullptr};
+    const bool last = p + 1 == passes;
+    const uint32_t* const src = last ? gather_src : nullptr;
+    uint32_t* const out = last ? gather_out : nullptr;
     if (wide)
       onesweep_kernel<true, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), vin,
                                                                static_cast<uint2*>(kout), vout, count, capacity, shift,
-                                                               ws.offsets + p * 256, now, next, ws.tickets + p);
+                                                               ws.offsets + p * 256, now, next, ws.tickets + p, src, out);
     else
       onesweep_kernel<false, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), vin,
                                                                 static_cast<uint32_t*>(kout), vout, count, capacity,
-                                                                shift, ws.offsets + p * 256, now, next, ws.tickets + p);
+                                                                shift, ws.offsets + p * 256, now, next, ws.tickets + p, src,
+                                                                out);
     DIO_LAUNCHED("radix onesweep");
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -442,16 +470,20 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
 
 bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
                 const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
-                const SortWorkspace& ws, uint32_t digit_bits) {
+                const SortWorkspace& ws, uint32_t digit_bits, const uint32_t* gather_src, uint32_t* gather_out) {
   if (digit_bits != 8 && digit_bits != 4) throw Error(-1, "radix digit width must be 4 or 8 bits");
   if (begin_bit % digit_bits != 0) throw Error(-2, "radix sort: begin bit must be a multiple of the digit width");
   if (end_bit > (wide ? 64u : 32u)) throw Error(-2, "radix sort: end bit past the key");
   if (capacity >= (1u << 30)) throw Error(-2, "radix sort: at most 2^30 keys");
+  if (!gather_src != !gather_out) throw Error(-1, "radix sort: a gather needs both its source and its output");
+  if (gather_out && end_bit <= begin_bit) throw Error(-1, "radix sort: a gather needs at least one pass");
   if (capacity == 0 || end_bit <= begin_bit) return false;
   if (ceil_div(capacity, kSortTile) > ws.tiles) throw Error(-2, "sort workspace too small");
   return digit_bits == 8
-             ? radix_sort_bits<8>(s, wide, keys, values, alt_keys, alt_values, count, capacity, begin_bit, end_bit, ws)
-             : radix_sort_bits<4>(s, wide, keys, values, alt_keys, alt_values, count, capacity, begin_bit, end_bit, ws);
+             ? radix_sort_bits<8>(s, wide, keys, values, alt_keys, alt_values, count, capacity, begin_bit, end_bit, ws,
+                                    gather_src, gather_out)
+             : radix_sort_bits<4>(s, wide, keys, values, alt_keys, alt_values, count, capacity, begin_bit, end_bit, ws,
+                                    gather_src, gather_out);
 }
 
 }  // namespace dio
