@@ -45,8 +45,10 @@ STAGE_BYTES = {
     "duplicate": (20, 12, 0, 0),
     "ranges": (0, 4, 0, 8),
     "rasterize": (0, 40, 24, 0),
-    "loss": (0, 0, 32, 0),
-    "clear_grads": (36, 0, 0, 0),
+    # The loss's partials kernel also zeroes the splat gradients (36 B/Gaussian) that the
+    # rasterizer's backward adds into; the clear stage then holds only its marker.
+    "loss": (36, 0, 32, 0),
+    "clear_grads": (0, 0, 0, 0),
     "rasterize_backward": (36, 40, 32, 0),
     # preprocess backward + statistics + Adam in one pass: R activated 56, raw 56, SH 180,
     # splat 4 + 36 grads, moments 472, stats 16; W raw 56, SH 180, moments 472,

@@ -592,10 +592,12 @@ void tiles_after_submit(dio_tile_rasterizer* t, bool wait) {
   }
 }
 
-// encodeBackward (src/tileRasterizer.ts:200-222) of the frame encoded last.
-void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pixel_grads, dio_buf* splat_grads) {
+// encodeBackward (src/tileRasterizer.ts:200-222) of the frame encoded last. The pass
+// adds into splat_grads, which is cleared first unless the caller already did.
+void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pixel_grads, dio_buf* splat_grads,
+                    bool cleared = false) {
   hipStream_t s = t->ctx->stream;
-  dio::fill_u32(s, dev<uint32_t>(splat_grads), 0u, static_cast<size_t>(t->count) * 9);  // the pass adds into it
+  if (!cleared) dio::fill_u32(s, dev<uint32_t>(splat_grads), 0u, static_cast<size_t>(t->count) * 9);
   if (t->clock) t->clock->mark(s, kMarkClearGrads);
   dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
   dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges),
@@ -735,23 +737,29 @@ void renderer_tiles(dio_renderer* r, dio_buf* target, const float bg[3]) {
   encode_tiles(r->scene->rasterizer, target, w, h, bg, true);
 }
 
-// The rasterizer's half of encodeBackward: per-splat gradients into grad_splats.
-void renderer_backward_splats(dio_renderer* r, dio_buf* pixel_grads) {
+void ensure_grad_buffers(dio_renderer* r) {
   auto* sc = r->scene;
   if (!sc->grad_splats) {
     sc->grad_splats = alloc(r->ctx, static_cast<size_t>(sc->count) * 9 * 4, true);
     sc->grad_gaussians = alloc(r->ctx, static_cast<size_t>(sc->count) * dio::kFloatsPerGaussian * 4, true);
     sc->grad_sh = alloc(r->ctx, static_cast<size_t>(sc->count) * dio::kShRest * 4, true);
   }
-  auto* t = sc->rasterizer;
-  if (!t->framed) invalid("Encode a frame before its backward pass.");
-  backward_tiles(t, t->frame_w, t->frame_h, pixel_grads, sc->grad_splats);
 }
 
-void renderer_backward(dio_renderer* r, dio_buf* pixel_grads) {
+// The rasterizer's half of encodeBackward: per-splat gradients into grad_splats
+// (already zeroed when `cleared`).
+void renderer_backward_splats(dio_renderer* r, dio_buf* pixel_grads, bool cleared = false) {
+  auto* sc = r->scene;
+  ensure_grad_buffers(r);
+  auto* t = sc->rasterizer;
+  if (!t->framed) invalid("Encode a frame before its backward pass.");
+  backward_tiles(t, t->frame_w, t->frame_h, pixel_grads, sc->grad_splats, cleared);
+}
+
+void renderer_backward(dio_renderer* r, dio_buf* pixel_grads, bool cleared = false) {
   auto* sc = r->scene;
   if (!sc) return;
-  renderer_backward_splats(r, pixel_grads);
+  renderer_backward_splats(r, pixel_grads, cleared);
   dio::preprocess_backward(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count,
                            r->camera, dev<float>(sc->splats), dev<float>(sc->grad_splats),
                            dev<float>(sc->grad_gaussians), dev<float>(sc->grad_sh));
@@ -1241,9 +1249,12 @@ void trainer_optimize(dio_trainer* t) {
     renderer_preprocess(t->renderer, view.camera, t->vis);
   }
   renderer_tiles(t->renderer, nullptr, st.background);
+  // The loss's partials kernel also clears the splat gradients the backward adds into.
+  ensure_grad_buffers(t->renderer);
   dio::training_loss(t->ctx->stream, t->width, t->height, dev<float>(tiles->pixel_color),
                      dev<uint32_t>(view.image), dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
-                     dev<float>(t->loss->pixel_loss));
+                     dev<float>(t->loss->pixel_loss), dev<uint32_t>(t->renderer->scene->grad_splats),
+                     static_cast<size_t>(t->renderer->scene->count) * 9);
   if (tiles->clock) tiles->clock->mark(t->ctx->stream, kMarkLoss);
   if (!densify_now) t->adam_steps++;
   const double progress = std::min(1.0, static_cast<double>(step) / st.position_lr_steps);
@@ -1267,7 +1278,7 @@ void trainer_optimize(dio_trainer* t) {
   auto* sc = t->renderer->scene;
   if (densify_now) {
     // No Adam step on a densification step: the gradients feed only the statistics.
-    renderer_backward(t->renderer, t->loss->pixel_grads);
+    renderer_backward(t->renderer, t->loss->pixel_grads, true);
     dio::accumulate_stats(t->ctx->stream, t->count, dev<float>(sc->splats), dev<float>(sc->grad_splats), a,
                           dev<float>(t->stats), dev<float>(t->learned.extra));
   } else {
@@ -1275,7 +1286,7 @@ void trainer_optimize(dio_trainer* t) {
     // one pass; the renderer's per-Gaussian gradient buffers are not written. Unless
     // this step's maintenance resets the opacities, the same pass also runs the next
     // step's preprocess (next view drawn now, SH degree as that step will have it).
-    renderer_backward_splats(t->renderer, t->loss->pixel_grads);
+    renderer_backward_splats(t->renderer, t->loss->pixel_grads, true);
     const bool reset_next = densifying && st.opacity_reset_interval && step % st.opacity_reset_interval == 0;
     dio::PrepArgs next;
     const bool prepare = !reset_next;
@@ -2186,7 +2197,8 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       run = [&] {
         dio::training_loss(s, t->width, t->height, dev<float>(tr->pixel_color), dev<uint32_t>(view.image),
                            dev<float>(t->loss->partials), dev<float>(t->loss->pixel_grads),
-                           dev<float>(t->loss->pixel_loss));
+                           dev<float>(t->loss->pixel_loss), dev<uint32_t>(sc->grad_splats),
+                           static_cast<size_t>(sc->count) * 9);
       };
     } else if (name == "backward_adam") {
       run = [&] {

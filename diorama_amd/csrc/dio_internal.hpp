@@ -179,8 +179,12 @@ void preprocess_backward_batch(hipStream_t s, const float* gaussians, const floa
                                float* gaussian_grads, float* sh_grads, float* stats, float* extra = nullptr);
 
 // ---------------------------------------------------------------- loss --
+// `clear` (optional, 16-byte aligned): `clear_words` words that the partials kernel
+// zeroes on the side (the trainer's splat gradients, which the rasterizer's backward
+// adds into next); the loss kernels are compute-bound and leave HBM time to spare.
 void training_loss(hipStream_t s, uint32_t width, uint32_t height, const float* rendered, const uint32_t* target,
-                   float* partials, float* pixel_grads, float* pixel_loss);
+                   float* partials, float* pixel_grads, float* pixel_loss, uint32_t* clear = nullptr,
+                   size_t clear_words = 0);
 
 // ----------------------------------------------------------- optimizer --
 struct AdamArgs {
