@@ -304,3 +304,33 @@ def test_prepared_frame_is_dropped_after_a_buffer_write():
     want = fresh_splats(g1, sh1, 3, cam)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     t.destroy()
+
+
+def test_step_gradients_start_from_zero():
+    """The rasterizer's backward adds into the splat gradients, which the step's loss
+    kernel clears: a second step's gradients equal a fresh backward of its own frame
+    (loss and backward through the renderer's own calls, which clear with a fill)."""
+    params, sh, g, cam = config0_scene(seed=8, n=4000)
+    images = render_views(g, [cam], sh, 3)
+    ctx = context()
+    settings = diorama_amd.default_settings(1.0)
+    settings.update(densify_until=0)
+    say("gradients start from zero")
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], images, settings)
+    t.iteration = 5000
+    t.sh_degree = 3
+    t.step()
+    t.step()
+    r = t.renderer
+    got = r.gradients["splats"].download(np.float32).reshape(-1, 9)
+    w, h = int(cam["viewport"][0]), int(cam["viewport"][1])
+    loss = diorama_amd.TrainingLoss(ctx, w, h)
+    loss.encode(r.pixel_buffer, ctx.upload(images[0]))
+    r.encode_backward(loss.pixel_grads)
+    want = r.gradients["splats"].download(np.float32).reshape(-1, 9)
+    scale = np.abs(want).max()
+    assert scale > 0
+    # float atomics add in varying order; a missed clear would add the first step's gradients
+    assert np.abs(got - want).max() <= 1e-4 * scale
+    loss.destroy()
+    t.destroy()
