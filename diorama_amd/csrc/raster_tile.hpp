@@ -204,19 +204,38 @@ __device__ inline float swap_sum(decltype(__builtin_amdgcn_permlane32_swap(0u, 0
   return __builtin_bit_cast(float, r0) + __builtin_bit_cast(float, r1);
 }
 
-// Four splats' per-lane values (a, b, c, d) to four wave totals at once.
+// Four splats' per-lane values (a, b, c, d) = v[0..3][q] to four wave totals at once,
+// for each of the kGrads gradients q.
 // v_permlane32_swap(x, y) returns (x.lo y.lo, x.hi y.hi) by 32-lane halves and
 // v_permlane16_swap(x, y) (x.r0 y.r0 x.r2 y.r2, x.r1 y.r1 x.r3 y.r3) by 16-lane
 // rows: swapping halves and adding leaves a|c and b|d by half, swapping rows and
 // adding leaves a|b|c|d by row, and one DPP row reduction finishes all four, so
-// row r of the wave ends with splat r's total.
-__device__ inline float four_totals(float a, float b, float c, float d) {
-  const float lo = swap_sum(__builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a),
-                                                             __builtin_bit_cast(uint32_t, c), false, false));
-  const float hi = swap_sum(__builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, b),
-                                                             __builtin_bit_cast(uint32_t, d), false, false));
-  return row_total(swap_sum(__builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, lo),
-                                                             __builtin_bit_cast(uint32_t, hi), false, false)));
+// row r of the wave ends with splat r's total. The gradients go one step at a time
+// across all of them: each cross-lane step then has eight independent ones between
+// it and the step it depends on, which fill the DPP and permlane read-after-write
+// wait states that one gradient's chain alone pads with s_nop.
+__device__ inline void four_totals_all(const float (&v)[4][kGrads], float (&total)[kGrads]) {
+  float lo[kGrads], hi[kGrads];
+#pragma unroll
+  for (int q = 0; q < kGrads; q++)
+    lo[q] = swap_sum(__builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, v[0][q]),
+                                                      __builtin_bit_cast(uint32_t, v[2][q]), false, false));
+#pragma unroll
+  for (int q = 0; q < kGrads; q++)
+    hi[q] = swap_sum(__builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, v[1][q]),
+                                                      __builtin_bit_cast(uint32_t, v[3][q]), false, false));
+#pragma unroll
+  for (int q = 0; q < kGrads; q++)
+    total[q] = swap_sum(__builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, lo[q]),
+                                                         __builtin_bit_cast(uint32_t, hi[q]), false, false));
+#pragma unroll
+  for (int q = 0; q < kGrads; q++) total[q] = dpp_add<0xb1>(total[q]);
+#pragma unroll
+  for (int q = 0; q < kGrads; q++) total[q] = dpp_add<0x4e>(total[q]);
+#pragma unroll
+  for (int q = 0; q < kGrads; q++) total[q] = dpp_add<0x141>(total[q]);
+#pragma unroll
+  for (int q = 0; q < kGrads; q++) total[q] = dpp_add<0x140>(total[q]);
 }
 
 __device__ inline uint32_t wave_max(uint32_t v) {
@@ -391,10 +410,11 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
         }
         if (!__any(any)) continue;
         const int mine = row == 0 ? js[0] : (row == 1 ? js[1] : (row == 2 ? js[2] : js[3]));
+        float total[kGrads];
+        four_totals_all(share, total);
+        if (row_leader && mine >= 0) {
 #pragma unroll
-        for (int v = 0; v < kGrads; v++) {
-          const float total = four_totals(share[0][v], share[1][v], share[2][v], share[3][v]);
-          if (row_leader && mine >= 0) s_sums[wave][v][mine] = total;
+          for (int v = 0; v < kGrads; v++) s_sums[wave][v][mine] = total[v];
         }
       }
     }
