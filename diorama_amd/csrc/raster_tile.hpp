@@ -245,10 +245,14 @@ __device__ inline uint32_t wave_max(uint32_t v) {
 }
 
 // The per-pixel replay state of the backward pass (rasterizeBackward.wgsl:161-177).
+// The colour behind a splat and the next blended splat's colour enter its gradient
+// only through their dot products with dL/dpixel, and the update
+// behind += a_next (next - behind) is linear, so the replay carries those two dot
+// products (behind, next) rather than the colours: the same sums, 8 fewer VALU
+// operations per splat and pixel.
 struct Replay {
   float T, final_bg, gr, gg, gb;  // final_bg = final T x (background . dL/dpixel)
-  float behind_r, behind_g, behind_b;
-  float next_alpha, next_r, next_g, next_b;
+  float behind, next_alpha, next;
 };
 
 // One splat's 9 gradient terms at one pixel (rasterizeBackward.wgsl:199-230),
@@ -270,16 +274,11 @@ __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy
   const float keep = __builtin_amdgcn_rcpf(1.f - alpha);  // one hardware reciprocal for both divisions; 1 when idle
   st.T *= keep;
   const float na = valid ? st.next_alpha : 0.f;
-  st.behind_r += na * (st.next_r - st.behind_r);
-  st.behind_g += na * (st.next_g - st.behind_g);
-  st.behind_b += na * (st.next_b - st.behind_b);
+  st.behind += na * (st.next - st.behind);
+  const float color = sr * st.gr + sg * st.gg + sb * st.gb;  // this splat's colour . dL/dpixel
   st.next_alpha = valid ? alpha : st.next_alpha;
-  st.next_r = valid ? sr : st.next_r;
-  st.next_g = valid ? sg : st.next_g;
-  st.next_b = valid ? sb : st.next_b;
-  const float alpha_grad =
-      st.T * ((sr - st.behind_r) * st.gr + (sg - st.behind_g) * st.gg + (sb - st.behind_b) * st.gb) -
-      st.final_bg * keep;
+  st.next = valid ? color : st.next;
+  const float alpha_grad = st.T * (color - st.behind) - st.final_bg * keep;
   const float opacity_grad = valid ? alpha_grad * G : 0.f;
   const float g = opacity_grad * op;  // dL/dpower
   const float gdx = g * dx, gdy = g * dy;

@@ -440,22 +440,34 @@ void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, 
   DIO_LAUNCHED("rasterize");
 }
 
+// The backward runs fastest with at most 5 workgroups (20 waves) per CU: measured at
+// 64 splats a round, 7 took 371 µs, 6 369 µs and 5 365.5 µs (its 72 VGPRs would let 7
+// in). Dynamic LDS pads a workgroup to just over a sixth of the CU's 160 KB.
+constexpr uint32_t kBackwardGroupsPerCu = 5;
+
+template <int kPer>
+constexpr size_t backward_lds_pad() {
+  constexpr size_t group = 160 * 1024 / (kBackwardGroupsPerCu + 1) + 1;
+  return sizeof(raster::BackwardShared<kPer>) >= group ? 0 : group - sizeof(raster::BackwardShared<kPer>);
+}
+
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
                         float* splat_grads, uint32_t splat_batch) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
-  auto launch = [&](auto kernel) {
-    kernel<<<tx * ty, kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
-                                        reinterpret_cast<const uint2*>(ranges), order, width, height, tx, background[0],
-                                        background[1], background[2], reinterpret_cast<const float4*>(pixel_color),
-                                        pixel_last, reinterpret_cast<const float4*>(pixel_grads), splat_grads);
+  auto launch = [&](auto kernel, size_t pad) {
+    kernel<<<tx * ty, kThreads, pad, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                          reinterpret_cast<const uint2*>(ranges), order, width, height, tx,
+                                          background[0], background[1], background[2],
+                                          reinterpret_cast<const float4*>(pixel_color), pixel_last,
+                                          reinterpret_cast<const float4*>(pixel_grads), splat_grads);
   };
   switch (splat_batch) {
-    case 64: launch(rasterize_backward_kernel<1>); break;
-    case 128: launch(rasterize_backward_kernel<2>); break;
-    case 256: launch(rasterize_backward_kernel<4>); break;
+    case 64: launch(rasterize_backward_kernel<1>, backward_lds_pad<1>()); break;
+    case 128: launch(rasterize_backward_kernel<2>, backward_lds_pad<2>()); break;
+    case 256: launch(rasterize_backward_kernel<4>, backward_lds_pad<4>()); break;
     default: throw Error(-1, "splat batch must be 64, 128 or 256");
   }
   DIO_LAUNCHED("rasterize_backward");
