@@ -544,19 +544,23 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   dio::scan_exclusive(s, dev<uint32_t>(t->sorted_counts), dev<uint32_t>(t->offsets), nullptr, t->count,
                       dev<uint32_t>(t->pair_total), dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
   mark(kMarkScan);
+  // The pair count is known on the device only; the last frame's (0 before the first
+  // readback) sizes the grids of the pair kernels, which loop over whatever it is.
+  const uint32_t pair_hint = t->last_pair_count;
   dio::duplicate(s, splats, dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets), dev<uint32_t>(t->tile_counts),
                  t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total), dev<uint32_t>(t->pair_tiles),
-                 dev<uint32_t>(t->pair_splats));
+                 dev<uint32_t>(t->pair_splats), pair_hint);
   mark(kMarkDuplicate);
   t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats), t->alt_tiles->ptr,
                                      dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total), t->capacity, 0,
                                      tile_sort_bits(tiles, t->digit_bits),
-                                     dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity), t->digit_bits);
+                                     dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity), t->digit_bits, nullptr,
+                                     nullptr, pair_hint ? pair_hint + pair_hint / 8 + 4096 : 0);
   mark(kMarkTileSort);
   dio_buf* sorted_tiles = t->sorted_in_alt ? t->alt_tiles : t->pair_tiles;
   dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
   dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(t->pair_total), t->capacity, tiles,
-                   dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order));
+                   dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), pair_hint);
   mark(kMarkRanges);
   dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), w,
                  h, bg, dev<uint32_t>(target), dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),

@@ -84,10 +84,13 @@ SortWorkspace sort_workspace(void* base, uint32_t capacity);
 // digit_bits: 8 or 4 (more passes of fewer buckets); the result is the same.
 // With gather_src and gather_out, the last pass also writes gather_out[i] =
 // gather_src[sorted value i] (e.g. each depth-sorted splat's tile count).
+// count_hint (0: none): the key count the host expects for a device-side `count`,
+// which sizes the work the idle part of the capacity-sized grids does; any value
+// gives the same result.
 bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
                 const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
                 const SortWorkspace& ws, uint32_t digit_bits = 8, const uint32_t* gather_src = nullptr,
-                uint32_t* gather_out = nullptr);
+                uint32_t* gather_out = nullptr, uint32_t count_hint = 0);
 
 // ------------------------------------------------------------ renderer --
 struct CameraArgs {
@@ -105,14 +108,16 @@ void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t
 void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width, uint32_t height,
                  uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values);
 
+// pair_hint (0: none), here and in find_ranges: the pair count the host expects,
+// which sizes the grid; the kernels loop over whatever *pair_total holds.
 void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
                const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
-               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats);
+               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_hint = 0);
 
 // Each tile's [start, end) run of the sorted pairs; with `order` (nullable, one
 // word per tile) also the rasterizer's launch order, heaviest tiles first.
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
-                 uint32_t tiles, uint32_t* ranges, uint32_t* order = nullptr);
+                 uint32_t tiles, uint32_t* ranges, uint32_t* order = nullptr, uint32_t pair_hint = 0);
 
 // splat_batch: pairs a wave stages per round (64, 128 or 256); `order` (nullable)
 // maps workgroups to tiles. Results depend on neither.

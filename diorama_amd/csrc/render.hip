@@ -90,7 +90,8 @@ __global__ __launch_bounds__(kThreads) void count_tiles_kernel(const float4* __r
 }
 
 // duplicate (tiles.wgsl:36-59), load-balanced over pairs rather than splats: each
-// workgroup writes a fixed 2048-slot stretch of the pair buffers. It finds the
+// workgroup writes 2048-slot stretches of the pair buffers (one, unless the pairs
+// outgrow the grid sized for the expected count). For a stretch it finds the
 // depth-ordered splat owning its first slot with a 64-ary search of the offsets
 // (one wavefront, a few rounds), then deals its slots out to threads from LDS
 // windows of 256 splats. The nearest splats cover the most tiles and sit together
@@ -115,74 +116,74 @@ __global__ __launch_bounds__(kThreads) void duplicate_kernel(const float4* __res
   __shared__ uint32_t s_first;
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const uint32_t total = min(*pair_total, capacity);  // slots past the capacity are dropped
-  const uint32_t p0 = blockIdx.x * kDupPairs;
-  if (p0 >= total) return;
-  const uint32_t p_end = min(p0 + kDupPairs, total);
-  if (tid < kWave) {
-    // The last j with offsets[j] <= p0 (offsets never decrease; offsets[0] = 0).
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-      const uint32_t stride = ceil_div(hi - lo, kWave);
-      const uint32_t probe = lo + lane * stride;
-      const uint64_t ok = __ballot(probe < hi && offsets[probe] <= p0);
-      lo += static_cast<uint32_t>(kWave - 1 - __builtin_clzll(ok)) * stride;
-      hi = min(hi, lo + stride);
+  for (uint32_t p0 = blockIdx.x * kDupPairs; p0 < total; p0 += gridDim.x * kDupPairs) {
+    const uint32_t p_end = min(p0 + kDupPairs, total);
+    if (tid < kWave) {
+      // The last j with offsets[j] <= p0 (offsets never decrease; offsets[0] = 0).
+      uint32_t lo = 0, hi = n;
+      while (hi - lo > 1) {
+        const uint32_t stride = ceil_div(hi - lo, kWave);
+        const uint32_t probe = lo + lane * stride;
+        const uint64_t ok = __ballot(probe < hi && offsets[probe] <= p0);
+        lo += static_cast<uint32_t>(kWave - 1 - __builtin_clzll(ok)) * stride;
+        hi = min(hi, lo + stride);
+      }
+      if (lane == 0) s_first = lo;
     }
-    if (lane == 0) s_first = lo;
-  }
-  __syncthreads();
-  uint32_t j_base = s_first, p_lo = p0;
-  while (p_lo < p_end && j_base < n) {
-    const uint32_t j = j_base + tid;
-    uint32_t start = 0xffffffffu, count = 0, splat = 0, r[4] = {0, 0, 0, 0};
-    uint32_t row_tiles = tiles_x, base = 0;
-    if (j < n) {
-      splat = order[j];
-      start = offsets[j];
-      count = tile_counts[splat];
-      if (count) {
-        uint32_t grid_y = tiles_y;
-        if (kBatched) {
-          const ViewDesc& view = views[splat / per_view];
-          row_tiles = view.tiles_x;
-          grid_y = view.tiles_y;
-          base = view.tile_base;
+    __syncthreads();
+    uint32_t j_base = s_first, p_lo = p0;
+    while (p_lo < p_end && j_base < n) {
+      const uint32_t j = j_base + tid;
+      uint32_t start = 0xffffffffu, count = 0, splat = 0, r[4] = {0, 0, 0, 0};
+      uint32_t row_tiles = tiles_x, base = 0;
+      if (j < n) {
+        splat = order[j];
+        start = offsets[j];
+        count = tile_counts[splat];
+        if (count) {
+          uint32_t grid_y = tiles_y;
+          if (kBatched) {
+            const ViewDesc& view = views[splat / per_view];
+            row_tiles = view.tiles_x;
+            grid_y = view.tiles_y;
+            base = view.tile_base;
+          }
+          const float4 s0 = splats[static_cast<size_t>(splat) * 3];
+          tile_rect(s0.x, s0.y, s0.w, row_tiles, grid_y, r);
         }
-        const float4 s0 = splats[static_cast<size_t>(splat) * 3];
-        tile_rect(s0.x, s0.y, s0.w, row_tiles, grid_y, r);
       }
-    }
-    s_start[tid] = start;
-    s_count[tid] = count;
-    s_x0[tid] = r[0];
-    s_y0[tid] = r[1];
-    s_w[tid] = r[2] - r[0];
-    s_splat[tid] = splat;
-    s_row[tid] = row_tiles;
-    s_base[tid] = base;
-    __syncthreads();
-    const int valid = static_cast<int>(min(static_cast<uint32_t>(kThreads), n - j_base));
-    const uint32_t window_end = s_start[valid - 1] + s_count[valid - 1];
-    const uint32_t p_hi = min(p_end, window_end);
-    for (uint32_t p = p_lo + tid; p < p_hi; p += kThreads) {
-      int lo = 0, hi = valid - 1;  // the last entry starting at or before p
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_start[mid] <= p) lo = mid; else hi = mid - 1;
+      s_start[tid] = start;
+      s_count[tid] = count;
+      s_x0[tid] = r[0];
+      s_y0[tid] = r[1];
+      s_w[tid] = r[2] - r[0];
+      s_splat[tid] = splat;
+      s_row[tid] = row_tiles;
+      s_base[tid] = base;
+      __syncthreads();
+      const int valid = static_cast<int>(min(static_cast<uint32_t>(kThreads), n - j_base));
+      const uint32_t window_end = s_start[valid - 1] + s_count[valid - 1];
+      const uint32_t p_hi = min(p_end, window_end);
+      for (uint32_t p = p_lo + tid; p < p_hi; p += kThreads) {
+        int lo = 0, hi = valid - 1;  // the last entry starting at or before p
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_start[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t k = p - s_start[lo], width = s_w[lo];
+        const uint32_t row = k / width;
+        pair_tiles[p] = s_base[lo] + (s_y0[lo] + row) * s_row[lo] + s_x0[lo] + (k - row * width);
+        pair_splats[p] = s_splat[lo];
       }
-      const uint32_t k = p - s_start[lo], width = s_w[lo];
-      const uint32_t row = k / width;
-      pair_tiles[p] = s_base[lo] + (s_y0[lo] + row) * s_row[lo] + s_x0[lo] + (k - row * width);
-      pair_splats[p] = s_splat[lo];
+      p_lo = max(p_lo, p_hi);
+      j_base += kThreads;
+      __syncthreads();
     }
-    p_lo = max(p_lo, p_hi);
-    j_base += kThreads;
-    __syncthreads();
   }
 }
 
 // find_ranges (tiles.wgsl:61-84): each tile's [start, end) run of the sorted pairs,
-// (0, 0) for a tile without pairs, one thread per pair. The pair at a change of
+// (0, 0) for a tile without pairs, a thread per pair (grid-stride). The pair at a change of
 // tile writes the end of the previous tile, the start of its own and the empty
 // tiles between them (the first and last pairs also the tiles before and after
 // them), so no clearing pass is needed.
@@ -191,28 +192,29 @@ __global__ __launch_bounds__(kThreads) void find_ranges_kernel(const uint32_t* _
                                                                 uint32_t capacity, uint32_t tiles,
                                                                 uint2* __restrict__ ranges) {
   const uint32_t count = min(*pair_total, capacity);
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t first = blockIdx.x * kThreads + threadIdx.x, stride = gridDim.x * kThreads;
   const uint2 empty = make_uint2(0u, 0u);
   if (count == 0) {
-    for (uint32_t t = i; t < tiles; t += gridDim.x * kThreads) ranges[t] = empty;
+    for (uint32_t t = first; t < tiles; t += stride) ranges[t] = empty;
     return;
   }
-  if (i >= count) return;
-  const uint32_t tile = min(pair_tiles[i], tiles - 1);
-  if (i == 0) {
-    for (uint32_t t = 0; t < tile; t++) ranges[t] = empty;
-    ranges[tile].x = 0;
-  } else {
-    const uint32_t previous = min(pair_tiles[i - 1], tiles - 1);
-    if (tile != previous) {
-      ranges[previous].y = i;
-      for (uint32_t t = previous + 1; t < tile; t++) ranges[t] = empty;
-      ranges[tile].x = i;
+  for (uint32_t i = first; i < count; i += stride) {
+    const uint32_t tile = min(pair_tiles[i], tiles - 1);
+    if (i == 0) {
+      for (uint32_t t = 0; t < tile; t++) ranges[t] = empty;
+      ranges[tile].x = 0;
+    } else {
+      const uint32_t previous = min(pair_tiles[i - 1], tiles - 1);
+      if (tile != previous) {
+        ranges[previous].y = i;
+        for (uint32_t t = previous + 1; t < tile; t++) ranges[t] = empty;
+        ranges[tile].x = i;
+      }
     }
-  }
-  if (i == count - 1) {
-    ranges[tile].y = count;
-    for (uint32_t t = tile + 1; t < tiles; t++) ranges[t] = empty;
+    if (i == count - 1) {
+      ranges[tile].y = count;
+      for (uint32_t t = tile + 1; t < tiles; t++) ranges[t] = empty;
+    }
   }
 }
 
@@ -387,11 +389,18 @@ void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width,
   DIO_LAUNCHED("count_tiles");
 }
 
+// The grid for an expected pair count: the hint (0: the capacity) with an eighth more
+// and one more workgroup, at most the capacity's.
+static uint32_t hinted_groups(uint32_t capacity, uint32_t pair_hint, uint32_t per_group) {
+  const uint64_t want = pair_hint ? static_cast<uint64_t>(pair_hint) + pair_hint / 8 + per_group : capacity;
+  return std::max(1u, ceil_div(static_cast<uint32_t>(std::min<uint64_t>(want, capacity)), per_group));
+}
+
 void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
                const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
-               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats) {
+               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_hint) {
   if (n == 0 || capacity == 0) return;
-  duplicate_kernel<false><<<ceil_div(capacity, kDupPairs), kThreads, 0, s>>>(
+  duplicate_kernel<false><<<hinted_groups(capacity, pair_hint, kDupPairs), kThreads, 0, s>>>(
       reinterpret_cast<const float4*>(splats), order, offsets, tile_counts, n, ceil_div(width, kTile),
       ceil_div(height, kTile), nullptr, 0, capacity, pair_total, pair_tiles, pair_splats);
   DIO_LAUNCHED("duplicate");
@@ -409,9 +418,9 @@ void duplicate_batch(hipStream_t s, const float* splats, const uint32_t* order, 
 }
 
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
-                 uint32_t tiles, uint32_t* ranges, uint32_t* order) {
+                 uint32_t tiles, uint32_t* ranges, uint32_t* order, uint32_t pair_hint) {
   if (tiles == 0) return;
-  find_ranges_kernel<<<std::max(1u, ceil_div(capacity, static_cast<uint32_t>(kThreads))), kThreads, 0, s>>>(
+  find_ranges_kernel<<<hinted_groups(capacity, pair_hint, kThreads), kThreads, 0, s>>>(
       pair_tiles, pair_total, capacity, tiles, reinterpret_cast<uint2*>(ranges));
   DIO_LAUNCHED("find_ranges");
   if (order) {

@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
                                                              uint32_t* __restrict__ values_out, const uint32_t* count,
                                                              uint32_t capacity, uint32_t shift,
                                                              const uint32_t* __restrict__ digit_start, PassState now,
-                                                             PassState next, uint32_t* ticket,
+                                                             PassState next, uint32_t* ticket, uint32_t eager,
                                                              const uint32_t* __restrict__ gather_src,
                                                              uint32_t* __restrict__ gather_out) {
   using Ops = KeyOps<kWide, kBits>;
@@ -190,7 +190,15 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
   __shared__ uint32_t staged_values[kSortTile];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  if (tid == 0) {
+    // The grid covers the capacity. Workgroups from `eager` on first read the key
+    // count and take no ticket past the last tile, so the idle ones do not queue
+    // on the ticket (one same-address atomic each). Tiles 0 .. tiles-1 still all
+    // get a ticket, whatever `eager` is.
+    const bool wanted =
+        blockIdx.x < eager || blockIdx.x < ceil_div(key_count(count, capacity), static_cast<uint32_t>(kSortTile));
+    s_tile = wanted ? atomicAdd(ticket, 1u) : ~0u;
+  }
   if (tid < kRadix) {
     running[tid] = 0;
 #pragma unroll
@@ -422,11 +430,16 @@ namespace {
 template <int kBits>
 bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
                      const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
-                     const SortWorkspace& ws, const uint32_t* gather_src, uint32_t* gather_out) {
+                     const SortWorkspace& ws, const uint32_t* gather_src, uint32_t* gather_out,
+                     uint32_t count_hint) {
   const uint32_t tiles = ceil_div(capacity, kSortTile);
   const uint32_t passes = (end_bit - begin_bit + kBits - 1) / kBits;
   if (passes > kMaxPasses) throw Error(-2, "radix sort: too many passes");
-  const uint32_t hist_blocks = std::min<uint32_t>(tiles, 1024);
+  // With a device-side count, the tiles expected from the host's hint (any value
+  // gives the same result; the grids still cover the capacity).
+  const uint32_t eager =
+      count && count_hint ? std::min(tiles, ceil_div(std::min(count_hint, capacity), kSortTile)) : tiles;
+  const uint32_t hist_blocks = std::max<uint32_t>(1, std::min<uint32_t>(eager, 1024));
   if (wide)
     histogram_kernel<true, kBits><<<hist_blocks, kThreads, 0, s>>>(static_cast<const uint2*>(keys), count, capacity,
                                                                     begin_bit, passes, ws.hist, ws.offsets, ws.state[0],
@@ -453,12 +466,13 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
     if (wide)
       onesweep_kernel<true, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint2*>(kin), vin,
                                                                static_cast<uint2*>(kout), vout, count, capacity, shift,
-                                                               ws.offsets + p * 256, now, next, ws.tickets + p, src, out);
+                                                               ws.offsets + p * 256, now, next, ws.tickets + p, eager, src,
+                                                               out);
     else
       onesweep_kernel<false, kBits><<<tiles, kThreads, 0, s>>>(static_cast<const uint32_t*>(kin), vin,
                                                                 static_cast<uint32_t*>(kout), vout, count, capacity,
-                                                                shift, ws.offsets + p * 256, now, next, ws.tickets + p, src,
-                                                                out);
+                                                                shift, ws.offsets + p * 256, now, next, ws.tickets + p, eager,
+                                                                src, out);
     DIO_LAUNCHED("radix onesweep");
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -470,7 +484,8 @@ bool radix_sort_bits(hipStream_t s, bool wide, void* keys, uint32_t* values, voi
 
 bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* alt_keys, uint32_t* alt_values,
                 const uint32_t* count, uint32_t capacity, uint32_t begin_bit, uint32_t end_bit,
-                const SortWorkspace& ws, uint32_t digit_bits, const uint32_t* gather_src, uint32_t* gather_out) {
+                const SortWorkspace& ws, uint32_t digit_bits, const uint32_t* gather_src, uint32_t* gather_out,
+                uint32_t count_hint) {
   if (digit_bits != 8 && digit_bits != 4) throw Error(-1, "radix digit width must be 4 or 8 bits");
   if (begin_bit % digit_bits != 0) throw Error(-2, "radix sort: begin bit must be a multiple of the digit width");
   if (end_bit > (wide ? 64u : 32u)) throw Error(-2, "radix sort: end bit past the key");
@@ -481,9 +496,9 @@ bool radix_sort(hipStream_t s, bool wide, void* keys, uint32_t* values, void* al
   if (ceil_div(capacity, kSortTile) > ws.tiles) throw Error(-2, "sort workspace too small");
   return digit_bits == 8
              ? radix_sort_bits<8>(s, wide, keys, values, alt_keys, alt_values, count, capacity, begin_bit, end_bit, ws,
-                                    gather_src, gather_out)
+                                    gather_src, gather_out, count_hint)
              : radix_sort_bits<4>(s, wide, keys, values, alt_keys, alt_values, count, capacity, begin_bit, end_bit, ws,
-                                    gather_src, gather_out);
+                                    gather_src, gather_out, count_hint);
 }
 
 }  // namespace dio

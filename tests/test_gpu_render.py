@@ -290,3 +290,28 @@ def test_viewport_tile_limit_is_an_error():
     with pytest.raises(diorama_amd.DioramaError, match="more tiles than the sort can tell apart"):
         r.encode(None, cam, [0, 0, 0])
     r.destroy()
+
+
+def test_many_more_pairs_than_the_last_frame():
+    """The pair kernels size their grids from the previous frame's pair count and loop
+    over the rest; the sort's workgroups past that count read the real one. A close
+    view after a distant one (many times the pairs, within the capacity, so no
+    re-encode) still gives the oracle's lists and image."""
+    g = random_scene(3000, 5)
+    proj = scene.perspective(math.pi / 2, 160 / 120, 0.1, 100)
+    far = scene.camera(scene.look_at([0, 0, 40], [0, 0, -5], [0, 1, 0]), proj, 160, 120)
+    near = scene.camera(scene.IDENTITY, proj, 160, 120)
+    bg = [0.1, 0.2, 0.3]
+    say("many more pairs")
+    r = render(g, far, bg, capacity=1 << 20)
+    far_pairs = r.tiles.last_pair_count
+    r = render(g, near, bg, renderer=r)
+    near_pairs = r.tiles.last_pair_count
+    print(f"pairs: far {far_pairs}, near {near_pairs}, capacity {r.tiles.pair_capacity}")
+    # past what the hinted grids cover: the widest is the sort's, whole 4096-key tiles
+    # for the last count plus an eighth and one more tile
+    hinted = far_pairs + far_pairs // 8 + 4096
+    assert far_pairs > 0 and near_pairs > -(-hinted // 4096) * 4096
+    assert r.tiles.pair_capacity == 1 << 20
+    check_frame_against_oracle(r, 160, 120, bg, "many more pairs")
+    r.destroy()
