@@ -1136,9 +1136,9 @@ void batch_free_frames(BatchState* b) {
   b->exec = nullptr;
   b->graph = nullptr;
   for (dio_buf** p : {&b->splats, &b->tile_counts, &b->depth_keys, &b->depth_values, &b->depth_alt_keys,
-                      &b->depth_alt_values, &b->sorted_counts, &b->offsets, &b->scan_ws, &b->sort_ws, &b->pair_total, &b->pair_tiles,
-                      &b->pair_splats, &b->alt_tiles, &b->alt_splats, &b->pair_sort_ws, &b->splat_grads,
-                      &b->grad_gaussians, &b->grad_sh})
+                      &b->depth_alt_values, &b->sorted_counts, &b->offsets, &b->scan_ws, &b->sort_ws,
+                      &b->pair_total, &b->pair_tiles, &b->pair_splats, &b->alt_tiles, &b->alt_splats,
+                      &b->pair_sort_ws, &b->splat_grads, &b->grad_gaussians, &b->grad_sh})
     release(*p);
   b->n = 0;
   b->capacity = 0;
