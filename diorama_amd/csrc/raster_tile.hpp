@@ -260,9 +260,10 @@ struct Replay {
 // alpha < 1/255, or past the pixel's last splat) runs the same instructions with
 // alpha = 0, which leaves T, the colour behind and the next splat as they were and
 // zeroes every term. The terms are linear in the splat's constants, so the conic
-// is applied once per splat after the reduction: with g = dL/dpower they are
-// g dx, g dy (mean2: a, b, c times these), g dx², g dx dy, g dy² (conic: times
-// -1/2, -1, -1/2), dL/dopacity and dL/drgb.
+// and the opacity are applied once per splat after the reduction: with
+// g = dL/dpower = o dL/dopacity (o the opacity) they are dL/dopacity times dx, dy
+// (mean2: o a, b, c times these), dx², dx dy, dy² (conic: o times -1/2, -1, -1/2),
+// then dL/dopacity and dL/drgb.
 __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy, float mx, float my, float ha,
                                     float hb, float hc, float op, float sr, float sg, float sb, float share[kGrads]) {
   const float dx = cx - mx, dy = cy - my;
@@ -280,8 +281,7 @@ __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy
   st.next = valid ? color : st.next;
   const float alpha_grad = st.T * (color - st.behind) - st.final_bg * keep;
   const float opacity_grad = valid ? alpha_grad * G : 0.f;
-  const float g = opacity_grad * op;  // dL/dpower
-  const float gdx = g * dx, gdy = g * dy;
+  const float gdx = opacity_grad * dx, gdy = opacity_grad * dy;  // x opacity after the reduction
   share[0] = gdx;
   share[1] = gdy;
   share[2] = gdx * dx;
@@ -419,25 +419,27 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
     }
     __syncthreads();
     // Per (slot, gradient pair): the four quadrants' sums, then the splat's
-    // constants: mean2 from the (g dx, g dy) sums through the conic, the conic
+    // constants, each of the first five sums times the opacity (g = o dL/dopacity):
+    // mean2 from the (g dx, g dy) sums through the conic, the conic
     // terms scaled by -1/2, -1, -1/2; the rest as summed.
     for (int e = tid; e < 5 * kRound; e += 256) {
       const int part = e / kRound, slot = e % kRound;
       auto sum = [&](int v) { return s_sums[0][v][slot] + s_sums[1][v][slot] + s_sums[2][v][slot] + s_sums[3][v][slot]; };
+      const float op = sh.records[slot / kChunk].c[slot % kChunk].y;
       float* out = splat_grads + static_cast<size_t>(s_index[slot]) * kGrads;
       float t0, t1;
       int v0, v1;
       if (part == 0) {
         const float4 k = sh.conic[slot];
-        const float u = sum(0), w = sum(1);
+        const float u = sum(0) * op, w = sum(1) * op;
         t0 = k.x * u + k.y * w;
         t1 = k.y * u + k.z * w;
         v0 = 0; v1 = 1;
       } else if (part == 1) {
-        t0 = -0.5f * sum(2); t1 = -sum(3);
+        t0 = -0.5f * sum(2) * op; t1 = -sum(3) * op;
         v0 = 2; v1 = 3;
       } else if (part == 2) {
-        t0 = -0.5f * sum(4); t1 = sum(5);
+        t0 = -0.5f * sum(4) * op; t1 = sum(5);
         v0 = 4; v1 = 5;
       } else if (part == 3) {
         t0 = sum(6); t1 = sum(7);
