@@ -359,3 +359,62 @@ def test_tile_lists_brute_force():
         members.sort(key=lambda i: s[i, 2])  # Python's sort is stable
         got = list(pairs[ranges[t, 0]:ranges[t, 1]])
         assert got == members, t
+
+
+def stats_case():
+    """Four splats on a 200x100 viewport, worked by hand (adam.wgsl:125-134).
+
+    Splat 0: dL/dmean = (3/(0.5*200), 4/(0.5*100)) = (0.03, 0.08), so the NDC
+    gradient is (3, 4) and its length 5; radius 7 is below the row's 9.
+    Splat 1: dL/dmean = (0.01, 0): NDC (1, 0) from the width alone; radius 12
+    raises the row's 9. Splat 2: radius 0 (culled) with a large gradient: its row
+    must stay exactly as it was, including the last column. Splat 3: the negative
+    of splat 0's gradient: lengths add, not vectors."""
+    splats = np.zeros((4, 12))
+    splats[:, 3] = [7.0, 12.0, 0.0, 3.0]
+    splats[:, 2] = [2.5, 4.0, 8.0, 1.5]
+    grads = np.zeros((4, 9))
+    grads[0, :2] = [0.03, 0.08]
+    grads[1, :2] = [0.01, 0.0]
+    grads[2, :2] = [0.5, -0.7]
+    grads[3, :2] = [-0.03, -0.08]
+    grads[:, 2:] = 1.0  # conic, opacity and colour gradients play no part
+    stats = np.array([[1.0, 2.0, 9.0, 6.0], [0.0, 0.0, 9.0, 0.0], [0.25, 1.0, 4.0, 6.0], [0.0, 0.0, 0.0, 0.0]])
+    return splats, grads, [200.0, 100.0], stats
+
+
+def test_accumulate_stats_known_answers():
+    splats, grads, viewport, stats = stats_case()
+    out = oracle.accumulate_stats(splats, grads, viewport, stats).reshape(4, 4)
+    want = np.array([[6.0, 3.0, 9.0, 0.0],     # 1 + 5; one more view; max(9, 7); w cleared
+                     [1.0, 1.0, 12.0, 0.0],    # NDC (1, 0); max(9, 12)
+                     [0.25, 1.0, 4.0, 6.0],    # radius 0: untouched
+                     [5.0, 1.0, 3.0, 0.0]])    # length 5 of (-3, -4)
+    assert np.allclose(out, want, rtol=0, atol=1e-12), out
+    # a second view adds again: sums and counts grow, the radius is a max, not a sum
+    again = oracle.accumulate_stats(splats, grads, viewport, out).reshape(4, 4)
+    assert np.allclose(again[:, 0], [11.0, 2.0, 0.25, 10.0], atol=1e-12)
+    assert np.allclose(again[:, 1], [4.0, 2.0, 1.0, 2.0])
+    assert np.allclose(again[:, 2], [9.0, 12.0, 4.0, 3.0])
+
+
+def test_accumulate_stats_catches_planted_mistakes():
+    """The known answers above tell the intended rule from each plausible slip."""
+    splats, grads, viewport, stats = stats_case()
+    want = oracle.accumulate_stats(splats, grads, viewport, stats).reshape(4, 4)
+
+    def slipped(half=0.5, units=True, swap=False, radius_sum=False, gate=True):
+        out = stats.copy()
+        for i in range(4):
+            r = splats[i, 3]
+            if gate and r == 0:
+                continue
+            w, h = (viewport[1], viewport[0]) if swap else viewport
+            gx = grads[i, 0] * (w * half if units else 1.0)
+            gy = grads[i, 1] * (h * half if units else 1.0)
+            out[i] = [out[i, 0] + math.hypot(gx, gy), out[i, 1] + 1, out[i, 2] + r if radius_sum else max(out[i, 2], r), 0]
+        return out
+
+    assert np.allclose(slipped(), want, atol=1e-12)  # the restatement itself agrees
+    for mistake in [dict(half=1.0), dict(units=False), dict(swap=True), dict(radius_sum=True), dict(gate=False)]:
+        assert not np.allclose(slipped(**mistake), want, atol=1e-6), mistake

@@ -61,3 +61,42 @@ def test_features_by_hand():
             0.299 * c[0] + 0.587 * c[1] + 0.114 * c[2], math.log(4.0), math.log(701.0), math.log(5.5),
             math.log(4.0), 1 - 5e-4 / 6e-4]
     assert np.allclose(f, want, rtol=1e-12, atol=1e-12)
+
+
+def test_extra_stats_known_answers():
+    """Extra statistics per view (largest NDC gradient length, summed depth, summed
+    NDC gradient x and y) on a 200x100 viewport, worked by hand. dL/dmean of
+    (0.03, 0.08) is the NDC gradient (3, 4), length 5; its negative sums back to
+    the start; a radius-0 splat leaves its row alone."""
+    splats = np.zeros((3, 12))
+    splats[:, 2] = [2.5, 4.0, 9.0]  # depth
+    splats[:, 3] = [7.0, 3.0, 0.0]  # radius
+    grads = np.zeros((3, 9))
+    grads[0, :2] = [0.03, 0.08]
+    grads[1, :2] = [-0.03, -0.08]
+    grads[2, :2] = [1.0, 1.0]
+    extra = np.array([[2.0, 10.0, 1.0, -1.0], [7.5, 0.0, 0.0, 0.0], [1.0, 2.0, 3.0, 4.0]])
+    out = mlp.extra_stats(splats, grads, [200.0, 100.0], extra)
+    want = np.array([[5.0, 12.5, 4.0, 3.0],    # max(2, 5); 10 + 2.5; 1 + 3; -1 + 4
+                     [7.5, 4.0, -3.0, -4.0],   # max(7.5, 5) keeps 7.5; signs kept in the sums
+                     [1.0, 2.0, 3.0, 4.0]])    # culled
+    assert np.allclose(out, want, rtol=0, atol=1e-12), out
+    # a dropped 0.5 or pixel units would move the first column away from 5
+    assert not np.isclose(np.hypot(0.03 * 200, 0.08 * 100), 5.0)
+    assert not np.isclose(np.hypot(0.03, 0.08), 5.0)
+    # the input array is not modified in place
+    assert extra[0, 0] == 2.0
+
+
+def test_features_read_the_extra_statistics():
+    """The direction-variance input is 1 - |summed NDC gradient| / summed length:
+    two opposite views cancel to 1, two equal views give 0."""
+    params = np.zeros((2, 16))
+    params[:, 4:7] = math.log(0.5)
+    params[:, 8] = 1.0
+    stats = np.array([[10.0, 2.0, 3.0, 0.0], [10.0, 2.0, 3.0, 0.0]])
+    extra = np.array([[5.0, 6.0, 0.0, 0.0], [5.0, 6.0, 6.0, 8.0]])
+    f = mlp.features(params, stats, extra, extent=1.0, iteration=10)
+    assert np.allclose(f[:, 11], [1.0, 0.0])
+    assert np.allclose(f[:, 1], math.log1p(5.0 / 2e-4))
+    assert np.allclose(f[:, 10], math.log(3.0))  # summed depth 6 over 2 views
