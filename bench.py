@@ -83,31 +83,16 @@ def ring_poses(count):
 
 def split_views(cams):
     """263 training poses and 33 held-out ones taken every 8th."""
-    held = [k for k in range(len(cams)) if k % 8 == 7][:HELD_OUT]
-    train = [c for k, c in enumerate(cams) if k not in set(held)]
-    return train[:TRAIN_VIEWS], [cams[k] for k in held]
+    from diorama_amd import scene
+
+    return scene.split_views(cams, HELD_OUT, TRAIN_VIEWS)
 
 
 def teacher_targets(ctx, cams, n):
     """rgba8 targets: the seed-7 teacher rendered on the GPU plus N(0, 0.01) noise."""
-    import diorama_amd
     from diorama_amd import scene
 
-    params, sh = scene.random_raw_gaussians(n, 7, box=5.0)
-    r = diorama_amd.GaussianRenderer(ctx)
-    r.set_gaussians(scene.activate(params), sh, 3)
-    rng = np.random.default_rng(7)
-    images = []
-    for cam in cams:
-        r.encode(None, cam, [0, 0, 0])
-        px = int(cam["viewport"][0]) * int(cam["viewport"][1])
-        color = r.pixel_buffer.download(np.float32, px * 4).reshape(-1, 4)[:, :3]
-        noisy = color + rng.standard_normal(color.shape, dtype=np.float32) * np.float32(0.01)
-        q = np.rint(np.clip(noisy, 0, 1) * 255).astype(np.uint32)
-        images.append(q[:, 0] | (q[:, 1] << 8) | (q[:, 2] << 16) | np.uint32(255 << 24))
-    r.after_submit()
-    r.destroy()
-    return images
+    return scene.teacher_images(ctx, cams, n, seed=7, box=5.0, noise=0.01, noise_seed=7)
 
 
 def cpu_baseline(params, sh, cam, image, sh_degree, budget_s=30.0):

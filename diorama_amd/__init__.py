@@ -18,6 +18,10 @@ LIB_PATH = os.path.join(HERE, "libdiorama_hip.so")
 _lib = None
 
 
+# Status codes of include/diorama.h.
+DIO_OK, DIO_ERR_INVALID, DIO_ERR_INTERNAL, DIO_ERR_HIP, DIO_ERR_MEMORY = 0, -1, -2, -3, -4
+
+
 class DioramaError(RuntimeError):
     pass
 
@@ -156,6 +160,8 @@ SIGNATURES = {
     "dio_trainer_samples": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_U32)]),
     "dio_trainer_features": (_I, [_P, ctypes.POINTER(_P)]),
     "dio_trainer_extra_stats": (_P, [_P]),
+    "dio_trainer_batch_frames": (_I, [_P, ctypes.POINTER(_U32), _U32, ctypes.POINTER(_U32), ctypes.POINTER(_P),
+                                     ctypes.POINTER(_U32), ctypes.POINTER(_P)]),
     "dio_trainer_step": (_I, [_P]),
     "dio_trainer_optimize": (_I, [_P]),
     "dio_trainer_maintain": (_I, [_P]),
@@ -265,6 +271,14 @@ class Context:
             self.handle = None
 
 
+def _size(value, what):
+    """A byte offset or size for the C ABI: a non-negative integer (ctypes would turn
+    -16 into 2**64 - 16 without complaint)."""
+    if isinstance(value, (bool, np.bool_)) or not isinstance(value, (int, np.integer)) or value < 0:
+        raise ValueError(f"{what} must be a non-negative integer, got {value!r}")
+    return int(value)
+
+
 class Buffer:
     """A device allocation. `owned` buffers are freed with the object."""
 
@@ -276,7 +290,7 @@ class Buffer:
     @classmethod
     def create(cls, ctx, nbytes):
         h = _P()
-        _check(lib().dio_buf_create(ctx.handle, int(nbytes), ctypes.byref(h)))
+        _check(lib().dio_buf_create(ctx.handle, _size(nbytes, "size"), ctypes.byref(h)))
         return cls(h, True)
 
     @property
@@ -285,13 +299,14 @@ class Buffer:
 
     def upload(self, array, offset=0):
         a = np.ascontiguousarray(array)
-        _check(lib().dio_buf_upload(self.handle, offset, a.ctypes.data_as(_P), a.nbytes))
+        _check(lib().dio_buf_upload(self.handle, _size(offset, "offset"), a.ctypes.data_as(_P), a.nbytes))
 
     def download(self, dtype=np.float32, count=None, offset=0):
         itemsize = np.dtype(dtype).itemsize
+        offset = _size(offset, "offset")
         if count is None:
-            count = (self.nbytes - offset) // itemsize
-        out = np.empty(count, dtype=dtype)
+            count = max(self.nbytes - offset, 0) // itemsize
+        out = np.empty(_size(count, "count"), dtype=dtype)
         _check(lib().dio_buf_download(self.handle, offset, out.ctypes.data_as(_P), out.nbytes))
         return out
 
@@ -300,7 +315,8 @@ class Buffer:
 
     def copy_from(self, src, nbytes=None, dst_offset=0, src_offset=0):
         """encoder.copyBufferToBuffer on this buffer's stream."""
-        n = src.nbytes - src_offset if nbytes is None else nbytes
+        src_offset, dst_offset = _size(src_offset, "src_offset"), _size(dst_offset, "dst_offset")
+        n = max(src.nbytes - src_offset, 0) if nbytes is None else _size(nbytes, "nbytes")
         _check(lib().dio_buf_copy(self.handle, dst_offset, src.handle, src_offset, n))
 
     def destroy(self):
@@ -696,6 +712,23 @@ class Trainer:
     @property
     def extra_stats(self):
         return _view(lib().dio_trainer_extra_stats(self.handle), self)
+
+    def batch_frames(self, cameras):
+        """A batch trainer's last step, downloaded: per slot (training view index,
+        splats (count, 12) float32, pixel colours (w*h, 4) float32); `cameras` are
+        the trainer's training cameras, for the image sizes."""
+        views = (_U32 * 64)()
+        count, stride, buf, pix = _U32(), _U32(), _P(), _P()
+        _check(lib().dio_trainer_batch_frames(self.handle, views, 64, ctypes.byref(count), ctypes.byref(buf),
+                                              ctypes.byref(stride), ctypes.byref(pix)))
+        n = self.count
+        splats = _view(buf.value, self).download(np.float32, count.value * stride.value * 12)
+        splats = splats.reshape(count.value, stride.value, 12)[:, :n]
+        sizes = [int(cameras[views[k]]["viewport"][0]) * int(cameras[views[k]]["viewport"][1])
+                 for k in range(count.value)]
+        colors = _view(pix.value, self).download(np.float32, 4 * sum(sizes)).reshape(-1, 4)
+        starts = np.concatenate([[0], np.cumsum(sizes)])
+        return [(int(views[k]), splats[k], colors[starts[k]:starts[k + 1]]) for k in range(count.value)]
 
     def step(self):
         _check(lib().dio_trainer_step(self.handle))

@@ -29,6 +29,7 @@ using dio::invalid;
 using dio::need;
 using dio::release;
 using dio::sync;
+using dio::within;
 
 namespace dio {
 std::string& last_error() {
@@ -44,6 +45,25 @@ void to_camera(const dio_camera& c, dio::CameraArgs& out) {
   std::memcpy(out.proj, c.proj, sizeof(out.proj));
   out.viewport[0] = c.viewport[0];
   out.viewport[1] = c.viewport[1];
+}
+
+// One side of a viewport as a pixel count. Cameras carry it as a float; one that is
+// negative, not finite or past 2^24 is refused before any cast or size product.
+uint32_t viewport_dim(float v) {
+  if (!(v >= 0.f && v <= 16777216.f)) invalid("a viewport side must be between 0 and 2^24 pixels");
+  return static_cast<uint32_t>(v);
+}
+
+// 16-pixel tiles of a w x h viewport, in 64 bits (a 2^20 x 2^20 viewport has 2^32).
+uint64_t viewport_tiles(uint32_t w, uint32_t h) {
+  return static_cast<uint64_t>(dio::ceil_div(w, dio::kTile)) * dio::ceil_div(h, dio::kTile);
+}
+
+// The tile sort tells at most 2^16 tiles apart (src/tileRasterizer.ts:10,136-138).
+void check_tile_limit(uint32_t w, uint32_t h) {
+  if (viewport_tiles(w, h) > 65536u) {
+    invalid("A " + std::to_string(w) + "×" + std::to_string(h) + " viewport has more tiles than the sort can tell apart.");
+  }
 }
 
 uint32_t next_pow2_capacity(uint32_t count) {
@@ -81,7 +101,7 @@ extern "C" int dio_ctx_synchronize(dio_ctx* ctx) {
 }
 
 extern "C" void dio_ctx_destroy(dio_ctx* ctx) {
-  if (!ctx || ctx->closed) return;
+  if (!ctx || ctx->closed.exchange(true)) return;
   (void)hipStreamSynchronize(ctx->stream);
   for (hipEvent_t& e : ctx->stamps)
     if (e) {
@@ -90,8 +110,7 @@ extern "C" void dio_ctx_destroy(dio_ctx* ctx) {
     }
   (void)hipStreamDestroy(ctx->stream);
   ctx->stream = nullptr;  // later syncs of buffers or objects that outlive it touch no stream
-  ctx->closed = true;
-  if (ctx->refs == 0) delete ctx;
+  dio::drop(ctx);              // the open context's own reference
 }
 
 extern "C" int dio_ctx_timestamp(dio_ctx* ctx, uint32_t slot) {
@@ -128,7 +147,7 @@ extern "C" int dio_buf_create(dio_ctx* ctx, size_t bytes, dio_buf** out) {
 extern "C" int dio_buf_upload(dio_buf* buf, size_t offset, const void* src, size_t bytes) {
   return guard([&] {
     need(buf, "buffer");
-    if (offset + bytes > buf->bytes) invalid("upload past the end of the buffer");
+    if (!within(buf, offset, bytes)) invalid("upload past the end of the buffer");
     if (bytes == 0) return;
     need(src, "source");
     DIO_HIP(hipMemcpyAsync(static_cast<char*>(buf->ptr) + offset, src, bytes, hipMemcpyHostToDevice, buf->ctx->stream));
@@ -139,7 +158,7 @@ extern "C" int dio_buf_upload(dio_buf* buf, size_t offset, const void* src, size
 extern "C" int dio_buf_download(const dio_buf* buf, size_t offset, void* dst, size_t bytes) {
   return guard_read([&] {
     need(buf, "buffer");
-    if (offset + bytes > buf->bytes) invalid("download past the end of the buffer");
+    if (!within(buf, offset, bytes)) invalid("download past the end of the buffer");
     if (bytes == 0) return;
     need(dst, "destination");
     DIO_HIP(hipMemcpyAsync(dst, static_cast<const char*>(buf->ptr) + offset, bytes, hipMemcpyDeviceToHost,
@@ -159,7 +178,7 @@ extern "C" int dio_buf_copy(dio_buf* dst, size_t dst_offset, const dio_buf* src,
   return guard([&] {
     need(dst, "destination");
     need(src, "source");
-    if (dst_offset + bytes > dst->bytes || src_offset + bytes > src->bytes) invalid("copy past the end of a buffer");
+    if (!within(dst, dst_offset, bytes) || !within(src, src_offset, bytes)) invalid("copy past the end of a buffer");
     if (bytes == 0) return;
     DIO_HIP(hipMemcpyAsync(static_cast<char*>(dst->ptr) + dst_offset, static_cast<const char*>(src->ptr) + src_offset,
                            bytes, hipMemcpyDeviceToDevice, dst->ctx->stream));
@@ -518,11 +537,9 @@ uint32_t tile_sort_bits(uint32_t tiles, uint32_t digit_bits = 8) {
 // the tile counts and depth keys exist.
 void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t h, const float bg[3],
                   bool counts_ready) {
+  check_tile_limit(w, h);
   const uint32_t tiles_x = dio::ceil_div(w, dio::kTile), tiles_y = dio::ceil_div(h, dio::kTile);
   const uint32_t tiles = tiles_x * tiles_y;
-  if (tiles > 65536u) {
-    invalid("A " + std::to_string(w) + "×" + std::to_string(h) + " viewport has more tiles than the sort can tell apart.");
-  }
   if (target && target->bytes < static_cast<size_t>(w) * h * 4) invalid("the target is smaller than the viewport");
   ensure_frame_buffers(t, tiles, w * h);
   hipStream_t s = t->ctx->stream;
@@ -731,7 +748,7 @@ void renderer_preprocess(dio_renderer* r, const dio_camera& camera, dio_buf* vis
 }
 
 void renderer_tiles(dio_renderer* r, dio_buf* target, const float bg[3]) {
-  const uint32_t w = static_cast<uint32_t>(r->camera.viewport[0]), h = static_cast<uint32_t>(r->camera.viewport[1]);
+  const uint32_t w = viewport_dim(r->camera.viewport[0]), h = viewport_dim(r->camera.viewport[1]);
   if (!r->scene) {
     fill_background(r->ctx, target, w, h, bg);
     return;
@@ -1506,7 +1523,7 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
 void batch_limits(dio_trainer* t, BatchState* b) {
   std::vector<uint32_t> tiles, pixels;
   for (const auto& v : t->views) {
-    const uint32_t w = static_cast<uint32_t>(v.camera.viewport[0]), h = static_cast<uint32_t>(v.camera.viewport[1]);
+    const uint32_t w = viewport_dim(v.camera.viewport[0]), h = viewport_dim(v.camera.viewport[1]);
     tiles.push_back(dio::ceil_div(w, dio::kTile) * dio::ceil_div(h, dio::kTile));
     pixels.push_back(w * h);
   }
@@ -1673,8 +1690,8 @@ void batch_optimize(dio_trainer* t) {
     std::memcpy(d.cam.proj, view.camera.proj, sizeof(d.cam.proj));
     d.cam.viewport[0] = view.camera.viewport[0];
     d.cam.viewport[1] = view.camera.viewport[1];
-    d.width = static_cast<uint32_t>(view.camera.viewport[0]);
-    d.height = static_cast<uint32_t>(view.camera.viewport[1]);
+    d.width = viewport_dim(view.camera.viewport[0]);
+    d.height = viewport_dim(view.camera.viewport[1]);
     d.tiles_x = dio::ceil_div(d.width, dio::kTile);
     d.tiles_y = dio::ceil_div(d.height, dio::kTile);
     d.tile_base = tile_base;
@@ -1777,6 +1794,8 @@ void trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t
     need(out, "out");
     if (view_count == 0) invalid("Training needs at least one view.");
     if (count == 0) invalid("Training needs at least one Gaussian.");
+    for (uint32_t v = 0; v < view_count; v++)
+      check_tile_limit(viewport_dim(cameras[v].viewport[0]), viewport_dim(cameras[v].viewport[1]));
     const float w = cameras[0].viewport[0], h = cameras[0].viewport[1];
     if (batch == 0) {
       for (uint32_t v = 0; v < view_count; v++)
@@ -1795,7 +1814,7 @@ void trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t
       t->height = static_cast<uint32_t>(h);
       for (uint32_t v = 0; v < view_count; v++) {
         need(images[v], "image");
-        const size_t px = static_cast<size_t>(cameras[v].viewport[0]) * static_cast<size_t>(cameras[v].viewport[1]);
+        const size_t px = static_cast<size_t>(viewport_dim(cameras[v].viewport[0])) * viewport_dim(cameras[v].viewport[1]);
         dio_buf* img = alloc(ctx, px * 4);
         t->views.push_back({cameras[v], img});
         DIO_HIP(hipMemcpyAsync(img->ptr, images[v], px * 4, hipMemcpyHostToDevice, ctx->stream));
@@ -1959,8 +1978,9 @@ void trainer_score(dio_trainer* t, const dio_camera* cameras, const uint32_t* co
   std::vector<uint32_t> first(view_count + 1, 0);
   size_t largest = 0;
   for (uint32_t v = 0; v < view_count; v++) {
-    const uint32_t w = static_cast<uint32_t>(cameras[v].viewport[0]), h = static_cast<uint32_t>(cameras[v].viewport[1]);
+    const uint32_t w = viewport_dim(cameras[v].viewport[0]), h = viewport_dim(cameras[v].viewport[1]);
     if (w == 0 || h == 0) invalid("a view has an empty viewport");
+    check_tile_limit(w, h);
     first[v + 1] = first[v] + dio::evaluate_tiles(w, h);
     largest = std::max<size_t>(largest, static_cast<size_t>(w) * h);
     if (device_images) {
@@ -1981,8 +2001,8 @@ void trainer_score(dio_trainer* t, const dio_camera* cameras, const uint32_t* co
     sse = alloc(t->ctx, static_cast<size_t>(tiles) * 4);
     structure = alloc(t->ctx, with_ssim ? static_cast<size_t>(tiles) * 8 : 0);
     for (uint32_t v = 0; v < view_count; v++) {
-      const uint32_t w = static_cast<uint32_t>(cameras[v].viewport[0]),
-                     h = static_cast<uint32_t>(cameras[v].viewport[1]);
+      const uint32_t w = viewport_dim(cameras[v].viewport[0]),
+                     h = viewport_dim(cameras[v].viewport[1]);
       const uint32_t* photo = nullptr;
       if (device_images) {
         photo = static_cast<const uint32_t*>(device_images[v]->ptr);
@@ -2501,6 +2521,22 @@ extern "C" int dio_trainer_features(dio_trainer* t, dio_buf** features) {
 }
 
 extern "C" dio_buf* dio_trainer_extra_stats(dio_trainer* t) { return t ? t->learned.extra : nullptr; }
+
+extern "C" int dio_trainer_batch_frames(dio_trainer* t, uint32_t* views, uint32_t capacity, uint32_t* count,
+                                        dio_buf** splats, uint32_t* stride, dio_buf** pixels) {
+  return guard_read([&] {
+    need(t, "trainer");
+    if (!t->batch) invalid("not a batch trainer");
+    const BatchState* b = t->batch;
+    const uint32_t used = static_cast<uint32_t>(b->step_views.size());
+    if (count) *count = used;
+    if (views)
+      for (uint32_t k = 0; k < std::min(capacity, used); k++) views[k] = b->step_views[k];
+    if (splats) *splats = b->splats;
+    if (stride) *stride = b->n;
+    if (pixels) *pixels = b->pixel_color;
+  });
+}
 
 // ================================================================= tuning
 

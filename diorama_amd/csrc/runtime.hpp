@@ -16,11 +16,11 @@ struct dio_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t stamps[DIO_TIMESTAMP_SLOTS] = {};
   bool stamped[DIO_TIMESTAMP_SLOTS] = {};
-  // Buffers and objects of this context still alive. A destroyed context that they
-  // outlive (a caller may free them afterwards) stays allocated, with no stream,
-  // until the last one is released.
-  uint64_t refs = 0;
-  bool closed = false;
+  // One reference while the context is open, plus one per buffer or object of it
+  // still alive. A destroyed context that they outlive (a caller may free them
+  // afterwards) stays allocated, with no stream, until the last one is released.
+  std::atomic<uint64_t> refs{1};
+  std::atomic<bool> closed{false};
 };
 
 struct dio_buf {
@@ -76,14 +76,23 @@ inline void need(const void* p, const char* what) {
   if (!p) invalid(std::string(what) + " is null");
 }
 
-// A device buffer of `bytes` (at least 256 are reserved), zero-filled on the
-// context's stream when asked.
-// A buffer or object of `ctx` comes into being / goes away.
-inline void hold(dio_ctx* ctx) { ctx->refs++; }
+// A buffer or object of `ctx` comes into being / goes away. Finalizers may release
+// handles from several host threads at once, so the count is atomic; the thread
+// that takes it to zero frees the context, which can only happen after
+// dio_ctx_destroy has dropped the open context's own reference.
+inline void hold(dio_ctx* ctx) { ctx->refs.fetch_add(1, std::memory_order_relaxed); }
 inline void drop(dio_ctx* ctx) {
-  if (ctx && --ctx->refs == 0 && ctx->closed) delete ctx;
+  if (ctx && ctx->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) delete ctx;
 }
 
+// [offset, offset + bytes) lies inside the buffer, without forming a sum that can
+// wrap (an offset of -16 from a binding arrives as 2^64 - 16).
+inline bool within(const dio_buf* b, size_t offset, size_t bytes) {
+  return offset <= b->bytes && bytes <= b->bytes - offset;
+}
+
+// A device buffer of `bytes` (at least 256 are reserved), zero-filled on the
+// context's stream when asked.
 inline dio_buf* alloc(dio_ctx* ctx, size_t bytes, bool zero = false) {
   auto* b = new dio_buf;
   b->ctx = ctx;

@@ -73,6 +73,18 @@ static double f64(call_t* c, size_t i) {
   return v;
 }
 
+/* A byte size or offset: a finite, non-negative integer below 2^53. Casting any
+   other double to size_t is undefined behaviour, and a negative one would wrap. */
+static int size_arg(call_t* c, size_t i, size_t* out) {
+  double v = f64(c, i);
+  if (!(v >= 0 && v <= 9007199254740991.0) || v != (double)(uint64_t)v) {
+    napi_throw_range_error(c->env, NULL, "byte offsets and sizes must be non-negative integers");
+    return 0;
+  }
+  *out = (size_t)v;
+  return 1;
+}
+
 static napi_value wrap(napi_env env, void* p) {
   napi_value v;
   if (!p) {
@@ -169,30 +181,37 @@ FN(version) {
 FN(buf_create) {
   ARGS;
   dio_buf* b = NULL;
-  if (!ok(env, dio_buf_create(handle(&c, 0), (size_t)f64(&c, 1), &b))) return NULL;
+  size_t bytes;
+  if (!size_arg(&c, 1, &bytes)) return NULL;
+  if (!ok(env, dio_buf_create(handle(&c, 0), bytes, &b))) return NULL;
   return wrap(env, b);
 }
 FN(buf_upload) {
   ARGS;
   void* data;
-  size_t length;
+  size_t length, offset;
+  if (!size_arg(&c, 1, &offset)) return NULL;
   if (!bytes_of(&c, 2, &data, &length)) return fail(env, "upload needs a TypedArray or ArrayBuffer");
-  if (!ok(env, dio_buf_upload(handle(&c, 0), (size_t)f64(&c, 1), data, length))) return NULL;
+  if (!ok(env, dio_buf_upload(handle(&c, 0), offset, data, length))) return NULL;
   return undefined(env);
 }
 FN(buf_download) {
   ARGS;
-  size_t bytes = (size_t)f64(&c, 2);
+  size_t bytes, offset;
+  if (!size_arg(&c, 1, &offset) || !size_arg(&c, 2, &bytes)) return NULL;
   void* data = NULL;
   napi_value ab;
+  if (!ok(env, dio_buf_download(handle(&c, 0), offset, NULL, 0))) return NULL; /* range check first */
+  if (bytes > dio_buf_size(handle(&c, 0)) - offset) return fail(env, "download past the end of the buffer");
   napi_create_arraybuffer(env, bytes, &data, &ab);
-  if (!ok(env, dio_buf_download(handle(&c, 0), (size_t)f64(&c, 1), data, bytes))) return NULL;
+  if (!ok(env, dio_buf_download(handle(&c, 0), offset, data, bytes))) return NULL;
   return ab;
 }
 FN(buf_copy) {
   ARGS;
-  if (!ok(env, dio_buf_copy(handle(&c, 0), (size_t)f64(&c, 1), handle(&c, 2), (size_t)f64(&c, 3), (size_t)f64(&c, 4))))
-    return NULL;
+  size_t dst_offset, src_offset, bytes;
+  if (!size_arg(&c, 1, &dst_offset) || !size_arg(&c, 3, &src_offset) || !size_arg(&c, 4, &bytes)) return NULL;
+  if (!ok(env, dio_buf_copy(handle(&c, 0), dst_offset, handle(&c, 2), src_offset, bytes))) return NULL;
   return undefined(env);
 }
 FN(ctx_timestamp) {

@@ -2,10 +2,16 @@
 // The reference's operator classes for Node, over the N-API addon and the C ABI
 // (include/diorama.h). Method and getter names follow src/prefixSum.ts,
 // src/radixSort.ts, src/tileRasterizer.ts, src/gaussianRenderer.ts,
-// src/trainingLoss.ts and src/trainer.ts. Where the reference records into a
-// GPUCommandEncoder, these enqueue on the context's HIP stream in call order;
-// methods the reference makes async (afterSubmit, read, step, ...) return Promises
-// that resolve once the GPU work they wait for is done. Plain ES2019.
+// src/trainingLoss.ts and src/trainer.ts, and so do the argument lists: encode
+// calls take the reference's GPUCommandEncoder (or compute pass) first. Those come
+// from device.createCommandEncoder() / beginComputePass() here. The library
+// enqueues each call on the context's HIP stream at once, in call order, so an
+// encoder is a token of the device it was made on, and finish() and
+// queue.submit() keep the reference's call sequence without deferring anything.
+// Targets the reference gives as rgba8unorm texture views are device.createTexture()
+// views (one u32 per pixel) or plain Buffers. Methods the reference makes async
+// (afterSubmit, read, step, ...) return Promises that resolve once the GPU work
+// they wait for is done. Plain ES2019.
 
 const path = require('path');
 const native = require(path.join(__dirname, 'diorama.node'));
@@ -53,13 +59,154 @@ function view(handle, keep) {
   return handle ? new Buffer(handle, false, keep) : undefined;
 }
 
+/** GPUComputePassEncoder: what PrefixSum/RadixSort.encode and the preprocess hooks receive. */
+class ComputePassEncoder {
+  constructor(encoder) {
+    this.encoder = encoder;
+    this.device = encoder.device;
+  }
+  end() {}
+}
+
+/** GPUCommandBuffer: what finish() returns and queue.submit() takes. */
+class CommandBuffer {
+  constructor(encoder) {
+    this.encoder = encoder;
+  }
+}
+
+/** GPUCommandEncoder. Work is already enqueued on the device's stream when recorded. */
+class CommandEncoder {
+  constructor(device) {
+    this.device = device;
+  }
+  beginComputePass() {
+    return new ComputePassEncoder(this);
+  }
+  copyBufferToBuffer(source, sourceOffset, destination, destinationOffset, size) {
+    destination.copyFrom(source, sourceOffset, destinationOffset, size);
+  }
+  /** clearBuffer over the whole buffer (the only form the reference uses). */
+  clearBuffer(buffer, offset, size) {
+    if ((offset || 0) !== 0 || (size !== undefined && size !== buffer.size)) {
+      throw new RangeError('clearBuffer clears whole buffers only');
+    }
+    buffer.clear();
+  }
+  finish() {
+    return new CommandBuffer(this);
+  }
+}
+
+/** An rgba8unorm GPUTexture: width x height pixels, one u32 each, row by row. */
+class Texture {
+  constructor(device, width, height) {
+    this.width = width;
+    this.height = height;
+    this.format = 'rgba8unorm';
+    this.buffer = device.createBuffer(Math.max(width * height, 1) * 4);
+  }
+  createView() {
+    return new TextureView(this);
+  }
+  destroy() {
+    this.buffer.destroy();
+  }
+}
+
+class TextureView {
+  constructor(texture) {
+    this.texture = texture;
+  }
+}
+
+/** The encoder argument of an encode call, checked: a misplaced argument list fails here. */
+function encoderOf(encoder, what) {
+  if (!(encoder instanceof CommandEncoder)) {
+    throw new TypeError(what + ' takes a command encoder (device.createCommandEncoder()) as its first argument');
+  }
+  return encoder;
+}
+
+function passOf(pass, what) {
+  if (!(pass instanceof ComputePassEncoder)) {
+    throw new TypeError(what + ' takes a compute pass (encoder.beginComputePass()) as its first argument');
+  }
+  return pass;
+}
+
+/** The device buffer behind a render target or photo: a texture, its view, a Buffer, or none. */
+function surface(target) {
+  if (target === null || target === undefined) return null;
+  if (target instanceof TextureView) return target.texture.buffer.handle;
+  if (target instanceof Texture) return target.buffer.handle;
+  if (target instanceof Buffer) return target.handle;
+  throw new TypeError('a target must be a texture, a texture view or a Buffer');
+}
+
+/** A training photo as host rgba8 words: Uint32Array as given, or read back from a texture. */
+function hostImage(image) {
+  if (image instanceof Texture) return new Uint32Array(image.buffer.read(0, image.width * image.height * 4));
+  return image;
+}
+
 /** One device and its in-order stream: the counterpart of initGpu (src/gpu.ts:8-47). */
 class Device {
   constructor(index) {
     this.handle = native.ctx_create(index || 0);
   }
-  createBuffer(bytes) {
+  /** createBuffer({size, usage, label}) as in WebGPU, or createBuffer(bytes). */
+  createBuffer(descriptor) {
+    const bytes = typeof descriptor === 'number' ? descriptor : descriptor.size;
     return new Buffer(native.buf_create(this.handle, bytes), true);
+  }
+  /** createTexture({size: [w, h] | {width, height}, format: 'rgba8unorm'}). */
+  createTexture(descriptor) {
+    const size = descriptor.size;
+    const width = Array.isArray(size) ? size[0] : size.width;
+    const height = Array.isArray(size) ? (size[1] === undefined ? 1 : size[1]) : (size.height === undefined ? 1 : size.height);
+    if (descriptor.format !== undefined && descriptor.format !== 'rgba8unorm') {
+      throw new TypeError('only rgba8unorm textures are supported');
+    }
+    return new Texture(this, width, height);
+  }
+  createCommandEncoder() {
+    return new CommandEncoder(this);
+  }
+  get queue() {
+    const device = this;
+    return {
+      submit(commandBuffers) {
+        for (const c of commandBuffers || []) {
+          if (!(c instanceof CommandBuffer) || c.encoder.device !== device) {
+            throw new TypeError('submit takes command buffers finished on this device');
+          }
+        }
+      },
+      /** writeBuffer(buffer, bufferOffset, data, dataOffset, size): offsets and size in elements of a TypedArray. */
+      writeBuffer(buffer, bufferOffset, data, dataOffset, size) {
+        const element = ArrayBuffer.isView(data) ? data.BYTES_PER_ELEMENT || 1 : 1;
+        const base = ArrayBuffer.isView(data) ? data.byteOffset : 0;
+        const total = ArrayBuffer.isView(data) ? data.byteLength / element : data.byteLength;
+        const from = dataOffset || 0;
+        const count = size === undefined ? total - from : size;
+        const source = ArrayBuffer.isView(data) ? data.buffer : data;
+        buffer.write(new Uint8Array(source, base + from * element, count * element), bufferOffset);
+      },
+      /** writeTexture({texture}, data, {bytesPerRow}, [w, h]) for whole rgba8 images. */
+      writeTexture(destination, data, layout, size) {
+        const texture = destination.texture;
+        const width = Array.isArray(size) ? size[0] : size.width;
+        const height = Array.isArray(size) ? size[1] : size.height;
+        if (width !== texture.width || height !== texture.height || (layout.bytesPerRow && layout.bytesPerRow !== width * 4)) {
+          throw new RangeError('writeTexture writes whole, tightly packed rgba8 images');
+        }
+        texture.buffer.write(data, 0);
+      },
+      onSubmittedWorkDone() {
+        return device.onSubmittedWorkDone();
+      },
+    };
   }
   upload(data) {
     const b = this.createBuffer(data.byteLength);
@@ -99,7 +246,8 @@ class PrefixSum {
   get total() {
     return view(native.prefix_sum_total(this.handle), this);
   }
-  encode() {
+  encode(pass) {
+    passOf(pass, 'PrefixSum.encode');
     native.prefix_sum_encode(this.handle);
   }
   destroy() {
@@ -125,7 +273,8 @@ class RadixSort {
   get values() {
     return view(native.radix_sort_values(this.handle), this);
   }
-  encode() {
+  encode(pass) {
+    passOf(pass, 'RadixSort.encode');
     native.radix_sort_encode(this.handle);
   }
   destroy() {
@@ -134,18 +283,30 @@ class RadixSort {
   }
 }
 
-/** src/tileRasterizer.ts:29-317, for splats already on the device. */
+/**
+ * src/tileRasterizer.ts:29-317, for `count` splats already on the device.
+ * `camera` is the Camera uniform buffer (144 bytes: view, proj, viewport) as in
+ * the reference; the tile stage takes its viewport from encode()'s `viewport`,
+ * which the reference's callers always pass equal to the camera's.
+ */
 class TileRasterizer {
-  constructor(device, splats, count, initialPairCapacity, handle, keep) {
-    if (handle) {
-      this.handle = handle;
-      this.owned = false;
-      this.keep = keep;
-      return;
+  constructor(device, camera, splats, count, initialPairCapacity) {
+    if (!(camera instanceof Buffer) || camera.size < 144 || !(splats instanceof Buffer) || typeof count !== 'number') {
+      throw new TypeError('TileRasterizer takes (device, camera buffer of 144 bytes, splats buffer, count, ' +
+        'initialPairCapacity?)');
     }
     this.handle = native.tiles_create(device.handle, splats.handle, count, initialPairCapacity || 0);
     this.owned = true;
+    this.camera = camera;
     this.keep = splats;
+  }
+  /** The rasterizer a renderer owns: a view that destroy() leaves alone. */
+  static wrap(handle, keep) {
+    const t = Object.create(TileRasterizer.prototype);
+    t.handle = handle;
+    t.owned = false;
+    t.keep = keep;
+    return t;
   }
   get lastPairCount() {
     return native.tiles_last_pair_count(this.handle);
@@ -162,10 +323,14 @@ class TileRasterizer {
   get pixelLast() {
     return view(native.tiles_pixel_last(this.handle), this);
   }
-  encode(target, viewport, background) {
-    native.tiles_encode(this.handle, target ? target.handle : null, viewport[0], viewport[1], background);
+  /** encode(encoder, target, viewport, background, before?): `before(pass)` runs ahead of the tile stage. */
+  encode(encoder, target, viewport, background, before) {
+    const enc = encoderOf(encoder, 'TileRasterizer.encode');
+    if (before) before(enc.beginComputePass());
+    native.tiles_encode(this.handle, surface(target), viewport[0], viewport[1], background);
   }
-  encodeBackward(viewport, pixelGrads, splatGrads) {
+  encodeBackward(encoder, viewport, pixelGrads, splatGrads) {
+    encoderOf(encoder, 'TileRasterizer.encodeBackward');
     native.tiles_encode_backward(this.handle, viewport[0], viewport[1], pixelGrads.handle, splatGrads.handle);
   }
   afterSubmit() {
@@ -180,17 +345,20 @@ class TileRasterizer {
 
 /** src/gaussianRenderer.ts:55-311 */
 class GaussianRenderer {
-  constructor(device, options, handle, keep) {
+  constructor(device, options) {
     this.device = device;
-    if (handle) {
-      this.handle = handle;
-      this.owned = false;
-      this.keep = keep;
-      return;
-    }
     const opts = options || {};
     this.handle = native.renderer_create(device.handle, opts.initialPairCapacity || 0);
     this.owned = true;
+  }
+  /** The renderer a trainer owns: a view that destroy() leaves alone. */
+  static wrap(device, handle, keep) {
+    const r = Object.create(GaussianRenderer.prototype);
+    r.device = device;
+    r.handle = handle;
+    r.owned = false;
+    r.keep = keep;
+    return r;
   }
   /** setGaussians(Float32Array of 16 floats each, {rest, degree}) */
   setGaussians(data, sh) {
@@ -204,17 +372,24 @@ class GaussianRenderer {
   setShDegree(degree) {
     native.renderer_set_sh_degree(this.handle, degree);
   }
-  /** encode(target, camera, background, afterPreprocess): the hook runs between the preprocess and the tiles. */
-  encode(target, camera, background, afterPreprocess) {
+  /**
+   * encode(encoder, target, camera, background, afterPreprocess?): the hook gets
+   * a compute pass between the preprocess and the tiles, as in the reference,
+   * and only when there are Gaussians (with none the target is just cleared).
+   */
+  encode(encoder, target, camera, background, afterPreprocess) {
+    const enc = encoderOf(encoder, 'GaussianRenderer.encode');
     native.renderer_encode_preprocess(this.handle, camera);
-    if (afterPreprocess) afterPreprocess(this);
-    native.renderer_encode_tiles(this.handle, target ? target.handle : null, background);
+    if (afterPreprocess && native.renderer_splat_buffer(this.handle)) afterPreprocess(enc.beginComputePass());
+    native.renderer_encode_tiles(this.handle, surface(target), background);
   }
-  encodeDraw(target, splats, count, viewport, background) {
-    native.renderer_encode_draw(this.handle, target ? target.handle : null, splats.handle, count, viewport[0],
-      viewport[1], background);
+  encodeDraw(encoder, target, splats, count, viewport, background) {
+    encoderOf(encoder, 'GaussianRenderer.encodeDraw');
+    native.renderer_encode_draw(this.handle, surface(target), splats.handle, count, viewport[0], viewport[1],
+      background);
   }
-  encodeBackward(pixelGrads) {
+  encodeBackward(encoder, pixelGrads) {
+    encoderOf(encoder, 'GaussianRenderer.encodeBackward');
     native.renderer_encode_backward(this.handle, pixelGrads.handle);
   }
   afterSubmit() {
@@ -235,7 +410,7 @@ class GaussianRenderer {
   }
   get tiles() {
     const h = native.renderer_tiles(this.handle);
-    return h ? new TileRasterizer(null, null, 0, 0, h, this) : undefined;
+    return h ? TileRasterizer.wrap(h, this) : undefined;
   }
   get pixelBuffer() {
     return view(native.renderer_pixel_buffer(this.handle), this);
@@ -265,8 +440,10 @@ class TrainingLoss {
     this.height = height;
     this.handle = native.loss_create(device.handle, width, height);
   }
-  encode(rendered, target) {
-    native.loss_encode(this.handle, rendered.handle, target.handle);
+  /** encode(encoder, rendered, target): `target` is the photo's texture view (or a Buffer of rgba8 words). */
+  encode(encoder, rendered, target) {
+    encoderOf(encoder, 'TrainingLoss.encode');
+    native.loss_encode(this.handle, rendered.handle, surface(target));
   }
   read() {
     return Promise.resolve(native.loss_read(this.handle));
@@ -305,7 +482,8 @@ const DEFAULT_SETTINGS = (() => {
 
 /**
  * src/trainer.ts:103-446. `raw` = {params: Float32Array (16 per Gaussian), sh:
- * Float32Array (45 per Gaussian)}; `views` = [{name, camera, image: Uint32Array rgba8}].
+ * Float32Array (45 per Gaussian)}; `views` = [{name, camera, image}] where the
+ * image is an rgba8unorm texture (as in the reference) or a Uint32Array of rgba8 words.
  * options.batch > 0 makes a multi-view batch trainer: that many views (of any
  * sizes) per step, their gradients summed into one Adam step.
  */
@@ -314,7 +492,8 @@ class Trainer {
     this.device = device;
     this.settings = settings;
     const batch = options && options.batch ? options.batch : 0;
-    this.handle = native.trainer_create(device.handle, raw.params, raw.sh, views, settings, batch);
+    const hostViews = views.map((v) => Object.assign({}, v, { image: hostImage(v.image) }));
+    this.handle = native.trainer_create(device.handle, raw.params, raw.sh, hostViews, settings, batch);
   }
   get batchSize() {
     return native.trainer_batch_size(this.handle);
@@ -369,7 +548,7 @@ class Trainer {
     };
   }
   get renderer() {
-    return new GaussianRenderer(this.device, null, native.trainer_renderer(this.handle), this);
+    return GaussianRenderer.wrap(this.device, native.trainer_renderer(this.handle), this);
   }
   /** writeGaussianPly(await trainer.exportRaw()) without the host round trip. */
   writePly() {
@@ -400,12 +579,13 @@ class Trainer {
     return Promise.resolve(native.trainer_last_loss(this.handle));
   }
   evaluate(views, withSsim) {
-    return Promise.resolve(native.trainer_evaluate(this.handle, views, withSsim === undefined ? true : withSsim));
+    const hostViews = views.map((v) => Object.assign({}, v, { image: hostImage(v.image) }));
+    return Promise.resolve(native.trainer_evaluate(this.handle, hostViews, withSsim === undefined ? true : withSsim));
   }
   /** measurePsnr (src/insideView.ts:233-267): views [{camera, image}] whose images are device Buffers. */
   measurePsnr(views) {
     return Promise.resolve(native.trainer_measure_psnr(this.handle, views.map((v) => v.camera),
-      views.map((v) => v.image.handle)));
+      views.map((v) => surface(v.image))));
   }
   destroy() {
     if (this.handle) native.trainer_destroy(this.handle);
@@ -550,6 +730,16 @@ function parseGaussianPly(buffer, device, options) {
   return out;
 }
 
+/** readBuffer (src/readback.ts:4-19): the whole buffer as an ArrayBuffer. */
+function readBuffer(device, buffer) {
+  return Promise.resolve(buffer.read());
+}
+
+/** readTexture (src/readback.ts:22-44): rgba8 bytes, row by row. */
+function readTexture(device, texture) {
+  return Promise.resolve(new Uint8Array(texture.buffer.read(0, texture.width * texture.height * 4)));
+}
+
 /** Raw parameters to the renderer's activated layout (src/trainer.ts:449-461). */
 function activate(params) {
   const out = new Float32Array(params.length);
@@ -570,6 +760,11 @@ module.exports = {
   initGpu,
   Device,
   Buffer,
+  CommandEncoder,
+  ComputePassEncoder,
+  Texture,
+  readBuffer,
+  readTexture,
   PrefixSum,
   RadixSort,
   TileRasterizer,

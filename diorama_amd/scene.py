@@ -126,6 +126,55 @@ def ring_cameras(count, width=980, height=545, fov_deg=50.0, radius=6.0, seed=3)
     return cams
 
 
+def split_views(cams, held_out=33, train=263):
+    """The reference's split (src/loadDataset.ts:37-38): every 8th view held out."""
+    held = [k for k in range(len(cams)) if k % 8 == 7][:held_out]
+    skip = set(held)
+    return [c for k, c in enumerate(cams) if k not in skip][:train], [cams[k] for k in held]
+
+
+def teacher_images(ctx, cams, n, seed=7, box=5.0, noise=0.01, noise_seed=7):
+    """rgba8 targets (u32 per pixel): a `seed` teacher of n Gaussians rendered on the
+    GPU for each camera, plus N(0, noise) pixel noise, rounded to 8 bits."""
+    import diorama_amd
+
+    params, sh = random_raw_gaussians(n, seed, box=box)
+    r = diorama_amd.GaussianRenderer(ctx)
+    r.set_gaussians(activate(params), sh, 3)
+    rng = np.random.default_rng(noise_seed)
+    images = []
+    for cam in cams:
+        r.encode(None, cam, [0, 0, 0])
+        px = int(cam["viewport"][0]) * int(cam["viewport"][1])
+        color = r.pixel_buffer.download(np.float32, px * 4).reshape(-1, 4)[:, :3]
+        if noise:
+            color = color + rng.standard_normal(color.shape, dtype=np.float32) * np.float32(noise)
+        q = np.rint(np.clip(color, 0, 1) * 255).astype(np.uint32)
+        images.append(q[:, 0] | (q[:, 1] << 8) | (q[:, 2] << 16) | np.uint32(255 << 24))
+    r.after_submit()
+    r.destroy()
+    return images
+
+
+def rgba8_channels(image):
+    """(pixels, 4) uint8 channels of u32 rgba8 words."""
+    out = np.zeros((len(image), 4), dtype=np.uint8)
+    for c in range(4):
+        out[:, c] = (np.asarray(image, dtype=np.uint32) >> (8 * c)) & 255
+    return out
+
+
+def eye_of(cam):
+    """Camera centre of a view matrix [R | t]: -R^T t."""
+    v = cam["view"]
+    return [-(v[c * 4] * v[12] + v[c * 4 + 1] * v[13] + v[c * 4 + 2] * v[14]) for c in range(3)]
+
+
+def scene_extent(cams):
+    """1.1 x the largest camera distance from the origin (src/loadDataset.ts:39-43)."""
+    return 1.1 * max(math.sqrt(sum(v * v for v in eye_of(c))) for c in cams)
+
+
 # Scan/sort microbenchmark inputs (shared bit-for-bit by the tests and the bench).
 
 def tile_touch_counts(n, seed):

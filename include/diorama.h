@@ -316,6 +316,14 @@ int dio_trainer_features(dio_trainer* t, dio_buf** features);
 /* Per Gaussian (largest gradient length, summed depth, summed gradient x, y)
  * since the last densification; null in mode 0. */
 dio_buf* dio_trainer_extra_stats(dio_trainer* t);
+/* A batch trainer's last step (no reference counterpart; for checks against a
+ * per-view oracle): the training-view index of each of its `*count` slots (up to
+ * `capacity` written to `views`); the splat buffer in which slot s holds its
+ * view's splats at records s * (*stride) ... s * (*stride) + count - 1; and the
+ * pixel colour buffer (vec4 per pixel) holding the slots' images one after
+ * another, row by row, in slot order. */
+int dio_trainer_batch_frames(dio_trainer* t, uint32_t* views, uint32_t capacity, uint32_t* count, dio_buf** splats,
+                             uint32_t* stride, dio_buf** pixels);
 
 /* Times `repeats` launches of one named stage between HIP events recorded on the
  * context's stream, for the benchmark's roofline figures. Stages: "preprocess",

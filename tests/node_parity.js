@@ -1,8 +1,10 @@
 'use strict';
 // Drives the Node classes (diorama_amd/node) the way the reference's checks drive
-// its operators: the prefix sum and radix sort against plain JS
-// (src/checks/prefixSum.ts, src/checks/radixSort.ts), the loss fixture
-// (src/checks/loss.ts:24-89) and the backward fixture (src/checks/backward.ts:49-114).
+// its operators, with the reference's own argument lists (an encoder or compute
+// pass first, texture views as targets): the prefix sum and radix sort against
+// plain JS (src/checks/depthSort.ts:46-104), the loss fixture
+// (src/checks/loss.ts:24-89) and the backward fixture (src/checks/backward.ts:49-114),
+// plus a standalone TileRasterizer over the renderer's splats.
 // Prints one JSON line of results; tests/test_node.py applies the bars.
 // Usage: node tests/node_parity.js <cpu|gpu>
 
@@ -23,6 +25,18 @@ function rng(seed) {
     t ^= t + Math.imul(t ^ (t >>> 7), t | 61);
     return ((t ^ (t >>> 14)) >>> 0) / 4294967296;
   };
+}
+
+function worstDiff(a, b) {
+  let most = 0;
+  for (let i = 0; i < a.length; i++) most = Math.max(most, Math.abs(a[i] - b[i]));
+  return most;
+}
+
+function maxAbs(a) {
+  let most = 0;
+  for (let i = 0; i < a.length; i++) most = Math.max(most, Math.abs(a[i]));
+  return most;
 }
 
 function lookAt(eye) {
@@ -129,7 +143,11 @@ async function gpuChecks() {
   for (let i = 0; i < n; i++) values[i] = Math.floor(random() * 100);
   const data = device.upload(values);
   const scan = new d.PrefixSum(device, data, n);
-  scan.encode();
+  let encoder = device.createCommandEncoder();
+  let pass = encoder.beginComputePass();
+  scan.encode(pass);
+  pass.end();
+  device.queue.submit([encoder.finish()]);
   const scanned = new Uint32Array(data.read());
   let running = 0;
   let scanBad = 0;
@@ -157,9 +175,13 @@ async function gpuChecks() {
       vals[i] = i;
     }
     const high = (i) => (highBits ? keys[i * words + 1] : 0);
-    sort.keys.write(keys);
-    sort.values.write(vals);
-    sort.encode();
+    device.queue.writeBuffer(sort.keys, 0, keys);
+    device.queue.writeBuffer(sort.values, 0, vals);
+    encoder = device.createCommandEncoder();
+    pass = encoder.beginComputePass();
+    sort.encode(pass);
+    pass.end();
+    device.queue.submit([encoder.finish()]);
     const sk = new Uint32Array(sort.keys.read(0, m * 4 * words));
     const sv = new Uint32Array(sort.values.read(0, m * 4));
     const order = Array.from(vals).sort((a, b) => high(a) - high(b) || keys[a * words] - keys[b * words] || a - b);
@@ -183,8 +205,11 @@ async function gpuChecks() {
   }
   const loss = new d.TrainingLoss(device, lw, lh);
   const rb = device.upload(rendered);
-  const tb = device.upload(target);
-  loss.encode(rb, tb);
+  const tb = device.createTexture({ size: [lw, lh], format: 'rgba8unorm' });
+  device.queue.writeTexture({ texture: tb }, target, { bytesPerRow: lw * 4 }, [lw, lh]);
+  encoder = device.createCommandEncoder();
+  loss.encode(encoder, rb, tb.createView());
+  device.queue.submit([encoder.finish()]);
   const value = await loss.read();
   const lg = new Float32Array(loss.pixelGrads.read(0, lw * lh * 16));
   let gmax = 0;
@@ -196,9 +221,6 @@ async function gpuChecks() {
     }
   }
   out.loss = { error: Math.abs(value - lf.loss), gradientError: gerr / gmax };
-  loss.destroy();
-  rb.destroy();
-  tb.destroy();
 
   // Backward fixture through GaussianRenderer, with the afterPreprocess hook.
   const bf = fixture('backward.json');
@@ -216,10 +238,17 @@ async function gpuChecks() {
   renderer.setGaussians(g, { rest, degree: bf.shDegree });
   const [w, h] = bf.viewport;
   let hookSaw = -1;
-  renderer.encode(null, { view: bf.view, proj: bf.proj, viewport: bf.viewport }, bf.background, (r) => {
-    hookSaw = r.splatBuffer.size / (d.FLOATS_PER_SPLAT * 4);
+  let hookPass = false;
+  const frame = device.createTexture({ size: [w, h], format: 'rgba8unorm' });
+  const camera = { view: bf.view, proj: bf.proj, viewport: bf.viewport };
+  encoder = device.createCommandEncoder();
+  renderer.encode(encoder, frame.createView(), camera, bf.background, (p) => {
+    hookPass = p instanceof d.ComputePassEncoder;
+    hookSaw = renderer.splatBuffer.size / (d.FLOATS_PER_SPLAT * 4);
   });
+  device.queue.submit([encoder.finish()]);
   await renderer.afterSubmit();
+  const frameBytes = await d.readTexture(device, frame);
   const color = new Float32Array(renderer.pixelBuffer.read(0, w * h * 16));
   let imageErr = 0;
   for (let p = 0; p < w * h; p++) {
@@ -228,7 +257,9 @@ async function gpuChecks() {
   const pg = new Float32Array(w * h * 4);
   for (let p = 0; p < w * h; p++) for (let c = 0; c < 3; c++) pg[p * 4 + c] = bf.pixelGrads[p * 3 + c];
   const pgb = device.upload(pg);
-  renderer.encodeBackward(pgb);
+  encoder = device.createCommandEncoder();
+  renderer.encodeBackward(encoder, pgb);
+  device.queue.submit([encoder.finish()]);
   const grads = renderer.gradients;
   const gg = new Float32Array(grads.gaussians.read(0, count * 64));
   const shg = new Float32Array(grads.sh.read(0, count * 180));
@@ -256,7 +287,54 @@ async function gpuChecks() {
     }
   });
   gradientErrors.sh = shErr / shRef;
-  out.render = { count: renderer.count, hookSaw, imageError: imageErr, gradientErrors };
+  out.render = { count: renderer.count, hookSaw, hookPass, imageError: imageErr, gradientErrors,
+    textureMatches: frameBytes.length === w * h * 4 && frameBytes[3] === 255 };
+
+  // A standalone TileRasterizer over the renderer's splats (src/tileRasterizer.ts:59,127,200):
+  // the same pair lists, so the same image; its splat gradients equal the renderer's
+  // up to the order of float atomics.
+  const cameraBuffer = device.createBuffer({ size: 144, usage: 0 });
+  const splatGrads = device.createBuffer({ size: count * 36, usage: 0 });
+  const tiles = new d.TileRasterizer(device, cameraBuffer, renderer.splatBuffer, renderer.count);
+  let beforePass = false;
+  encoder = device.createCommandEncoder();
+  tiles.encode(encoder, null, [w, h], bf.background, (p) => {
+    beforePass = p instanceof d.ComputePassEncoder;
+  });
+  tiles.encodeBackward(encoder, [w, h], pgb, splatGrads);
+  device.queue.submit([encoder.finish()]);
+  await tiles.afterSubmit();
+  const own = new Float32Array(tiles.pixelColor.read(0, w * h * 16));
+  const rendererColor = new Float32Array(renderer.pixelBuffer.read(0, w * h * 16));
+  const sgA = new Float32Array(splatGrads.read(0, count * 36));
+  const sgB = new Float32Array(grads.splats.read(0, count * 36));
+  out.tiles = { beforePass, pairs: tiles.lastPairCount, rendererPairs: renderer.tiles.lastPairCount,
+    imageDiff: worstDiff(own, rendererColor), gradientDiff: worstDiff(sgA, sgB) / Math.max(1e-30, maxAbs(sgB)) };
+  tiles.destroy();
+
+  // The argument lists of the pre-drop-in forms now fail loudly instead of binding
+  // an encoder or camera to the wrong parameter.
+  const refused = [];
+  const expectTypeError = (name, f) => {
+    try {
+      f();
+      refused.push(name + ': accepted');
+    } catch (e) {
+      refused.push(e instanceof TypeError ? name : name + ': ' + e.message);
+    }
+  };
+  expectTypeError('renderer.encode(target, camera, bg)', () => renderer.encode(null, camera, bf.background));
+  expectTypeError('renderer.encodeBackward(grads)', () => renderer.encodeBackward(pgb));
+  expectTypeError('loss.encode(rendered, target)', () => loss.encode(rb, tb.createView()));
+  expectTypeError('scan.encode()', () => new d.PrefixSum(device, pgb, 4).encode());
+  expectTypeError('new TileRasterizer(device, splats, count)', () => new d.TileRasterizer(device, renderer.splatBuffer, count));
+  out.refused = refused;
+  loss.destroy();
+  rb.destroy();
+  tb.destroy();
+  frame.destroy();
+  cameraBuffer.destroy();
+  splatGrads.destroy();
   pgb.destroy();
   renderer.destroy();
 
@@ -274,11 +352,19 @@ async function gpuChecks() {
   logits.destroy();
 
   // A batch trainer over views of two sizes: one step.
+  // The reference's TrainingView holds its photo as an rgba8unorm texture; one view
+  // here passes a Uint32Array of rgba8 words instead.
   const views = [[64, 48], [48, 64], [64, 48]].map(([vw, vh], k) => {
     const a = (2 * Math.PI * k) / 3;
     const eye = [4 * Math.sin(a), 0, 4 * Math.cos(a)];
-    return { camera: { view: lookAt(eye), proj: perspective(Math.PI / 3, vw / vh), viewport: [vw, vh] },
-      image: new Uint32Array(vw * vh).fill(0xff808080) };
+    const words = new Uint32Array(vw * vh).fill(0xff808080);
+    let image = words;
+    if (k > 0) {
+      image = device.createTexture({ size: [vw, vh], format: 'rgba8unorm' });
+      device.queue.writeTexture({ texture: image }, words, { bytesPerRow: vw * 4 }, [vw, vh]);
+    }
+    return { name: 'view' + k, camera: { view: lookAt(eye), proj: perspective(Math.PI / 3, vw / vh), viewport: [vw, vh] },
+      image };
   });
   const raw = { params: new Float32Array(16 * 64), sh: new Float32Array(45 * 64) };
   for (let i = 0; i < 64; i++) {

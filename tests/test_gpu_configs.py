@@ -1,0 +1,324 @@
+"""The benchmark configurations end to end on the GPU, against the oracle.
+
+* Config 0: one plain step (seed-1 target, L1 + D-SSIM loss, backward, Adam) of
+  16,384 Gaussians at 256x256 against the oracle's float64 step, oc_train_step
+  (src/trainer.ts:192-244).
+* Config 1: one plain Trainer step of 1.1M Gaussians at 980x545 on the ring scene,
+  stage by stage given the GPU's own splats (pair lists bit-exact, image, loss,
+  splat gradients, preprocess backward, statistics, Adam) and against
+  oc_train_step; then 20 more steps checked by properties (finite values, a
+  falling loss).
+* Config 3: one step of the batch trainer over 16 views of the five sizes
+  {320x240 ... 1600x900}, against the oracle's render, loss and backward of every
+  view, summed, and its Adam step.
+
+Adam's first step divides each moment by its bias correction, so the first
+moment after one step is (1 - 0.9) x the gradient: the moments compare the
+gradients end to end. A parameter then moves by the learning rate times the
+gradient's sign, which float rounding can flip only where the gradient is a tiny
+fraction of its group's largest; those entries may differ by twice the learning
+rate and are counted.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import diorama_amd
+from diorama_amd import scene
+from oracle import oracle
+from tests.gpu_util import config0_scene, context, say
+from tests.test_gpu_render import assert_image_matches, gpu_frame, group_error
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 1e-3
+# Moment slot groups: (name, slots, learning-rate index into the oracle's lr list).
+GROUPS = [("position", [0, 1, 2], 2), ("opacity", [3], 3), ("scale", [4, 5, 6], 4), ("rotation", [8, 9, 10, 11], 5),
+          ("color", [12, 13, 14], 6), ("sh", list(range(16, 61)), 7)]
+
+
+def adam_lr(iteration, adam_steps, extent):
+    """Bias corrections and per-group learning rates of step `iteration`
+    (src/trainer.ts:209-227)."""
+    progress = min(1.0, iteration / 30000)
+    position = math.exp(math.log(1.6e-4) * (1 - progress) + math.log(1.6e-6) * progress) * extent
+    return [1 - 0.9 ** adam_steps, 1 - 0.999 ** adam_steps, position, 0.025, 0.005, 0.001, 0.0025, 0.0025 / 20]
+
+
+def check_first_adam_step(label, p_gpu, sh_gpu, m_gpu, p_ref, sh_ref, m_ref, lr, visible_mismatch=0):
+    """Moments within GRAD_TOL of each group's largest; parameters equal where the
+    gradient's sign is determined. Returns the errors per group."""
+    n = len(p_gpu)
+    raw_gpu = np.concatenate([p_gpu, sh_gpu], axis=1)
+    raw_ref = np.concatenate([p_ref, sh_ref], axis=1)
+    errors, flips = {}, 0
+    for name, slots, li in GROUPS:
+        g_ref = m_ref[:, [2 * k for k in slots]]
+        g_gpu = m_gpu[:, [2 * k for k in slots]]
+        v_ref = m_ref[:, [2 * k + 1 for k in slots]]
+        v_gpu = m_gpu[:, [2 * k + 1 for k in slots]]
+        errors[name] = group_error(g_gpu, g_ref)
+        errors[name + " v"] = group_error(v_gpu, v_ref)
+        scale = np.abs(g_ref).max()
+        settled = np.abs(g_ref) > 1e-3 * scale
+        d = np.abs(raw_gpu[:, slots] - raw_ref[:, slots])
+        tol = 2e-6 + 2e-7 * np.abs(raw_ref[:, slots])
+        bad = (d > tol) & settled
+        unsettled = (d > tol) & ~settled
+        flips += int(unsettled.sum())
+        assert (d <= 2 * lr[li] + tol).all(), f"{label}: {name} moved by more than twice its learning rate"
+        errors[name + " params off"] = int(bad.sum())
+    print(f"{label}: first-moment (gradient) err / group max " +
+          ", ".join(f"{k} {v:.2e}" for k, v in errors.items() if not k.endswith("off")) +
+          f"; parameters off where the gradient sign is settled: "
+          f"{ {k[:-11]: v for k, v in errors.items() if k.endswith('params off')} }; "
+          f"unsettled-sign moves {flips} of {n * 59}")
+    for name, _, _ in GROUPS:
+        assert errors[name] < GRAD_TOL, f"{label}: {name} gradient"
+        assert errors[name + " v"] < 2 * GRAD_TOL, f"{label}: {name} second moment"
+        assert errors[name + " params off"] <= visible_mismatch * 16, f"{label}: {name} parameters"
+    return errors
+
+
+def assert_splats_match(label, splats, o_splats, n):
+    """Preprocess outputs against the oracle's float64 preprocess of the same
+    activated Gaussians: each field relative to its own scale (means against at
+    least 64 px, the conic against its diagonal, colours against 1), the radius
+    exact but for rare roundings."""
+    vis = (splats[:, 3] > 0) & (o_splats[:, 3] > 0)
+    flips = int(((splats[:, 3] > 0) != (o_splats[:, 3] > 0)).sum())
+    a, b = splats[vis].astype(np.float64), o_splats[vis]
+    scale = np.maximum(np.abs(b), 1e-3)
+    # Pixel coordinates against at least 64 px: the float32 projection of a position
+    # ~5 units out carries a few 1e-7 of NDC error, a few 1e-4 px at 1600 wide.
+    scale[:, 0:2] = np.maximum(scale[:, 0:2], 64.0)
+    scale[:, 4:7] = np.maximum(np.abs(b[:, 4]), np.abs(b[:, 6]))[:, None]
+    scale[:, 8:11] = np.maximum(scale[:, 8:11], 1.0)
+    rel = np.abs(a - b) / scale
+    fields = {"mean": [0, 1], "depth": [2], "conic": [4, 5, 6], "color": [8, 9, 10], "opacity": [11]}
+    worst = {k: float(rel[:, c].max()) for k, c in fields.items()}
+    radius_off = int((a[:, 3] != b[:, 3]).sum())
+    print(f"{label} preprocess: {vis.sum()} visible; max rel err per field "
+          + ", ".join(f"{k} {v:.2e}" for k, v in worst.items())
+          + f"; visibility flips {flips}; radius mismatches {radius_off}")
+    if max(worst.values()) >= 1e-4:
+        k = int(np.argmax(rel.max(axis=1)))
+        print(f"{label} preprocess worst row: gpu {a[k].tolist()} oracle {b[k].tolist()}")
+    assert max(worst.values()) < 1e-4
+    assert flips <= max(2, n // 100000) and radius_off <= max(2, vis.sum() // 2000)
+
+
+def test_config0_step_against_oracle_train_step():
+    """16,384 Gaussians (seed 0), 256x256, the seed-1 set rendered as the target."""
+    ctx = context()
+    params, sh, _, cam = config0_scene(seed=0)
+    n = len(params)
+    image = scene.teacher_images(ctx, [cam], n, seed=1, box=1.0, noise=0)[0]
+    target = scene.rgba8_channels(image)
+    settings = diorama_amd.default_settings(1.0)
+    say("config0 step")
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], [image], settings)
+    t.sh_degree = 3
+    t.optimize()
+    tiles = t.renderer.tiles
+    tiles.after_submit()
+    gpu_pairs = tiles.last_pair_count
+    gpu_loss = t.last_loss()
+    p_gpu = t.buffers["params"].download(np.float32).reshape(n, 16).astype(np.float64)
+    sh_gpu = t.renderer.sh_buffer.download(np.float32).reshape(n, 45).astype(np.float64)
+    m_gpu = t.buffers["moments"].download(np.float32).reshape(n, 122).astype(np.float64)
+    st_gpu = t.buffers["stats"].download(np.float32).reshape(n, 4).astype(np.float64)
+    act_gpu = t.renderer.gaussian_buffer.download(np.float32).reshape(n, 16).astype(np.float64)
+    t.destroy()
+
+    p = params.astype(np.float64)
+    s = sh.astype(np.float64)
+    m = np.zeros((n, 122))
+    st = np.zeros((n, 4))
+    act = oracle.activate(p)
+    lr = adam_lr(1, 1, 1.0)
+    o_loss, o_pairs = oracle.train_step(p, s, m, st, act, 3, cam, [0, 0, 0], target, lr)
+    print(f"config0 step: loss {gpu_loss:.8f} vs oracle {o_loss:.8f} (err {abs(gpu_loss - o_loss):.2e}); "
+          f"pairs {gpu_pairs} vs {o_pairs}")
+    assert abs(gpu_loss - o_loss) < 1e-5
+    assert abs(gpu_pairs - o_pairs) <= max(16, 1e-3 * o_pairs)
+    seen = (st_gpu[:, 1] > 0) != (st[:, 1] > 0)
+    check_first_adam_step("config0 step", p_gpu, sh_gpu, m_gpu, p, s, m, lr, visible_mismatch=int(seen.sum()) + 2)
+    assert seen.sum() <= 2
+    both = (st_gpu[:, 1] > 0) & (st[:, 1] > 0)
+    assert group_error(st_gpu[both, 0], st[both, 0]) < GRAD_TOL
+    assert (st_gpu[both, 2] != st[both, 2]).sum() <= max(2, both.sum() // 2000)
+    # the activated Gaussians are the GPU's own parameters activated (rows whose
+    # parameters took the other sign of a vanishing gradient differ from the oracle's)
+    cols = [k for k in range(16) if k not in (7, 15)]
+    assert np.allclose(act_gpu[:, cols], oracle.activate(p_gpu)[:, cols], rtol=1e-5, atol=2e-6)
+
+
+@pytest.fixture(scope="module")
+def config1():
+    """The bench's ring scene at full size: 1.1M seed-0 Gaussians in the 10-unit box,
+    the first training pose at 980x545, its target the seed-7 1.1M teacher plus
+    N(0, 0.01) noise."""
+    ctx = context()
+    cams = scene.ring_cameras(296)
+    train, _ = scene.split_views(cams)
+    cam = train[0]
+    say("config1 teacher")
+    image = scene.teacher_images(ctx, [cam], 1_100_000, seed=7, box=5.0, noise=0.01, noise_seed=7)[0]
+    params, sh = scene.random_raw_gaussians(1_100_000, 0, box=5.0)
+    return ctx, cam, image, params, sh, scene.scene_extent(train)
+
+
+def test_config1_step_against_oracle(config1):
+    ctx, cam, image, params, sh, extent = config1
+    n = len(params)
+    w, h = 980, 545
+    target = scene.rgba8_channels(image)
+    settings = diorama_amd.default_settings(extent)
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], [image], settings)
+    t.iteration = 15000
+    t.sh_degree = 3
+    r = t.renderer
+    act0 = r.gaussian_buffer.download(np.float32).reshape(n, 16)
+    say("config1 step")
+    t.optimize()
+    r.tiles.after_submit()
+    gpu_loss = t.last_loss()
+    splats, color, last, ranges, pairs = gpu_frame(r, w, h)
+    sgrad = r.gradients["splats"].download(np.float32).reshape(-1, 9)[:n]
+    p_gpu = t.buffers["params"].download(np.float32).reshape(n, 16).astype(np.float64)
+    sh_gpu = r.sh_buffer.download(np.float32).reshape(n, 45).astype(np.float64)
+    m_gpu = t.buffers["moments"].download(np.float32).reshape(n, 122).astype(np.float64)
+    st_gpu = t.buffers["stats"].download(np.float32).reshape(n, 4).astype(np.float64)
+    t.destroy()
+    print(f"config1 step: {n} Gaussians, {len(pairs)} pairs, {(splats[:, 3] > 0).sum()} visible")
+
+    # Stage by stage, given the GPU's own splats.
+    o_splats = oracle.preprocess(act0, cam, sh, 3)
+    vis = (splats[:, 3] > 0) & (o_splats[:, 3] > 0)
+    flips = int(((splats[:, 3] > 0) != (o_splats[:, 3] > 0)).sum())
+    assert_splats_match("config1", splats, o_splats, n)
+    o_ranges, o_pairs = oracle.tile_lists(splats.astype(np.float64), w, h)
+    assert np.array_equal(ranges, o_ranges), "config1: tile ranges differ"
+    assert np.array_equal(pairs, o_pairs), "config1: tile lists differ"
+    o_color, o_last, margin = oracle.rasterize(splats.astype(np.float64), o_ranges, o_pairs, w, h, [0, 0, 0],
+                                               margin=True)
+    assert_image_matches(color, last, o_color, o_last, margin, "config1")
+    o_loss, o_pgrad, _ = oracle.loss(w, h, color.astype(np.float64), target)
+    print(f"config1 loss: {gpu_loss:.8f} vs oracle on the GPU image {o_loss:.8f}")
+    assert abs(gpu_loss - o_loss) < 1e-5
+    o_sgrad = oracle.rasterize_backward(splats.astype(np.float64), o_ranges, o_pairs, w, h, [0, 0, 0],
+                                        color.astype(np.float64), last, o_pgrad)
+    for cols_, name in [((0, 1), "mean"), ((2, 3, 4), "conic"), ((5,), "opacity"), ((6, 7, 8), "color")]:
+        e = group_error(sgrad[:, cols_], o_sgrad[:, cols_])
+        print(f"config1 rasterize backward {name}: err / group max {e:.2e}")
+        assert e < GRAD_TOL, name
+    gg, shg = oracle.preprocess_backward(act0, sh, 3, cam, splats, sgrad)
+    lr = adam_lr(15001, 1, extent)
+    o_p, o_sh, o_m, _ = oracle.adam_step(params, sh, np.zeros((n, 122)), gg, shg, lr)
+    check_first_adam_step("config1 Adam on the GPU's gradients", p_gpu, sh_gpu, m_gpu, o_p, o_sh, o_m, lr)
+    o_stats = oracle.accumulate_stats(splats, sgrad, cam["viewport"], np.zeros((n, 4)))
+    assert np.allclose(st_gpu, o_stats, rtol=1e-5, atol=1e-7)
+
+    # End to end: the oracle's own float64 step from the same raw parameters.
+    p = params.astype(np.float64)
+    s = sh.astype(np.float64)
+    m = np.zeros((n, 122))
+    st = np.zeros((n, 4))
+    e_loss, e_pairs = oracle.train_step(p, s, m, st, oracle.activate(p), 3, cam, [0, 0, 0], target, lr)
+    print(f"config1 end to end: loss {gpu_loss:.8f} vs oc_train_step {e_loss:.8f}; pairs {len(pairs)} vs {e_pairs}")
+    assert abs(gpu_loss - e_loss) < 1e-5
+    assert abs(len(pairs) - e_pairs) <= max(16, 1e-3 * e_pairs)
+    seen = int(((st_gpu[:, 1] > 0) != (st[:, 1] > 0)).sum())
+    check_first_adam_step("config1 vs oc_train_step", p_gpu, sh_gpu, m_gpu, p, s, m, lr, visible_mismatch=seen + 2)
+
+
+def test_config1_twenty_steps(config1):
+    """20 more steps on the same view: every loss finite and falling, parameters finite."""
+    ctx, cam, image, params, sh, extent = config1
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], [image], diorama_amd.default_settings(extent))
+    t.iteration = 15000
+    t.sh_degree = 3
+    say("config1 twenty steps")
+    losses = []
+    for _ in range(21):
+        t.step()
+        losses.append(t.last_loss())
+    raw = t.export_raw()
+    t.destroy()
+    print("config1 losses:", [round(v, 5) for v in losses])
+    assert all(math.isfinite(v) for v in losses)
+    assert losses[-1] < losses[0] and losses[-1] < losses[10]
+    assert np.isfinite(raw["params"]).all() and np.isfinite(raw["sh"]).all()
+
+
+BATCH_SIZES = [(320, 240), (640, 360), (980, 545), (1280, 720), (1600, 900)]
+
+
+def test_config3_batch_of_16_mixed_views_against_oracle():
+    """16 views (each of the five sizes at least three times) in one batch step of
+    200,000 Gaussians: the summed gradients (through the first moments), the
+    parameters, the statistics and the mean loss against the oracle's per-view
+    render, loss and backward."""
+    ctx = context()
+    n = 200_000
+    ring = scene.ring_cameras(16)
+    cams = []
+    for k, base in enumerate(ring):
+        bw, bh = BATCH_SIZES[k % len(BATCH_SIZES)]
+        cams.append(scene.camera(base["view"], scene.perspective(math.radians(50.0), bw / bh, 0.01, 100.0), bw, bh))
+    say("config3 teacher")
+    images = scene.teacher_images(ctx, cams, n, seed=7, box=5.0, noise=0.01, noise_seed=9)
+    params, sh = scene.random_raw_gaussians(n, 0, box=5.0)
+    extent = scene.scene_extent(cams)
+    t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(extent), batch=16)
+    assert t.batch_size == 16
+    t.sh_degree = 3
+    act0 = t.renderer.gaussian_buffer.download(np.float32).reshape(n, 16)
+    say("config3 batch step")
+    t.optimize()
+    gpu_loss = t.last_loss()
+    p_gpu = t.buffers["params"].download(np.float32).reshape(n, 16).astype(np.float64)
+    sh_gpu = t.renderer.sh_buffer.download(np.float32).reshape(n, 45).astype(np.float64)
+    m_gpu = t.buffers["moments"].download(np.float32).reshape(n, 122).astype(np.float64)
+    st_gpu = t.buffers["stats"].download(np.float32).reshape(n, 4).astype(np.float64)
+    frames = t.batch_frames(cams)
+    t.destroy()
+    assert sorted(v for v, _, _ in frames) == list(range(16))
+
+    # Every view's render, loss and backward in the oracle, given that view's splats
+    # as the GPU's batch preprocess wrote them, and the loss of the GPU's image (L1's
+    # gradient jumps where a pixel equals its 8-bit target, and float32 and float64
+    # renders land on either side of that at a few thousand of 42M channels).
+    gg = np.zeros((n, 16))
+    shg = np.zeros((n, 45))
+    stats = np.zeros((n, 4))
+    losses, pairs = [], 0
+    for v, gpu_splats, gpu_color in frames:
+        cam, image = cams[v], images[v]
+        w, h = int(cam["viewport"][0]), int(cam["viewport"][1])
+        assert_splats_match(f"config3 view {v} ({w}x{h})", gpu_splats, oracle.preprocess(act0, cam, sh, 3), n)
+        splats = gpu_splats.astype(np.float64)
+        ranges, plist = oracle.tile_lists(splats, w, h)
+        color, last, margin = oracle.rasterize(splats, ranges, plist, w, h, [0, 0, 0], margin=True)
+        # the batch keeps no per-view pixel_last to compare; the image is compared
+        assert_image_matches(gpu_color, last, color, last, margin, f"config3 view {v}")
+        value, pgrad, _ = oracle.loss(w, h, gpu_color.astype(np.float64), scene.rgba8_channels(image))
+        sgrad = oracle.rasterize_backward(splats, ranges, plist, w, h, [0, 0, 0], color, last, pgrad)
+        g1, s1 = oracle.preprocess_backward(act0, sh, 3, cam, splats, sgrad)
+        gg += g1
+        shg += s1
+        stats = oracle.accumulate_stats(splats, sgrad, cam["viewport"], stats)
+        losses.append(value)
+        pairs += len(plist)
+    lr = adam_lr(1, 1, extent)
+    o_p, o_sh, o_m, _ = oracle.adam_step(params, sh, np.zeros((n, 122)), gg, shg, lr)
+    print(f"config3: 16 views, {pairs} pairs in all; mean loss {gpu_loss:.8f} vs oracle {np.mean(losses):.8f}")
+    assert abs(gpu_loss - float(np.mean(losses))) < 1e-5
+    seen = int(((st_gpu[:, 1] > 0) != (stats[:, 1] > 0)).sum())
+    check_first_adam_step("config3 batch", p_gpu, sh_gpu, m_gpu, o_p, o_sh, o_m, lr, visible_mismatch=seen + 2)
+    assert group_error(st_gpu[:, 0], stats[:, 0]) < GRAD_TOL
+    views_off = int((st_gpu[:, 1] != stats[:, 1]).sum())
+    print(f"config3 statistics: view-count mismatches {views_off}, visibility flips {seen}")
+    assert views_off <= max(4, n // 20000)

@@ -289,7 +289,46 @@ def test_viewport_tile_limit_is_an_error():
     cam = scene.camera(scene.IDENTITY, scene.perspective(1.0, 1.0, 0.1, 100), 4112, 4112)
     with pytest.raises(diorama_amd.DioramaError, match="more tiles than the sort can tell apart"):
         r.encode(None, cam, [0, 0, 0])
+    # 2^20 x 2^20 pixels is 2^32 tiles, which a 32-bit product would wrap to 0
+    cam = scene.camera(scene.IDENTITY, scene.perspective(1.0, 1.0, 0.1, 100), 1 << 20, 1 << 20)
+    with pytest.raises(diorama_amd.DioramaError, match="more tiles than the sort can tell apart"):
+        r.encode(None, cam, [0, 0, 0])
+    for side in (-16.0, float("nan"), float("inf"), 2.0 ** 40):
+        cam = scene.camera(scene.IDENTITY, scene.perspective(1.0, 1.0, 0.1, 100), side, 64)
+        with pytest.raises(diorama_amd.DioramaError, match="viewport side"):
+            r.encode(None, cam, [0, 0, 0])
     r.destroy()
+
+
+def test_buffer_ranges_that_would_wrap_are_refused():
+    """An offset of -16 reaches the C ABI as 2^64 - 16; offset + bytes would wrap past
+    the size check. The library refuses it on the host before any copy is issued,
+    and the Python wrapper refuses negative offsets before the call."""
+    import ctypes
+
+    ctx = context()
+    buf = ctx.buffer(64)
+    src = np.arange(4, dtype=np.float32)
+    lib = diorama_amd.lib()
+    wrapped = ctypes.c_size_t(-16).value
+    assert wrapped == 2 ** 64 - 16
+    p = src.ctypes.data_as(ctypes.c_void_p)
+    assert lib.dio_buf_upload(buf.handle, wrapped, p, 16) == diorama_amd.DIO_ERR_INVALID
+    assert lib.dio_buf_download(buf.handle, wrapped, p, 16) == diorama_amd.DIO_ERR_INVALID
+    assert lib.dio_buf_upload(buf.handle, 60, p, wrapped) == diorama_amd.DIO_ERR_INVALID
+    assert lib.dio_buf_copy(buf.handle, 0, buf.handle, wrapped, 16) == diorama_amd.DIO_ERR_INVALID
+    assert lib.dio_buf_copy(buf.handle, wrapped, buf.handle, 0, 16) == diorama_amd.DIO_ERR_INVALID
+    assert lib.dio_buf_upload(buf.handle, 65, p, 0) == diorama_amd.DIO_ERR_INVALID
+    for bad in (-16, 1.5, True):
+        with pytest.raises(ValueError):
+            buf.upload(src, offset=bad)
+        with pytest.raises(ValueError):
+            buf.download(np.float32, 4, offset=bad)
+    # the edges themselves are in range
+    buf.upload(src, offset=48)
+    assert np.array_equal(buf.download(np.float32, 4, offset=48), src)
+    assert buf.download(np.float32, 0, offset=64).size == 0
+    buf.destroy()
 
 
 def test_many_more_pairs_than_the_last_frame():

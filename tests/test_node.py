@@ -60,9 +60,16 @@ def test_node_parity_on_gpu():
     assert out["sort"] == {"high0": 0, "high16": 0}
     assert out["loss"]["error"] < 1e-5 and out["loss"]["gradientError"] < 1e-3
     r = out["render"]
-    assert r["count"] == 10 and r["hookSaw"] >= 10
+    assert r["count"] == 10 and r["hookSaw"] >= 10 and r["hookPass"] and r["textureMatches"]
     assert r["imageError"] < 1e-4
     assert max(r["gradientErrors"].values()) < 1e-3 and not any(math.isnan(v) for v in r["gradientErrors"].values())
+    t = out["tiles"]
+    assert t["beforePass"] and t["pairs"] == t["rendererPairs"] > 0
+    assert t["imageDiff"] == 0 and t["gradientDiff"] < 1e-5
+    # every pre-drop-in argument list is refused with a TypeError
+    assert out["refused"] == ["renderer.encode(target, camera, bg)", "renderer.encodeBackward(grads)",
+                              "loss.encode(rendered, target)", "scan.encode()",
+                              "new TileRasterizer(device, splats, count)"]
     assert out["mlp"]["logits"] == [0.5, -1.25, 2.0, 3.75] * 5 and out["mlp"]["weightsBack"] == 3.75
     assert out["batch"]["size"] == 2 and out["batch"]["iteration"] == 1 and math.isfinite(out["batch"]["loss"])
     p = out["ply"]
