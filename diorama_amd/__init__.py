@@ -94,6 +94,7 @@ SIGNATURES = {
     "dio_prefix_sum_total": (_P, [_P]),
     "dio_prefix_sum_encode": (_I, [_P]),
     "dio_prefix_sum_destroy": (None, [_P]),
+    "dio_prefix_sum_encode_static": (_I, [_P, _P, _U32, _P, _P]),
     "dio_radix_sort_create": (_I, [_P, _U32, _U32, _P, ctypes.POINTER(_P)]),
     "dio_radix_sort_keys": (_P, [_P]),
     "dio_radix_sort_values": (_P, [_P]),
@@ -350,6 +351,13 @@ class PrefixSum:
 
     def encode(self):
         _check(lib().dio_prefix_sum_encode(self.handle))
+
+    @staticmethod
+    def encode_static(ctx, data, capacity, total, count=None):
+        """The allocation-free scan (no reference counterpart): `data` scanned in place,
+        the grand total into element 0 of `total`, with the library's static workspace."""
+        _check(lib().dio_prefix_sum_encode_static(ctx.handle, data.handle, capacity, count.handle if count else None,
+                                                  total.handle))
 
     def destroy(self):
         if self.handle:

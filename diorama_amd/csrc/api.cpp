@@ -239,6 +239,27 @@ extern "C" int dio_prefix_sum_encode(dio_prefix_sum* s) {
   });
 }
 
+extern "C" int dio_prefix_sum_encode_static(dio_ctx* ctx, dio_buf* data, uint32_t capacity, const dio_buf* count,
+                                            dio_buf* total) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(data, "data");
+    need(total, "total");
+    if (capacity > dio::kStaticScanMaxValues) {
+      invalid("the allocation-free prefix sum takes at most " + std::to_string(dio::kStaticScanMaxValues) + " values");
+    }
+    if (static_cast<size_t>(capacity) * 4 > data->bytes) invalid("the data buffer is smaller than the capacity");
+    if (total->bytes < 4) invalid("the total buffer is smaller than 4 bytes");
+    if (count && count->bytes < 4) invalid("the count buffer is smaller than 4 bytes");
+    const dio::ScanWorkspace ws = dio::static_scan_workspace(ctx->device, capacity);
+    // The tile states, the ticket and the finish counter are one run of words; a
+    // fill kernel (no memset node) clears them, whatever an earlier call left.
+    dio::fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(ws.tile_state), 0u, static_cast<size_t>(ws.tiles) * 2 + 2);
+    dio::scan_exclusive(ctx->stream, dev<uint32_t>(data), dev<uint32_t>(data), count ? dev<uint32_t>(const_cast<dio_buf*>(count)) : nullptr,
+                        capacity, dev<uint32_t>(total), ws);
+  });
+}
+
 extern "C" void dio_prefix_sum_destroy(dio_prefix_sum* s) {
   if (!s) return;
   (void)hipStreamSynchronize(s->ctx->stream);

@@ -62,6 +62,11 @@ size_t scan_workspace_bytes(uint32_t capacity);
 ScanWorkspace scan_workspace(void* base, uint32_t capacity);
 void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
                     uint32_t* total, const ScanWorkspace& ws);
+// The allocation-free scan's workspace on `device`, laid out for `capacity` values
+// inside a static device array (at most kStaticScanMaxValues; 128 KiB per GPU). Its
+// contents are whatever the last call left: clear the words a call uses first.
+constexpr uint32_t kStaticScanMaxValues = 1u << 26;
+ScanWorkspace static_scan_workspace(int device, uint32_t capacity);
 
 // ------------------------------------------------------------------ sort --
 // Stable LSD onesweep radix sort, 8- or 4-bit digits, of u32 keys or u64 keys stored as

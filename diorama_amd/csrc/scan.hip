@@ -8,6 +8,8 @@
 // the ticket and the finish counter for the next call. Replaces the Blelloch block scan + recursive levels of
 // src/shaders/prefixSum.wgsl:30-91 / src/prefixSum.ts:59-98 (same results: an
 // exclusive scan of min(count, capacity) values, total in element 0 of `total`).
+#include <mutex>
+
 #include "dio_internal.hpp"
 
 namespace dio {
@@ -184,7 +186,41 @@ __global__ void fill_kernel(uint32_t* p, uint32_t value, size_t count) {
   for (; i < count; i += stride) p[i] = value;
 }
 
+// The allocation-free scan's workspace: one look-back state per 4096-value tile of
+// up to kStaticScanMaxValues values, then the ticket and the finish counter. Every
+// GPU the library runs on holds its own copy for the life of the process. Host code
+// reaches it only through the address hipGetSymbolAddress gives for the current
+// device (the array's name on the host is a stand-in, not a device address).
+constexpr uint32_t kStaticScanTiles = kStaticScanMaxValues / kScanTile;
+static __device__ __align__(256) uint64_t scratch[kStaticScanTiles + 1];
+
 }  // namespace
+
+ScanWorkspace static_scan_workspace(int device, uint32_t capacity) {
+  if (capacity > kStaticScanMaxValues) {
+    throw Error(-1, "the allocation-free scan takes at most " + std::to_string(kStaticScanMaxValues) + " values");
+  }
+  constexpr int kDevices = 64;
+  static std::mutex lock;
+  static void* address[kDevices] = {};
+  if (device < 0 || device >= kDevices) throw Error(-1, "no such device");
+  void* base;
+  {
+    std::lock_guard<std::mutex> guard(lock);
+    if (!address[device]) {
+      int current = 0;
+      check_hip(hipGetDevice(&current), "hipGetDevice");
+      if (current != device) check_hip(hipSetDevice(device), "hipSetDevice");
+      void* p = nullptr;
+      hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (current != device) (void)hipSetDevice(current);
+      check_hip(e, "hipGetSymbolAddress(scratch)");
+      address[device] = p;
+    }
+    base = address[device];
+  }
+  return scan_workspace(base, capacity);
+}
 
 void fill_u32(hipStream_t s, uint32_t* p, uint32_t value, size_t count) {
   if (count == 0) return;

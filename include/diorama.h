@@ -103,6 +103,15 @@ int dio_prefix_sum_create(dio_ctx* ctx, dio_buf* data, uint32_t capacity, dio_bu
 dio_buf* dio_prefix_sum_total(dio_prefix_sum* scan);
 int dio_prefix_sum_encode(dio_prefix_sum* scan);
 void dio_prefix_sum_destroy(dio_prefix_sum* scan);
+/* No counterpart in the reference: PrefixSum's create + encode + total in one call
+ * that allocates nothing, for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops). It scans min(*count, capacity) values of
+ * `data` in place (all `capacity` when `count` is null) and writes the grand total
+ * to element 0 of `total`, as dio_prefix_sum_encode does, with its workspace in a
+ * static device array of the library (128 KiB per GPU, kept until the process
+ * ends). At most 2^26 values; larger capacities fail with DIO_ERR_INVALID. Calls
+ * that overlap on one GPU share that array and race: serialise them. */
+int dio_prefix_sum_encode_static(dio_ctx* ctx, dio_buf* data, uint32_t capacity, const dio_buf* count, dio_buf* total);
 
 /* ------------------------------------------------------------ RadixSort -- */
 /* src/radixSort.ts:27-142: stable sort of u32 keys (high_bits = 0) or of 64-bit
