@@ -176,6 +176,7 @@ SIGNATURES = {
     "dio_trainer_iteration": (_U32, [_P]),
     "dio_trainer_set_iteration": (None, [_P, _U32]),
     "dio_trainer_sh_degree": (_U32, [_P]),
+    "dio_trainer_last_view": (_U32, [_P]),
     "dio_trainer_set_sh_degree": (_I, [_P, _U32]),
     "dio_trainer_densification_count": (_U32, [_P]),
     "dio_trainer_densification": (_I, [_P, _U32, ctypes.POINTER(_Densification)]),
@@ -822,6 +823,11 @@ class Trainer:
     @sh_degree.setter
     def sh_degree(self, value):
         _check(lib().dio_trainer_set_sh_degree(self.handle, int(value)))
+
+    @property
+    def last_view(self):
+        """The training view the last plain step used."""
+        return lib().dio_trainer_last_view(self.handle)
 
     @property
     def densifications(self):

@@ -66,6 +66,12 @@ void check_tile_limit(uint32_t w, uint32_t h) {
   }
 }
 
+// Exchanges the device memory behind two buffers of one context; the handles stay.
+void swap_storage(dio_buf* a, dio_buf* b) {
+  std::swap(a->ptr, b->ptr);
+  std::swap(a->bytes, b->bytes);
+}
+
 uint32_t next_pow2_capacity(uint32_t count) {
   double want = std::ceil(std::log2(count * 1.25));
   return static_cast<uint32_t>(std::min(std::pow(2.0, want), 4294967295.0));
@@ -1023,7 +1029,7 @@ extern "C" int dio_training_loss_encode(dio_training_loss* l, dio_buf* rendered,
 }
 
 extern "C" int dio_training_loss_read(dio_training_loss* l, double* out) {
-  return guard([&] {
+  return guard_read([&] {
     need(l, "loss");
     need(out, "out");
     *out = loss_read(l);
@@ -1273,12 +1279,13 @@ void trainer_optimize(dio_trainer* t) {
   if (use_prepared) {
     // The previous step's fused backward already ran this preprocess: swap its
     // outputs in (the previous frame's buffers take their place for the next one).
+    // The storage moves, not the handles, so a handle a caller took from
+    // dio_renderer_splat_buffer keeps naming the renderer's current splats.
     dio_renderer::Scene* sc = t->renderer->scene;
-    std::swap(sc->splats, prep.splats);
-    tiles->splats = sc->splats;
-    std::swap(tiles->tile_counts, prep.tile_counts);
-    std::swap(tiles->depth_keys, prep.depth_keys);
-    std::swap(tiles->depth_values, prep.depth_values);
+    swap_storage(sc->splats, prep.splats);
+    swap_storage(tiles->tile_counts, prep.tile_counts);
+    swap_storage(tiles->depth_keys, prep.depth_keys);
+    swap_storage(tiles->depth_values, prep.depth_values);
     std::swap(t->vis, prep.vis);
     to_camera(view.camera, t->renderer->camera);
     t->renderer->preprocessed = true;
@@ -1952,7 +1959,7 @@ extern "C" int dio_trainer_reset_opacity(dio_trainer* t) {
 }
 
 extern "C" int dio_trainer_export_raw(dio_trainer* t, float* params, float* sh) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     need(params, "params");
     need(sh, "sh");
@@ -2096,6 +2103,7 @@ extern "C" void dio_trainer_set_iteration(dio_trainer* t, uint32_t iteration) {
   if (t) t->iteration = iteration;
 }
 extern "C" uint32_t dio_trainer_sh_degree(const dio_trainer* t) { return t ? t->sh_degree : 0; }
+extern "C" uint32_t dio_trainer_last_view(const dio_trainer* t) { return t ? t->last_view : 0; }
 extern "C" int dio_trainer_set_sh_degree(dio_trainer* t, uint32_t degree) {
   return guard([&] {
     need(t, "trainer");
@@ -2108,7 +2116,7 @@ extern "C" uint32_t dio_trainer_densification_count(const dio_trainer* t) {
   return t ? static_cast<uint32_t>(t->densifications.size()) : 0;
 }
 extern "C" int dio_trainer_densification(const dio_trainer* t, uint32_t index, dio_densification* out) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     need(out, "out");
     if (index >= t->densifications.size()) invalid("no such densification");
@@ -2367,7 +2375,7 @@ extern "C" int dio_mlp_create(dio_ctx* ctx, uint32_t seed, dio_densify_mlp** out
 }
 
 extern "C" int dio_mlp_get_weights(dio_densify_mlp* m, float* out) {
-  return guard([&] {
+  return guard_read([&] {
     need(m, "mlp");
     need(out, "out");
     std::vector<uint16_t> bf(dio::kMlpParams);
@@ -2587,7 +2595,7 @@ extern "C" int dio_trainer_set_tuning(dio_trainer* t, const dio_tuning* tuning) 
 }
 
 extern "C" int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out) {
-  return guard([&] {
+  return guard_read([&] {
     need(t, "trainer");
     need(out, "out");
     *out = t->tuning;

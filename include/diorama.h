@@ -256,6 +256,9 @@ void dio_trainer_set_iteration(dio_trainer* t, uint32_t iteration);
 uint32_t dio_trainer_sh_degree(const dio_trainer* t);
 /* shDegree is a public field of the reference Trainer (src/trainer.ts:105). */
 int dio_trainer_set_sh_degree(dio_trainer* t, uint32_t degree);
+/* The training view (index into the views given at creation) of the last plain
+ * step: the one nextView() returned (src/trainer.ts:395-404). */
+uint32_t dio_trainer_last_view(const dio_trainer* t);
 uint32_t dio_trainer_densification_count(const dio_trainer* t);
 int dio_trainer_densification(const dio_trainer* t, uint32_t index, dio_densification* out);
 dio_buf* dio_trainer_params(dio_trainer* t);

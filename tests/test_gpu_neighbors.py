@@ -76,3 +76,28 @@ def test_starting_gaussians():
     assert d[:, [0, 1, 2, 3, 7, 8, 9, 10, 11, 12, 13, 14, 15]].max() == 0
     assert d[:, 4:7].max() <= 1          # log in double on each side, rounded once
     assert not raw["sh"].any() and raw["sh"].shape == (n, 45)
+
+
+def test_colmap_reconstruction_to_starting_gaussians():
+    """A hand-assembled COLMAP reconstruction read through the C ABI, its points turned
+    into starting Gaussians on the GPU (src/loadDataset.ts:87-149), against the oracle's
+    reader (oracle/colmap.py), brute-force 3-NN and initial_gaussians."""
+    from oracle import colmap
+    from tests.test_colmap import scene_files
+
+    cb, ib, pb, _, _ = scene_files(seed=5, photos=6, points=2000)
+    got = diorama_amd.parse_colmap(cb, ib, pb)
+    want = colmap.parse(cb, ib, pb)
+    positions = np.array(want["positions"], np.float32)
+    colors = np.array(want["colors"], np.uint8)
+    assert np.array_equal(got["positions"], positions) and np.array_equal(got["colors"], colors)
+    ctx = context()
+    say("colmap to gaussians", len(positions))
+    raw = diorama_amd.initial_gaussians(ctx, got["positions"], got["colors"])
+    d2 = oc.mean_neighbor_distance2(positions)
+    o_params, o_sh = oc.initial_gaussians(positions, colors, d2)
+    assert np.array_equal(oc.mean_neighbor_distance2(got["positions"]), d2)
+    d = np.abs(raw["params"].view(np.int32).astype(np.int64) - o_params.view(np.int32).astype(np.int64))
+    assert d[:, [0, 1, 2, 3, 7, 8, 9, 10, 11, 12, 13, 14, 15]].max() == 0
+    assert d[:, 4:7].max() <= 1          # log in double on each side, rounded once
+    assert not raw["sh"].any() and raw["sh"].shape == o_sh.shape

@@ -334,3 +334,73 @@ def test_step_gradients_start_from_zero():
     assert np.abs(got - want).max() <= 1e-4 * scale
     loss.destroy()
     t.destroy()
+
+
+def reference_view_order(views, steps):
+    """nextView (src/trainer.ts:395-404) with mulberry32(1): every view once, in a
+    Fisher-Yates shuffled order popped from the end, then reshuffle."""
+    random = scene.mulberry32(1)
+    queue, out = [], []
+    for _ in range(steps):
+        if not queue:
+            queue = list(range(views))
+            for i in range(len(queue) - 1, 0, -1):
+                j = int(math.floor(random() * (i + 1)))
+                queue[i], queue[j] = queue[j], queue[i]
+        out.append(queue.pop())
+    return out
+
+
+def test_schedule_view_order_sh_degree_and_opacity_reset():
+    """The Trainer's schedule against src/trainer.ts:192-252: the view order of a
+    mulberry32(1) Fisher-Yates replay, the SH degree one higher at steps 1000, 2000
+    and 3000, and the opacity reset at step 3000 (while densifying). Around the SH
+    step at 1000 every step's splats equal a fresh preprocess of the Gaussians the
+    step started from, at that step's degree, for that step's view: the frame the
+    previous step prepared ahead was drawn for the right view and degree. A handle
+    to the renderer's splats taken before those steps still names its splats."""
+    ctx = context()
+    n = 600
+    params, sh = scene.random_raw_gaussians(n, 3, box=1.0)
+    cams = []
+    for k in range(5):
+        a = 2 * math.pi * k / 5
+        cams.append(scene.camera(scene.look_at([3 * math.sin(a), 0.4, 3 * math.cos(a)], [0, 0, 0], [0, 1, 0]),
+                                 scene.perspective(math.pi / 3, 1.0, 0.01, 100.0), 48, 48))
+    teacher, teacher_sh = scene.random_raw_gaussians(n, 4, box=1.0)
+    images = render_views(scene.activate(teacher), cams, teacher_sh, 3)
+    settings = diorama_amd.default_settings(100.0)
+    # Densify without cloning, splitting or pruning (only the cadence is under test),
+    # every 300 steps so that none falls in 997-1003: a densification hands the
+    # renderer new buffers, as adoptGaussians does in the reference.
+    settings.update(densify_grad_threshold=1e9, min_opacity=0.0, densify_interval=300)
+    say("schedule")
+    t = diorama_amd.Trainer(ctx, params, sh, cams, images, settings)
+    r = t.renderer
+    held = None
+    order = reference_view_order(len(cams), 3001)
+    cap = math.log(0.01 / 0.99)
+    for step in range(1, 3001):
+        check = 997 <= step <= 1003
+        if step == 997:
+            held = r.splat_buffer
+        if check:
+            g0 = r.gaussian_buffer.download(np.float32).reshape(-1, 16)[:n]
+            sh0 = r.sh_buffer.download(np.float32).reshape(-1, 45)[:n]
+        if step == 3000:
+            before = t.buffers["params"].download(np.float32).reshape(-1, 16)[:, 3]
+            assert (before > cap + 0.5).any()
+        t.step()
+        assert t.iteration == step
+        assert t.last_view == order[step - 1], f"step {step}"
+        assert t.sh_degree == min(3, step // 1000), f"step {step}"
+        if check:
+            got = held.download(np.float32).reshape(-1, 12)[:n]
+            want = fresh_splats(g0, sh0, t.sh_degree, cams[t.last_view])
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"step {step}"
+    assert len(t.densifications) == len([k for k in range(501, 3001) if k % 300 == 0])
+    p = t.buffers["params"].download(np.float32).reshape(-1, 16)
+    m = t.buffers["moments"].download(np.float32).reshape(-1, 122)
+    assert (p[:, 3] <= np.float32(cap) + 1e-6).all()
+    assert not m[:, 6:8].any()
+    t.destroy()
