@@ -95,39 +95,64 @@ def teacher_targets(ctx, cams, n):
     return scene.teacher_images(ctx, cams, n, seed=7, box=5.0, noise=0.01, noise_seed=7)
 
 
-def cpu_baseline(params, sh, cam, image, sh_degree, budget_s=30.0):
-    """One step of the float64 CPU restatement on the host cores: the full step when
-    it fits the budget, else the same step on a random subset of the Gaussians with
-    the time scaled to the full count."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(params, sh, cam, image, sh_degree, repeats=5, budget_s=40.0):
+    """The float64 CPU restatement of one optimize() step (render, loss, backward,
+    stats, Adam) on every host core this process may use: one warm-up step, then the
+    median of `repeats` timed steps, on the full Gaussian set when a step fits the
+    budget, else on a random subset with the time scaled to the full count."""
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle import oracle
 
-    threads = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     n = len(params)
     rng = np.random.default_rng(0)
     target = np.zeros((WIDTH * HEIGHT, 4), dtype=np.uint8)
     for c in range(3):
         target[:, c] = (image >> (8 * c)) & 255
     lr = [1 - 0.9 ** 15000, 1 - 0.999 ** 15000, 1.6e-4 * 0.01 ** 0.5, 0.025, 0.005, 0.001, 0.0025, 0.0025 / 20]
-    def run(sample):
+
+    def inputs(sample):
         idx = np.sort(rng.choice(n, size=sample, replace=False)) if sample < n else np.arange(n)
         p = params[idx].astype(np.float64)
         s = sh[idx].astype(np.float64)
-        m = np.zeros((sample, 122))
-        st = np.zeros((sample, 4))
+        return p, s
+
+    def run(base):
+        p, s = base[0].copy(), base[1].copy()
+        m = np.zeros((len(p), 122))
+        st = np.zeros((len(p), 4))
         g = oracle.activate(p)
         t0 = time.perf_counter()
         oracle.train_step(p, s, m, st, g, sh_degree, cam, [0, 0, 0], target, lr)
         return time.perf_counter() - t0
 
-    sample = min(n, 110_000)
-    dt = run(sample)
-    if sample < n and dt * n / sample <= budget_s:
-        sample, dt = n, run(n)
+    sample = n
+    base = inputs(n)
+    first = run(base)  # warm-up: page-in, OpenMP thread start
+    if first * (repeats + 1) > budget_s:
+        sample = max(10_000, int(n * budget_s / (first * (repeats + 1))))
+        base = inputs(sample)
+        run(base)
+    times = [run(base) for _ in range(repeats)]
+    dt = float(np.median(times))
     scaled = dt * n / sample
     return {"value": 1.0 / scaled, "unit": "steps/s", "cores": oracle.threads(), "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"one optimize() step (render, loss, backward, stats, Adam) in float64 on {sample:,} of "
-                      f"the {n:,} Gaussians at {WIDTH}x{HEIGHT}: {dt:.2f} s"
+                      f"the {n:,} Gaussians at {WIDTH}x{HEIGHT}, OpenMP on {oracle.threads()} threads: median "
+                      f"{dt:.3f} s of {repeats} steps after a warm-up step (all: "
+                      + ", ".join(f"{t:.3f}" for t in times) + " s)"
                       + ("" if sample == n else f", time scaled by {n / sample:.2f} to the full count")}
 
 
@@ -612,7 +637,8 @@ def main():
     if not args.no_cpu:
         t0 = time.perf_counter()
         cpu = cpu_baseline(params, sh, train[0], images[0], 3)
-        log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads ({time.perf_counter() - t0:.1f} s)")
+        log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads of {cpu['cpu_model']} "
+            f"({time.perf_counter() - t0:.1f} s)")
     trainer_tuning = trainer.tuning
     autotune_table = None if tuned is None else [[k["splat_batch"], k["digit_bits"], round(ms, 4)] for k, ms in tuned]
     trainer.destroy()
