@@ -540,6 +540,8 @@ def main():
     ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
+    ap.add_argument("--tuning", default=None,
+                    help="train: tile_size,splat_batch,digit_bits to use instead of autotuning (e.g. 32,128,8)")
     ap.add_argument("--workload", choices=["train", "scan-sort", "batch", "learned"], default="train",
                     help="train: the training step (default); scan-sort: the scan/sort microbenchmark; "
                          "batch: the multi-view batch trainer; learned: learned vs plain densification")
@@ -581,10 +583,13 @@ def main():
     for _ in range(args.warmup):
         trainer.step()
     tuned = None
-    if not args.no_autotune:
+    if args.tuning:
+        tile, batch, bits = (int(v) for v in args.tuning.split(","))
+        trainer.tuning = {"tile_size": tile, "splat_batch": batch, "digit_bits": bits}
+    elif not args.no_autotune:
         tuned = trainer.autotune(steps=3)
-        log("autotune: " + ", ".join(f"batch {k['splat_batch']} digits {k['digit_bits']}: {ms:.3f} ms"
-                                     for k, ms in tuned))
+        log("autotune: " + ", ".join(f"tile {k['tile_size']} batch {k['splat_batch']} digits {k['digit_bits']}: "
+                                     f"{ms:.3f} ms" for k, ms in tuned))
     ctx.synchronize()
     log(f"warmup {args.warmup} steps done; tuning {trainer.tuning}")
     t0 = time.perf_counter()
@@ -640,7 +645,8 @@ def main():
         log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads of {cpu['cpu_model']} "
             f"({time.perf_counter() - t0:.1f} s)")
     trainer_tuning = trainer.tuning
-    autotune_table = None if tuned is None else [[k["splat_batch"], k["digit_bits"], round(ms, 4)] for k, ms in tuned]
+    autotune_table = None if tuned is None else [[k["tile_size"], k["splat_batch"], k["digit_bits"], round(ms, 4)]
+                                                 for k, ms in tuned]
     trainer.destroy()
     ctx.close()
     line = {

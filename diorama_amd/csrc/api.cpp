@@ -40,11 +40,13 @@ std::string& last_error() {
 
 namespace {
 
-void to_camera(const dio_camera& c, dio::CameraArgs& out) {
+// `bin_shift`: the tile counts the preprocess writes are per (16 << bin_shift)-px bin.
+void to_camera(const dio_camera& c, dio::CameraArgs& out, uint32_t bin_shift = 0) {
   std::memcpy(out.view, c.view, sizeof(out.view));
   std::memcpy(out.proj, c.proj, sizeof(out.proj));
   out.viewport[0] = c.viewport[0];
   out.viewport[1] = c.viewport[1];
+  out.bin_shift = bin_shift;
 }
 
 // One side of a viewport as a pixel count. Cameras carry it as a float; one that is
@@ -470,6 +472,8 @@ struct dio_tile_rasterizer {
   uint32_t frame_w = 0, frame_h = 0;
   float frame_bg[3] = {0.f, 0.f, 0.f};
   uint32_t splat_batch = 64, digit_bits = 8;  // tuning; results do not depend on them
+  uint32_t tile_size = 16;  // tuning: 8, 16 or 32 (32: pairs binned by 32-px bins)
+  uint32_t bin_shift() const { return tile_size == 32 ? 1u : 0u; }
   StageClock* clock = nullptr;  // set by a trainer only while it runs a clocked step
 };
 
@@ -565,7 +569,10 @@ uint32_t tile_sort_bits(uint32_t tiles, uint32_t digit_bits = 8) {
 void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t h, const float bg[3],
                   bool counts_ready) {
   check_tile_limit(w, h);
-  const uint32_t tiles_x = dio::ceil_div(w, dio::kTile), tiles_y = dio::ceil_div(h, dio::kTile);
+  // Pair lists per 16-px tile, or per 32-px bin (tile size 32).
+  const uint32_t shift = t->bin_shift();
+  const uint32_t tiles_x = dio::ceil_div(dio::ceil_div(w, dio::kTile), 1u << shift);
+  const uint32_t tiles_y = dio::ceil_div(dio::ceil_div(h, dio::kTile), 1u << shift);
   const uint32_t tiles = tiles_x * tiles_y;
   if (target && target->bytes < static_cast<size_t>(w) * h * 4) invalid("the target is smaller than the viewport");
   ensure_frame_buffers(t, tiles, w * h);
@@ -573,7 +580,7 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   const float* splats = dev<float>(t->splats);
   if (!counts_ready)
     dio::count_tiles(s, splats, t->count, w, h, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
-                     dev<uint32_t>(t->depth_values));
+                     dev<uint32_t>(t->depth_values), shift);
   // Depth order of the splats (culled ones last), with their tile counts in that
   // order, then each splat's first pair slot.
   if (dio::radix_sort(s, false, t->depth_keys->ptr, dev<uint32_t>(t->depth_values), t->depth_alt_keys->ptr,
@@ -593,7 +600,7 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   const uint32_t pair_hint = t->last_pair_count;
   dio::duplicate(s, splats, dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets), dev<uint32_t>(t->tile_counts),
                  t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total), dev<uint32_t>(t->pair_tiles),
-                 dev<uint32_t>(t->pair_splats), pair_hint);
+                 dev<uint32_t>(t->pair_splats), pair_hint, shift);
   mark(kMarkDuplicate);
   t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats), t->alt_tiles->ptr,
                                      dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total), t->capacity, 0,
@@ -608,7 +615,7 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   mark(kMarkRanges);
   dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), w,
                  h, bg, dev<uint32_t>(target), dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
-                 t->splat_batch);
+                 t->splat_batch, t->tile_size);
   mark(kMarkRasterize);
   t->framed = true;
   t->frame_w = w;
@@ -651,7 +658,7 @@ void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pix
   dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges),
                           dev<uint32_t>(t->tile_order), w, h, t->frame_bg, dev<float>(t->pixel_color),
                           dev<uint32_t>(t->pixel_last), dev<float>(pixel_grads), dev<float>(splat_grads),
-                          t->splat_batch);
+                          t->splat_batch, t->tile_size);
   if (t->clock) t->clock->mark(s, kMarkBackward);
 }
 
@@ -763,8 +770,8 @@ void fill_background(dio_ctx* ctx, dio_buf* target, uint32_t w, uint32_t h, cons
 
 // `vis` (nullable): each Gaussian's (depth, radius) for the trainer's fused backward.
 void renderer_preprocess(dio_renderer* r, const dio_camera& camera, dio_buf* vis = nullptr) {
-  to_camera(camera, r->camera);
   auto* sc = r->scene;
+  to_camera(camera, r->camera, sc ? sc->rasterizer->bin_shift() : 0u);
   if (!sc) return;
   auto* t = sc->rasterizer;
   dio::preprocess(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
@@ -1163,7 +1170,7 @@ struct dio_trainer {
             *vis = nullptr;
     uint32_t count = 0;  // Gaussians the buffers are sized for
     bool ready = false;
-    uint32_t iteration = 0, view = 0, sh_degree = 0;
+    uint32_t iteration = 0, view = 0, sh_degree = 0, bin_shift = 0;
     uint64_t epoch = 0;
     const dio_tile_rasterizer* tiles = nullptr;
   } prepared;
@@ -1267,6 +1274,7 @@ void trainer_optimize(dio_trainer* t) {
   dio_tile_rasterizer* tiles = t->renderer->scene->rasterizer;
   tiles->splat_batch = t->tuning.splat_batch;
   tiles->digit_bits = t->tuning.digit_bits;
+  tiles->tile_size = t->tuning.tile_size;
   if (t->clock.enabled) {
     t->clock.begin(t->ctx->stream);
     tiles->clock = &t->clock;
@@ -1274,6 +1282,7 @@ void trainer_optimize(dio_trainer* t) {
   auto& prep = t->prepared;
   const bool use_prepared = prep.ready && prep.iteration == step && prep.view == t->last_view &&
                             prep.sh_degree == t->sh_degree && prep.tiles == tiles && prep.count == t->count &&
+                            prep.bin_shift == tiles->bin_shift() &&
                             prep.epoch == dio::mutation_epoch().load(std::memory_order_relaxed);
   prep.ready = false;
   if (use_prepared) {
@@ -1287,7 +1296,7 @@ void trainer_optimize(dio_trainer* t) {
     swap_storage(tiles->depth_keys, prep.depth_keys);
     swap_storage(tiles->depth_values, prep.depth_values);
     std::swap(t->vis, prep.vis);
-    to_camera(view.camera, t->renderer->camera);
+    to_camera(view.camera, t->renderer->camera, tiles->bin_shift());
     t->renderer->preprocessed = true;
     if (tiles->clock) tiles->clock->mark(t->ctx->stream, kMarkPreprocess);
   } else {
@@ -1356,7 +1365,7 @@ void trainer_optimize(dio_trainer* t) {
         prep.vis = alloc(t->ctx, n * 8);
         prep.count = t->count;
       }
-      to_camera(t->views[t->pending_view].camera, next.cam);
+      to_camera(t->views[t->pending_view].camera, next.cam, tiles->bin_shift());
       next.sh_degree = next_degree;
       next.splats = dev<float>(prep.splats);
       next.tile_counts = dev<uint32_t>(prep.tile_counts);
@@ -1367,6 +1376,7 @@ void trainer_optimize(dio_trainer* t) {
       prep.view = t->pending_view;
       prep.sh_degree = next_degree;
       prep.tiles = tiles;
+      prep.bin_shift = tiles->bin_shift();
     }
     dio::backward_adam(t->ctx->stream, t->count, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree,
                        t->renderer->camera, dev<float>(t->vis), dev<float>(sc->grad_splats),
@@ -2182,7 +2192,9 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       // The whole binning sort (Gaussians by depth, then pairs by tile) on the data a
       // step really sorts: the chain preprocess -> depth sort -> scan -> duplicate ->
       // tile sort, less the same chain without the two sorts (timed apart below).
-      const uint32_t tiles = dio::ceil_div(tr->frame_w, dio::kTile) * dio::ceil_div(tr->frame_h, dio::kTile);
+      const uint32_t shift = tr->bin_shift();
+      const uint32_t tiles = dio::ceil_div(dio::ceil_div(tr->frame_w, dio::kTile), 1u << shift) *
+                             dio::ceil_div(dio::ceil_div(tr->frame_h, dio::kTile), 1u << shift);
       auto chain = [&](bool sorts) {
         dio::preprocess(s, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
                         dev<float>(sc->splats), dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_keys),
@@ -2196,7 +2208,8 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
                             dev<uint32_t>(tr->pair_total), dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
         dio::duplicate(s, dev<float>(sc->splats), dev<uint32_t>(tr->depth_values), dev<uint32_t>(tr->offsets),
                        dev<uint32_t>(tr->tile_counts), tr->count, tr->frame_w, tr->frame_h, tr->capacity,
-                       dev<uint32_t>(tr->pair_total), dev<uint32_t>(tr->pair_tiles), dev<uint32_t>(tr->pair_splats));
+                       dev<uint32_t>(tr->pair_total), dev<uint32_t>(tr->pair_tiles), dev<uint32_t>(tr->pair_splats), 0,
+                       shift);
         if (sorts)
           dio::radix_sort(s, false, tr->pair_tiles->ptr, dev<uint32_t>(tr->pair_splats), tr->alt_tiles->ptr,
                           dev<uint32_t>(tr->alt_splats), dev<uint32_t>(tr->pair_total), tr->capacity, 0,
@@ -2230,7 +2243,7 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       run = [&] {
         dio::rasterize(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges),
                        dev<uint32_t>(tr->tile_order), tr->frame_w, tr->frame_h, tr->frame_bg, nullptr,
-                       dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last), tr->splat_batch);
+                       dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last), tr->splat_batch, tr->tile_size);
       };
     } else if (name == "rasterize_backward") {
       run = [&] {
@@ -2238,7 +2251,7 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
                                 dev<uint32_t>(tr->tile_order), tr->frame_w, tr->frame_h, tr->frame_bg,
                                 dev<float>(tr->pixel_color),
                                 dev<uint32_t>(tr->pixel_last), dev<float>(t->loss->pixel_grads),
-                                dev<float>(sc->grad_splats), tr->splat_batch);
+                                dev<float>(sc->grad_splats), tr->splat_batch, tr->tile_size);
       };
     } else if (name == "preprocess_backward") {
       run = [&] {
@@ -2571,15 +2584,30 @@ extern "C" int dio_trainer_batch_frames(dio_trainer* t, uint32_t* views, uint32_
 
 namespace {
 
-void check_tuning(const dio_tuning& k) {
-  if (k.tile_size != 16) invalid("the tile size is 16 pixels (the reference's tiling decides which pixels a splat touches)");
+void check_tuning(const dio_trainer* t, const dio_tuning& k) {
+  if (k.tile_size != 8 && k.tile_size != 16 && k.tile_size != 32) invalid("the tile size must be 8, 16 or 32 pixels");
   if (k.splat_batch != 64 && k.splat_batch != 128 && k.splat_batch != 256) invalid("splat batch must be 64, 128 or 256");
+  if (k.tile_size == 32 && k.splat_batch == 256) invalid("32-pixel tiles take a splat batch of 64 or 128");
   if (k.digit_bits != 4 && k.digit_bits != 8) invalid("digit width must be 4 or 8 bits");
+  if (t->batch && k.tile_size != 16) invalid("a batch trainer rasterizes 16-pixel tiles");
+}
+
+// The autotuner's candidates: every valid (tile size, splat batch, digit width).
+std::vector<dio_tuning> tuning_candidates(const dio_trainer* t) {
+  std::vector<dio_tuning> out;
+  for (uint32_t tile : {16u, 8u, 32u})
+    for (uint32_t bits : {8u, 4u})
+      for (uint32_t batch : {64u, 128u, 256u}) {
+        if ((t->batch && tile != 16) || (tile == 32 && batch == 256)) continue;
+        out.push_back(dio_tuning{tile, batch, bits});
+      }
+  return out;
 }
 
 void apply_tuning(dio_trainer* t, const dio_tuning& k) {
-  check_tuning(k);
-  const bool changed = k.splat_batch != t->tuning.splat_batch || k.digit_bits != t->tuning.digit_bits;
+  check_tuning(t, k);
+  const bool changed = k.splat_batch != t->tuning.splat_batch || k.digit_bits != t->tuning.digit_bits ||
+                       k.tile_size != t->tuning.tile_size;
   t->tuning = k;
   if (changed) t->generation++;  // a captured batch graph bakes the launches in
 }
@@ -2613,29 +2641,28 @@ extern "C" int dio_trainer_autotune(dio_trainer* t, uint32_t steps, dio_tuning_r
     DIO_HIP(hipEventCreate(&e1));
     std::vector<dio_tuning_result> found;
     try {
-      for (uint32_t bits : {8u, 4u})
-        for (uint32_t batch : {64u, 128u, 256u}) {
-          apply_tuning(t, dio_tuning{16, batch, bits});
-          trainer_optimize(t);  // untimed: captures a batch graph, warms the launches
-          trainer_maintain(t);
-          const bool replay = t->batch && t->batch->exec && !t->due_densify;
-          DIO_HIP(hipEventRecord(e0, s));
-          for (uint32_t k = 0; k < steps; k++) {
-            if (replay) {
-              // A batch trainer replays its step graph on the same views, so every
-              // candidate is timed on the same work.
-              DIO_HIP(hipGraphLaunch(t->batch->exec, s));
-            } else {
-              trainer_optimize(t);
-              trainer_maintain(t);
-            }
+      for (const dio_tuning& candidate : tuning_candidates(t)) {
+        apply_tuning(t, candidate);
+        trainer_optimize(t);  // untimed: captures a batch graph, warms the launches
+        trainer_maintain(t);
+        const bool replay = t->batch && t->batch->exec && !t->due_densify;
+        DIO_HIP(hipEventRecord(e0, s));
+        for (uint32_t k = 0; k < steps; k++) {
+          if (replay) {
+            // A batch trainer replays its step graph on the same views, so every
+            // candidate is timed on the same work.
+            DIO_HIP(hipGraphLaunch(t->batch->exec, s));
+          } else {
+            trainer_optimize(t);
+            trainer_maintain(t);
           }
-          DIO_HIP(hipEventRecord(e1, s));
-          DIO_HIP(hipEventSynchronize(e1));
-          float ms = 0.f;
-          DIO_HIP(hipEventElapsedTime(&ms, e0, e1));
-          found.push_back({t->tuning, static_cast<double>(ms) / steps});
         }
+        DIO_HIP(hipEventRecord(e1, s));
+        DIO_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        DIO_HIP(hipEventElapsedTime(&ms, e0, e1));
+        found.push_back({t->tuning, static_cast<double>(ms) / steps});
+      }
     } catch (...) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

@@ -102,7 +102,38 @@ struct CameraArgs {
   float view[16];
   float proj[16];
   float viewport[2];
+  uint32_t bin_shift;  // pair bins of (16 << bin_shift) px: 0 the reference's 16-px tiles, 1 32-px bins
 };
+
+// The tiles a splat's square touches: [x0, x1) x [y0, y1) (common.wgsl:37-42).
+__device__ inline void tile_rect(float mx, float my, float radius, uint32_t tiles_x, uint32_t tiles_y,
+                                 uint32_t rect[4]) {
+  const float size = static_cast<float>(kTile);
+  const float tx = static_cast<float>(tiles_x), ty = static_cast<float>(tiles_y);
+  rect[0] = static_cast<uint32_t>(fminf(fmaxf(floorf((mx - radius) / size), 0.f), tx));
+  rect[1] = static_cast<uint32_t>(fminf(fmaxf(floorf((my - radius) / size), 0.f), ty));
+  rect[2] = static_cast<uint32_t>(fminf(fmaxf(floorf((mx + radius) / size) + 1.f, 0.f), tx));
+  rect[3] = static_cast<uint32_t>(fminf(fmaxf(floorf((my + radius) / size) + 1.f, 0.f), ty));
+}
+
+// The bins a splat's square touches: the 16-px tile rectangle merged into bins of
+// (1 << shift) x (1 << shift) tiles ([x0, x1) x [y0, y1) in bin units); an empty
+// tile rectangle stays empty.
+__device__ inline void bin_rect(float mx, float my, float radius, uint32_t tiles_x, uint32_t tiles_y, uint32_t shift,
+                                uint32_t rect[4]) {
+  tile_rect(mx, my, radius, tiles_x, tiles_y, rect);
+  if (shift == 0) return;
+  if (rect[2] <= rect[0] || rect[3] <= rect[1]) {
+    rect[0] = rect[1] = rect[2] = rect[3] = 0;
+    return;
+  }
+  const uint32_t round = (1u << shift) - 1u;
+  rect[0] >>= shift;
+  rect[1] >>= shift;
+  rect[2] = (rect[2] + round) >> shift;
+  rect[3] = (rect[3] + round) >> shift;
+}
+
 
 // `vis` (nullable): each Gaussian's (depth, radius) as two floats, radius 0 when culled.
 void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
@@ -110,14 +141,16 @@ void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t
                 uint32_t* depth_values, float* vis = nullptr);
 
 // The tile stage from existing splats: counts (+ depth keys) only.
+// Tile counts of splats in bins of (16 << bin_shift) px, their depth keys and indices.
 void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width, uint32_t height,
-                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values);
+                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values, uint32_t bin_shift = 0);
 
 // pair_hint (0: none), here and in find_ranges: the pair count the host expects,
 // which sizes the grid; the kernels loop over whatever *pair_total holds.
 void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
                const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
-               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_hint = 0);
+               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_hint = 0,
+               uint32_t bin_shift = 0);
 
 // Each tile's [start, end) run of the sorted pairs; with `order` (nullable, one
 // word per tile) also the rasterizer's launch order, heaviest tiles first.
@@ -126,14 +159,16 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
 
 // splat_batch: pairs a wave stages per round (64, 128 or 256); `order` (nullable)
 // maps workgroups to tiles. Results depend on neither.
+// tile_size: 16 (a workgroup per 16-px tile), 8 (a wave per 8x8 quadrant, same
+// 16-px lists) or 32 (a workgroup per 32-px bin: `ranges` and `order` are the bins').
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
-               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch = 64);
+               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch = 64, uint32_t tile_size = 16);
 
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
-                        float* splat_grads, uint32_t splat_batch = 64);
+                        float* splat_grads, uint32_t splat_batch = 64, uint32_t tile_size = 16);
 
 void preprocess_backward(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                          const CameraArgs& cam, const float* splats, const float* splat_grads,

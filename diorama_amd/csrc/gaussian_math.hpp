@@ -133,20 +133,10 @@ __device__ inline void camera_position(const CameraArgs& cam, float cp[3]) {
   for (int c = 0; c < 3; c++) cp[c] = -(v[c * 4] * v[12] + v[c * 4 + 1] * v[13] + v[c * 4 + 2] * v[14]);
 }
 
-// The tiles a splat's square touches: [x0, x1) x [y0, y1) (common.wgsl:37-42).
-__device__ inline void tile_rect(float mx, float my, float radius, uint32_t tiles_x, uint32_t tiles_y,
-                                 uint32_t rect[4]) {
-  const float size = static_cast<float>(kTile);
-  const float tx = static_cast<float>(tiles_x), ty = static_cast<float>(tiles_y);
-  rect[0] = static_cast<uint32_t>(fminf(fmaxf(floorf((mx - radius) / size), 0.f), tx));
-  rect[1] = static_cast<uint32_t>(fminf(fmaxf(floorf((my - radius) / size), 0.f), ty));
-  rect[2] = static_cast<uint32_t>(fminf(fmaxf(floorf((mx + radius) / size) + 1.f, 0.f), tx));
-  rect[3] = static_cast<uint32_t>(fminf(fmaxf(floorf((my + radius) / size) + 1.f, 0.f), ty));
-}
-
 // One Gaussian's splat (preprocess.wgsl:20-87 + tiles.wgsl:23-34) from its
-// projection `p`: the three float4 of the Splat2D record, the number of 16x16
-// tiles it touches in cam's viewport and its depth-sort key. An invisible
+// projection `p`: the three float4 of the Splat2D record, the number of tiles (16
+// px, or 32-px bins with cam.bin_shift 1) it touches in cam's viewport and its
+// depth-sort key. An invisible
 // Gaussian gets a zero record, no tiles and the largest key. `coeff` holds its
 // `count` SH coefficient triples.
 struct SplatRecord {
@@ -183,7 +173,7 @@ __device__ inline void splat_record(const float g[16], const Projection& p, bool
   const uint32_t tiles_x = ceil_div(static_cast<uint32_t>(cam.viewport[0]), kTile);
   const uint32_t tiles_y = ceil_div(static_cast<uint32_t>(cam.viewport[1]), kTile);
   uint32_t r[4];
-  tile_rect(p.mean[0], p.mean[1], p.radius, tiles_x, tiles_y, r);
+  bin_rect(p.mean[0], p.mean[1], p.radius, tiles_x, tiles_y, cam.bin_shift, r);
   o.touched = (r[2] - r[0]) * (r[3] - r[1]);
   o.key = __float_as_uint(p.depth);
 }

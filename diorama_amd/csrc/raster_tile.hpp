@@ -19,6 +19,16 @@ namespace raster {
 constexpr int kChunk = kWave;
 constexpr int kGrads = 9;
 
+// Where a wavefront works: the 8x8 quadrant `quad` (0-3, row-major) of the 16-px
+// tile (sx, sy). With a 32-px bin the pair list is the union of its four 16-px
+// tiles' lists in the same depth order, and a splat counts for a tile only if its
+// 16-px tile rectangle (common.wgsl:37-42) holds the tile: what is left is exactly
+// that tile's own list, so every pixel blends the same splats in the same order as
+// with 16-px tiles. Only the pixel_last indices (positions in the list) differ.
+struct WaveTile {
+  uint32_t sx, sy, quad;
+};
+
 // One splat of a chunk, staged in a lane's registers. The conic (a, b, c) is
 // kept as the coefficients of the Gaussian's exponent in log2 units,
 // ha = -a/2 log2(e), hb = -b log2(e), hc = -c/2 log2(e), so G = exp2(power2).
@@ -26,6 +36,7 @@ struct Staged {
   float mx, my, ha, hb, hc, op, r, g, b;
   float x0, x1, y0, y1;  // box holding every pixel centre where opacity * G >= 1/255
   uint32_t index;
+  uint32_t rect[4];      // the 16-px tile rectangle [x0, x1) x [y0, y1), with kCull
 };
 
 constexpr float kLog2e = 1.4426950408889634f;
@@ -36,11 +47,14 @@ __device__ inline float splat_power2(float dx, float dy, float ha, float hb, flo
   return __builtin_fmaf(dx, __builtin_fmaf(ha, dx, hb * dy), (hc * dy) * dy);
 }
 
-// Stages pair k's splat; `conic` (optional) receives its raw (a, b, c).
+// Stages pair k's splat; `conic` (optional) receives its raw (a, b, c). With kCull
+// it also gets the splat's 16-px tile rectangle in a tiles_x x tiles_y grid.
+template <bool kCull = false>
 __device__ inline void stage(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, uint32_t k,
-                             Staged& s, float4* conic = nullptr) {
+                             Staged& s, float4* conic = nullptr, uint32_t tiles_x = 0, uint32_t tiles_y = 0) {
   s.index = pair_splats[k];
   const float4 a = splats[s.index * 3], b = splats[s.index * 3 + 1], c = splats[s.index * 3 + 2];
+  if (kCull) tile_rect(a.x, a.y, a.w, tiles_x, tiles_y, s.rect);
   s.mx = a.x; s.my = a.y;
   const float ca = b.x, cb = b.y, cc = b.z;
   if (conic) *conic = make_float4(ca, cb, cc, 0.f);
@@ -66,6 +80,10 @@ __device__ inline bool meets(const Staged& s, float bx0, float bx1, float by0, f
   return s.x1 >= bx0 && s.x0 <= bx1 && s.y1 >= by0 && s.y0 <= by1;
 }
 
+__device__ inline bool holds(const uint32_t rect[4], uint32_t sx, uint32_t sy) {
+  return sx >= rect[0] && sx < rect[2] && sy >= rect[1] && sy < rect[3];
+}
+
 // A staged splat's blend constants as LDS records: (mean x, mean y, ha, hb) and
 // (hc, opacity, r, g), with b apart. A wave reads one back with a uniform
 // address (a broadcast on the LDS pipe) instead of nine v_readlane on the VALU.
@@ -89,18 +107,21 @@ __device__ inline void wave_sync() {
 
 // rasterize.wgsl:31-100: front-to-back blend; a wave stops once all its pixels
 // are nearly opaque.
-// One 16x16 tile (tile_x, tile_y) of a width x height frame whose pair list is
-// `run`; the per-pixel outputs are indexed by the frame's own row-major pixel index.
-// A wave stages kPer x 64 pairs per round (kPer per lane); the per-pixel order and
-// every result are the same for any kPer.
-template <int kPer>
-__device__ inline void rasterize_tile(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
-                                      uint2 run, uint32_t tile_x, uint32_t tile_y, uint32_t width, uint32_t height,
-                                      float bg_r, float bg_g, float bg_b, uint32_t* __restrict__ target,
-                                      float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const uint32_t qx = tile_x * kTile + (wave & 1) * 8, qy = tile_y * kTile + (wave >> 1) * 8;
+// One wave's quadrant `wt` of a width x height frame whose pair list is `run`; the
+// per-pixel outputs are indexed by the frame's own row-major pixel index. A wave
+// stages kPer x 64 pairs per round (kPer per lane) into its own `records`; the
+// per-pixel order and every result are the same for any kPer. kCull: `run` is a
+// 32-px bin's list (see WaveTile).
+template <int kPer, bool kCull>
+__device__ inline void rasterize_wave(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
+                                      uint2 run, WaveTile wt, uint32_t width, uint32_t height, float bg_r,
+                                      float bg_g, float bg_b, uint32_t* __restrict__ target,
+                                      float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last,
+                                      SplatRecords (&records)[kPer]) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint32_t qx = wt.sx * kTile + (wt.quad & 1) * 8, qy = wt.sy * kTile + (wt.quad >> 1) * 8;
   const uint32_t px = qx + (lane & 7), py = qy + (lane >> 3);
+  const uint32_t tiles_x = ceil_div(width, kTile), tiles_y = ceil_div(height, kTile);
   const bool inside = px < width && py < height;
   const float cx = static_cast<float>(px) + 0.5f, cy = static_cast<float>(py) + 0.5f;
   const float bx0 = static_cast<float>(qx) + 0.5f, bx1 = bx0 + 7.f;
@@ -108,7 +129,6 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
   bool done = !inside;
   uint32_t last = 0;
-  __shared__ SplatRecords s_records[4][kPer];  // one slot per wave: waves never share records
   for (uint32_t round = run.x; round < run.y; round += kPer * kChunk) {
     if (__all(done)) break;
     bool hits[kPer];
@@ -118,15 +138,15 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
       const uint32_t k = round + q * kChunk + lane;
       Staged st{};
       if (k < run.y) {
-        stage(splats, pair_splats, k, st);
-        hits[q] = meets(st, bx0, bx1, by0, by1);
+        stage<kCull>(splats, pair_splats, k, st, nullptr, tiles_x, tiles_y);
+        hits[q] = meets(st, bx0, bx1, by0, by1) && (!kCull || holds(st.rect, wt.sx, wt.sy));
       }
-      put_record(s_records[wave][q], lane, st);
+      put_record(records[q], lane, st);
     }
     wave_sync();
 #pragma unroll
     for (int q = 0; q < kPer; q++) {
-      const SplatRecords& rec = s_records[wave][q];
+      const SplatRecords& rec = records[q];
       const uint32_t base = round + q * kChunk;
       uint64_t mask = __ballot(hits[q]);
       // Two splats per iteration: both alphas first (they do not depend on T), then
@@ -178,6 +198,18 @@ __device__ inline void rasterize_tile(const float4* __restrict__ splats, const u
       target[index] = u8(r) | (u8(g) << 8) | (u8(b) << 16) | (255u << 24);
     }
   }
+}
+
+// One 16x16 tile (tile_x, tile_y) by a 256-thread workgroup: wave w takes quadrant w.
+template <int kPer>
+__device__ inline void rasterize_tile(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
+                                      uint2 run, uint32_t tile_x, uint32_t tile_y, uint32_t width, uint32_t height,
+                                      float bg_r, float bg_g, float bg_b, uint32_t* __restrict__ target,
+                                      float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last) {
+  __shared__ SplatRecords s_records[4][kPer];  // one slot per wave: waves never share records
+  const uint32_t wave = threadIdx.x / kWave;
+  rasterize_wave<kPer, false>(splats, pair_splats, run, WaveTile{tile_x, tile_y, wave}, width, height, bg_r, bg_g,
+                              bg_b, target, pixel_color, pixel_last, s_records[wave]);
 }
 
 template <int kCtrl>
@@ -298,37 +330,43 @@ __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy
 // rasterizeBackward.wgsl:49-163: each pixel replays its list back to front from
 // its last blended splat. A wave takes the splats that touch its quadrant four at
 // a time, reduces their 9 gradients across its 64 pixels together, and its row
-// leaders park the totals in LDS; per 64-splat chunk the four quadrant waves add
-// theirs and one float atomic per (splat, tile, gradient) goes out.
-// The workgroup's LDS for one backward tile.
-// The workgroup's LDS for one backward tile with kPer x 64 pairs per round.
-template <int kPer>
+// leaders park the totals in LDS; per 64-splat chunk the workgroup's waves add
+// theirs and one float atomic per (splat, list, gradient) goes out.
+// The workgroup's LDS for one backward list with kPer x 64 pairs per round and
+// kWaves waves (kCull: each staged splat's 16-px tile rectangle too).
+template <int kPer, int kWaves = 4, bool kCull = false>
 struct BackwardShared {
-  float sums[4][kGrads][kPer * kChunk];
+  float sums[kWaves][kGrads][kPer * kChunk];
   float4 conic[kPer * kChunk];
   float4 box[kPer * kChunk];   // each splat's alpha >= 1/255 box (x0, x1, y0, y1)
-  SplatRecords records[kPer];  // the round's splats, staged once by wave 0 for all four
+  uint4 rect[kCull ? kPer * kChunk : 1];
+  SplatRecords records[kPer];  // the round's splats, staged once for all waves
   uint32_t index[kPer * kChunk];
-  uint32_t last[4];
+  uint32_t last[kWaves];
 };
 
-// One 16x16 tile of the backward pass, kPer x 64 pairs per round; the workgroup
-// may call it again for another tile once it returns (it ends on a barrier).
-template <int kPer>
-__device__ inline void rasterize_backward_tile(const float4* __restrict__ splats,
-                                               const uint32_t* __restrict__ pair_splats, uint2 run, uint32_t tile_x,
-                                               uint32_t tile_y, uint32_t width, uint32_t height, float bg_r,
-                                               float bg_g, float bg_b, const float4* __restrict__ pixel_color,
-                                               const uint32_t* __restrict__ pixel_last,
-                                               const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads,
-                                               BackwardShared<kPer>& sh) {
+// One pair list of the backward pass by a workgroup of kWaves waves, wave w on its
+// quadrant `wt` (kCull: `run` is a 32-px bin's list, see WaveTile), kPer x 64 pairs
+// per round; the workgroup may call it again for another list once it returns (it
+// ends on a barrier).
+template <int kPer, int kWaves, bool kCull>
+__device__ inline void rasterize_backward_group(const float4* __restrict__ splats,
+                                                const uint32_t* __restrict__ pair_splats, uint2 run, WaveTile wt,
+                                                uint32_t width, uint32_t height, float bg_r, float bg_g, float bg_b,
+                                                const float4* __restrict__ pixel_color,
+                                                const uint32_t* __restrict__ pixel_last,
+                                                const float4* __restrict__ pixel_grads,
+                                                float* __restrict__ splat_grads,
+                                                BackwardShared<kPer, kWaves, kCull>& sh) {
   constexpr int kRound = kPer * kChunk;
+  constexpr int kGroupThreads = kWaves * kWave;
   auto& s_sums = sh.sums;
   auto& s_index = sh.index;
   auto& s_last = sh.last;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t qx = tile_x * kTile + (wave & 1) * 8, qy = tile_y * kTile + (wave >> 1) * 8;
+  const uint32_t qx = wt.sx * kTile + (wt.quad & 1) * 8, qy = wt.sy * kTile + (wt.quad >> 1) * 8;
   const uint32_t px = qx + (lane & 7), py = qy + (lane >> 3);
+  const uint32_t tiles_x = ceil_div(width, kTile), tiles_y = ceil_div(height, kTile);
   const bool inside = px < width && py < height;
   const float cx = static_cast<float>(px) + 0.5f, cy = static_cast<float>(py) + 0.5f;
   const float bx0 = static_cast<float>(qx) + 0.5f, bx1 = bx0 + 7.f;
@@ -348,37 +386,41 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
   const uint32_t my_last = wave_max(last);
   if (lane == 0) s_last[wave] = my_last;
   __syncthreads();
-  const uint32_t tile_last = max(max(s_last[0], s_last[1]), max(s_last[2], s_last[3]));
+  uint32_t tile_last = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) tile_last = max(tile_last, s_last[w]);
   const int row = lane >> 4;
   const bool row_leader = (lane & 15) == 0;
 
   for (int c = static_cast<int>(ceil_div(tile_last, kRound)) - 1; c >= 0; c--) {
     const uint32_t round = run.x + static_cast<uint32_t>(c) * kRound;
-    // Wave 0 stages the round's splats for all four waves.
-    if (wave == 0) {
-#pragma unroll
-      for (int q = 0; q < kPer; q++) {
-        const uint32_t k = round + q * kChunk + lane;
-        Staged st{};
-        float4 conic = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (k < run.y) stage(splats, pair_splats, k, st, &conic);
-        s_index[q * kChunk + lane] = st.index;
-        sh.conic[q * kChunk + lane] = conic;
-        sh.box[q * kChunk + lane] = make_float4(st.x0, st.x1, st.y0, st.y1);
-        put_record(sh.records[q], lane, st);
-      }
+    // The round's splats are staged once for every wave, 64 per staging wave.
+    for (int q = wave; q < kPer; q += kWaves) {
+      const uint32_t k = round + q * kChunk + lane;
+      Staged st{};
+      float4 conic = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (k < run.y) stage<kCull>(splats, pair_splats, k, st, &conic, tiles_x, tiles_y);
+      s_index[q * kChunk + lane] = st.index;
+      sh.conic[q * kChunk + lane] = conic;
+      sh.box[q * kChunk + lane] = make_float4(st.x0, st.x1, st.y0, st.y1);
+      if (kCull) sh.rect[q * kChunk + lane] = make_uint4(st.rect[0], st.rect[1], st.rect[2], st.rect[3]);
+      put_record(sh.records[q], lane, st);
     }
 #pragma unroll
     for (int q = 0; q < kPer; q++)
 #pragma unroll
       for (int v = 0; v < kGrads; v++) s_sums[wave][v][q * kChunk + lane] = 0.f;
-    __syncthreads();  // wave 0's records are read by every wave
+    __syncthreads();  // the staged records are read by every wave
     bool hits[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; q++) {
       const uint32_t k = round + q * kChunk + lane;
       const float4 b = sh.box[q * kChunk + lane];
       hits[q] = k < run.y && k - run.x < my_last && b.y >= bx0 && b.x <= bx1 && b.w >= by0 && b.z <= by1;
+      if (kCull) {
+        const uint4 r = sh.rect[q * kChunk + lane];
+        hits[q] = hits[q] && wt.sx >= r.x && wt.sx < r.z && wt.sy >= r.y && wt.sy < r.w;
+      }
     }
     // Back to front: the last 64 of the round first.
 #pragma unroll
@@ -418,13 +460,18 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
       }
     }
     __syncthreads();
-    // Per (slot, gradient pair): the four quadrants' sums, then the splat's
-    // constants, each of the first five sums times the opacity (g = o dL/dopacity):
-    // mean2 from the (g dx, g dy) sums through the conic, the conic
-    // terms scaled by -1/2, -1, -1/2; the rest as summed.
-    for (int e = tid; e < 5 * kRound; e += 256) {
+    // Per (slot, gradient pair): the waves' sums, then the splat's constants, each of
+    // the first five sums times the opacity (g = o dL/dopacity): mean2 from the
+    // (g dx, g dy) sums through the conic, the conic terms scaled by -1/2, -1, -1/2;
+    // the rest as summed.
+    for (int e = tid; e < 5 * kRound; e += kGroupThreads) {
       const int part = e / kRound, slot = e % kRound;
-      auto sum = [&](int v) { return s_sums[0][v][slot] + s_sums[1][v][slot] + s_sums[2][v][slot] + s_sums[3][v][slot]; };
+      auto sum = [&](int v) {
+        float total = 0.f;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) total += s_sums[w][v][slot];
+        return total;
+      };
       const float op = sh.records[slot / kChunk].c[slot % kChunk].y;
       float* out = splat_grads + static_cast<size_t>(s_index[slot]) * kGrads;
       float t0, t1;
@@ -453,7 +500,23 @@ __device__ inline void rasterize_backward_tile(const float4* __restrict__ splats
     }
     __syncthreads();
   }
-  __syncthreads();  // s_last is rewritten by the next tile
+  __syncthreads();  // s_last is rewritten by the next list
+}
+
+// One 16x16 tile of the backward pass by a 256-thread workgroup (wave w on
+// quadrant w), kPer x 64 pairs per round.
+template <int kPer>
+__device__ inline void rasterize_backward_tile(const float4* __restrict__ splats,
+                                               const uint32_t* __restrict__ pair_splats, uint2 run, uint32_t tile_x,
+                                               uint32_t tile_y, uint32_t width, uint32_t height, float bg_r,
+                                               float bg_g, float bg_b, const float4* __restrict__ pixel_color,
+                                               const uint32_t* __restrict__ pixel_last,
+                                               const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads,
+                                               BackwardShared<kPer>& sh) {
+  rasterize_backward_group<kPer, 4, false>(splats, pair_splats, run,
+                                           WaveTile{tile_x, tile_y, static_cast<uint32_t>(threadIdx.x / kWave)},
+                                           width, height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads,
+                                           splat_grads, sh);
 }
 
 }  // namespace raster

@@ -68,9 +68,10 @@ __global__ __launch_bounds__(kThreads) void preprocess_kernel(const float4* __re
   if (vis) vis[i] = make_float2(rec.s0.z, rec.s0.w);
 }
 
-// count_tiles (tiles.wgsl:23-34) for splats that did not come from the preprocess.
+// count_tiles (tiles.wgsl:23-34) for splats that did not come from the preprocess,
+// in bins of (1 << shift) x (1 << shift) 16-px tiles.
 __global__ __launch_bounds__(kThreads) void count_tiles_kernel(const float4* __restrict__ splats, uint32_t n,
-                                                                uint32_t tiles_x, uint32_t tiles_y,
+                                                                uint32_t tiles_x, uint32_t tiles_y, uint32_t shift,
                                                                 uint32_t* __restrict__ tile_counts,
                                                                 uint32_t* __restrict__ depth_keys,
                                                                 uint32_t* __restrict__ depth_values) {
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(kThreads) void count_tiles_kernel(const float4* __r
   uint32_t touched = 0, key = 0xffffffffu;
   if (s0.w > 0.f) {
     uint32_t r[4];
-    tile_rect(s0.x, s0.y, s0.w, tiles_x, tiles_y, r);
+    bin_rect(s0.x, s0.y, s0.w, tiles_x, tiles_y, shift, r);
     touched = (r[2] - r[0]) * (r[3] - r[1]);
     key = __float_as_uint(s0.z);
   }
@@ -99,13 +100,14 @@ __global__ __launch_bounds__(kThreads) void count_tiles_kernel(const float4* __r
 constexpr uint32_t kDupPairs = 2048;
 
 // kBatched: splat s belongs to view s / per_view of `views`, whose tile grid and
-// tile_base replace tiles_x, tiles_y and 0.
+// tile_base replace tiles_x, tiles_y and 0. Pairs go to bins of (1 << shift) x
+// (1 << shift) 16-px tiles (tiles_x, tiles_y count 16-px tiles).
 template <bool kBatched>
 __global__ __launch_bounds__(kThreads) void duplicate_kernel(const float4* __restrict__ splats,
                                                               const uint32_t* __restrict__ order,
                                                               const uint32_t* __restrict__ offsets,
                                                               const uint32_t* __restrict__ tile_counts, uint32_t n,
-                                                              uint32_t tiles_x, uint32_t tiles_y,
+                                                              uint32_t tiles_x, uint32_t tiles_y, uint32_t shift,
                                                               const ViewDesc* __restrict__ views, uint32_t per_view,
                                                               uint32_t capacity,
                                                               const uint32_t* __restrict__ pair_total,
@@ -149,7 +151,8 @@ __global__ __launch_bounds__(kThreads) void duplicate_kernel(const float4* __res
             base = view.tile_base;
           }
           const float4 s0 = splats[static_cast<size_t>(splat) * 3];
-          tile_rect(s0.x, s0.y, s0.w, row_tiles, grid_y, r);
+          bin_rect(s0.x, s0.y, s0.w, row_tiles, grid_y, shift, r);
+          row_tiles = (row_tiles + (1u << shift) - 1u) >> shift;
         }
       }
       s_start[tid] = start;
@@ -256,9 +259,17 @@ __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint2* 
 }
 
 // -------------------------------------------------------------- rasterize --
+// Three workgroup shapes, one per tile size; results do not depend on the shape
+// (only pixel_last's list positions do, with 32-px bins):
+// * 16: a 256-thread workgroup per 16-px tile, wave w on quadrant w;
+// * 8: a 64-thread workgroup (one wave) per 8x8 quadrant of the 16-px lists. An
+//   8-px tile's list culled by the exact 16-px rectangle is its 16-px tile's list,
+//   so the quadrant reads that list instead of a copy of it;
+// * 32: a 1024-thread workgroup per 32-px bin, wave w on quadrant w % 4 of the
+//   bin's 16-px tile w / 4, culling the bin's list to that tile's.
+// `order`, if given, maps workgroups to tiles or bins (heaviest first).
 
-// rasterize.wgsl:31-100: front-to-back blend, one workgroup per tile (`order`, if
-// given, maps workgroups to tiles).
+// rasterize.wgsl:31-100: front-to-back blend.
 template <int kPer>
 __global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict__ splats,
                                                          const uint32_t* __restrict__ pair_splats,
@@ -273,7 +284,41 @@ __global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict
                                bg_g, bg_b, target, pixel_color, pixel_last);
 }
 
-// rasterizeBackward.wgsl:49-163, one workgroup per tile (`order` as above).
+template <int kPer>
+__global__ __launch_bounds__(64) void rasterize_quad_kernel(const float4* __restrict__ splats,
+                                                            const uint32_t* __restrict__ pair_splats,
+                                                            const uint2* __restrict__ ranges,
+                                                            const uint32_t* __restrict__ order, uint32_t width,
+                                                            uint32_t height, uint32_t tiles_x, float bg_r,
+                                                            float bg_g, float bg_b, uint32_t* __restrict__ target,
+                                                            float4* __restrict__ pixel_color,
+                                                            uint32_t* __restrict__ pixel_last) {
+  __shared__ raster::SplatRecords s_records[kPer];
+  const uint32_t tile = order ? order[blockIdx.x >> 2] : blockIdx.x >> 2;
+  raster::rasterize_wave<kPer, false>(splats, pair_splats, ranges[tile],
+                                      raster::WaveTile{tile % tiles_x, tile / tiles_x, blockIdx.x & 3u}, width, height,
+                                      bg_r, bg_g, bg_b, target, pixel_color, pixel_last, s_records);
+}
+
+template <int kPer>
+__global__ __launch_bounds__(1024) void rasterize_bin_kernel(const float4* __restrict__ splats,
+                                                             const uint32_t* __restrict__ pair_splats,
+                                                             const uint2* __restrict__ ranges,
+                                                             const uint32_t* __restrict__ order, uint32_t width,
+                                                             uint32_t height, uint32_t bins_x, float bg_r,
+                                                             float bg_g, float bg_b, uint32_t* __restrict__ target,
+                                                             float4* __restrict__ pixel_color,
+                                                             uint32_t* __restrict__ pixel_last) {
+  __shared__ raster::SplatRecords s_records[16][kPer];
+  const uint32_t bin = order ? order[blockIdx.x] : blockIdx.x;
+  const uint32_t wave = threadIdx.x / kWave, sub = wave >> 2;
+  const uint32_t sx = (bin % bins_x) * 2 + (sub & 1), sy = (bin / bins_x) * 2 + (sub >> 1);
+  if (sx * kTile >= width || sy * kTile >= height) return;  // past the frame's edge (no barriers follow)
+  raster::rasterize_wave<kPer, true>(splats, pair_splats, ranges[bin], raster::WaveTile{sx, sy, wave & 3u}, width,
+                                     height, bg_r, bg_g, bg_b, target, pixel_color, pixel_last, s_records[wave]);
+}
+
+// rasterizeBackward.wgsl:49-163.
 template <int kPer>
 __global__ __launch_bounds__(256) void rasterize_backward_kernel(
     const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
@@ -285,6 +330,36 @@ __global__ __launch_bounds__(256) void rasterize_backward_kernel(
   raster::rasterize_backward_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width,
                                         height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads,
                                         shared);
+}
+
+template <int kPer>
+__global__ __launch_bounds__(64) void rasterize_backward_quad_kernel(
+    const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
+    const uint32_t* __restrict__ order, uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
+    float bg_b, const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
+    const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads) {
+  __shared__ raster::BackwardShared<kPer, 1, false> shared;
+  const uint32_t tile = order ? order[blockIdx.x >> 2] : blockIdx.x >> 2;
+  raster::rasterize_backward_group<kPer, 1, false>(
+      splats, pair_splats, ranges[tile], raster::WaveTile{tile % tiles_x, tile / tiles_x, blockIdx.x & 3u}, width,
+      height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads, shared);
+}
+
+// Waves whose tile lies past the frame's edge have no pixels and no hits, but join
+// the workgroup's barriers.
+template <int kPer>
+__global__ __launch_bounds__(1024) void rasterize_backward_bin_kernel(
+    const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
+    const uint32_t* __restrict__ order, uint32_t width, uint32_t height, uint32_t bins_x, float bg_r, float bg_g,
+    float bg_b, const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
+    const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads) {
+  __shared__ raster::BackwardShared<kPer, 16, true> shared;
+  const uint32_t bin = order ? order[blockIdx.x] : blockIdx.x;
+  const uint32_t wave = threadIdx.x / kWave, sub = wave >> 2;
+  const uint32_t sx = (bin % bins_x) * 2 + (sub & 1), sy = (bin / bins_x) * 2 + (sub >> 1);
+  raster::rasterize_backward_group<kPer, 16, true>(splats, pair_splats, ranges[bin],
+                                                   raster::WaveTile{sx, sy, wave & 3u}, width, height, bg_r, bg_g,
+                                                   bg_b, pixel_color, pixel_last, pixel_grads, splat_grads, shared);
 }
 
 // ----------------------------------------------------- preprocess backward --
@@ -381,11 +456,11 @@ void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t
 }
 
 void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width, uint32_t height,
-                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values) {
+                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values, uint32_t bin_shift) {
   if (n == 0) return;
   count_tiles_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), n,
                                                                  ceil_div(width, kTile), ceil_div(height, kTile),
-                                                                 tile_counts, depth_keys, depth_values);
+                                                                 bin_shift, tile_counts, depth_keys, depth_values);
   DIO_LAUNCHED("count_tiles");
 }
 
@@ -398,11 +473,12 @@ static uint32_t hinted_groups(uint32_t capacity, uint32_t pair_hint, uint32_t pe
 
 void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
                const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
-               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_hint) {
+               const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_hint,
+               uint32_t bin_shift) {
   if (n == 0 || capacity == 0) return;
   duplicate_kernel<false><<<hinted_groups(capacity, pair_hint, kDupPairs), kThreads, 0, s>>>(
       reinterpret_cast<const float4*>(splats), order, offsets, tile_counts, n, ceil_div(width, kTile),
-      ceil_div(height, kTile), nullptr, 0, capacity, pair_total, pair_tiles, pair_splats);
+      ceil_div(height, kTile), bin_shift, nullptr, 0, capacity, pair_total, pair_tiles, pair_splats);
   DIO_LAUNCHED("duplicate");
 }
 
@@ -412,7 +488,7 @@ void duplicate_batch(hipStream_t s, const float* splats, const uint32_t* order, 
   const uint32_t n = b.view_count * b.per_view;
   if (n == 0 || capacity == 0) return;
   duplicate_kernel<true><<<ceil_div(capacity, kDupPairs), kThreads, 0, s>>>(
-      reinterpret_cast<const float4*>(splats), order, offsets, tile_counts, n, 0, 0, b.views, b.per_view, capacity,
+      reinterpret_cast<const float4*>(splats), order, offsets, tile_counts, n, 0, 0, 0, b.views, b.per_view, capacity,
       pair_total, pair_tiles, pair_splats);
   DIO_LAUNCHED("duplicate_batch");
 }
@@ -431,27 +507,35 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
 
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
-               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch) {
+               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch, uint32_t tile_size) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
-  auto launch = [&](auto kernel) {
-    kernel<<<tx * ty, kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
-                                        reinterpret_cast<const uint2*>(ranges), order, width, height, tx, background[0],
-                                        background[1], background[2], target, reinterpret_cast<float4*>(pixel_color),
-                                        pixel_last);
+  const uint32_t bx = ceil_div(tx, 2u), by = ceil_div(ty, 2u);
+  auto launch = [&](auto kernel, uint32_t groups, uint32_t threads, uint32_t row) {
+    kernel<<<groups, threads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                      reinterpret_cast<const uint2*>(ranges), order, width, height, row, background[0],
+                                      background[1], background[2], target, reinterpret_cast<float4*>(pixel_color),
+                                      pixel_last);
   };
-  switch (splat_batch) {
-    case 64: launch(rasterize_kernel<1>); break;
-    case 128: launch(rasterize_kernel<2>); break;
-    case 256: launch(rasterize_kernel<4>); break;
-    default: throw Error(-1, "splat batch must be 64, 128 or 256");
+  const uint32_t key = tile_size * 1000 + splat_batch;
+  switch (key) {
+    case 16064: launch(rasterize_kernel<1>, tx * ty, 256, tx); break;
+    case 16128: launch(rasterize_kernel<2>, tx * ty, 256, tx); break;
+    case 16256: launch(rasterize_kernel<4>, tx * ty, 256, tx); break;
+    case 8064: launch(rasterize_quad_kernel<1>, 4 * tx * ty, 64, tx); break;
+    case 8128: launch(rasterize_quad_kernel<2>, 4 * tx * ty, 64, tx); break;
+    case 8256: launch(rasterize_quad_kernel<4>, 4 * tx * ty, 64, tx); break;
+    case 32064: launch(rasterize_bin_kernel<1>, bx * by, 1024, bx); break;
+    case 32128: launch(rasterize_bin_kernel<2>, bx * by, 1024, bx); break;
+    default: throw Error(-1, "tile size 8, 16 or 32 with splat batch 64, 128 or 256 (32: 64 or 128)");
   }
   DIO_LAUNCHED("rasterize");
 }
 
-// The backward runs fastest with at most 5 workgroups (20 waves) per CU: measured at
-// 64 splats a round, 7 took 371 µs, 6 369 µs and 5 365.5 µs (its 72 VGPRs would let 7
-// in). Dynamic LDS pads a workgroup to just over a sixth of the CU's 160 KB.
+// The 16-px backward runs fastest with at most 5 workgroups (20 waves) per CU:
+// measured at 64 splats a round, 7 took 371 µs, 6 369 µs and 5 365.5 µs (its 72
+// VGPRs would let 7 in). Dynamic LDS pads a workgroup to just over a sixth of the
+// CU's 160 KB.
 constexpr uint32_t kBackwardGroupsPerCu = 5;
 
 template <int kPer>
@@ -463,21 +547,28 @@ constexpr size_t backward_lds_pad() {
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
-                        float* splat_grads, uint32_t splat_batch) {
+                        float* splat_grads, uint32_t splat_batch, uint32_t tile_size) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
-  auto launch = [&](auto kernel, size_t pad) {
-    kernel<<<tx * ty, kThreads, pad, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
-                                          reinterpret_cast<const uint2*>(ranges), order, width, height, tx,
-                                          background[0], background[1], background[2],
-                                          reinterpret_cast<const float4*>(pixel_color), pixel_last,
-                                          reinterpret_cast<const float4*>(pixel_grads), splat_grads);
+  const uint32_t bx = ceil_div(tx, 2u), by = ceil_div(ty, 2u);
+  auto launch = [&](auto kernel, uint32_t groups, uint32_t threads, uint32_t row, size_t pad) {
+    kernel<<<groups, threads, pad, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                        reinterpret_cast<const uint2*>(ranges), order, width, height, row,
+                                        background[0], background[1], background[2],
+                                        reinterpret_cast<const float4*>(pixel_color), pixel_last,
+                                        reinterpret_cast<const float4*>(pixel_grads), splat_grads);
   };
-  switch (splat_batch) {
-    case 64: launch(rasterize_backward_kernel<1>, backward_lds_pad<1>()); break;
-    case 128: launch(rasterize_backward_kernel<2>, backward_lds_pad<2>()); break;
-    case 256: launch(rasterize_backward_kernel<4>, backward_lds_pad<4>()); break;
-    default: throw Error(-1, "splat batch must be 64, 128 or 256");
+  const uint32_t key = tile_size * 1000 + splat_batch;
+  switch (key) {
+    case 16064: launch(rasterize_backward_kernel<1>, tx * ty, 256, tx, backward_lds_pad<1>()); break;
+    case 16128: launch(rasterize_backward_kernel<2>, tx * ty, 256, tx, backward_lds_pad<2>()); break;
+    case 16256: launch(rasterize_backward_kernel<4>, tx * ty, 256, tx, backward_lds_pad<4>()); break;
+    case 8064: launch(rasterize_backward_quad_kernel<1>, 4 * tx * ty, 64, tx, 0); break;
+    case 8128: launch(rasterize_backward_quad_kernel<2>, 4 * tx * ty, 64, tx, 0); break;
+    case 8256: launch(rasterize_backward_quad_kernel<4>, 4 * tx * ty, 64, tx, 0); break;
+    case 32064: launch(rasterize_backward_bin_kernel<1>, bx * by, 1024, bx, 0); break;
+    case 32128: launch(rasterize_backward_bin_kernel<2>, bx * by, 1024, bx, 0); break;
+    default: throw Error(-1, "tile size 8, 16 or 32 with splat batch 64, 128 or 256 (32: 64 or 128)");
   }
   DIO_LAUNCHED("rasterize_backward");
 }

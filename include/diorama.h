@@ -52,7 +52,13 @@ typedef struct dio_trainer dio_trainer;
 /* CameraData (src/gaussianRenderer.ts:11-15). */
 /* Launch settings that change speed, never results (see dio_trainer_autotune). */
 typedef struct dio_tuning {
-  uint32_t tile_size;   /* pixels per tile side: 16 (the reference's, src/tileRasterizer.ts) */
+  /* pixels per tile side of the rasterizer: 16 (the reference's tiling,
+   * src/tileRasterizer.ts:8), 8 (a workgroup per 8x8 quadrant of the 16-px lists) or
+   * 32 (pairs binned per 32-px bin, each 16-px tile culling the bin's list by its
+   * exact 16-px rectangle; splat batch 64 or 128). Images and gradients are the same
+   * for all three; with 32, pixelLast counts positions in the bin's list. Batch
+   * trainers take 16. */
+  uint32_t tile_size;
   uint32_t splat_batch; /* pairs a wave stages per round: 64, 128 or 256 */
   uint32_t digit_bits;  /* radix digit width of the binning sorts: 4 or 8 */
 } dio_tuning;
@@ -273,8 +279,8 @@ void dio_trainer_destroy(dio_trainer* t);
  * callers that tune a trainer to their GPU and views. */
 int dio_trainer_set_tuning(dio_trainer* t, const dio_tuning* tuning);
 int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out);
-/* The autotuner: for every candidate (tile size 16; splat batch 64, 128, 256;
- * digit width 4, 8) run one untimed optimize() step, then time `steps` more
+/* The autotuner: for every candidate (tile size 16, 8, 32; splat batch 64, 128,
+ * 256; digit width 4, 8; see dio_tuning) run one untimed optimize() step, then time `steps` more
  * (ordinary training steps; a batch trainer replays its step graph on the same
  * views), keep the fastest setting, and list up to `capacity` results. */
 int dio_trainer_autotune(dio_trainer* t, uint32_t steps, dio_tuning_result* results, uint32_t capacity,

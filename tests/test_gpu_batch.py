@@ -163,38 +163,48 @@ def test_batch_training_learns():
 
 @pytest.mark.parametrize("batch", [0, 3])
 def test_tuning_changes_speed_not_results(batch):
-    """Every splat batch and digit width gives the same images, and gradients equal up
-    to the order of float atomics; the autotuner keeps the fastest setting."""
+    """Every tile size (8, 16, 32; the plain trainer), splat batch and digit width gives
+    the same images, and gradients equal up to the order of float atomics; the
+    autotuner keeps the fastest setting. 200x120 has 13 x 8 tiles: the 32-px bins
+    of the last column and row are partly outside the frame."""
     ctx = context()
     n = 4096
     cams = orbit_cameras(SIZES) if batch else orbit_cameras([(200, 120)] * 3)
     images = teacher_images(ctx, n, cams, seed=21)
     params, sh = scene.random_raw_gaussians(n, 22, box=1.0)
     outs = {}
-    for bits in (8, 4):
-        for splats in (64, 128, 256):
-            t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(1.0), batch=batch)
-            t.sh_degree = 3
-            t.tuning = {"tile_size": 16, "splat_batch": splats, "digit_bits": bits}
-            say("tuning", batch, bits, splats)
-            t.optimize()
-            p = t.buffers["params"].download(np.float32).reshape(n, 16)
-            pix = None if batch else t.renderer.pixel_buffer.download(np.float32, 200 * 120 * 4)
-            outs[(bits, splats)] = (p, pix, t.last_loss())
-            t.destroy()
-    ref_p, ref_pix, ref_loss = outs[(8, 64)]
-    for key, (p, pix, loss) in outs.items():
+    for tile in ((16,) if batch else (16, 8, 32)):
+        for bits in (8, 4):
+            for splats in (64, 128, 256):
+                if tile == 32 and splats == 256:
+                    continue
+                t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(1.0), batch=batch)
+                t.sh_degree = 3
+                t.tuning = {"tile_size": tile, "splat_batch": splats, "digit_bits": bits}
+                say("tuning", batch, tile, bits, splats)
+                t.optimize()
+                p = t.buffers["params"].download(np.float32).reshape(n, 16)
+                pix = None if batch else t.renderer.pixel_buffer.download(np.float32, 200 * 120 * 4)
+                sg = None if batch else t.renderer.gradients["splats"].download(np.float32).reshape(-1, 9)[:n]
+                outs[(tile, bits, splats)] = (p, pix, t.last_loss(), sg)
+                t.destroy()
+    ref_p, ref_pix, ref_loss, ref_sg = outs[(16, 8, 64)]
+    for key, (p, pix, loss, sg) in outs.items():
         if pix is not None:
             assert np.array_equal(pix, ref_pix), key
         assert abs(loss - ref_loss) < 1e-6, key
         assert np.abs(p - ref_p).max() < 1e-5, key
+        if not batch:
+            assert np.abs(sg - ref_sg).max() <= 1e-5 * np.abs(ref_sg).max(), key
     t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(1.0), batch=batch)
     say("autotune", batch)
     results = t.autotune(steps=2)
-    assert len(results) == 6
+    assert len(results) == (6 if batch else 16)
     best = min(results, key=lambda r: r[1])
     assert t.tuning == best[0]
-    print("autotune:", [(r[0]["splat_batch"], r[0]["digit_bits"], round(r[1], 3)) for r in results])
+    print("autotune:", [(r[0]["tile_size"], r[0]["splat_batch"], r[0]["digit_bits"], round(r[1], 3)) for r in results])
     with pytest.raises(diorama_amd.DioramaError):
-        t.tuning = {"tile_size": 8, "splat_batch": 64, "digit_bits": 8}
+        t.tuning = {"tile_size": 8 if batch else 12, "splat_batch": 64, "digit_bits": 8}
+    with pytest.raises(diorama_amd.DioramaError):
+        t.tuning = {"tile_size": 32, "splat_batch": 256, "digit_bits": 8}
     t.destroy()
