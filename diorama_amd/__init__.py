@@ -161,6 +161,7 @@ SIGNATURES = {
     "dio_trainer_samples": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_U32)]),
     "dio_trainer_features": (_I, [_P, ctypes.POINTER(_P)]),
     "dio_trainer_extra_stats": (_P, [_P]),
+    "dio_trainer_footprint": (_P, [_P]),
     "dio_trainer_batch_frames": (_I, [_P, ctypes.POINTER(_U32), _U32, ctypes.POINTER(_U32), ctypes.POINTER(_P),
                                      ctypes.POINTER(_U32), ctypes.POINTER(_P)]),
     "dio_trainer_step": (_I, [_P]),
@@ -721,6 +722,11 @@ class Trainer:
     @property
     def extra_stats(self):
         return _view(lib().dio_trainer_extra_stats(self.handle), self)
+
+    @property
+    def footprint(self):
+        """Mode "log": per Gaussian (sum of blend weight x loss share, sum of blend weight)."""
+        return _view(lib().dio_trainer_footprint(self.handle), self)
 
     def batch_frames(self, cameras):
         """A batch trainer's last step, downloaded: per slot (training view index,

@@ -650,7 +650,7 @@ void tiles_after_submit(dio_tile_rasterizer* t, bool wait) {
 // encodeBackward (src/tileRasterizer.ts:200-222) of the frame encoded last. The pass
 // adds into splat_grads, which is cleared first unless the caller already did.
 void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pixel_grads, dio_buf* splat_grads,
-                    bool cleared = false) {
+                    bool cleared = false, dio_buf* pixel_loss = nullptr, dio_buf* footprint = nullptr) {
   hipStream_t s = t->ctx->stream;
   if (!cleared) dio::fill_u32(s, dev<uint32_t>(splat_grads), 0u, static_cast<size_t>(t->count) * 9);
   if (t->clock) t->clock->mark(s, kMarkClearGrads);
@@ -658,7 +658,7 @@ void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pix
   dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges),
                           dev<uint32_t>(t->tile_order), w, h, t->frame_bg, dev<float>(t->pixel_color),
                           dev<uint32_t>(t->pixel_last), dev<float>(pixel_grads), dev<float>(splat_grads),
-                          t->splat_batch, t->tile_size);
+                          t->splat_batch, t->tile_size, dev<float>(pixel_loss), dev<float>(footprint));
   if (t->clock) t->clock->mark(s, kMarkBackward);
 }
 
@@ -803,18 +803,21 @@ void ensure_grad_buffers(dio_renderer* r) {
 
 // The rasterizer's half of encodeBackward: per-splat gradients into grad_splats
 // (already zeroed when `cleared`).
-void renderer_backward_splats(dio_renderer* r, dio_buf* pixel_grads, bool cleared = false) {
+// pixel_loss and footprint (both or neither): also gather the footprint terms.
+void renderer_backward_splats(dio_renderer* r, dio_buf* pixel_grads, bool cleared = false,
+                              dio_buf* pixel_loss = nullptr, dio_buf* footprint = nullptr) {
   auto* sc = r->scene;
   ensure_grad_buffers(r);
   auto* t = sc->rasterizer;
   if (!t->framed) invalid("Encode a frame before its backward pass.");
-  backward_tiles(t, t->frame_w, t->frame_h, pixel_grads, sc->grad_splats, cleared);
+  backward_tiles(t, t->frame_w, t->frame_h, pixel_grads, sc->grad_splats, cleared, pixel_loss, footprint);
 }
 
-void renderer_backward(dio_renderer* r, dio_buf* pixel_grads, bool cleared = false) {
+void renderer_backward(dio_renderer* r, dio_buf* pixel_grads, bool cleared = false, dio_buf* pixel_loss = nullptr,
+                       dio_buf* footprint = nullptr) {
   auto* sc = r->scene;
   if (!sc) return;
-  renderer_backward_splats(r, pixel_grads, cleared);
+  renderer_backward_splats(r, pixel_grads, cleared, pixel_loss, footprint);
   dio::preprocess_backward(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count,
                            r->camera, dev<float>(sc->splats), dev<float>(sc->grad_splats),
                            dev<float>(sc->grad_gaussians), dev<float>(sc->grad_sh));
@@ -1126,6 +1129,9 @@ struct LearnedState {
   float confidence = 0.5f;
   uint32_t samples_per_densify = 80000;
   dio_buf* extra = nullptr;        // per Gaussian: largest gradient, summed depth, summed gradient
+  // mode 1: per Gaussian, since the last densification, the sums over its blended
+  // pixels of blend weight x loss share and of blend weight (the footprint loss)
+  dio_buf* footprint = nullptr;
   dio_buf *features = nullptr, *logits = nullptr;
   uint32_t features_count = 0;
   dio_buf *pending = nullptr, *pending_count = nullptr;
@@ -1219,7 +1225,8 @@ void destroy_trainer(dio_trainer* t) {
   (void)hipStreamSynchronize(t->ctx->stream);
   t->clock.destroy();
   batch_destroy(t->batch);
-  for (dio_buf** p : {&t->learned.extra, &t->learned.features, &t->learned.logits, &t->learned.pending,
+  for (dio_buf** p : {&t->learned.extra, &t->learned.footprint, &t->learned.features, &t->learned.logits,
+                      &t->learned.pending,
                       &t->learned.pending_count, &t->learned.store_features, &t->learned.store_labels,
                       &t->learned.store_count})
     release(*p);
@@ -1334,9 +1341,11 @@ void trainer_optimize(dio_trainer* t) {
   a.viewport[1] = view.camera.viewport[1];
   t->last_adam = a;
   auto* sc = t->renderer->scene;
+  dio_buf* const foot = t->learned.mode == 1 ? t->learned.footprint : nullptr;
+  dio_buf* const foot_loss = foot ? t->loss->pixel_loss : nullptr;
   if (densify_now) {
     // No Adam step on a densification step: the gradients feed only the statistics.
-    renderer_backward(t->renderer, t->loss->pixel_grads, true);
+    renderer_backward(t->renderer, t->loss->pixel_grads, true, foot_loss, foot);
     dio::accumulate_stats(t->ctx->stream, t->count, dev<float>(sc->splats), dev<float>(sc->grad_splats), a,
                           dev<float>(t->stats), dev<float>(t->learned.extra));
   } else {
@@ -1344,7 +1353,7 @@ void trainer_optimize(dio_trainer* t) {
     // one pass; the renderer's per-Gaussian gradient buffers are not written. Unless
     // this step's maintenance resets the opacities, the same pass also runs the next
     // step's preprocess (next view drawn now, SH degree as that step will have it).
-    renderer_backward_splats(t->renderer, t->loss->pixel_grads, true);
+    renderer_backward_splats(t->renderer, t->loss->pixel_grads, true, foot_loss, foot);
     const bool reset_next = densifying && st.opacity_reset_interval && step % st.opacity_reset_interval == 0;
     dio::PrepArgs next;
     const bool prepare = !reset_next;
@@ -1438,7 +1447,7 @@ bool densify_codes(dio_trainer* t, const dio::DensifyArgs& a, dio_buf* codes, di
     }
     if (L.mode == 1 && L.pending_valid)  // the last logging step's samples, judged on these statistics
       dio::mlp_resolve(s, dev<uint32_t>(L.pending_count), L.pending_capacity, dev<float>(L.pending),
-                       dev<float>(t->stats), n, L.store_capacity, dev<float>(L.store_features),
+                       dev<float>(L.footprint), n, L.store_capacity, dev<float>(L.store_features),
                        dev<uint32_t>(L.store_labels), dev<uint32_t>(L.store_count));
   }
   if (L.budget) {
@@ -1490,7 +1499,7 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
   a.seed = t->iteration;
   hipStream_t s = t->ctx->stream;
   dio_buf* outputs = alloc(t->ctx, static_cast<size_t>(n) * 4);
-  dio_buf *scratch = nullptr, *codes = nullptr, *priority = nullptr, *extra = nullptr;
+  dio_buf *scratch = nullptr, *codes = nullptr, *priority = nullptr, *extra = nullptr, *foot = nullptr;
   dio_buf *params = nullptr, *sh = nullptr, *moments = nullptr, *stats = nullptr, *gaussians = nullptr;
   try {
     scratch = alloc(t->ctx, 64 + dio::scan_workspace_bytes(n), true);
@@ -1507,8 +1516,9 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
       const double fraction = std::min(1.0, static_cast<double>(L.samples_per_densify) / std::max(n, 1u));
       const uint32_t threshold = static_cast<uint32_t>(std::min(4294967295.0, fraction * 4294967296.0));
       dio::fill_u32(s, dev<uint32_t>(L.pending_count), 0u, 1);
-      dio::mlp_log_pending(s, n, dev<float>(L.features), forced_codes, dev<uint32_t>(outputs), t->iteration,
-                           threshold, L.pending_capacity, dev<float>(L.pending), dev<uint32_t>(L.pending_count));
+      dio::mlp_log_pending(s, n, dev<float>(L.features), dev<float>(L.footprint), forced_codes,
+                           dev<uint32_t>(outputs), t->iteration, threshold, L.pending_capacity, dev<float>(L.pending),
+                           dev<uint32_t>(L.pending_count));
       L.pending_valid = true;
     }
     uint32_t host[5];
@@ -1523,13 +1533,14 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
     stats = alloc(t->ctx, static_cast<size_t>(new_count) * 16, true);
     gaussians = alloc(t->ctx, static_cast<size_t>(new_count) * dio::kFloatsPerGaussian * 4, true);
     if (L.extra) extra = alloc(t->ctx, static_cast<size_t>(new_count) * 16, true);
+    if (L.footprint) foot = alloc(t->ctx, static_cast<size_t>(new_count) * 8, true);
     dio::densify_scatter(s, n, dev<float>(t->params), dev<float>(t->stats), dev<uint32_t>(outputs),
                          dev<float>(t->renderer->scene->sh), dev<float>(t->moments), a, forced_codes,
                          dev<float>(params), dev<float>(sh), dev<float>(moments));
     dio::activate(s, new_count, dev<float>(params), dev<float>(gaussians));
   } catch (...) {
     release(outputs); release(scratch); release(codes); release(priority); release(params); release(sh);
-    release(moments); release(stats); release(gaussians); release(extra);
+    release(moments); release(stats); release(gaussians); release(extra); release(foot);
     throw;
   }
   sync(t->ctx);
@@ -1550,6 +1561,10 @@ void trainer_densify(dio_trainer* t, bool prune_large) {
   if (L.extra) {
     release(L.extra);
     L.extra = extra;
+  }
+  if (L.footprint) {
+    release(L.footprint);
+    L.footprint = foot;
   }
   t->generation++;
 }
@@ -2509,6 +2524,7 @@ extern "C" int dio_trainer_set_densify_mode(dio_trainer* t, uint32_t mode, dio_d
     need(t, "trainer");
     if (mode > 2) invalid("densify mode is 0 (rule), 1 (rule, logging samples) or 2 (learned)");
     if (mode == 2) need(mlp, "mlp");
+    if (mode == 1 && t->batch) invalid("samples are logged by the plain (one view per step) trainer");
     LearnedState& L = t->learned;
     L.mode = mode;
     L.mlp = mlp;
@@ -2516,6 +2532,9 @@ extern "C" int dio_trainer_set_densify_mode(dio_trainer* t, uint32_t mode, dio_d
     L.confidence = confidence;
     L.samples_per_densify = samples_per_densify ? samples_per_densify : 80000;
     if (mode != 0 && !L.extra) L.extra = alloc(t->ctx, static_cast<size_t>(t->count) * 16, true);
+    if (mode == 1 && !L.footprint) L.footprint = alloc(t->ctx, static_cast<size_t>(t->count) * 8, true);
+    // The footprint terms come from the 16-px backward.
+    if (mode == 1) t->tuning.tile_size = 16;
     if (mode == 1 && !L.pending) {
       L.pending_capacity = 2 * L.samples_per_densify + 1024;
       L.pending = alloc(t->ctx, static_cast<size_t>(L.pending_capacity) * dio::kMlpPendingWords * 4, true);
@@ -2563,6 +2582,7 @@ extern "C" int dio_trainer_features(dio_trainer* t, dio_buf** features) {
 }
 
 extern "C" dio_buf* dio_trainer_extra_stats(dio_trainer* t) { return t ? t->learned.extra : nullptr; }
+extern "C" dio_buf* dio_trainer_footprint(dio_trainer* t) { return t ? t->learned.footprint : nullptr; }
 
 extern "C" int dio_trainer_batch_frames(dio_trainer* t, uint32_t* views, uint32_t capacity, uint32_t* count,
                                         dio_buf** splats, uint32_t* stride, dio_buf** pixels) {
@@ -2590,6 +2610,7 @@ void check_tuning(const dio_trainer* t, const dio_tuning& k) {
   if (k.tile_size == 32 && k.splat_batch == 256) invalid("32-pixel tiles take a splat batch of 64 or 128");
   if (k.digit_bits != 4 && k.digit_bits != 8) invalid("digit width must be 4 or 8 bits");
   if (t->batch && k.tile_size != 16) invalid("a batch trainer rasterizes 16-pixel tiles");
+  if (t->learned.mode == 1 && k.tile_size != 16) invalid("a trainer logging samples rasterizes 16-pixel tiles");
 }
 
 // The autotuner's candidates: every valid (tile size, splat batch, digit width).
@@ -2598,7 +2619,7 @@ std::vector<dio_tuning> tuning_candidates(const dio_trainer* t) {
   for (uint32_t tile : {16u, 8u, 32u})
     for (uint32_t bits : {8u, 4u})
       for (uint32_t batch : {64u, 128u, 256u}) {
-        if ((t->batch && tile != 16) || (tile == 32 && batch == 256)) continue;
+        if ((t->batch && tile != 16) || (t->learned.mode == 1 && tile != 16) || (tile == 32 && batch == 256)) continue;
         out.push_back(dio_tuning{tile, batch, bits});
       }
   return out;

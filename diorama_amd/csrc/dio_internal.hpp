@@ -165,10 +165,14 @@ void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, 
                const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
                float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch = 64, uint32_t tile_size = 16);
 
+// footprint (nullable; tile size 16): per splat, the sums of alpha T x the pixel's
+// loss share (`pixel_loss`) and of alpha T over its blended pixels are added into
+// footprint[2 i], footprint[2 i + 1].
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
-                        float* splat_grads, uint32_t splat_batch = 64, uint32_t tile_size = 16);
+                        float* splat_grads, uint32_t splat_batch = 64, uint32_t tile_size = 16,
+                        const float* pixel_loss = nullptr, float* footprint = nullptr);
 
 void preprocess_backward(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                          const CameraArgs& cam, const float* splats, const float* splat_grads,
@@ -319,10 +323,17 @@ void mlp_learned_codes(hipStream_t s, uint32_t n, const float* logits, float con
 // pending samples (16 words each); at the next densification the ones whose
 // outcome held join the training set.
 constexpr uint32_t kMlpPendingWords = 16;
-void mlp_log_pending(hipStream_t s, uint32_t n, const float* features, const uint32_t* codes, const uint32_t* outputs,
-                     uint32_t step, uint32_t threshold, uint32_t capacity, float* pending, uint32_t* pending_count);
+// Logs a hashed subset of the Gaussians as pending samples: inputs, the rule's (or
+// forced) class, the first output index and the loss over the footprint since the
+// last densification (`footprint`: per Gaussian the sums of w x loss share and of w).
+void mlp_log_pending(hipStream_t s, uint32_t n, const float* features, const float* footprint, const uint32_t* codes,
+                     const uint32_t* outputs, uint32_t step, uint32_t threshold, uint32_t capacity, float* pending,
+                     uint32_t* pending_count);
+// Moves the pending samples whose outcome held into the training set: pruned, or
+// the first descendant's footprint loss since (`footprint`, n Gaussians) below the
+// Gaussian's before.
 void mlp_resolve(hipStream_t s, const uint32_t* pending_count, uint32_t pending_capacity, const float* pending,
-                 const float* stats, uint32_t n, uint32_t store_capacity, float* store_features,
+                 const float* footprint, uint32_t n, uint32_t store_capacity, float* store_features,
                  uint32_t* store_labels, uint32_t* store_count);
 
 // ------------------------------------------------------------------- ply --

@@ -315,12 +315,23 @@ __global__ __launch_bounds__(kThreads) void learned_codes_kernel(uint32_t n, con
 }
 
 // A pending sample: the 12 inputs, the rule's class, the Gaussian's first index
-// after the densification, and its mean view-space gradient before it.
+// after the densification, the loss over its footprint before it (the mean
+// per-pixel loss share over the pixels it blended into, weighted by its blend
+// weight) and whether it blended into any pixel at all (1 or 0). Loss shares may
+// be negative (the SSIM term's constant is kept apart from them), so only the
+// comparison of two footprint losses of one view size is meaningful.
 constexpr int kPendingWords = 16;
+
+__device__ inline bool footprint_loss(const float* footprint, uint32_t i, float& loss) {
+  const float w = footprint[2 * static_cast<size_t>(i) + 1];
+  loss = w > 0.f ? footprint[2 * static_cast<size_t>(i)] / w : 0.f;
+  return w > 0.f;
+}
 
 __device__ inline uint32_t class_of_code(uint32_t code) { return code == 0u ? 3u : code - 1u; }
 
 __global__ __launch_bounds__(kThreads) void log_pending_kernel(uint32_t n, const float* __restrict__ features,
+                                                                const float* __restrict__ footprint,
                                                                 const uint32_t* __restrict__ codes,
                                                                 const uint32_t* __restrict__ outputs,
                                                                 uint32_t step, uint32_t threshold,
@@ -335,17 +346,20 @@ __global__ __launch_bounds__(kThreads) void log_pending_kernel(uint32_t n, const
   for (int k = 0; k < kIn; k++) out[k] = features[static_cast<size_t>(i) * kIn + k];
   out[12] = __uint_as_float(class_of_code(codes[i]));
   out[13] = __uint_as_float(outputs[i]);
-  out[14] = features[static_cast<size_t>(i) * kIn];  // log(1 + mean gradient / 2e-4)
-  out[15] = 0.f;
+  float loss;
+  out[15] = footprint_loss(footprint, i, loss) ? 1.f : 0.f;
+  out[14] = loss;
 }
 
 // A pending sample enters the training set when its outcome held: the Gaussian
-// (its first descendant) was pruned, or its mean view-space gradient over the
-// steps since fell below what it was.
+// was pruned (its footprint is gone, so its loss cannot be compared), or the loss
+// over the footprint of its first descendant, in the views since, fell below the
+// Gaussian's own before. A Gaussian that blended into no pixel before, or whose
+// descendant blended into none since, gives no verdict.
 __global__ __launch_bounds__(kThreads) void resolve_kernel(const uint32_t* __restrict__ pending_count,
                                                             uint32_t pending_capacity,
                                                             const float* __restrict__ pending,
-                                                            const float4* __restrict__ stats, uint32_t n,
+                                                            const float* __restrict__ footprint, uint32_t n,
                                                             uint32_t store_capacity, float* __restrict__ store_features,
                                                             uint32_t* __restrict__ store_labels,
                                                             uint32_t* __restrict__ store_count) {
@@ -357,9 +371,9 @@ __global__ __launch_bounds__(kThreads) void resolve_kernel(const uint32_t* __res
   if (!held) {
     const uint32_t o = __float_as_uint(in[13]);
     if (o >= n) return;
-    const float4 s = stats[o];
-    const float after = logf(1.f + (s.y > 0.f ? s.x / s.y : 0.f) / 2e-4f);
-    held = after < in[14];
+    float after;
+    const bool seen = footprint_loss(footprint, o, after);
+    held = in[15] != 0.f && seen && after < in[14];
   }
   if (!held) return;
   const uint32_t slot = atomicAdd(store_count, 1u);
@@ -378,21 +392,22 @@ void mlp_learned_codes(hipStream_t s, uint32_t n, const float* logits, float con
   DIO_LAUNCHED("mlp_learned_codes");
 }
 
-void mlp_log_pending(hipStream_t s, uint32_t n, const float* features, const uint32_t* codes, const uint32_t* outputs,
-                     uint32_t step, uint32_t threshold, uint32_t capacity, float* pending, uint32_t* pending_count) {
+void mlp_log_pending(hipStream_t s, uint32_t n, const float* features, const float* footprint, const uint32_t* codes,
+                     const uint32_t* outputs, uint32_t step, uint32_t threshold, uint32_t capacity, float* pending,
+                     uint32_t* pending_count) {
   if (n == 0) return;
-  log_pending_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, features, codes, outputs, step, threshold, capacity,
-                                                                pending, pending_count);
+  log_pending_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, features, footprint, codes, outputs, step,
+                                                                threshold, capacity, pending, pending_count);
   DIO_LAUNCHED("mlp_log_pending");
 }
 
 void mlp_resolve(hipStream_t s, const uint32_t* pending_count, uint32_t pending_capacity, const float* pending,
-                 const float* stats, uint32_t n, uint32_t store_capacity, float* store_features,
+                 const float* footprint, uint32_t n, uint32_t store_capacity, float* store_features,
                  uint32_t* store_labels, uint32_t* store_count) {
   if (pending_capacity == 0) return;
   resolve_kernel<<<ceil_div(pending_capacity, kThreads), kThreads, 0, s>>>(
-      pending_count, pending_capacity, pending, reinterpret_cast<const float4*>(stats), n, store_capacity,
-      store_features, store_labels, store_count);
+      pending_count, pending_capacity, pending, footprint, n, store_capacity, store_features, store_labels,
+      store_count);
   DIO_LAUNCHED("mlp_resolve");
 }
 

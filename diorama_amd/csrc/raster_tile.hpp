@@ -246,28 +246,29 @@ __device__ inline float swap_sum(decltype(__builtin_amdgcn_permlane32_swap(0u, 0
 // across all of them: each cross-lane step then has eight independent ones between
 // it and the step it depends on, which fill the DPP and permlane read-after-write
 // wait states that one gradient's chain alone pads with s_nop.
-__device__ inline void four_totals_all(const float (&v)[4][kGrads], float (&total)[kGrads]) {
-  float lo[kGrads], hi[kGrads];
+template <int kTerms>
+__device__ inline void four_totals_all(const float (&v)[4][kTerms], float (&total)[kTerms]) {
+  float lo[kTerms], hi[kTerms];
 #pragma unroll
-  for (int q = 0; q < kGrads; q++)
+  for (int q = 0; q < kTerms; q++)
     lo[q] = swap_sum(__builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, v[0][q]),
                                                       __builtin_bit_cast(uint32_t, v[2][q]), false, false));
 #pragma unroll
-  for (int q = 0; q < kGrads; q++)
+  for (int q = 0; q < kTerms; q++)
     hi[q] = swap_sum(__builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, v[1][q]),
                                                       __builtin_bit_cast(uint32_t, v[3][q]), false, false));
 #pragma unroll
-  for (int q = 0; q < kGrads; q++)
+  for (int q = 0; q < kTerms; q++)
     total[q] = swap_sum(__builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, lo[q]),
                                                          __builtin_bit_cast(uint32_t, hi[q]), false, false));
 #pragma unroll
-  for (int q = 0; q < kGrads; q++) total[q] = dpp_add<0xb1>(total[q]);
+  for (int q = 0; q < kTerms; q++) total[q] = dpp_add<0xb1>(total[q]);
 #pragma unroll
-  for (int q = 0; q < kGrads; q++) total[q] = dpp_add<0x4e>(total[q]);
+  for (int q = 0; q < kTerms; q++) total[q] = dpp_add<0x4e>(total[q]);
 #pragma unroll
-  for (int q = 0; q < kGrads; q++) total[q] = dpp_add<0x141>(total[q]);
+  for (int q = 0; q < kTerms; q++) total[q] = dpp_add<0x141>(total[q]);
 #pragma unroll
-  for (int q = 0; q < kGrads; q++) total[q] = dpp_add<0x140>(total[q]);
+  for (int q = 0; q < kTerms; q++) total[q] = dpp_add<0x140>(total[q]);
 }
 
 __device__ inline uint32_t wave_max(uint32_t v) {
@@ -285,7 +286,14 @@ __device__ inline uint32_t wave_max(uint32_t v) {
 struct Replay {
   float T, final_bg, gr, gg, gb;  // final_bg = final T x (background . dL/dpixel)
   float behind, next_alpha, next;
+  float loss;                     // the pixel's loss share, for the footprint terms
 };
+
+// With kFoot the replay also yields two footprint terms per (splat, pixel): the
+// blend weight alpha T times the pixel's loss share, and alpha T. Summed over a
+// splat's pixels and views they give the loss over the Gaussian's footprint.
+template <bool kFoot>
+constexpr int kTermsOf = kGrads + (kFoot ? 2 : 0);
 
 // One splat's 9 gradient terms at one pixel (rasterizeBackward.wgsl:199-230),
 // written without branches: a pixel the splat does not blend into (power > 0,
@@ -296,8 +304,10 @@ struct Replay {
 // g = dL/dpower = o dL/dopacity (o the opacity) they are dL/dopacity times dx, dy
 // (mean2: o a, b, c times these), dx², dx dy, dy² (conic: o times -1/2, -1, -1/2),
 // then dL/dopacity and dL/drgb.
+template <bool kFoot = false>
 __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy, float mx, float my, float ha,
-                                    float hb, float hc, float op, float sr, float sg, float sb, float share[kGrads]) {
+                                    float hb, float hc, float op, float sr, float sg, float sb,
+                                    float (&share)[kTermsOf<kFoot>]) {
   const float dx = cx - mx, dy = cy - my;
   const float power2 = splat_power2(dx, dy, ha, hb, hc);
   const float G = __builtin_amdgcn_exp2f(power2);
@@ -324,6 +334,10 @@ __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy
   share[6] = at * st.gr;
   share[7] = at * st.gg;
   share[8] = at * st.gb;
+  if constexpr (kFoot) {
+    share[9] = at * st.loss;
+    share[10] = at;
+  }
   return valid;
 }
 
@@ -334,9 +348,9 @@ __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy
 // theirs and one float atomic per (splat, list, gradient) goes out.
 // The workgroup's LDS for one backward list with kPer x 64 pairs per round and
 // kWaves waves (kCull: each staged splat's 16-px tile rectangle too).
-template <int kPer, int kWaves = 4, bool kCull = false>
+template <int kPer, int kWaves = 4, bool kCull = false, bool kFoot = false>
 struct BackwardShared {
-  float sums[kWaves][kGrads][kPer * kChunk];
+  float sums[kWaves][kTermsOf<kFoot>][kPer * kChunk];
   float4 conic[kPer * kChunk];
   float4 box[kPer * kChunk];   // each splat's alpha >= 1/255 box (x0, x1, y0, y1)
   uint4 rect[kCull ? kPer * kChunk : 1];
@@ -348,8 +362,9 @@ struct BackwardShared {
 // One pair list of the backward pass by a workgroup of kWaves waves, wave w on its
 // quadrant `wt` (kCull: `run` is a 32-px bin's list, see WaveTile), kPer x 64 pairs
 // per round; the workgroup may call it again for another list once it returns (it
-// ends on a barrier).
-template <int kPer, int kWaves, bool kCull>
+// ends on a barrier). kFoot: also adds each splat's two footprint terms (see
+// Replay) into footprint[2 i], footprint[2 i + 1], reading the per-pixel loss shares.
+template <int kPer, int kWaves, bool kCull, bool kFoot = false>
 __device__ inline void rasterize_backward_group(const float4* __restrict__ splats,
                                                 const uint32_t* __restrict__ pair_splats, uint2 run, WaveTile wt,
                                                 uint32_t width, uint32_t height, float bg_r, float bg_g, float bg_b,
@@ -357,8 +372,11 @@ __device__ inline void rasterize_backward_group(const float4* __restrict__ splat
                                                 const uint32_t* __restrict__ pixel_last,
                                                 const float4* __restrict__ pixel_grads,
                                                 float* __restrict__ splat_grads,
-                                                BackwardShared<kPer, kWaves, kCull>& sh) {
+                                                BackwardShared<kPer, kWaves, kCull, kFoot>& sh,
+                                                const float* __restrict__ pixel_loss = nullptr,
+                                                float* __restrict__ footprint = nullptr) {
   constexpr int kRound = kPer * kChunk;
+  constexpr int kTerms = kTermsOf<kFoot>;
   constexpr int kGroupThreads = kWaves * kWave;
   auto& s_sums = sh.sums;
   auto& s_index = sh.index;
@@ -380,6 +398,7 @@ __device__ inline void rasterize_backward_group(const float4* __restrict__ splat
     last = pixel_last[index];
     const float4 pg = pixel_grads[index];
     st.gr = pg.x; st.gg = pg.y; st.gb = pg.z;
+    if (kFoot) st.loss = pixel_loss[index];
   }
   st.T = final_T;
   st.final_bg = final_T * (bg_r * st.gr + bg_g * st.gg + bg_b * st.gb);
@@ -409,7 +428,7 @@ __device__ inline void rasterize_backward_group(const float4* __restrict__ splat
 #pragma unroll
     for (int q = 0; q < kPer; q++)
 #pragma unroll
-      for (int v = 0; v < kGrads; v++) s_sums[wave][v][q * kChunk + lane] = 0.f;
+      for (int v = 0; v < kTerms; v++) s_sums[wave][v][q * kChunk + lane] = 0.f;
     __syncthreads();  // the staged records are read by every wave
     bool hits[kPer];
 #pragma unroll
@@ -430,7 +449,7 @@ __device__ inline void rasterize_backward_group(const float4* __restrict__ splat
       uint64_t mask = __ballot(hits[q]);
       while (mask) {
         // Up to four splats, back to front; missing ones contribute zeros.
-        float share[4][kGrads];
+        float share[4][kTerms];
         int js[4];
         bool any = false;
 #pragma unroll
@@ -447,15 +466,15 @@ __device__ inline void rasterize_backward_group(const float4* __restrict__ splat
             hc = rc.x; op = rc.y; sr = rc.z; sg = rc.w; sb = rec.b[j];
             blended = base + j - run.x < last;
           }
-          any |= replay_splat(st, blended, cx, cy, mx, my, ha, hb, hc, op, sr, sg, sb, share[g]);
+          any |= replay_splat<kFoot>(st, blended, cx, cy, mx, my, ha, hb, hc, op, sr, sg, sb, share[g]);
         }
         if (!__any(any)) continue;
         const int mine = row == 0 ? js[0] : (row == 1 ? js[1] : (row == 2 ? js[2] : js[3]));
-        float total[kGrads];
-        four_totals_all(share, total);
+        float total[kTerms];
+        four_totals_all<kTerms>(share, total);
         if (row_leader && mine >= 0) {
 #pragma unroll
-          for (int v = 0; v < kGrads; v++) s_sums[wave][v][mine] = total[v];
+          for (int v = 0; v < kTerms; v++) s_sums[wave][v][mine] = total[v];
         }
       }
     }
@@ -464,7 +483,8 @@ __device__ inline void rasterize_backward_group(const float4* __restrict__ splat
     // the first five sums times the opacity (g = o dL/dopacity): mean2 from the
     // (g dx, g dy) sums through the conic, the conic terms scaled by -1/2, -1, -1/2;
     // the rest as summed.
-    for (int e = tid; e < 5 * kRound; e += kGroupThreads) {
+    constexpr int kParts = kFoot ? 6 : 5;
+    for (int e = tid; e < kParts * kRound; e += kGroupThreads) {
       const int part = e / kRound, slot = e % kRound;
       auto sum = [&](int v) {
         float total = 0.f;
@@ -491,9 +511,13 @@ __device__ inline void rasterize_backward_group(const float4* __restrict__ splat
       } else if (part == 3) {
         t0 = sum(6); t1 = sum(7);
         v0 = 6; v1 = 7;
-      } else {
+      } else if (part == 4 || !kFoot) {
         t0 = sum(8); t1 = 0.f;
         v0 = 8; v1 = -1;
+      } else {  // part 5: the footprint terms
+        t0 = sum(kFoot ? kGrads : 0); t1 = sum(kFoot ? kGrads + 1 : 0);
+        out = footprint + static_cast<size_t>(s_index[slot]) * 2;
+        v0 = 0; v1 = 1;
       }
       if (t0 != 0.f) unsafeAtomicAdd(out + v0, t0);
       if (v1 >= 0 && t1 != 0.f) unsafeAtomicAdd(out + v1, t1);

@@ -332,6 +332,22 @@ __global__ __launch_bounds__(256) void rasterize_backward_kernel(
                                         shared);
 }
 
+// The same with the footprint terms (see raster::Replay) added into `footprint`.
+template <int kPer>
+__global__ __launch_bounds__(256) void rasterize_backward_foot_kernel(
+    const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
+    const uint32_t* __restrict__ order, uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
+    float bg_b, const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
+    const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads, const float* __restrict__ pixel_loss,
+    float* __restrict__ footprint) {
+  __shared__ raster::BackwardShared<kPer, 4, false, true> shared;
+  const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
+  raster::rasterize_backward_group<kPer, 4, false, true>(
+      splats, pair_splats, ranges[tile],
+      raster::WaveTile{tile % tiles_x, tile / tiles_x, static_cast<uint32_t>(threadIdx.x / kWave)}, width, height,
+      bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads, shared, pixel_loss, footprint);
+}
+
 template <int kPer>
 __global__ __launch_bounds__(64) void rasterize_backward_quad_kernel(
     const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
@@ -538,18 +554,39 @@ void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, 
 // CU's 160 KB.
 constexpr uint32_t kBackwardGroupsPerCu = 5;
 
-template <int kPer>
+template <int kPer, bool kFoot = false>
 constexpr size_t backward_lds_pad() {
   constexpr size_t group = 160 * 1024 / (kBackwardGroupsPerCu + 1) + 1;
-  return sizeof(raster::BackwardShared<kPer>) >= group ? 0 : group - sizeof(raster::BackwardShared<kPer>);
+  constexpr size_t used = sizeof(raster::BackwardShared<kPer, 4, false, kFoot>);
+  return used >= group ? 0 : group - used;
 }
 
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
-                        float* splat_grads, uint32_t splat_batch, uint32_t tile_size) {
+                        float* splat_grads, uint32_t splat_batch, uint32_t tile_size, const float* pixel_loss,
+                        float* footprint) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
+  if (footprint) {
+    if (tile_size != 16) throw Error(-1, "the footprint terms are gathered with 16-pixel tiles");
+    auto foot = [&](auto kernel, size_t pad) {
+      kernel<<<tx * ty, 256, pad, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                       reinterpret_cast<const uint2*>(ranges), order, width, height, tx,
+                                       background[0], background[1], background[2],
+                                       reinterpret_cast<const float4*>(pixel_color), pixel_last,
+                                       reinterpret_cast<const float4*>(pixel_grads), splat_grads, pixel_loss,
+                                       footprint);
+    };
+    switch (splat_batch) {
+      case 64: foot(rasterize_backward_foot_kernel<1>, backward_lds_pad<1, true>()); break;
+      case 128: foot(rasterize_backward_foot_kernel<2>, backward_lds_pad<2, true>()); break;
+      case 256: foot(rasterize_backward_foot_kernel<4>, backward_lds_pad<4, true>()); break;
+      default: throw Error(-1, "splat batch must be 64, 128 or 256");
+    }
+    DIO_LAUNCHED("rasterize_backward_foot");
+    return;
+  }
   const uint32_t bx = ceil_div(tx, 2u), by = ceil_div(ty, 2u);
   auto launch = [&](auto kernel, uint32_t groups, uint32_t threads, uint32_t row, size_t pad) {
     kernel<<<groups, threads, pad, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,

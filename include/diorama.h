@@ -334,6 +334,11 @@ int dio_trainer_features(dio_trainer* t, dio_buf** features);
 /* Per Gaussian (largest gradient length, summed depth, summed gradient x, y)
  * since the last densification; null in mode 0. */
 dio_buf* dio_trainer_extra_stats(dio_trainer* t);
+/* Mode 1 only (else null): per Gaussian, since the last densification, the sums
+ * over the pixels it blended into of blend weight (alpha T) x the pixel's loss
+ * share, and of blend weight (2 floats each). Their ratio is the loss over the
+ * Gaussian's footprint, the outcome signal of the logged samples. */
+dio_buf* dio_trainer_footprint(dio_trainer* t);
 /* A batch trainer's last step (no reference counterpart; for checks against a
  * per-view oracle): the training-view index of each of its `*count` slots (up to
  * `capacity` written to `views`); the splat buffer in which slot s holds its

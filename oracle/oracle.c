@@ -316,6 +316,42 @@ void oc_rasterize(const double* splats, const uint32_t* ranges, const uint32_t* 
   oc_rasterize_margin(splats, ranges, pair_splats, width, height, background, pixel_color, pixel_last, NULL);
 }
 
+/* The loss over each splat's footprint (no counterpart in the reference; the
+ * learned densification's outcome signal): per splat, the sums over the pixels it
+ * blends into of w x pixel_loss and of w, with w = alpha T its blend weight in the
+ * front-to-back blend of rasterize.wgsl:64-82 (the weight its colour enters the
+ * pixel with). footprint holds n x 2 doubles, overwritten. */
+void oc_footprint(long n, const double* splats, const uint32_t* ranges, const uint32_t* pair_splats, int width,
+                  int height, const double* pixel_loss, double* footprint) {
+  memset(footprint, 0, sizeof(double) * 2 * n);
+  int tx = (width + OC_TILE - 1) / OC_TILE;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long p = 0; p < (long)width * height; p++) {
+    int x = (int)(p % width), y = (int)(p / width);
+    int tile = (y / OC_TILE) * tx + x / OC_TILE;
+    uint32_t start = ranges[2 * tile], end = ranges[2 * tile + 1];
+    double T = 1;
+    for (uint32_t k = start; k < end; k++) {
+      const double* s = splats + (long)pair_splats[k] * 12;
+      double dx = x + 0.5 - s[0], dy = y + 0.5 - s[1];
+      double power = splat_power(s, dx, dy);
+      if (power > 0) continue;
+      double alpha = s[11] * exp(power);
+      if (alpha > 0.99) alpha = 0.99;
+      if (alpha < 1.0 / 255.0) continue;
+      double remaining = T * (1 - alpha);
+      if (remaining < 0.0001) break;
+      double w = alpha * T;
+      double* f = footprint + (long)pair_splats[k] * 2;
+#pragma omp atomic
+      f[0] += w * pixel_loss[p];
+#pragma omp atomic
+      f[1] += w;
+      T = remaining;
+    }
+  }
+}
+
 /* rasterizeBackward.wgsl:49-163: the back-to-front replay, summed per splat. */
 void oc_rasterize_backward(long n, const double* splats, const uint32_t* ranges, const uint32_t* pair_splats,
                            int width, int height, const double* background, const double* pixel_color,

@@ -38,6 +38,8 @@ void oc_rasterize_margin(const double* splats, const uint32_t* ranges, const uin
                          int width, int height, const double* background, double* pixel_color,
                          uint32_t* pixel_last, double* margin);
 
+void oc_footprint(long n, const double* splats, const uint32_t* ranges, const uint32_t* pair_splats, int width,
+                  int height, const double* pixel_loss, double* footprint);
 void oc_rasterize_backward(long n, const double* splats, const uint32_t* ranges,
                            const uint32_t* pair_splats, int width, int height,
                            const double* background, const double* pixel_color,

@@ -38,6 +38,7 @@ def lib():
         L.oc_rasterize.argtypes = [_d, _u32, _u32, c_int, c_int, _d, _d, _u32]
         L.oc_rasterize_margin.argtypes = [_d, _u32, _u32, c_int, c_int, _d, _d, _u32, _d]
         L.oc_rasterize_backward.argtypes = [c_long, _d, _u32, _u32, c_int, c_int, _d, _d, _u32, _d, _d]
+        L.oc_footprint.argtypes = [c_long, _d, _u32, _u32, c_int, c_int, _d, _d]
         L.oc_preprocess_backward.argtypes = [c_long, _d, _d, c_int, _d, _d, _d, _d, _d, _d, _d]
         L.oc_loss.argtypes = [c_int, c_int, _d, _u8, vp, vp]
         L.oc_loss.restype = ctypes.c_double
@@ -110,6 +111,20 @@ def rasterize(splats, ranges, pairs, width, height, background, margin=False):
     if margin:
         return color.reshape(-1, 4), last, near
     return color.reshape(-1, 4), last
+
+
+def footprint(splats, ranges, pairs, width, height, pixel_loss):
+    """Per splat (sum of w x pixel_loss, sum of w) over the pixels it blends into, w
+    its blend weight alpha T (oc_footprint)."""
+    s = _f64(splats).reshape(-1)
+    n = s.size // 12
+    out = np.zeros(n * 2)
+    pr = np.ascontiguousarray(pairs, dtype=np.uint32)
+    if pr.size == 0:
+        pr = np.zeros(1, dtype=np.uint32)
+    lib().oc_footprint(n, s, np.ascontiguousarray(ranges, dtype=np.uint32).reshape(-1), pr, width, height,
+                       _f64(pixel_loss).reshape(-1), out)
+    return out.reshape(n, 2)
 
 
 def rasterize_backward(splats, ranges, pairs, width, height, background, pixel_color, pixel_last, pixel_grads):

@@ -155,3 +155,78 @@ def test_logging_learning_and_budget_end_to_end():
     assert np.isfinite(raw["params"]).all()
     t2.destroy()
     net.destroy()
+
+
+def test_footprint_loss_against_oracle():
+    """Mode "log" gathers each Gaussian's footprint terms in the backward pass: the
+    sums over its blended pixels of alpha T x the pixel's loss share, and of alpha T
+    (oc_footprint), given the GPU's splats, image and pair lists; two steps add up."""
+    params, sh, g, cam = config0_scene(seed=2, n=4096)
+    teacher_p, teacher_sh = scene.random_raw_gaussians(4096, 3, box=1.0)
+    ctx = context()
+    image = scene.teacher_images(ctx, [cam], 4096, seed=3, box=1.0, noise=0)[0]
+    target = scene.rgba8_channels(image)
+    t = diorama_amd.Trainer(ctx, params, sh, [cam], [image], diorama_amd.default_settings(1.0))
+    t.set_densify_mode("log")
+    t.sh_degree = 3
+    want = np.zeros((4096, 2))
+    say("footprint")
+    for _ in range(2):
+        t.optimize()
+        r = t.renderer
+        r.tiles.after_submit()
+        splats = r.splat_buffer.download(np.float32).reshape(-1, 12)[:4096].astype(np.float64)
+        color = r.pixel_buffer.download(np.float32, 256 * 256 * 4).reshape(-1, 4).astype(np.float64)
+        ranges, pairs = oracle.tile_lists(splats, 256, 256)
+        _, _, shares = oracle.loss(256, 256, color, target)
+        want += oracle.footprint(splats, ranges, pairs, 256, 256, shares)
+    got = t.footprint.download(np.float32, 4096 * 2).reshape(-1, 2)
+    err = [float(np.abs(got[:, k] - want[:, k]).max() / np.abs(want[:, k]).max()) for k in range(2)]
+    print(f"footprint: err / max {err}; {int((want[:, 1] > 0).sum())} Gaussians blended")
+    assert max(err) < 1e-4
+    t.destroy()
+
+
+def test_learned_densification_against_plain_at_a_binding_budget():
+    """What the trained network gains: the plain rule (logging samples whose footprint
+    loss fell over the next 100 steps) and the MLP trained on them each grow 2,000
+    Gaussians under a 3,000 budget that binds (the rule alone grows past it), then
+    are scored on held-out views. Like src/checks/training.ts:99-104, the learned run
+    must not lose: at most 0.1 dB below the plain rule."""
+    ctx = context()
+    cams = orbit_cameras([(160, 120)] * 12, radius=3.5)
+    held, train = cams[::4], [c for k, c in enumerate(cams) if k % 4]
+    images = teacher_images(ctx, 8192, cams, seed=11)
+    held_images, train_images = images[::4], [im for k, im in enumerate(images) if k % 4]
+    params, sh = scene.random_raw_gaussians(2000, 12, box=1.0)
+    steps = 2001
+    say("unbudgeted rule")
+    free = densify_run(ctx, train, train_images, params, sh, "plain", steps)
+    grown = free.count
+    free.destroy()
+    budget = 3000
+    assert grown > budget, f"the rule alone grows to {grown}: the budget would not bind"
+    say("plain rule, logging")
+    a = densify_run(ctx, train, train_images, params, sh, "log", steps, budget=budget)
+    plain = a.evaluate(held, held_images, with_ssim=False)["psnr"]
+    feats, labels, count = a.samples()
+    lab = labels.download(np.uint32, count)
+    x = feats.download(np.float32, count * 12).reshape(count, 12).astype(np.float64)
+    net = diorama_amd.DensifyMLP(ctx, seed=13)
+    net.set_input_norm(x.mean(axis=0), 1.0 / np.maximum(x.std(axis=0), 1e-6))
+    classes = np.bincount(lab, minlength=4)
+    net.set_class_weights([min(30.0, math.sqrt(count / (4.0 * max(c, 1)))) for c in classes])
+    net.train(feats, labels, count, steps=600, batch=2048, lr=2e-3, seed=3)
+    plain_count = a.count
+    a.destroy()
+    say("learned")
+    b = densify_run(ctx, train, train_images, params, sh, "learned", steps, net=net, budget=budget)
+    learned = b.evaluate(held, held_images, with_ssim=False)["psnr"]
+    learned_count = b.count
+    b.destroy()
+    net.destroy()
+    print(f"binding budget {budget} (the rule alone grows to {grown}): plain {plain:.3f} dB with {plain_count}, "
+          f"learned {learned:.3f} dB with {learned_count} Gaussians; delta {learned - plain:+.3f} dB; "
+          f"{count} samples {classes.tolist()}")
+    assert plain_count <= budget and learned_count <= budget
+    assert learned >= plain - 0.1

@@ -418,3 +418,45 @@ def test_accumulate_stats_catches_planted_mistakes():
     assert np.allclose(slipped(), want, atol=1e-12)  # the restatement itself agrees
     for mistake in [dict(half=1.0), dict(units=False), dict(swap=True), dict(radius_sum=True), dict(gate=False)]:
         assert not np.allclose(slipped(**mistake), want, atol=1e-6), mistake
+
+
+def test_footprint_hand_values_and_blend_weight_invariant():
+    """oc_footprint: two splats over one 16x16 tile, worked by hand, then the invariant
+    that a pixel's blend weights add up to 1 - T (its final transmittance, from the
+    separate oc_rasterize) on a random scene."""
+    # Splat 0 in front (depth 1), splat 1 behind (depth 2), both centred on pixel
+    # (3, 5) with a tiny conic so every pixel sees almost the same alpha; the loss
+    # share is 1 at pixel (3, 5) and 0 elsewhere.
+    splats = np.zeros((2, 12))
+    splats[0, :4] = [3.5, 5.5, 1.0, 8.0]
+    splats[1, :4] = [3.5, 5.5, 2.0, 8.0]
+    splats[:, 4:7] = [1e-12, 0.0, 1e-12]
+    splats[0, 11], splats[1, 11] = 0.5, 0.25
+    ranges, pairs = oracle.tile_lists(splats, 16, 16)
+    loss = np.zeros(256)
+    loss[5 * 16 + 3] = 1.0
+    f = oracle.footprint(splats, ranges, pairs, 16, 16, loss)
+    # at (3, 5): w0 = 0.5, w1 = 0.25 x (1 - 0.5) = 0.125; summed over 256 pixels
+    assert np.allclose(f[:, 0], [0.5, 0.125], atol=1e-9)
+    assert np.allclose(f[:, 1], [256 * 0.5, 256 * 0.125], rtol=1e-9)
+    # a planted swap of the order (w1 taken at full transmittance) would give 0.25
+    assert not np.isclose(f[1, 0], 0.25)
+
+    rng = np.random.default_rng(3)
+    n, w, h = 300, 64, 48
+    sp = np.zeros((n, 12))
+    sp[:, 0] = rng.uniform(0, w, n)
+    sp[:, 1] = rng.uniform(0, h, n)
+    sp[:, 2] = rng.uniform(1, 10, n)
+    sp[:, 3] = 12.0
+    sx, sy = rng.uniform(1, 4, n), rng.uniform(1, 4, n)
+    sp[:, 4], sp[:, 6] = 1 / sx ** 2, 1 / sy ** 2
+    sp[:, 11] = rng.uniform(0.1, 0.99, n)
+    ranges, pairs = oracle.tile_lists(sp, w, h)
+    color, _ = oracle.rasterize(sp, ranges, pairs, w, h, [0, 0, 0])
+    ones = oracle.footprint(sp, ranges, pairs, w, h, np.ones(w * h))
+    assert np.allclose(ones[:, 0], ones[:, 1])
+    assert abs(ones[:, 1].sum() - (1 - color[:, 3]).sum()) < 1e-9 * w * h
+    share = rng.uniform(0, 1, w * h)
+    weighted = oracle.footprint(sp, ranges, pairs, w, h, share)
+    assert abs(weighted[:, 0].sum() - (share * (1 - color[:, 3])).sum()) < 1e-9 * w * h
