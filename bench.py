@@ -379,14 +379,16 @@ def scaled_settings(settings, factor):
 
 
 def learned_bench(args):
-    """Learned densification on the ring scene at a fixed Gaussian budget.
+    """Learned densification on the ring scene, scored on the 33 held-out views.
 
-    A: the plain rule from 100k Gaussians for --train-steps steps, logging samples
-       (inputs and outcome-filtered rule actions) from step 500 to --log-until;
-    the MLP then trains on them (2,000 Adam steps of 4,096 samples);
-    B: the same run densified by the MLP (the rule where it is unsure);
-    C: B batched, 16 views per step with the schedule divided by 16.
-    Every run holds the 1.1M budget; all are scored on the 33 held-out views."""
+    A: the plain rule from 100k Gaussians for --train-steps steps under the config's
+       1.1M budget, logging samples (inputs and the rule's action, kept when the loss
+       over the Gaussian's footprint fell over the next 100 steps) from step 500 to
+       --log-until; the MLP then trains on them (2,000 Adam steps of 4,096 samples);
+    B: the same run densified by the MLP (the rule where it is unsure), 1.1M budget;
+    C, D: the plain rule and the MLP under --binding-budget, which this synthetic
+       scene's growth does reach (the plain rule alone stops near 116k);
+    E: B batched, 16 views per step with the schedule divided by 16."""
     import diorama_amd
     from diorama_amd import scene
 
@@ -410,7 +412,7 @@ def learned_bench(args):
             trainer.step()
             if on_step:
                 on_step(k)
-            if k % 2000 == 0:
+            if k % 5000 == 0:
                 log(f"{name}: step {k}/{n_steps}, {trainer.count} Gaussians, {time.perf_counter() - t0:.1f} s")
         ctx.synchronize()
         elapsed = time.perf_counter() - t0
@@ -418,6 +420,7 @@ def learned_bench(args):
         log(f"{name}: {n_steps} steps in {elapsed:.1f} s, {trainer.count} Gaussians, held-out PSNR {psnr:.3f} dB")
         return {"steps": n_steps, "seconds": round(elapsed, 2), "steps_per_s": round(n_steps / elapsed, 2),
                 "views_per_s": round(n_steps * max(1, trainer.batch_size) / elapsed, 2), "gaussians": trainer.count,
+                "largest_count": max([trainer.count] + [d["after"] for d in trainer.densifications]),
                 "held_out_psnr": round(psnr, 4), "densifications": len(trainer.densifications)}
 
     out = {}
@@ -434,13 +437,7 @@ def learned_bench(args):
     net = diorama_amd.DensifyMLP(ctx, seed=1)
     if count:  # standardise the inputs over the logged samples
         x = feats.download(np.float32, count * 12).reshape(count, 12).astype(np.float64)
-        bad = ~np.isfinite(x)
-        if bad.any():
-            rows = np.nonzero(bad.any(axis=1))[0]
-            log(f"non-finite inputs: {bad.sum(axis=0).tolist()} per input, {len(rows)} samples; first: "
-                f"{x[rows[:3]].tolist()} labels {labels.download(np.uint32, count)[rows[:3]].tolist()}")
-        good = x[~bad.any(axis=1)]
-        log(f"inputs min {np.round(good.min(axis=0), 3).tolist()} max {np.round(good.max(axis=0), 3).tolist()}")
+        good = x[np.isfinite(x).all(axis=1)]
         net.set_input_norm(good.mean(axis=0), 1.0 / np.maximum(good.std(axis=0), 1e-6))
     # square-root inverse class frequency: the rule keeps almost every Gaussian
     weights = [min(30.0, math.sqrt(max(count, 1) / (4.0 * max(c, 1)))) for c in classes]
@@ -457,10 +454,6 @@ def learned_bench(args):
         lab = labels.download(np.uint32, count)
         confusion = np.zeros((4, 4), dtype=np.int64)
         np.add.at(confusion, (lab, pred), 1)
-        x = feats.download(np.float32, count * 12).reshape(count, 12)
-        for c in range(4):
-            if (lab == c).any():
-                log(f"class {c}: mean inputs {np.round(x[lab == c].mean(axis=0), 3).tolist()}")
         log(f"mlp confusion (rows: label, cols: predicted): {confusion.tolist()}")
         logits.destroy()
     a.destroy()
@@ -468,12 +461,21 @@ def learned_bench(args):
     b.set_densify_mode("learned", net, budget=budget)
     out["learned"] = run("learned", b, steps)
     b.destroy()
-    if not args.no_batch:
-        c = diorama_amd.Trainer(ctx, params, sh, train, train_images, scaled_settings(settings, BATCH_VIEWS),
-                                batch=BATCH_VIEWS)
-        c.set_densify_mode("learned", net, budget=budget)
-        out["batched_learned"] = run("batched+learned", c, steps // BATCH_VIEWS)
+    if args.binding_budget:
+        c = diorama_amd.Trainer(ctx, params, sh, train, train_images, settings)
+        c.set_densify_mode("plain", budget=args.binding_budget)
+        out["plain_binding"] = run("plain, binding budget", c, steps)
         c.destroy()
+        d = diorama_amd.Trainer(ctx, params, sh, train, train_images, settings)
+        d.set_densify_mode("learned", net, budget=args.binding_budget)
+        out["learned_binding"] = run("learned, binding budget", d, steps)
+        d.destroy()
+    if not args.no_batch:
+        e = diorama_amd.Trainer(ctx, params, sh, train, train_images, scaled_settings(settings, BATCH_VIEWS),
+                                batch=BATCH_VIEWS)
+        e.set_densify_mode("learned", net, budget=budget)
+        out["batched_learned"] = run("batched+learned", e, steps // BATCH_VIEWS)
+        e.destroy()
     net.destroy()
     ctx.close()
     delta = out["learned"]["held_out_psnr"] - out["plain"]["held_out_psnr"]
@@ -489,19 +491,114 @@ def learned_bench(args):
         "data": "synthetic: seed-7 1.1M teacher renders + N(0,0.01) noise, 263 train / 33 held-out ring poses "
                 "at 980x545; seed-0 student",
         "config": {"workload": "plain rule vs learned densification vs batched+learned", "initial": args.initial,
-                   "budget": budget, "train_steps": steps, "log_until": args.log_until,
+                   "budget": budget, "binding_budget": args.binding_budget, "train_steps": steps,
+                   "log_until": args.log_until, "labels": "rule action kept if the footprint loss fell over 100 steps",
                    "samples": count, "sample_classes": classes, "class_weights": [round(w, 3) for w in weights],
                    "mlp_loss": round(mlp_loss, 4),
                    "mlp_confusion": confusion.tolist() if confusion is not None else None,
                    "mlp_train_s": round(mlp_s, 3)},
         "runs": out,
+        "budget_binds": out["plain"]["largest_count"] >= budget,
     }
+    if "plain_binding" in out:
+        line["binding_budget_psnr_delta"] = round(out["learned_binding"]["held_out_psnr"] -
+                                                  out["plain_binding"]["held_out_psnr"], 4)
     if "batched_learned" in out:
         line["batched_learned_psnr_delta"] = round(out["batched_learned"]["held_out_psnr"] -
                                                    out["plain"]["held_out_psnr"], 4)
         line["batched_learned_views_per_s_ratio"] = round(out["batched_learned"]["views_per_s"] /
                                                           out["plain"]["views_per_s"], 3)
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def short_psnr_delta(ctx, train, train_images, held, held_images, initial=100_000, steps=11_500, log_until=7000,
+                     budget=105_000):
+    """The second half of the metric on a shortened config-4 protocol, for the
+    default bench line: the plain rule (logging samples) and the MLP trained on them
+    each grow `initial` Gaussians for `steps` steps under a `budget` that binds on
+    this scene, then both are scored on the held-out views. The steps end 2,500 past
+    the opacity reset of step 9000 and before the next one. The full 30k-step
+    comparison is --workload learned."""
+    import diorama_amd
+    from diorama_amd import scene
+
+    params, sh = scene.random_raw_gaussians(initial, 0, box=5.0)
+    extent = 1.1 * max(math.sqrt(sum(v * v for v in _eye(c))) for c in train)
+    settings = diorama_amd.default_settings(extent)
+    t0 = time.perf_counter()
+    a = diorama_amd.Trainer(ctx, params, sh, train, train_images, settings)
+    a.set_densify_mode("log", budget=budget, samples_per_densify=200_000)
+    for k in range(1, steps + 1):
+        a.step()
+        if k == log_until:
+            a.set_densify_mode("plain", budget=budget)
+    plain = a.evaluate(held, held_images, with_ssim=False)["psnr"]
+    plain_count = a.count
+    feats, labels, count = a.samples()
+    classes = np.bincount(labels.download(np.uint32, count), minlength=4) if count else np.zeros(4)
+    net = diorama_amd.DensifyMLP(ctx, seed=1)
+    if count:
+        x = feats.download(np.float32, count * 12).reshape(count, 12).astype(np.float64)
+        good = x[np.isfinite(x).all(axis=1)]
+        net.set_input_norm(good.mean(axis=0), 1.0 / np.maximum(good.std(axis=0), 1e-6))
+        net.set_class_weights([min(30.0, math.sqrt(count / (4.0 * max(c, 1)))) for c in classes])
+        net.train(feats, labels, count, steps=2000, batch=4096, lr=2e-3, seed=7)
+    a.destroy()
+    b = diorama_amd.Trainer(ctx, params, sh, train, train_images, settings)
+    b.set_densify_mode("learned", net, budget=budget)
+    for _ in range(steps):
+        b.step()
+    learned = b.evaluate(held, held_images, with_ssim=False)["psnr"]
+    learned_count = b.count
+    b.destroy()
+    net.destroy()
+    log(f"held-out PSNR at a {budget:,} budget after {steps} steps: plain {plain:.3f} dB ({plain_count:,}), "
+        f"learned {learned:.3f} dB ({learned_count:,}), {time.perf_counter() - t0:.1f} s")
+    return {"value": round(learned - plain, 4), "unit": "dB", "plain_db": round(plain, 4),
+            "learned_db": round(learned, 4), "gaussians": [plain_count, learned_count], "budget": budget,
+            "initial": initial, "steps": steps, "samples": count, "held_out_views": len(held),
+            "protocol": "shortened config 4 (the full 30k-step run: --workload learned)"}
+
+
+def growth_bench(args):
+    """Why the plain rule grows the ring scene only a little: per densification window
+    (100 steps from step 500), the distribution of the mean view-space gradient
+    (NDC units, the statistic src/trainer.ts:258-337 thresholds at 2e-4) over the
+    Gaussians seen, and how many pass the threshold."""
+    import diorama_amd
+    from diorama_amd import scene
+
+    ctx = diorama_amd.Context(0)
+    cams = ring_poses(TRAIN_VIEWS + HELD_OUT)
+    train, _ = split_views(cams)
+    images = teacher_targets(ctx, train, N_GAUSSIANS)
+    params, sh = scene.random_raw_gaussians(args.initial, 0, box=5.0)
+    extent = 1.1 * max(math.sqrt(sum(v * v for v in _eye(c))) for c in train)
+    settings = diorama_amd.default_settings(extent)
+    threshold = settings["densify_grad_threshold"]
+    t = diorama_amd.Trainer(ctx, params, sh, train, images, settings)
+    windows = []
+    for step in range(1, args.train_steps + 1):
+        if step > settings["densify_from"] and step % settings["densify_interval"] == 0 and step < settings["densify_until"]:
+            st = t.buffers["stats"].download(np.float32).reshape(-1, 4)[:t.count]
+            seen = st[:, 1] > 0
+            g = st[seen, 0] / st[seen, 1]
+            windows.append({"step": step, "gaussians": t.count, "seen": int(seen.sum()),
+                            "views_mean": round(float(st[seen, 1].mean()), 1),
+                            "grad_percentiles_50_90_99_99.9": [float(f"{v:.3g}") for v in
+                                                               np.percentile(g, [50, 90, 99, 99.9])],
+                            "at_or_above_threshold": int((g >= threshold).sum())})
+            log(json.dumps(windows[-1]))
+        t.step()
+    t.destroy()
+    ctx.close()
+    print(json.dumps({"metric": "Gaussians at or above the densify threshold per window", "unit": "Gaussians",
+                      "value": int(np.median([w["at_or_above_threshold"] for w in windows[1:]])) if len(windows) > 1
+                      else None, "threshold": threshold, "config": {"workload": "plain rule growth statistics",
+                                                                    "initial": args.initial,
+                                                                    "steps": args.train_steps},
+                      "windows": windows}), flush=True)
     return 0
 
 
@@ -539,16 +636,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--gaussians", type=int, default=N_GAUSSIANS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="train: skip the shortened learned-vs-plain held-out PSNR comparison")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
     ap.add_argument("--tuning", default=None,
                     help="train: tile_size,splat_batch,digit_bits to use instead of autotuning (e.g. 32,128,8)")
-    ap.add_argument("--workload", choices=["train", "scan-sort", "batch", "learned"], default="train",
+    ap.add_argument("--workload", choices=["train", "scan-sort", "batch", "learned", "growth"], default="train",
                     help="train: the training step (default); scan-sort: the scan/sort microbenchmark; "
-                         "batch: the multi-view batch trainer; learned: learned vs plain densification")
+                         "batch: the multi-view batch trainer; learned: learned vs plain densification; "
+                         "growth: the plain rule's gradient statistics per densification window")
     ap.add_argument("--train-steps", type=int, default=30000, help="learned: training steps per run")
     ap.add_argument("--log-until", type=int, default=7000, help="learned: last step that logs samples")
     ap.add_argument("--initial", type=int, default=100_000, help="learned: initial Gaussians")
     ap.add_argument("--no-batch", action="store_true", help="learned: skip the batched run")
+    ap.add_argument("--binding-budget", type=int, default=105_000,
+                    help="learned: the second, binding Gaussian budget (0: skip those runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 256 MiB device copies (the FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -564,16 +666,19 @@ def main():
         return batch_bench(args)
     if args.workload == "learned":
         return learned_bench(args)
+    if args.workload == "growth":
+        return growth_bench(args)
 
     import diorama_amd
     from diorama_amd import scene
 
     ctx = diorama_amd.Context(0)
     cams = ring_poses(TRAIN_VIEWS + HELD_OUT)
-    train, _ = split_views(cams)
+    train, held = split_views(cams)
     t0 = time.perf_counter()
-    images = teacher_targets(ctx, train, args.gaussians)
-    log(f"teacher targets: {len(images)} views in {time.perf_counter() - t0:.1f} s")
+    images = teacher_targets(ctx, train + held, args.gaussians)
+    images, held_images = images[:len(train)], images[len(train):]
+    log(f"teacher targets: {len(images) + len(held_images)} views in {time.perf_counter() - t0:.1f} s")
     params, sh = scene.random_raw_gaussians(args.gaussians, 0, box=5.0)
     extent = 1.1 * max(math.sqrt(sum(v * v for v in _eye(c))) for c in train)
     settings = diorama_amd.default_settings(extent)
@@ -645,9 +750,10 @@ def main():
         log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads of {cpu['cpu_model']} "
             f"({time.perf_counter() - t0:.1f} s)")
     trainer_tuning = trainer.tuning
+    trainer.destroy()
+    delta = None if args.no_learned else short_psnr_delta(ctx, train, images, held, held_images)
     autotune_table = None if tuned is None else [[k["tile_size"], k["splat_batch"], k["digit_bits"], round(ms, 4)]
                                                  for k, ms in tuned]
-    trainer.destroy()
     ctx.close()
     line = {
         "metric": METRIC,
@@ -676,6 +782,7 @@ def main():
         "stages_hbm_frac": {k: round(v["GB/s"] / HBM_PEAK_GBS, 4) for k, v in stages.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "held_out_psnr_delta": delta,
     }
     if cpu:
         line["config"]["speedup_vs_cpu"] = round(steps_per_s / cpu["value"], 1)
