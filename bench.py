@@ -640,7 +640,8 @@ def main():
                     help="train: skip the shortened learned-vs-plain held-out PSNR comparison")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
     ap.add_argument("--tuning", default=None,
-                    help="train: tile_size,splat_batch,digit_bits to use instead of autotuning (e.g. 32,128,8)")
+                    help="train: tile_size,splat_batch,digit_bits[,persistent] to use instead of autotuning "
+                         "(e.g. 32,128,8 or 16,64,8,1)")
     ap.add_argument("--workload", choices=["train", "scan-sort", "batch", "learned", "growth"], default="train",
                     help="train: the training step (default); scan-sort: the scan/sort microbenchmark; "
                          "batch: the multi-view batch trainer; learned: learned vs plain densification; "
@@ -689,12 +690,13 @@ def main():
         trainer.step()
     tuned = None
     if args.tuning:
-        tile, batch, bits = (int(v) for v in args.tuning.split(","))
-        trainer.tuning = {"tile_size": tile, "splat_batch": batch, "digit_bits": bits}
+        v = [int(x) for x in args.tuning.split(",")]
+        trainer.tuning = {"tile_size": v[0], "splat_batch": v[1], "digit_bits": v[2],
+                          "persistent": v[3] if len(v) > 3 else 0}
     elif not args.no_autotune:
         tuned = trainer.autotune(steps=3)
-        log("autotune: " + ", ".join(f"tile {k['tile_size']} batch {k['splat_batch']} digits {k['digit_bits']}: "
-                                     f"{ms:.3f} ms" for k, ms in tuned))
+        log("autotune: " + ", ".join(f"tile {k['tile_size']}{'p' if k['persistent'] else ''} batch "
+                                     f"{k['splat_batch']} digits {k['digit_bits']}: {ms:.3f} ms" for k, ms in tuned))
     ctx.synchronize()
     log(f"warmup {args.warmup} steps done; tuning {trainer.tuning}")
     t0 = time.perf_counter()
@@ -752,8 +754,8 @@ def main():
     trainer_tuning = trainer.tuning
     trainer.destroy()
     delta = None if args.no_learned else short_psnr_delta(ctx, train, images, held, held_images)
-    autotune_table = None if tuned is None else [[k["tile_size"], k["splat_batch"], k["digit_bits"], round(ms, 4)]
-                                                 for k, ms in tuned]
+    autotune_table = None if tuned is None else [[k["tile_size"], k["splat_batch"], k["digit_bits"], k["persistent"],
+                                                  round(ms, 4)] for k, ms in tuned]
     ctx.close()
     line = {
         "metric": METRIC,

@@ -473,6 +473,8 @@ struct dio_tile_rasterizer {
   float frame_bg[3] = {0.f, 0.f, 0.f};
   uint32_t splat_batch = 64, digit_bits = 8;  // tuning; results do not depend on them
   uint32_t tile_size = 16;  // tuning: 8, 16 or 32 (32: pairs binned by 32-px bins)
+  bool persistent = false;  // tuning: persistent rasterizer workgroups (16-px tiles)
+  dio_buf* work = nullptr;  // their two tile counters, zeroed between launches
   uint32_t bin_shift() const { return tile_size == 32 ? 1u : 0u; }
   StageClock* clock = nullptr;  // set by a trainer only while it runs a clocked step
 };
@@ -503,6 +505,7 @@ void destroy_tiles(dio_tile_rasterizer* t) {
   free_pairs(t);
   release(t->ranges);
   release(t->tile_order);
+  release(t->work);
   release(t->pixel_color);
   release(t->pixel_last);
   if (t->readback_host) (void)hipHostFree(t->readback_host);
@@ -530,6 +533,7 @@ dio_tile_rasterizer* create_tiles(dio_ctx* ctx, dio_buf* splats, uint32_t count,
     t->splat_scan_ws = alloc(ctx, dio::scan_workspace_bytes(count), true);
     t->splat_sort_ws = alloc(ctx, dio::sort_workspace_bytes(count), true);
     t->pair_total = alloc(ctx, 16, true);
+    t->work = alloc(ctx, 16, true);
     create_pairs(t, initial ? initial : std::max<uint32_t>(1u << 16, count * 8u));
     DIO_HIP(hipHostMalloc(reinterpret_cast<void**>(&t->readback_host), 16, hipHostMallocDefault));
     DIO_HIP(hipEventCreateWithFlags(&t->readback_done, hipEventDisableTiming));
@@ -615,7 +619,7 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   mark(kMarkRanges);
   dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), w,
                  h, bg, dev<uint32_t>(target), dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
-                 t->splat_batch, t->tile_size);
+                 t->splat_batch, t->tile_size, t->persistent ? dev<uint32_t>(t->work) : nullptr);
   mark(kMarkRasterize);
   t->framed = true;
   t->frame_w = w;
@@ -658,7 +662,8 @@ void backward_tiles(dio_tile_rasterizer* t, uint32_t w, uint32_t h, dio_buf* pix
   dio::rasterize_backward(s, dev<float>(t->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges),
                           dev<uint32_t>(t->tile_order), w, h, t->frame_bg, dev<float>(t->pixel_color),
                           dev<uint32_t>(t->pixel_last), dev<float>(pixel_grads), dev<float>(splat_grads),
-                          t->splat_batch, t->tile_size, dev<float>(pixel_loss), dev<float>(footprint));
+                          t->splat_batch, t->tile_size, dev<float>(pixel_loss), dev<float>(footprint),
+                          t->persistent ? dev<uint32_t>(t->work) : nullptr);
   if (t->clock) t->clock->mark(s, kMarkBackward);
 }
 
@@ -1164,7 +1169,7 @@ struct dio_trainer {
   BatchState* batch = nullptr;  // set for a multi-view batch trainer
   uint32_t generation = 0;      // bumped whenever densification replaces the buffers
   LearnedState learned;
-  dio_tuning tuning{16, 64, 8};
+  dio_tuning tuning{16, 64, 8, 0};
   StageClock clock;
   // The next step's frame, prepared at the end of a plain step by the fused
   // backward (its preprocess for the next view), and what it was prepared for. The
@@ -1282,6 +1287,7 @@ void trainer_optimize(dio_trainer* t) {
   tiles->splat_batch = t->tuning.splat_batch;
   tiles->digit_bits = t->tuning.digit_bits;
   tiles->tile_size = t->tuning.tile_size;
+  tiles->persistent = t->tuning.persistent != 0;
   if (t->clock.enabled) {
     t->clock.begin(t->ctx->stream);
     tiles->clock = &t->clock;
@@ -2258,7 +2264,8 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       run = [&] {
         dio::rasterize(s, dev<float>(sc->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(tr->ranges),
                        dev<uint32_t>(tr->tile_order), tr->frame_w, tr->frame_h, tr->frame_bg, nullptr,
-                       dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last), tr->splat_batch, tr->tile_size);
+                       dev<float>(tr->pixel_color), dev<uint32_t>(tr->pixel_last), tr->splat_batch, tr->tile_size,
+                       tr->persistent ? dev<uint32_t>(tr->work) : nullptr);
       };
     } else if (name == "rasterize_backward") {
       run = [&] {
@@ -2266,7 +2273,8 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
                                 dev<uint32_t>(tr->tile_order), tr->frame_w, tr->frame_h, tr->frame_bg,
                                 dev<float>(tr->pixel_color),
                                 dev<uint32_t>(tr->pixel_last), dev<float>(t->loss->pixel_grads),
-                                dev<float>(sc->grad_splats), tr->splat_batch, tr->tile_size);
+                                dev<float>(sc->grad_splats), tr->splat_batch, tr->tile_size, nullptr, nullptr,
+                                tr->persistent ? dev<uint32_t>(tr->work) : nullptr);
       };
     } else if (name == "preprocess_backward") {
       run = [&] {
@@ -2609,6 +2617,9 @@ void check_tuning(const dio_trainer* t, const dio_tuning& k) {
   if (k.splat_batch != 64 && k.splat_batch != 128 && k.splat_batch != 256) invalid("splat batch must be 64, 128 or 256");
   if (k.tile_size == 32 && k.splat_batch == 256) invalid("32-pixel tiles take a splat batch of 64 or 128");
   if (k.digit_bits != 4 && k.digit_bits != 8) invalid("digit width must be 4 or 8 bits");
+  if (k.persistent > 1) invalid("persistent is 0 or 1");
+  if (k.persistent && k.tile_size != 16) invalid("persistent workgroups rasterize 16-pixel tiles");
+  if (k.persistent && t->batch) invalid("a batch trainer launches a workgroup per tile");
   if (t->batch && k.tile_size != 16) invalid("a batch trainer rasterizes 16-pixel tiles");
   if (t->learned.mode == 1 && k.tile_size != 16) invalid("a trainer logging samples rasterizes 16-pixel tiles");
 }
@@ -2617,18 +2628,20 @@ void check_tuning(const dio_trainer* t, const dio_tuning& k) {
 std::vector<dio_tuning> tuning_candidates(const dio_trainer* t) {
   std::vector<dio_tuning> out;
   for (uint32_t tile : {16u, 8u, 32u})
-    for (uint32_t bits : {8u, 4u})
-      for (uint32_t batch : {64u, 128u, 256u}) {
-        if ((t->batch && tile != 16) || (t->learned.mode == 1 && tile != 16) || (tile == 32 && batch == 256)) continue;
-        out.push_back(dio_tuning{tile, batch, bits});
-      }
+    for (uint32_t persistent : {0u, 1u})
+      for (uint32_t bits : {8u, 4u})
+        for (uint32_t batch : {64u, 128u, 256u}) {
+          if ((t->batch && tile != 16) || (t->learned.mode == 1 && tile != 16) || (tile == 32 && batch == 256)) continue;
+          if (persistent && (tile != 16 || t->batch)) continue;
+          out.push_back(dio_tuning{tile, batch, bits, persistent});
+        }
   return out;
 }
 
 void apply_tuning(dio_trainer* t, const dio_tuning& k) {
   check_tuning(t, k);
   const bool changed = k.splat_batch != t->tuning.splat_batch || k.digit_bits != t->tuning.digit_bits ||
-                       k.tile_size != t->tuning.tile_size;
+                       k.tile_size != t->tuning.tile_size || k.persistent != t->tuning.persistent;
   t->tuning = k;
   if (changed) t->generation++;  // a captured batch graph bakes the launches in
 }

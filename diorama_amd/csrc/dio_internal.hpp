@@ -161,9 +161,12 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
 // maps workgroups to tiles. Results depend on neither.
 // tile_size: 16 (a workgroup per 16-px tile), 8 (a wave per 8x8 quadrant, same
 // 16-px lists) or 32 (a workgroup per 32-px bin: `ranges` and `order` are the bins').
+// work (nullable, 16-px tiles): two zeroed words; persistent workgroups take tiles
+// from them and leave them zeroed.
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
-               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch = 64, uint32_t tile_size = 16);
+               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch = 64, uint32_t tile_size = 16,
+               uint32_t* work = nullptr);
 
 // footprint (nullable; tile size 16): per splat, the sums of alpha T x the pixel's
 // loss share (`pixel_loss`) and of alpha T over its blended pixels are added into
@@ -172,7 +175,7 @@ void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
                         float* splat_grads, uint32_t splat_batch = 64, uint32_t tile_size = 16,
-                        const float* pixel_loss = nullptr, float* footprint = nullptr);
+                        const float* pixel_loss = nullptr, float* footprint = nullptr, uint32_t* work = nullptr);
 
 void preprocess_backward(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                          const CameraArgs& cam, const float* splats, const float* splat_grads,

@@ -113,7 +113,7 @@ __device__ inline void wave_sync() {
 // per-pixel order and every result are the same for any kPer. kCull: `run` is a
 // 32-px bin's list (see WaveTile).
 template <int kPer, bool kCull>
-__device__ inline void rasterize_wave(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
+__device__ __forceinline__ void rasterize_wave(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
                                       uint2 run, WaveTile wt, uint32_t width, uint32_t height, float bg_r,
                                       float bg_g, float bg_b, uint32_t* __restrict__ target,
                                       float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last,
@@ -202,7 +202,7 @@ __device__ inline void rasterize_wave(const float4* __restrict__ splats, const u
 
 // One 16x16 tile (tile_x, tile_y) by a 256-thread workgroup: wave w takes quadrant w.
 template <int kPer>
-__device__ inline void rasterize_tile(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
+__device__ __forceinline__ void rasterize_tile(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
                                       uint2 run, uint32_t tile_x, uint32_t tile_y, uint32_t width, uint32_t height,
                                       float bg_r, float bg_g, float bg_b, uint32_t* __restrict__ target,
                                       float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last) {
@@ -365,7 +365,7 @@ struct BackwardShared {
 // ends on a barrier). kFoot: also adds each splat's two footprint terms (see
 // Replay) into footprint[2 i], footprint[2 i + 1], reading the per-pixel loss shares.
 template <int kPer, int kWaves, bool kCull, bool kFoot = false>
-__device__ inline void rasterize_backward_group(const float4* __restrict__ splats,
+__device__ __forceinline__ void rasterize_backward_group(const float4* __restrict__ splats,
                                                 const uint32_t* __restrict__ pair_splats, uint2 run, WaveTile wt,
                                                 uint32_t width, uint32_t height, float bg_r, float bg_g, float bg_b,
                                                 const float4* __restrict__ pixel_color,
@@ -530,7 +530,7 @@ __device__ inline void rasterize_backward_group(const float4* __restrict__ splat
 // One 16x16 tile of the backward pass by a 256-thread workgroup (wave w on
 // quadrant w), kPer x 64 pairs per round.
 template <int kPer>
-__device__ inline void rasterize_backward_tile(const float4* __restrict__ splats,
+__device__ __forceinline__ void rasterize_backward_tile(const float4* __restrict__ splats,
                                                const uint32_t* __restrict__ pair_splats, uint2 run, uint32_t tile_x,
                                                uint32_t tile_y, uint32_t width, uint32_t height, float bg_r,
                                                float bg_g, float bg_b, const float4* __restrict__ pixel_color,

@@ -8,6 +8,9 @@
 // stable sort of (tile, depth) keys written in splat order
 // (src/shaders/tiles.wgsl:36-59, src/radixSort.ts:103-104), so the lists and
 // ranges are bit-identical while the pair traffic drops from 6 passes to 1-2.
+#include <map>
+#include <mutex>
+
 #include "gaussian_math.hpp"
 #include "block_copy.hpp"
 #include "raster_tile.hpp"
@@ -284,6 +287,38 @@ __global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict
                                bg_g, bg_b, target, pixel_color, pixel_last);
 }
 
+// Persistent workgroups (16-px tiles): a grid of as many workgroups as the GPU holds
+// at once, each taking the next entry of `order` from work[0] until none is left.
+// A workgroup counts itself out in work[1] after its last take, so the last one out
+// knows every take is done and leaves both counters at zero for the next launch.
+__device__ inline bool take_tile(uint32_t* work, uint32_t tiles, uint32_t& k) {
+  __shared__ uint32_t s_k;
+  __syncthreads();  // the previous tile is done and s_k has been read
+  if (threadIdx.x == 0) s_k = atomicAdd(work, 1u);
+  __syncthreads();
+  k = s_k;
+  if (k < tiles) return true;
+  if (threadIdx.x == 0 && atomicAdd(work + 1, 1u) == gridDim.x - 1) {
+    __hip_atomic_store(work, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(work + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return false;
+}
+
+template <int kPer>
+__global__ __launch_bounds__(256) void rasterize_persistent_kernel(
+    const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
+    const uint32_t* __restrict__ order, uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
+    float bg_b, uint32_t* __restrict__ target, float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last,
+    uint32_t tiles, uint32_t* __restrict__ work) {
+  uint32_t k;
+  while (take_tile(work, tiles, k)) {
+    const uint32_t tile = order ? order[k] : k;
+    raster::rasterize_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width, height,
+                                 bg_r, bg_g, bg_b, target, pixel_color, pixel_last);
+  }
+}
+
 template <int kPer>
 __global__ __launch_bounds__(64) void rasterize_quad_kernel(const float4* __restrict__ splats,
                                                             const uint32_t* __restrict__ pair_splats,
@@ -330,6 +365,23 @@ __global__ __launch_bounds__(256) void rasterize_backward_kernel(
   raster::rasterize_backward_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width,
                                         height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads,
                                         shared);
+}
+
+template <int kPer>
+__global__ __launch_bounds__(256) void rasterize_backward_persistent_kernel(
+    const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
+    const uint32_t* __restrict__ order, uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
+    float bg_b, const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
+    const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads, uint32_t tiles,
+    uint32_t* __restrict__ work) {
+  __shared__ raster::BackwardShared<kPer> shared;
+  uint32_t k;
+  while (take_tile(work, tiles, k)) {
+    const uint32_t tile = order ? order[k] : k;
+    raster::rasterize_backward_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width,
+                                          height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads,
+                                          shared);
+  }
 }
 
 // The same with the footprint terms (see raster::Replay) added into `footprint`.
@@ -521,11 +573,49 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
   }
 }
 
+// Workgroups of `kernel` (256 threads, `lds` dynamic LDS) the current GPU holds at
+// once, looked up once per (device, kernel).
+template <class K>
+uint32_t resident_groups(K kernel, size_t lds) {
+  static std::mutex lock;
+  static std::map<std::pair<int, const void*>, uint32_t> known;
+  int device = 0;
+  check_hip(hipGetDevice(&device), "hipGetDevice");
+  const auto key = std::make_pair(device, reinterpret_cast<const void*>(kernel));
+  std::lock_guard<std::mutex> guard(lock);
+  auto it = known.find(key);
+  if (it != known.end()) return it->second;
+  int cus = 0, per_cu = 0;
+  check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+  check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds), "hipOccupancy");
+  const uint32_t groups = static_cast<uint32_t>(std::max(1, cus * per_cu));
+  known[key] = groups;
+  return groups;
+}
+
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
-               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch, uint32_t tile_size) {
+               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch, uint32_t tile_size, uint32_t* work) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
+  if (work) {
+    if (tile_size != 16) throw Error(-1, "persistent workgroups rasterize 16-pixel tiles");
+    auto persist = [&](auto kernel) {
+      const uint32_t resident = resident_groups(kernel, 0);
+      kernel<<<std::min(resident, tx * ty), 256, 0, s>>>(
+          reinterpret_cast<const float4*>(splats), pair_splats, reinterpret_cast<const uint2*>(ranges), order, width,
+          height, tx, background[0], background[1], background[2], target, reinterpret_cast<float4*>(pixel_color),
+          pixel_last, tx * ty, work);
+    };
+    switch (splat_batch) {
+      case 64: persist(rasterize_persistent_kernel<1>); break;
+      case 128: persist(rasterize_persistent_kernel<2>); break;
+      case 256: persist(rasterize_persistent_kernel<4>); break;
+      default: throw Error(-1, "splat batch must be 64, 128 or 256");
+    }
+    DIO_LAUNCHED("rasterize_persistent");
+    return;
+  }
   const uint32_t bx = ceil_div(tx, 2u), by = ceil_div(ty, 2u);
   auto launch = [&](auto kernel, uint32_t groups, uint32_t threads, uint32_t row) {
     kernel<<<groups, threads, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
@@ -565,9 +655,27 @@ void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
                         float* splat_grads, uint32_t splat_batch, uint32_t tile_size, const float* pixel_loss,
-                        float* footprint) {
+                        float* footprint, uint32_t* work) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
+  if (work && !footprint) {
+    if (tile_size != 16) throw Error(-1, "persistent workgroups rasterize 16-pixel tiles");
+    auto persist = [&](auto kernel, size_t pad) {
+      const uint32_t resident = resident_groups(kernel, pad);
+      kernel<<<std::min(resident, tx * ty), 256, pad, s>>>(
+          reinterpret_cast<const float4*>(splats), pair_splats, reinterpret_cast<const uint2*>(ranges), order, width,
+          height, tx, background[0], background[1], background[2], reinterpret_cast<const float4*>(pixel_color),
+          pixel_last, reinterpret_cast<const float4*>(pixel_grads), splat_grads, tx * ty, work);
+    };
+    switch (splat_batch) {
+      case 64: persist(rasterize_backward_persistent_kernel<1>, backward_lds_pad<1>()); break;
+      case 128: persist(rasterize_backward_persistent_kernel<2>, backward_lds_pad<2>()); break;
+      case 256: persist(rasterize_backward_persistent_kernel<4>, backward_lds_pad<4>()); break;
+      default: throw Error(-1, "splat batch must be 64, 128 or 256");
+    }
+    DIO_LAUNCHED("rasterize_backward_persistent");
+    return;
+  }
   if (footprint) {
     if (tile_size != 16) throw Error(-1, "the footprint terms are gathered with 16-pixel tiles");
     auto foot = [&](auto kernel, size_t pad) {

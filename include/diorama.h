@@ -61,6 +61,10 @@ typedef struct dio_tuning {
   uint32_t tile_size;
   uint32_t splat_batch; /* pairs a wave stages per round: 64, 128 or 256 */
   uint32_t digit_bits;  /* radix digit width of the binning sorts: 4 or 8 */
+  /* 16-px tiles only: 0 launches a workgroup per tile; 1 launches as many
+   * workgroups as the GPU holds at once, each taking the next tile (heaviest first)
+   * from a counter until none is left */
+  uint32_t persistent;
 } dio_tuning;
 
 typedef struct dio_tuning_result {
@@ -280,7 +284,8 @@ void dio_trainer_destroy(dio_trainer* t);
 int dio_trainer_set_tuning(dio_trainer* t, const dio_tuning* tuning);
 int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out);
 /* The autotuner: for every candidate (tile size 16, 8, 32; splat batch 64, 128,
- * 256; digit width 4, 8; see dio_tuning) run one untimed optimize() step, then time `steps` more
+ * 256; digit width 4, 8; persistent 0, 1 with 16-px tiles; see dio_tuning) run one
+ * untimed optimize() step, then time `steps` more
  * (ordinary training steps; a batch trainer replays its step graph on the same
  * views), keep the fastest setting, and list up to `capacity` results. */
 int dio_trainer_autotune(dio_trainer* t, uint32_t steps, dio_tuning_result* results, uint32_t capacity,

@@ -173,22 +173,23 @@ def test_tuning_changes_speed_not_results(batch):
     images = teacher_images(ctx, n, cams, seed=21)
     params, sh = scene.random_raw_gaussians(n, 22, box=1.0)
     outs = {}
-    for tile in ((16,) if batch else (16, 8, 32)):
+    shapes = [(16, 0)] if batch else [(16, 0), (16, 1), (8, 0), (32, 0)]
+    for tile, persistent in shapes:
         for bits in (8, 4):
             for splats in (64, 128, 256):
                 if tile == 32 and splats == 256:
                     continue
                 t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(1.0), batch=batch)
                 t.sh_degree = 3
-                t.tuning = {"tile_size": tile, "splat_batch": splats, "digit_bits": bits}
-                say("tuning", batch, tile, bits, splats)
+                t.tuning = {"tile_size": tile, "splat_batch": splats, "digit_bits": bits, "persistent": persistent}
+                say("tuning", batch, tile, persistent, bits, splats)
                 t.optimize()
                 p = t.buffers["params"].download(np.float32).reshape(n, 16)
                 pix = None if batch else t.renderer.pixel_buffer.download(np.float32, 200 * 120 * 4)
                 sg = None if batch else t.renderer.gradients["splats"].download(np.float32).reshape(-1, 9)[:n]
-                outs[(tile, bits, splats)] = (p, pix, t.last_loss(), sg)
+                outs[(tile, persistent, bits, splats)] = (p, pix, t.last_loss(), sg)
                 t.destroy()
-    ref_p, ref_pix, ref_loss, ref_sg = outs[(16, 8, 64)]
+    ref_p, ref_pix, ref_loss, ref_sg = outs[(16, 0, 8, 64)]
     for key, (p, pix, loss, sg) in outs.items():
         if pix is not None:
             assert np.array_equal(pix, ref_pix), key
@@ -199,12 +200,15 @@ def test_tuning_changes_speed_not_results(batch):
     t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(1.0), batch=batch)
     say("autotune", batch)
     results = t.autotune(steps=2)
-    assert len(results) == (6 if batch else 16)
+    assert len(results) == (6 if batch else 22)
     best = min(results, key=lambda r: r[1])
     assert t.tuning == best[0]
-    print("autotune:", [(r[0]["tile_size"], r[0]["splat_batch"], r[0]["digit_bits"], round(r[1], 3)) for r in results])
+    print("autotune:", [(r[0]["tile_size"], r[0]["persistent"], r[0]["splat_batch"], r[0]["digit_bits"],
+                         round(r[1], 3)) for r in results])
     with pytest.raises(diorama_amd.DioramaError):
         t.tuning = {"tile_size": 8 if batch else 12, "splat_batch": 64, "digit_bits": 8}
     with pytest.raises(diorama_amd.DioramaError):
         t.tuning = {"tile_size": 32, "splat_batch": 256, "digit_bits": 8}
+    with pytest.raises(diorama_amd.DioramaError):
+        t.tuning = {"tile_size": 8, "splat_batch": 64, "digit_bits": 8, "persistent": 1}
     t.destroy()
