@@ -154,8 +154,35 @@ void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const 
 
 // Each tile's [start, end) run of the sorted pairs; with `order` (nullable, one
 // word per tile) also the rasterizer's launch order, heaviest tiles first.
+// clear (nullable): 1 + kPartClasses words set to zero before the rasterizer runs
+// (the segmented backward's item counts).
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
-                 uint32_t tiles, uint32_t* ranges, uint32_t* order = nullptr, uint32_t pair_hint = 0);
+                 uint32_t tiles, uint32_t* ranges, uint32_t* order = nullptr, uint32_t pair_hint = 0,
+                 uint32_t* clear = nullptr);
+
+// The segmented backward (16-px tiles). Segment boundaries are the absolute pair
+// positions that are multiples of 2^shift: the forward pass keeps each pixel's
+// transmittance and blended colour (T, r, g, b) in front of every boundary inside
+// a tile's list that the pixel reaches, at ckpt[(b >> shift) * 256 + quadrant * 64
+// + lane]. Its workgroups then list the backward's work items, the parts of their
+// tile's list between boundaries up to the tile's last blended splat, as (tile,
+// b >> shift), by length class: whole segments at items[0 ...], shorter parts in
+// kPartClasses classes of falling length at items[whole_room + (c - 1) part_room
+// ...], counted in counts[c] (zeroed by find_ranges). The backward takes them in
+// that order, longest first; each pixel's replay of a part starts from the
+// checkpoint at its end, so a long list is several workgroups' work, not one's.
+constexpr uint32_t kPartClasses = 8;
+struct Segments {
+  uint32_t shift = 0;             // log2 of the segment length: 8, 9 or 10 (0: off)
+  uint32_t whole_room = 0;        // (pair capacity >> shift) + 1
+  uint32_t part_room = 0;         // per class: 2 per tile
+  uint32_t grid = 0;              // backward workgroups; each loops over the items
+  float* ckpt = nullptr;          // whole_room entries of 256 float4
+  uint32_t* items = nullptr;      // uint2 per item, whole_room + kPartClasses part_room of them
+  uint32_t* counts = nullptr;     // 1 + kPartClasses words
+};
+// Segment lengths: 2^8 to 2^10 pairs (a multiple of every splat batch).
+constexpr uint32_t kSegmentShiftMin = 8, kSegmentShiftMax = 10;
 
 // splat_batch: pairs a wave stages per round (64, 128 or 256); `order` (nullable)
 // maps workgroups to tiles. Results depend on neither.
@@ -163,19 +190,24 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
 // 16-px lists) or 32 (a workgroup per 32-px bin: `ranges` and `order` are the bins').
 // work (nullable, 16-px tiles): two zeroed words; persistent workgroups take tiles
 // from them and leave them zeroed.
+// seg (nullable, 16-px tiles, not with work): also writes the segmented backward's
+// checkpoints and items.
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
                float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch = 64, uint32_t tile_size = 16,
-               uint32_t* work = nullptr);
+               uint32_t* work = nullptr, const Segments* seg = nullptr);
 
 // footprint (nullable; tile size 16): per splat, the sums of alpha T x the pixel's
 // loss share (`pixel_loss`) and of alpha T over its blended pixels are added into
 // footprint[2 i], footprint[2 i + 1].
+// seg (nullable): the segmented backward over the checkpoints and items the
+// forward pass wrote with the same `seg`.
 void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
                         float* splat_grads, uint32_t splat_batch = 64, uint32_t tile_size = 16,
-                        const float* pixel_loss = nullptr, float* footprint = nullptr, uint32_t* work = nullptr);
+                        const float* pixel_loss = nullptr, float* footprint = nullptr, uint32_t* work = nullptr,
+                        const Segments* seg = nullptr);
 
 void preprocess_backward(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                          const CameraArgs& cam, const float* splats, const float* splat_grads,

@@ -105,6 +105,12 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+__device__ inline uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, kWave)));
+  return v;
+}
+
 // rasterize.wgsl:31-100: front-to-back blend; a wave stops once all its pixels
 // are nearly opaque.
 // One wave's quadrant `wt` of a width x height frame whose pair list is `run`; the
@@ -112,12 +118,19 @@ __device__ inline void wave_sync() {
 // stages kPer x 64 pairs per round (kPer per lane) into its own `records`; the
 // per-pixel order and every result are the same for any kPer. kCull: `run` is a
 // 32-px bin's list (see WaveTile).
+// ckpt (16-px tiles, nullable): the segmented backward's checkpoints. Rounds then
+// start at absolute multiples of kPer x 64, so every boundary (a multiple of
+// 2^seg_shift) starts a round; before it, each pixel's (T, r, g, b) goes to
+// ckpt[(b >> seg_shift) * 256 + quadrant * 64 + lane], and the wave returns its
+// largest pixel_last (0 without ckpt).
 template <int kPer, bool kCull>
-__device__ __forceinline__ void rasterize_wave(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
-                                      uint2 run, WaveTile wt, uint32_t width, uint32_t height, float bg_r,
-                                      float bg_g, float bg_b, uint32_t* __restrict__ target,
-                                      float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last,
-                                      SplatRecords (&records)[kPer]) {
+__device__ __forceinline__ uint32_t rasterize_wave(const float4* __restrict__ splats,
+                                                   const uint32_t* __restrict__ pair_splats, uint2 run, WaveTile wt,
+                                                   uint32_t width, uint32_t height, float bg_r, float bg_g,
+                                                   float bg_b, uint32_t* __restrict__ target,
+                                                   float4* __restrict__ pixel_color,
+                                                   uint32_t* __restrict__ pixel_last, SplatRecords (&records)[kPer],
+                                                   float4* __restrict__ ckpt = nullptr, uint32_t seg_shift = 0) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t qx = wt.sx * kTile + (wt.quad & 1) * 8, qy = wt.sy * kTile + (wt.quad >> 1) * 8;
   const uint32_t px = qx + (lane & 7), py = qy + (lane >> 3);
@@ -129,15 +142,20 @@ __device__ __forceinline__ void rasterize_wave(const float4* __restrict__ splats
   float T = 1.f, cr = 0.f, cg = 0.f, cb = 0.f;
   bool done = !inside;
   uint32_t last = 0;
-  for (uint32_t round = run.x; round < run.y; round += kPer * kChunk) {
+  constexpr uint32_t kRound = kPer * kChunk;
+  const uint32_t first = ckpt && run.x < run.y ? run.x - run.x % kRound : run.x;
+  for (uint32_t round = first; round < run.y; round += kRound) {
     if (__all(done)) break;
+    if (ckpt && round > run.x && (round & ((1u << seg_shift) - 1u)) == 0)
+      ckpt[static_cast<size_t>(round >> seg_shift) * (4 * kChunk) + wt.quad * kChunk + lane] =
+          make_float4(T, cr, cg, cb);
     bool hits[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; q++) {
       hits[q] = false;
       const uint32_t k = round + q * kChunk + lane;
       Staged st{};
-      if (k < run.y) {
+      if (k >= run.x && k < run.y) {
         stage<kCull>(splats, pair_splats, k, st, nullptr, tiles_x, tiles_y);
         hits[q] = meets(st, bx0, bx1, by0, by1) && (!kCull || holds(st.rect, wt.sx, wt.sy));
       }
@@ -198,18 +216,60 @@ __device__ __forceinline__ void rasterize_wave(const float4* __restrict__ splats
       target[index] = u8(r) | (u8(g) << 8) | (u8(b) << 16) | (255u << 24);
     }
   }
+  return ckpt ? wave_max(last) : 0u;
+}
+
+// The segmented backward's items of a tile whose list is `run` and whose last
+// blended splat is `most` pairs in (see Segments): one thread's work. Only a list's
+// first and last parts can be short, so a tile adds at most two of them.
+__device__ inline void emit_segments(const Segments& seg, uint32_t tile, uint2 run, uint32_t most) {
+  if (most == 0) return;
+  const uint32_t size = 1u << seg.shift, end = run.x + most;
+  const uint32_t first = run.x >> seg.shift, final = (end - 1) >> seg.shift;
+  auto length = [&](uint32_t b) { return min(end, (b << seg.shift) + size) - max(run.x, b << seg.shift); };
+  uint2* items = reinterpret_cast<uint2*>(seg.items);
+  uint32_t whole = 0;
+  for (uint32_t b = first; b <= final; b++) {
+    const uint32_t len = length(b);
+    if (len == size) {
+      whole++;
+    } else {
+      const uint32_t c = 1 + (size - 1 - len) * kPartClasses / size;  // 1: longest
+      const uint32_t k = atomicAdd(seg.counts + c, 1u);
+      if (k < seg.part_room) items[seg.whole_room + (c - 1) * seg.part_room + k] = make_uint2(tile, b);
+    }
+  }
+  if (whole == 0) return;
+  uint32_t w = atomicAdd(seg.counts, whole);
+  for (uint32_t b = first; b <= final; b++)
+    if (length(b) == size) {
+      if (w < seg.whole_room) items[w] = make_uint2(tile, b);
+      w++;
+    }
 }
 
 // One 16x16 tile (tile_x, tile_y) by a 256-thread workgroup: wave w takes quadrant w.
 template <int kPer>
-__device__ __forceinline__ void rasterize_tile(const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats,
-                                      uint2 run, uint32_t tile_x, uint32_t tile_y, uint32_t width, uint32_t height,
-                                      float bg_r, float bg_g, float bg_b, uint32_t* __restrict__ target,
-                                      float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last) {
+__device__ __forceinline__ void rasterize_tile(const float4* __restrict__ splats,
+                                               const uint32_t* __restrict__ pair_splats, uint2 run, uint32_t tile_x,
+                                               uint32_t tile_y, uint32_t width, uint32_t height, float bg_r,
+                                               float bg_g, float bg_b, uint32_t* __restrict__ target,
+                                               float4* __restrict__ pixel_color, uint32_t* __restrict__ pixel_last,
+                                               const Segments& seg = Segments{}) {
   __shared__ SplatRecords s_records[4][kPer];  // one slot per wave: waves never share records
+  __shared__ uint32_t s_most[4];
   const uint32_t wave = threadIdx.x / kWave;
-  rasterize_wave<kPer, false>(splats, pair_splats, run, WaveTile{tile_x, tile_y, wave}, width, height, bg_r, bg_g,
-                              bg_b, target, pixel_color, pixel_last, s_records[wave]);
+  float4* ckpt = seg.shift ? reinterpret_cast<float4*>(seg.ckpt) : nullptr;
+  const uint32_t most =
+      rasterize_wave<kPer, false>(splats, pair_splats, run, WaveTile{tile_x, tile_y, wave}, width, height, bg_r,
+                                  bg_g, bg_b, target, pixel_color, pixel_last, s_records[wave], ckpt, seg.shift);
+  if (seg.shift) {
+    if ((threadIdx.x & (kWave - 1)) == 0) s_most[wave] = most;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      emit_segments(seg, tile_y * ceil_div(width, kTile) + tile_x, run,
+                    max(max(s_most[0], s_most[1]), max(s_most[2], s_most[3])));
+  }
 }
 
 template <int kCtrl>
@@ -271,22 +331,20 @@ __device__ inline void four_totals_all(const float (&v)[4][kTerms], float (&tota
   for (int q = 0; q < kTerms; q++) total[q] = dpp_add<0x140>(total[q]);
 }
 
-__device__ inline uint32_t wave_max(uint32_t v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, kWave)));
-  return v;
-}
-
 // The per-pixel replay state of the backward pass (rasterizeBackward.wgsl:161-177).
-// The colour behind a splat and the next blended splat's colour enter its gradient
-// only through their dot products with dL/dpixel, and the update
-// behind += a_next (next - behind) is linear, so the replay carries those two dot
-// products (behind, next) rather than the colours: the same sums, 8 fewer VALU
-// operations per splat and pixel.
+// The colour behind a splat enters its gradient only through its dot product with
+// dL/dpixel, so the replay carries that dot product (`behind`) rather than the
+// colour. It also counts the background in: with B_i the colour behind splat i
+// seen from just behind it, background included, dL/dalpha_i = T_i (c_i - B_i) . g
+// holds without the reference's separate -T_final (bg . g) / (1 - alpha_i) term,
+// B starts at bg . g behind the last blended splat, and B_{i-1} = B_i +
+// alpha_i (c_i - B_i) follows right after splat i's gradient, sharing its c_i - B_i
+// (the reference defers that update by one splat and keeps the last splat's alpha
+// and colour for it): about 5 fewer VALU instructions per splat and pixel.
 struct Replay {
-  float T, final_bg, gr, gg, gb;  // final_bg = final T x (background . dL/dpixel)
-  float behind, next_alpha, next;
-  float loss;                     // the pixel's loss share, for the footprint terms
+  float T, gr, gg, gb;
+  float behind;  // B . g
+  float loss;    // the pixel's loss share, for the footprint terms
 };
 
 // With kFoot the replay also yields two footprint terms per (splat, pixel): the
@@ -298,9 +356,9 @@ constexpr int kTermsOf = kGrads + (kFoot ? 2 : 0);
 // One splat's 9 gradient terms at one pixel (rasterizeBackward.wgsl:199-230),
 // written without branches: a pixel the splat does not blend into (power > 0,
 // alpha < 1/255, or past the pixel's last splat) runs the same instructions with
-// alpha = 0, which leaves T, the colour behind and the next splat as they were and
-// zeroes every term. The terms are linear in the splat's constants, so the conic
-// and the opacity are applied once per splat after the reduction: with
+// alpha = 0, which leaves T and the colour behind as they were and zeroes every
+// term. The terms are linear in the splat's constants, so the conic and the
+// opacity are applied once per splat after the reduction: with
 // g = dL/dpower = o dL/dopacity (o the opacity) they are dL/dopacity times dx, dy
 // (mean2: o a, b, c times these), dx², dx dy, dy² (conic: o times -1/2, -1, -1/2),
 // then dL/dopacity and dL/drgb.
@@ -314,14 +372,11 @@ __device__ inline bool replay_splat(Replay& st, bool blended, float cx, float cy
   const float clamped = fminf(0.99f, op * G);
   const bool valid = blended && power2 <= 0.f && clamped >= 1.f / 255.f;
   const float alpha = valid ? clamped : 0.f;
-  const float keep = __builtin_amdgcn_rcpf(1.f - alpha);  // one hardware reciprocal for both divisions; 1 when idle
-  st.T *= keep;
-  const float na = valid ? st.next_alpha : 0.f;
-  st.behind += na * (st.next - st.behind);
+  st.T *= __builtin_amdgcn_rcpf(1.f - alpha);  // T in front of the splat; unchanged when idle
   const float color = sr * st.gr + sg * st.gg + sb * st.gb;  // this splat's colour . dL/dpixel
-  st.next_alpha = valid ? alpha : st.next_alpha;
-  st.next = valid ? color : st.next;
-  const float alpha_grad = st.T * (color - st.behind) - st.final_bg * keep;
+  const float d = color - st.behind;
+  const float alpha_grad = st.T * d;
+  st.behind += alpha * d;
   const float opacity_grad = valid ? alpha_grad * G : 0.f;
   const float gdx = opacity_grad * dx, gdy = opacity_grad * dy;  // x opacity after the reduction
   share[0] = gdx;
@@ -359,11 +414,25 @@ struct BackwardShared {
   uint32_t last[kWaves];
 };
 
-// One pair list of the backward pass by a workgroup of kWaves waves, wave w on its
-// quadrant `wt` (kCull: `run` is a 32-px bin's list, see WaveTile), kPer x 64 pairs
-// per round; the workgroup may call it again for another list once it returns (it
-// ends on a barrier). kFoot: also adds each splat's two footprint terms (see
-// Replay) into footprint[2 i], footprint[2 i + 1], reading the per-pixel loss shares.
+// The part [lo, hi) of a pair list that a backward workgroup replays (absolute pair
+// positions). With ckpt, hi is a segment boundary inside the list (or the list's
+// end) and a pixel whose last blended splat lies at or past hi starts its replay at
+// hi from the forward pass's checkpoint there: T = the checkpoint's T, and the
+// colour behind = (blended colour - checkpoint colour) / T, the colour of every
+// splat from hi on as seen through the splats in front of hi. A pixel that ends
+// inside the part starts from its own end as for the whole list.
+struct Part {
+  uint32_t lo, hi;
+  const float4* __restrict__ ckpt;  // nullptr: the whole list, no checkpoint
+  uint32_t shift;
+};
+
+// One pair list (or the part `part` of it) of the backward pass by a workgroup of
+// kWaves waves, wave w on its quadrant `wt` (kCull: `run` is a 32-px bin's list,
+// see WaveTile), kPer x 64 pairs per round; the workgroup may call it again for
+// another list once it returns (it ends on a barrier). kFoot: also adds each
+// splat's two footprint terms (see Replay) into footprint[2 i], footprint[2 i + 1],
+// reading the per-pixel loss shares.
 template <int kPer, int kWaves, bool kCull, bool kFoot = false>
 __device__ __forceinline__ void rasterize_backward_group(const float4* __restrict__ splats,
                                                 const uint32_t* __restrict__ pair_splats, uint2 run, WaveTile wt,
@@ -374,7 +443,8 @@ __device__ __forceinline__ void rasterize_backward_group(const float4* __restric
                                                 float* __restrict__ splat_grads,
                                                 BackwardShared<kPer, kWaves, kCull, kFoot>& sh,
                                                 const float* __restrict__ pixel_loss = nullptr,
-                                                float* __restrict__ footprint = nullptr) {
+                                                float* __restrict__ footprint = nullptr,
+                                                Part part = Part{0u, 0u, nullptr, 0u}) {
   constexpr int kRound = kPer * kChunk;
   constexpr int kTerms = kTermsOf<kFoot>;
   constexpr int kGroupThreads = kWaves * kWave;
@@ -392,16 +462,29 @@ __device__ __forceinline__ void rasterize_backward_group(const float4* __restric
   Replay st{};
   uint32_t last = 0;
   float final_T = 0.f;
+  float4 color = make_float4(0.f, 0.f, 0.f, 0.f);
   if (inside) {
     const uint32_t index = py * width + px;
-    final_T = pixel_color[index].w;
+    color = pixel_color[index];
+    final_T = color.w;
     last = pixel_last[index];
     const float4 pg = pixel_grads[index];
     st.gr = pg.x; st.gg = pg.y; st.gb = pg.z;
     if (kFoot) st.loss = pixel_loss[index];
   }
   st.T = final_T;
-  st.final_bg = final_T * (bg_r * st.gr + bg_g * st.gg + bg_b * st.gb);
+  st.behind = bg_r * st.gr + bg_g * st.gg + bg_b * st.gb;  // behind the last blended splat: the background
+  // The replayed positions [lo, hi), relative to the list.
+  const uint32_t lo = part.ckpt ? part.lo - run.x : 0u;
+  const uint32_t hi = part.ckpt ? part.hi - run.x : run.y - run.x;
+  if (part.ckpt && last > hi) {
+    const float4 c = part.ckpt[static_cast<size_t>(part.hi >> part.shift) * (4 * kChunk) + wt.quad * kChunk + lane];
+    // The pixel's colour (background included) less what the splats in front of hi
+    // gave, through their transmittance T: everything behind them.
+    st.T = c.x;
+    st.behind = ((color.x - c.y) * st.gr + (color.y - c.z) * st.gg + (color.z - c.w) * st.gb) / c.x;
+    last = hi;
+  }
   const uint32_t my_last = wave_max(last);
   if (lane == 0) s_last[wave] = my_last;
   __syncthreads();
@@ -411,14 +494,16 @@ __device__ __forceinline__ void rasterize_backward_group(const float4* __restric
   const int row = lane >> 4;
   const bool row_leader = (lane & 15) == 0;
 
-  for (int c = static_cast<int>(ceil_div(tile_last, kRound)) - 1; c >= 0; c--) {
-    const uint32_t round = run.x + static_cast<uint32_t>(c) * kRound;
+  const uint32_t end = run.x + hi;
+  const int rounds = tile_last > lo ? static_cast<int>(ceil_div(tile_last - lo, kRound)) : 0;
+  for (int c = rounds - 1; c >= 0; c--) {
+    const uint32_t round = run.x + lo + static_cast<uint32_t>(c) * kRound;
     // The round's splats are staged once for every wave, 64 per staging wave.
     for (int q = wave; q < kPer; q += kWaves) {
       const uint32_t k = round + q * kChunk + lane;
       Staged st{};
       float4 conic = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (k < run.y) stage<kCull>(splats, pair_splats, k, st, &conic, tiles_x, tiles_y);
+      if (k < end) stage<kCull>(splats, pair_splats, k, st, &conic, tiles_x, tiles_y);
       s_index[q * kChunk + lane] = st.index;
       sh.conic[q * kChunk + lane] = conic;
       sh.box[q * kChunk + lane] = make_float4(st.x0, st.x1, st.y0, st.y1);
@@ -435,7 +520,7 @@ __device__ __forceinline__ void rasterize_backward_group(const float4* __restric
     for (int q = 0; q < kPer; q++) {
       const uint32_t k = round + q * kChunk + lane;
       const float4 b = sh.box[q * kChunk + lane];
-      hits[q] = k < run.y && k - run.x < my_last && b.y >= bx0 && b.x <= bx1 && b.w >= by0 && b.z <= by1;
+      hits[q] = k < end && k - run.x < my_last && b.y >= bx0 && b.x <= bx1 && b.w >= by0 && b.z <= by1;
       if (kCull) {
         const uint4 r = sh.rect[q * kChunk + lane];
         hits[q] = hits[q] && wt.sx >= r.x && wt.sx < r.z && wt.sy >= r.y && wt.sy < r.w;

@@ -232,10 +232,12 @@ __global__ __launch_bounds__(kThreads) void find_ranges_kernel(const uint32_t* _
 constexpr uint32_t kOrderThreads = 1024, kOrderStep = 8;
 
 __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint2* __restrict__ ranges, uint32_t tiles,
-                                                                   uint32_t* __restrict__ order) {
+                                                                   uint32_t* __restrict__ order,
+                                                                   uint32_t* __restrict__ clear) {
   __shared__ uint32_t s_next[kOrderThreads];
   __shared__ uint32_t s_wave[kOrderThreads / 64];
   const uint32_t tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  if (clear && tid <= kPartClasses) clear[tid] = 0;
   auto bucket = [&](uint32_t tile) {  // 0 for the heaviest
     const uint2 r = ranges[tile];
     return kOrderThreads - 1 - min((r.y - r.x) / kOrderStep, kOrderThreads - 1);
@@ -281,10 +283,10 @@ __global__ __launch_bounds__(256) void rasterize_kernel(const float4* __restrict
                                                          uint32_t height, uint32_t tiles_x, float bg_r, float bg_g,
                                                          float bg_b, uint32_t* __restrict__ target,
                                                          float4* __restrict__ pixel_color,
-                                                         uint32_t* __restrict__ pixel_last) {
+                                                         uint32_t* __restrict__ pixel_last, Segments seg) {
   const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
   raster::rasterize_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width, height, bg_r,
-                               bg_g, bg_b, target, pixel_color, pixel_last);
+                               bg_g, bg_b, target, pixel_color, pixel_last, seg);
 }
 
 // Persistent workgroups (16-px tiles): a grid of as many workgroups as the GPU holds
@@ -381,6 +383,39 @@ __global__ __launch_bounds__(256) void rasterize_backward_persistent_kernel(
     raster::rasterize_backward_tile<kPer>(splats, pair_splats, ranges[tile], tile % tiles_x, tile / tiles_x, width,
                                           height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads,
                                           shared);
+  }
+}
+
+// The segmented backward: each workgroup replays the items blockIdx.x,
+// blockIdx.x + gridDim.x, ... of `items` (see Segments), with the footprint terms
+// too when kFoot.
+template <int kPer, bool kFoot>
+__global__ __launch_bounds__(256) void rasterize_backward_segment_kernel(
+    const float4* __restrict__ splats, const uint32_t* __restrict__ pair_splats, const uint2* __restrict__ ranges,
+    Segments seg, uint32_t width, uint32_t height, uint32_t tiles_x, float bg_r, float bg_g, float bg_b,
+    const float4* __restrict__ pixel_color, const uint32_t* __restrict__ pixel_last,
+    const float4* __restrict__ pixel_grads, float* __restrict__ splat_grads, const float* __restrict__ pixel_loss,
+    float* __restrict__ footprint) {
+  __shared__ raster::BackwardShared<kPer, 4, false, kFoot> shared;
+  const uint2* items = reinterpret_cast<const uint2*>(seg.items);
+  uint32_t count[1 + kPartClasses], total = 0;
+#pragma unroll
+  for (uint32_t c = 0; c <= kPartClasses; c++) {
+    count[c] = min(seg.counts[c], c ? seg.part_room : seg.whole_room);
+    total += count[c];
+  }
+  for (uint32_t i = blockIdx.x; i < total; i += gridDim.x) {
+    // Item i in class order: whole segments, then the short parts, longest first.
+    uint32_t k = i, c = 0;
+    while (k >= count[c]) k -= count[c++];
+    const uint2 item = items[c ? seg.whole_room + (c - 1) * seg.part_room + k : k];
+    const uint2 run = ranges[item.x];
+    const uint32_t lo = max(run.x, item.y << seg.shift), hi = min(run.y, (item.y << seg.shift) + (1u << seg.shift));
+    raster::rasterize_backward_group<kPer, 4, false, kFoot>(
+        splats, pair_splats, run,
+        raster::WaveTile{item.x % tiles_x, item.x / tiles_x, static_cast<uint32_t>(threadIdx.x / kWave)}, width,
+        height, bg_r, bg_g, bg_b, pixel_color, pixel_last, pixel_grads, splat_grads, shared, pixel_loss, footprint,
+        raster::Part{lo, hi, reinterpret_cast<const float4*>(seg.ckpt), seg.shift});
   }
 }
 
@@ -562,14 +597,16 @@ void duplicate_batch(hipStream_t s, const float* splats, const uint32_t* order, 
 }
 
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
-                 uint32_t tiles, uint32_t* ranges, uint32_t* order, uint32_t pair_hint) {
+                 uint32_t tiles, uint32_t* ranges, uint32_t* order, uint32_t pair_hint, uint32_t* clear) {
   if (tiles == 0) return;
   find_ranges_kernel<<<hinted_groups(capacity, pair_hint, kThreads), kThreads, 0, s>>>(
       pair_tiles, pair_total, capacity, tiles, reinterpret_cast<uint2*>(ranges));
   DIO_LAUNCHED("find_ranges");
   if (order) {
-    tile_order_kernel<<<1, kOrderThreads, 0, s>>>(reinterpret_cast<const uint2*>(ranges), tiles, order);
+    tile_order_kernel<<<1, kOrderThreads, 0, s>>>(reinterpret_cast<const uint2*>(ranges), tiles, order, clear);
     DIO_LAUNCHED("tile_order");
+  } else if (clear) {
+    fill_u32(s, clear, 0u, 1 + kPartClasses);
   }
 }
 
@@ -595,9 +632,11 @@ uint32_t resident_groups(K kernel, size_t lds) {
 
 void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, const uint32_t* ranges,
                const uint32_t* order, uint32_t width, uint32_t height, const float* background, uint32_t* target,
-               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch, uint32_t tile_size, uint32_t* work) {
+               float* pixel_color, uint32_t* pixel_last, uint32_t splat_batch, uint32_t tile_size, uint32_t* work,
+               const Segments* seg) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
+  if (seg && (tile_size != 16 || work)) throw Error(-1, "the segmented backward runs on 16-pixel tiles, one workgroup each");
   if (work) {
     if (tile_size != 16) throw Error(-1, "persistent workgroups rasterize 16-pixel tiles");
     auto persist = [&](auto kernel) {
@@ -623,11 +662,17 @@ void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, 
                                       background[1], background[2], target, reinterpret_cast<float4*>(pixel_color),
                                       pixel_last);
   };
+  auto tiles16 = [&](auto kernel) {
+    kernel<<<tx * ty, 256, 0, s>>>(reinterpret_cast<const float4*>(splats), pair_splats,
+                                   reinterpret_cast<const uint2*>(ranges), order, width, height, tx, background[0],
+                                   background[1], background[2], target, reinterpret_cast<float4*>(pixel_color),
+                                   pixel_last, seg ? *seg : Segments{});
+  };
   const uint32_t key = tile_size * 1000 + splat_batch;
   switch (key) {
-    case 16064: launch(rasterize_kernel<1>, tx * ty, 256, tx); break;
-    case 16128: launch(rasterize_kernel<2>, tx * ty, 256, tx); break;
-    case 16256: launch(rasterize_kernel<4>, tx * ty, 256, tx); break;
+    case 16064: tiles16(rasterize_kernel<1>); break;
+    case 16128: tiles16(rasterize_kernel<2>); break;
+    case 16256: tiles16(rasterize_kernel<4>); break;
     case 8064: launch(rasterize_quad_kernel<1>, 4 * tx * ty, 64, tx); break;
     case 8128: launch(rasterize_quad_kernel<2>, 4 * tx * ty, 64, tx); break;
     case 8256: launch(rasterize_quad_kernel<4>, 4 * tx * ty, 64, tx); break;
@@ -644,6 +689,8 @@ void rasterize(hipStream_t s, const float* splats, const uint32_t* pair_splats, 
 // CU's 160 KB.
 constexpr uint32_t kBackwardGroupsPerCu = 5;
 
+// The segmented backward measured the same at 4 to 8 workgroups per CU (323-327 µs
+// with 512-pair segments) and keeps the same pad.
 template <int kPer, bool kFoot = false>
 constexpr size_t backward_lds_pad() {
   constexpr size_t group = 160 * 1024 / (kBackwardGroupsPerCu + 1) + 1;
@@ -655,9 +702,32 @@ void rasterize_backward(hipStream_t s, const float* splats, const uint32_t* pair
                         const uint32_t* order, uint32_t width, uint32_t height, const float* background,
                         const float* pixel_color, const uint32_t* pixel_last, const float* pixel_grads,
                         float* splat_grads, uint32_t splat_batch, uint32_t tile_size, const float* pixel_loss,
-                        float* footprint, uint32_t* work) {
+                        float* footprint, uint32_t* work, const Segments* seg) {
   const uint32_t tx = ceil_div(width, kTile), ty = ceil_div(height, kTile);
   if (tx * ty == 0) return;
+  if (seg) {
+    if (tile_size != 16 || work) throw Error(-1, "the segmented backward runs on 16-pixel tiles, one workgroup each");
+    if (seg->shift < kSegmentShiftMin || seg->shift > kSegmentShiftMax)
+      throw Error(-1, "segment length must be 256, 512 or 1024 pairs");
+    auto run = [&](auto kernel, size_t pad) {
+      kernel<<<std::max(seg->grid, 1u), 256, pad, s>>>(
+          reinterpret_cast<const float4*>(splats), pair_splats, reinterpret_cast<const uint2*>(ranges), *seg, width,
+          height, tx, background[0], background[1], background[2], reinterpret_cast<const float4*>(pixel_color),
+          pixel_last, reinterpret_cast<const float4*>(pixel_grads), splat_grads, pixel_loss, footprint);
+    };
+    const uint32_t key = splat_batch * 2 + (footprint ? 1 : 0);
+    switch (key) {
+      case 128: run(rasterize_backward_segment_kernel<1, false>, backward_lds_pad<1>()); break;
+      case 256: run(rasterize_backward_segment_kernel<2, false>, backward_lds_pad<2>()); break;
+      case 512: run(rasterize_backward_segment_kernel<4, false>, backward_lds_pad<4>()); break;
+      case 129: run(rasterize_backward_segment_kernel<1, true>, backward_lds_pad<1, true>()); break;
+      case 257: run(rasterize_backward_segment_kernel<2, true>, backward_lds_pad<2, true>()); break;
+      case 513: run(rasterize_backward_segment_kernel<4, true>, backward_lds_pad<4, true>()); break;
+      default: throw Error(-1, "splat batch must be 64, 128 or 256");
+    }
+    DIO_LAUNCHED("rasterize_backward_segment");
+    return;
+  }
   if (work && !footprint) {
     if (tile_size != 16) throw Error(-1, "persistent workgroups rasterize 16-pixel tiles");
     auto persist = [&](auto kernel, size_t pad) {

@@ -509,7 +509,8 @@ class Trainer {
     return native.trainer_get_tuning(this.handle);
   }
   set tuning(k) {
-    native.trainer_set_tuning(this.handle, [k.tileSize || 16, k.splatBatch || 64, k.digitBits || 8, k.persistent || 0]);
+    native.trainer_set_tuning(this.handle, [k.tileSize || 16, k.splatBatch || 64, k.digitBits || 8, k.persistent || 0,
+      k.segment || 0]);
   }
   /** Times every candidate setting over `steps` training steps and keeps the fastest. */
   autotune(steps) {

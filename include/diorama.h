@@ -65,7 +65,14 @@ typedef struct dio_tuning {
    * workgroups as the GPU holds at once, each taking the next tile (heaviest first)
    * from a counter until none is left */
   uint32_t persistent;
-} dio_tuning;
+  /* 16-px tiles, one workgroup each, single-view trainers: 0 replays each tile's
+   * list in one backward workgroup; 256, 512 or 1024 splits the lists at the pair
+   * positions that are multiples of it, the forward pass keeping every pixel's
+   * transmittance and colour there, and the backward replays the parts (longest
+   * first) in workgroups of their own. Images are the same; gradients agree to
+   * float rounding. */
+  uint32_t segment;
+} dio_tuning; /* a trainer starts at {16, 64, 8, 0, 512}; a batch trainer at {16, 64, 8, 0, 0} */
 
 typedef struct dio_tuning_result {
   dio_tuning tuning;
@@ -284,7 +291,8 @@ void dio_trainer_destroy(dio_trainer* t);
 int dio_trainer_set_tuning(dio_trainer* t, const dio_tuning* tuning);
 int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out);
 /* The autotuner: for every candidate (tile size 16, 8, 32; splat batch 64, 128,
- * 256; digit width 4, 8; persistent 0, 1 with 16-px tiles; see dio_tuning) run one
+ * 256; digit width 4, 8; persistent 0, 1 with 16-px tiles; segment 0, 256, 512
+ * with 16-px tiles and a workgroup per tile; see dio_tuning) run one
  * untimed optimize() step, then time `steps` more
  * (ordinary training steps; a batch trainer replays its step graph on the same
  * views), keep the fastest setting, and list up to `capacity` results. */

@@ -170,7 +170,11 @@ def config1():
     return ctx, cam, image, params, sh, scene.scene_extent(train)
 
 
-def test_config1_step_against_oracle(config1):
+@pytest.mark.parametrize("segment", [0, 256])
+def test_config1_step_against_oracle(config1, segment):
+    """segment 256: the backward replays the lists in parts of 256 pairs, each pixel
+    whose blended splats run past a part's end starting there from the forward
+    pass's checkpoint; the gradients meet the same bar against the oracle."""
     ctx, cam, image, params, sh, extent = config1
     n = len(params)
     w, h = 980, 545
@@ -179,9 +183,10 @@ def test_config1_step_against_oracle(config1):
     t = diorama_amd.Trainer(ctx, params, sh, [cam], [image], settings)
     t.iteration = 15000
     t.sh_degree = 3
+    t.tuning = dict(t.tuning, segment=segment)
     r = t.renderer
     act0 = r.gaussian_buffer.download(np.float32).reshape(n, 16)
-    say("config1 step")
+    say("config1 step, segment", segment)
     t.optimize()
     r.tiles.after_submit()
     gpu_loss = t.last_loss()
@@ -193,6 +198,15 @@ def test_config1_step_against_oracle(config1):
     st_gpu = t.buffers["stats"].download(np.float32).reshape(n, 4).astype(np.float64)
     t.destroy()
     print(f"config1 step: {n} Gaussians, {len(pairs)} pairs, {(splats[:, 3] > 0).sum()} visible")
+    if segment:
+        # Pixels that start a part from a checkpoint: their blended splats cross a
+        # multiple of the segment length inside their tile's list.
+        tx = (w + 15) // 16
+        py, px = np.divmod(np.arange(w * h), w)
+        start = ranges[(py // 16) * tx + px // 16, 0].astype(np.int64)
+        crossing = (start + last.astype(np.int64) - 1) // segment > start // segment
+        print(f"config1 segment {segment}: {int(crossing.sum())} pixels resume from a checkpoint")
+        assert crossing.sum() > 10_000
 
     # Stage by stage, given the GPU's own splats.
     o_splats = oracle.preprocess(act0, cam, sh, 3)
