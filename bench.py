@@ -62,9 +62,16 @@ STAGE_BYTES = {
 STAGE_PARTS = {"sort": ("depth_sort", "tile_sort")}
 # The reference's per-step byte model (separate preprocess backward, stats and Adam).
 STEP_BYTES = (2612, 240, 88)
-# Stage name -> kernel name, for the PMC traffic and VALU lines.
+# Stage name -> kernel name, for the PMC traffic, VALU and occupancy lines (the
+# backward's kernel depends on the segment setting, see stage_kernel).
 STAGE_KERNEL = {"preprocess": "preprocess_kernel", "rasterize": "rasterize_kernel",
                 "rasterize_backward": "rasterize_backward_kernel", "backward_adam": "backward_adam_kernel"}
+
+
+def stage_kernel(stage, tuning):
+    if stage == "rasterize_backward" and tuning.get("segment"):
+        return "rasterize_backward_segment_kernel"
+    return STAGE_KERNEL.get(stage, stage)
 # VALU issue: a wave64 instruction occupies a SIMD for 4 cycles; 1024 SIMDs at 2.4 GHz.
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 TRAFFIC_FILE = "pmc_traffic_train_step_1.1M_980x545.txt"
@@ -615,6 +622,19 @@ def measured_valu(kernel):
     return None
 
 
+def measured_occupancy(kernel):
+    """Achieved waves per SIMD of `kernel` from the committed PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == "occupancy" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE and
     WRITE_SIZE already scaled by the copy calibration), or None."""
@@ -735,11 +755,12 @@ def main():
     step_gpu_ms = sum(totals.values()) / clocked
     dominant = max(STAGE_KERNEL, key=lambda k: stages[k]["ms"])
     d = stages[dominant]
+    tuning_now = trainer.tuning
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(d["GB/s"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(d["GB/s"] / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(STAGE_KERNEL.get(dominant, dominant)),
+                "traffic": measured_traffic(stage_kernel(dominant, tuning_now)),
                 "algorithmic_bytes": d["bytes"]}
-    valu = measured_valu(STAGE_KERNEL.get(dominant, dominant))
+    valu = measured_valu(stage_kernel(dominant, tuning_now))
     if valu:
         # The rasterizer passes are bound by VALU issue, not HBM: the share of the
         # issue limit their PMC-counted instructions take at the measured time.
@@ -784,6 +805,10 @@ def main():
                         "parts_us": {k: round(totals[k] / clocked * 1000, 1) for k in totals}},
         "stages_hbm_frac": {k: round(v["GB/s"] / HBM_PEAK_GBS, 4) for k, v in stages.items()},
         "roofline": roofline,
+        # Achieved occupancy of the rasterizer passes, from the committed PMC summary
+        # (SQ_WAVE_CYCLES over GRBM_GUI_ACTIVE per CU; 8 waves per SIMD at most).
+        "rasterizer_waves_per_simd": {k: measured_occupancy(stage_kernel(k, trainer_tuning))
+                                      for k in ("rasterize", "rasterize_backward")},
         "cpu_baseline": cpu,
         "held_out_psnr_delta": delta,
     }
