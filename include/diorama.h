@@ -295,7 +295,9 @@ int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out);
  * with 16-px tiles and a workgroup per tile; see dio_tuning) run one
  * untimed optimize() step, then time `steps` more
  * (ordinary training steps; a batch trainer replays its step graph on the same
- * views), keep the fastest setting, and list up to `capacity` results. */
+ * views); a single-view trainer then times its four fastest again over 3 x steps
+ * (their listed times are those). Keep the fastest setting, and list up to
+ * `capacity` results. */
 int dio_trainer_autotune(dio_trainer* t, uint32_t steps, dio_tuning_result* results, uint32_t capacity,
                          uint32_t* count);
 
