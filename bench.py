@@ -59,7 +59,7 @@ STAGE_BYTES = {
     "backward_adam": (1668, 0, 0, 0),
 }
 # Clocked stages summed into a reported one.
-STAGE_PARTS = {"sort": ("depth_sort", "tile_sort")}
+STAGE_PARTS = {"sort": ("depth_sort", "tile_sort", "list_order")}
 # The reference's per-step byte model (separate preprocess backward, stats and Adam).
 STEP_BYTES = (2612, 240, 88)
 # Stage name -> kernel name, for the PMC traffic, VALU and occupancy lines (the
@@ -660,8 +660,8 @@ def main():
                     help="train: skip the shortened learned-vs-plain held-out PSNR comparison")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch settings")
     ap.add_argument("--tuning", default=None,
-                    help="train: tile_size,splat_batch,digit_bits[,persistent[,segment]] to use instead of "
-                         "autotuning (e.g. 32,128,8 or 16,64,8,1 or 16,64,8,0,256)")
+                    help="train: tile_size,splat_batch,digit_bits[,persistent[,segment[,per_tile_depth]]] to use "
+                         "instead of autotuning (e.g. 32,128,8 or 16,64,8,1 or 16,64,8,0,256 or 16,64,8,0,512,1)")
     ap.add_argument("--workload", choices=["train", "scan-sort", "batch", "learned", "growth"], default="train",
                     help="train: the training step (default); scan-sort: the scan/sort microbenchmark; "
                          "batch: the multi-view batch trainer; learned: learned vs plain densification; "
@@ -712,12 +712,13 @@ def main():
     if args.tuning:
         v = [int(x) for x in args.tuning.split(",")]
         trainer.tuning = {"tile_size": v[0], "splat_batch": v[1], "digit_bits": v[2],
-                          "persistent": v[3] if len(v) > 3 else 0, "segment": v[4] if len(v) > 4 else 0}
+                          "persistent": v[3] if len(v) > 3 else 0, "segment": v[4] if len(v) > 4 else 0,
+                          "per_tile_depth": v[5] if len(v) > 5 else 0}
     elif not args.no_autotune:
         tuned = trainer.autotune(steps=3)
         log("autotune: " + ", ".join(f"tile {k['tile_size']}{'p' if k['persistent'] else ''} batch "
-                                     f"{k['splat_batch']} digits {k['digit_bits']} segment {k['segment']}: "
-                                     f"{ms:.3f} ms" for k, ms in tuned))
+                                     f"{k['splat_batch']} digits {k['digit_bits']} segment {k['segment']} "
+                                     f"per-tile depth {k['per_tile_depth']}: {ms:.3f} ms" for k, ms in tuned))
     ctx.synchronize()
     log(f"warmup {args.warmup} steps done; tuning {trainer.tuning}")
     t0 = time.perf_counter()
@@ -777,7 +778,8 @@ def main():
     trainer.destroy()
     delta = None if args.no_learned else short_psnr_delta(ctx, train, images, held, held_images)
     autotune_table = None if tuned is None else [[k["tile_size"], k["splat_batch"], k["digit_bits"], k["persistent"],
-                                                  k["segment"], round(ms, 4)] for k, ms in tuned]
+                                                  k["segment"], k["per_tile_depth"], round(ms, 4)]
+                                                 for k, ms in tuned]
     ctx.close()
     line = {
         "metric": METRIC,

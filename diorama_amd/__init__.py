@@ -32,7 +32,8 @@ class _Camera(ctypes.Structure):
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("tile_size", ctypes.c_uint32), ("splat_batch", ctypes.c_uint32), ("digit_bits", ctypes.c_uint32),
-                ("persistent", ctypes.c_uint32), ("segment", ctypes.c_uint32)]
+                ("persistent", ctypes.c_uint32), ("segment", ctypes.c_uint32),
+                ("per_tile_depth", ctypes.c_uint32)]
 
 
 class _TuningResult(ctypes.Structure):
@@ -695,24 +696,24 @@ class Trainer:
         k = _Tuning()
         _check(lib().dio_trainer_get_tuning(self.handle, ctypes.byref(k)))
         return {"tile_size": k.tile_size, "splat_batch": k.splat_batch, "digit_bits": k.digit_bits,
-                "persistent": k.persistent, "segment": k.segment}
+                "persistent": k.persistent, "segment": k.segment, "per_tile_depth": k.per_tile_depth}
 
     @tuning.setter
     def tuning(self, value):
         k = _Tuning(value.get("tile_size", 16), value.get("splat_batch", 64), value.get("digit_bits", 8),
-                    value.get("persistent", 0), value.get("segment", 0))
+                    value.get("persistent", 0), value.get("segment", 0), value.get("per_tile_depth", 0))
         _check(lib().dio_trainer_set_tuning(self.handle, ctypes.byref(k)))
 
     def autotune(self, steps=3):
         """Times every candidate setting over `steps` training steps, keeps the fastest;
         returns [(tuning, ms per step)]."""
-        out = (_TuningResult * 64)()
+        out = (_TuningResult * 128)()
         n = _U32()
-        _check(lib().dio_trainer_autotune(self.handle, steps, out, 64, ctypes.byref(n)))
+        _check(lib().dio_trainer_autotune(self.handle, steps, out, 128, ctypes.byref(n)))
         return [({"tile_size": r.tuning.tile_size, "splat_batch": r.tuning.splat_batch,
                   "digit_bits": r.tuning.digit_bits, "persistent": r.tuning.persistent,
-                  "segment": r.tuning.segment}, r.ms_per_step)
-                for r in out[:min(n.value, 64)]]
+                  "segment": r.tuning.segment, "per_tile_depth": r.tuning.per_tile_depth}, r.ms_per_step)
+                for r in out[:min(n.value, 128)]]
 
     def samples(self):
         f, l, n = _P(), _P(), _U32()

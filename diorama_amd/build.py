@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdiorama_hip.so")
-SOURCES = ["scan.hip", "sort.hip", "render.hip", "batch.hip", "loss.hip", "optim.hip", "mlp.hip", "ply.hip", "knn.hip", "metrics.hip", "api.cpp", "io.cpp"]
+SOURCES = ["scan.hip", "sort.hip", "render.hip", "depth_order.hip", "batch.hip", "loss.hip", "optim.hip", "mlp.hip", "ply.hip", "knn.hip", "metrics.hip", "api.cpp", "io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",

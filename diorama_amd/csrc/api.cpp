@@ -350,10 +350,10 @@ namespace {
 // k - 1 (kStageNames) and opens stage k.
 enum StageMark : uint32_t {
   kMarkStart, kMarkPreprocess, kMarkDepthSort, kMarkScan, kMarkDuplicate, kMarkTileSort, kMarkRanges,
-  kMarkRasterize, kMarkLoss, kMarkClearGrads, kMarkBackward, kMarkBackwardAdam, kMarks
+  kMarkListOrder, kMarkRasterize, kMarkLoss, kMarkClearGrads, kMarkBackward, kMarkBackwardAdam, kMarks
 };
 constexpr const char* kStageNames[kMarks - 1] = {
-    "preprocess", "depth_sort", "scan", "duplicate", "tile_sort", "ranges",
+    "preprocess", "depth_sort", "scan", "duplicate", "tile_sort", "ranges", "list_order",
     "rasterize", "loss", "clear_grads", "rasterize_backward", "backward_adam"};
 
 // HIP events between the stages of clocked steps, summed per stage, and each step's
@@ -474,6 +474,9 @@ struct dio_tile_rasterizer {
   uint32_t splat_batch = 64, digit_bits = 8;  // tuning; results do not depend on them
   uint32_t tile_size = 16;  // tuning: 8, 16 or 32 (32: pairs binned by 32-px bins)
   bool persistent = false;  // tuning: persistent rasterizer workgroups (16-px tiles)
+  // Tuning: pairs written in splat order and binned by tile, then each tile's list
+  // sorted by depth (no depth sort of all the splats); the lists are the same.
+  bool per_tile_depth = false;
   dio_buf* work = nullptr;  // their two tile counters, zeroed between launches
   // Tuning: the segmented backward's segment length 2^segment_shift (0: a workgroup
   // per tile); framed_shift is the one the last forward pass wrote checkpoints for.
@@ -628,8 +631,11 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
     dio::count_tiles(s, splats, t->count, w, h, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
                      dev<uint32_t>(t->depth_values), shift);
   // Depth order of the splats (culled ones last), with their tile counts in that
-  // order, then each splat's first pair slot.
-  if (dio::radix_sort(s, false, t->depth_keys->ptr, dev<uint32_t>(t->depth_values), t->depth_alt_keys->ptr,
+  // order, then each splat's first pair slot. Per-tile depth order skips the sort:
+  // pairs are written in splat order and each tile's list is sorted after binning.
+  const bool per_tile = t->per_tile_depth;
+  if (!per_tile &&
+      dio::radix_sort(s, false, t->depth_keys->ptr, dev<uint32_t>(t->depth_values), t->depth_alt_keys->ptr,
                       dev<uint32_t>(t->depth_alt_values), nullptr, t->count, 0, 32,
                       dio::sort_workspace(t->splat_sort_ws->ptr, t->count), t->digit_bits,
                       dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->sorted_counts)))
@@ -638,15 +644,16 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
     if (t->clock) t->clock->mark(s, k);
   };
   mark(kMarkDepthSort);
-  dio::scan_exclusive(s, dev<uint32_t>(t->sorted_counts), dev<uint32_t>(t->offsets), nullptr, t->count,
-                      dev<uint32_t>(t->pair_total), dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
+  dio::scan_exclusive(s, dev<uint32_t>(per_tile ? t->tile_counts : t->sorted_counts), dev<uint32_t>(t->offsets),
+                      nullptr, t->count, dev<uint32_t>(t->pair_total),
+                      dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
   mark(kMarkScan);
   // The pair count is known on the device only; the last frame's (0 before the first
   // readback) sizes the grids of the pair kernels, which loop over whatever it is.
   const uint32_t pair_hint = t->last_pair_count;
-  dio::duplicate(s, splats, dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets), dev<uint32_t>(t->tile_counts),
-                 t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total), dev<uint32_t>(t->pair_tiles),
-                 dev<uint32_t>(t->pair_splats), pair_hint, shift);
+  dio::duplicate(s, splats, per_tile ? nullptr : dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets),
+                 dev<uint32_t>(t->tile_counts), t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total),
+                 dev<uint32_t>(t->pair_tiles), dev<uint32_t>(t->pair_splats), pair_hint, shift);
   mark(kMarkDuplicate);
   t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats), t->alt_tiles->ptr,
                                      dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total), t->capacity, 0,
@@ -663,6 +670,10 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
                    dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), pair_hint,
                    segments ? segments->counts : nullptr);
   mark(kMarkRanges);
+  if (per_tile)
+    dio::depth_order(s, dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), tiles,
+                     dev<uint32_t>(sorted_splats), dev<uint32_t>(t->sorted_in_alt ? t->pair_splats : t->alt_splats));
+  mark(kMarkListOrder);
   dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), w,
                  h, bg, dev<uint32_t>(target), dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
                  t->splat_batch, t->tile_size, t->persistent ? dev<uint32_t>(t->work) : nullptr, segments);
@@ -1349,6 +1360,7 @@ void trainer_optimize(dio_trainer* t) {
   tiles->digit_bits = t->tuning.digit_bits;
   tiles->tile_size = t->tuning.tile_size;
   tiles->persistent = t->tuning.persistent != 0;
+  tiles->per_tile_depth = t->tuning.per_tile_depth != 0;
   tiles->segment_shift = t->tuning.segment ? static_cast<uint32_t>(__builtin_ctz(t->tuning.segment)) : 0u;
   if (t->clock.enabled) {
     t->clock.begin(t->ctx->stream);
@@ -2275,36 +2287,48 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       };
     } else if (name == "scan") {
       run = [&] {
-        dio::scan_exclusive(s, dev<uint32_t>(tr->sorted_counts), dev<uint32_t>(tr->offsets), nullptr, tr->count,
+        dio::scan_exclusive(s, dev<uint32_t>(tr->per_tile_depth ? tr->tile_counts : tr->sorted_counts),
+                            dev<uint32_t>(tr->offsets), nullptr, tr->count,
                             dev<uint32_t>(tr->pair_total), dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
       };
     } else if (name == "sort") {
-      // The whole binning sort (Gaussians by depth, then pairs by tile) on the data a
-      // step really sorts: the chain preprocess -> depth sort -> scan -> duplicate ->
-      // tile sort, less the same chain without the two sorts (timed apart below).
+      // The whole binning sort (Gaussians by depth, then pairs by tile; or pairs by
+      // tile, then each tile's list by depth) on the data a step really sorts: the
+      // chain preprocess -> depth sort -> scan -> duplicate -> tile sort (-> ranges ->
+      // list sort), less the same chain without the sorts (timed apart below).
       const uint32_t shift = tr->bin_shift();
       const uint32_t tiles = dio::ceil_div(dio::ceil_div(tr->frame_w, dio::kTile), 1u << shift) *
                              dio::ceil_div(dio::ceil_div(tr->frame_h, dio::kTile), 1u << shift);
+      const bool per_tile = tr->per_tile_depth;
+      const bool in_alt = (tile_sort_bits(tiles, tr->digit_bits) / tr->digit_bits) % 2 == 1;
       auto chain = [&](bool sorts) {
         dio::preprocess(s, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
                         dev<float>(sc->splats), dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_keys),
                         dev<uint32_t>(tr->depth_values));
-        if (sorts)
+        if (sorts && !per_tile)
           dio::radix_sort(s, false, tr->depth_keys->ptr, dev<uint32_t>(tr->depth_values), tr->depth_alt_keys->ptr,
                           dev<uint32_t>(tr->depth_alt_values), nullptr, tr->count, 0, 32,
                           dio::sort_workspace(tr->splat_sort_ws->ptr, tr->count), tr->digit_bits,
                           dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->sorted_counts));
-        dio::scan_exclusive(s, dev<uint32_t>(tr->sorted_counts), dev<uint32_t>(tr->offsets), nullptr, tr->count,
-                            dev<uint32_t>(tr->pair_total), dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
-        dio::duplicate(s, dev<float>(sc->splats), dev<uint32_t>(tr->depth_values), dev<uint32_t>(tr->offsets),
-                       dev<uint32_t>(tr->tile_counts), tr->count, tr->frame_w, tr->frame_h, tr->capacity,
-                       dev<uint32_t>(tr->pair_total), dev<uint32_t>(tr->pair_tiles), dev<uint32_t>(tr->pair_splats), 0,
-                       shift);
+        dio::scan_exclusive(s, dev<uint32_t>(per_tile ? tr->tile_counts : tr->sorted_counts), dev<uint32_t>(tr->offsets),
+                            nullptr, tr->count, dev<uint32_t>(tr->pair_total),
+                            dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
+        dio::duplicate(s, dev<float>(sc->splats), per_tile ? nullptr : dev<uint32_t>(tr->depth_values),
+                       dev<uint32_t>(tr->offsets), dev<uint32_t>(tr->tile_counts), tr->count, tr->frame_w, tr->frame_h,
+                       tr->capacity, dev<uint32_t>(tr->pair_total), dev<uint32_t>(tr->pair_tiles),
+                       dev<uint32_t>(tr->pair_splats), 0, shift);
         if (sorts)
           dio::radix_sort(s, false, tr->pair_tiles->ptr, dev<uint32_t>(tr->pair_splats), tr->alt_tiles->ptr,
                           dev<uint32_t>(tr->alt_splats), dev<uint32_t>(tr->pair_total), tr->capacity, 0,
                           tile_sort_bits(tiles, tr->digit_bits), dio::sort_workspace(tr->pair_sort_ws->ptr, tr->capacity),
                           tr->digit_bits);
+        if (sorts && per_tile) {
+          dio::find_ranges(s, dev<uint32_t>(in_alt ? tr->alt_tiles : tr->pair_tiles), dev<uint32_t>(tr->pair_total),
+                           tr->capacity, tiles, dev<uint32_t>(tr->ranges), dev<uint32_t>(tr->tile_order));
+          dio::depth_order(s, dev<uint32_t>(tr->depth_keys), dev<uint32_t>(tr->ranges), dev<uint32_t>(tr->tile_order),
+                           tiles, dev<uint32_t>(in_alt ? tr->alt_splats : tr->pair_splats),
+                           dev<uint32_t>(in_alt ? tr->pair_splats : tr->alt_splats));
+        }
       };
       hipEvent_t e0, e1, e2;
       DIO_HIP(hipEventCreate(&e0));
@@ -2698,23 +2722,28 @@ void check_tuning(const dio_trainer* t, const dio_tuning& k) {
   if (k.segment && (k.tile_size != 16 || k.persistent))
     invalid("the segmented backward runs on 16-pixel tiles, one workgroup each");
   if (k.segment && t->batch) invalid("a batch trainer replays whole lists");
+  if (k.per_tile_depth > 1) invalid("per_tile_depth is 0 or 1");
+  if (k.per_tile_depth && t->batch) invalid("a batch trainer sorts its splats by depth");
 }
 
-// The autotuner's candidates: every valid (tile size, persistent, segment, digit
-// width, splat batch); segments of 1024 pairs are left to explicit settings.
+// The autotuner's candidates: every valid (depth order, tile size, persistent,
+// segment, digit width, splat batch); segments of 1024 pairs are left to explicit
+// settings.
 std::vector<dio_tuning> tuning_candidates(const dio_trainer* t) {
   std::vector<dio_tuning> out;
-  for (uint32_t tile : {16u, 8u, 32u})
-    for (uint32_t persistent : {0u, 1u})
-      for (uint32_t segment : {0u, 256u, 512u})
-        for (uint32_t bits : {8u, 4u})
-          for (uint32_t batch : {64u, 128u, 256u}) {
-            if ((t->batch && tile != 16) || (t->learned.mode == 1 && tile != 16) || (tile == 32 && batch == 256))
-              continue;
-            if (persistent && (tile != 16 || t->batch)) continue;
-            if (segment && (tile != 16 || persistent || t->batch)) continue;
-            out.push_back(dio_tuning{tile, batch, bits, persistent, segment});
-          }
+  for (uint32_t per_tile : {0u, 1u})
+    for (uint32_t tile : {16u, 8u, 32u})
+      for (uint32_t persistent : {0u, 1u})
+        for (uint32_t segment : {0u, 256u, 512u})
+          for (uint32_t bits : {8u, 4u})
+            for (uint32_t batch : {64u, 128u, 256u}) {
+              if ((t->batch && tile != 16) || (t->learned.mode == 1 && tile != 16) || (tile == 32 && batch == 256))
+                continue;
+              if (persistent && (tile != 16 || t->batch)) continue;
+              if (segment && (tile != 16 || persistent || t->batch)) continue;
+              if (per_tile && t->batch) continue;
+              out.push_back(dio_tuning{tile, batch, bits, persistent, segment, per_tile});
+            }
   return out;
 }
 
@@ -2722,7 +2751,7 @@ void apply_tuning(dio_trainer* t, const dio_tuning& k) {
   check_tuning(t, k);
   const bool changed = k.splat_batch != t->tuning.splat_batch || k.digit_bits != t->tuning.digit_bits ||
                        k.tile_size != t->tuning.tile_size || k.persistent != t->tuning.persistent ||
-                       k.segment != t->tuning.segment;
+                       k.segment != t->tuning.segment || k.per_tile_depth != t->tuning.per_tile_depth;
   t->tuning = k;
   if (changed) t->generation++;  // a captured batch graph bakes the launches in
 }

@@ -147,6 +147,8 @@ void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width,
 
 // pair_hint (0: none), here and in find_ranges: the pair count the host expects,
 // which sizes the grid; the kernels loop over whatever *pair_total holds.
+// order (nullable): the splats in the order their pairs are written, offsets
+// following that order; null writes them in splat order.
 void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const uint32_t* offsets,
                const uint32_t* tile_counts, uint32_t n, uint32_t width, uint32_t height, uint32_t capacity,
                const uint32_t* pair_total, uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_hint = 0,
@@ -159,6 +161,14 @@ void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const 
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
                  uint32_t tiles, uint32_t* ranges, uint32_t* order = nullptr, uint32_t pair_hint = 0,
                  uint32_t* clear = nullptr);
+
+// Sorts each tile's run of `list` (pairs binned by tile, each run in splat order)
+// stably by depth_keys[splat], in place: the run of pairs a depth-ordered
+// duplicate would have produced. tile_order (nullable) is the launch order of the
+// tiles; scratch is a pair-sized buffer the runs longer than 2,048 pairs pass
+// through at their own positions.
+void depth_order(hipStream_t s, const uint32_t* depth_keys, const uint32_t* ranges, const uint32_t* tile_order,
+                 uint32_t tiles, uint32_t* list, uint32_t* scratch);
 
 // The segmented backward (16-px tiles). Segment boundaries are the absolute pair
 // positions that are multiples of 2^shift: the forward pass keeps each pixel's

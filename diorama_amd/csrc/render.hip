@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void duplicate_kernel(const float4* __res
       uint32_t start = 0xffffffffu, count = 0, splat = 0, r[4] = {0, 0, 0, 0};
       uint32_t row_tiles = tiles_x, base = 0;
       if (j < n) {
-        splat = order[j];
+        splat = order ? order[j] : j;
         start = offsets[j];
         count = tile_counts[splat];
         if (count) {

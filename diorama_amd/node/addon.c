@@ -991,6 +991,7 @@ static napi_value tuning_object(napi_env env, const dio_tuning* k) {
   napi_set_named_property(env, o, "digitBits", number(env, k->digit_bits));
   napi_set_named_property(env, o, "persistent", number(env, k->persistent));
   napi_set_named_property(env, o, "segment", number(env, k->segment));
+  napi_set_named_property(env, o, "perTileDepth", number(env, k->per_tile_depth));
   return o;
 }
 
@@ -1003,24 +1004,25 @@ FN(trainer_get_tuning) {
 FN(trainer_set_tuning) {
   ARGS;
   dio_tuning k;
-  float v[5];
-  if (!floats(&c, c.argv[1], v, 5))
-    return fail(env, "tuning must be [tileSize, splatBatch, digitBits, persistent, segment]");
+  float v[6];
+  if (!floats(&c, c.argv[1], v, 6))
+    return fail(env, "tuning must be [tileSize, splatBatch, digitBits, persistent, segment, perTileDepth]");
   k.tile_size = (uint32_t)v[0];
   k.splat_batch = (uint32_t)v[1];
   k.digit_bits = (uint32_t)v[2];
   k.persistent = (uint32_t)v[3];
   k.segment = (uint32_t)v[4];
+  k.per_tile_depth = (uint32_t)v[5];
   if (!ok(env, dio_trainer_set_tuning(handle(&c, 0), &k))) return NULL;
   return undefined(env);
 }
 FN(trainer_autotune) {
   ARGS;
-  dio_tuning_result r[64];
+  dio_tuning_result r[128];
   uint32_t n = 0;
   napi_value arr;
-  if (!ok(env, dio_trainer_autotune(handle(&c, 0), u32(&c, 1), r, 64, &n))) return NULL;
-  if (n > 64) n = 64;
+  if (!ok(env, dio_trainer_autotune(handle(&c, 0), u32(&c, 1), r, 128, &n))) return NULL;
+  if (n > 128) n = 128;
   napi_create_array_with_length(env, n, &arr);
   for (uint32_t k = 0; k < n; k++) {
     napi_value o = tuning_object(env, &r[k].tuning);

@@ -510,7 +510,7 @@ class Trainer {
   }
   set tuning(k) {
     native.trainer_set_tuning(this.handle, [k.tileSize || 16, k.splatBatch || 64, k.digitBits || 8, k.persistent || 0,
-      k.segment || 0]);
+      k.segment || 0, k.perTileDepth || 0]);
   }
   /** Times every candidate setting over `steps` training steps and keeps the fastest. */
   autotune(steps) {

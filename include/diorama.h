@@ -72,7 +72,14 @@ typedef struct dio_tuning {
    * first) in workgroups of their own. Images are the same; gradients agree to
    * float rounding. */
   uint32_t segment;
-} dio_tuning; /* a trainer starts at {16, 64, 8, 0, 512}; a batch trainer at {16, 64, 8, 0, 0} */
+  /* single-view trainers: 0 sorts the splats by depth before writing their pairs
+   * (src/tileRasterizer.ts:127-192); 1 writes the pairs in splat order, bins them by
+   * tile and sorts each tile's list by depth. The lists, and so images and
+   * gradients, are the same, except on a frame whose pairs overflow the pair
+   * buffers: with 1 it drops the pairs of the last splats in splat order rather
+   * than those of the farthest ones. */
+  uint32_t per_tile_depth;
+} dio_tuning; /* a trainer starts at {16, 64, 8, 0, 512, 0}; a batch trainer at {16, 64, 8, 0, 0, 0} */
 
 typedef struct dio_tuning_result {
   dio_tuning tuning;
@@ -292,7 +299,8 @@ int dio_trainer_set_tuning(dio_trainer* t, const dio_tuning* tuning);
 int dio_trainer_get_tuning(const dio_trainer* t, dio_tuning* out);
 /* The autotuner: for every candidate (tile size 16, 8, 32; splat batch 64, 128,
  * 256; digit width 4, 8; persistent 0, 1 with 16-px tiles; segment 0, 256, 512
- * with 16-px tiles and a workgroup per tile; see dio_tuning) run one
+ * with 16-px tiles and a workgroup per tile; per-tile depth order 0, 1 for
+ * single-view trainers; see dio_tuning) run one
  * untimed optimize() step, then time `steps` more
  * (ordinary training steps; a batch trainer replays its step graph on the same
  * views); a single-view trainer then times its four fastest again over 3 x steps

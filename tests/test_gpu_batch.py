@@ -164,7 +164,8 @@ def test_batch_training_learns():
 @pytest.mark.parametrize("batch", [0, 3])
 def test_tuning_changes_speed_not_results(batch):
     """Every tile size (8, 16, 32; the plain trainer), launch form (persistent,
-    segmented backward), splat batch and digit width gives the same images, and
+    segmented backward), depth order (all splats, or each tile's list; the plain
+    trainer), splat batch and digit width gives the same images, and
     gradients equal up to float rounding (the order of float atomics; a segment's
     replay starting from the forward's checkpoint); the autotuner keeps the fastest
     setting. 200x120 has 13 x 8 tiles: the 32-px bins of the last column and row are
@@ -175,9 +176,10 @@ def test_tuning_changes_speed_not_results(batch):
     images = teacher_images(ctx, n, cams, seed=21)
     params, sh = scene.random_raw_gaussians(n, 22, box=1.0)
     outs = {}
-    shapes = [(16, 0, 0)] if batch else [(16, 0, 0), (16, 1, 0), (8, 0, 0), (32, 0, 0), (16, 0, 256), (16, 0, 512),
-                                         (16, 0, 1024)]
-    for tile, persistent, segment in shapes:
+    shapes = [(16, 0, 0, 0)] if batch else [(16, 0, 0, 0), (16, 1, 0, 0), (8, 0, 0, 0), (32, 0, 0, 0), (16, 0, 256, 0),
+                                            (16, 0, 512, 0), (16, 0, 1024, 0), (16, 0, 0, 1), (16, 1, 0, 1),
+                                            (8, 0, 0, 1), (32, 0, 0, 1), (16, 0, 512, 1)]
+    for tile, persistent, segment, per_tile in shapes:
         for bits in (8, 4):
             for splats in (64, 128, 256):
                 if tile == 32 and splats == 256:
@@ -185,15 +187,15 @@ def test_tuning_changes_speed_not_results(batch):
                 t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(1.0), batch=batch)
                 t.sh_degree = 3
                 t.tuning = {"tile_size": tile, "splat_batch": splats, "digit_bits": bits, "persistent": persistent,
-                            "segment": segment}
-                say("tuning", batch, tile, persistent, segment, bits, splats)
+                            "segment": segment, "per_tile_depth": per_tile}
+                say("tuning", batch, tile, persistent, segment, per_tile, bits, splats)
                 t.optimize()
                 p = t.buffers["params"].download(np.float32).reshape(n, 16)
                 pix = None if batch else t.renderer.pixel_buffer.download(np.float32, 200 * 120 * 4)
                 sg = None if batch else t.renderer.gradients["splats"].download(np.float32).reshape(-1, 9)[:n]
-                outs[(tile, persistent, segment, bits, splats)] = (p, pix, t.last_loss(), sg)
+                outs[(tile, persistent, segment, per_tile, bits, splats)] = (p, pix, t.last_loss(), sg)
                 t.destroy()
-    ref_p, ref_pix, ref_loss, ref_sg = outs[(16, 0, 0, 8, 64)]
+    ref_p, ref_pix, ref_loss, ref_sg = outs[(16, 0, 0, 0, 8, 64)]
     for key, (p, pix, loss, sg) in outs.items():
         if pix is not None:
             assert np.array_equal(pix, ref_pix), key
@@ -204,11 +206,11 @@ def test_tuning_changes_speed_not_results(batch):
     t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(1.0), batch=batch)
     say("autotune", batch)
     results = t.autotune(steps=2)
-    assert len(results) == (6 if batch else 34)
+    assert len(results) == (6 if batch else 68)
     best = min(results, key=lambda r: r[1])
     assert t.tuning == best[0]
-    print("autotune:", [(r[0]["tile_size"], r[0]["persistent"], r[0]["segment"], r[0]["splat_batch"],
-                         r[0]["digit_bits"], round(r[1], 3)) for r in results])
+    print("autotune:", [(r[0]["tile_size"], r[0]["persistent"], r[0]["segment"], r[0]["per_tile_depth"],
+                         r[0]["splat_batch"], r[0]["digit_bits"], round(r[1], 3)) for r in results])
     with pytest.raises(diorama_amd.DioramaError):
         t.tuning = {"tile_size": 8 if batch else 12, "splat_batch": 64, "digit_bits": 8}
     with pytest.raises(diorama_amd.DioramaError):
@@ -216,7 +218,10 @@ def test_tuning_changes_speed_not_results(batch):
     with pytest.raises(diorama_amd.DioramaError):
         t.tuning = {"tile_size": 8, "splat_batch": 64, "digit_bits": 8, "persistent": 1}
     for bad in ({"tile_size": 16, "segment": 128}, {"tile_size": 8, "segment": 256},
-                {"tile_size": 16, "persistent": 1, "segment": 256}):
+                {"tile_size": 16, "persistent": 1, "segment": 256}, {"tile_size": 16, "per_tile_depth": 2}):
         with pytest.raises(diorama_amd.DioramaError):
             t.tuning = dict({"splat_batch": 64, "digit_bits": 8}, **bad)
+    if batch:
+        with pytest.raises(diorama_amd.DioramaError):
+            t.tuning = {"tile_size": 16, "splat_batch": 64, "digit_bits": 8, "per_tile_depth": 1}
     t.destroy()

@@ -170,11 +170,13 @@ def config1():
     return ctx, cam, image, params, sh, scene.scene_extent(train)
 
 
-@pytest.mark.parametrize("segment", [0, 256])
-def test_config1_step_against_oracle(config1, segment):
+@pytest.mark.parametrize("segment, per_tile", [(0, 0), (256, 0), (512, 1)])
+def test_config1_step_against_oracle(config1, segment, per_tile):
     """segment 256: the backward replays the lists in parts of 256 pairs, each pixel
     whose blended splats run past a part's end starting there from the forward
-    pass's checkpoint; the gradients meet the same bar against the oracle."""
+    pass's checkpoint; the gradients meet the same bar against the oracle.
+    per_tile 1: the pairs are written in splat order and each tile's list is sorted
+    by depth after binning; the lists are the oracle's all the same."""
     ctx, cam, image, params, sh, extent = config1
     n = len(params)
     w, h = 980, 545
@@ -183,10 +185,10 @@ def test_config1_step_against_oracle(config1, segment):
     t = diorama_amd.Trainer(ctx, params, sh, [cam], [image], settings)
     t.iteration = 15000
     t.sh_degree = 3
-    t.tuning = dict(t.tuning, segment=segment)
+    t.tuning = dict(t.tuning, segment=segment, per_tile_depth=per_tile)
     r = t.renderer
     act0 = r.gaussian_buffer.download(np.float32).reshape(n, 16)
-    say("config1 step, segment", segment)
+    say("config1 step, segment", segment, "per-tile depth order", per_tile)
     t.optimize()
     r.tiles.after_submit()
     gpu_loss = t.last_loss()

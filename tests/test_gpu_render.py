@@ -354,3 +354,64 @@ def test_many_more_pairs_than_the_last_frame():
     assert r.tiles.pair_capacity == 1 << 20
     check_frame_against_oracle(r, 160, 120, bg, "many more pairs")
     r.destroy()
+
+
+def test_long_lists_and_depth_ties_in_both_depth_orders():
+    """Tile lists of several thousand pairs and exact depth ties (copies of Gaussians),
+    with the splats sorted by depth before their pairs are written (per_tile_depth 0)
+    or each tile's list sorted by depth after binning (1; lists past 2,048 pairs pass
+    through global memory, in three digit passes where a list's depths share their top
+    byte and four where they do not; shorter lists are bucketed by depth in LDS, or
+    take the same radix passes when one bucket would hold more than 16 pairs): both
+    give the oracle's (tile, depth, splat) lists, and the same image."""
+    rng = np.random.default_rng(5)
+    w = h = 256
+    cam = scene.camera(scene.IDENTITY, scene.perspective(math.pi / 2, 1, 0.1, 100), w, h)
+
+    def cluster(n, depth, x, y):
+        p = np.zeros((n, 16), np.float32)
+        d = rng.uniform(*depth, n)
+        p[:, 0] = rng.uniform(*x, n) * d
+        p[:, 1] = rng.uniform(*y, n) * d
+        p[:, 2] = -d
+        p[:, 3] = rng.normal(0, 1, n)
+        p[:, 4:7] = rng.normal(-4.0, 0.4, (n, 3))
+        q = rng.normal(0, 1, (n, 4))
+        p[:, 8:12] = q / np.linalg.norm(q, axis=1, keepdims=True)
+        p[:, 12:15] = rng.normal(0, 1, (n, 3))
+        return p
+
+    near = cluster(60000, (1.2, 1.8), (-0.25, 0.25), (-0.25, 0.25))  # depths in [1, 2): one top byte
+    far = cluster(60000, (2.0, 12.0), (-0.9, -0.5), (-0.2, 0.2))
+    spread = cluster(20000, (1.05, 1.95), (-1.0, 1.0), (-1.0, 1.0))  # in every list; the near lists' top byte
+    # Copies tie in depth: in the long lists, in the short ones (at most 16 pairs to a
+    # bucket of the LDS order), and 40 copies of one Gaussian (a bucket past 16 pairs).
+    params = np.concatenate([near, far, spread, near[:5000], far[:5000], spread[:300], np.repeat(spread[300:301], 40, 0)])
+    n = len(params)
+    frames = {}
+    for per_tile in (0, 1):
+        t = diorama_amd.Trainer(context(), params, np.zeros((n, 45), np.float32), [cam], [np.zeros(w * h, np.uint32)],
+                                diorama_amd.default_settings(2.0))
+        t.tuning = dict(t.tuning, per_tile_depth=per_tile)
+        say("long lists and depth ties, per_tile_depth", per_tile)
+        t.optimize()
+        r = t.renderer
+        r.tiles.after_submit()
+        frames[per_tile] = gpu_frame(r, w, h)
+        t.destroy()
+    splats, color, last, ranges, pairs = frames[1]
+    o_ranges, o_pairs = oracle.tile_lists(splats.astype(np.float64), w, h)
+    lengths = ranges[:, 1].astype(np.int64) - ranges[:, 0]
+    keys = splats[:, 2].copy().view(np.uint32)
+    long = [k for k in np.nonzero(lengths > 4096)[0]]
+    shared = [len(np.unique(keys[pairs[ranges[k, 0]:ranges[k, 1]]] >> 24)) == 1 for k in long]
+    tied = sum(int(np.sum(np.diff(keys[pairs[a:b]].astype(np.int64)) == 0)) for a, b in ranges)
+    print(f"{len(pairs)} pairs, longest list {lengths.max()}, {len(long)} lists past 4096 pairs "
+          f"({sum(shared)} with one top byte), {tied} tied neighbours")
+    assert lengths.max() > 2 * 2048 and any(shared) and not all(shared) and tied > 1000
+    for per_tile in (0, 1):
+        s_, c_, l_, r_, p_ = frames[per_tile]
+        assert np.array_equal(s_, splats), per_tile
+        assert np.array_equal(r_, o_ranges), per_tile
+        assert np.array_equal(p_, o_pairs), per_tile
+        assert np.array_equal(c_, color) and np.array_equal(l_, last), per_tile
