@@ -109,6 +109,7 @@ SIGNATURES = {
     "dio_tile_rasterizer_after_submit": (_I, [_P]),
     "dio_tile_rasterizer_last_pair_count": (_U32, [_P]),
     "dio_tile_rasterizer_pair_capacity": (_U32, [_P]),
+    "dio_tile_rasterizer_set_tuning": (_I, [_P, ctypes.POINTER(_Tuning)]),
     "dio_tile_rasterizer_pair_splats": (_P, [_P]),
     "dio_tile_rasterizer_ranges": (_P, [_P]),
     "dio_tile_rasterizer_pixel_color": (_P, [_P]),
@@ -418,6 +419,12 @@ class TileRasterizer:
 
     def after_submit(self):
         _check(lib().dio_tile_rasterizer_after_submit(self.handle))
+
+    def set_tuning(self, value):
+        """Launch settings (dio_tuning keys; missing ones take their defaults)."""
+        k = _Tuning(value.get("tile_size", 16), value.get("splat_batch", 64), value.get("digit_bits", 8),
+                    value.get("persistent", 0), value.get("segment", 0), value.get("per_tile_depth", 0))
+        _check(lib().dio_tile_rasterizer_set_tuning(self.handle, ctypes.byref(k)))
 
     @property
     def last_pair_count(self):

@@ -647,6 +647,11 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   dio::scan_exclusive(s, dev<uint32_t>(per_tile ? t->tile_counts : t->sorted_counts), dev<uint32_t>(t->offsets),
                       nullptr, t->count, dev<uint32_t>(t->pair_total),
                       dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
+  // Pairs in splat order: a frame past the capacity keeps the nearest pair slots, as
+  // the depth-ordered duplicate does.
+  if (per_tile)
+    dio::pair_cut(s, dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->offsets), t->count,
+                  dev<uint32_t>(t->pair_total), t->capacity);
   mark(kMarkScan);
   // The pair count is known on the device only; the last frame's (0 before the first
   // readback) sizes the grids of the pair kernels, which loop over whatever it is.
@@ -769,6 +774,40 @@ extern "C" int dio_tile_rasterizer_encode_backward(dio_tile_rasterizer* t, uint3
     if (splat_grads->bytes < static_cast<size_t>(t->count) * 36) invalid("the splat gradient buffer is too small");
     if (pixel_grads->bytes < static_cast<size_t>(width) * height * 16) invalid("the pixel gradient buffer is too small");
     backward_tiles(t, width, height, pixel_grads, splat_grads);
+  });
+}
+
+namespace {
+
+// The launch settings any tile rasterizer takes (dio_tuning); trainers add their own.
+void check_launch_tuning(const dio_tuning& k) {
+  if (k.tile_size != 8 && k.tile_size != 16 && k.tile_size != 32) invalid("the tile size must be 8, 16 or 32 pixels");
+  if (k.splat_batch != 64 && k.splat_batch != 128 && k.splat_batch != 256) invalid("splat batch must be 64, 128 or 256");
+  if (k.tile_size == 32 && k.splat_batch == 256) invalid("32-pixel tiles take a splat batch of 64 or 128");
+  if (k.digit_bits != 4 && k.digit_bits != 8) invalid("digit width must be 4 or 8 bits");
+  if (k.persistent > 1) invalid("persistent is 0 or 1");
+  if (k.persistent && k.tile_size != 16) invalid("persistent workgroups rasterize 16-pixel tiles");
+  if (k.segment != 0 && k.segment != 256 && k.segment != 512 && k.segment != 1024)
+    invalid("segment must be 0, 256, 512 or 1024 pairs");
+  if (k.segment && (k.tile_size != 16 || k.persistent))
+    invalid("the segmented backward runs on 16-pixel tiles, one workgroup each");
+  if (k.per_tile_depth > 1) invalid("per_tile_depth is 0 or 1");
+}
+
+}  // namespace
+
+extern "C" int dio_tile_rasterizer_set_tuning(dio_tile_rasterizer* t, const dio_tuning* k) {
+  return guard([&] {
+    need(t, "tile rasterizer");
+    need(k, "tuning");
+    check_launch_tuning(*k);
+    t->tile_size = k->tile_size;
+    t->splat_batch = k->splat_batch;
+    t->digit_bits = k->digit_bits;
+    t->persistent = k->persistent != 0;
+    t->segment_shift = k->segment ? static_cast<uint32_t>(__builtin_ctz(k->segment)) : 0u;
+    t->per_tile_depth = k->per_tile_depth != 0;
+    t->framed = false;  // the next backward pass needs a frame of these settings
   });
 }
 
@@ -1241,7 +1280,7 @@ struct dio_trainer {
   // The default launch settings: 512-pair backward segments for a single-view
   // trainer (measured fastest at 1.1M Gaussians, 980x545); a batch trainer replays
   // whole lists.
-  dio_tuning tuning{16, 64, 8, 0, 512};
+  dio_tuning tuning{16, 64, 8, 0, 512, 1};
   StageClock clock;
   // The next step's frame, prepared at the end of a plain step by the fused
   // backward (its preprocess for the next view), and what it was prepared for. The
@@ -1977,6 +2016,7 @@ void trainer_create(dio_ctx* ctx, const float* params, const float* sh, uint32_t
       t->loss = l;
       if (batch) {
         t->tuning.segment = 0;
+        t->tuning.per_tile_depth = 0;
         t->batch = new BatchState;
         BatchState* b = t->batch;
         b->views = batch;
@@ -2313,6 +2353,9 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
         dio::scan_exclusive(s, dev<uint32_t>(per_tile ? tr->tile_counts : tr->sorted_counts), dev<uint32_t>(tr->offsets),
                             nullptr, tr->count, dev<uint32_t>(tr->pair_total),
                             dio::scan_workspace(tr->splat_scan_ws->ptr, tr->count));
+        if (per_tile)
+          dio::pair_cut(s, dev<uint32_t>(tr->depth_keys), dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->offsets),
+                        tr->count, dev<uint32_t>(tr->pair_total), tr->capacity);
         dio::duplicate(s, dev<float>(sc->splats), per_tile ? nullptr : dev<uint32_t>(tr->depth_values),
                        dev<uint32_t>(tr->offsets), dev<uint32_t>(tr->tile_counts), tr->count, tr->frame_w, tr->frame_h,
                        tr->capacity, dev<uint32_t>(tr->pair_total), dev<uint32_t>(tr->pair_tiles),
@@ -2708,21 +2751,11 @@ extern "C" int dio_trainer_batch_frames(dio_trainer* t, uint32_t* views, uint32_
 namespace {
 
 void check_tuning(const dio_trainer* t, const dio_tuning& k) {
-  if (k.tile_size != 8 && k.tile_size != 16 && k.tile_size != 32) invalid("the tile size must be 8, 16 or 32 pixels");
-  if (k.splat_batch != 64 && k.splat_batch != 128 && k.splat_batch != 256) invalid("splat batch must be 64, 128 or 256");
-  if (k.tile_size == 32 && k.splat_batch == 256) invalid("32-pixel tiles take a splat batch of 64 or 128");
-  if (k.digit_bits != 4 && k.digit_bits != 8) invalid("digit width must be 4 or 8 bits");
-  if (k.persistent > 1) invalid("persistent is 0 or 1");
-  if (k.persistent && k.tile_size != 16) invalid("persistent workgroups rasterize 16-pixel tiles");
+  check_launch_tuning(k);
   if (k.persistent && t->batch) invalid("a batch trainer launches a workgroup per tile");
   if (t->batch && k.tile_size != 16) invalid("a batch trainer rasterizes 16-pixel tiles");
   if (t->learned.mode == 1 && k.tile_size != 16) invalid("a trainer logging samples rasterizes 16-pixel tiles");
-  if (k.segment != 0 && k.segment != 256 && k.segment != 512 && k.segment != 1024)
-    invalid("segment must be 0, 256, 512 or 1024 pairs");
-  if (k.segment && (k.tile_size != 16 || k.persistent))
-    invalid("the segmented backward runs on 16-pixel tiles, one workgroup each");
   if (k.segment && t->batch) invalid("a batch trainer replays whole lists");
-  if (k.per_tile_depth > 1) invalid("per_tile_depth is 0 or 1");
   if (k.per_tile_depth && t->batch) invalid("a batch trainer sorts its splats by depth");
 }
 

@@ -382,7 +382,151 @@ __global__ __launch_bounds__(kThreads) void depth_order_kernel(const uint32_t* _
     order_streamed(L, depth_keys, list, scratch, range.x, n);
 }
 
+// A frame whose pairs overflow the capacity keeps, as the reference does
+// (tiles.wgsl:47-58), the pair slots below the capacity in depth order: splats by
+// (depth key, splat), each splat's tiles row by row. With pairs written in splat
+// order that is a cut in (key, splat) order: every splat before the cut splat keeps
+// all its pairs, the cut splat its first `keep`, the rest none. One workgroup finds
+// the cut (the key 8 bits at a time, weighting each splat by its tile count, then
+// the splat among those of that key), rewrites each splat's tile count to the pairs
+// it keeps and rescans the slot offsets, so the duplicate writes exactly the kept
+// pairs. Frames that fit return at once.
+constexpr int kCutThreads = 1024;
+constexpr int kCutWaves = kCutThreads / kWave;
+constexpr int kCutItems = 4;  // consecutive splats per thread per sweep step
+constexpr uint32_t kCutStep = kCutThreads * kCutItems;
+
+struct CutLds {
+  uint32_t hist[kRadix];
+  uint32_t wave_sum[kCutWaves];
+  uint32_t fixed, target, cut, keep;
+};
+
+// Exclusive prefix of x over the 1024 threads; *total gets the sum.
+__device__ inline uint32_t cut_exclusive(CutLds& L, uint32_t x, uint32_t* total) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  uint32_t inclusive = x;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(inclusive, o, kWave);
+    if (lane >= o) inclusive += y;
+  }
+  if (lane == kWave - 1) L.wave_sum[wave] = inclusive;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kCutWaves; w++) {
+    const uint32_t v = L.wave_sum[w];
+    before += w < wave ? v : 0u;
+    all += v;
+  }
+  __syncthreads();  // wave_sum is written again by the next call
+  *total = all;
+  return before + inclusive - x;
+}
+
+__global__ __launch_bounds__(kCutThreads) void pair_cut_kernel(const uint32_t* __restrict__ depth_keys,
+                                                                uint32_t* __restrict__ counts,
+                                                                uint32_t* __restrict__ offsets, uint32_t n,
+                                                                const uint32_t* __restrict__ pair_total,
+                                                                uint32_t capacity) {
+  if (*pair_total <= capacity) return;
+  __shared__ CutLds L;
+  const int tid = threadIdx.x;
+  // 1. The cut splat's key, most significant digit first: the digit whose splats hold
+  //    the target-th of the remaining slots.
+  uint32_t fixed = 0, fixed_mask = 0, target = capacity;
+  for (int shift = 24; shift >= 0 && capacity; shift -= 8) {
+    if (tid < kRadix) L.hist[tid] = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kCutStep) {
+      uint32_t c[kCutItems], k[kCutItems];
+#pragma unroll
+      for (int i = 0; i < kCutItems; i++) {
+        const uint32_t j = base + tid * kCutItems + i;
+        c[i] = j < n ? counts[j] : 0u;
+        k[i] = j < n ? depth_keys[j] : 0u;
+      }
+#pragma unroll
+      for (int i = 0; i < kCutItems; i++)
+        if (c[i] && (k[i] & fixed_mask) == fixed) atomicAdd(&L.hist[(k[i] >> shift) & (kRadix - 1)], c[i]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t before = 0, d = 0;
+      for (; d + 1 < kRadix && before + L.hist[d] < target; d++) before += L.hist[d];
+      L.fixed = fixed | (d << shift);
+      L.target = target - before;
+    }
+    __syncthreads();
+    fixed = L.fixed;
+    target = L.target;
+    fixed_mask |= static_cast<uint32_t>(kRadix - 1) << shift;
+  }
+  // 2. The cut splat among the splats of that key, in splat order, and how many of
+  //    its pairs it keeps.
+  if (tid == 0) {
+    L.cut = n;
+    L.keep = 0;
+  }
+  __syncthreads();
+  uint32_t running = 0;
+  for (uint32_t base = 0; base < n && capacity && running < target; base += kCutStep) {
+    uint32_t w[kCutItems], mine = 0;
+#pragma unroll
+    for (int i = 0; i < kCutItems; i++) {
+      const uint32_t j = base + tid * kCutItems + i;
+      w[i] = j < n && depth_keys[j] == fixed ? counts[j] : 0u;
+      mine += w[i];
+    }
+    uint32_t step_total;
+    uint32_t before = running + cut_exclusive(L, mine, &step_total);
+#pragma unroll
+    for (int i = 0; i < kCutItems; i++) {
+      if (w[i] && before < target && target <= before + w[i]) {
+        L.cut = base + tid * kCutItems + i;
+        L.keep = target - before;
+      }
+      before += w[i];
+    }
+    running += step_total;
+  }
+  __syncthreads();
+  const uint32_t cut = L.cut, keep = L.keep;
+  // 3. Kept pairs per splat, and the slot offsets over them.
+  running = 0;
+  for (uint32_t base = 0; base < n; base += kCutStep) {
+    uint32_t kept[kCutItems], mine = 0;
+#pragma unroll
+    for (int i = 0; i < kCutItems; i++) {
+      const uint32_t j = base + tid * kCutItems + i;
+      const uint32_t c = j < n ? counts[j] : 0u, k = j < n ? depth_keys[j] : 0u;
+      kept[i] = c == 0 || capacity == 0 ? 0u : k < fixed || (k == fixed && j < cut) ? c : j == cut ? keep : 0u;
+      mine += kept[i];
+    }
+    uint32_t step_total;
+    uint32_t at = running + cut_exclusive(L, mine, &step_total);
+#pragma unroll
+    for (int i = 0; i < kCutItems; i++) {
+      const uint32_t j = base + tid * kCutItems + i;
+      if (j < n) {
+        counts[j] = kept[i];
+        offsets[j] = at;
+      }
+      at += kept[i];
+    }
+    running += step_total;
+  }
+}
+
 }  // namespace
+
+void pair_cut(hipStream_t s, const uint32_t* depth_keys, uint32_t* tile_counts, uint32_t* offsets, uint32_t n,
+              const uint32_t* pair_total, uint32_t capacity) {
+  if (n == 0) return;
+  pair_cut_kernel<<<1, kCutThreads, 0, s>>>(depth_keys, tile_counts, offsets, n, pair_total, capacity);
+  DIO_LAUNCHED("pair_cut");
+}
 
 void depth_order(hipStream_t s, const uint32_t* depth_keys, const uint32_t* ranges, const uint32_t* tile_order,
                  uint32_t tiles, uint32_t* list, uint32_t* scratch) {

@@ -169,6 +169,12 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
 // through at their own positions.
 void depth_order(hipStream_t s, const uint32_t* depth_keys, const uint32_t* ranges, const uint32_t* tile_order,
                  uint32_t tiles, uint32_t* list, uint32_t* scratch);
+// For pairs written in splat order (offsets the exclusive scan of tile_counts in
+// splat order, *pair_total their sum): when the pairs overflow `capacity`, rewrites
+// tile_counts to the pairs each splat keeps under the depth-order slot rule and
+// offsets to their scan, so exactly the capacity's nearest pairs are written.
+void pair_cut(hipStream_t s, const uint32_t* depth_keys, uint32_t* tile_counts, uint32_t* offsets, uint32_t n,
+              const uint32_t* pair_total, uint32_t capacity);
 
 // The segmented backward (16-px tiles). Segment boundaries are the absolute pair
 // positions that are multiples of 2^shift: the forward pass keeps each pixel's

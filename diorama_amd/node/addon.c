@@ -1001,19 +1001,31 @@ FN(trainer_get_tuning) {
   if (!ok(env, dio_trainer_get_tuning(handle(&c, 0), &k))) return NULL;
   return tuning_object(env, &k);
 }
+static int read_tuning(call_t* c, napi_value arr, dio_tuning* k) {
+  float v[6];
+  if (!floats(c, arr, v, 6)) return 0;
+  k->tile_size = (uint32_t)v[0];
+  k->splat_batch = (uint32_t)v[1];
+  k->digit_bits = (uint32_t)v[2];
+  k->persistent = (uint32_t)v[3];
+  k->segment = (uint32_t)v[4];
+  k->per_tile_depth = (uint32_t)v[5];
+  return 1;
+}
 FN(trainer_set_tuning) {
   ARGS;
   dio_tuning k;
-  float v[6];
-  if (!floats(&c, c.argv[1], v, 6))
+  if (!read_tuning(&c, c.argv[1], &k))
     return fail(env, "tuning must be [tileSize, splatBatch, digitBits, persistent, segment, perTileDepth]");
-  k.tile_size = (uint32_t)v[0];
-  k.splat_batch = (uint32_t)v[1];
-  k.digit_bits = (uint32_t)v[2];
-  k.persistent = (uint32_t)v[3];
-  k.segment = (uint32_t)v[4];
-  k.per_tile_depth = (uint32_t)v[5];
   if (!ok(env, dio_trainer_set_tuning(handle(&c, 0), &k))) return NULL;
+  return undefined(env);
+}
+FN(tiles_set_tuning) {
+  ARGS;
+  dio_tuning k;
+  if (!read_tuning(&c, c.argv[1], &k))
+    return fail(env, "tuning must be [tileSize, splatBatch, digitBits, persistent, segment, perTileDepth]");
+  if (!ok(env, dio_tile_rasterizer_set_tuning(handle(&c, 0), &k))) return NULL;
   return undefined(env);
 }
 FN(trainer_autotune) {
@@ -1043,7 +1055,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT(prefix_sum_create), EXPORT(prefix_sum_total), EXPORT(prefix_sum_encode), EXPORT(prefix_sum_destroy),
       EXPORT(radix_sort_create), EXPORT(radix_sort_keys), EXPORT(radix_sort_values), EXPORT(radix_sort_capacity),
       EXPORT(radix_sort_encode), EXPORT(radix_sort_destroy), EXPORT(tiles_create), EXPORT(tiles_encode),
-      EXPORT(tiles_encode_backward), EXPORT(tiles_after_submit), EXPORT(tiles_last_pair_count),
+      EXPORT(tiles_encode_backward), EXPORT(tiles_after_submit), EXPORT(tiles_set_tuning), EXPORT(tiles_last_pair_count),
       EXPORT(tiles_pair_capacity), EXPORT(tiles_pair_splats), EXPORT(tiles_ranges), EXPORT(tiles_pixel_color),
       EXPORT(tiles_pixel_last), EXPORT(tiles_destroy), EXPORT(renderer_create), EXPORT(renderer_set_gaussians),
       EXPORT(renderer_adopt_gaussians), EXPORT(renderer_set_sh_degree), EXPORT(renderer_encode_preprocess),

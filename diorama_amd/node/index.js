@@ -337,6 +337,11 @@ class TileRasterizer {
     native.tiles_after_submit(this.handle);
     return Promise.resolve();
   }
+  /** Launch settings (see dio_tile_rasterizer_set_tuning); results do not depend on them. */
+  setTuning(k) {
+    native.tiles_set_tuning(this.handle, [k.tileSize || 16, k.splatBatch || 64, k.digitBits || 8, k.persistent || 0,
+      k.segment || 0, k.perTileDepth || 0]);
+  }
   destroy() {
     if (this.owned && this.handle) native.tiles_destroy(this.handle);
     this.handle = null;

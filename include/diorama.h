@@ -75,11 +75,10 @@ typedef struct dio_tuning {
   /* single-view trainers: 0 sorts the splats by depth before writing their pairs
    * (src/tileRasterizer.ts:127-192); 1 writes the pairs in splat order, bins them by
    * tile and sorts each tile's list by depth. The lists, and so images and
-   * gradients, are the same, except on a frame whose pairs overflow the pair
-   * buffers: with 1 it drops the pairs of the last splats in splat order rather
-   * than those of the farthest ones. */
+   * gradients, are the same, also on a frame whose pairs overflow the pair
+   * buffers (both keep the nearest pair slots). */
   uint32_t per_tile_depth;
-} dio_tuning; /* a trainer starts at {16, 64, 8, 0, 512, 0}; a batch trainer at {16, 64, 8, 0, 0, 0} */
+} dio_tuning; /* a trainer starts at {16, 64, 8, 0, 512, 1}; a batch trainer at {16, 64, 8, 0, 0, 0} */
 
 typedef struct dio_tuning_result {
   dio_tuning tuning;
@@ -165,6 +164,11 @@ int dio_tile_rasterizer_encode_backward(dio_tile_rasterizer* tr, uint32_t width,
 /* afterSubmit (:229-244): reads back the frame's pair count and grows the pair
  * buffers to 2^ceil(log2(1.25 * count)) when they were too small. Waits. */
 int dio_tile_rasterizer_after_submit(dio_tile_rasterizer* tr);
+/* No counterpart in the reference: the operator's launch settings (dio_tuning;
+ * it starts at {16, 64, 8, 0, 0, 0}), for callers that tune them. Images and
+ * gradients do not depend on them. Takes effect at the next encode, which a
+ * backward pass then needs. */
+int dio_tile_rasterizer_set_tuning(dio_tile_rasterizer* tr, const dio_tuning* tuning);
 uint32_t dio_tile_rasterizer_last_pair_count(const dio_tile_rasterizer* tr);
 uint32_t dio_tile_rasterizer_pair_capacity(const dio_tile_rasterizer* tr);
 /* debugBuffers: each sorted pair's splat, and each tile's [start, end) run. */

@@ -9,6 +9,9 @@
  * reference repository):
  *   oc_preprocess           src/shaders/preprocess.wgsl:20-87, src/checks/reference.ts:27-80
  *   oc_tile_lists           src/shaders/tiles.wgsl:23-84, src/checks/reference.ts:128-153
+ *   oc_tile_lists_capped    the same with a pair capacity: tiles.wgsl:47-58 (slots past the
+ *                           capacity dropped), src/tileRasterizer.ts:166,279 (the sort and
+ *                           ranges over min(total, capacity) pairs)
  *   oc_rasterize            src/shaders/rasterize.wgsl:31-100, src/checks/reference.ts:156-176
  *   oc_rasterize_backward   src/shaders/rasterizeBackward.wgsl:49-163
  *   oc_preprocess_backward  src/shaders/preprocessBackward.wgsl:27-181
@@ -260,6 +263,68 @@ long oc_tile_lists(long n, const double* splats, int width, int height, uint32_t
     if (p == total - 1 || pairs[p + 1].tile != t) ranges[2 * t + 1] = (uint32_t)(p + 1);
   }
   free(pairs);
+  return total;
+}
+
+typedef struct {
+  double depth;
+  long splat;
+} depth_t;
+
+static int depth_less(const void* pa, const void* pb) {
+  const depth_t* a = (const depth_t*)pa;
+  const depth_t* b = (const depth_t*)pb;
+  if (a->depth != b->depth) return a->depth < b->depth ? -1 : 1;
+  return a->splat < b->splat ? -1 : (a->splat > b->splat);
+}
+
+/* A frame with a pair capacity (tiles.wgsl:47-58): splats take pair slots in depth
+ * order (a stable sort, so ties in splat order), each splat's tiles row by row from
+ * its first slot; pairs whose slot is at or past `capacity` are dropped, and the
+ * kept ones are sorted and ranged as in oc_tile_lists. Returns the full pair count
+ * (what the reference reads back); writes the min(total, capacity) kept pairs. */
+long oc_tile_lists_capped(long n, const double* splats, int width, int height, long capacity, uint32_t* ranges,
+                          uint32_t* pair_splats) {
+  int tx = (width + OC_TILE - 1) / OC_TILE, ty = (height + OC_TILE - 1) / OC_TILE;
+  depth_t* order = (depth_t*)malloc(sizeof(depth_t) * (n > 0 ? n : 1));
+  long visible = 0, total = 0;
+  for (long i = 0; i < n; i++) {
+    if (splats[i * 12 + 3] == 0) continue;
+    order[visible].depth = (double)(float)splats[i * 12 + 2];
+    order[visible].splat = i;
+    visible++;
+  }
+  qsort(order, visible, sizeof(depth_t), depth_less);
+  for (long v = 0; v < visible; v++) {
+    int r[4];
+    tile_rect(splats + order[v].splat * 12, tx, ty, r);
+    total += (long)(r[2] - r[0]) * (r[3] - r[1]);
+  }
+  long kept = total < capacity ? total : capacity;
+  pair_t* pairs = (pair_t*)malloc(sizeof(pair_t) * (kept > 0 ? kept : 1));
+  long slot = 0;
+  for (long v = 0; v < visible && slot < kept; v++) {
+    long i = order[v].splat;
+    int r[4];
+    tile_rect(splats + i * 12, tx, ty, r);
+    for (int y = r[1]; y < r[3] && slot < kept; y++)
+      for (int x = r[0]; x < r[2] && slot < kept; x++) {
+        pairs[slot].tile = (uint32_t)(y * tx + x);
+        pairs[slot].depth = order[v].depth;
+        pairs[slot].splat = (uint32_t)i;
+        slot++;
+      }
+  }
+  qsort(pairs, kept, sizeof(pair_t), pair_less);
+  memset(ranges, 0, sizeof(uint32_t) * 2 * tx * ty);
+  for (long p = 0; p < kept; p++) {
+    pair_splats[p] = pairs[p].splat;
+    uint32_t t = pairs[p].tile;
+    if (p == 0 || pairs[p - 1].tile != t) ranges[2 * t] = (uint32_t)p;
+    if (p == kept - 1 || pairs[p + 1].tile != t) ranges[2 * t + 1] = (uint32_t)(p + 1);
+  }
+  free(pairs);
+  free(order);
   return total;
 }
 

@@ -29,6 +29,8 @@ void oc_preprocess(long n, const double* gaussians, const double* sh, int sh_deg
 
 long oc_tile_lists(long n, const double* splats, int width, int height,
                    uint32_t* ranges, uint32_t* pair_splats, long capacity);
+long oc_tile_lists_capped(long n, const double* splats, int width, int height, long capacity, uint32_t* ranges,
+                          uint32_t* pair_splats);
 
 void oc_rasterize(const double* splats, const uint32_t* ranges, const uint32_t* pair_splats,
                   int width, int height, const double* background,

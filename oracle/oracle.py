@@ -35,6 +35,8 @@ def lib():
         L.oc_preprocess.argtypes = [c_long, _d, vp, c_int, _d, _d, _d, _d]
         L.oc_tile_lists.argtypes = [c_long, _d, c_int, c_int, _u32, vp, c_long]
         L.oc_tile_lists.restype = c_long
+        L.oc_tile_lists_capped.argtypes = [c_long, _d, c_int, c_int, c_long, _u32, _u32]
+        L.oc_tile_lists_capped.restype = c_long
         L.oc_rasterize.argtypes = [_d, _u32, _u32, c_int, c_int, _d, _d, _u32]
         L.oc_rasterize_margin.argtypes = [_d, _u32, _u32, c_int, c_int, _d, _d, _u32, _d]
         L.oc_rasterize_backward.argtypes = [c_long, _d, _u32, _u32, c_int, c_int, _d, _d, _u32, _d, _d]
@@ -94,6 +96,18 @@ def tile_lists(splats, width, height):
     pairs = np.zeros(max(total, 1), dtype=np.uint32)
     lib().oc_tile_lists(n, s, width, height, ranges, _ptr(pairs), total)
     return ranges.reshape(tiles, 2), pairs[:total]
+
+
+def tile_lists_capped(splats, width, height, capacity):
+    """(ranges[T, 2] uint32, kept pair splats uint32, full pair count) of a frame whose
+    pair buffers hold `capacity` pairs: the nearest pairs are kept (oc_tile_lists_capped)."""
+    s = _f64(splats).reshape(-1)
+    n = s.size // 12
+    tiles = ((width + 15) // 16) * ((height + 15) // 16)
+    ranges = np.zeros(2 * tiles, dtype=np.uint32)
+    pairs = np.zeros(max(capacity, 1), dtype=np.uint32)
+    total = lib().oc_tile_lists_capped(n, s, width, height, capacity, ranges, pairs)
+    return ranges.reshape(tiles, 2), pairs[:min(total, capacity)], total
 
 
 def rasterize(splats, ranges, pairs, width, height, background, margin=False):

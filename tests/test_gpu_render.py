@@ -415,3 +415,57 @@ def test_long_lists_and_depth_ties_in_both_depth_orders():
         assert np.array_equal(r_, o_ranges), per_tile
         assert np.array_equal(p_, o_pairs), per_tile
         assert np.array_equal(c_, color) and np.array_equal(l_, last), per_tile
+
+
+@pytest.mark.parametrize("per_tile", [0, 1])
+def test_overflowing_frame_keeps_the_nearest_pair_slots(per_tile):
+    """A frame past its pair capacity keeps the pair slots below the capacity in depth
+    order (tiles.wgsl:47-58; the oracle's oc_tile_lists_capped) and grows its buffers
+    afterwards (src/tileRasterizer.ts:229-244). per_tile_depth 1 writes the pairs in
+    splat order and keeps the same pairs by a cut in (depth, splat) order. The
+    capacities end between splats, inside one splat's tiles and inside a group of
+    equal depths (copies of Gaussians)."""
+    ctx = context()
+    w, h = 160, 120
+    cam = scene.camera(scene.IDENTITY, scene.perspective(math.pi / 2, w / h, 0.1, 100), w, h)
+    g = random_scene(3000, 9, scale=(0.05, 0.3))
+    g = np.concatenate([g, g[:400]])
+    splats = oracle.preprocess(g, cam).astype(np.float32)
+    s64 = splats.astype(np.float64)
+    n = len(splats)
+    bg = [0.1, 0.2, 0.3]
+    full_ranges, full_pairs = oracle.tile_lists(s64, w, h)
+    total = len(full_pairs)
+    counts = np.bincount(full_pairs, minlength=n)
+    order = np.lexsort((np.arange(n), splats[:, 2].view(np.uint32)))
+    order = order[counts[order] > 0]
+    start = np.zeros(n, np.int64)
+    start[order] = np.cumsum(counts[order]) - counts[order]
+    copy = next(3000 + k for k in range(400) if counts[k] > 0)  # ties its original, which comes first
+    wide = next(i for i in order[len(order) // 2:] if counts[i] >= 4)
+    capacities = sorted({total // 3, int(start[copy]) + 1, int(start[wide] + counts[wide] // 2), total - 1})
+    tiles = ((w + 15) // 16) * ((h + 15) // 16)
+    buf = ctx.upload(splats)
+    for capacity in capacities:
+        tr = diorama_amd.TileRasterizer(ctx, buf, n, capacity)
+        tr.set_tuning({"per_tile_depth": per_tile})
+        say("overflowing frame, per_tile_depth", per_tile, "capacity", capacity, "of", total)
+        tr.encode(None, (w, h), bg)
+        ctx.synchronize()
+        ranges = tr.debug_buffers["ranges"].download(np.uint32, 2 * tiles).reshape(-1, 2)
+        pairs = tr.debug_buffers["pairSplats"].download(np.uint32, capacity)
+        color = tr.pixel_color.download(np.float32, w * h * 4).reshape(-1, 4)
+        last = tr.pixel_last.download(np.uint32, w * h)
+        o_ranges, o_pairs, o_total = oracle.tile_lists_capped(s64, w, h, capacity)
+        assert o_total == total
+        assert np.array_equal(ranges, o_ranges), capacity
+        assert np.array_equal(pairs, o_pairs), capacity
+        o_color, o_last, margin = oracle.rasterize(s64, o_ranges, o_pairs, w, h, bg, margin=True)
+        assert_image_matches(color, last, o_color, o_last, margin, f"capacity {capacity}")
+        tr.after_submit()
+        assert tr.last_pair_count == total and tr.pair_capacity >= total
+        tr.encode(None, (w, h), bg)
+        ctx.synchronize()
+        assert np.array_equal(tr.debug_buffers["ranges"].download(np.uint32, 2 * tiles).reshape(-1, 2), full_ranges)
+        assert np.array_equal(tr.debug_buffers["pairSplats"].download(np.uint32, total), full_pairs)
+        tr.destroy()

@@ -208,7 +208,7 @@ def test_training_learns_and_densifying_helps():
     assert count > 12 and dens > plain + 1
 
 
-STAGES = ["preprocess", "depth_sort", "scan", "duplicate", "tile_sort", "ranges", "rasterize", "loss",
+STAGES = ["preprocess", "depth_sort", "scan", "duplicate", "tile_sort", "ranges", "list_order", "rasterize", "loss",
           "clear_grads", "rasterize_backward", "backward_adam"]
 
 

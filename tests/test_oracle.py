@@ -460,3 +460,39 @@ def test_footprint_hand_values_and_blend_weight_invariant():
     share = rng.uniform(0, 1, w * h)
     weighted = oracle.footprint(sp, ranges, pairs, w, h, share)
     assert abs(weighted[:, 0].sum() - (share * (1 - color[:, 3])).sum()) < 1e-9 * w * h
+
+
+def test_capped_tile_lists_keep_the_nearest_pair_slots():
+    """tiles.wgsl:47-58: with a pair capacity, splats take pair slots in depth order
+    (ties in splat order), each splat's tiles row by row, and slots at or past the
+    capacity are dropped before the tile sort. Worked by hand on a 48x32 viewport
+    (3 x 2 tiles, tile = 3y + x): P (depth 3) covers tiles 1, 2, 4, 5; Q and R tie at
+    depth 1 (Q first); Q covers tile 0, R tiles 3 and 4; S (depth 2) covers tile 5.
+    Slots: Q 0; R 1, 2; S 3; P 4..7 (tiles 1, 2, then 4, 5). Cutting in splat order,
+    or taking P's tiles column by column, fails these lists."""
+    def splat(mx, my, depth, radius):
+        s = np.zeros(12)
+        s[:4] = mx, my, depth, radius
+        return s
+
+    splats = np.array([splat(32, 16, 3.0, 8), splat(8, 8, 1.0, 4), splat(16, 24, 1.0, 4), splat(40, 24, 2.0, 4)])
+    P, Q, R, S = 0, 1, 2, 3
+    cases = {
+        8: [[Q], [P], [P], [R], [R, P], [S, P]],
+        6: [[Q], [P], [P], [R], [R], [S]],
+        5: [[Q], [P], [], [R], [R], [S]],
+        2: [[Q], [], [], [R], [], []],
+        0: [[], [], [], [], [], []],
+    }
+    for capacity, lists in cases.items():
+        ranges, pairs, total = oracle.tile_lists_capped(splats, 48, 32, capacity)
+        want_ranges, want_pairs = [], []
+        for tile_list in lists:
+            start = len(want_pairs)
+            want_ranges.append([start, start + len(tile_list)] if tile_list else [0, 0])
+            want_pairs += tile_list
+        assert total == 8
+        assert ranges.tolist() == want_ranges and pairs.tolist() == want_pairs, capacity
+    full_ranges, full_pairs = oracle.tile_lists(splats, 48, 32)
+    ranges, pairs, _ = oracle.tile_lists_capped(splats, 48, 32, 100)
+    assert np.array_equal(ranges, full_ranges) and np.array_equal(pairs, full_pairs)
