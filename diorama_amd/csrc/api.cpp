@@ -450,7 +450,9 @@ struct dio_tile_rasterizer {
   uint32_t count = 0;
   uint32_t capacity = 0;
   // Per splat: tile counts, depth sort keys/values (+ scratch), the tile counts in
-  // depth order (written by the depth sort's last pass), pair offsets.
+  // depth order (written by the depth sort's last pass), pair offsets, and the tile
+  // (or bin) rectangle, packed (dio::pack_rect), for binning by counts.
+  dio_buf* rects = nullptr;
   dio_buf *tile_counts = nullptr, *depth_keys = nullptr, *depth_values = nullptr, *depth_alt_keys = nullptr,
           *depth_alt_values = nullptr, *sorted_counts = nullptr, *offsets = nullptr, *splat_scan_ws = nullptr,
           *splat_sort_ws = nullptr;
@@ -458,6 +460,9 @@ struct dio_tile_rasterizer {
   // Pairs: tile index and splat, ping-pong buffers for the tile sort.
   dio_buf *pair_tiles = nullptr, *pair_splats = nullptr, *alt_tiles = nullptr, *alt_splats = nullptr,
           *pair_sort_ws = nullptr;
+  // Binning by counts: the pairs as (depth key, splat), then a scratch of the same
+  // size for the long lists' depth order (allocated with its first frame).
+  dio_buf* pair_keyed = nullptr;
   bool sorted_in_alt = false;
   dio_buf* ranges = nullptr;
   dio_buf* tile_order = nullptr;  // the rasterizer's launch order, heaviest tiles first
@@ -474,9 +479,15 @@ struct dio_tile_rasterizer {
   uint32_t splat_batch = 64, digit_bits = 8;  // tuning; results do not depend on them
   uint32_t tile_size = 16;  // tuning: 8, 16 or 32 (32: pairs binned by 32-px bins)
   bool persistent = false;  // tuning: persistent rasterizer workgroups (16-px tiles)
-  // Tuning: pairs written in splat order and binned by tile, then each tile's list
-  // sorted by depth (no depth sort of all the splats); the lists are the same.
-  bool per_tile_depth = false;
+  // Tuning: 0 sorts the splats by depth before writing their pairs; 1 writes the
+  // pairs in splat order, sorts them stably by tile and then each tile's list by
+  // depth; 2 bins the pairs by tile from per-tile counts (no sort over the pairs)
+  // and then sorts each tile's list by (depth, splat). The lists are the same.
+  uint32_t per_tile_depth = 0;
+  // Binning by counts (per_tile_depth 2): the chunk x tile count matrix, the chunk
+  // totals and the tile totals.
+  dio_buf *bin_matrix = nullptr, *bin_partial = nullptr, *bin_totals = nullptr;
+  uint32_t bin_groups = 0, bin_tiles = 0;
   dio_buf* work = nullptr;  // their two tile counters, zeroed between launches
   // Tuning: the segmented backward's segment length 2^segment_shift (0: a workgroup
   // per tile); framed_shift is the one the last forward pass wrote checkpoints for.
@@ -492,7 +503,7 @@ namespace {
 
 void free_pairs(dio_tile_rasterizer* t) {
   release(t->pair_tiles); release(t->pair_splats); release(t->alt_tiles); release(t->alt_splats);
-  release(t->pair_sort_ws);
+  release(t->pair_sort_ws); release(t->pair_keyed);
 }
 
 void create_pairs(dio_tile_rasterizer* t, uint32_t capacity) {
@@ -507,6 +518,7 @@ void create_pairs(dio_tile_rasterizer* t, uint32_t capacity) {
 void destroy_tiles(dio_tile_rasterizer* t) {
   if (!t) return;
   (void)hipStreamSynchronize(t->ctx->stream);
+  release(t->rects);
   release(t->tile_counts); release(t->depth_keys); release(t->depth_values); release(t->depth_alt_keys);
   release(t->depth_alt_values); release(t->sorted_counts); release(t->offsets); release(t->splat_scan_ws);
   release(t->splat_sort_ws);
@@ -514,6 +526,9 @@ void destroy_tiles(dio_tile_rasterizer* t) {
   free_pairs(t);
   release(t->ranges);
   release(t->tile_order);
+  release(t->bin_matrix);
+  release(t->bin_partial);
+  release(t->bin_totals);
   release(t->work);
   release(t->ckpt);
   release(t->items);
@@ -535,6 +550,7 @@ dio_tile_rasterizer* create_tiles(dio_ctx* ctx, dio_buf* splats, uint32_t count,
   try {
     const size_t n = std::max<uint32_t>(count, 1);
     t->tile_counts = alloc(ctx, n * 4, true);
+    t->rects = alloc(ctx, n * 8, true);
     t->depth_keys = alloc(ctx, n * 4);
     t->depth_values = alloc(ctx, n * 4);
     t->depth_alt_keys = alloc(ctx, n * 4);
@@ -570,6 +586,39 @@ void ensure_frame_buffers(dio_tile_rasterizer* t, uint32_t tiles, uint32_t pixel
     t->pixel_last = alloc(t->ctx, static_cast<size_t>(pixels) * 4, true);
     t->pixel_count = pixels;
   }
+}
+
+// The binning-by-counts workspace for `tiles` tiles (at most dio::kBinMaxTiles).
+dio::BinWorkspace bin_workspace(dio_tile_rasterizer* t, uint32_t tiles) {
+  const uint32_t groups = dio::bin_layout(t->count).groups;
+  if (!t->bin_matrix || t->bin_groups < groups || t->bin_tiles < tiles) {
+    release(t->bin_matrix);
+    release(t->bin_partial);
+    release(t->bin_totals);
+    t->bin_matrix = alloc(t->ctx, static_cast<size_t>(groups) * tiles * 4);
+    t->bin_partial = alloc(t->ctx, static_cast<size_t>(groups) * 4);
+    t->bin_totals = alloc(t->ctx, static_cast<size_t>(tiles) * 4);
+    t->bin_groups = groups;
+    t->bin_tiles = tiles;
+  }
+  dio::BinWorkspace ws;
+  ws.matrix = dev<uint32_t>(t->bin_matrix);
+  ws.partial = dev<uint32_t>(t->bin_partial);
+  ws.totals = dev<uint32_t>(t->bin_totals);
+  ws.groups = t->bin_groups;
+  ws.tiles = t->bin_tiles;
+  return ws;
+}
+
+// Whether a frame of `tiles` tiles (or bins) bins its pairs by counts.
+bool bins_by_counts(const dio_tile_rasterizer* t, uint32_t tiles) {
+  return t->per_tile_depth == 2 && tiles <= dio::kBinMaxTiles;
+}
+
+// The keyed pairs of binning by counts (two words a pair), then their scratch.
+uint32_t* keyed_pairs(dio_tile_rasterizer* t) {
+  if (!t->pair_keyed) t->pair_keyed = alloc(t->ctx, static_cast<size_t>(t->capacity) * 16);
+  return dev<uint32_t>(t->pair_keyed);
 }
 
 // The segmented backward's buffers for `tiles` tiles and the pair capacity, or
@@ -629,56 +678,79 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
   const float* splats = dev<float>(t->splats);
   if (!counts_ready)
     dio::count_tiles(s, splats, t->count, w, h, dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
-                     dev<uint32_t>(t->depth_values), shift);
-  // Depth order of the splats (culled ones last), with their tile counts in that
-  // order, then each splat's first pair slot. Per-tile depth order skips the sort:
-  // pairs are written in splat order and each tile's list is sorted after binning.
-  const bool per_tile = t->per_tile_depth;
-  if (!per_tile &&
-      dio::radix_sort(s, false, t->depth_keys->ptr, dev<uint32_t>(t->depth_values), t->depth_alt_keys->ptr,
-                      dev<uint32_t>(t->depth_alt_values), nullptr, t->count, 0, 32,
-                      dio::sort_workspace(t->splat_sort_ws->ptr, t->count), t->digit_bits,
-                      dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->sorted_counts)))
-    throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
+                     dev<uint32_t>(t->depth_values), shift, dev<uint32_t>(t->rects));
   auto mark = [&](StageMark k) {
     if (t->clock) t->clock->mark(s, k);
   };
-  mark(kMarkDepthSort);
-  dio::scan_exclusive(s, dev<uint32_t>(per_tile ? t->tile_counts : t->sorted_counts), dev<uint32_t>(t->offsets),
-                      nullptr, t->count, dev<uint32_t>(t->pair_total),
-                      dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
-  // Pairs in splat order: a frame past the capacity keeps the nearest pair slots, as
-  // the depth-ordered duplicate does.
-  if (per_tile)
-    dio::pair_cut(s, dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->offsets), t->count,
-                  dev<uint32_t>(t->pair_total), t->capacity);
-  mark(kMarkScan);
-  // The pair count is known on the device only; the last frame's (0 before the first
-  // readback) sizes the grids of the pair kernels, which loop over whatever it is.
-  const uint32_t pair_hint = t->last_pair_count;
-  dio::duplicate(s, splats, per_tile ? nullptr : dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets),
-                 dev<uint32_t>(t->tile_counts), t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total),
-                 dev<uint32_t>(t->pair_tiles), dev<uint32_t>(t->pair_splats), pair_hint, shift);
-  mark(kMarkDuplicate);
-  t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats), t->alt_tiles->ptr,
-                                     dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total), t->capacity, 0,
-                                     tile_sort_bits(tiles, t->digit_bits),
-                                     dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity), t->digit_bits, nullptr,
-                                     nullptr, pair_hint ? pair_hint + pair_hint / 8 + 4096 : 0);
-  mark(kMarkTileSort);
-  dio_buf* sorted_tiles = t->sorted_in_alt ? t->alt_tiles : t->pair_tiles;
-  dio_buf* sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
   dio::Segments seg;
   const dio::Segments* segments =
       t->tile_size == 16 && !t->persistent ? segments_for(t, tiles, seg, true) : nullptr;
-  dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(t->pair_total), t->capacity, tiles,
-                   dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), pair_hint,
-                   segments ? segments->counts : nullptr);
-  mark(kMarkRanges);
-  if (per_tile)
+  dio_buf* sorted_splats = nullptr;
+  if (bins_by_counts(t, tiles)) {
+    // Pairs binned by tile from counts, then each tile's list by (depth, splat).
+    const dio::BinWorkspace ws = bin_workspace(t, tiles);
+    mark(kMarkDepthSort);
+    dio::bin_count(s, dev<uint32_t>(t->rects), dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->tile_counts), t->count,
+                   w, h, shift, t->capacity, dev<uint32_t>(t->pair_total), ws);
+    mark(kMarkScan);
+    uint32_t* keyed = keyed_pairs(t);
+    dio::bin_scatter(s, dev<uint32_t>(t->rects), dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys), t->count,
+                     w, h, shift, ws, keyed, dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order),
+                     segments ? segments->counts : nullptr);
+    mark(kMarkDuplicate);
+    mark(kMarkTileSort);
+    mark(kMarkRanges);
+    t->sorted_in_alt = false;
+    sorted_splats = t->pair_splats;
     dio::depth_order(s, dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), tiles,
-                     dev<uint32_t>(sorted_splats), dev<uint32_t>(t->sorted_in_alt ? t->pair_splats : t->alt_splats));
-  mark(kMarkListOrder);
+                     dev<uint32_t>(t->pair_splats), keyed + 2 * static_cast<size_t>(t->capacity), false, keyed);
+    mark(kMarkListOrder);
+  } else {
+    // Depth order of the splats (culled ones last), with their tile counts in that
+    // order, then each splat's first pair slot. Per-tile depth order skips the sort:
+    // pairs are written in splat order and each tile's list is sorted after binning.
+    const bool per_tile = t->per_tile_depth != 0;
+    if (!per_tile &&
+        dio::radix_sort(s, false, t->depth_keys->ptr, dev<uint32_t>(t->depth_values), t->depth_alt_keys->ptr,
+                        dev<uint32_t>(t->depth_alt_values), nullptr, t->count, 0, 32,
+                        dio::sort_workspace(t->splat_sort_ws->ptr, t->count), t->digit_bits,
+                        dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->sorted_counts)))
+      throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
+    mark(kMarkDepthSort);
+    dio::scan_exclusive(s, dev<uint32_t>(per_tile ? t->tile_counts : t->sorted_counts), dev<uint32_t>(t->offsets),
+                        nullptr, t->count, dev<uint32_t>(t->pair_total),
+                        dio::scan_workspace(t->splat_scan_ws->ptr, t->count));
+    // Pairs in splat order: a frame past the capacity keeps the nearest pair slots, as
+    // the depth-ordered duplicate does.
+    if (per_tile)
+      dio::pair_cut(s, dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->offsets),
+                    t->count, dev<uint32_t>(t->pair_total), t->capacity);
+    mark(kMarkScan);
+    // The pair count is known on the device only; the last frame's (0 before the first
+    // readback) sizes the grids of the pair kernels, which loop over whatever it is.
+    const uint32_t pair_hint = t->last_pair_count;
+    dio::duplicate(s, splats, per_tile ? nullptr : dev<uint32_t>(t->depth_values), dev<uint32_t>(t->offsets),
+                   dev<uint32_t>(t->tile_counts), t->count, w, h, t->capacity, dev<uint32_t>(t->pair_total),
+                   dev<uint32_t>(t->pair_tiles), dev<uint32_t>(t->pair_splats), pair_hint, shift);
+    mark(kMarkDuplicate);
+    t->sorted_in_alt = dio::radix_sort(s, false, t->pair_tiles->ptr, dev<uint32_t>(t->pair_splats),
+                                       t->alt_tiles->ptr, dev<uint32_t>(t->alt_splats), dev<uint32_t>(t->pair_total),
+                                       t->capacity, 0, tile_sort_bits(tiles, t->digit_bits),
+                                       dio::sort_workspace(t->pair_sort_ws->ptr, t->capacity), t->digit_bits, nullptr,
+                                       nullptr, pair_hint ? pair_hint + pair_hint / 8 + 4096 : 0);
+    mark(kMarkTileSort);
+    dio_buf* sorted_tiles = t->sorted_in_alt ? t->alt_tiles : t->pair_tiles;
+    sorted_splats = t->sorted_in_alt ? t->alt_splats : t->pair_splats;
+    dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(t->pair_total), t->capacity, tiles,
+                     dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), pair_hint,
+                     segments ? segments->counts : nullptr);
+    mark(kMarkRanges);
+    if (per_tile)
+      dio::depth_order(s, dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order),
+                       tiles, dev<uint32_t>(sorted_splats),
+                       dev<uint32_t>(t->sorted_in_alt ? t->pair_splats : t->alt_splats), true);
+    mark(kMarkListOrder);
+  }
   dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), w,
                  h, bg, dev<uint32_t>(target), dev<float>(t->pixel_color), dev<uint32_t>(t->pixel_last),
                  t->splat_batch, t->tile_size, t->persistent ? dev<uint32_t>(t->work) : nullptr, segments);
@@ -791,7 +863,7 @@ void check_launch_tuning(const dio_tuning& k) {
     invalid("segment must be 0, 256, 512 or 1024 pairs");
   if (k.segment && (k.tile_size != 16 || k.persistent))
     invalid("the segmented backward runs on 16-pixel tiles, one workgroup each");
-  if (k.per_tile_depth > 1) invalid("per_tile_depth is 0 or 1");
+  if (k.per_tile_depth > 2) invalid("per_tile_depth is 0, 1 or 2");
 }
 
 }  // namespace
@@ -806,7 +878,7 @@ extern "C" int dio_tile_rasterizer_set_tuning(dio_tile_rasterizer* t, const dio_
     t->digit_bits = k->digit_bits;
     t->persistent = k->persistent != 0;
     t->segment_shift = k->segment ? static_cast<uint32_t>(__builtin_ctz(k->segment)) : 0u;
-    t->per_tile_depth = k->per_tile_depth != 0;
+    t->per_tile_depth = k->per_tile_depth;
     t->framed = false;  // the next backward pass needs a frame of these settings
   });
 }
@@ -889,7 +961,7 @@ void renderer_preprocess(dio_renderer* r, const dio_camera& camera, dio_buf* vis
   auto* t = sc->rasterizer;
   dio::preprocess(r->ctx->stream, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
                   dev<float>(sc->splats), dev<uint32_t>(t->tile_counts), dev<uint32_t>(t->depth_keys),
-                  dev<uint32_t>(t->depth_values), vis ? dev<float>(vis) : nullptr);
+                  dev<uint32_t>(t->depth_values), vis ? dev<float>(vis) : nullptr, dev<uint32_t>(t->rects));
   if (t->clock) t->clock->mark(r->ctx->stream, kMarkPreprocess);
   r->preprocessed = true;
 }
@@ -1277,10 +1349,10 @@ struct dio_trainer {
   BatchState* batch = nullptr;  // set for a multi-view batch trainer
   uint32_t generation = 0;      // bumped whenever densification replaces the buffers
   LearnedState learned;
-  // The default launch settings: 512-pair backward segments for a single-view
-  // trainer (measured fastest at 1.1M Gaussians, 980x545); a batch trainer replays
-  // whole lists.
-  dio_tuning tuning{16, 64, 8, 0, 512, 1};
+  // The default launch settings: 512-pair backward segments and pairs binned by
+  // counts for a single-view trainer (measured fastest at 1.1M Gaussians, 980x545);
+  // a batch trainer replays whole lists.
+  dio_tuning tuning{16, 64, 8, 0, 512, 2};
   StageClock clock;
   // The next step's frame, prepared at the end of a plain step by the fused
   // backward (its preprocess for the next view), and what it was prepared for. The
@@ -1289,7 +1361,7 @@ struct dio_trainer {
   // and no mutating call through the C ABI since (mutation_epoch).
   struct Prepared {
     dio_buf *splats = nullptr, *tile_counts = nullptr, *depth_keys = nullptr, *depth_values = nullptr,
-            *vis = nullptr;
+            *vis = nullptr, *rects = nullptr;
     uint32_t count = 0;  // Gaussians the buffers are sized for
     bool ready = false;
     uint32_t iteration = 0, view = 0, sh_degree = 0, bin_shift = 0;
@@ -1332,6 +1404,7 @@ void batch_destroy(BatchState* b) {
 
 void release_prepared(dio_trainer::Prepared& p) {
   release(p.splats); release(p.tile_counts); release(p.depth_keys); release(p.depth_values); release(p.vis);
+  release(p.rects);
   p.count = 0;
   p.ready = false;
 }
@@ -1399,7 +1472,7 @@ void trainer_optimize(dio_trainer* t) {
   tiles->digit_bits = t->tuning.digit_bits;
   tiles->tile_size = t->tuning.tile_size;
   tiles->persistent = t->tuning.persistent != 0;
-  tiles->per_tile_depth = t->tuning.per_tile_depth != 0;
+  tiles->per_tile_depth = t->tuning.per_tile_depth;
   tiles->segment_shift = t->tuning.segment ? static_cast<uint32_t>(__builtin_ctz(t->tuning.segment)) : 0u;
   if (t->clock.enabled) {
     t->clock.begin(t->ctx->stream);
@@ -1421,6 +1494,7 @@ void trainer_optimize(dio_trainer* t) {
     swap_storage(tiles->tile_counts, prep.tile_counts);
     swap_storage(tiles->depth_keys, prep.depth_keys);
     swap_storage(tiles->depth_values, prep.depth_values);
+    swap_storage(tiles->rects, prep.rects);
     std::swap(t->vis, prep.vis);
     to_camera(view.camera, t->renderer->camera, tiles->bin_shift());
     t->renderer->preprocessed = true;
@@ -1491,6 +1565,7 @@ void trainer_optimize(dio_trainer* t) {
         prep.depth_keys = alloc(t->ctx, n * 4);
         prep.depth_values = alloc(t->ctx, n * 4);
         prep.vis = alloc(t->ctx, n * 8);
+        prep.rects = alloc(t->ctx, n * 8);
         prep.count = t->count;
       }
       to_camera(t->views[t->pending_view].camera, next.cam, tiles->bin_shift());
@@ -1500,6 +1575,7 @@ void trainer_optimize(dio_trainer* t) {
       next.depth_keys = dev<uint32_t>(prep.depth_keys);
       next.depth_values = dev<uint32_t>(prep.depth_values);
       next.vis = dev<float>(prep.vis);
+      next.rects = dev<uint32_t>(prep.rects);
       prep.iteration = step + 1;
       prep.view = t->pending_view;
       prep.sh_degree = next_degree;
@@ -2323,7 +2399,7 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       run = [&] {
         dio::preprocess(s, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
                         dev<float>(sc->splats), dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_keys),
-                        dev<uint32_t>(tr->depth_values));
+                        dev<uint32_t>(tr->depth_values), nullptr, dev<uint32_t>(tr->rects));
       };
     } else if (name == "scan") {
       run = [&] {
@@ -2339,12 +2415,28 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       const uint32_t shift = tr->bin_shift();
       const uint32_t tiles = dio::ceil_div(dio::ceil_div(tr->frame_w, dio::kTile), 1u << shift) *
                              dio::ceil_div(dio::ceil_div(tr->frame_h, dio::kTile), 1u << shift);
-      const bool per_tile = tr->per_tile_depth;
+      const bool per_tile = tr->per_tile_depth != 0;
+      const bool binned = bins_by_counts(tr, tiles);
       const bool in_alt = (tile_sort_bits(tiles, tr->digit_bits) / tr->digit_bits) % 2 == 1;
       auto chain = [&](bool sorts) {
         dio::preprocess(s, dev<float>(sc->gaussians), dev<float>(sc->sh), sc->degree, sc->count, r->camera,
                         dev<float>(sc->splats), dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_keys),
-                        dev<uint32_t>(tr->depth_values));
+                        dev<uint32_t>(tr->depth_values), nullptr, dev<uint32_t>(tr->rects));
+        if (binned) {
+          // Everything from the counts to the ordered lists is the sort here.
+          if (!sorts) return;
+          const dio::BinWorkspace ws = bin_workspace(tr, tiles);
+          dio::bin_count(s, dev<uint32_t>(tr->rects), dev<uint32_t>(tr->depth_keys), dev<uint32_t>(tr->tile_counts),
+                         tr->count, tr->frame_w, tr->frame_h, shift, tr->capacity, dev<uint32_t>(tr->pair_total), ws);
+          uint32_t* keyed = keyed_pairs(tr);
+          dio::bin_scatter(s, dev<uint32_t>(tr->rects), dev<uint32_t>(tr->tile_counts), dev<uint32_t>(tr->depth_keys),
+                           tr->count, tr->frame_w, tr->frame_h, shift, ws, keyed, dev<uint32_t>(tr->ranges),
+                           dev<uint32_t>(tr->tile_order), nullptr);
+          dio::depth_order(s, dev<uint32_t>(tr->depth_keys), dev<uint32_t>(tr->ranges), dev<uint32_t>(tr->tile_order),
+                           tiles, dev<uint32_t>(tr->pair_splats), keyed + 2 * static_cast<size_t>(tr->capacity), false,
+                           keyed);
+          return;
+        }
         if (sorts && !per_tile)
           dio::radix_sort(s, false, tr->depth_keys->ptr, dev<uint32_t>(tr->depth_values), tr->depth_alt_keys->ptr,
                           dev<uint32_t>(tr->depth_alt_values), nullptr, tr->count, 0, 32,
@@ -2370,7 +2462,7 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
                            tr->capacity, tiles, dev<uint32_t>(tr->ranges), dev<uint32_t>(tr->tile_order));
           dio::depth_order(s, dev<uint32_t>(tr->depth_keys), dev<uint32_t>(tr->ranges), dev<uint32_t>(tr->tile_order),
                            tiles, dev<uint32_t>(in_alt ? tr->alt_splats : tr->pair_splats),
-                           dev<uint32_t>(in_alt ? tr->pair_splats : tr->alt_splats));
+                           dev<uint32_t>(in_alt ? tr->pair_splats : tr->alt_splats), true);
         }
       };
       hipEvent_t e0, e1, e2;
@@ -2392,7 +2484,7 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
       (void)hipEventDestroy(e2);
       // Leave the frame as a step leaves it.
       chain(true);
-      tr->sorted_in_alt = (tile_sort_bits(tiles, tr->digit_bits) / tr->digit_bits) % 2 == 1;
+      tr->sorted_in_alt = !binned && (tile_sort_bits(tiles, tr->digit_bits) / tr->digit_bits) % 2 == 1;
       sync(t->ctx);
       *ms = std::max(0.0, static_cast<double>(with_sorts - without)) / repeats;
       return;
@@ -2764,7 +2856,7 @@ void check_tuning(const dio_trainer* t, const dio_tuning& k) {
 // settings.
 std::vector<dio_tuning> tuning_candidates(const dio_trainer* t) {
   std::vector<dio_tuning> out;
-  for (uint32_t per_tile : {0u, 1u})
+  for (uint32_t per_tile : {0u, 1u, 2u})
     for (uint32_t tile : {16u, 8u, 32u})
       for (uint32_t persistent : {0u, 1u})
         for (uint32_t segment : {0u, 256u, 512u})
@@ -2775,6 +2867,7 @@ std::vector<dio_tuning> tuning_candidates(const dio_trainer* t) {
               if (persistent && (tile != 16 || t->batch)) continue;
               if (segment && (tile != 16 || persistent || t->batch)) continue;
               if (per_tile && t->batch) continue;
+              if (per_tile == 2 && bits != 8) continue;  // binning by counts sorts no pairs by radix
               out.push_back(dio_tuning{tile, batch, bits, persistent, segment, per_tile});
             }
   return out;

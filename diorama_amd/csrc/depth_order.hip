@@ -17,6 +17,8 @@
 // list of the same layout: per pass one sweep counts the digits, and a second
 // sweep ranks each chunk in order and writes it behind the pairs already placed.
 // Keys are never stored: each pass reads them again as depth_keys[splat].
+#include <type_traits>
+
 #include "dio_internal.hpp"
 
 namespace dio {
@@ -160,11 +162,22 @@ __device__ inline void block_min_max(OrderLds& L, uint32_t& low, uint32_t& high)
   __syncthreads();  // L.reduce is written again by the next call
 }
 
-// Loads chunk [first, first + n) of `list` (n <= kChunk) in the (wave, row, lane)
-// layout, with each pair's depth key.
-__device__ inline void load_chunk(const uint32_t* __restrict__ list, const uint32_t* __restrict__ depth_keys,
-                                  uint32_t first, uint32_t n, uint32_t rows, uint32_t (&key)[kRows],
-                                  uint32_t (&value)[kRows]) {
+// Loads chunk [first, first + n) of a list (n <= kChunk) in the (wave, row, lane)
+// layout with each pair's depth key: from `keyed` ((key, splat) pairs) when the
+// pairs carry their keys, else from `list` (splats) and depth_keys[splat].
+__device__ inline void load_chunk(const uint32_t* __restrict__ list, const uint2* __restrict__ keyed,
+                                  const uint32_t* __restrict__ depth_keys, uint32_t first, uint32_t n, uint32_t rows,
+                                  uint32_t (&key)[kRows], uint32_t (&value)[kRows]) {
+  if (keyed) {
+#pragma unroll
+    for (int r = 0; r < kRows; r++) {
+      const uint32_t e = element(rows, r);
+      const uint2 p = static_cast<uint32_t>(r) < rows && e < n ? keyed[first + e] : make_uint2(0u, 0u);
+      key[r] = p.x;
+      value[r] = p.y;
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < kRows; r++) {
     const uint32_t e = element(rows, r);
@@ -177,16 +190,23 @@ __device__ inline void load_chunk(const uint32_t* __restrict__ list, const uint3
   }
 }
 
-// Radix passes over the 8-bit digits where `diff` has bits, in LDS, on a list of at
-// most kChunk pairs held in registers; writes the list back in place.
+// Radix passes in LDS on a list of at most kChunk pairs held in registers, over the
+// 8-bit digits where `value_diff` has bits (splat indices, first) and then where
+// `diff` has bits (depth keys): stable passes, so the list ends in (key, splat)
+// order whatever order it arrived in. Writes the list back in place.
 __device__ void radix_in_lds(OrderLds& L, uint32_t* __restrict__ list, uint32_t lo, uint32_t n, uint32_t rows,
-                             uint32_t diff, uint32_t (&key)[kRows], uint32_t (&value)[kRows]) {
+                             uint32_t value_diff, uint32_t diff, uint32_t (&key)[kRows], uint32_t (&value)[kRows]) {
   uint32_t pos[kRows];
   // One LDS copy serves every pass: a pass reads its keys back before the next
   // pass's first barrier, and scatters only after it.
-  for (uint32_t shift = 0; shift < 32; shift += 8) {
-    if (((diff >> shift) & (kRadix - 1)) == 0) continue;
-    rank_chunk(L, key, n, rows, shift, pos);
+  for (uint32_t p = 0; p < 8; p++) {
+    const bool by_value = p < 4;
+    const uint32_t shift = (p & 3u) * 8u;
+    if ((((by_value ? value_diff : diff) >> shift) & (kRadix - 1)) == 0) continue;
+    uint32_t digits[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; r++) digits[r] = by_value ? value[r] : key[r];
+    rank_chunk(L, digits, n, rows, shift, pos);
 #pragma unroll
     for (int r = 0; r < kRows; r++) {
       if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
@@ -211,6 +231,18 @@ __device__ void radix_in_lds(OrderLds& L, uint32_t* __restrict__ list, uint32_t 
   }
 }
 
+// The splat-index bits that differ somewhere in a list held in registers (0 for a
+// list that arrives in splat order, which needs no passes over them).
+__device__ inline uint32_t value_bits(OrderLds& L, bool ordered, uint32_t first, uint32_t n, uint32_t rows,
+                                      const uint32_t (&value)[kRows]) {
+  if (ordered) return 0u;
+  uint32_t diff = 0;
+#pragma unroll
+  for (int r = 0; r < kRows; r++)
+    if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) diff |= value[r] ^ first;
+  return block_reduce<Reduce::kOr>(L, diff);
+}
+
 // (key, splat) before (key', splat'): depth first, then splat order. The splats of
 // one list are distinct, so this is a strict order.
 __device__ inline bool before(uint32_t k, uint32_t v, uint32_t k2, uint32_t v2) {
@@ -221,16 +253,18 @@ __device__ inline bool before(uint32_t k, uint32_t v, uint32_t k2, uint32_t v2) 
 // keys go to kBuckets buckets by a map that never decreases with the key (a float
 // scale of key - min over the list's key range); each pair's place inside its
 // bucket, whose pairs the LDS atomics put in no particular order, is the number of
-// the bucket's pairs before it in (key, splat) order. A list arrives in splat
-// order, so (key, splat) order is the stable depth order. A bucket of more than
+// the bucket's pairs before it in (key, splat) order, the stable depth order of a
+// list in splat order, whatever order the list arrives in. A bucket of more than
 // kBucketLimit pairs (depths packed closer than the buckets resolve, or many equal
-// depths) sends the list through radix passes instead.
+// depths) sends the list through radix passes instead, which also pass over the
+// splat indices unless the list is `ordered` (arrives in splat order). The list
+// comes from `keyed` when given (never `ordered` then) and always leaves in `list`.
 __device__ void order_in_lds(OrderLds& L, const uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ list,
-                             uint32_t lo, uint32_t n) {
+                             const uint2* __restrict__ keyed, uint32_t lo, uint32_t n, bool ordered) {
   const int tid = threadIdx.x;
   const uint32_t rows = ceil_div(n, kThreads);
   uint32_t key[kRows], value[kRows];
-  load_chunk(list, depth_keys, lo, n, rows, key, value);
+  load_chunk(list, keyed, depth_keys, lo, n, rows, key, value);
   uint32_t low = ~0u, high = 0u;
 #pragma unroll
   for (int r = 0; r < kRows; r++) {
@@ -243,7 +277,11 @@ __device__ void order_in_lds(OrderLds& L, const uint32_t* __restrict__ depth_key
   for (uint32_t b = 0; b < kBucketsPerThread; b++) L.bucket[tid * kBucketsPerThread + b] = 0;
   if (tid == 0) L.crowded = 0;
   block_min_max(L, low, high);  // its barriers also order the clearing above
-  if (low == high) return;  // every key the same: the list is already in order
+  const uint32_t first = keyed ? keyed[lo].y : list[lo];  // one splat of the list: the reference its bits differ from
+  if (low == high) {  // every key the same: splat order is the order
+    if (!ordered) radix_in_lds(L, list, lo, n, rows, value_bits(L, false, first, n, rows, value), 0u, key, value);
+    return;
+  }
   const float scale = static_cast<float>(kBuckets) / (static_cast<float>(high - low) + 1.f);
   uint32_t bucket[kRows], slot[kRows];
 #pragma unroll
@@ -268,7 +306,8 @@ __device__ void order_in_lds(OrderLds& L, const uint32_t* __restrict__ depth_key
 #pragma unroll
     for (int r = 0; r < kRows; r++)
       if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) diff |= key[r] ^ low;
-    radix_in_lds(L, list, lo, n, rows, block_reduce<Reduce::kOr>(L, diff), key, value);
+    diff = block_reduce<Reduce::kOr>(L, diff);
+    radix_in_lds(L, list, lo, n, rows, value_bits(L, ordered, first, n, rows, value), diff, key, value);
     return;
   }
 #pragma unroll
@@ -306,35 +345,60 @@ __device__ void order_in_lds(OrderLds& L, const uint32_t* __restrict__ depth_key
   }
 }
 
-// A list longer than kChunk: the same passes streamed through `list` and `scratch`
-// (the same pair positions of another pair-sized buffer).
-__device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_keys, uint32_t* list,
-                               uint32_t* scratch, uint32_t lo, uint32_t n) {
+// A list longer than kChunk: the same passes streamed through the list and a
+// scratch list (the same pair positions of another pair-sized buffer), over the
+// splat indices first unless the list is `ordered`. kKeyed: the list is `keyed`
+// ((key, splat) pairs, ping-ponged with an 8-byte scratch) and its splats leave in
+// `list`; else it is `list` itself (splats, ping-ponged with a 4-byte scratch).
+template <bool kKeyed>
+__device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_keys, uint32_t* list, uint2* keyed,
+                               void* scratch, uint32_t lo, uint32_t n, bool ordered) {
+  using Elem = typename std::conditional<kKeyed, uint2, uint32_t>::type;
   const int tid = threadIdx.x;
   uint32_t key[kRows], value[kRows], pos[kRows];
-  const uint32_t key0 = depth_keys[list[lo]];
-  uint32_t diff = 0;
+  Elem* src = kKeyed ? reinterpret_cast<Elem*>(keyed) : reinterpret_cast<Elem*>(list);
+  Elem* dst = static_cast<Elem*>(scratch);
+  auto load = [&](const Elem* from, uint32_t first, uint32_t m) {
+    load_chunk(kKeyed ? nullptr : reinterpret_cast<const uint32_t*>(from),
+               kKeyed ? reinterpret_cast<const uint2*>(from) : nullptr, depth_keys, first, m, kRows, key, value);
+  };
+  uint32_t value0, key0;
+  if (kKeyed) {
+    const uint2 p = keyed[lo];
+    key0 = p.x;
+    value0 = p.y;
+  } else {
+    value0 = list[lo];
+    key0 = depth_keys[value0];
+  }
+  uint32_t diff = 0, value_diff = 0;
   for (uint32_t c = 0; c < n; c += kChunk) {
     const uint32_t m = min(kChunk, n - c);
-    load_chunk(list, depth_keys, lo + c, m, kRows, key, value);
+    load(src, lo + c, m);
 #pragma unroll
-    for (int r = 0; r < kRows; r++)
-      if (element(kRows, r) < m) diff |= key[r] ^ key0;
+    for (int r = 0; r < kRows; r++) {
+      if (element(kRows, r) < m) {
+        diff |= key[r] ^ key0;
+        value_diff |= value[r] ^ value0;
+      }
+    }
   }
   diff = block_reduce<Reduce::kOr>(L, diff);
-  uint32_t* src = list;
-  uint32_t* dst = scratch;
-  for (uint32_t shift = 0; shift < 32; shift += 8) {
-    if (((diff >> shift) & (kRadix - 1)) == 0) continue;
+  value_diff = ordered ? 0u : block_reduce<Reduce::kOr>(L, value_diff);
+  for (uint32_t p = 0; p < 8; p++) {
+    const bool by_value = p < 4;
+    const uint32_t shift = (p & 3u) * 8u;
+    if ((((by_value ? value_diff : diff) >> shift) & (kRadix - 1)) == 0) continue;
     // Sweep 1: where each digit's run starts in the output.
     L.run_start[tid] = 0;
     __syncthreads();
     for (uint32_t c = 0; c < n; c += kChunk) {
       const uint32_t m = min(kChunk, n - c);
-      load_chunk(src, depth_keys, lo + c, m, kRows, key, value);
+      load(src, lo + c, m);
 #pragma unroll
       for (int r = 0; r < kRows; r++)
-        if (element(kRows, r) < m) atomicAdd(&L.run_start[(key[r] >> shift) & (kRadix - 1)], 1u);
+        if (element(kRows, r) < m)
+          atomicAdd(&L.run_start[((by_value ? value[r] : key[r]) >> shift) & (kRadix - 1)], 1u);
     }
     __syncthreads();
     const uint32_t start = block_exclusive(L, L.run_start[tid]);
@@ -343,13 +407,20 @@ __device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_k
     // Sweep 2: chunks in order, each digit's pairs behind those placed before.
     for (uint32_t c = 0; c < n; c += kChunk) {
       const uint32_t m = min(kChunk, n - c);
-      load_chunk(src, depth_keys, lo + c, m, kRows, key, value);
-      const uint32_t total = rank_chunk(L, key, m, kRows, shift, pos);
+      load(src, lo + c, m);
+      uint32_t digits[kRows];
+#pragma unroll
+      for (int r = 0; r < kRows; r++) digits[r] = by_value ? value[r] : key[r];
+      const uint32_t total = rank_chunk(L, digits, m, kRows, shift, pos);
 #pragma unroll
       for (int r = 0; r < kRows; r++) {
         if (element(kRows, r) < m) {
-          const uint32_t d = (key[r] >> shift) & (kRadix - 1);
-          dst[lo + L.run_start[d] + pos[r] - L.chunk_start[d]] = value[r];
+          const uint32_t d = (digits[r] >> shift) & (kRadix - 1);
+          const uint32_t at = lo + L.run_start[d] + pos[r] - L.chunk_start[d];
+          if constexpr (kKeyed)
+            dst[at] = make_uint2(key[r], value[r]);
+          else
+            dst[at] = value[r];
         }
       }
       __syncthreads();
@@ -358,28 +429,42 @@ __device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_k
     // The workgroup's own stores are seen by its later loads after a barrier (all
     // its waves share one CU's L1, which stores write through).
     __syncthreads();
-    uint32_t* t = src;
+    Elem* t = src;
     src = dst;
     dst = t;
   }
-  if (src != list) {
-    for (uint32_t k = tid; k < n; k += kThreads) list[lo + k] = src[lo + k];
+  if constexpr (kKeyed) {
+    for (uint32_t k = tid; k < n; k += kThreads) list[lo + k] = src[lo + k].y;
+  } else {
+    if (src != list)
+      for (uint32_t k = tid; k < n; k += kThreads) list[lo + k] = src[lo + k];
   }
 }
 
+// `ordered`: every list arrives in splat order (pairs written in splat order and
+// sorted stably by tile); otherwise the lists are in any order (binned by count).
+// keyed (nullable): the lists as (depth key, splat) pairs, at the same positions,
+// sorted into `list`; `scratch` then holds 8 bytes a pair, else 4.
 __global__ __launch_bounds__(kThreads) void depth_order_kernel(const uint32_t* __restrict__ depth_keys,
                                                                 const uint2* __restrict__ ranges,
                                                                 const uint32_t* __restrict__ tile_order,
-                                                                uint32_t* list, uint32_t* scratch) {
+                                                                uint32_t* list, uint2* keyed, void* scratch,
+                                                                bool ordered) {
   __shared__ OrderLds L;
   const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
   const uint2 range = ranges[tile];
   const uint32_t n = range.y - range.x;
-  if (n <= 1) return;
+  if (n == 0) return;
+  if (n == 1) {
+    if (keyed && threadIdx.x == 0) list[range.x] = keyed[range.x].y;
+    return;
+  }
   if (n <= kChunk)
-    order_in_lds(L, depth_keys, list, range.x, n);
+    order_in_lds(L, depth_keys, list, keyed, range.x, n, ordered);
+  else if (keyed)
+    order_streamed<true>(L, depth_keys, list, keyed, scratch, range.x, n, ordered);
   else
-    order_streamed(L, depth_keys, list, scratch, range.x, n);
+    order_streamed<false>(L, depth_keys, list, nullptr, scratch, range.x, n, ordered);
 }
 
 // A frame whose pairs overflow the capacity keeps, as the reference does
@@ -425,13 +510,12 @@ __device__ inline uint32_t cut_exclusive(CutLds& L, uint32_t x, uint32_t* total)
   return before + inclusive - x;
 }
 
-__global__ __launch_bounds__(kCutThreads) void pair_cut_kernel(const uint32_t* __restrict__ depth_keys,
-                                                                uint32_t* __restrict__ counts,
-                                                                uint32_t* __restrict__ offsets, uint32_t n,
-                                                                const uint32_t* __restrict__ pair_total,
-                                                                uint32_t capacity) {
-  if (*pair_total <= capacity) return;
-  __shared__ CutLds L;
+// The cut of a frame whose pairs overflow `capacity`: the depth key of the splat
+// holding the capacity-th pair slot in (key, splat) order (`fixed`), that splat
+// (`cut`) and how many of its pairs it keeps (`keep`). Every thread of the
+// workgroup gets the same three.
+__device__ void find_cut(CutLds& L, const uint32_t* __restrict__ depth_keys, const uint32_t* __restrict__ counts,
+                         uint32_t n, uint32_t capacity, uint32_t& fixed_out, uint32_t& cut_out, uint32_t& keep_out) {
   const int tid = threadIdx.x;
   // 1. The cut splat's key, most significant digit first: the digit whose splats hold
   //    the target-th of the remaining slots.
@@ -492,16 +576,36 @@ __global__ __launch_bounds__(kCutThreads) void pair_cut_kernel(const uint32_t* _
     running += step_total;
   }
   __syncthreads();
-  const uint32_t cut = L.cut, keep = L.keep;
+  fixed_out = fixed;
+  cut_out = L.cut;
+  keep_out = L.keep;
+}
+
+// The pairs splat j keeps under the cut: all before the cut in (key, splat) order,
+// `keep` at the cut, none after (none at all at capacity 0).
+__device__ inline uint32_t kept_pairs(uint32_t c, uint32_t k, uint32_t j, uint32_t capacity, uint32_t fixed,
+                                      uint32_t cut, uint32_t keep) {
+  return c == 0 || capacity == 0 ? 0u : k < fixed || (k == fixed && j < cut) ? c : j == cut ? keep : 0u;
+}
+
+__global__ __launch_bounds__(kCutThreads) void pair_cut_kernel(const uint32_t* __restrict__ depth_keys,
+                                                                uint32_t* __restrict__ counts,
+                                                                uint32_t* __restrict__ offsets, uint32_t n,
+                                                                const uint32_t* __restrict__ pair_total,
+                                                                uint32_t capacity) {
+  if (*pair_total <= capacity) return;
+  __shared__ CutLds L;
+  const int tid = threadIdx.x;
+  uint32_t fixed, cut, keep;
+  find_cut(L, depth_keys, counts, n, capacity, fixed, cut, keep);
   // 3. Kept pairs per splat, and the slot offsets over them.
-  running = 0;
+  uint32_t running = 0;
   for (uint32_t base = 0; base < n; base += kCutStep) {
     uint32_t kept[kCutItems], mine = 0;
 #pragma unroll
     for (int i = 0; i < kCutItems; i++) {
       const uint32_t j = base + tid * kCutItems + i;
-      const uint32_t c = j < n ? counts[j] : 0u, k = j < n ? depth_keys[j] : 0u;
-      kept[i] = c == 0 || capacity == 0 ? 0u : k < fixed || (k == fixed && j < cut) ? c : j == cut ? keep : 0u;
+      kept[i] = j < n ? kept_pairs(counts[j], depth_keys[j], j, capacity, fixed, cut, keep) : 0u;
       mine += kept[i];
     }
     uint32_t step_total;
@@ -519,6 +623,434 @@ __global__ __launch_bounds__(kCutThreads) void pair_cut_kernel(const uint32_t* _
   }
 }
 
+// ------------------------------------------------------ binning by counts --
+// The pairs binned by tile without a sort (per_tile_depth 2): the splats are cut
+// into chunks, one per workgroup, and
+// 1. bin_count: each chunk counts its pairs per tile in LDS. A splat covers a
+//    rectangle of tiles, so it adds +1 / -1 at the rectangle's four corners of a
+//    per-tile difference grid (four LDS atomics however many tiles it covers), and
+//    a 2D prefix sum turns the grid into counts: one row of the (chunk x tile)
+//    count matrix per chunk, and the chunk's pair total.
+// 2. bin_cut: one workgroup adds the chunk totals into the frame's pair count and,
+//    when the pairs overflow the capacity, cuts them exactly as pair_cut does and
+//    counts the matrix again from the kept pairs.
+// 3. bin_scan: per tile, the exclusive prefix of its counts over the chunks, and its
+//    total.
+// 4. bin_scatter: each workgroup places its tiles' bases (a prefix over the tile
+//    totals in LDS) plus its own chunk's prefixes in LDS cursors and writes each
+//    pair to a slot taken from its tile's cursor. Workgroup 0 also writes the ranges
+//    and the rasterizer's tile order.
+// Within a tile the pairs land in no particular order; depth_order then sorts each
+// list by (depth key, splat), the order of the reference's lists.
+constexpr int kBinThreads = 256;
+constexpr int kBinWaves = kBinThreads / kWave;
+constexpr uint32_t kBinItems = 8;                        // splats per thread per round
+constexpr uint32_t kBinRound = kBinThreads * kBinItems;  // 2048
+constexpr uint32_t kBinMaxGroups = 512;
+constexpr int kScanColumns = 64, kScanBlocks = 16;       // bin_scan: 1024 threads
+constexpr uint32_t kScanRows = kBinMaxGroups / kScanBlocks;
+
+// The chunk workgroup b takes. Workgroup b runs on XCD b mod 8, and each tile's run
+// holds the chunks' pairs in chunk order, a few pairs per chunk: with the chunks of
+// one XCD adjacent (groups a multiple of 8), a run is written through one XCD's L2
+// in whole lines but for 8 seams, where consecutive chunks on different XCDs would
+// leave a partly written line in two L2s at almost every pair.
+__device__ inline uint32_t chunk_of(uint32_t b, uint32_t groups) {
+  return groups % 8 == 0 ? (b % 8) * (groups / 8) + b / 8 : b;
+}
+
+struct BinShape {
+  uint32_t tiles_x, tiles_y, shift;  // 16-px tiles and the bin shift
+  uint32_t bx, by, tiles;            // bins
+};
+
+__device__ inline BinShape bin_shape(uint32_t width, uint32_t height, uint32_t shift) {
+  BinShape b;
+  b.tiles_x = ceil_div(width, kTile);
+  b.tiles_y = ceil_div(height, kTile);
+  b.shift = shift;
+  b.bx = (b.tiles_x + (1u << shift) - 1u) >> shift;
+  b.by = (b.tiles_y + (1u << shift) - 1u) >> shift;
+  b.tiles = b.bx * b.by;
+  return b;
+}
+
+// A splat's bin rectangle (pack_rect) and its pair count, at most the rectangle's
+// area; a rectangle past the grid (a stale one) counts no pairs.
+__device__ inline uint32_t splat_bins(uint2 packed, uint32_t count, const BinShape& b, uint32_t r[4]) {
+  unpack_rect(packed, r);
+  if (r[2] > b.bx || r[3] > b.by || r[2] <= r[0] || r[3] <= r[1]) return 0u;
+  return min(count, (r[2] - r[0]) * (r[3] - r[1]));
+}
+
+// Loads of inputs read once (non-temporal: they do not take L2 lines from the pairs
+// being written).
+__device__ inline uint32_t stream_load(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+__device__ inline uint2 stream_load(const uint2* p) {
+  const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(p));
+  return make_uint2(static_cast<uint32_t>(v), static_cast<uint32_t>(v >> 32));
+}
+
+// +1 over the bins [x0, x1) x [y0, y1) of a bx x by difference grid: four corner
+// updates (corners past the grid's last row or column fall away).
+__device__ inline void add_rect(uint32_t* grid, const BinShape& b, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1) {
+  if (x1 <= x0 || y1 <= y0) return;
+  atomicAdd(&grid[y0 * b.bx + x0], 1u);
+  if (x1 < b.bx) atomicAdd(&grid[y0 * b.bx + x1], ~0u);
+  if (y1 < b.by) {
+    atomicAdd(&grid[y1 * b.bx + x0], ~0u);
+    if (x1 < b.bx) atomicAdd(&grid[y1 * b.bx + x1], 1u);
+  }
+}
+
+// The first c bins of rectangle r, row by row (the order tiles.wgsl:47-58 writes
+// them): its whole rows, then the start of the next.
+__device__ inline void add_pairs(uint32_t* grid, const BinShape& b, const uint32_t r[4], uint32_t c) {
+  const uint32_t w = r[2] - r[0];
+  const uint32_t rows = c / w, rest = c - rows * w;
+  add_rect(grid, b, r[0], r[1], r[2], r[1] + rows);
+  add_rect(grid, b, r[0], r[1] + rows, r[0] + rest, r[1] + rows + 1);
+}
+
+// The difference grid summed into counts: rows a wave each (64 bins a step), then
+// columns a thread each. Ends with a barrier.
+template <int kN>
+__device__ inline void grid_prefix(uint32_t* grid, const BinShape& b) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  for (uint32_t y = wave; y < b.by; y += kN / kWave) {
+    uint32_t carry = 0;
+    for (uint32_t x0 = 0; x0 < b.bx; x0 += kWave) {
+      const uint32_t x = x0 + lane;
+      uint32_t v = x < b.bx ? grid[y * b.bx + x] : 0u;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, kWave);
+        if (lane >= o) v += u;
+      }
+      v += carry;
+      if (x < b.bx) grid[y * b.bx + x] = v;
+      carry = __shfl(v, kWave - 1, kWave);
+    }
+  }
+  __syncthreads();
+  for (uint32_t x = tid; x < b.bx; x += kN) {
+    uint32_t run = 0;
+    for (uint32_t y = 0; y < b.by; y++) {
+      run += grid[y * b.bx + x];
+      grid[y * b.bx + x] = run;
+    }
+  }
+  __syncthreads();
+}
+
+// Sum of x over the workgroup (kN threads), in `slots` (kN / 64 words); every thread
+// gets it. Starts and ends with its own barriers.
+template <int kN>
+__device__ inline uint32_t group_sum(uint32_t* slots, uint32_t x) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+  __syncthreads();
+  if (lane == 0) slots[wave] = x;
+  __syncthreads();
+  uint32_t all = 0;
+#pragma unroll
+  for (int w = 0; w < kN / kWave; w++) all += slots[w];
+  return all;
+}
+
+__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(const uint2* __restrict__ rects,
+                                                                 const uint32_t* __restrict__ counts, uint32_t n,
+                                                                 uint32_t chunk, uint32_t width, uint32_t height,
+                                                                 uint32_t shift, uint32_t* __restrict__ matrix,
+                                                                 uint32_t* __restrict__ partial) {
+  __shared__ uint32_t grid[kBinMaxTiles];
+  __shared__ uint32_t slots[kBinWaves];
+  const int tid = threadIdx.x;
+  const BinShape b = bin_shape(width, height, shift);
+  for (uint32_t t = tid; t < b.tiles; t += kBinThreads) grid[t] = 0;
+  __syncthreads();
+  const uint32_t g = chunk_of(blockIdx.x, gridDim.x);
+  const uint32_t first = min(n, g * chunk), last = min(first + chunk, n);
+  uint32_t sum = 0;
+  for (uint32_t base = first; base < last; base += kBinRound) {
+    uint32_t c[kBinItems];
+    uint2 rect[kBinItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kBinItems; k++) {
+      const uint32_t j = base + k * kBinThreads + tid;
+      c[k] = j < last ? stream_load(counts + j) : 0u;
+      rect[k] = j < last ? stream_load(rects + j) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kBinItems; k++) {
+      uint32_t r[4];
+      const uint32_t kept = splat_bins(rect[k], c[k], b, r);
+      if (kept) {
+        add_pairs(grid, b, r, kept);
+        sum += kept;
+      }
+    }
+  }
+  __syncthreads();
+  grid_prefix<kBinThreads>(grid, b);
+  uint32_t* row = matrix + static_cast<size_t>(g) * b.tiles;
+  for (uint32_t t = tid; t < b.tiles; t += kBinThreads) row[t] = grid[t];
+  sum = group_sum<kBinThreads>(slots, sum);
+  if (tid == 0) partial[g] = sum;
+}
+
+// The frame's pair count (pair_total[0], uncut) from the chunk totals; on overflow
+// the cut, the count matrix counted again from the kept pairs and the tile counts
+// rewritten to them. One workgroup of kCutThreads.
+__global__ __launch_bounds__(kCutThreads) void bin_cut_kernel(const uint2* __restrict__ rects,
+                                                               const uint32_t* __restrict__ depth_keys,
+                                                               uint32_t* __restrict__ counts, uint32_t n,
+                                                               uint32_t chunk, uint32_t groups, uint32_t width,
+                                                               uint32_t height, uint32_t shift,
+                                                               uint32_t* __restrict__ matrix,
+                                                               const uint32_t* __restrict__ partial,
+                                                               uint32_t* __restrict__ pair_total, uint32_t capacity) {
+  __shared__ CutLds L;
+  __shared__ uint32_t grid[kBinMaxTiles];
+  __shared__ uint32_t slots[kCutWaves];
+  const int tid = threadIdx.x;
+  uint32_t mine = 0;
+  for (uint32_t g = tid; g < groups; g += kCutThreads) mine += partial[g];
+  const uint32_t total = group_sum<kCutThreads>(slots, mine);
+  if (tid == 0) pair_total[0] = total;
+  if (total <= capacity) return;
+  uint32_t fixed, cut, keep;
+  find_cut(L, depth_keys, counts, n, capacity, fixed, cut, keep);
+  const BinShape b = bin_shape(width, height, shift);
+  // Rows again, from the kept pairs, before the counts are rewritten.
+  for (uint32_t g = 0; g < groups; g++) {
+    for (uint32_t t = tid; t < b.tiles; t += kCutThreads) grid[t] = 0;
+    __syncthreads();
+    const uint32_t first = min(n, g * chunk), last = min(first + chunk, n);
+    for (uint32_t j = first + tid; j < last; j += kCutThreads) {
+      const uint32_t kept = kept_pairs(counts[j], depth_keys[j], j, capacity, fixed, cut, keep);
+      if (kept) {
+        uint32_t r[4];
+        const uint32_t c = splat_bins(rects[j], kept, b, r);
+        if (c) add_pairs(grid, b, r, c);
+      }
+    }
+    __syncthreads();
+    grid_prefix<kCutThreads>(grid, b);
+    uint32_t* row = matrix + static_cast<size_t>(g) * b.tiles;
+    for (uint32_t t = tid; t < b.tiles; t += kCutThreads) row[t] = grid[t];
+    __syncthreads();
+  }
+  for (uint32_t j = tid; j < n; j += kCutThreads) counts[j] = kept_pairs(counts[j], depth_keys[j], j, capacity, fixed, cut, keep);
+}
+
+// Per tile (a column of the count matrix): the exclusive prefix of its counts over
+// the chunks, in place, and its total. A workgroup takes 64 tiles; each of its 16
+// row blocks sums its rows, the blocks' sums are scanned in LDS, and each block
+// writes its prefixes back.
+__global__ __launch_bounds__(kScanColumns * kScanBlocks) void bin_scan_kernel(uint32_t* __restrict__ matrix,
+                                                                               uint32_t groups, uint32_t tiles,
+                                                                               uint32_t* __restrict__ totals) {
+  __shared__ uint32_t block_sum[kScanBlocks][kScanColumns];
+  const int tid = threadIdx.x, c = tid % kScanColumns, blk = tid / kScanColumns;
+  const uint32_t t = blockIdx.x * kScanColumns + c;
+  const uint32_t rows = ceil_div(groups, kScanBlocks), r0 = blk * rows;
+  uint32_t v[kScanRows];
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanRows; k++) {
+    v[k] = 0;
+    if (k < rows && r0 + k < groups && t < tiles) v[k] = matrix[static_cast<size_t>(r0 + k) * tiles + t];
+    sum += v[k];
+  }
+  block_sum[blk][c] = sum;
+  __syncthreads();
+  uint32_t run = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < kScanBlocks; k++) {
+    const uint32_t x = block_sum[k][c];
+    run += k < blk ? x : 0u;
+    total += x;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kScanRows; k++) {
+    if (k < rows && r0 + k < groups && t < tiles) {
+      matrix[static_cast<size_t>(r0 + k) * tiles + t] = run;
+      run += v[k];
+    }
+  }
+  if (blk == 0 && t < tiles) totals[t] = total;
+}
+
+struct ScatterLds {
+  uint32_t cursor[kBinMaxTiles];
+  uint32_t start[kBinWaves][kWave], x0[kBinWaves][kWave], y0[kBinWaves][kWave], w[kBinWaves][kWave],
+      key[kBinWaves][kWave];
+  uint32_t next[kOrderBuckets];  // workgroup 0: the tile order's buckets
+  uint32_t slots[kBinWaves];
+};
+
+// Exclusive prefix of x over the workgroup; brackets its use of `slots` with barriers.
+__device__ inline uint32_t bin_exclusive(uint32_t* slots, uint32_t x) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  uint32_t inclusive = x;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(inclusive, o, kWave);
+    if (lane >= o) inclusive += y;
+  }
+  __syncthreads();
+  if (lane == kWave - 1) slots[wave] = inclusive;
+  __syncthreads();
+  uint32_t before = 0;
+#pragma unroll
+  for (int w = 0; w < kBinWaves; w++) before += w < wave ? slots[w] : 0u;
+  return before + inclusive - x;
+}
+
+constexpr uint32_t kTilesPerThread = kBinMaxTiles / kBinThreads;  // 32
+
+// Workgroup 0's share: the ranges, the rasterizer's launch order (tiles by falling
+// pair count in steps of kOrderStep, ties in any order) and the cleared item counts.
+// tot[k] is tile tid + k kBinThreads's total; L.cursor holds the tiles' bases.
+__device__ void write_ranges_and_order(ScatterLds& L, const uint32_t (&tot)[kTilesPerThread], uint32_t tiles,
+                                       uint2* __restrict__ ranges, uint32_t* __restrict__ order,
+                                       uint32_t* __restrict__ clear) {
+  const int tid = threadIdx.x;
+  if (clear && tid <= static_cast<int>(kPartClasses)) clear[tid] = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kTilesPerThread; k++) {
+    const uint32_t t = tid + k * kBinThreads;
+    if (t < tiles) ranges[t] = make_uint2(L.cursor[t], L.cursor[t] + tot[k]);
+  }
+  if (!order) return;
+  auto bucket = [](uint32_t total) { return kOrderBuckets - 1 - min(total / kOrderStep, kOrderBuckets - 1); };
+  for (uint32_t k = tid; k < kOrderBuckets; k += kBinThreads) L.next[k] = 0;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < kTilesPerThread; k++)
+    if (tid + k * kBinThreads < tiles) atomicAdd(&L.next[bucket(tot[k])], 1u);
+  __syncthreads();
+  constexpr uint32_t kPer = kOrderBuckets / kBinThreads;
+  uint32_t c[kPer], mine = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; k++) {
+    c[k] = L.next[tid * kPer + k];
+    mine += c[k];
+  }
+  uint32_t run = bin_exclusive(L.slots, mine);
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; k++) {
+    L.next[tid * kPer + k] = run;
+    run += c[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < kTilesPerThread; k++) {
+    const uint32_t t = tid + k * kBinThreads;
+    if (t < tiles) order[atomicAdd(&L.next[bucket(tot[k])], 1u)] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(const uint2* __restrict__ rects,
+                                                                   const uint32_t* __restrict__ counts,
+                                                                   const uint32_t* __restrict__ depth_keys, uint32_t n,
+                                                                   uint32_t chunk, uint32_t width, uint32_t height,
+                                                                   uint32_t shift, const uint32_t* __restrict__ matrix,
+                                                                   const uint32_t* __restrict__ totals,
+                                                                   uint2* __restrict__ keyed,
+                                                                   uint2* __restrict__ ranges,
+                                                                   uint32_t* __restrict__ order,
+                                                                   uint32_t* __restrict__ clear) {
+  __shared__ ScatterLds L;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const BinShape b = bin_shape(width, height, shift);
+  const uint32_t g = chunk_of(blockIdx.x, gridDim.x);
+  const uint32_t* row = matrix + static_cast<size_t>(g) * b.tiles;
+  // 1. The tile totals and this chunk's prefixes, tile tid + k kBinThreads in slot k,
+  //    every load in flight at once.
+  uint32_t tot[kTilesPerThread], mine_row[kTilesPerThread];
+#pragma unroll
+  for (uint32_t k = 0; k < kTilesPerThread; k++) {
+    const uint32_t t = tid + k * kBinThreads;
+    tot[k] = t < b.tiles ? stream_load(totals + t) : 0u;
+    mine_row[k] = t < b.tiles ? stream_load(row + t) : 0u;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kTilesPerThread; k++)
+    if (tid + k * kBinThreads < b.tiles) L.cursor[tid + k * kBinThreads] = tot[k];
+  __syncthreads();
+  // Where each tile's run starts: an exclusive prefix of the totals, a thread taking
+  // consecutive tiles in LDS.
+  const uint32_t per = ceil_div(b.tiles, kBinThreads), t0 = min(b.tiles, tid * per), t1 = min(b.tiles, t0 + per);
+  uint32_t mine = 0;
+  for (uint32_t t = t0; t < t1; t++) mine += L.cursor[t];
+  uint32_t run = bin_exclusive(L.slots, mine);  // its barriers order the reads above before the writes below
+  for (uint32_t t = t0; t < t1; t++) {
+    const uint32_t total = L.cursor[t];
+    L.cursor[t] = run;
+    run += total;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    write_ranges_and_order(L, tot, b.tiles, ranges, order, clear);
+    __syncthreads();
+  }
+  // 2. This chunk's slots in each tile: behind the earlier chunks' pairs.
+#pragma unroll
+  for (uint32_t k = 0; k < kTilesPerThread; k++)
+    if (tid + k * kBinThreads < b.tiles) L.cursor[tid + k * kBinThreads] += mine_row[k];
+  __syncthreads();
+  // 3. Each wave takes windows of 64 consecutive splats (kBinItems of them loaded at
+  //    once) and deals their pairs out to its lanes: pair q of a window belongs to
+  //    the last splat whose first pair is at or before q.
+  const uint32_t first = min(n, g * chunk), last = min(first + chunk, n);
+  for (uint32_t base = first + wave * kWave; base < last; base += kBinWaves * kWave * kBinItems) {
+    uint32_t c[kBinItems], key[kBinItems];
+    uint2 rect[kBinItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kBinItems; k++) {
+      const uint32_t j = base + k * kBinWaves * kWave + lane;
+      c[k] = j < last ? stream_load(counts + j) : 0u;
+      rect[k] = j < last ? stream_load(rects + j) : make_uint2(0u, 0u);
+      key[k] = j < last ? stream_load(depth_keys + j) : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kBinItems; k++) {
+      const uint32_t window = base + k * kBinWaves * kWave;
+      if (window >= last) break;
+      uint32_t r[4];
+      const uint32_t kept = splat_bins(rect[k], c[k], b, r);
+      uint32_t inclusive = kept;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(inclusive, o, kWave);
+        if (lane >= o) inclusive += y;
+      }
+      const uint32_t pairs = __shfl(inclusive, kWave - 1, kWave);
+      if (pairs == 0) continue;
+      L.start[wave][lane] = inclusive - kept;
+      L.x0[wave][lane] = r[0];
+      L.y0[wave][lane] = r[1];
+      L.w[wave][lane] = r[2] - r[0];
+      L.key[wave][lane] = key[k];
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t q = lane; q < pairs; q += kWave) {
+        uint32_t lo = 0;  // the last lane whose first pair is at or before q
+#pragma unroll
+        for (uint32_t step = kWave / 2; step > 0; step >>= 1)
+          if (L.start[wave][lo + step] <= q) lo += step;
+        const uint32_t k2 = q - L.start[wave][lo], width_ = L.w[wave][lo];
+        const uint32_t rowi = k2 / width_;
+        const uint32_t tile = (L.y0[wave][lo] + rowi) * b.bx + L.x0[wave][lo] + (k2 - rowi * width_);
+        keyed[atomicAdd(&L.cursor[tile], 1u)] = make_uint2(L.key[wave][lo], window + lo);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
+
 }  // namespace
 
 void pair_cut(hipStream_t s, const uint32_t* depth_keys, uint32_t* tile_counts, uint32_t* offsets, uint32_t n,
@@ -529,11 +1061,50 @@ void pair_cut(hipStream_t s, const uint32_t* depth_keys, uint32_t* tile_counts, 
 }
 
 void depth_order(hipStream_t s, const uint32_t* depth_keys, const uint32_t* ranges, const uint32_t* tile_order,
-                 uint32_t tiles, uint32_t* list, uint32_t* scratch) {
+                 uint32_t tiles, uint32_t* list, void* scratch, bool ordered, uint32_t* keyed) {
   if (tiles == 0) return;
   depth_order_kernel<<<tiles, kThreads, 0, s>>>(depth_keys, reinterpret_cast<const uint2*>(ranges), tile_order, list,
-                                                scratch);
+                                                reinterpret_cast<uint2*>(keyed), scratch, ordered);
   DIO_LAUNCHED("depth_order");
+}
+
+BinLayout bin_layout(uint32_t n) {
+  BinLayout l;
+  l.groups = std::max<uint32_t>(1, std::min(kBinMaxGroups, ceil_div(n, kBinRound)));
+  if (l.groups > 8) l.groups = (l.groups + 7) / 8 * 8;  // whole XCD shares (kBinMaxGroups is one too)
+  l.chunk = n == 0 ? 0 : ceil_div(n, l.groups);
+  return l;
+}
+
+void bin_count(hipStream_t s, const uint32_t* rects, const uint32_t* depth_keys, uint32_t* tile_counts, uint32_t n,
+               uint32_t width, uint32_t height, uint32_t shift, uint32_t capacity, uint32_t* pair_total,
+               const BinWorkspace& ws) {
+  const uint32_t tiles = ((ceil_div(width, kTile) + (1u << shift) - 1u) >> shift) *
+                         ((ceil_div(height, kTile) + (1u << shift) - 1u) >> shift);
+  if (tiles == 0 || tiles > kBinMaxTiles) throw Error(-2, "binning by counts takes 1 to 8192 tiles");
+  const BinLayout l = bin_layout(n);
+  if (l.groups > ws.groups || tiles > ws.tiles) throw Error(-2, "bin workspace too small");
+  bin_count_kernel<<<l.groups, kBinThreads, 0, s>>>(reinterpret_cast<const uint2*>(rects), tile_counts, n, l.chunk,
+                                                    width, height, shift, ws.matrix, ws.partial);
+  DIO_LAUNCHED("bin_count");
+  bin_cut_kernel<<<1, kCutThreads, 0, s>>>(reinterpret_cast<const uint2*>(rects), depth_keys, tile_counts, n, l.chunk,
+                                           l.groups, width, height, shift, ws.matrix, ws.partial, pair_total,
+                                           capacity);
+  DIO_LAUNCHED("bin_cut");
+  bin_scan_kernel<<<ceil_div(tiles, kScanColumns), kScanColumns * kScanBlocks, 0, s>>>(ws.matrix, l.groups, tiles,
+                                                                                       ws.totals);
+  DIO_LAUNCHED("bin_scan");
+}
+
+void bin_scatter(hipStream_t s, const uint32_t* rects, const uint32_t* tile_counts, const uint32_t* depth_keys,
+                 uint32_t n, uint32_t width, uint32_t height, uint32_t shift, const BinWorkspace& ws, uint32_t* keyed,
+                 uint32_t* ranges, uint32_t* order, uint32_t* clear) {
+  const BinLayout l = bin_layout(n);
+  bin_scatter_kernel<<<l.groups, kBinThreads, 0, s>>>(reinterpret_cast<const uint2*>(rects), tile_counts, depth_keys,
+                                                      n, l.chunk, width, height, shift, ws.matrix, ws.totals,
+                                                      reinterpret_cast<uint2*>(keyed), reinterpret_cast<uint2*>(ranges),
+                                                      order, clear);
+  DIO_LAUNCHED("bin_scatter");
 }
 
 }  // namespace dio

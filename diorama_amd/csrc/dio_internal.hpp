@@ -134,16 +134,30 @@ __device__ inline void bin_rect(float mx, float my, float radius, uint32_t tiles
   rect[3] = (rect[3] + round) >> shift;
 }
 
+// A bin rectangle [x0, x1) x [y0, y1) packed two 16-bit coordinates to a word (the
+// frames that read it have at most 8192 bins).
+__host__ __device__ inline uint2 pack_rect(const uint32_t r[4]) {
+  return make_uint2(r[0] | (r[1] << 16), r[2] | (r[3] << 16));
+}
+__host__ __device__ inline void unpack_rect(uint2 p, uint32_t r[4]) {
+  r[0] = p.x & 0xffffu;
+  r[1] = p.x >> 16;
+  r[2] = p.y & 0xffffu;
+  r[3] = p.y >> 16;
+}
+
 
 // `vis` (nullable): each Gaussian's (depth, radius) as two floats, radius 0 when culled.
+// `rects` (nullable): each splat's touched tiles (bins) as pack_rect, zero when culled.
 void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                 const CameraArgs& cam, float* splats, uint32_t* tile_counts, uint32_t* depth_keys,
-                uint32_t* depth_values, float* vis = nullptr);
+                uint32_t* depth_values, float* vis = nullptr, uint32_t* rects = nullptr);
 
 // The tile stage from existing splats: counts (+ depth keys) only.
 // Tile counts of splats in bins of (16 << bin_shift) px, their depth keys and indices.
 void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width, uint32_t height,
-                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values, uint32_t bin_shift = 0);
+                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values, uint32_t bin_shift = 0,
+                 uint32_t* rects = nullptr);
 
 // pair_hint (0: none), here and in find_ranges: the pair count the host expects,
 // which sizes the grid; the kernels loop over whatever *pair_total holds.
@@ -162,13 +176,53 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
                  uint32_t tiles, uint32_t* ranges, uint32_t* order = nullptr, uint32_t pair_hint = 0,
                  uint32_t* clear = nullptr);
 
-// Sorts each tile's run of `list` (pairs binned by tile, each run in splat order)
-// stably by depth_keys[splat], in place: the run of pairs a depth-ordered
-// duplicate would have produced. tile_order (nullable) is the launch order of the
-// tiles; scratch is a pair-sized buffer the runs longer than 2,048 pairs pass
-// through at their own positions.
+// Sorts each tile's run of `list` (pairs binned by tile) by (depth_keys[splat],
+// splat), in place: the run of pairs a depth-ordered duplicate would have
+// produced. `ordered`: every run arrives in splat order (pairs written in splat
+// order, then sorted stably by tile), which saves passes over the splat indices on
+// the runs that take radix passes; otherwise runs may arrive in any order.
+// tile_order (nullable) is the launch order of the tiles; scratch is a pair-sized
+// buffer the runs longer than 2,048 pairs pass through at their own positions.
+// keyed (nullable): the runs as (depth key, splat) pairs (two words a pair) at the
+// same positions, read instead of `list` and depth_keys[splat] and sorted into
+// `list`; scratch then holds two words a pair. keyed is overwritten.
 void depth_order(hipStream_t s, const uint32_t* depth_keys, const uint32_t* ranges, const uint32_t* tile_order,
-                 uint32_t tiles, uint32_t* list, uint32_t* scratch);
+                 uint32_t tiles, uint32_t* list, void* scratch, bool ordered, uint32_t* keyed = nullptr);
+
+// The rasterizer's launch order: tiles by falling pair count, in buckets of
+// kOrderStep pairs (ties in any order).
+constexpr uint32_t kOrderBuckets = 1024, kOrderStep = 8;
+
+// Pairs binned by tile from counts, without a sort (at most kBinMaxTiles tiles or
+// bins). The splats are cut into chunks (bin_layout); each chunk counts its pairs
+// per tile, the counts are scanned per tile over the chunks, and each chunk writes
+// its pairs into its slots of the tiles' runs (in no particular order inside a
+// run: depth_order with ordered = false then orders them). A frame past the pair
+// capacity keeps the same pairs as pair_cut (and rewrites tile_counts to them).
+constexpr uint32_t kBinMaxTiles = 8192;
+struct BinLayout {
+  uint32_t groups = 0;  // chunks, one workgroup each (at most 512)
+  uint32_t chunk = 0;   // splats per chunk
+};
+BinLayout bin_layout(uint32_t n);
+struct BinWorkspace {
+  uint32_t* matrix = nullptr;   // groups x tiles counts, then their prefixes per tile
+  uint32_t* partial = nullptr;  // per chunk: its pairs
+  uint32_t* totals = nullptr;   // per tile: its pairs
+  uint32_t groups = 0, tiles = 0;
+};
+// Counts, the overflow cut (pair_total[0] gets the frame's uncut pair count) and the
+// per-tile scan.
+// `rects`: each splat's pack_rect (preprocess, count_tiles).
+void bin_count(hipStream_t s, const uint32_t* rects, const uint32_t* depth_keys, uint32_t* tile_counts, uint32_t n,
+               uint32_t width, uint32_t height, uint32_t shift, uint32_t capacity, uint32_t* pair_total,
+               const BinWorkspace& ws);
+// The pairs as (depth key, splat) into `keyed` (two words a pair), each tile's
+// [start, end) into `ranges`, the launch order into `order` (nullable) and zeros
+// into clear[0 .. kPartClasses] (nullable).
+void bin_scatter(hipStream_t s, const uint32_t* rects, const uint32_t* tile_counts, const uint32_t* depth_keys,
+                 uint32_t n, uint32_t width, uint32_t height, uint32_t shift, const BinWorkspace& ws, uint32_t* keyed,
+                 uint32_t* ranges, uint32_t* order, uint32_t* clear);
 // For pairs written in splat order (offsets the exclusive scan of tile_counts in
 // splat order, *pair_total their sum): when the pairs overflow `capacity`, rewrites
 // tile_counts to the pairs each splat keeps under the depth-order slot rule and
@@ -315,6 +369,7 @@ struct PrepArgs {
   uint32_t* depth_keys = nullptr;
   uint32_t* depth_values = nullptr;
   float* vis = nullptr;
+  uint32_t* rects = nullptr;  // nullable
 };
 void backward_adam(hipStream_t s, uint32_t n, float* gaussians, float* sh, uint32_t sh_degree, const CameraArgs& cam,
                    const float* vis, const float* splat_grads, float* params, float* moments, const AdamArgs& a,

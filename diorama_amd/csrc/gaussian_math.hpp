@@ -142,13 +142,16 @@ __device__ inline void camera_position(const CameraArgs& cam, float cp[3]) {
 struct SplatRecord {
   float4 s0, s1, s2;
   uint32_t touched, key;
+  uint2 rect;  // the tiles (bins) touched, packed: x0 | y0 << 16, x1 | y1 << 16
 };
+
 
 __device__ inline void splat_record(const float g[16], const Projection& p, bool visible, const CameraArgs& cam,
                                     const float* coeff, int count, SplatRecord& o) {
   o.s0 = o.s1 = o.s2 = make_float4(0.f, 0.f, 0.f, 0.f);
   o.touched = 0;
   o.key = 0xffffffffu;
+  o.rect = make_uint2(0u, 0u);
   if (!visible) return;
   float color[3] = {g[12], g[13], g[14]};
   if (count > 0) {
@@ -176,6 +179,7 @@ __device__ inline void splat_record(const float g[16], const Projection& p, bool
   bin_rect(p.mean[0], p.mean[1], p.radius, tiles_x, tiles_y, cam.bin_shift, r);
   o.touched = (r[2] - r[0]) * (r[3] - r[1]);
   o.key = __float_as_uint(p.depth);
+  o.rect = pack_rect(r);
 }
 
 // One Gaussian's gradients for one view (preprocessBackward.wgsl:27-181): the

@@ -401,6 +401,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
     prep.depth_keys[i] = rec.key;
     prep.depth_values[i] = i;
     reinterpret_cast<float2*>(prep.vis)[i] = make_float2(rec.s0.z, rec.s0.w);
+    if (prep.rects) reinterpret_cast<uint2*>(prep.rects)[i] = rec.rect;
   }
 }
 

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_kernel(const float4* __re
                                                                uint32_t* __restrict__ tile_counts,
                                                                uint32_t* __restrict__ depth_keys,
                                                                uint32_t* __restrict__ depth_values,
-                                                               float2* __restrict__ vis) {
+                                                               float2* __restrict__ vis, uint2* __restrict__ rects) {
   __shared__ __align__(16) float s_sh[kThreads * kShRest];
   const uint32_t first = blockIdx.x * kThreads, i = first + threadIdx.x;
   const bool active = i < n;
@@ -69,6 +69,7 @@ __global__ __launch_bounds__(kThreads) void preprocess_kernel(const float4* __re
   depth_keys[i] = rec.key;
   depth_values[i] = i;
   if (vis) vis[i] = make_float2(rec.s0.z, rec.s0.w);
+  if (rects) rects[i] = rec.rect;
 }
 
 // count_tiles (tiles.wgsl:23-34) for splats that did not come from the preprocess,
@@ -77,20 +78,24 @@ __global__ __launch_bounds__(kThreads) void count_tiles_kernel(const float4* __r
                                                                 uint32_t tiles_x, uint32_t tiles_y, uint32_t shift,
                                                                 uint32_t* __restrict__ tile_counts,
                                                                 uint32_t* __restrict__ depth_keys,
-                                                                uint32_t* __restrict__ depth_values) {
+                                                                uint32_t* __restrict__ depth_values,
+                                                                uint2* __restrict__ rects) {
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const float4 s0 = splats[i * 3];
   uint32_t touched = 0, key = 0xffffffffu;
+  uint2 rect = make_uint2(0u, 0u);
   if (s0.w > 0.f) {
     uint32_t r[4];
     bin_rect(s0.x, s0.y, s0.w, tiles_x, tiles_y, shift, r);
     touched = (r[2] - r[0]) * (r[3] - r[1]);
     key = __float_as_uint(s0.z);
+    rect = pack_rect(r);
   }
   tile_counts[i] = touched;
   depth_keys[i] = key;
   depth_values[i] = i;
+  if (rects) rects[i] = rect;
 }
 
 // duplicate (tiles.wgsl:36-59), load-balanced over pairs rather than splats: each
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(kThreads) void find_ranges_kernel(const uint32_t* _
 // the light ones fill in behind them instead of the centre tiles of a view, which
 // row-major order dispatches halfway through, finishing last. One workgroup
 // counting-sorts the tiles, a thread per bucket.
-constexpr uint32_t kOrderThreads = 1024, kOrderStep = 8;
+constexpr uint32_t kOrderThreads = kOrderBuckets;
 
 __global__ __launch_bounds__(kOrderThreads) void tile_order_kernel(const uint2* __restrict__ ranges, uint32_t tiles,
                                                                    uint32_t* __restrict__ order,
@@ -550,20 +555,22 @@ __global__ __launch_bounds__(kThreads) void preprocess_backward_kernel(
 
 void preprocess(hipStream_t s, const float* gaussians, const float* sh, uint32_t sh_degree, uint32_t n,
                 const CameraArgs& cam, float* splats, uint32_t* tile_counts, uint32_t* depth_keys,
-                uint32_t* depth_values, float* vis) {
+                uint32_t* depth_values, float* vis, uint32_t* rects) {
   if (n == 0) return;
   preprocess_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(
       reinterpret_cast<const float4*>(gaussians), sh, sh_degree, n, cam, reinterpret_cast<float4*>(splats),
-      tile_counts, depth_keys, depth_values, reinterpret_cast<float2*>(vis));
+      tile_counts, depth_keys, depth_values, reinterpret_cast<float2*>(vis), reinterpret_cast<uint2*>(rects));
   DIO_LAUNCHED("preprocess");
 }
 
 void count_tiles(hipStream_t s, const float* splats, uint32_t n, uint32_t width, uint32_t height,
-                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values, uint32_t bin_shift) {
+                 uint32_t* tile_counts, uint32_t* depth_keys, uint32_t* depth_values, uint32_t bin_shift,
+                 uint32_t* rects) {
   if (n == 0) return;
   count_tiles_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(reinterpret_cast<const float4*>(splats), n,
                                                                  ceil_div(width, kTile), ceil_div(height, kTile),
-                                                                 bin_shift, tile_counts, depth_keys, depth_values);
+                                                                 bin_shift, tile_counts, depth_keys, depth_values,
+                                                                 reinterpret_cast<uint2*>(rects));
   DIO_LAUNCHED("count_tiles");
 }
 

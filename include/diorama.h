@@ -73,12 +73,14 @@ typedef struct dio_tuning {
    * float rounding. */
   uint32_t segment;
   /* single-view trainers: 0 sorts the splats by depth before writing their pairs
-   * (src/tileRasterizer.ts:127-192); 1 writes the pairs in splat order, bins them by
-   * tile and sorts each tile's list by depth. The lists, and so images and
-   * gradients, are the same, also on a frame whose pairs overflow the pair
-   * buffers (both keep the nearest pair slots). */
+   * (src/tileRasterizer.ts:127-192); 1 writes the pairs in splat order, sorts them
+   * stably by tile and then each tile's list by depth; 2 bins the pairs by tile
+   * from per-tile counts, with no sort over the pairs, and then sorts each tile's
+   * list by (depth, splat) (frames of more than 8192 tiles or bins run 1 instead).
+   * The lists, and so images and gradients, are the same, also on a frame whose
+   * pairs overflow the pair buffers (all keep the nearest pair slots). */
   uint32_t per_tile_depth;
-} dio_tuning; /* a trainer starts at {16, 64, 8, 0, 512, 1}; a batch trainer at {16, 64, 8, 0, 0, 0} */
+} dio_tuning; /* a trainer starts at {16, 64, 8, 0, 512, 2}; a batch trainer at {16, 64, 8, 0, 0, 0} */
 
 typedef struct dio_tuning_result {
   dio_tuning tuning;

@@ -164,8 +164,9 @@ def test_batch_training_learns():
 @pytest.mark.parametrize("batch", [0, 3])
 def test_tuning_changes_speed_not_results(batch):
     """Every tile size (8, 16, 32; the plain trainer), launch form (persistent,
-    segmented backward), depth order (all splats, or each tile's list; the plain
-    trainer), splat batch and digit width gives the same images, and
+    segmented backward), depth order (all splats, or each tile's list after a stable
+    tile sort or after binning from counts; the plain trainer), splat batch and digit
+    width gives the same images, and
     gradients equal up to float rounding (the order of float atomics; a segment's
     replay starting from the forward's checkpoint); the autotuner keeps the fastest
     setting. 200x120 has 13 x 8 tiles: the 32-px bins of the last column and row are
@@ -178,7 +179,8 @@ def test_tuning_changes_speed_not_results(batch):
     outs = {}
     shapes = [(16, 0, 0, 0)] if batch else [(16, 0, 0, 0), (16, 1, 0, 0), (8, 0, 0, 0), (32, 0, 0, 0), (16, 0, 256, 0),
                                             (16, 0, 512, 0), (16, 0, 1024, 0), (16, 0, 0, 1), (16, 1, 0, 1),
-                                            (8, 0, 0, 1), (32, 0, 0, 1), (16, 0, 512, 1)]
+                                            (8, 0, 0, 1), (32, 0, 0, 1), (16, 0, 512, 1), (16, 0, 0, 2),
+                                            (16, 1, 0, 2), (8, 0, 0, 2), (32, 0, 0, 2), (16, 0, 512, 2)]
     for tile, persistent, segment, per_tile in shapes:
         for bits in (8, 4):
             for splats in (64, 128, 256):
@@ -206,7 +208,7 @@ def test_tuning_changes_speed_not_results(batch):
     t = diorama_amd.Trainer(ctx, params, sh, cams, images, diorama_amd.default_settings(1.0), batch=batch)
     say("autotune", batch)
     results = t.autotune(steps=2)
-    assert len(results) == (6 if batch else 68)
+    assert len(results) == (6 if batch else 85)
     best = min(results, key=lambda r: r[1])
     assert t.tuning == best[0]
     print("autotune:", [(r[0]["tile_size"], r[0]["persistent"], r[0]["segment"], r[0]["per_tile_depth"],
@@ -218,10 +220,11 @@ def test_tuning_changes_speed_not_results(batch):
     with pytest.raises(diorama_amd.DioramaError):
         t.tuning = {"tile_size": 8, "splat_batch": 64, "digit_bits": 8, "persistent": 1}
     for bad in ({"tile_size": 16, "segment": 128}, {"tile_size": 8, "segment": 256},
-                {"tile_size": 16, "persistent": 1, "segment": 256}, {"tile_size": 16, "per_tile_depth": 2}):
+                {"tile_size": 16, "persistent": 1, "segment": 256}, {"tile_size": 16, "per_tile_depth": 3}):
         with pytest.raises(diorama_amd.DioramaError):
             t.tuning = dict({"splat_batch": 64, "digit_bits": 8}, **bad)
     if batch:
-        with pytest.raises(diorama_amd.DioramaError):
-            t.tuning = {"tile_size": 16, "splat_batch": 64, "digit_bits": 8, "per_tile_depth": 1}
+        for per_tile in (1, 2):
+            with pytest.raises(diorama_amd.DioramaError):
+                t.tuning = {"tile_size": 16, "splat_batch": 64, "digit_bits": 8, "per_tile_depth": per_tile}
     t.destroy()

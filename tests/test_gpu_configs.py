@@ -170,13 +170,14 @@ def config1():
     return ctx, cam, image, params, sh, scene.scene_extent(train)
 
 
-@pytest.mark.parametrize("segment, per_tile", [(0, 0), (256, 0), (512, 1)])
+@pytest.mark.parametrize("segment, per_tile", [(0, 0), (256, 0), (512, 1), (512, 2)])
 def test_config1_step_against_oracle(config1, segment, per_tile):
     """segment 256: the backward replays the lists in parts of 256 pairs, each pixel
     whose blended splats run past a part's end starting there from the forward
     pass's checkpoint; the gradients meet the same bar against the oracle.
     per_tile 1: the pairs are written in splat order and each tile's list is sorted
-    by depth after binning; the lists are the oracle's all the same."""
+    by depth after binning; per_tile 2: the pairs are binned from per-tile counts and
+    each list is sorted by (depth, splat); the lists are the oracle's all the same."""
     ctx, cam, image, params, sh, extent = config1
     n = len(params)
     w, h = 980, 545

@@ -356,14 +356,16 @@ def test_many_more_pairs_than_the_last_frame():
     r.destroy()
 
 
-def test_long_lists_and_depth_ties_in_both_depth_orders():
+def test_long_lists_and_depth_ties_in_every_depth_order():
     """Tile lists of several thousand pairs and exact depth ties (copies of Gaussians),
     with the splats sorted by depth before their pairs are written (per_tile_depth 0)
-    or each tile's list sorted by depth after binning (1; lists past 2,048 pairs pass
-    through global memory, in three digit passes where a list's depths share their top
-    byte and four where they do not; shorter lists are bucketed by depth in LDS, or
-    take the same radix passes when one bucket would hold more than 16 pairs): both
-    give the oracle's (tile, depth, splat) lists, and the same image."""
+    or each tile's list sorted by depth after binning (1: binned by a stable sort, so
+    lists arrive in splat order; 2: binned from counts, so lists arrive in any order
+    and the radix passes also pass over the splat indices). Lists past 2,048 pairs
+    pass through global memory, in three digit passes where a list's depths share
+    their top byte and four where they do not; shorter lists are bucketed by depth in
+    LDS, or take the same radix passes when one bucket would hold more than 16 pairs.
+    All give the oracle's (tile, depth, splat) lists, and the same image."""
     rng = np.random.default_rng(5)
     w = h = 256
     cam = scene.camera(scene.IDENTITY, scene.perspective(math.pi / 2, 1, 0.1, 100), w, h)
@@ -389,7 +391,7 @@ def test_long_lists_and_depth_ties_in_both_depth_orders():
     params = np.concatenate([near, far, spread, near[:5000], far[:5000], spread[:300], np.repeat(spread[300:301], 40, 0)])
     n = len(params)
     frames = {}
-    for per_tile in (0, 1):
+    for per_tile in (0, 1, 2):
         t = diorama_amd.Trainer(context(), params, np.zeros((n, 45), np.float32), [cam], [np.zeros(w * h, np.uint32)],
                                 diorama_amd.default_settings(2.0))
         t.tuning = dict(t.tuning, per_tile_depth=per_tile)
@@ -409,7 +411,7 @@ def test_long_lists_and_depth_ties_in_both_depth_orders():
     print(f"{len(pairs)} pairs, longest list {lengths.max()}, {len(long)} lists past 4096 pairs "
           f"({sum(shared)} with one top byte), {tied} tied neighbours")
     assert lengths.max() > 2 * 2048 and any(shared) and not all(shared) and tied > 1000
-    for per_tile in (0, 1):
+    for per_tile in (0, 1, 2):
         s_, c_, l_, r_, p_ = frames[per_tile]
         assert np.array_equal(s_, splats), per_tile
         assert np.array_equal(r_, o_ranges), per_tile
@@ -417,12 +419,13 @@ def test_long_lists_and_depth_ties_in_both_depth_orders():
         assert np.array_equal(c_, color) and np.array_equal(l_, last), per_tile
 
 
-@pytest.mark.parametrize("per_tile", [0, 1])
+@pytest.mark.parametrize("per_tile", [0, 1, 2])
 def test_overflowing_frame_keeps_the_nearest_pair_slots(per_tile):
     """A frame past its pair capacity keeps the pair slots below the capacity in depth
     order (tiles.wgsl:47-58; the oracle's oc_tile_lists_capped) and grows its buffers
     afterwards (src/tileRasterizer.ts:229-244). per_tile_depth 1 writes the pairs in
-    splat order and keeps the same pairs by a cut in (depth, splat) order. The
+    splat order and keeps the same pairs by a cut in (depth, splat) order; 2 makes the
+    same cut and counts the kept pairs per tile again before binning them. The
     capacities end between splats, inside one splat's tiles and inside a group of
     equal depths (copies of Gaussians)."""
     ctx = context()
