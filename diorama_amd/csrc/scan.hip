@@ -16,7 +16,12 @@ namespace dio {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kItems = kScanTile / kThreads;  // 16
+// Tiles of 256 threads x 16 values (4096), or for large scans 256 x 64 (16384): a
+// large scan's cost is otherwise the tile tickets, as every tile takes one from the
+// same word and those atomics complete one after another.
+constexpr int kItemsSmall = kScanTile / kThreads;  // 16
+constexpr int kItemsLarge = 64, kThreadsLarge = 256;
+constexpr uint32_t kLargeScan = 1u << 20;  // values from which the large tiles pay
 constexpr uint64_t kFlagAggregate = 1ull << 32;
 constexpr uint64_t kFlagPrefix = 2ull << 32;
 constexpr uint32_t kSelfCleanTiles = 1024;
@@ -28,6 +33,21 @@ __device__ inline uint32_t wave_inclusive_scan(uint32_t x, int lane) {
     if (lane >= o) x += y;
   }
   return x;
+}
+
+// The same inclusive scan across the wave with DPP moves (no LDS crossbar): the
+// sums within each row of 16 lanes by row shifts, then lane 15 of each row added
+// into the next row and lane 31 into rows 2 and 3 by row broadcasts. Lanes without
+// a source in a move add the 0 of `old`.
+__device__ inline uint32_t wave_scan_dpp(uint32_t x) {
+  int v = static_cast<int>(x);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
+  return static_cast<uint32_t>(v);
 }
 
 __device__ inline uint32_t wave_sum(uint32_t x) {
@@ -45,24 +65,48 @@ __device__ inline uint64_t peek(const uint64_t* state, int32_t tile) {
 }
 
 // The exclusive prefix of `tile`: sums of the aggregates back to the first
-// published inclusive prefix, 64 predecessors per step. Wave 0 only.
+// published inclusive prefix. Wave 0 only. A step reads kLook x 64 predecessors
+// with all its loads in flight (item k of lane l is predecessor k 64 + l back):
+// when every tile of a large grid starts at once, few of them have published a
+// prefix yet, and a step of 64 would take one memory round trip per 64 tiles.
+constexpr int kLook = 8;
 __device__ uint32_t look_back(const uint64_t* state, uint32_t tile, int lane) {
   uint32_t exclusive = 0;
   int32_t base = static_cast<int32_t>(tile) - 1;
   for (;;) {
-    int32_t j = base - lane;
-    uint64_t word = j >= 0 ? peek(state, j) : kFlagPrefix;
-    for (;;) {
-      bool waiting = (word >> 32) == 0;
-      if (!__any(waiting)) break;
-      if (waiting) word = peek(state, j);
+    uint64_t word[kLook];
+#pragma unroll
+    for (int k = 0; k < kLook; k++) {
+      const int32_t j = base - k * kWave - lane;
+      word[k] = j >= 0 ? peek(state, j) : kFlagPrefix;
     }
-    uint64_t prefixes = __ballot((word >> 32) == 2);
-    int first = prefixes ? __ffsll(static_cast<long long>(prefixes)) - 1 : kWave - 1;
-    uint32_t v = lane <= first ? static_cast<uint32_t>(word) : 0u;
+#pragma unroll
+    for (int k = 0; k < kLook; k++) {
+      const int32_t j = base - k * kWave - lane;
+      for (;;) {
+        const bool waiting = (word[k] >> 32) == 0;
+        if (!__any(waiting)) break;
+        if (waiting) word[k] = peek(state, j);
+      }
+    }
+    // The nearest predecessor with a published prefix: the first item with any,
+    // and the first lane of that item.
+    int item = kLook, first = kWave - 1;
+#pragma unroll
+    for (int k = kLook - 1; k >= 0; k--) {
+      const uint64_t prefixes = __ballot((word[k] >> 32) == 2);
+      if (prefixes) {
+        item = k;
+        first = __ffsll(static_cast<long long>(prefixes)) - 1;
+      }
+    }
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kLook; k++)
+      v += k < item || (k == item && lane <= first) ? static_cast<uint32_t>(word[k]) : 0u;
     exclusive += wave_sum(v);
-    if (prefixes) return exclusive;
-    base -= kWave;
+    if (item < kLook) return exclusive;
+    base -= kLook * kWave;
   }
 }
 
@@ -75,50 +119,77 @@ __device__ inline bool finish(uint32_t* done) {
   return atomicAdd(done, 1u) == gridDim.x - 1;
 }
 
-// One tile of the scan (tile < tiles). Wave 0 counts the workgroup finished with
-// the tile states once it has published its inclusive prefix, before the outputs
-// are written; `last` tells whether it was the last of the grid.
+// One tile of the scan (tile < tiles): kItems / 4 blocks of 1024 values, thread tid
+// holding values [tid 4, tid 4 + 4) of every block (one 16-byte load per block,
+// coalesced across the workgroup). Wave 0 counts the workgroup finished with the tile
+// states once it has published its inclusive prefix, before the outputs are
+// written; `last` tells whether it was the last of the grid.
+template <int kThreads, int kItems>
 __device__ void scan_tile(const uint32_t* in, uint32_t* out, uint32_t n,
                           uint32_t tiles, uint32_t tile, uint32_t* total, uint64_t* state, uint32_t* done,
                           bool& last) {
+  constexpr int kBlocks = kItems / 4, kWaves = kThreads / kWave;
+  constexpr uint32_t kBlock = kThreads * 4;
   __shared__ bool s_last;
-  __shared__ uint32_t s_wave[kThreads / kWave];
+  __shared__ uint32_t s_wave[kBlocks][kWaves];
   __shared__ uint32_t s_prefix;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t base = tile * kScanTile + tid * kItems;
-  uint32_t v[kItems];
-  if (base + kItems <= n) {
-    const uint4* src = reinterpret_cast<const uint4*>(in + base);
+  const uint32_t tile_base = tile * (kThreads * kItems);
+  const bool full = tile_base + kThreads * kItems <= n;
+  uint4 v[kBlocks];
+  const uint4* src = reinterpret_cast<const uint4*>(in + tile_base);
+  if (full) {
 #pragma unroll
-    for (int q = 0; q < kItems / 4; q++) {
-      uint4 w = src[q];
-      v[4 * q] = w.x; v[4 * q + 1] = w.y; v[4 * q + 2] = w.z; v[4 * q + 3] = w.w;
-    }
+    for (int q = 0; q < kBlocks; q++) v[q] = src[q * kThreads + tid];
   } else {
+    // The last tile: whole 16-byte loads where they fit, one value at a time only for
+    // the one that straddles the end.
 #pragma unroll
-    for (int k = 0; k < kItems; k++) {
-      uint32_t i = base + k;
-      v[k] = i < n ? in[i] : 0u;
+    for (int q = 0; q < kBlocks; q++) {
+      const uint32_t i = tile_base + q * kBlock + tid * 4;
+      if (i + 4 <= n) {
+        v[q] = src[q * kThreads + tid];
+      } else {
+        v[q] = make_uint4(0u, 0u, 0u, 0u);
+        if (i < n) v[q].x = in[i];
+        if (i + 1 < n) v[q].y = in[i + 1];
+        if (i + 2 < n) v[q].z = in[i + 2];
+      }
     }
   }
-  uint32_t sum = 0;
+  // Per block: the values before the thread's four within its wave (its wave's
+  // scan of the threads' sums), and each wave's total in LDS.
+  uint32_t before[kBlocks];
 #pragma unroll
-  for (int k = 0; k < kItems; k++) {
-    uint32_t x = v[k];
-    v[k] = sum;  // exclusive within the thread
-    sum += x;
+  for (int q = 0; q < kBlocks; q++) {
+    const uint32_t sum = v[q].x + v[q].y + v[q].z + v[q].w;
+    const uint32_t inclusive = wave_scan_dpp(sum);
+    if (lane == kWave - 1) s_wave[q][wave] = inclusive;
+    before[q] = inclusive - sum;
   }
-  const uint32_t inclusive = wave_inclusive_scan(sum, lane);
-  if (lane == kWave - 1) s_wave[wave] = inclusive;
   __syncthreads();
-  uint32_t before = 0, aggregate = 0;
+  // Then the values before them in tile order (block, then wave, then lane), and the
+  // tile's aggregate.
+  uint32_t aggregate = 0;
 #pragma unroll
-  for (int w = 0; w < kThreads / kWave; w++) {
-    uint32_t t = s_wave[w];
-    before += w < wave ? t : 0u;
-    aggregate += t;
+  for (int q = 0; q < kBlocks; q++) {
+    uint32_t earlier_waves = 0, block_total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+      const uint32_t t = s_wave[q][w];
+      earlier_waves += w < wave ? t : 0u;
+      block_total += t;
+    }
+    before[q] += aggregate + earlier_waves;
+    aggregate += block_total;
   }
-  const uint32_t thread_prefix = before + inclusive - sum;
+  // The outputs less the tile's prefix, so that only they stay live across the
+  // look-back.
+#pragma unroll
+  for (int q = 0; q < kBlocks; q++) {
+    const uint32_t a = before[q], b = a + v[q].x, c = b + v[q].y;
+    v[q] = make_uint4(a, b, c, c + v[q].z);
+  }
 
   if (wave == 0) {
     uint32_t exclusive = 0;
@@ -137,19 +208,25 @@ __device__ void scan_tile(const uint32_t* in, uint32_t* out, uint32_t n,
   }
   __syncthreads();
   last = s_last;
-  const uint32_t add = s_prefix + thread_prefix;
-  if (base + kItems <= n) {
-    uint4* dst = reinterpret_cast<uint4*>(out + base);
+  const uint32_t prefix = s_prefix;
+  if (full) {
+    uint4* dst = reinterpret_cast<uint4*>(out + tile_base);
 #pragma unroll
-    for (int q = 0; q < kItems / 4; q++)
-      dst[q] = make_uint4(v[4 * q] + add, v[4 * q + 1] + add, v[4 * q + 2] + add, v[4 * q + 3] + add);
+    for (int q = 0; q < kBlocks; q++)
+      dst[q * kThreads + tid] = make_uint4(v[q].x + prefix, v[q].y + prefix, v[q].z + prefix, v[q].w + prefix);
   } else {
 #pragma unroll
-    for (int k = 0; k < kItems; k++)
-      if (base + k < n) out[base + k] = v[k] + add;
+    for (int q = 0; q < kBlocks; q++) {
+      const uint32_t i = tile_base + q * kBlock + tid * 4;
+      if (i < n) out[i] = v[q].x + prefix;
+      if (i + 1 < n) out[i + 1] = v[q].y + prefix;
+      if (i + 2 < n) out[i + 2] = v[q].z + prefix;
+      if (i + 3 < n) out[i + 3] = v[q].w + prefix;
+    }
   }
 }
 
+template <int kThreads, int kItems>
 __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
                                                          uint32_t* total, uint64_t* state, uint32_t* ticket,
                                                          uint32_t* done) {
@@ -160,13 +237,13 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const uint32_t* in, uint
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t n = count ? min(*count, capacity) : capacity;
-  const uint32_t tiles = n == 0 ? 1 : ceil_div(n, kScanTile);
+  const uint32_t tiles = n == 0 ? 1 : ceil_div(n, kThreads * kItems);
   // The states only see device-scope atomics, and a workgroup counts as finished
   // after its look-back reads and once its publishes have completed, so the last
   // one can clear them.
   bool last;
   if (tile < tiles) {
-    scan_tile(in, out, n, tiles, tile, total, state, done, last);
+    scan_tile<kThreads, kItems>(in, out, n, tiles, tile, total, state, done, last);
   } else {
     if (tid == 0) s_idle_last = finish(done);
     __syncthreads();
@@ -246,13 +323,21 @@ ScanWorkspace scan_workspace(void* base, uint32_t capacity) {
 
 void scan_exclusive(hipStream_t s, const uint32_t* in, uint32_t* out, const uint32_t* count, uint32_t capacity,
                     uint32_t* total, const ScanWorkspace& ws) {
-  uint32_t tiles = capacity == 0 ? 1 : ceil_div(capacity, kScanTile);
-  if (tiles > ws.tiles) throw Error(-2, "scan workspace too small");
+  if ((capacity == 0 ? 1 : ceil_div(capacity, kScanTile)) > ws.tiles) throw Error(-2, "scan workspace too small");
+  const bool large = capacity >= kLargeScan;
+  const uint32_t tiles =
+      capacity == 0 ? 1 : ceil_div(capacity, large ? kThreadsLarge * kItemsLarge : kThreads * kItemsSmall);
   // Up to kSelfCleanTiles workgroups the last one clears the workspace; past that,
-  // one more same-address atomic per workgroup costs more than a clearing launch.
-  const bool self_clean = tiles <= kSelfCleanTiles;
+  // and for large tiles (whose scans are bound by same-address atomics: the tickets),
+  // one more such atomic per workgroup costs more than a clearing launch.
+  const bool self_clean = !large && tiles <= kSelfCleanTiles;
   uint32_t* done = self_clean ? ws.done : nullptr;
-  scan_kernel<<<tiles, kThreads, 0, s>>>(in, out, count, capacity, total, ws.tile_state, ws.ticket, done);
+  if (large)
+    scan_kernel<kThreadsLarge, kItemsLarge><<<tiles, kThreadsLarge, 0, s>>>(in, out, count, capacity, total,
+                                                                             ws.tile_state, ws.ticket, done);
+  else
+    scan_kernel<kThreads, kItemsSmall><<<tiles, kThreads, 0, s>>>(in, out, count, capacity, total, ws.tile_state,
+                                                                   ws.ticket, done);
   DIO_LAUNCHED("scan");
   // The states, the ticket and the finish counter are one run of words.
   if (!self_clean) fill_u32(s, reinterpret_cast<uint32_t*>(ws.tile_state), 0u, static_cast<size_t>(ws.tiles) * 2 + 2);
