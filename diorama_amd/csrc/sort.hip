@@ -179,15 +179,29 @@ __global__ __launch_bounds__(kThreads) void onesweep_kernel(const typename KeyOp
   using Ops = KeyOps<kWide, kBits>;
   using Key = typename Ops::T;
   constexpr int kRadix = 1 << kBits;  // thread d < kRadix owns digit d
-  __shared__ uint32_t wave_count[kWaves][kRadix];
-  __shared__ uint32_t wave_offset[kWaves][kRadix];
+  // The per-wave digit counts are dead once each key's rank is offset by them (a
+  // barrier follows), before the first key is staged: the two share their LDS, which
+  // lets three workgroups of 64-bit keys fit on a CU instead of two.
+  union Shared {
+    struct {
+      uint32_t count[kWaves][kRadix];
+      uint32_t offset[kWaves][kRadix];
+    } waves;
+    struct {
+      Key keys[kSortTile];
+      uint32_t values[kSortTile];
+    } staged;
+  };
+  __shared__ Shared shared;
+  auto& wave_count = shared.waves.count;
+  auto& wave_offset = shared.waves.offset;
+  auto& staged_keys = shared.staged.keys;
+  auto& staged_values = shared.staged.values;
   __shared__ uint32_t running[kRadix];
   __shared__ uint32_t local_start[kRadix];
   __shared__ uint32_t run_start[kRadix];
   __shared__ uint32_t wave_totals[kWaves];
   __shared__ uint32_t s_tile;
-  __shared__ Key staged_keys[kSortTile];
-  __shared__ uint32_t staged_values[kSortTile];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   if (tid == 0) {
