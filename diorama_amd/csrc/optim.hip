@@ -511,56 +511,86 @@ __device__ inline uint32_t pcg(uint32_t v) {
 
 __device__ inline float unit(uint32_t h) { return (static_cast<float>(h >> 8) + 0.5f) / 16777216.f; }
 
-__device__ inline void copy_gaussian(uint32_t i, uint32_t o, bool keep_moments, const float* params, const float* sh,
-                                     const float2* moments, float* new_params, float* new_sh, float2* new_moments) {
-  for (uint32_t k = 0; k < kFloatsPerGaussian; k++)
-    new_params[static_cast<size_t>(o) * kFloatsPerGaussian + k] = params[static_cast<size_t>(i) * kFloatsPerGaussian + k];
-  for (uint32_t k = 0; k < kShRest; k++)
-    new_sh[static_cast<size_t>(o) * kShRest + k] = sh[static_cast<size_t>(i) * kShRest + k];
-  for (uint32_t k = 0; k < kMomentSlots; k++)
-    new_moments[static_cast<size_t>(o) * kMomentSlots + k] =
-        keep_moments ? moments[static_cast<size_t>(i) * kMomentSlots + k] : make_float2(0.f, 0.f);
-}
+// densify.wgsl's scatter as a stream compaction written in runs: the 256 inputs of
+// a workgroup own one contiguous run of outputs (offsets from the scan of their
+// output counts). Each input first lays out its outputs in LDS: which input each
+// comes from, whether it keeps that input's moments (a kept Gaussian and a clone's
+// first copy) or starts them at zero (a clone's second copy and both split
+// children), and its 16 parameters (copied, or as the clone and split rules
+// change them). Then the workgroup writes the run's parameters, SH and moments as
+// consecutive words, each read from its output's source record, so stores are
+// coalesced and loads nearly so (a kept Gaussian's record follows the one before).
+constexpr uint32_t kOutputsPerGroup = 2 * kThreads;
 
 __global__ __launch_bounds__(kThreads) void densify_scatter_kernel(
     uint32_t n, const float* __restrict__ params, const float4* __restrict__ stats,
     const uint32_t* __restrict__ outputs, const float* __restrict__ sh, const float2* __restrict__ moments,
     DensifyArgs a, const uint32_t* __restrict__ forced, float* __restrict__ new_params, float* __restrict__ new_sh,
     float2* __restrict__ new_moments) {
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t act = action(params, stats, a, forced, i);
-  const uint32_t o = outputs[i];
-  if (act == kKeep || act == kClone) copy_gaussian(i, o, true, params, sh, moments, new_params, new_sh, new_moments);
-  if (act == kClone) {
-    copy_gaussian(i, o + 1, false, params, sh, moments, new_params, new_sh, new_moments);
-    new_params[static_cast<size_t>(o + 1) * kFloatsPerGaussian + 7] = 1.f;
-    new_params[static_cast<size_t>(o + 1) * kFloatsPerGaussian + 15] = static_cast<float>(a.seed);
-  }
-  if (act == kSplit) {
+  __shared__ float s_params[kOutputsPerGroup * kFloatsPerGaussian];
+  __shared__ uint32_t s_source[kOutputsPerGroup];  // input (within the group) | zero moments << 16
+  __shared__ uint32_t s_first, s_end;
+  const uint32_t first = blockIdx.x * kThreads, i = first + threadIdx.x;
+  const uint32_t last = min(first + kThreads, n) - 1;
+  if (threadIdx.x == 0) s_first = outputs[first];
+  __syncthreads();
+  const uint32_t o0 = s_first;
+  if (i < n) {
+    const uint32_t act = action(params, stats, a, forced, i);
+    const uint32_t o = outputs[i] - o0;  // this input's first output within the run
     const float* p = params + static_cast<size_t>(i) * kFloatsPerGaussian;
-    float r[3][3], q[4], l;
-    rotation_matrix(p[8], p[9], p[10], p[11], r, q, &l);
-    const float sx = __expf(p[4]), sy = __expf(p[5]), sz = __expf(p[6]);
-    const float shrink = logf(kSplitShrink);
-    for (uint32_t c = 0; c < 2; c++) {
-      const uint32_t child = o + c;
-      copy_gaussian(i, child, false, params, sh, moments, new_params, new_sh, new_moments);
-      // normal3(pcg(seed ^ pcg(2i + c))) of densify.wgsl:270-279,305: the seed is hashed once more.
-      const uint32_t h1 = pcg(pcg(a.seed ^ pcg(2u * i + c))), h2 = pcg(h1), h3 = pcg(h2), h4 = pcg(h3);
-      // Box-Muller: three standard normal numbers from four hashes.
-      const float tau = 6.283185307179586f;
-      const float r1 = sqrtf(-2.f * logf(unit(h1))), r2 = sqrtf(-2.f * logf(unit(h3)));
-      const float nx = r1 * cosf(tau * unit(h2)), ny = r1 * sinf(tau * unit(h2)), nz = r2 * cosf(tau * unit(h4));
-      const float vx = sx * nx, vy = sy * ny, vz = sz * nz;
-      float* out = new_params + static_cast<size_t>(child) * kFloatsPerGaussian;
-      for (int k = 0; k < 3; k++) {
-        out[k] = p[k] + r[k][0] * vx + r[k][1] * vy + r[k][2] * vz;
-        out[4 + k] = p[4 + k] - shrink;
-      }
-      out[7] = 2.f;
-      out[15] = static_cast<float>(a.seed);
+    const uint32_t t = threadIdx.x;
+    auto put = [&](uint32_t slot, bool zero_moments) {
+      s_source[slot] = t | (zero_moments ? 1u << 16 : 0u);
+      for (uint32_t k = 0; k < kFloatsPerGaussian; k++) s_params[slot * kFloatsPerGaussian + k] = p[k];
+    };
+    if (act == kKeep || act == kClone) put(o, false);
+    if (act == kClone) {
+      put(o + 1, true);
+      s_params[(o + 1) * kFloatsPerGaussian + 7] = 1.f;
+      s_params[(o + 1) * kFloatsPerGaussian + 15] = static_cast<float>(a.seed);
     }
+    if (act == kSplit) {
+      float r[3][3], q[4], l;
+      rotation_matrix(p[8], p[9], p[10], p[11], r, q, &l);
+      const float sx = __expf(p[4]), sy = __expf(p[5]), sz = __expf(p[6]);
+      const float shrink = logf(kSplitShrink);
+      for (uint32_t c = 0; c < 2; c++) {
+        const uint32_t child = o + c;
+        put(child, true);
+        // normal3(pcg(seed ^ pcg(2i + c))) of densify.wgsl:270-279,305: the seed is hashed once more.
+        const uint32_t h1 = pcg(pcg(a.seed ^ pcg(2u * i + c))), h2 = pcg(h1), h3 = pcg(h2), h4 = pcg(h3);
+        // Box-Muller: three standard normal numbers from four hashes.
+        const float tau = 6.283185307179586f;
+        const float r1 = sqrtf(-2.f * logf(unit(h1))), r2 = sqrtf(-2.f * logf(unit(h3)));
+        const float nx = r1 * cosf(tau * unit(h2)), ny = r1 * sinf(tau * unit(h2)), nz = r2 * cosf(tau * unit(h4));
+        const float vx = sx * nx, vy = sy * ny, vz = sz * nz;
+        float* out = s_params + child * kFloatsPerGaussian;
+        for (int k = 0; k < 3; k++) {
+          out[k] = p[k] + r[k][0] * vx + r[k][1] * vy + r[k][2] * vz;
+          out[4 + k] = p[4 + k] - shrink;
+        }
+        out[7] = 2.f;
+        out[15] = static_cast<float>(a.seed);
+      }
+    }
+    if (i == last) s_end = o + (act == kDrop ? 0u : act == kKeep ? 1u : 2u);
+  }
+  __syncthreads();
+  const uint32_t count = s_end;  // outputs of the group, at most 2 per input
+  for (uint32_t f = threadIdx.x; f < count * kFloatsPerGaussian; f += kThreads)
+    new_params[static_cast<size_t>(o0) * kFloatsPerGaussian + f] = s_params[f];
+  for (uint32_t f = threadIdx.x; f < count * kShRest; f += kThreads) {
+    const uint32_t slot = f / kShRest, k = f - slot * kShRest;
+    const uint32_t src = first + (s_source[slot] & 0xffffu);
+    new_sh[static_cast<size_t>(o0) * kShRest + f] = sh[static_cast<size_t>(src) * kShRest + k];
+  }
+  for (uint32_t f = threadIdx.x; f < count * kMomentSlots; f += kThreads) {
+    const uint32_t slot = f / kMomentSlots, k = f - slot * kMomentSlots;
+    const uint32_t source = s_source[slot];
+    const uint32_t src = first + (source & 0xffffu);
+    new_moments[static_cast<size_t>(o0) * kMomentSlots + f] =
+        source >> 16 ? make_float2(0.f, 0.f) : moments[static_cast<size_t>(src) * kMomentSlots + k];
   }
 }
 
