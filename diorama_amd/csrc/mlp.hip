@@ -113,22 +113,100 @@ __global__ __launch_bounds__(kThreads) void features_kernel(uint32_t n, const fl
   for (int k = 0; k < kIn; k++) features[static_cast<size_t>(i) * kIn + k] = f[k];
 }
 
+// Inference on the matrix cores: v_mfma_f32_16x16x16_bf16, BF16 operands and FP32
+// accumulation. Each layer runs transposed, out^T = W^T in^T, so the 16 samples of
+// a group are the N dimension: the accumulator a lane holds (rows 4 (lane / 16) + r,
+// r = 0..3, of an output block, for sample lane % 16) is exactly the B operand the
+// next layer needs for that block of its K dimension (k = 4 (lane / 16) + r, same
+// sample), so the layers chain in registers with no exchange between lanes. A wave
+// keeps every weight tile as an A operand in registers (48 VGPRs) and takes groups
+// of 16 Gaussians in turn: 4 MFMAs for layer 1 (12 inputs padded to 16), 16 for
+// layer 2 and 4 for layer 3 (4 classes padded to 16). The standardised inputs and
+// both hidden layers are rounded to BF16 (nearest even) as the operands they become.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline bf16x4 pack_bf16(float a, float b, float c, float d) {
+  bf16x4 v;
+  v[0] = static_cast<short>(float_to_bf16(a));
+  v[1] = static_cast<short>(float_to_bf16(b));
+  v[2] = static_cast<short>(float_to_bf16(c));
+  v[3] = static_cast<short>(float_to_bf16(d));
+  return v;
+}
+
+constexpr int kBlocks = kHidden / 16;  // 16-wide blocks of a hidden layer
+
 __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(uint32_t n, const float* __restrict__ features,
                                                                 const uint16_t* __restrict__ weights,
                                                                 const float* __restrict__ norm,
                                                                 float* __restrict__ logits) {
-  __shared__ float w[kLds];
-  load_weights(w, weights, norm);
-  __syncthreads();
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  float x[kIn], h1[kHidden], h2[kHidden], z[kOut];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int col = lane % 16, quad = lane / 16;  // A/B: row or column `col`, k = 4 quad + r; D: rows 4 quad + r
+  // The weights as A operands: W1^T, W2^T and W3^T tiles, element r of lane (col, quad)
+  // at row col, k = 4 quad + r of the tile.
+  bf16x4 w1[kBlocks], w2[kBlocks][kBlocks], w3[kBlocks];
+  float b1[kBlocks][4], b2[kBlocks][4], b3[4];
 #pragma unroll
-  for (int k = 0; k < kIn; k++) x[k] = features[static_cast<size_t>(i) * kIn + k];
-  standardise(w, x);
-  forward(w, x, h1, h2, z);
+  for (int m = 0; m < kBlocks; m++) {
 #pragma unroll
-  for (int c = 0; c < kOut; c++) logits[static_cast<size_t>(i) * kOut + c] = z[c];
+    for (int r = 0; r < 4; r++) {
+      const int k = 4 * quad + r;
+      w1[m][r] = k < kIn ? static_cast<short>(weights[kW1 + k * kHidden + m * 16 + col]) : short(0);
+      b1[m][r] = bf16_to_float(weights[kB1 + m * 16 + 4 * quad + r]);
+      b2[m][r] = bf16_to_float(weights[kB2 + m * 16 + 4 * quad + r]);
+#pragma unroll
+      for (int ks = 0; ks < kBlocks; ks++)
+        w2[m][ks][r] = static_cast<short>(weights[kW2 + (ks * 16 + k) * kHidden + m * 16 + col]);
+      w3[m][r] = col < kOut ? static_cast<short>(weights[kW3 + (m * 16 + k) * kOut + col]) : short(0);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) b3[r] = bf16_to_float(weights[kB3 + r]);
+  float mean[4], scale[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int k = 4 * quad + r;
+    mean[r] = k < kIn ? norm[k] : 0.f;
+    scale[r] = k < kIn ? norm[kIn + k] : 0.f;
+  }
+  const uint32_t groups = ceil_div(n, 16u);
+  const uint32_t wave = (blockIdx.x * kThreads + threadIdx.x) / kWave, waves = gridDim.x * (kThreads / kWave);
+  for (uint32_t g = wave; g < groups; g += waves) {
+    const uint32_t sample = g * 16 + col;
+    // Inputs 4 quad .. 4 quad + 3 of this lane's sample (quad 3 holds the padding).
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (sample < n && quad < kIn / 4) {
+      const float4 v = *reinterpret_cast<const float4*>(features + static_cast<size_t>(sample) * kIn + 4 * quad);
+      x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    }
+    const bf16x4 in = pack_bf16((x[0] - mean[0]) * scale[0], (x[1] - mean[1]) * scale[1],
+                                (x[2] - mean[2]) * scale[2], (x[3] - mean[3]) * scale[3]);
+    bf16x4 h1[kBlocks];
+#pragma unroll
+    for (int m = 0; m < kBlocks; m++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[m], in, acc, 0, 0, 0);
+      h1[m] = pack_bf16(fmaxf(acc[0] + b1[m][0], 0.f), fmaxf(acc[1] + b1[m][1], 0.f),
+                        fmaxf(acc[2] + b1[m][2], 0.f), fmaxf(acc[3] + b1[m][3], 0.f));
+    }
+    bf16x4 h2[kBlocks];
+#pragma unroll
+    for (int m = 0; m < kBlocks; m++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kBlocks; ks++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[m][ks], h1[ks], acc, 0, 0, 0);
+      h2[m] = pack_bf16(fmaxf(acc[0] + b2[m][0], 0.f), fmaxf(acc[1] + b2[m][1], 0.f),
+                        fmaxf(acc[2] + b2[m][2], 0.f), fmaxf(acc[3] + b2[m][3], 0.f));
+    }
+    f32x4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kBlocks; ks++) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[ks], h2[ks], z, 0, 0, 0);
+    // Rows 0..3 of the logits tile (the classes) sit in lanes 0..15, one sample each.
+    if (quad == 0 && sample < n)
+      *reinterpret_cast<float4*>(logits + static_cast<size_t>(sample) * kOut) =
+          make_float4(z[0] + b3[0], z[1] + b3[1], z[2] + b3[2], z[3] + b3[3]);
+  }
 }
 
 // The sample a minibatch slot reads: every sample when `seed` is 0 (slot s reads
@@ -424,7 +502,9 @@ void mlp_features(hipStream_t s, uint32_t n, const float* params, const float* s
 void mlp_forward(hipStream_t s, uint32_t n, const float* features, const uint16_t* weights, const float* norm,
                  float* logits) {
   if (n == 0) return;
-  mlp_forward_kernel<<<ceil_div(n, kThreads), kThreads, 0, s>>>(n, features, weights, norm, logits);
+  // A wave takes groups of 16 Gaussians; enough waves to fill the GPU a few times over.
+  const uint32_t blocks = std::min<uint32_t>(ceil_div(ceil_div(n, 16u), kThreads / kWave), 2048);
+  mlp_forward_kernel<<<blocks, kThreads, 0, s>>>(n, features, weights, norm, logits);
   DIO_LAUNCHED("mlp_forward");
 }
 

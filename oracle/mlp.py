@@ -34,9 +34,24 @@ def bf16(w):
     return r.astype(np.uint32).view(np.float32)
 
 
-def forward(w, x, mean=None, scale=None):
-    """Logits (n, 4) and the hidden activations; inputs standardised first."""
+def forward(w, x, mean=None, scale=None, bf16_operands=False):
+    """Logits (n, 4) and the hidden activations; inputs standardised first.
+
+    bf16_operands: the matrix-core inference's arithmetic. The inputs are
+    standardised in float32 ((x - mean) * scale, two roundings) and rounded to BF16;
+    each hidden layer's float32 value (bias added, ReLU) is rounded to BF16 before
+    it multiplies the next layer's weights. Sums are exact here (float64)."""
     w1, b1, w2, b2, w3, b3 = unpack(w)
+    if bf16_operands:
+        x32 = np.asarray(x, dtype=np.float32)
+        if mean is not None:
+            x32 = (x32 - np.asarray(mean, dtype=np.float32)) * np.asarray(scale, dtype=np.float32)
+        x = bf16(x32).astype(np.float64)
+        a1 = x @ w1 + b1
+        h1 = bf16(np.maximum(a1, 0.0).astype(np.float32)).astype(np.float64)
+        a2 = h1 @ w2 + b2
+        h2 = bf16(np.maximum(a2, 0.0).astype(np.float32)).astype(np.float64)
+        return h2 @ w3 + b3, (x, a1, h1, a2, h2)
     x = np.asarray(x, dtype=np.float64)
     if mean is not None:
         x = (x - np.asarray(mean, dtype=np.float64)) * np.asarray(scale, dtype=np.float64)

@@ -49,16 +49,24 @@ def test_forward_and_gradients_against_oracle():
     say("mlp forward")
     net.forward(fx, n, logits)
     z = logits.download(np.float32).reshape(n, 4)
-    oz, _ = omlp.forward(wb, x, mean, scale)
-    err = np.abs(z - oz).max() / np.abs(oz).max()
+    # Inference runs on the matrix cores with BF16 operands: the oracle rounds the
+    # same operands. A hidden value whose FP32 sum lies within an FP32 ulp of a BF16
+    # tie can round the other way (the sums' order differs), which moves that
+    # sample's logits by about a BF16 ulp of one hidden unit times its weights.
+    oz, _ = omlp.forward(wb, x, mean, scale, bf16_operands=True)
+    per_sample = np.abs(z - oz).max(axis=1) / np.abs(oz).max()
+    err = float(per_sample.max())
+    exact_share = float((per_sample < 1e-5).mean())
+    print(f"mlp forward (BF16 operands): {exact_share:.4f} of samples within 1e-5, worst {err:.2e}")
+    assert exact_share > 0.98
+    assert err < 5e-3
     say("mlp gradients")
     cw = [0.5, 2.0, 3.0, 1.5]
     net.set_class_weights(cw)
     g, loss = net.gradients(fx, lb, n)
     oloss, og = omlp.loss_and_grads(wb, x, labels, cw, mean, scale)
     gerr = np.abs(g - og).max() / np.abs(og).max()
-    print(f"mlp: logits rel err {err:.2e}, loss {loss:.6f} vs {oloss:.6f}, gradient rel err {gerr:.2e}")
-    assert err < 1e-5
+    print(f"mlp: loss {loss:.6f} vs {oloss:.6f}, gradient rel err {gerr:.2e}")
     assert abs(loss - oloss) < 1e-5 * max(1.0, oloss)
     assert gerr < 1e-4
     net.destroy()

@@ -47,6 +47,37 @@ def test_bf16_rounds_to_nearest_even():
     assert np.array_equal(got, want)
 
 
+def test_bf16_operand_forward_by_hand():
+    """The matrix-core inference's arithmetic (forward(bf16_operands=True)): one input
+    of 1 feeds hidden unit 0 of layer 1 with weight 1 + 2^-8 and unit 1 with
+    1 + 3 * 2^-8. As BF16 operands (7 mantissa bits) they become 1 (the tie rounds to
+    the even neighbour) and 1 + 2^-6 (rounding up); layer 2 passes each to its own
+    unit with weight 1, and logit c reads unit c with weight 2^c. The exact forward
+    keeps the unrounded values. An input of 1 + 2^-9 standardises to itself and rounds
+    to 1 as well. A mean and scale that land it on 2.5 (exact in BF16) give hidden
+    values 2.5 (1 + 2^-8) = 2.509765625 and 2.5 (1 + 3 * 2^-8) = 2.529296875, which
+    round on BF16's grid of 2^-6 in [2, 4) to 2.515625 and 2.53125."""
+    w = np.zeros(mlp.PARAMS)
+    w1, b1, w2, b2, w3, b3 = [a for a in mlp.unpack(w)]  # views into w
+    w1[0, 0] = 1 + 2.0 ** -8
+    w1[0, 1] = 1 + 3 * 2.0 ** -8
+    w2[0, 0] = w2[1, 1] = 1.0
+    w3[0, 0], w3[1, 1] = 1.0, 2.0
+    x = np.zeros((1, mlp.INPUTS))
+    x[0, 0] = 1.0
+    z, _ = mlp.forward(w, x, bf16_operands=True)
+    assert np.array_equal(z[0], [1.0, 2 * (1 + 2.0 ** -6), 0.0, 0.0])
+    z, _ = mlp.forward(w, x)
+    assert np.array_equal(z[0], [1 + 2.0 ** -8, 2 * (1 + 3 * 2.0 ** -8), 0.0, 0.0])
+    x[0, 0] = 1 + 2.0 ** -9
+    z, _ = mlp.forward(w, x, bf16_operands=True)
+    assert np.array_equal(z[0], [1.0, 2 * (1 + 2.0 ** -6), 0.0, 0.0])
+    mean, scale = np.zeros(mlp.INPUTS), np.ones(mlp.INPUTS)
+    mean[0], scale[0] = 1 + 2.0 ** -9 - 1.25, 2.0
+    z, _ = mlp.forward(w, x, mean, scale, bf16_operands=True)
+    assert np.array_equal(z[0], [2.515625, 2 * 2.53125, 0.0, 0.0])
+
+
 def test_features_by_hand():
     p = np.zeros((1, 16))
     p[0, 3] = 0.0                          # opacity 0.5
