@@ -29,47 +29,6 @@ __device__ inline uint16_t float_to_bf16(float f) {
   return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-// The weights as FP32 in LDS, followed by the input standardisation (mean[12],
-// scale[12]: x' = (x - mean) scale), which is not trained.
-constexpr int kNorm = kParams, kLds = kParams + 2 * kIn;
-
-__device__ inline void load_weights(float* __restrict__ w, const uint16_t* __restrict__ weights,
-                                    const float* __restrict__ norm) {
-  for (int k = threadIdx.x; k < kParams; k += blockDim.x) w[k] = bf16_to_float(weights[k]);
-  for (int k = threadIdx.x; k < 2 * kIn; k += blockDim.x) w[kNorm + k] = norm[k];
-}
-
-__device__ inline void standardise(const float* __restrict__ w, float x[kIn]) {
-#pragma unroll
-  for (int i = 0; i < kIn; i++) x[i] = (x[i] - w[kNorm + i]) * w[kNorm + kIn + i];
-}
-
-// One sample's forward pass: both hidden layers (after ReLU) and the logits.
-__device__ inline void forward(const float* __restrict__ w, const float x[kIn], float h1[kHidden], float h2[kHidden],
-                               float z[kOut]) {
-#pragma unroll
-  for (int j = 0; j < kHidden; j++) {
-    float a = w[kB1 + j];
-#pragma unroll
-    for (int i = 0; i < kIn; i++) a += x[i] * w[kW1 + i * kHidden + j];
-    h1[j] = fmaxf(a, 0.f);
-  }
-#pragma unroll
-  for (int k = 0; k < kHidden; k++) {
-    float a = w[kB2 + k];
-#pragma unroll
-    for (int j = 0; j < kHidden; j++) a += h1[j] * w[kW2 + j * kHidden + k];
-    h2[k] = fmaxf(a, 0.f);
-  }
-#pragma unroll
-  for (int c = 0; c < kOut; c++) {
-    float a = w[kB3 + c];
-#pragma unroll
-    for (int k = 0; k < kHidden; k++) a += h2[k] * w[kW3 + k * kOut + c];
-    z[c] = a;
-  }
-}
-
 // The 12 inputs of one Gaussian, each brought to order one:
 //  0 log(1 + mean view-space gradient / 2e-4)   1 the same of the largest one
 //  2 opacity                                     3 log(largest scale / extent)
@@ -113,16 +72,18 @@ __global__ __launch_bounds__(kThreads) void features_kernel(uint32_t n, const fl
   for (int k = 0; k < kIn; k++) features[static_cast<size_t>(i) * kIn + k] = f[k];
 }
 
-// Inference on the matrix cores: v_mfma_f32_16x16x16_bf16, BF16 operands and FP32
-// accumulation. Each layer runs transposed, out^T = W^T in^T, so the 16 samples of
-// a group are the N dimension: the accumulator a lane holds (rows 4 (lane / 16) + r,
-// r = 0..3, of an output block, for sample lane % 16) is exactly the B operand the
-// next layer needs for that block of its K dimension (k = 4 (lane / 16) + r, same
-// sample), so the layers chain in registers with no exchange between lanes. A wave
-// keeps every weight tile as an A operand in registers (48 VGPRs) and takes groups
-// of 16 Gaussians in turn: 4 MFMAs for layer 1 (12 inputs padded to 16), 16 for
-// layer 2 and 4 for layer 3 (4 classes padded to 16). The standardised inputs and
-// both hidden layers are rounded to BF16 (nearest even) as the operands they become.
+// The network on the matrix cores: v_mfma_f32_16x16x16_bf16, BF16 operands and FP32
+// accumulation. A wave takes 16 samples at a time. Each layer of the forward pass
+// runs transposed, out^T = W^T in^T, so the samples are the N dimension: the
+// accumulator a lane holds (rows 4 quad + r, r = 0..3, of an output block, for
+// sample `col`, where col = lane % 16 and quad = lane / 16) is exactly the B operand
+// the next layer needs for that block of its K dimension (k = 4 quad + r, the same
+// sample), so the layers chain in registers with no exchange between lanes. The
+// same holds for the backward pass's deltas, d2^T = W3 dz^T and d1^T = W2 d2^T.
+// Weight tiles are A operands held in registers; element r of lane (col, quad) is
+// row col, k = 4 quad + r of its tile. The standardised inputs, both hidden layers
+// and the deltas are rounded to BF16 (nearest even) as the operands they become;
+// the bias gradients sum the deltas before that rounding.
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -135,77 +96,104 @@ __device__ inline bf16x4 pack_bf16(float a, float b, float c, float d) {
   return v;
 }
 
+__device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
 constexpr int kBlocks = kHidden / 16;  // 16-wide blocks of a hidden layer
 
+// The forward pass's weights as A operands and its biases in the accumulator layout.
+struct ForwardTiles {
+  bf16x4 w1[kBlocks], w2[kBlocks][kBlocks], w3[kBlocks];
+  float b1[kBlocks][4], b2[kBlocks][4], b3[4];
+  float mean[4], scale[4];  // the standardisation of inputs 4 quad .. 4 quad + 3
+
+  __device__ void load(const uint16_t* __restrict__ weights, const float* __restrict__ norm, int col, int quad) {
+#pragma unroll
+    for (int m = 0; m < kBlocks; m++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int k = 4 * quad + r;
+        w1[m][r] = k < kIn ? static_cast<short>(weights[kW1 + k * kHidden + m * 16 + col]) : short(0);
+        b1[m][r] = bf16_to_float(weights[kB1 + m * 16 + 4 * quad + r]);
+        b2[m][r] = bf16_to_float(weights[kB2 + m * 16 + 4 * quad + r]);
+#pragma unroll
+        for (int ks = 0; ks < kBlocks; ks++)
+          w2[m][ks][r] = static_cast<short>(weights[kW2 + (ks * 16 + k) * kHidden + m * 16 + col]);
+        w3[m][r] = col < kOut ? static_cast<short>(weights[kW3 + (m * 16 + k) * kOut + col]) : short(0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int k = 4 * quad + r;
+      b3[r] = bf16_to_float(weights[kB3 + r]);
+      mean[r] = k < kIn ? norm[k] : 0.f;
+      scale[r] = k < kIn ? norm[kIn + k] : 0.f;
+    }
+  }
+
+  // One group's forward pass from a lane's 4 raw inputs (zeros past input 11): the
+  // BF16 operands of each layer and the logits (rows 0..3 = classes, in lanes of
+  // quad 0).
+  __device__ void run(const float x[4], bf16x4& in, bf16x4 (&h1)[kBlocks], bf16x4 (&h2)[kBlocks], f32x4& z) const {
+    in = pack_bf16((x[0] - mean[0]) * scale[0], (x[1] - mean[1]) * scale[1], (x[2] - mean[2]) * scale[2],
+                   (x[3] - mean[3]) * scale[3]);
+#pragma unroll
+    for (int m = 0; m < kBlocks; m++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(w1[m], in, acc);
+      h1[m] = pack_bf16(fmaxf(acc[0] + b1[m][0], 0.f), fmaxf(acc[1] + b1[m][1], 0.f),
+                        fmaxf(acc[2] + b1[m][2], 0.f), fmaxf(acc[3] + b1[m][3], 0.f));
+    }
+#pragma unroll
+    for (int m = 0; m < kBlocks; m++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kBlocks; ks++) acc = mfma(w2[m][ks], h1[ks], acc);
+      h2[m] = pack_bf16(fmaxf(acc[0] + b2[m][0], 0.f), fmaxf(acc[1] + b2[m][1], 0.f),
+                        fmaxf(acc[2] + b2[m][2], 0.f), fmaxf(acc[3] + b2[m][3], 0.f));
+    }
+    z = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kBlocks; ks++) z = mfma(w3[ks], h2[ks], z);
+#pragma unroll
+    for (int r = 0; r < 4; r++) z[r] += b3[r];
+  }
+};
+
+// A lane's 4 inputs of `sample` (inputs 4 quad .. 4 quad + 3; zeros past input 11 or
+// past the samples).
+__device__ inline void load_inputs(const float* __restrict__ features, uint32_t sample, bool valid, int quad,
+                                   float x[4]) {
+  x[0] = x[1] = x[2] = x[3] = 0.f;
+  if (valid && quad < kIn / 4) {
+    const float4 v = *reinterpret_cast<const float4*>(features + static_cast<size_t>(sample) * kIn + 4 * quad);
+    x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+  }
+}
+
+// Inference: a wave takes groups of 16 Gaussians: 4 MFMAs for layer 1 (12 inputs
+// padded to 16), 16 for layer 2 and 4 for layer 3 (4 classes padded to 16).
 __global__ __launch_bounds__(kThreads) void mlp_forward_kernel(uint32_t n, const float* __restrict__ features,
                                                                 const uint16_t* __restrict__ weights,
                                                                 const float* __restrict__ norm,
                                                                 float* __restrict__ logits) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int col = lane % 16, quad = lane / 16;  // A/B: row or column `col`, k = 4 quad + r; D: rows 4 quad + r
-  // The weights as A operands: W1^T, W2^T and W3^T tiles, element r of lane (col, quad)
-  // at row col, k = 4 quad + r of the tile.
-  bf16x4 w1[kBlocks], w2[kBlocks][kBlocks], w3[kBlocks];
-  float b1[kBlocks][4], b2[kBlocks][4], b3[4];
-#pragma unroll
-  for (int m = 0; m < kBlocks; m++) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int k = 4 * quad + r;
-      w1[m][r] = k < kIn ? static_cast<short>(weights[kW1 + k * kHidden + m * 16 + col]) : short(0);
-      b1[m][r] = bf16_to_float(weights[kB1 + m * 16 + 4 * quad + r]);
-      b2[m][r] = bf16_to_float(weights[kB2 + m * 16 + 4 * quad + r]);
-#pragma unroll
-      for (int ks = 0; ks < kBlocks; ks++)
-        w2[m][ks][r] = static_cast<short>(weights[kW2 + (ks * 16 + k) * kHidden + m * 16 + col]);
-      w3[m][r] = col < kOut ? static_cast<short>(weights[kW3 + (m * 16 + k) * kOut + col]) : short(0);
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < 4; r++) b3[r] = bf16_to_float(weights[kB3 + r]);
-  float mean[4], scale[4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int k = 4 * quad + r;
-    mean[r] = k < kIn ? norm[k] : 0.f;
-    scale[r] = k < kIn ? norm[kIn + k] : 0.f;
-  }
+  const int col = lane % 16, quad = lane / 16;
+  ForwardTiles net;
+  net.load(weights, norm, col, quad);
   const uint32_t groups = ceil_div(n, 16u);
   const uint32_t wave = (blockIdx.x * kThreads + threadIdx.x) / kWave, waves = gridDim.x * (kThreads / kWave);
   for (uint32_t g = wave; g < groups; g += waves) {
     const uint32_t sample = g * 16 + col;
-    // Inputs 4 quad .. 4 quad + 3 of this lane's sample (quad 3 holds the padding).
-    float x[4] = {0.f, 0.f, 0.f, 0.f};
-    if (sample < n && quad < kIn / 4) {
-      const float4 v = *reinterpret_cast<const float4*>(features + static_cast<size_t>(sample) * kIn + 4 * quad);
-      x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-    }
-    const bf16x4 in = pack_bf16((x[0] - mean[0]) * scale[0], (x[1] - mean[1]) * scale[1],
-                                (x[2] - mean[2]) * scale[2], (x[3] - mean[3]) * scale[3]);
-    bf16x4 h1[kBlocks];
-#pragma unroll
-    for (int m = 0; m < kBlocks; m++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1[m], in, acc, 0, 0, 0);
-      h1[m] = pack_bf16(fmaxf(acc[0] + b1[m][0], 0.f), fmaxf(acc[1] + b1[m][1], 0.f),
-                        fmaxf(acc[2] + b1[m][2], 0.f), fmaxf(acc[3] + b1[m][3], 0.f));
-    }
-    bf16x4 h2[kBlocks];
-#pragma unroll
-    for (int m = 0; m < kBlocks; m++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < kBlocks; ks++) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[m][ks], h1[ks], acc, 0, 0, 0);
-      h2[m] = pack_bf16(fmaxf(acc[0] + b2[m][0], 0.f), fmaxf(acc[1] + b2[m][1], 0.f),
-                        fmaxf(acc[2] + b2[m][2], 0.f), fmaxf(acc[3] + b2[m][3], 0.f));
-    }
-    f32x4 z = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < kBlocks; ks++) z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[ks], h2[ks], z, 0, 0, 0);
+    float x[4];
+    load_inputs(features, sample, sample < n, quad, x);
+    bf16x4 in, h1[kBlocks], h2[kBlocks];
+    f32x4 z;
+    net.run(x, in, h1, h2, z);
     // Rows 0..3 of the logits tile (the classes) sit in lanes 0..15, one sample each.
     if (quad == 0 && sample < n)
-      *reinterpret_cast<float4*>(logits + static_cast<size_t>(sample) * kOut) =
-          make_float4(z[0] + b3[0], z[1] + b3[1], z[2] + b3[2], z[3] + b3[3]);
+      *reinterpret_cast<float4*>(logits + static_cast<size_t>(sample) * kOut) = make_float4(z[0], z[1], z[2], z[3]);
   }
 }
 
@@ -222,24 +210,35 @@ __device__ inline uint32_t sample_of(uint32_t slot, uint32_t count, uint32_t see
   return pcg_hash(pcg_hash(seed) ^ slot) % count;
 }
 
-// Gradients of the mean class-weighted softmax cross-entropy over `batch`
-// samples, added into `grads` (which the caller cleared), and the loss sum into
-// loss[0]. A workgroup
-// takes 32 samples: threads 0..31 run one sample each forward and backward and
-// leave its activations and their gradients in LDS (55 KB with the weights); then
-// all 256 threads form the weight gradients as sums of outer products over the 32
-// samples and add them out with one atomic per parameter.
-constexpr int kTrainSamples = 32;
+// Gradients of the mean class-weighted softmax cross-entropy over `batch` samples,
+// added into `grads` (which the caller cleared), and the loss sum into loss[0]. A
+// workgroup takes 64 samples, 16 per wave:
+// 1. each wave runs its samples forward, forms dz = (p - onehot) w_class / batch,
+//    and runs the deltas back, d2^T = (W3 dz^T) masked by ReLU and d1^T = (W2 d2^T)
+//    masked; the BF16 operands (inputs, hidden layers, deltas) go to LDS sample by
+//    sample, and the bias gradients' sums over the wave's samples to LDS atomics;
+// 2. the weight gradients dW1 = x^T d1, dW2 = h1^T d2, dW3 = h2^T dz are 24 output
+//    tiles of 16 x 16 over K = the 64 samples, six a wave, each read from LDS in the
+//    layout of its operand; every tile goes out with one float atomic per element.
+constexpr int kTrainWaveSamples = 16, kTrainSamples = kTrainWaveSamples * (kThreads / kWave);  // 64
+constexpr int kPad = 16;  // inputs and classes padded to one K block
 
 struct TrainShared {
-  float w[kLds];
-  float x[kTrainSamples][kIn];
-  float h1[kTrainSamples][kHidden];
-  float d1[kTrainSamples][kHidden];  // dL/d(pre-activation of layer 1)
-  float h2[kTrainSamples][kHidden];
-  float d2[kTrainSamples][kHidden];  // dL/d(pre-activation of layer 2)
-  float dz[kTrainSamples][kOut];
+  uint16_t x[kTrainSamples][kPad];      // BF16, by sample
+  uint16_t h1[kTrainSamples][kHidden];
+  uint16_t h2[kTrainSamples][kHidden];
+  uint16_t d1[kTrainSamples][kHidden];  // dL/d(pre-activation of layer 1), BF16
+  uint16_t d2[kTrainSamples][kHidden];  // dL/d(pre-activation of layer 2), BF16
+  uint16_t dz[kTrainSamples][kPad];     // dL/dlogits, BF16
+  float bias[kHidden + kHidden + kOut];  // the bias gradients' sums over the workgroup
 };
+
+// Sums of v over the 16 lanes of a quad (lanes with the same `quad`), in every lane.
+__device__ inline float quad_sum(float v) {
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
 
 __global__ __launch_bounds__(kThreads) void mlp_train_kernel(uint32_t batch, uint32_t count, uint32_t seed,
                                                               const float* __restrict__ features,
@@ -248,94 +247,139 @@ __global__ __launch_bounds__(kThreads) void mlp_train_kernel(uint32_t batch, uin
                                                               const float* __restrict__ norm, float4 class_weight,
                                                               float* __restrict__ grads, float* __restrict__ loss) {
   __shared__ TrainShared sh;
-  load_weights(sh.w, weights, norm);
-  __syncthreads();
-  const uint32_t first = blockIdx.x * kTrainSamples;
-  const uint32_t live = min(static_cast<uint32_t>(kTrainSamples), batch - first);
-  const int t = threadIdx.x;
-  const float inv_batch = 1.f / static_cast<float>(batch);
-  if (t < kTrainSamples) {
-    float x[kIn], h1[kHidden], h2[kHidden], z[kOut], dz[kOut];
-    const bool active = static_cast<uint32_t>(t) < live;
-    uint32_t label = 0;
-    if (active) {
-      const uint32_t sample = sample_of(first + t, count, seed);
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int col = lane % 16, quad = lane / 16;
+  for (int k = tid; k < kHidden + kHidden + kOut; k += kThreads) sh.bias[k] = 0.f;
+  __syncthreads();  // the bias sums start at zero before any wave adds into them
+  // Phase 1: this wave's 16 samples forward and back.
+  {
+    ForwardTiles net;
+    net.load(weights, norm, col, quad);
+    const uint32_t slot = blockIdx.x * kTrainSamples + wave * kTrainWaveSamples + col;
+    const bool active = slot < batch;
+    const uint32_t sample = active ? sample_of(slot, count, seed) : 0u;
+    float x[4];
+    load_inputs(features, sample, active, quad, x);
+    bf16x4 in, h1[kBlocks], h2[kBlocks];
+    f32x4 z;
+    net.run(x, in, h1, h2, z);
+    const int row = wave * kTrainWaveSamples + col;  // this lane's sample in LDS
+    // Softmax cross-entropy in the lanes holding the logits (quad 0).
+    float dz[4] = {0.f, 0.f, 0.f, 0.f}, sample_loss = 0.f;
+    if (quad == 0 && active) {
+      const uint32_t label = labels[sample];
+      const float top = fmaxf(fmaxf(z[0], z[1]), fmaxf(z[2], z[3]));
+      float total = 0.f;
 #pragma unroll
-      for (int k = 0; k < kIn; k++) x[k] = features[static_cast<size_t>(sample) * kIn + k];
-      standardise(sh.w, x);
-      label = labels[sample];
-    } else {
+      for (int c = 0; c < kOut; c++) total += __expf(z[c] - top);
+      const float log_total = logf(total);
+      const float cw = label == 0u ? class_weight.x : (label == 1u ? class_weight.y : (label == 2u ? class_weight.z
+                                                                                                    : class_weight.w));
+      const float inv_batch = 1.f / static_cast<float>(batch);
 #pragma unroll
-      for (int k = 0; k < kIn; k++) x[k] = 0.f;
+      for (int c = 0; c < kOut; c++)
+        dz[c] = (__expf(z[c] - top - log_total) - (static_cast<uint32_t>(c) == label ? 1.f : 0.f)) * cw * inv_batch;
+      const float zl = label == 0u ? z[0] : (label == 1u ? z[1] : (label == 2u ? z[2] : z[3]));
+      sample_loss = cw * (top + log_total - zl);
     }
-    forward(sh.w, x, h1, h2, z);
-    float top = z[0];
+    const float wave_loss = quad_sum(sample_loss);
+    if (lane == 0 && wave_loss != 0.f) atomicAdd(loss, wave_loss);
+    // dz^T as a B operand: rows (classes) 4 quad + r, only quad 0 holds classes 0..3.
+    const bf16x4 dzb = pack_bf16(dz[0], dz[1], dz[2], dz[3]);
+    // d2^T = W3 dz^T (A tile m: rows = hidden m 16 + col, k = class 4 quad + r), masked.
+    bf16x4 d2b[kBlocks];
 #pragma unroll
-    for (int c = 1; c < kOut; c++) top = fmaxf(top, z[c]);
-    float total = 0.f;
+    for (int m = 0; m < kBlocks; m++) {
+      bf16x4 a;
 #pragma unroll
-    for (int c = 0; c < kOut; c++) total += __expf(z[c] - top);
-    const float log_total = logf(total);
-    const float cw = label == 0u ? class_weight.x : (label == 1u ? class_weight.y : (label == 2u ? class_weight.z
-                                                                                                  : class_weight.w));
+      for (int r = 0; r < 4; r++)
+        a[r] = quad == 0 ? static_cast<short>(weights[kW3 + (m * 16 + col) * kOut + r]) : short(0);
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(a, dzb, acc);
+      float d[4];
 #pragma unroll
-    for (int c = 0; c < kOut; c++) {
-      const float prob = __expf(z[c] - top - log_total);
-      dz[c] = active ? (prob - (static_cast<uint32_t>(c) == label ? 1.f : 0.f)) * cw * inv_batch : 0.f;
+      for (int r = 0; r < 4; r++)  // ReLU: h2 > 0 exactly where its pre-activation is
+        d[r] = bf16_to_float(static_cast<uint16_t>(h2[m][r])) > 0.f ? acc[r] : 0.f;
+      d2b[m] = pack_bf16(d[0], d[1], d[2], d[3]);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const float s = quad_sum(d[r]);
+        if (col == 0 && s != 0.f) atomicAdd(&sh.bias[kHidden + m * 16 + 4 * quad + r], s);
+      }
+      *reinterpret_cast<bf16x4*>(&sh.d2[row][m * 16 + 4 * quad]) = d2b[m];
+      *reinterpret_cast<bf16x4*>(&sh.h2[row][m * 16 + 4 * quad]) = h2[m];
+      *reinterpret_cast<bf16x4*>(&sh.h1[row][m * 16 + 4 * quad]) = h1[m];
     }
-    if (active) {
-      float zl = z[0];
+    // d1^T = W2 d2^T (A tile (mj, ks): rows = j = mj 16 + col, k = ks 16 + 4 quad + r), masked.
 #pragma unroll
-      for (int c = 1; c < kOut; c++) zl = static_cast<uint32_t>(c) == label ? z[c] : zl;
-      atomicAdd(loss, cw * (top + log_total - zl));
+    for (int mj = 0; mj < kBlocks; mj++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kBlocks; ks++) {
+        bf16x4 a;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          a[r] = static_cast<short>(weights[kW2 + (mj * 16 + col) * kHidden + ks * 16 + 4 * quad + r]);
+        acc = mfma(a, d2b[ks], acc);
+      }
+      float d[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) d[r] = bf16_to_float(static_cast<uint16_t>(h1[mj][r])) > 0.f ? acc[r] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const float s = quad_sum(d[r]);
+        if (col == 0 && s != 0.f) atomicAdd(&sh.bias[mj * 16 + 4 * quad + r], s);
+      }
+      *reinterpret_cast<bf16x4*>(&sh.d1[row][mj * 16 + 4 * quad]) = pack_bf16(d[0], d[1], d[2], d[3]);
     }
-    float d2[kHidden];
+    *reinterpret_cast<bf16x4*>(&sh.x[row][4 * quad]) = in;
+    *reinterpret_cast<bf16x4*>(&sh.dz[row][4 * quad]) = dzb;
 #pragma unroll
-    for (int k = 0; k < kHidden; k++) {
-      float a = 0.f;
-#pragma unroll
-      for (int c = 0; c < kOut; c++) a += dz[c] * sh.w[kW3 + k * kOut + c];
-      d2[k] = h2[k] > 0.f ? a : 0.f;
+    for (int r = 0; r < 4; r++) {
+      const float s = quad_sum(dz[r]);
+      if (lane == 0 && s != 0.f) atomicAdd(&sh.bias[2 * kHidden + r], s);
     }
-#pragma unroll
-    for (int j = 0; j < kHidden; j++) {
-      float a = 0.f;
-#pragma unroll
-      for (int k = 0; k < kHidden; k++) a += d2[k] * sh.w[kW2 + j * kHidden + k];
-      sh.d1[t][j] = h1[j] > 0.f ? a : 0.f;
-      sh.h1[t][j] = h1[j];
-      sh.h2[t][j] = h2[j];
-      sh.d2[t][j] = d2[j];
-    }
-#pragma unroll
-    for (int k = 0; k < kIn; k++) sh.x[t][k] = x[k];
-#pragma unroll
-    for (int c = 0; c < kOut; c++) sh.dz[t][c] = dz[c];
   }
   __syncthreads();
-  // dW1 = Σ x ⊗ d1, db1 = Σ d1, dW2 = Σ h1 ⊗ d2, db2 = Σ d2, dW3 = Σ h2 ⊗ dz, db3 = Σ dz.
-  for (int e = t; e < kParams; e += kThreads) {
-    float g = 0.f;
-    if (e < kB1) {
-      const int i = e / kHidden, j = e % kHidden;
-      for (uint32_t s = 0; s < live; s++) g += sh.x[s][i] * sh.d1[s][j];
-    } else if (e < kW2) {
-      const int j = e - kB1;
-      for (uint32_t s = 0; s < live; s++) g += sh.d1[s][j];
-    } else if (e < kB2) {
-      const int j = (e - kW2) / kHidden, k = (e - kW2) % kHidden;
-      for (uint32_t s = 0; s < live; s++) g += sh.h1[s][j] * sh.d2[s][k];
-    } else if (e < kW3) {
-      const int k = e - kB2;
-      for (uint32_t s = 0; s < live; s++) g += sh.d2[s][k];
-    } else if (e < kB3) {
-      const int k = (e - kW3) / kOut, c = (e - kW3) % kOut;
-      for (uint32_t s = 0; s < live; s++) g += sh.h2[s][k] * sh.dz[s][c];
-    } else {
-      const int c = e - kB3;
-      for (uint32_t s = 0; s < live; s++) g += sh.dz[s][c];
+  // Phase 2: weight-gradient tiles, dW[i][j] = sum over samples s of P[s][i] Q[s][j]:
+  // A = P^T (row i = col, k = sample 4 quad + r of the K block), B = Q (k = sample
+  // 4 quad + r, column j = col); the accumulator holds rows 4 quad + r, column col.
+  for (int tile = wave; tile < 24; tile += kThreads / kWave) {
+    const uint16_t* P;
+    const uint16_t* Q;
+    int p_stride, q_stride, ib, jb, rows, cols, base;
+    if (tile < 4) {  // dW1: 16 (12) x 64
+      P = &sh.x[0][0]; p_stride = kPad; Q = &sh.d1[0][0]; q_stride = kHidden;
+      ib = 0; jb = tile; rows = kIn; cols = kHidden; base = kW1;
+    } else if (tile < 20) {  // dW2: 64 x 64
+      P = &sh.h1[0][0]; p_stride = kHidden; Q = &sh.d2[0][0]; q_stride = kHidden;
+      ib = (tile - 4) / 4; jb = (tile - 4) % 4; rows = kHidden; cols = kHidden; base = kW2;
+    } else {  // dW3: 64 x 16 (4)
+      P = &sh.h2[0][0]; p_stride = kHidden; Q = &sh.dz[0][0]; q_stride = kPad;
+      ib = tile - 20; jb = 0; rows = kHidden; cols = kOut; base = kW3;
     }
-    atomicAdd(grads + e, g);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kTrainSamples / 16; ks++) {
+      bf16x4 a, b;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int s = ks * 16 + 4 * quad + r;
+        a[r] = static_cast<short>(P[s * p_stride + ib * 16 + col]);
+        b[r] = static_cast<short>(Q[s * q_stride + jb * 16 + col]);
+      }
+      acc = mfma(a, b, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = ib * 16 + 4 * quad + r, j = jb * 16 + col;
+      if (i < rows && j < cols && acc[r] != 0.f) atomicAdd(grads + base + i * cols + j, acc[r]);
+    }
+  }
+  for (int k = tid; k < kHidden + kHidden + kOut; k += kThreads) {
+    const float g = sh.bias[k];
+    const int at = k < kHidden ? kB1 + k : (k < 2 * kHidden ? kB2 + (k - kHidden) : kB3 + (k - 2 * kHidden));
+    if (g != 0.f) atomicAdd(grads + at, g);
   }
 }
 

@@ -62,8 +62,15 @@ def forward(w, x, mean=None, scale=None, bf16_operands=False):
     return h2 @ w3 + b3, (x, a1, h1, a2, h2)
 
 
-def loss_and_grads(w, x, labels, class_weights=None, mean=None, scale=None):
-    """Mean class-weighted softmax cross-entropy over the samples and its gradient (PARAMS,)."""
+def loss_and_grads(w, x, labels, class_weights=None, mean=None, scale=None, bf16_operands=False):
+    """Mean class-weighted softmax cross-entropy over the samples and its gradient (PARAMS,).
+
+    bf16_operands: the matrix-core training's arithmetic. The forward pass is
+    forward(bf16_operands=True); dL/dlogits and the masked deltas of both hidden
+    layers are rounded (through float32) to BF16 where they multiply weights or
+    activations, and the bias gradients sum them before that rounding."""
+    if bf16_operands:
+        return _loss_and_grads_bf16(w, x, labels, class_weights, mean, scale)
     z, (x, a1, h1, a2, h2) = forward(w, x, mean, scale)
     n = len(x)
     labels = np.asarray(labels, dtype=np.int64)
@@ -129,3 +136,30 @@ def extra_stats(splats, splat_grads, viewport, extra):
     out[vis, 2] += gx[vis]
     out[vis, 3] += gy[vis]
     return out
+
+
+def _bf16_64(v):
+    return bf16(np.asarray(v, dtype=np.float64).astype(np.float32)).astype(np.float64)
+
+
+def _loss_and_grads_bf16(w, x, labels, class_weights, mean, scale):
+    z, (xb, a1, h1, a2, h2) = forward(w, x, mean, scale, bf16_operands=True)
+    n = len(xb)
+    labels = np.asarray(labels, dtype=np.int64)
+    zmax = z.max(axis=1, keepdims=True)
+    logsum = np.log(np.exp(z - zmax).sum(axis=1, keepdims=True)) + zmax
+    cw = np.ones(CLASSES) if class_weights is None else np.asarray(class_weights, dtype=np.float64)
+    per = cw[labels]
+    loss = float(np.mean(per * (logsum[:, 0] - z[np.arange(n), labels])))
+    dz = np.exp(z - logsum)
+    dz[np.arange(n), labels] -= 1.0
+    dz *= per[:, None] / n
+    _, _, w2, _, w3, _ = unpack(w)
+    dzb = _bf16_64(dz)
+    d2 = (dzb @ w3.T) * (a2 > 0)
+    d2b = _bf16_64(d2)
+    d1 = (d2b @ w2.T) * (a1 > 0)
+    d1b = _bf16_64(d1)
+    grads = [xb.T @ d1b, d1.sum(0), h1.T @ d2b, d2.sum(0), h2.T @ dzb, dz.sum(0)]
+    return loss, np.concatenate([g.reshape(-1) for g in grads])
+

@@ -78,6 +78,47 @@ def test_bf16_operand_forward_by_hand():
     assert np.array_equal(z[0], [2.515625, 2 * 2.53125, 0.0, 0.0])
 
 
+def test_bf16_operand_gradients_by_hand():
+    """The matrix-core training's arithmetic (loss_and_grads(bf16_operands=True)) on one
+    sample: input 0 is 1; layer 1 passes it to unit 0 with weight 1 + 2^-8 (a BF16
+    operand of 1: the tie rounds to even), layers 2 and 3 pass unit 0 on with weight 1
+    to logit 0; the label is class 1. So z = (1, 0, 0, 0), dz = softmax(z) - e_1, and
+    every delta on the path is dz_0 rounded to BF16 (d2_0 = W3[0,0] bf16(dz_0),
+    d1_0 = W2[0,0] bf16(d2_0)). The weight gradients multiply rounded operands
+    (dW2[0,0] = h1_0 bf16(d2_0) with h1_0 = 1, not 1 + 2^-8), the bias gradients
+    sum the deltas before rounding, and every other entry is zero."""
+    w = np.zeros(mlp.PARAMS)
+    w1, b1, w2, b2, w3, b3 = mlp.unpack(w)
+    w1[0, 0] = 1 + 2.0 ** -8
+    w2[0, 0] = 1.0
+    w3[0, 0] = 1.0
+    x = np.zeros((1, mlp.INPUTS))
+    x[0, 0] = 1.0
+    loss, g = mlp.loss_and_grads(w, x, [1], bf16_operands=True)
+    p = np.exp([1.0, 0.0, 0.0, 0.0])
+    p /= p.sum()
+    dz = p - np.array([0.0, 1.0, 0.0, 0.0])
+    dzb = mlp.bf16(dz.astype(np.float32)).astype(np.float64)
+    assert dzb[0] != dz[0] and dzb[1] != dz[1]  # the rounding shows
+    assert abs(loss + math.log(p[1])) < 1e-15
+    gw1, gb1, gw2, gb2, gw3, gb3 = mlp.unpack(g)
+    want = [np.zeros(s_) for s_ in mlp.SHAPES]
+    want[0][0, 0] = dzb[0]        # x_0 d1_0
+    want[1][0] = dzb[0]           # d1_0 (the rounded d2_0 times W2[0,0] = 1)
+    want[2][0, 0] = dzb[0]        # h1_0 bf16(d2_0), h1_0 = 1
+    want[3][0] = dzb[0]           # d2_0 = W3[0,0] bf16(dz_0)
+    want[4][0, :] = dzb           # h2_0 bf16(dz)
+    want[5][:] = dz               # dz, unrounded
+    for got, exp in zip((gw1, gb1, gw2, gb2, gw3), want[:5]):
+        assert np.array_equal(got, exp)
+    assert np.abs(gb3 - dz).max() < 1e-15  # the softmax's float64 rounding only
+    # Without the rounding, h1_0 = h2_0 = z_0 = 1 + 2^-8 all the way through.
+    _, exact = mlp.loss_and_grads(w, x, [1])
+    z0 = 1 + 2.0 ** -8
+    p0 = math.exp(z0) / (math.exp(z0) + 3.0)
+    assert abs(mlp.unpack(exact)[2][0, 0] - z0 * p0) < 1e-15
+
+
 def test_features_by_hand():
     p = np.zeros((1, 16))
     p[0, 3] = 0.0                          # opacity 0.5
