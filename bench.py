@@ -60,6 +60,18 @@ STAGE_BYTES = {
 }
 # Clocked stages summed into a reported one.
 STAGE_PARTS = {"sort": ("depth_sort", "tile_sort", "list_order")}
+# Tile lists binned by counts (per_tile_depth 2) run no radix sort and no duplicate
+# pass: their stages are charged what they move. bin_count reads each Gaussian's
+# packed rectangle (8) and tile count (4); bin_scatter reads those and its depth key
+# (4) and writes a (key, splat) pair per (splat, tile) pair (8) and each tile's range
+# (8) and order slot (4); list_order reads the pairs (8) and the ranges (8 a tile)
+# and writes the splat lists (4 a pair).
+BINNED_STAGES = {
+    "bin_count": ((12, 0, 0, 0), ("depth_sort", "scan")),
+    "bin_scatter": ((16, 8, 0, 12), ("duplicate",)),
+    "list_order": ((0, 12, 0, 8), ("tile_sort", "ranges", "list_order")),
+}
+BINNED_REPLACES = ("scan", "sort", "duplicate", "ranges")
 # The reference's per-step byte model (separate preprocess backward, stats and Adam).
 STEP_BYTES = (2612, 240, 88)
 # Stage name -> kernel name, for the PMC traffic, VALU and occupancy lines (the
@@ -786,8 +798,15 @@ def main():
     x = WIDTH * HEIGHT
     n_tiles = math.ceil(WIDTH / 16) * math.ceil(HEIGHT / 16)
     stages = {}
-    for name, (bg, bp, bx, bt) in STAGE_BYTES.items():
-        ms = sum(totals[p] for p in STAGE_PARTS.get(name, (name,))) / clocked
+    model = {name: (b, STAGE_PARTS.get(name, (name,))) for name, b in STAGE_BYTES.items()}
+    if trainer.tuning.get("per_tile_depth") == 2:
+        model = {name: v for name, v in model.items() if name not in BINNED_REPLACES}
+        model.update(BINNED_STAGES)
+        order = list(STAGE_BYTES)
+        model = dict(sorted(model.items(), key=lambda kv: order.index("ranges") + 0.5 if kv[0] in BINNED_STAGES
+                            else order.index(kv[0])))
+    for name, ((bg, bp, bx, bt), parts) in model.items():
+        ms = sum(totals[p] for p in parts) / clocked
         nbytes = bg * n + bp * pairs + bx * x + bt * n_tiles
         stages[name] = {"ms": ms, "bytes": nbytes, "GB/s": nbytes / (ms * 1e-3) / 1e9}
         log(f"{name:20s} {ms * 1000:9.1f} us  {stages[name]['GB/s']:8.1f} GB/s algorithmic")

@@ -642,13 +642,19 @@ __global__ __launch_bounds__(kCutThreads) void pair_cut_kernel(const uint32_t* _
 //    and the rasterizer's tile order.
 // Within a tile the pairs land in no particular order; depth_order then sorts each
 // list by (depth key, splat), the order of the reference's lists.
-constexpr int kBinThreads = 256;
-constexpr int kBinWaves = kBinThreads / kWave;
-constexpr uint32_t kBinItems = 8;                        // splats per thread per round
-constexpr uint32_t kBinRound = kBinThreads * kBinItems;  // 2048
+constexpr uint32_t kBinRound = 2048;  // splats a chunk takes at least (below kBinMaxGroups chunks)
 constexpr uint32_t kBinMaxGroups = 512;
 constexpr int kScanColumns = 64, kScanBlocks = 16;       // bin_scan: 1024 threads
 constexpr uint32_t kScanRows = kBinMaxGroups / kScanBlocks;
+// bin_count: 16 waves a chunk, kCountItems splats a thread per round.
+constexpr int kCountThreads = 1024;
+constexpr int kCountWaves = kCountThreads / kWave;
+constexpr uint32_t kCountItems = 2;
+// bin_scatter: 16 waves a chunk (two chunks a CU, 8 waves a SIMD), each wave taking
+// windows of 64 splats, kScatterItems windows' loads in flight.
+constexpr int kScatterThreads = 1024;
+constexpr int kScatterWaves = kScatterThreads / kWave;
+constexpr uint32_t kScatterItems = 2;
 
 // The chunk workgroup b takes. Workgroup b runs on XCD b mod 8, and each tile's run
 // holds the chunks' pairs in chunk order, a few pairs per chunk: with the chunks of
@@ -759,31 +765,31 @@ __device__ inline uint32_t group_sum(uint32_t* slots, uint32_t x) {
   return all;
 }
 
-__global__ __launch_bounds__(kBinThreads) void bin_count_kernel(const uint2* __restrict__ rects,
+__global__ __launch_bounds__(kCountThreads) void bin_count_kernel(const uint2* __restrict__ rects,
                                                                  const uint32_t* __restrict__ counts, uint32_t n,
                                                                  uint32_t chunk, uint32_t width, uint32_t height,
                                                                  uint32_t shift, uint32_t* __restrict__ matrix,
                                                                  uint32_t* __restrict__ partial) {
   __shared__ uint32_t grid[kBinMaxTiles];
-  __shared__ uint32_t slots[kBinWaves];
+  __shared__ uint32_t slots[kCountWaves];
   const int tid = threadIdx.x;
   const BinShape b = bin_shape(width, height, shift);
-  for (uint32_t t = tid; t < b.tiles; t += kBinThreads) grid[t] = 0;
+  for (uint32_t t = tid; t < b.tiles; t += kCountThreads) grid[t] = 0;
   __syncthreads();
   const uint32_t g = chunk_of(blockIdx.x, gridDim.x);
   const uint32_t first = min(n, g * chunk), last = min(first + chunk, n);
   uint32_t sum = 0;
-  for (uint32_t base = first; base < last; base += kBinRound) {
-    uint32_t c[kBinItems];
-    uint2 rect[kBinItems];
+  for (uint32_t base = first; base < last; base += kCountThreads * kCountItems) {
+    uint32_t c[kCountItems];
+    uint2 rect[kCountItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kBinItems; k++) {
-      const uint32_t j = base + k * kBinThreads + tid;
+    for (uint32_t k = 0; k < kCountItems; k++) {
+      const uint32_t j = base + k * kCountThreads + tid;
       c[k] = j < last ? stream_load(counts + j) : 0u;
       rect[k] = j < last ? stream_load(rects + j) : make_uint2(0u, 0u);
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kBinItems; k++) {
+    for (uint32_t k = 0; k < kCountItems; k++) {
       uint32_t r[4];
       const uint32_t kept = splat_bins(rect[k], c[k], b, r);
       if (kept) {
@@ -793,10 +799,10 @@ __global__ __launch_bounds__(kBinThreads) void bin_count_kernel(const uint2* __r
     }
   }
   __syncthreads();
-  grid_prefix<kBinThreads>(grid, b);
+  grid_prefix<kCountThreads>(grid, b);
   uint32_t* row = matrix + static_cast<size_t>(g) * b.tiles;
-  for (uint32_t t = tid; t < b.tiles; t += kBinThreads) row[t] = grid[t];
-  sum = group_sum<kBinThreads>(slots, sum);
+  for (uint32_t t = tid; t < b.tiles; t += kCountThreads) row[t] = grid[t];
+  sum = group_sum<kCountThreads>(slots, sum);
   if (tid == 0) partial[g] = sum;
 }
 
@@ -885,10 +891,10 @@ __global__ __launch_bounds__(kScanColumns * kScanBlocks) void bin_scan_kernel(ui
 
 struct ScatterLds {
   uint32_t cursor[kBinMaxTiles];
-  uint32_t start[kBinWaves][kWave], x0[kBinWaves][kWave], y0[kBinWaves][kWave], w[kBinWaves][kWave],
-      key[kBinWaves][kWave];
+  uint32_t start[kScatterWaves][kWave], x0[kScatterWaves][kWave], y0[kScatterWaves][kWave], w[kScatterWaves][kWave],
+      key[kScatterWaves][kWave];
   uint32_t next[kOrderBuckets];  // workgroup 0: the tile order's buckets
-  uint32_t slots[kBinWaves];
+  uint32_t slots[kScatterWaves];
 };
 
 // Exclusive prefix of x over the workgroup; brackets its use of `slots` with barriers.
@@ -905,15 +911,15 @@ __device__ inline uint32_t bin_exclusive(uint32_t* slots, uint32_t x) {
   __syncthreads();
   uint32_t before = 0;
 #pragma unroll
-  for (int w = 0; w < kBinWaves; w++) before += w < wave ? slots[w] : 0u;
+  for (int w = 0; w < kScatterWaves; w++) before += w < wave ? slots[w] : 0u;
   return before + inclusive - x;
 }
 
-constexpr uint32_t kTilesPerThread = kBinMaxTiles / kBinThreads;  // 32
+constexpr uint32_t kTilesPerThread = kBinMaxTiles / kScatterThreads;  // 8
 
 // Workgroup 0's share: the ranges, the rasterizer's launch order (tiles by falling
 // pair count in steps of kOrderStep, ties in any order) and the cleared item counts.
-// tot[k] is tile tid + k kBinThreads's total; L.cursor holds the tiles' bases.
+// tot[k] is tile tid + k kScatterThreads's total; L.cursor holds the tiles' bases.
 __device__ void write_ranges_and_order(ScatterLds& L, const uint32_t (&tot)[kTilesPerThread], uint32_t tiles,
                                        uint2* __restrict__ ranges, uint32_t* __restrict__ order,
                                        uint32_t* __restrict__ clear) {
@@ -921,18 +927,18 @@ __device__ void write_ranges_and_order(ScatterLds& L, const uint32_t (&tot)[kTil
   if (clear && tid <= static_cast<int>(kPartClasses)) clear[tid] = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kTilesPerThread; k++) {
-    const uint32_t t = tid + k * kBinThreads;
+    const uint32_t t = tid + k * kScatterThreads;
     if (t < tiles) ranges[t] = make_uint2(L.cursor[t], L.cursor[t] + tot[k]);
   }
   if (!order) return;
   auto bucket = [](uint32_t total) { return kOrderBuckets - 1 - min(total / kOrderStep, kOrderBuckets - 1); };
-  for (uint32_t k = tid; k < kOrderBuckets; k += kBinThreads) L.next[k] = 0;
+  for (uint32_t k = tid; k < kOrderBuckets; k += kScatterThreads) L.next[k] = 0;
   __syncthreads();
 #pragma unroll
   for (uint32_t k = 0; k < kTilesPerThread; k++)
-    if (tid + k * kBinThreads < tiles) atomicAdd(&L.next[bucket(tot[k])], 1u);
+    if (tid + k * kScatterThreads < tiles) atomicAdd(&L.next[bucket(tot[k])], 1u);
   __syncthreads();
-  constexpr uint32_t kPer = kOrderBuckets / kBinThreads;
+  constexpr uint32_t kPer = kOrderBuckets / kScatterThreads;
   uint32_t c[kPer], mine = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kPer; k++) {
@@ -948,12 +954,12 @@ __device__ void write_ranges_and_order(ScatterLds& L, const uint32_t (&tot)[kTil
   __syncthreads();
 #pragma unroll
   for (uint32_t k = 0; k < kTilesPerThread; k++) {
-    const uint32_t t = tid + k * kBinThreads;
+    const uint32_t t = tid + k * kScatterThreads;
     if (t < tiles) order[atomicAdd(&L.next[bucket(tot[k])], 1u)] = t;
   }
 }
 
-__global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(const uint2* __restrict__ rects,
+__global__ __launch_bounds__(kScatterThreads) void bin_scatter_kernel(const uint2* __restrict__ rects,
                                                                    const uint32_t* __restrict__ counts,
                                                                    const uint32_t* __restrict__ depth_keys, uint32_t n,
                                                                    uint32_t chunk, uint32_t width, uint32_t height,
@@ -968,22 +974,22 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(const uint2* _
   const BinShape b = bin_shape(width, height, shift);
   const uint32_t g = chunk_of(blockIdx.x, gridDim.x);
   const uint32_t* row = matrix + static_cast<size_t>(g) * b.tiles;
-  // 1. The tile totals and this chunk's prefixes, tile tid + k kBinThreads in slot k,
+  // 1. The tile totals and this chunk's prefixes, tile tid + k kScatterThreads in slot k,
   //    every load in flight at once.
   uint32_t tot[kTilesPerThread], mine_row[kTilesPerThread];
 #pragma unroll
   for (uint32_t k = 0; k < kTilesPerThread; k++) {
-    const uint32_t t = tid + k * kBinThreads;
+    const uint32_t t = tid + k * kScatterThreads;
     tot[k] = t < b.tiles ? stream_load(totals + t) : 0u;
     mine_row[k] = t < b.tiles ? stream_load(row + t) : 0u;
   }
 #pragma unroll
   for (uint32_t k = 0; k < kTilesPerThread; k++)
-    if (tid + k * kBinThreads < b.tiles) L.cursor[tid + k * kBinThreads] = tot[k];
+    if (tid + k * kScatterThreads < b.tiles) L.cursor[tid + k * kScatterThreads] = tot[k];
   __syncthreads();
   // Where each tile's run starts: an exclusive prefix of the totals, a thread taking
   // consecutive tiles in LDS.
-  const uint32_t per = ceil_div(b.tiles, kBinThreads), t0 = min(b.tiles, tid * per), t1 = min(b.tiles, t0 + per);
+  const uint32_t per = ceil_div(b.tiles, kScatterThreads), t0 = min(b.tiles, tid * per), t1 = min(b.tiles, t0 + per);
   uint32_t mine = 0;
   for (uint32_t t = t0; t < t1; t++) mine += L.cursor[t];
   uint32_t run = bin_exclusive(L.slots, mine);  // its barriers order the reads above before the writes below
@@ -1000,25 +1006,25 @@ __global__ __launch_bounds__(kBinThreads) void bin_scatter_kernel(const uint2* _
   // 2. This chunk's slots in each tile: behind the earlier chunks' pairs.
 #pragma unroll
   for (uint32_t k = 0; k < kTilesPerThread; k++)
-    if (tid + k * kBinThreads < b.tiles) L.cursor[tid + k * kBinThreads] += mine_row[k];
+    if (tid + k * kScatterThreads < b.tiles) L.cursor[tid + k * kScatterThreads] += mine_row[k];
   __syncthreads();
-  // 3. Each wave takes windows of 64 consecutive splats (kBinItems of them loaded at
+  // 3. Each wave takes windows of 64 consecutive splats (kScatterItems of them loaded at
   //    once) and deals their pairs out to its lanes: pair q of a window belongs to
   //    the last splat whose first pair is at or before q.
   const uint32_t first = min(n, g * chunk), last = min(first + chunk, n);
-  for (uint32_t base = first + wave * kWave; base < last; base += kBinWaves * kWave * kBinItems) {
-    uint32_t c[kBinItems], key[kBinItems];
-    uint2 rect[kBinItems];
+  for (uint32_t base = first + wave * kWave; base < last; base += kScatterWaves * kWave * kScatterItems) {
+    uint32_t c[kScatterItems], key[kScatterItems];
+    uint2 rect[kScatterItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kBinItems; k++) {
-      const uint32_t j = base + k * kBinWaves * kWave + lane;
+    for (uint32_t k = 0; k < kScatterItems; k++) {
+      const uint32_t j = base + k * kScatterWaves * kWave + lane;
       c[k] = j < last ? stream_load(counts + j) : 0u;
       rect[k] = j < last ? stream_load(rects + j) : make_uint2(0u, 0u);
       key[k] = j < last ? stream_load(depth_keys + j) : 0u;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kBinItems; k++) {
-      const uint32_t window = base + k * kBinWaves * kWave;
+    for (uint32_t k = 0; k < kScatterItems; k++) {
+      const uint32_t window = base + k * kScatterWaves * kWave;
       if (window >= last) break;
       uint32_t r[4];
       const uint32_t kept = splat_bins(rect[k], c[k], b, r);
@@ -1084,7 +1090,7 @@ void bin_count(hipStream_t s, const uint32_t* rects, const uint32_t* depth_keys,
   if (tiles == 0 || tiles > kBinMaxTiles) throw Error(-2, "binning by counts takes 1 to 8192 tiles");
   const BinLayout l = bin_layout(n);
   if (l.groups > ws.groups || tiles > ws.tiles) throw Error(-2, "bin workspace too small");
-  bin_count_kernel<<<l.groups, kBinThreads, 0, s>>>(reinterpret_cast<const uint2*>(rects), tile_counts, n, l.chunk,
+  bin_count_kernel<<<l.groups, kCountThreads, 0, s>>>(reinterpret_cast<const uint2*>(rects), tile_counts, n, l.chunk,
                                                     width, height, shift, ws.matrix, ws.partial);
   DIO_LAUNCHED("bin_count");
   bin_cut_kernel<<<1, kCutThreads, 0, s>>>(reinterpret_cast<const uint2*>(rects), depth_keys, tile_counts, n, l.chunk,
@@ -1100,7 +1106,7 @@ void bin_scatter(hipStream_t s, const uint32_t* rects, const uint32_t* tile_coun
                  uint32_t n, uint32_t width, uint32_t height, uint32_t shift, const BinWorkspace& ws, uint32_t* keyed,
                  uint32_t* ranges, uint32_t* order, uint32_t* clear) {
   const BinLayout l = bin_layout(n);
-  bin_scatter_kernel<<<l.groups, kBinThreads, 0, s>>>(reinterpret_cast<const uint2*>(rects), tile_counts, depth_keys,
+  bin_scatter_kernel<<<l.groups, kScatterThreads, 0, s>>>(reinterpret_cast<const uint2*>(rects), tile_counts, depth_keys,
                                                       n, l.chunk, width, height, shift, ws.matrix, ws.totals,
                                                       reinterpret_cast<uint2*>(keyed), reinterpret_cast<uint2*>(ranges),
                                                       order, clear);
