@@ -52,10 +52,31 @@ __device__ inline HaloPixel halo_pixel(int e, int x0, int y0, uint32_t width, ui
   return {static_cast<size_t>(cy) * width + cx, inside};
 }
 
+// The horizontal blur's outputs (kSpan rows x kT columns) over two rounds of the 256
+// threads: wave w's lane l takes row (w mod 8) + 8 (l / 16), column l mod 16, so the
+// four rows a wave reads lie 8 x kSpan = 208 floats apart, 16 banks further round
+// each time: a wave's 64 reads of one tap hit 64 different banks of the kSpan-pitch
+// halo rows (with consecutive rows, 26 floats apart, two lanes shared a bank). Rows
+// past kSpan (26..31) are idle lanes.
+struct BlurSlot {
+  int row, col;
+  bool valid;
+};
+__device__ inline BlurSlot blur_slot(int e) {
+  const int w = e / 64, l = e % 64;
+  const int row = (w & 7) + 8 * (l / 16);
+  return {row, l % 16, row < kSpan};
+}
+constexpr int kBlurSlots = 512;
+// The horizontal blur's outputs stay rows of kT: the vertical blur reads four
+// consecutive rows a wave (64 banks, no conflict), 55 reads a pixel against the
+// horizontal blur's 5 writes an item, whose four rows 8 apart share banks.
+__device__ inline int blur_index(int row, int col) { return row * kT + col; }
+
 struct PartialsShared {
   float x[3][kHaloPad];  // per channel, row-major kSpan x kSpan
   float y[3][kHaloPad];
-  float h[5][kSpan][kT];
+  float h[5][kSpan * kT];
 };
 
 // Pass 1 over the 16x16 tile (tile_x, tile_y) of a width x height image.
@@ -97,8 +118,10 @@ __device__ inline void ssim_partials_tile(uint32_t tile_x, uint32_t tile_y, uint
 #pragma unroll 1
   for (int ch = 0; ch < 3; ch++) {
     // Horizontal blur of x, y, x², y², xy for every halo row and the tile's columns.
-    for (int e = tid; e < kSpan * kT; e += 256) {
-      const int col = e % kT, row = e / kT;
+    for (int e = tid; e < kBlurSlots; e += 256) {
+      const BlurSlot b = blur_slot(e);
+      if (!b.valid) continue;
+      const int col = b.col, row = b.row;
       float m[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int k = -kR; k <= kR; k++) {
@@ -111,7 +134,7 @@ __device__ inline void ssim_partials_tile(uint32_t tile_x, uint32_t tile_y, uint
         m[4] += w * (x * y);
       }
 #pragma unroll
-      for (int q = 0; q < 5; q++) s_h[q][row][col] = m[q];
+      for (int q = 0; q < 5; q++) s_h[q][blur_index(row, col)] = m[q];
     }
     __syncthreads();
     float mu[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
@@ -119,7 +142,7 @@ __device__ inline void ssim_partials_tile(uint32_t tile_x, uint32_t tile_y, uint
     for (int k = -kR; k <= kR; k++) {
       const float w = win.w[k < 0 ? -k : k];
 #pragma unroll
-      for (int q = 0; q < 5; q++) mu[q] += w * s_h[q][ly + kR + k][lx];
+      for (int q = 0; q < 5; q++) mu[q] += w * s_h[q][blur_index(ly + kR + k, lx)];
     }
     const float mx = mu[0], my = mu[1];
     const float vx = mu[2] - mx * mx, vy = mu[3] - my * my, cxy = mu[4] - mx * my;
@@ -146,7 +169,7 @@ __device__ inline void ssim_partials_tile(uint32_t tile_x, uint32_t tile_y, uint
 
 struct GradientShared {
   float p[9][kHaloPad];  // (∂μ, ∂E[x²], ∂E[xy]) x channel, row-major kSpan x kSpan
-  float h[3][kSpan][kT];
+  float h[3][kSpan * kT];
 };
 
 // Pass 2 over one tile: dL/dpixel (scaled by `scale` = 1 / (3 pixels)) and each
@@ -196,8 +219,10 @@ __device__ inline void loss_gradient_tile(uint32_t tile_x, uint32_t tile_y, uint
   __syncthreads();
   float grad[3], share = 0.f;
   for (int ch = 0; ch < 3; ch++) {
-    for (int e = tid; e < kSpan * kT; e += 256) {
-      const int col = e % kT, row = e / kT;
+    for (int e = tid; e < kBlurSlots; e += 256) {
+      const BlurSlot bs = blur_slot(e);
+      if (!bs.valid) continue;
+      const int col = bs.col, row = bs.row;
       float m[3] = {0.f, 0.f, 0.f};
 #pragma unroll
       for (int k = -kR; k <= kR; k++) {
@@ -206,7 +231,7 @@ __device__ inline void loss_gradient_tile(uint32_t tile_x, uint32_t tile_y, uint
         for (int q = 0; q < 3; q++) m[q] += w * s_p[q * 3 + ch][row * kSpan + col + kR + k];
       }
 #pragma unroll
-      for (int q = 0; q < 3; q++) s_h[q][row][col] = m[q];
+      for (int q = 0; q < 3; q++) s_h[q][blur_index(row, col)] = m[q];
     }
     __syncthreads();
     float b[3] = {0.f, 0.f, 0.f};
@@ -214,7 +239,7 @@ __device__ inline void loss_gradient_tile(uint32_t tile_x, uint32_t tile_y, uint
     for (int k = -kR; k <= kR; k++) {
       const float w = win.w[k < 0 ? -k : k];
 #pragma unroll
-      for (int q = 0; q < 3; q++) b[q] += w * s_h[q][ly + kR + k][lx];
+      for (int q = 0; q < 3; q++) b[q] += w * s_h[q][blur_index(ly + kR + k, lx)];
     }
     const float x = ch == 0 ? x4.x : (ch == 1 ? x4.y : x4.z);
     const float y = unorm8(t8, ch);
