@@ -466,6 +466,8 @@ struct dio_tile_rasterizer {
   bool sorted_in_alt = false;
   dio_buf* ranges = nullptr;
   dio_buf* tile_order = nullptr;  // the rasterizer's launch order, heaviest tiles first
+  dio_buf* order_queue = nullptr;  // per-tile depth order: the lists left for radix passes (2 tiles + 2 words)
+  uint32_t order_parity = 0;       // which of its two queues the next depth order fills
   uint32_t ranges_tiles = 0;
   dio_buf *pixel_color = nullptr, *pixel_last = nullptr;
   uint32_t pixel_count = 0;
@@ -526,6 +528,7 @@ void destroy_tiles(dio_tile_rasterizer* t) {
   free_pairs(t);
   release(t->ranges);
   release(t->tile_order);
+  release(t->order_queue);
   release(t->bin_matrix);
   release(t->bin_partial);
   release(t->bin_totals);
@@ -575,8 +578,11 @@ void ensure_frame_buffers(dio_tile_rasterizer* t, uint32_t tiles, uint32_t pixel
   if (!t->ranges || t->ranges_tiles < tiles) {
     release(t->ranges);
     release(t->tile_order);
+    release(t->order_queue);
     t->ranges = alloc(t->ctx, static_cast<size_t>(tiles) * 8, true);
     t->tile_order = alloc(t->ctx, static_cast<size_t>(tiles) * 4, true);
+    t->order_queue = alloc(t->ctx, (2 * static_cast<size_t>(tiles) + 2) * 4, true);
+    t->order_parity = 0;
     t->ranges_tiles = tiles;
   }
   if (!t->pixel_color || t->pixel_count < pixels) {
@@ -703,7 +709,8 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
     t->sorted_in_alt = false;
     sorted_splats = t->pair_splats;
     dio::depth_order(s, dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), tiles,
-                     dev<uint32_t>(t->pair_splats), keyed + 2 * static_cast<size_t>(t->capacity), false, keyed);
+                     dev<uint32_t>(t->pair_splats), keyed + 2 * static_cast<size_t>(t->capacity), false,
+                     dev<uint32_t>(t->order_queue), t->order_parity++, keyed);
     mark(kMarkListOrder);
   } else {
     // Depth order of the splats (culled ones last), with their tile counts in that
@@ -748,7 +755,8 @@ void encode_tiles(dio_tile_rasterizer* t, dio_buf* target, uint32_t w, uint32_t 
     if (per_tile)
       dio::depth_order(s, dev<uint32_t>(t->depth_keys), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order),
                        tiles, dev<uint32_t>(sorted_splats),
-                       dev<uint32_t>(t->sorted_in_alt ? t->pair_splats : t->alt_splats), true);
+                       dev<uint32_t>(t->sorted_in_alt ? t->pair_splats : t->alt_splats), true,
+                       dev<uint32_t>(t->order_queue), t->order_parity++);
     mark(kMarkListOrder);
   }
   dio::rasterize(s, splats, dev<uint32_t>(sorted_splats), dev<uint32_t>(t->ranges), dev<uint32_t>(t->tile_order), w,
@@ -2434,7 +2442,7 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
                            dev<uint32_t>(tr->tile_order), nullptr);
           dio::depth_order(s, dev<uint32_t>(tr->depth_keys), dev<uint32_t>(tr->ranges), dev<uint32_t>(tr->tile_order),
                            tiles, dev<uint32_t>(tr->pair_splats), keyed + 2 * static_cast<size_t>(tr->capacity), false,
-                           keyed);
+                           dev<uint32_t>(tr->order_queue), tr->order_parity++, keyed);
           return;
         }
         if (sorts && !per_tile)
@@ -2462,7 +2470,8 @@ extern "C" int dio_trainer_time_stage(dio_trainer* t, const char* stage, uint32_
                            tr->capacity, tiles, dev<uint32_t>(tr->ranges), dev<uint32_t>(tr->tile_order));
           dio::depth_order(s, dev<uint32_t>(tr->depth_keys), dev<uint32_t>(tr->ranges), dev<uint32_t>(tr->tile_order),
                            tiles, dev<uint32_t>(in_alt ? tr->alt_splats : tr->pair_splats),
-                           dev<uint32_t>(in_alt ? tr->pair_splats : tr->alt_splats), true);
+                           dev<uint32_t>(in_alt ? tr->pair_splats : tr->alt_splats), true,
+                           dev<uint32_t>(tr->order_queue), tr->order_parity++);
         }
       };
       hipEvent_t e0, e1, e2;

@@ -29,17 +29,22 @@ constexpr int kWaves = kThreads / kWave;
 constexpr int kRows = 8;
 constexpr uint32_t kChunk = kThreads * kRows;  // 2048
 constexpr int kRadix = 256;
-constexpr uint32_t kBuckets = 2048;
-constexpr uint32_t kBucketsPerThread = kBuckets / kThreads;  // 8
-constexpr uint32_t kBucketLimit = 16;
+constexpr uint32_t kBuckets = 2048;  // for a list of kChunk pairs; in proportion for other sizes
+constexpr uint32_t kBucketLimit = 32;
+// The long-list kernel sorts lists of up to kLongChunk pairs in LDS.
+constexpr int kLongRows = 16;
+constexpr uint32_t kLongChunk = kThreads * kLongRows;  // 4096
+template <int R>
+constexpr uint32_t buckets_for = kBuckets * R / kRows;
 
+template <int R>
 struct OrderLds {
-  uint32_t key[kChunk];
-  uint32_t value[kChunk];
+  uint32_t key[kThreads * R];
+  uint32_t value[kThreads * R];
   uint32_t wave_count[kWaves][kRadix];  // per wave and digit: count, then where the wave's run starts
   uint32_t chunk_start[kRadix];         // where each digit's run starts in the chunk
   uint32_t run_start[kRadix];           // long lists: where each digit's run continues in the output
-  uint32_t bucket[kBuckets];             // short lists: pairs per bucket, then where each bucket starts
+  uint32_t bucket[buckets_for<R>];       // in-LDS lists: pairs per bucket, then where each bucket starts
   uint32_t wave_totals[kWaves];
   uint32_t reduce[2][kWaves];
   uint32_t crowded;  // short lists: set when a bucket holds more than kBucketLimit pairs
@@ -51,7 +56,8 @@ __device__ inline uint32_t element(uint32_t rows, int r) {
 }
 
 // Exclusive prefix over the 256 threads of `x` (thread t holds digit t's value).
-__device__ inline uint32_t block_exclusive(OrderLds& L, uint32_t x) {
+template <int R>
+__device__ inline uint32_t block_exclusive(OrderLds<R>& L, uint32_t x) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   uint32_t inclusive = x;
 #pragma unroll
@@ -71,16 +77,17 @@ __device__ inline uint32_t block_exclusive(OrderLds& L, uint32_t x) {
 // are valid, laid out in `rows` rows per wave. Returns each key's position in the
 // chunk in pos[r]; leaves in L.chunk_start where each digit's run starts in the
 // chunk and returns the chunk's count of digit threadIdx.x.
-__device__ inline uint32_t rank_chunk(OrderLds& L, const uint32_t (&key)[kRows], uint32_t n, uint32_t rows,
-                                      uint32_t shift, uint32_t (&pos)[kRows]) {
+template <int R>
+__device__ inline uint32_t rank_chunk(OrderLds<R>& L, const uint32_t (&key)[R], uint32_t n, uint32_t rows,
+                                      uint32_t shift, uint32_t (&pos)[R]) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const uint64_t lower_lanes = (1ull << lane) - 1;
 #pragma unroll
   for (int w = 0; w < kWaves; w++) L.wave_count[w][tid] = 0;
   __syncthreads();
-  uint32_t rank[kRows], digit[kRows];
+  uint32_t rank[R], digit[R];
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     if (static_cast<uint32_t>(r) >= rows) break;
     const bool valid = element(rows, r) < n;
     const uint32_t d = (key[r] >> shift) & (kRadix - 1);
@@ -116,7 +123,7 @@ __device__ inline uint32_t rank_chunk(OrderLds& L, const uint32_t (&key)[kRows],
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     if (static_cast<uint32_t>(r) >= rows) break;
     pos[r] = L.wave_count[wave][digit[r]] + rank[r];
   }
@@ -126,8 +133,8 @@ __device__ inline uint32_t rank_chunk(OrderLds& L, const uint32_t (&key)[kRows],
 enum class Reduce { kOr, kMin, kMax };
 
 // x combined over the workgroup.
-template <Reduce kOp>
-__device__ inline uint32_t block_reduce(OrderLds& L, uint32_t x) {
+template <Reduce kOp, int R>
+__device__ inline uint32_t block_reduce(OrderLds<R>& L, uint32_t x) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   auto op = [](uint32_t a, uint32_t b) { return kOp == Reduce::kOr ? a | b : kOp == Reduce::kMin ? min(a, b) : max(a, b); };
 #pragma unroll
@@ -142,7 +149,8 @@ __device__ inline uint32_t block_reduce(OrderLds& L, uint32_t x) {
 }
 
 // The workgroup's minimum of `low` and maximum of `high`, in one exchange.
-__device__ inline void block_min_max(OrderLds& L, uint32_t& low, uint32_t& high) {
+template <int R>
+__device__ inline void block_min_max(OrderLds<R>& L, uint32_t& low, uint32_t& high) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) {
@@ -162,15 +170,16 @@ __device__ inline void block_min_max(OrderLds& L, uint32_t& low, uint32_t& high)
   __syncthreads();  // L.reduce is written again by the next call
 }
 
-// Loads chunk [first, first + n) of a list (n <= kChunk) in the (wave, row, lane)
+// Loads chunk [first, first + n) of a list (n <= kThreads x R) in the (wave, row, lane)
 // layout with each pair's depth key: from `keyed` ((key, splat) pairs) when the
 // pairs carry their keys, else from `list` (splats) and depth_keys[splat].
+template <int R>
 __device__ inline void load_chunk(const uint32_t* __restrict__ list, const uint2* __restrict__ keyed,
                                   const uint32_t* __restrict__ depth_keys, uint32_t first, uint32_t n, uint32_t rows,
-                                  uint32_t (&key)[kRows], uint32_t (&value)[kRows]) {
+                                  uint32_t (&key)[R], uint32_t (&value)[R]) {
   if (keyed) {
 #pragma unroll
-    for (int r = 0; r < kRows; r++) {
+    for (int r = 0; r < R; r++) {
       const uint32_t e = element(rows, r);
       const uint2 p = static_cast<uint32_t>(r) < rows && e < n ? keyed[first + e] : make_uint2(0u, 0u);
       key[r] = p.x;
@@ -179,36 +188,37 @@ __device__ inline void load_chunk(const uint32_t* __restrict__ list, const uint2
     return;
   }
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     const uint32_t e = element(rows, r);
     value[r] = static_cast<uint32_t>(r) < rows && e < n ? list[first + e] : 0u;
   }
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     const uint32_t e = element(rows, r);
     key[r] = static_cast<uint32_t>(r) < rows && e < n ? depth_keys[value[r]] : 0u;
   }
 }
 
-// Radix passes in LDS on a list of at most kChunk pairs held in registers, over the
+// Radix passes in LDS on a list of at most kThreads x R pairs held in registers, over the
 // 8-bit digits where `value_diff` has bits (splat indices, first) and then where
 // `diff` has bits (depth keys): stable passes, so the list ends in (key, splat)
 // order whatever order it arrived in. Writes the list back in place.
-__device__ void radix_in_lds(OrderLds& L, uint32_t* __restrict__ list, uint32_t lo, uint32_t n, uint32_t rows,
-                             uint32_t value_diff, uint32_t diff, uint32_t (&key)[kRows], uint32_t (&value)[kRows]) {
-  uint32_t pos[kRows];
+template <int R>
+__device__ void radix_in_lds(OrderLds<R>& L, uint32_t* __restrict__ list, uint32_t lo, uint32_t n, uint32_t rows,
+                             uint32_t value_diff, uint32_t diff, uint32_t (&key)[R], uint32_t (&value)[R]) {
+  uint32_t pos[R];
   // One LDS copy serves every pass: a pass reads its keys back before the next
   // pass's first barrier, and scatters only after it.
   for (uint32_t p = 0; p < 8; p++) {
     const bool by_value = p < 4;
     const uint32_t shift = (p & 3u) * 8u;
     if ((((by_value ? value_diff : diff) >> shift) & (kRadix - 1)) == 0) continue;
-    uint32_t digits[kRows];
+    uint32_t digits[R];
 #pragma unroll
-    for (int r = 0; r < kRows; r++) digits[r] = by_value ? value[r] : key[r];
+    for (int r = 0; r < R; r++) digits[r] = by_value ? value[r] : key[r];
     rank_chunk(L, digits, n, rows, shift, pos);
 #pragma unroll
-    for (int r = 0; r < kRows; r++) {
+    for (int r = 0; r < R; r++) {
       if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
         L.key[pos[r]] = key[r];
         L.value[pos[r]] = value[r];
@@ -216,7 +226,7 @@ __device__ void radix_in_lds(OrderLds& L, uint32_t* __restrict__ list, uint32_t 
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kRows; r++) {
+    for (int r = 0; r < R; r++) {
       const uint32_t e = element(rows, r);
       if (static_cast<uint32_t>(r) < rows && e < n) {
         key[r] = L.key[e];
@@ -225,7 +235,7 @@ __device__ void radix_in_lds(OrderLds& L, uint32_t* __restrict__ list, uint32_t 
     }
   }
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     const uint32_t e = element(rows, r);
     if (static_cast<uint32_t>(r) < rows && e < n) list[lo + e] = value[r];
   }
@@ -233,12 +243,13 @@ __device__ void radix_in_lds(OrderLds& L, uint32_t* __restrict__ list, uint32_t 
 
 // The splat-index bits that differ somewhere in a list held in registers (0 for a
 // list that arrives in splat order, which needs no passes over them).
-__device__ inline uint32_t value_bits(OrderLds& L, bool ordered, uint32_t first, uint32_t n, uint32_t rows,
-                                      const uint32_t (&value)[kRows]) {
+template <int R>
+__device__ inline uint32_t value_bits(OrderLds<R>& L, bool ordered, uint32_t first, uint32_t n, uint32_t rows,
+                                      const uint32_t (&value)[R]) {
   if (ordered) return 0u;
   uint32_t diff = 0;
 #pragma unroll
-  for (int r = 0; r < kRows; r++)
+  for (int r = 0; r < R; r++)
     if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) diff |= value[r] ^ first;
   return block_reduce<Reduce::kOr>(L, diff);
 }
@@ -249,79 +260,70 @@ __device__ inline bool before(uint32_t k, uint32_t v, uint32_t k2, uint32_t v2) 
   return k < k2 || (k == k2 && v < v2);
 }
 
-// A list of at most kChunk pairs, sorted in LDS and written back in place. The
-// keys go to kBuckets buckets by a map that never decreases with the key (a float
+// A list of at most kThreads x R pairs, sorted in LDS and written back in place. The
+// keys go to buckets_for<R> buckets by a map that never decreases with the key (a float
 // scale of key - min over the list's key range); each pair's place inside its
 // bucket, whose pairs the LDS atomics put in no particular order, is the number of
 // the bucket's pairs before it in (key, splat) order, the stable depth order of a
-// list in splat order, whatever order the list arrives in. A bucket of more than
-// kBucketLimit pairs (depths packed closer than the buckets resolve, or many equal
-// depths) sends the list through radix passes instead, which also pass over the
-// splat indices unless the list is `ordered` (arrives in splat order). The list
-// comes from `keyed` when given (never `ordered` then) and always leaves in `list`.
-__device__ void order_in_lds(OrderLds& L, const uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ list,
+// list in splat order, whatever order the list arrives in. Returns false, writing
+// nothing, for a list this cannot order: a bucket of more than kBucketLimit pairs
+// (depths packed closer than the buckets resolve, or many equal depths) or a list
+// of one depth that does not arrive in splat order; order_by_digits takes those.
+// The list comes from `keyed` when given (never `ordered` then) and always leaves
+// in `list`.
+template <int R>
+__device__ bool order_in_lds(OrderLds<R>& L, const uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ list,
                              const uint2* __restrict__ keyed, uint32_t lo, uint32_t n, bool ordered) {
+  constexpr uint32_t kPer = buckets_for<R> / kThreads;  // buckets a thread counts
   const int tid = threadIdx.x;
   const uint32_t rows = ceil_div(n, kThreads);
-  uint32_t key[kRows], value[kRows];
+  uint32_t key[R], value[R];
   load_chunk(list, keyed, depth_keys, lo, n, rows, key, value);
   uint32_t low = ~0u, high = 0u;
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
       low = min(low, key[r]);
       high = max(high, key[r]);
     }
   }
 #pragma unroll
-  for (uint32_t b = 0; b < kBucketsPerThread; b++) L.bucket[tid * kBucketsPerThread + b] = 0;
+  for (uint32_t b = 0; b < kPer; b++) L.bucket[tid * kPer + b] = 0;
   if (tid == 0) L.crowded = 0;
   block_min_max(L, low, high);  // its barriers also order the clearing above
-  const uint32_t first = keyed ? keyed[lo].y : list[lo];  // one splat of the list: the reference its bits differ from
-  if (low == high) {  // every key the same: splat order is the order
-    if (!ordered) radix_in_lds(L, list, lo, n, rows, value_bits(L, false, first, n, rows, value), 0u, key, value);
-    return;
-  }
-  const float scale = static_cast<float>(kBuckets) / (static_cast<float>(high - low) + 1.f);
-  uint32_t bucket[kRows], slot[kRows];
+  if (low == high) return ordered;  // every key the same: splat order is the order
+  const float scale = static_cast<float>(buckets_for<R>) / (static_cast<float>(high - low) + 1.f);
+  uint32_t bucket[R], slot[R];
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
-      bucket[r] = min(kBuckets - 1, static_cast<uint32_t>(static_cast<float>(key[r] - low) * scale));
+      bucket[r] = min(buckets_for<R> - 1, static_cast<uint32_t>(static_cast<float>(key[r] - low) * scale));
       slot[r] = atomicAdd(&L.bucket[bucket[r]], 1u);
     }
   }
   __syncthreads();
-  uint32_t count[kBucketsPerThread], mine = 0, largest = 0;
+  uint32_t count[kPer], mine = 0, largest = 0;
 #pragma unroll
-  for (uint32_t b = 0; b < kBucketsPerThread; b++) {
-    count[b] = L.bucket[tid * kBucketsPerThread + b];
+  for (uint32_t b = 0; b < kPer; b++) {
+    count[b] = L.bucket[tid * kPer + b];
     mine += count[b];
     largest = max(largest, count[b]);
   }
   if (largest > kBucketLimit) L.crowded = 1;
   uint32_t run = block_exclusive(L, mine);  // its barrier also publishes L.crowded
-  if (L.crowded) {
-    uint32_t diff = 0;  // the key bits that differ somewhere in the list
+  if (L.crowded) return false;
 #pragma unroll
-    for (int r = 0; r < kRows; r++)
-      if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) diff |= key[r] ^ low;
-    diff = block_reduce<Reduce::kOr>(L, diff);
-    radix_in_lds(L, list, lo, n, rows, value_bits(L, ordered, first, n, rows, value), diff, key, value);
-    return;
-  }
-#pragma unroll
-  for (uint32_t b = 0; b < kBucketsPerThread; b++) {
-    L.bucket[tid * kBucketsPerThread + b] = run;  // each thread rewrites only its own buckets
+  for (uint32_t b = 0; b < kPer; b++) {
+    L.bucket[tid * kPer + b] = run;  // each thread rewrites only its own buckets
     run += count[b];
   }
   __syncthreads();
-  uint32_t start[kRows], end[kRows];
+  uint32_t start[R], end[R];
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
       start[r] = L.bucket[bucket[r]];
-      end[r] = bucket[r] + 1 < kBuckets ? L.bucket[bucket[r] + 1] : n;
+      end[r] = bucket[r] + 1 < buckets_for<R> ? L.bucket[bucket[r] + 1] : n;
       L.key[start[r] + slot[r]] = key[r];
       L.value[start[r] + slot[r]] = value[r];
     }
@@ -330,7 +332,7 @@ __device__ void order_in_lds(OrderLds& L, const uint32_t* __restrict__ depth_key
   // Each pair's place in its bucket: how many of the bucket's pairs come before it.
   // The bucket starts are no longer read, so their words take the ordered list.
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
       uint32_t place = start[r];
       for (uint32_t j = start[r]; j < end[r]; j++) place += before(L.key[j], L.value[j], key[r], value[r]) ? 1u : 0u;
@@ -339,28 +341,50 @@ __device__ void order_in_lds(OrderLds& L, const uint32_t* __restrict__ depth_key
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kRows; r++) {
+  for (int r = 0; r < R; r++) {
     const uint32_t e = element(rows, r);
     if (static_cast<uint32_t>(r) < rows && e < n) list[lo + e] = L.bucket[e];
   }
+  return true;
 }
 
-// A list longer than kChunk: the same passes streamed through the list and a
+// A list of at most kThreads x R pairs that order_in_lds left: radix passes in LDS over
+// the splat-index digits that differ in it (none if `ordered`), then the depth-key
+// digits that differ.
+template <int R>
+__device__ void order_by_digits(OrderLds<R>& L, const uint32_t* __restrict__ depth_keys, uint32_t* __restrict__ list,
+                                const uint2* __restrict__ keyed, uint32_t lo, uint32_t n, bool ordered) {
+  const uint32_t rows = ceil_div(n, kThreads);
+  uint32_t key[R], value[R];
+  load_chunk(list, keyed, depth_keys, lo, n, rows, key, value);
+  // One pair of the list: the reference its bits differ from.
+  const uint32_t first = keyed ? keyed[lo].y : list[lo];
+  const uint32_t key0 = keyed ? keyed[lo].x : depth_keys[first];
+  uint32_t diff = 0;
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) diff |= key[r] ^ key0;
+  diff = block_reduce<Reduce::kOr>(L, diff);
+  radix_in_lds(L, list, lo, n, rows, value_bits(L, ordered, first, n, rows, value), diff, key, value);
+}
+
+// A list longer than kThreads x R: the same passes streamed through the list and a
 // scratch list (the same pair positions of another pair-sized buffer), over the
 // splat indices first unless the list is `ordered`. kKeyed: the list is `keyed`
 // ((key, splat) pairs, ping-ponged with an 8-byte scratch) and its splats leave in
 // `list`; else it is `list` itself (splats, ping-ponged with a 4-byte scratch).
-template <bool kKeyed>
-__device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_keys, uint32_t* list, uint2* keyed,
+template <int R, bool kKeyed>
+__device__ void order_streamed(OrderLds<R>& L, const uint32_t* __restrict__ depth_keys, uint32_t* list, uint2* keyed,
                                void* scratch, uint32_t lo, uint32_t n, bool ordered) {
   using Elem = typename std::conditional<kKeyed, uint2, uint32_t>::type;
+  constexpr uint32_t kN = kThreads * R;  // pairs a chunk
   const int tid = threadIdx.x;
-  uint32_t key[kRows], value[kRows], pos[kRows];
+  uint32_t key[R], value[R], pos[R];
   Elem* src = kKeyed ? reinterpret_cast<Elem*>(keyed) : reinterpret_cast<Elem*>(list);
   Elem* dst = static_cast<Elem*>(scratch);
   auto load = [&](const Elem* from, uint32_t first, uint32_t m) {
     load_chunk(kKeyed ? nullptr : reinterpret_cast<const uint32_t*>(from),
-               kKeyed ? reinterpret_cast<const uint2*>(from) : nullptr, depth_keys, first, m, kRows, key, value);
+               kKeyed ? reinterpret_cast<const uint2*>(from) : nullptr, depth_keys, first, m, R, key, value);
   };
   uint32_t value0, key0;
   if (kKeyed) {
@@ -372,12 +396,12 @@ __device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_k
     key0 = depth_keys[value0];
   }
   uint32_t diff = 0, value_diff = 0;
-  for (uint32_t c = 0; c < n; c += kChunk) {
-    const uint32_t m = min(kChunk, n - c);
+  for (uint32_t c = 0; c < n; c += kN) {
+    const uint32_t m = min(kN, n - c);
     load(src, lo + c, m);
 #pragma unroll
-    for (int r = 0; r < kRows; r++) {
-      if (element(kRows, r) < m) {
+    for (int r = 0; r < R; r++) {
+      if (element(R, r) < m) {
         diff |= key[r] ^ key0;
         value_diff |= value[r] ^ value0;
       }
@@ -392,12 +416,12 @@ __device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_k
     // Sweep 1: where each digit's run starts in the output.
     L.run_start[tid] = 0;
     __syncthreads();
-    for (uint32_t c = 0; c < n; c += kChunk) {
-      const uint32_t m = min(kChunk, n - c);
+    for (uint32_t c = 0; c < n; c += kN) {
+      const uint32_t m = min(kN, n - c);
       load(src, lo + c, m);
 #pragma unroll
-      for (int r = 0; r < kRows; r++)
-        if (element(kRows, r) < m)
+      for (int r = 0; r < R; r++)
+        if (element(R, r) < m)
           atomicAdd(&L.run_start[((by_value ? value[r] : key[r]) >> shift) & (kRadix - 1)], 1u);
     }
     __syncthreads();
@@ -405,16 +429,16 @@ __device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_k
     __syncthreads();
     L.run_start[tid] = start;
     // Sweep 2: chunks in order, each digit's pairs behind those placed before.
-    for (uint32_t c = 0; c < n; c += kChunk) {
-      const uint32_t m = min(kChunk, n - c);
+    for (uint32_t c = 0; c < n; c += kN) {
+      const uint32_t m = min(kN, n - c);
       load(src, lo + c, m);
-      uint32_t digits[kRows];
+      uint32_t digits[R];
 #pragma unroll
-      for (int r = 0; r < kRows; r++) digits[r] = by_value ? value[r] : key[r];
-      const uint32_t total = rank_chunk(L, digits, m, kRows, shift, pos);
+      for (int r = 0; r < R; r++) digits[r] = by_value ? value[r] : key[r];
+      const uint32_t total = rank_chunk(L, digits, m, R, shift, pos);
 #pragma unroll
-      for (int r = 0; r < kRows; r++) {
-        if (element(kRows, r) < m) {
+      for (int r = 0; r < R; r++) {
+        if (element(R, r) < m) {
           const uint32_t d = (digits[r] >> shift) & (kRadix - 1);
           const uint32_t at = lo + L.run_start[d] + pos[r] - L.chunk_start[d];
           if constexpr (kKeyed)
@@ -444,27 +468,74 @@ __device__ void order_streamed(OrderLds& L, const uint32_t* __restrict__ depth_k
 // `ordered`: every list arrives in splat order (pairs written in splat order and
 // sorted stably by tile); otherwise the lists are in any order (binned by count).
 // keyed (nullable): the lists as (depth key, splat) pairs, at the same positions,
-// sorted into `list`; `scratch` then holds 8 bytes a pair, else 4.
+// sorted into `list`. One workgroup a tile sorts the lists of at most kChunk pairs
+// in LDS and queues those its buckets cannot order (`queued` counts the tiles in
+// `entries`); depth_order_long_kernel takes the queued lists and the longer ones,
+// whose registers would otherwise limit this kernel's occupancy.
 __global__ __launch_bounds__(kThreads) void depth_order_kernel(const uint32_t* __restrict__ depth_keys,
                                                                 const uint2* __restrict__ ranges,
                                                                 const uint32_t* __restrict__ tile_order,
-                                                                uint32_t* list, uint2* keyed, void* scratch,
-                                                                bool ordered) {
-  __shared__ OrderLds L;
+                                                                uint32_t* list, const uint2* keyed, bool ordered,
+                                                                uint32_t* __restrict__ queued,
+                                                                uint32_t* __restrict__ entries) {
+  __shared__ OrderLds<kRows> L;
   const uint32_t tile = tile_order ? tile_order[blockIdx.x] : blockIdx.x;
   const uint2 range = ranges[tile];
   const uint32_t n = range.y - range.x;
-  if (n == 0) return;
+  if (n == 0 || n > kChunk) return;
   if (n == 1) {
     if (keyed && threadIdx.x == 0) list[range.x] = keyed[range.x].y;
     return;
   }
-  if (n <= kChunk)
-    order_in_lds(L, depth_keys, list, keyed, range.x, n, ordered);
-  else if (keyed)
-    order_streamed<true>(L, depth_keys, list, keyed, scratch, range.x, n, ordered);
-  else
-    order_streamed<false>(L, depth_keys, list, nullptr, scratch, range.x, n, ordered);
+  if (!order_in_lds(L, depth_keys, list, keyed, range.x, n, ordered) && threadIdx.x == 0) {
+    const uint32_t k = atomicAdd(queued, 1u);
+    if (k < gridDim.x) entries[k] = tile;  // one entry a tile at most: always true
+  }
+}
+
+// The queued lists (radix passes in LDS), then the lists longer than kChunk: in
+// LDS up to kLongChunk pairs, streamed through `scratch` (8 bytes a pair when keyed,
+// else 4) beyond. With `tile_order` (tiles by falling pair count in steps of
+// kOrderStep, which divides kChunk) a workgroup walks the order from its own index
+// and stops at its first list shorter than kChunk: every later list is shorter too,
+// so a grid of a few workgroups covers the long lists. Without it, one workgroup a
+// tile. Workgroup 0 clears `clear`, the count of the queue the next call fills.
+__global__ __launch_bounds__(kThreads) void depth_order_long_kernel(const uint32_t* __restrict__ depth_keys,
+                                                                     const uint2* __restrict__ ranges,
+                                                                     const uint32_t* __restrict__ tile_order,
+                                                                     uint32_t tiles, uint32_t* list, uint2* keyed,
+                                                                     void* scratch, bool ordered,
+                                                                     const uint32_t* __restrict__ queued,
+                                                                     const uint32_t* __restrict__ entries,
+                                                                     uint32_t* __restrict__ clear) {
+  static_assert(kChunk % kOrderStep == 0, "the early stop needs whole order steps below kChunk");
+  __shared__ OrderLds<kLongRows> L;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
+  const uint32_t count = min(*queued, tiles);
+  for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+    const uint32_t tile = entries[k];
+    if (tile >= tiles) continue;
+    const uint2 range = ranges[tile];
+    order_by_digits(L, depth_keys, list, keyed, range.x, range.y - range.x, ordered);
+    __syncthreads();  // the next list reuses L
+  }
+  for (uint32_t i = blockIdx.x; i < tiles; i += gridDim.x) {
+    const uint2 range = ranges[tile_order ? tile_order[i] : i];
+    const uint32_t n = range.y - range.x;
+    if (n < kChunk && tile_order) break;
+    if (n <= kChunk) continue;
+    if (n <= kLongChunk) {
+      if (!order_in_lds(L, depth_keys, list, keyed, range.x, n, ordered)) {
+        __syncthreads();  // L is read again below
+        order_by_digits(L, depth_keys, list, keyed, range.x, n, ordered);
+      }
+    } else if (keyed) {
+      order_streamed<kLongRows, true>(L, depth_keys, list, keyed, scratch, range.x, n, ordered);
+    } else {
+      order_streamed<kLongRows, false>(L, depth_keys, list, nullptr, scratch, range.x, n, ordered);
+    }
+    __syncthreads();  // the next list reuses L
+  }
 }
 
 // A frame whose pairs overflow the capacity keeps, as the reference does
@@ -1067,11 +1138,23 @@ void pair_cut(hipStream_t s, const uint32_t* depth_keys, uint32_t* tile_counts, 
 }
 
 void depth_order(hipStream_t s, const uint32_t* depth_keys, const uint32_t* ranges, const uint32_t* tile_order,
-                 uint32_t tiles, uint32_t* list, void* scratch, bool ordered, uint32_t* keyed) {
+                 uint32_t tiles, uint32_t* list, void* scratch, bool ordered, uint32_t* queue, uint32_t parity,
+                 uint32_t* keyed) {
   if (tiles == 0) return;
+  // Two queues, used by turns: this call fills queue `parity` (cleared by the call
+  // before) and clears the other one for the next call.
+  uint32_t* queued = queue + (parity & 1u);
+  uint32_t* entries = queue + 2 + static_cast<size_t>(parity & 1u) * tiles;
   depth_order_kernel<<<tiles, kThreads, 0, s>>>(depth_keys, reinterpret_cast<const uint2*>(ranges), tile_order, list,
-                                                reinterpret_cast<uint2*>(keyed), scratch, ordered);
+                                                reinterpret_cast<const uint2*>(keyed), ordered, queued, entries);
   DIO_LAUNCHED("depth_order");
+  // With the order, the long lists are the first tiles of it: 256 workgroups take
+  // them (each stops at its first short list) and the queued ones.
+  const uint32_t long_groups = tile_order ? std::min<uint32_t>(tiles, 256) : tiles;
+  depth_order_long_kernel<<<long_groups, kThreads, 0, s>>>(
+      depth_keys, reinterpret_cast<const uint2*>(ranges), tile_order, tiles, list, reinterpret_cast<uint2*>(keyed),
+      scratch, ordered, queued, entries, queue + ((parity + 1u) & 1u));
+  DIO_LAUNCHED("depth_order_long");
 }
 
 BinLayout bin_layout(uint32_t n) {

@@ -185,9 +185,12 @@ void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair
 // buffer the runs longer than 2,048 pairs pass through at their own positions.
 // keyed (nullable): the runs as (depth key, splat) pairs (two words a pair) at the
 // same positions, read instead of `list` and depth_keys[splat] and sorted into
-// `list`; scratch then holds two words a pair. keyed is overwritten.
+// `list`; scratch then holds two words a pair. keyed is overwritten. queue: 2 tiles + 2
+// words, zero before the first call: two queues of the runs that need radix passes,
+// used by turns (`parity` counts the calls on this queue).
 void depth_order(hipStream_t s, const uint32_t* depth_keys, const uint32_t* ranges, const uint32_t* tile_order,
-                 uint32_t tiles, uint32_t* list, void* scratch, bool ordered, uint32_t* keyed = nullptr);
+                 uint32_t tiles, uint32_t* list, void* scratch, bool ordered, uint32_t* queue, uint32_t parity,
+                 uint32_t* keyed = nullptr);
 
 // The rasterizer's launch order: tiles by falling pair count, in buckets of
 // kOrderStep pairs (ties in any order).

@@ -361,10 +361,11 @@ def test_long_lists_and_depth_ties_in_every_depth_order():
     with the splats sorted by depth before their pairs are written (per_tile_depth 0)
     or each tile's list sorted by depth after binning (1: binned by a stable sort, so
     lists arrive in splat order; 2: binned from counts, so lists arrive in any order
-    and the radix passes also pass over the splat indices). Lists past 2,048 pairs
+    and the radix passes also pass over the splat indices). Lists past 4,096 pairs
     pass through global memory, in three digit passes where a list's depths share
     their top byte and four where they do not; shorter lists are bucketed by depth in
-    LDS, or take the same radix passes when one bucket would hold more than 16 pairs.
+    LDS (those past 2,048 by a second kernel), or take radix passes in LDS when one
+    bucket would hold more than 32 pairs.
     All give the oracle's (tile, depth, splat) lists, and the same image."""
     rng = np.random.default_rng(5)
     w = h = 256
@@ -386,8 +387,8 @@ def test_long_lists_and_depth_ties_in_every_depth_order():
     near = cluster(60000, (1.2, 1.8), (-0.25, 0.25), (-0.25, 0.25))  # depths in [1, 2): one top byte
     far = cluster(60000, (2.0, 12.0), (-0.9, -0.5), (-0.2, 0.2))
     spread = cluster(20000, (1.05, 1.95), (-1.0, 1.0), (-1.0, 1.0))  # in every list; the near lists' top byte
-    # Copies tie in depth: in the long lists, in the short ones (at most 16 pairs to a
-    # bucket of the LDS order), and 40 copies of one Gaussian (a bucket past 16 pairs).
+    # Copies tie in depth: in the long lists, in the short ones (at most 32 pairs to a
+    # bucket of the LDS order), and 40 copies of one Gaussian (a bucket past 32 pairs).
     params = np.concatenate([near, far, spread, near[:5000], far[:5000], spread[:300], np.repeat(spread[300:301], 40, 0)])
     n = len(params)
     frames = {}
