@@ -39,8 +39,7 @@ constexpr uint32_t buckets_for = kBuckets * R / kRows;
 
 template <int R>
 struct OrderLds {
-  uint32_t key[kThreads * R];
-  uint32_t value[kThreads * R];
+  uint2 pair[kThreads * R];  // (depth key, splat): one 8-byte LDS access a pair
   uint32_t wave_count[kWaves][kRadix];  // per wave and digit: count, then where the wave's run starts
   uint32_t chunk_start[kRadix];         // where each digit's run starts in the chunk
   uint32_t run_start[kRadix];           // long lists: where each digit's run continues in the output
@@ -220,8 +219,7 @@ __device__ void radix_in_lds(OrderLds<R>& L, uint32_t* __restrict__ list, uint32
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
-        L.key[pos[r]] = key[r];
-        L.value[pos[r]] = value[r];
+        L.pair[pos[r]] = make_uint2(key[r], value[r]);
       }
     }
     __syncthreads();
@@ -229,8 +227,9 @@ __device__ void radix_in_lds(OrderLds<R>& L, uint32_t* __restrict__ list, uint32
     for (int r = 0; r < R; r++) {
       const uint32_t e = element(rows, r);
       if (static_cast<uint32_t>(r) < rows && e < n) {
-        key[r] = L.key[e];
-        value[r] = L.value[e];
+        const uint2 p = L.pair[e];
+        key[r] = p.x;
+        value[r] = p.y;
       }
     }
   }
@@ -324,8 +323,7 @@ __device__ bool order_in_lds(OrderLds<R>& L, const uint32_t* __restrict__ depth_
     if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
       start[r] = L.bucket[bucket[r]];
       end[r] = bucket[r] + 1 < buckets_for<R> ? L.bucket[bucket[r] + 1] : n;
-      L.key[start[r] + slot[r]] = key[r];
-      L.value[start[r] + slot[r]] = value[r];
+      L.pair[start[r] + slot[r]] = make_uint2(key[r], value[r]);
     }
   }
   __syncthreads();
@@ -335,7 +333,10 @@ __device__ bool order_in_lds(OrderLds<R>& L, const uint32_t* __restrict__ depth_
   for (int r = 0; r < R; r++) {
     if (static_cast<uint32_t>(r) < rows && element(rows, r) < n) {
       uint32_t place = start[r];
-      for (uint32_t j = start[r]; j < end[r]; j++) place += before(L.key[j], L.value[j], key[r], value[r]) ? 1u : 0u;
+      for (uint32_t j = start[r]; j < end[r]; j++) {
+        const uint2 p = L.pair[j];
+        place += before(p.x, p.y, key[r], value[r]) ? 1u : 0u;
+      }
       L.bucket[place] = value[r];
     }
   }
