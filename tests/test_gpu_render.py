@@ -473,3 +473,35 @@ def test_overflowing_frame_keeps_the_nearest_pair_slots(per_tile):
         assert np.array_equal(tr.debug_buffers["ranges"].download(np.uint32, 2 * tiles).reshape(-1, 2), full_ranges)
         assert np.array_equal(tr.debug_buffers["pairSplats"].download(np.uint32, total), full_pairs)
         tr.destroy()
+
+
+@pytest.mark.parametrize("per_tile", [1, 2])
+def test_depth_order_queue_over_repeated_frames(per_tile):
+    """The per-tile depth order queues the lists its LDS buckets cannot order (a
+    bucket past 32 pairs: here 60 copies of one Gaussian, so 60 equal depths) for a
+    second kernel, in two queues used by turns, each call clearing the other's count.
+    Five frames in a row from one TileRasterizer, with and without the copies, must
+    each give the oracle's lists (tiles.wgsl:47-58 order: tile, depth, splat)."""
+    ctx = context()
+    w, h = 160, 120
+    cam = scene.camera(scene.IDENTITY, scene.perspective(math.pi / 2, w / h, 0.1, 100), w, h)
+    g = random_scene(3000, 4, scale=(0.05, 0.3))
+    crowded = g.copy()
+    crowded[-60:] = g[:1]
+    bg = [0.0, 0.0, 0.0]
+    tiles = ((w + 15) // 16) * ((h + 15) // 16)
+    splat_sets = [oracle.preprocess(p, cam).astype(np.float32) for p in (crowded, g, crowded, crowded, g)]
+    buf = ctx.upload(splat_sets[0])
+    tr = diorama_amd.TileRasterizer(ctx, buf, len(g), 1 << 20)
+    tr.set_tuning({"per_tile_depth": per_tile})
+    for k, splats in enumerate(splat_sets):
+        say("depth order queue, per_tile_depth", per_tile, "frame", k)
+        buf.upload(splats)
+        tr.encode(None, (w, h), bg)
+        ctx.synchronize()
+        ranges = tr.debug_buffers["ranges"].download(np.uint32, 2 * tiles).reshape(-1, 2)
+        o_ranges, o_pairs = oracle.tile_lists(splats.astype(np.float64), w, h)
+        pairs = tr.debug_buffers["pairSplats"].download(np.uint32, len(o_pairs))
+        assert np.array_equal(ranges, o_ranges), k
+        assert np.array_equal(pairs, o_pairs), k
+        tr.after_submit()
