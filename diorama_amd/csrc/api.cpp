@@ -1271,7 +1271,8 @@ struct Mulberry32 {
 // rewritten before each replay.
 struct BatchState {
   uint32_t views = 0;                   // views per step
-  uint32_t n = 0;                       // Gaussians the buffers hold
+  uint32_t n = 0;                       // Gaussians per view in the frame layout (the trainer's count)
+  uint32_t n_cap = 0;                   // Gaussians per view the frame buffers hold (n_cap >= n)
   uint32_t max_tiles = 0, max_pixels = 0;
   uint32_t capacity = 0;                // (splat, tile) pairs
   dio_buf *desc = nullptr, *adam = nullptr;
@@ -1290,7 +1291,7 @@ struct BatchState {
   bool upload_pending = false, read_pending = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  uint32_t graph_degree = ~0u, graph_generation = ~0u, frames_generation = ~0u;
+  uint32_t graph_degree = ~0u, graph_generation = ~0u;
   bool sorted_in_alt = false;
   std::vector<uint32_t> step_views;  // training views of the last step
   uint32_t step_pixels = 0;
@@ -1394,6 +1395,7 @@ void batch_free_frames(BatchState* b) {
                       &b->pair_sort_ws, &b->splat_grads, &b->grad_gaussians, &b->grad_sh})
     release(*p);
   b->n = 0;
+  b->n_cap = 0;
   b->capacity = 0;
 }
 
@@ -1798,9 +1800,13 @@ void batch_limits(dio_trainer* t, BatchState* b) {
 void batch_alloc_frames(dio_trainer* t, uint32_t capacity) {
   BatchState* b = t->batch;
   dio_ctx* ctx = t->ctx;
-  const uint32_t n = t->count;
+  // Room for a quarter more Gaussians than now: densification grows the count a
+  // little at a time, and each reallocation of these buffers costs milliseconds.
+  if (static_cast<uint64_t>(t->count) * b->views >= (1ull << 30))
+    invalid("a batch holds at most 2^30 splats (views x Gaussians)");
+  const uint32_t n = static_cast<uint32_t>(
+      std::min<uint64_t>(static_cast<uint64_t>(t->count) + t->count / 4, ((1ull << 30) - 1) / b->views));
   const uint64_t total = static_cast<uint64_t>(n) * b->views;
-  if (total >= (1ull << 30)) invalid("a batch holds at most 2^30 splats (views x Gaussians)");
   b->splats = alloc(ctx, total * dio::kFloatsPerSplat * 4, true);
   b->tile_counts = alloc(ctx, total * 4, true);
   b->depth_keys = alloc(ctx, total * 4);
@@ -1820,9 +1826,9 @@ void batch_alloc_frames(dio_trainer* t, uint32_t capacity) {
   b->splat_grads = alloc(ctx, total * 9 * 4, true);
   b->grad_gaussians = alloc(ctx, static_cast<size_t>(n) * dio::kFloatsPerGaussian * 4, true);
   b->grad_sh = alloc(ctx, static_cast<size_t>(n) * dio::kShRest * 4, true);
-  b->n = n;
+  b->n = t->count;
+  b->n_cap = n;
   b->capacity = capacity;
-  b->frames_generation = t->generation;
 }
 
 dio::BatchArgs batch_args(const BatchState* b) {
@@ -1842,6 +1848,10 @@ void batch_launch(dio_trainer* t, bool with_adam) {
   auto* sc = t->renderer->scene;
   const dio::BatchArgs a = batch_args(b);
   const uint32_t total = b->views * b->n;
+  // The workspaces keep the layout of the buffers' allocation (n_cap Gaussians a
+  // view): their counters sit after the per-tile states, at an offset that a layout
+  // by the current count would move onto words another layout left non-zero.
+  const uint32_t held = b->views * b->n_cap;
   const float* bg = t->settings.background;
   dio::preprocess_batch(s, dev<float>(sc->gaussians), dev<float>(sc->sh), t->sh_degree, a, dev<float>(b->splats),
                         dev<uint32_t>(b->tile_counts), dev<uint32_t>(b->depth_keys), dev<uint32_t>(b->depth_values));
@@ -1850,11 +1860,11 @@ void batch_launch(dio_trainer* t, bool with_adam) {
   const dio_tuning& tune = t->tuning;
   if (dio::radix_sort(s, false, b->depth_keys->ptr, dev<uint32_t>(b->depth_values), b->depth_alt_keys->ptr,
                       dev<uint32_t>(b->depth_alt_values), nullptr, total, 0, 32,
-                      dio::sort_workspace(b->sort_ws->ptr, total), tune.digit_bits, dev<uint32_t>(b->tile_counts),
+                      dio::sort_workspace(b->sort_ws->ptr, held), tune.digit_bits, dev<uint32_t>(b->tile_counts),
                       dev<uint32_t>(b->sorted_counts)))
     throw Error(DIO_ERR_INTERNAL, "depth sort ended in its scratch buffers");
   dio::scan_exclusive(s, dev<uint32_t>(b->sorted_counts), dev<uint32_t>(b->offsets), nullptr, total,
-                      dev<uint32_t>(b->pair_total), dio::scan_workspace(b->scan_ws->ptr, total));
+                      dev<uint32_t>(b->pair_total), dio::scan_workspace(b->scan_ws->ptr, held));
   dio::duplicate_batch(s, dev<float>(b->splats), dev<uint32_t>(b->depth_values), dev<uint32_t>(b->offsets),
                        dev<uint32_t>(b->tile_counts), a, b->capacity, dev<uint32_t>(b->pair_total),
                        dev<uint32_t>(b->pair_tiles), dev<uint32_t>(b->pair_splats));
@@ -1865,7 +1875,7 @@ void batch_launch(dio_trainer* t, bool with_adam) {
   dio_buf* sorted_tiles = b->sorted_in_alt ? b->alt_tiles : b->pair_tiles;
   dio_buf* sorted_splats = b->sorted_in_alt ? b->alt_splats : b->pair_splats;
   dio::find_ranges(s, dev<uint32_t>(sorted_tiles), dev<uint32_t>(b->pair_total), b->capacity, b->max_tiles,
-                   dev<uint32_t>(b->ranges));
+                   dev<uint32_t>(b->ranges), nullptr, 0, nullptr, true);
   dio::rasterize_batch(s, dev<float>(b->splats), dev<uint32_t>(sorted_splats), dev<uint32_t>(b->ranges), a, bg,
                        dev<float>(b->pixel_color), dev<uint32_t>(b->pixel_last), tune.splat_batch);
   dio::training_loss_batch(s, a, dev<float>(b->pixel_color), dev<float>(b->partials), dev<float>(b->pixel_grads),
@@ -1927,11 +1937,12 @@ void batch_optimize(dio_trainer* t) {
     b->last_pairs = b->readback[0];
     if (b->last_pairs > b->capacity) capacity = next_pow2_capacity(b->last_pairs);
   }
-  if (b->n != t->count || capacity != b->capacity || b->frames_generation != t->generation) {
+  if (t->count > b->n_cap || capacity != b->capacity) {
     sync(t->ctx);
     batch_free_frames(b);
     batch_alloc_frames(t, capacity);
   }
+  b->n = t->count;  // the frame layout's stride; a changed count also changes the generation (recapture)
   if (b->upload_pending) DIO_HIP(hipEventSynchronize(b->uploaded));  // the staging memory is free again
   auto* header = reinterpret_cast<dio::BatchHeader*>(b->host);
   auto* views = reinterpret_cast<dio::ViewDesc*>(b->host + kBatchViewsOffset);

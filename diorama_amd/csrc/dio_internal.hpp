@@ -171,10 +171,13 @@ void duplicate(hipStream_t s, const float* splats, const uint32_t* order, const 
 // Each tile's [start, end) run of the sorted pairs; with `order` (nullable, one
 // word per tile) also the rasterizer's launch order, heaviest tiles first.
 // clear (nullable): 1 + kPartClasses words set to zero before the rasterizer runs
-// (the segmented backward's item counts).
+// (the segmented backward's item counts). sparse: most tiles hold no pair (a
+// batch's tile space, padded per view), so every range is first cleared by a fill
+// launch instead of by the thread that meets the gap, which would write a long gap
+// alone.
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
                  uint32_t tiles, uint32_t* ranges, uint32_t* order = nullptr, uint32_t pair_hint = 0,
-                 uint32_t* clear = nullptr);
+                 uint32_t* clear = nullptr, bool sparse = false);
 
 // Sorts each tile's run of `list` (pairs binned by tile) by (depth_keys[splat],
 // splat), in place: the run of pairs a depth-ordered duplicate would have

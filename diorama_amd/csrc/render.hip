@@ -201,30 +201,35 @@ __global__ __launch_bounds__(kThreads) void duplicate_kernel(const float4* __res
 __global__ __launch_bounds__(kThreads) void find_ranges_kernel(const uint32_t* __restrict__ pair_tiles,
                                                                 const uint32_t* __restrict__ pair_total,
                                                                 uint32_t capacity, uint32_t tiles,
-                                                                uint2* __restrict__ ranges) {
+                                                                uint2* __restrict__ ranges, bool cleared) {
   const uint32_t count = min(*pair_total, capacity);
   const uint32_t first = blockIdx.x * kThreads + threadIdx.x, stride = gridDim.x * kThreads;
   const uint2 empty = make_uint2(0u, 0u);
+  // With `cleared` every range is already empty: only the runs' ends are written.
   if (count == 0) {
-    for (uint32_t t = first; t < tiles; t += stride) ranges[t] = empty;
+    if (!cleared)
+      for (uint32_t t = first; t < tiles; t += stride) ranges[t] = empty;
     return;
   }
   for (uint32_t i = first; i < count; i += stride) {
     const uint32_t tile = min(pair_tiles[i], tiles - 1);
     if (i == 0) {
-      for (uint32_t t = 0; t < tile; t++) ranges[t] = empty;
+      if (!cleared)
+        for (uint32_t t = 0; t < tile; t++) ranges[t] = empty;
       ranges[tile].x = 0;
     } else {
       const uint32_t previous = min(pair_tiles[i - 1], tiles - 1);
       if (tile != previous) {
         ranges[previous].y = i;
-        for (uint32_t t = previous + 1; t < tile; t++) ranges[t] = empty;
+        if (!cleared)
+          for (uint32_t t = previous + 1; t < tile; t++) ranges[t] = empty;
         ranges[tile].x = i;
       }
     }
     if (i == count - 1) {
       ranges[tile].y = count;
-      for (uint32_t t = tile + 1; t < tiles; t++) ranges[t] = empty;
+      if (!cleared)
+        for (uint32_t t = tile + 1; t < tiles; t++) ranges[t] = empty;
     }
   }
 }
@@ -604,10 +609,12 @@ void duplicate_batch(hipStream_t s, const float* splats, const uint32_t* order, 
 }
 
 void find_ranges(hipStream_t s, const uint32_t* pair_tiles, const uint32_t* pair_total, uint32_t capacity,
-                 uint32_t tiles, uint32_t* ranges, uint32_t* order, uint32_t pair_hint, uint32_t* clear) {
+                 uint32_t tiles, uint32_t* ranges, uint32_t* order, uint32_t pair_hint, uint32_t* clear,
+                 bool sparse) {
   if (tiles == 0) return;
+  if (sparse) fill_u32(s, ranges, 0u, 2 * static_cast<size_t>(tiles));
   find_ranges_kernel<<<hinted_groups(capacity, pair_hint, kThreads), kThreads, 0, s>>>(
-      pair_tiles, pair_total, capacity, tiles, reinterpret_cast<uint2*>(ranges));
+      pair_tiles, pair_total, capacity, tiles, reinterpret_cast<uint2*>(ranges), sparse);
   DIO_LAUNCHED("find_ranges");
   if (order) {
     tile_order_kernel<<<1, kOrderThreads, 0, s>>>(reinterpret_cast<const uint2*>(ranges), tiles, order, clear);
